@@ -1,0 +1,137 @@
+"""Host-side model state in the reference's own array layout.
+
+Arrays are numpy arrays in Fortran (column-major) order with the shapes of
+src/arrays.f90:59-100, so `state.cord[j, i, d]` is the reference's
+`cord(j+1, i+1, d+1)`.  The C-ABI receives raw pointers to these buffers
+(`host_arrays()`), exactly as a Fortran caller would pass its module arrays.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+NODE2 = ("cord", "vel", "force", "bc")
+NODE1 = ("temp", "temp0", "amass", "rmass")
+ELEM1 = ("aps", "visn", "e2sr", "sshrheat", "shrheat", "source")
+
+
+class State:
+    """Complete set of arrays the explicit step reads or writes."""
+
+    FIELDS = [n for n, _ in abi.HOST_FIELDS]
+
+    def __init__(self, nx, nz, nphase):
+        self.nx, self.nz, self.nphase = int(nx), int(nz), int(nphase)
+        ex, ez = self.nx - 1, self.nz - 1
+        f8 = dict(dtype=np.float64, order="F")
+        for n in NODE2:
+            setattr(self, n, np.zeros((self.nz, self.nx, 2), **f8))
+        for n in NODE1:
+            setattr(self, n, np.zeros((self.nz, self.nx), **f8))
+        self.ncod = np.zeros((self.nz, self.nx, 2), dtype=np.int32, order="F")
+        self.stress0 = np.zeros((ez, ex, 4, 4), **f8)
+        self.area = np.zeros((ez, ex, 4), **f8)
+        self.dvol = np.zeros((ez, ex, 4), **f8)
+        self.strain = np.zeros((ez, ex, 3), **f8)
+        self.se2sr = np.zeros((ez, ex, 3), **f8)
+        for n in ELEM1:
+            setattr(self, n, np.zeros((ez, ex), **f8))
+        self.iphase = np.ones((ez, ex), dtype=np.int32, order="F")
+        self.phase_ratio = np.zeros((self.nphase, ez, ex), **f8)
+        self.clock = abi.GfClock()
+
+    @property
+    def nelem(self):
+        return (self.nx - 1) * (self.nz - 1)
+
+    def host_arrays(self):
+        h = abi.GfHostArrays()
+        for name, ptype in abi.HOST_FIELDS:
+            a = getattr(self, name)
+            assert a.flags.f_contiguous
+            setattr(h, name, a.ctypes.data_as(ptype))
+        return h
+
+    def copy(self):
+        s = State(self.nx, self.nz, self.nphase)
+        for n in self.FIELDS:
+            getattr(s, n)[...] = getattr(self, n)
+        C.memmove(C.addressof(s.clock), C.addressof(self.clock), C.sizeof(abi.GfClock))
+        return s
+
+    def clock_dict(self):
+        return {k: getattr(self.clock, k) for k, _ in abi.GfClock._fields_}
+
+
+def _shape_for(state, name):
+    return getattr(state, name).shape
+
+
+def load_dump(path, nphase=None):
+    """Read one state directory written by the reference harness
+    (oracle/ref_harness.f90: raw little-endian column-major arrays + scalars.txt)."""
+    scal = {}
+    with open(os.path.join(path, "scalars.txt")) as f:
+        for line in f:
+            k, v = line.split()
+            scal[k] = float(v) if k not in ("nloop", "nsrate") else int(v)
+    pr = np.fromfile(os.path.join(path, "phase_ratio.f64"))
+    ip = np.fromfile(os.path.join(path, "iphase.i32"), dtype=np.int32)
+    nph = pr.size // ip.size
+    # the harness writes params.txt (with nx, nz) one directory up
+    ptxt = os.path.join(os.path.dirname(path.rstrip("/")), "params.txt")
+    with open(ptxt) as f:
+        for line in f:
+            if line.startswith("nx_nz_nphase"):
+                _, nx, nz, _ = line.split()
+                nx, nz = int(nx), int(nz)
+    st = State(nx, nz, nph if nphase is None else nphase)
+    for name in State.FIELDS:
+        dt = np.int32 if name in ("ncod", "iphase") else np.float64
+        ext = ".i32" if dt is np.int32 else ".f64"
+        arr = np.fromfile(os.path.join(path, name + ext), dtype=dt)
+        tgt = getattr(st, name)
+        tgt[...] = arr.reshape(tgt.shape, order="F")
+    c = st.clock
+    c.nloop, c.nsrate = scal["nloop"], scal["nsrate"]
+    c.time, c.dt, c.dtavg = scal["time"], scal["dt"], scal["dtavg"]
+    c.dt_elastic, c.dt_maxwell = scal["dt_elastic"], scal["dt_maxwell"]
+    st.scalars = scal
+    return st
+
+
+def save_npz(state, path, fields=None):
+    fields = state.FIELDS if fields is None else fields
+    d = {n: getattr(state, n) for n in fields}
+    d["_dims"] = np.array([state.nx, state.nz, state.nphase], dtype=np.int64)
+    d["_clock_i"] = np.array([state.clock.nloop, state.clock.nsrate], dtype=np.int64)
+    d["_clock_f"] = np.array([state.clock.time, state.clock.dt, state.clock.dtavg,
+                              state.clock.dt_elastic, state.clock.dt_maxwell])
+    np.savez_compressed(path, **d)
+
+
+def load_npz(path):
+    z = np.load(path, allow_pickle=False)
+    nx, nz, nph = (int(v) for v in z["_dims"])
+    st = State(nx, nz, nph)
+    for n in State.FIELDS:
+        if n in z.files:
+            getattr(st, n)[...] = z[n]
+    st.clock.nloop, st.clock.nsrate = (int(v) for v in z["_clock_i"])
+    (st.clock.time, st.clock.dt, st.clock.dtavg,
+     st.clock.dt_elastic, st.clock.dt_maxwell) = (float(v) for v in z["_clock_f"])
+    st.present = set(z.files)
+    return st
+
+
+def rel_l2(a, b):
+    """Relative L2 difference ||a-b|| / ||b|| (0 when both are zero)."""
+    a = np.asarray(a, dtype=np.float64).ravel()
+    b = np.asarray(b, dtype=np.float64).ravel()
+    nb = np.linalg.norm(b)
+    nd = np.linalg.norm(a - b)
+    if nb == 0.0:
+        return 0.0 if nd == 0.0 else float("inf")
+    return float(nd / nb)

@@ -1,0 +1,159 @@
+/*
+ * gf_abi.h -- C-ABI of the MI355X-native GeoFLAC explicit time step (libgfhip.so).
+ *
+ * The reference has no plugin API: its hot path is the argument-less Fortran
+ * subroutine `flac` (src/flac.f90:5-91) and the subroutines it calls, which share
+ * state through the module globals of src/params.f90 and src/arrays.f90.  This
+ * header replaces that boundary with plain C: a POD mirror of the parameters, raw
+ * pointers to the Fortran arrays in their own column-major layout, and one entry
+ * point per reference subroutine.  A Fortran caller binds it with ISO_C_BINDING
+ * (geoflac_amd/fortran/gf_hip.f90); nothing here uses C++ or torch types.
+ *
+ * Layout (identical to src/arrays.f90:59-100, 0-based in C):
+ *   node   (nz,nx)        a[j + nz*i]                 temp, temp0, amass, rmass
+ *   node   (nz,nx,2)      a[j + nz*(i + nx*d)]        cord, vel, force, bc, ncod(int32)
+ *   elem   (nz-1,nx-1)    a[j + (nz-1)*i]             aps, visn, e2sr, sshrheat, shrheat, source, iphase(int32)
+ *   elem   (nz-1,nx-1,m)  a[e + ne*m]                 area(4), dvol(4), strain(3), se2sr(3)
+ *   stress0(nz-1,nx-1,4,4) a[e + ne*(c + 4*k)]        component c, triangle k
+ *   phase_ratio(nphase,nz-1,nx-1) a[p + nphase*e]     phase fastest
+ * with e = j + (nz-1)*i and ne = (nz-1)*(nx-1).
+ *
+ * Status codes: 0 is success; positive codes below 1000 are the reference's own
+ * STOP codes (22 Maxwell time step, src/rh_maxwell.f90:24-30; 40 negative area,
+ * src/fl_move.f90:110-126); codes from 1000 up are ABI errors.
+ *
+ * Threading: a gf_ctx is driven from one host thread; calls are ordered on the
+ * context's own HIP stream, as the reference calls `flac` serially (src/par.f90:81-91).
+ */
+#ifndef GF_ABI_H
+#define GF_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GF_MAXPH 20   /* src/params.f90:9  maxph  */
+#define GF_MAXINH 100 /* src/params.f90:12 maxinh */
+
+/* status codes */
+#define GF_OK 0
+#define GF_STOP_MAXWELL 22        /* src/rh_maxwell.f90:29 */
+#define GF_STOP_NEGATIVE_AREA 40  /* src/fl_move.f90:125   */
+#define GF_ERR_HIP 1000           /* a HIP runtime call failed */
+#define GF_ERR_ARG 1001           /* bad argument (null pointer, size) */
+#define GF_ERR_UNSUPPORTED 1002   /* parameter selects a path this build does not provide */
+#define GF_ERR_STATE 1003         /* call out of order (e.g. step before upload) */
+#define GF_ERR_THERM_SWEEPS 1004  /* thermal fixed-point sweeps did not converge bitwise */
+
+/* Mirror of the scalars and per-phase tables of module params (src/params.f90:17-135)
+ * that the explicit step reads.  Integers first, then reals; the Fortran
+ * counterpart is type(gf_params_t), bind(C) in gf_hip.f90 with the same order. */
+typedef struct gf_params {
+    int32_t nx, nz, nphase;                 /* node counts as in the reference (elements + 1) */
+    int32_t mix_strain, mix_stress, ndim, movegrid;
+    int32_t itherm, istress_therm, ishearh, itemp_bc, itype_melting;
+    int32_t nystressbc, nydrsides, nyhydro, i_prestress;
+    int32_t idt_scale, i_rey, ivis_present, iphsub;
+    int32_t ifreq_rmasses, ifreq_imasses, ifreq_visc, ifreq_avgsr;
+    int32_t inhom;
+    int32_t irheol[GF_MAXPH];
+    int32_t ix1[GF_MAXINH], ix2[GF_MAXINH], iy1[GF_MAXINH], iy2[GF_MAXINH], igeom[GF_MAXINH];
+    int32_t pad0;
+    double t_top, t_bot, bot_bc, g, demf, ten_off, tau_heal, v_min, v_max;
+    double frac, fracm, strain_inert, dt_scale, xReyn, vbc, rxbo, rzbo;
+    double pisos, drosub, fmagma_max, weaken_ratio_plastic, weaken_ratio_viscous, rho_magma;
+    double den[GF_MAXPH], alfa[GF_MAXPH], beta[GF_MAXPH], pln[GF_MAXPH], acoef[GF_MAXPH];
+    double eactiv[GF_MAXPH], vactiv[GF_MAXPH], rl[GF_MAXPH], rm[GF_MAXPH];
+    double plstrain1[GF_MAXPH], plstrain2[GF_MAXPH], fric1[GF_MAXPH], fric2[GF_MAXPH];
+    double cohesion1[GF_MAXPH], cohesion2[GF_MAXPH], dilat1[GF_MAXPH], dilat2[GF_MAXPH];
+    double conduct[GF_MAXPH], cp[GF_MAXPH];
+} gf_params;
+
+/* Raw pointers to the host arrays (module arrays, src/arrays.f90:9-53).  Any
+ * pointer may be NULL if the corresponding bit is never passed to upload/download. */
+typedef struct gf_host_arrays {
+    double *cord, *vel, *temp, *temp0, *force, *amass, *rmass, *bc;
+    int32_t *ncod;
+    double *stress0, *area, *dvol, *strain, *se2sr;
+    double *aps, *visn, *e2sr, *sshrheat, *shrheat, *source;
+    int32_t *iphase;
+    double *phase_ratio;
+} gf_host_arrays;
+
+/* field bits for gf_upload / gf_download */
+enum {
+    GF_CORD = 1u << 0, GF_VEL = 1u << 1, GF_TEMP = 1u << 2, GF_TEMP0 = 1u << 3,
+    GF_FORCE = 1u << 4, GF_AMASS = 1u << 5, GF_RMASS = 1u << 6, GF_BC = 1u << 7,
+    GF_NCOD = 1u << 8, GF_STRESS0 = 1u << 9, GF_AREA = 1u << 10, GF_DVOL = 1u << 11,
+    GF_STRAIN = 1u << 12, GF_SE2SR = 1u << 13, GF_APS = 1u << 14, GF_VISN = 1u << 15,
+    GF_E2SR = 1u << 16, GF_SSHRHEAT = 1u << 17, GF_SHRHEAT = 1u << 18, GF_SOURCE = 1u << 19,
+    GF_IPHASE = 1u << 20, GF_PHASE_RATIO = 1u << 21,
+    GF_ALL = (1u << 22) - 1u
+};
+
+/* Time-stepping scalars of module params that change during a run
+ * (nloop, nsrate, time, dt, dtavg, dt_elastic, dt_maxwell; src/params.f90:38,96-106). */
+typedef struct gf_clock {
+    int32_t nloop, nsrate;
+    double time, dt, dtavg, dt_elastic, dt_maxwell;
+} gf_clock;
+
+/* Arithmetic of the step.  GF_PREC_FP64 is the plain drop-in (bitwise order of the
+ * reference).  GF_PREC_MIXED keeps stress, strain rate, force and velocity in FP32
+ * and coordinates, time, temperature and plastic strain in FP64. */
+#define GF_PREC_FP64 0
+#define GF_PREC_MIXED 1
+
+typedef struct gf_ctx gf_ctx;
+
+/* Replaces allocate_arrays (src/arrays.f90:55-130) for the device: validates the
+ * parameters and allocates device state for one model. */
+int gf_create(const gf_params *p, int32_t precision, gf_ctx **out);
+void gf_destroy(gf_ctx *ctx);
+const char *gf_last_error(const gf_ctx *ctx);
+/* (j,i) 1-based of the first element or node that raised a STOP code, or 0,0. */
+int gf_error_site(const gf_ctx *ctx, int32_t *j, int32_t *i);
+
+int gf_bind_host(gf_ctx *ctx, const gf_host_arrays *h);
+/* Host -> device and device -> host copies of the fields in `mask`, with the
+ * layout conversion to the device's own planes.  Upload of everything is what
+ * the reference does at setflac/rsflac time (src/setflac.f90:4-100); download is
+ * the `!$ACC wait` before outflac/saveflac (src/par.f90:113-127). */
+int gf_upload(gf_ctx *ctx, uint32_t mask);
+int gf_download(gf_ctx *ctx, uint32_t mask);
+int gf_set_clock(gf_ctx *ctx, const gf_clock *c);
+int gf_get_clock(gf_ctx *ctx, gf_clock *c);
+
+/* Exactly `nsteps` iterations of the reference's inner loop (src/par.f90:81-91):
+ * flac; nloop += 1; time += dt.  Scalars stay on the device; the clock is read
+ * back once at the end.  Replaces `do j=1,ntest_rem; call flac; ...` */
+int gf_step(gf_ctx *ctx, int32_t nsteps, gf_clock *clock);
+/* One call of subroutine flac (src/flac.f90:5-91) without the loop accounting,
+ * for a host driver that keeps its own nloop/time (the Fortran drop-in). */
+int gf_flac(gf_ctx *ctx, gf_clock *clock);
+
+/* One call of each reference subroutine on the device state, for bisection. */
+int gf_fl_therm(gf_ctx *ctx, gf_clock *clock);   /* src/fl_therm.f90:4-602 (itype_melting=0 part) */
+int gf_fl_rheol(gf_ctx *ctx, gf_clock *clock);   /* src/fl_rheol.f90:4-424 */
+int gf_fl_node(gf_ctx *ctx, gf_clock *clock);    /* src/fl_node.f90:3-393  */
+int gf_fl_move(gf_ctx *ctx, gf_clock *clock);    /* src/fl_move.f90:4-132  */
+int gf_rmasses(gf_ctx *ctx, gf_clock *clock);    /* src/rmasses.f90:4-63   */
+int gf_dt_mass(gf_ctx *ctx, gf_clock *clock);    /* src/dt_mass.f90:16-160 */
+int gf_init_areas(gf_ctx *ctx);                  /* src/init_areas.f90:7-55 */
+int gf_init_visc(gf_ctx *ctx);                   /* src/init_visc.f90:1-18  */
+
+/* Thermal update: number of fixed-point sweeps that reproduce the reference's
+ * in-place (Gauss-Seidel) order, and the count of nodes found unconverged by the
+ * verification sweep since the last call (0 means bitwise equal to serial order). */
+int gf_set_therm_sweeps(gf_ctx *ctx, int32_t nsweeps);
+int gf_therm_unconverged(gf_ctx *ctx, int64_t *count);
+
+/* Synchronise the context's stream (all queued steps finished). */
+int gf_sync(gf_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GF_ABI_H */

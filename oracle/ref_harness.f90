@@ -1,0 +1,159 @@
+! Test harness for the compiled GeoFLAC reference (test infrastructure only).
+!
+! This program is NOT part of the product.  It is linked against the
+! reference's own subroutines, compiled from the reference's source files
+! where they lie (see oracle/Makefile), and replaces only the reference's
+! main program (src/par.f90:3-137) with a loop that runs an exact number of
+! steps and dumps the complete in-memory state, so that golden vectors hold
+! every array the explicit step reads or writes.
+!
+! The step loop follows src/par.f90:81-91 (flac; nloop += 1; time += dt)
+! without remeshing or periodic output.  Compiled with -DGF_DROPIN the same
+! harness drives the HIP drop-in `flac` (geoflac_amd/fortran/flac_hip.f90)
+! and syncs device state to the Fortran arrays before each dump.
+!
+! usage: ref_harness <inputfile> <outdir> <nsteps> [dump_step ...]
+program gf_ref_harness
+use arrays
+use params
+use marker_data
+implicit none
+character(len=200) :: inputfile
+character(len=512) :: outdir, arg
+integer :: nsteps, nd, k, istep, narg
+integer, allocatable :: dumps(:)
+
+narg = command_argument_count()
+if (narg < 3) then
+    write(*,*) 'usage: ref_harness inputfile outdir nsteps [dump_step ...]'
+    stop 1
+endif
+call get_command_argument(1, inputfile)
+call get_command_argument(2, outdir)
+call get_command_argument(3, arg)
+read(arg, *) nsteps
+nd = narg - 3
+allocate(dumps(max(nd, 1)))
+dumps = -1
+do k = 1, nd
+    call get_command_argument(3 + k, arg)
+    read(arg, *) dumps(k)
+enddo
+
+open(unit=333, file='output.asc')
+
+call read_params(inputfile)
+call allocate_arrays(nz, nx, nphase)
+call allocate_markers(nz, nx)
+! fields the reference never initialises on a new start but the step reads
+force = 0.d0
+dike_released = 0.d0
+call setflac
+call dump_params(outdir)
+if (any(dumps(1:nd) == 0)) call dump_state(outdir, 0)
+
+do istep = 1, nsteps
+    call flac
+    nloop = nloop + 1
+    time = time + dt
+    if (any(dumps(1:nd) == istep)) then
+#ifdef GF_DROPIN
+        call flac_hip_sync_host
+#endif
+        call dump_state(outdir, istep)
+    endif
+enddo
+#ifdef GF_DROPIN
+call flac_hip_finalize
+#endif
+close(333)
+
+contains
+
+subroutine wr8(dir, name, a, n)
+    character(len=*), intent(in) :: dir, name
+    integer, intent(in) :: n
+    double precision, intent(in) :: a(n)
+    integer :: u
+    open(newunit=u, file=trim(dir)//'/'//name//'.f64', access='stream', form='unformatted', status='replace')
+    write(u) a
+    close(u)
+end subroutine
+
+subroutine wr4(dir, name, a, n)
+    character(len=*), intent(in) :: dir, name
+    integer, intent(in) :: n
+    integer, intent(in) :: a(n)
+    integer :: u
+    open(newunit=u, file=trim(dir)//'/'//name//'.i32', access='stream', form='unformatted', status='replace')
+    write(u) a
+    close(u)
+end subroutine
+
+subroutine dump_state(base, step)
+    character(len=*), intent(in) :: base
+    integer, intent(in) :: step
+    character(len=600) :: dir
+    integer :: u, ne, nn
+    write(dir, '(A,A,I7.7)') trim(base), '/step_', step
+    call execute_command_line('mkdir -p '//trim(dir))
+    ne = (nz-1)*(nx-1)
+    nn = nz*nx
+    call wr8(dir, 'cord', cord, nn*2)
+    call wr8(dir, 'vel', vel, nn*2)
+    call wr8(dir, 'temp', temp, nn)
+    call wr8(dir, 'temp0', temp0, nn)
+    call wr8(dir, 'force', force, nn*2)
+    call wr8(dir, 'amass', amass, nn)
+    call wr8(dir, 'rmass', rmass, nn)
+    call wr8(dir, 'bc', bc, nn*2)
+    call wr4(dir, 'ncod', ncod, nn*2)
+    call wr8(dir, 'stress0', stress0, ne*16)
+    call wr8(dir, 'area', area, ne*4)
+    call wr8(dir, 'dvol', dvol, ne*4)
+    call wr8(dir, 'strain', strain, ne*3)
+    call wr8(dir, 'se2sr', se2sr, ne*3)
+    call wr8(dir, 'aps', aps, ne)
+    call wr8(dir, 'visn', visn, ne)
+    call wr8(dir, 'e2sr', e2sr, ne)
+    call wr8(dir, 'sshrheat', sshrheat, ne)
+    call wr8(dir, 'shrheat', shrheat, ne)
+    call wr8(dir, 'source', source, ne)
+    call wr4(dir, 'iphase', iphase, ne)
+    call wr8(dir, 'phase_ratio', phase_ratio, ne*nphase)
+    open(newunit=u, file=trim(dir)//'/scalars.txt', status='replace')
+    write(u, '(A,I12)') 'nloop ', nloop
+    write(u, '(A,I12)') 'nsrate ', nsrate
+    write(u, '(A,1X,ES25.17E3)') 'time ', time
+    write(u, '(A,1X,ES25.17E3)') 'dt ', dt
+    write(u, '(A,1X,ES25.17E3)') 'dtavg ', dtavg
+    write(u, '(A,1X,ES25.17E3)') 'dt_elastic ', dt_elastic
+    write(u, '(A,1X,ES25.17E3)') 'dt_maxwell ', dt_maxwell
+    write(u, '(A,1X,ES25.17E3)') 'frac ', frac
+    close(u)
+end subroutine
+
+subroutine dump_params(base)
+    character(len=*), intent(in) :: base
+    integer :: u, k
+    call execute_command_line('mkdir -p '//trim(base))
+    open(newunit=u, file=trim(base)//'/params.txt', status='replace')
+    write(u, '(A,3I12)') 'nx_nz_nphase ', nx, nz, nphase
+    write(u, '(A,99I12)') 'flags ', mix_strain, mix_stress, ndim, movegrid, itherm, istress_therm, &
+        ishearh, itemp_bc, itype_melting, nystressbc, nydrsides, nyhydro, i_prestress, idt_scale, &
+        i_rey, ivis_present, iphsub, ifreq_rmasses, ifreq_imasses, ifreq_visc, ifreq_avgsr, inhom
+    write(u, '(A,99(1X,ES25.17E3))') 'reals ', t_top, t_bot, bot_bc, g, demf, ten_off, tau_heal, v_min, &
+        v_max, frac, fracm, strain_inert, dt_scale, xReyn, vbc, rxbo, rzbo, pisos, drosub, &
+        fmagma_max, weaken_ratio_plastic, weaken_ratio_viscous, rho_magma, extra_pres
+    do k = 1, nphase
+        write(u, '(A,I4,I4,99(1X,ES25.17E3))') 'phase ', k, irheol(k), den(k), alfa(k), beta(k), pln(k), &
+            acoef(k), eactiv(k), vactiv(k), rl(k), rm(k), plstrain1(k), plstrain2(k), fric1(k), &
+            fric2(k), cohesion1(k), cohesion2(k), dilat1(k), dilat2(k), conduct(k), cp(k)
+    enddo
+    do k = 1, inhom
+        write(u, '(A,6I8,1X,ES25.17E3)') 'inhom ', ix1(k), ix2(k), iy1(k), iy2(k), inphase(k), igeom(k), xinitaps(k)
+    enddo
+    close(u)
+end subroutine
+
+end program gf_ref_harness

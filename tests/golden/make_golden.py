@@ -1,0 +1,57 @@
+"""Write the golden fixtures of tests/golden from the compiled reference.
+
+Runs oracle/_ref/ref_harness (the reference's own subroutines compiled from
+/root/reference/src by `make -C oracle ref`, with a dump driver of this
+project) on the inputs in tests/golden/inputs, serially, and stores:
+
+  <case>_s0.npz        the complete state after setflac (step 0)
+  <case>_s<N>.npz      the fields the step changes, after N steps
+
+Only run in the build container (the GPU box has no reference).
+usage: python tests/golden/make_golden.py [case ...]
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from geoflac_amd import state  # noqa: E402
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+
+MUTABLE = ["cord", "vel", "temp", "temp0", "force", "amass", "rmass", "stress0", "area", "dvol",
+           "strain", "se2sr", "aps", "visn", "e2sr", "sshrheat", "shrheat"]
+
+CASES = {
+    # 60x20 elements, 300 km x 100 km, crust/mantle, extension 0.5 cm/yr per side
+    "box60x20": dict(inp="box60x20.inp", steps=[1, 10, 100, 1000, 10000]),
+}
+
+
+def run_case(name, spec):
+    inp = os.path.join(HERE, "inputs", spec["inp"])
+    steps = spec["steps"]
+    with tempfile.TemporaryDirectory() as tmp:
+        out = os.path.join(tmp, "dump")
+        cmd = [HARNESS, inp, out, str(max(steps)), "0"] + [str(s) for s in steps]
+        subprocess.run(cmd, cwd=tmp, check=True, stdout=subprocess.DEVNULL)
+        with open(os.path.join(out, "params.txt")) as f:
+            ptxt = f.read()
+        with open(os.path.join(HERE, name + "_params.txt"), "w") as f:
+            f.write(ptxt)
+        s0 = state.load_dump(os.path.join(out, "step_0000000"))
+        state.save_npz(s0, os.path.join(HERE, "%s_s0.npz" % name))
+        for n in steps:
+            sn = state.load_dump(os.path.join(out, "step_%07d" % n))
+            state.save_npz(sn, os.path.join(HERE, "%s_s%d.npz" % (name, n)), fields=MUTABLE)
+            print(name, n, "written")
+
+
+if __name__ == "__main__":
+    names = sys.argv[1:] or list(CASES)
+    for n in names:
+        run_case(n, CASES[n])

@@ -1,0 +1,267 @@
+"""The CPU oracle against the reference's golden states and against closed forms.
+
+The golden states were dumped by the flang-built reference (tests/golden/make_golden.py);
+the oracle must reproduce them bit for bit.  The closed-form tests pin each
+constitutive law independently of the reference run.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+import pytest
+
+from geoflac_amd import abi, inp, state
+from tests import oracle_lib as ol
+
+CASE = "box60x20"
+
+
+def test_inp_reader_matches_reference_read_params():
+    ref = ol.golden_params(CASE)
+    q = inp.read_inp(os.path.join(ol.GOLDEN, "inputs", CASE + ".inp"))
+    mine = inp.to_gf_params(q, vbc=ref.vbc, ivis_present=ref.ivis_present, pisos=ref.pisos)
+    assert bytes(memoryview(mine)) == bytes(memoryview(ref))
+
+
+def test_params_layout_matches_header():
+    # 546 int32 (even count, no hidden padding) followed by 23 + 19*20 doubles
+    assert abi.SIZEOF_PARAMS == 546 * 4 + (23 + 19 * abi.MAXPH) * 8
+    assert abi.SIZEOF_CLOCK == 8 + 5 * 8
+
+
+@pytest.mark.parametrize("nsteps", [1, 10, 100, 1000, 10000])
+def test_oracle_bitwise_equal_to_reference(nsteps):
+    p = ol.golden_params(CASE)
+    st = ol.golden_state(CASE, 0)
+    assert ol.step(p, st, nsteps) == 0
+    ref = ol.golden_state(CASE, nsteps)
+    for f in ref.present:
+        if f.startswith("_"):
+            continue
+        a, b = getattr(st, f), getattr(ref, f)
+        assert np.array_equal(a, b), (f, state.rel_l2(a, b))
+    assert st.clock.nloop == ref.clock.nloop == nsteps
+    assert st.clock.time == ref.clock.time
+    assert st.clock.dt == ref.clock.dt
+    assert st.clock.dtavg == ref.clock.dtavg
+    assert st.clock.dt_elastic == ref.clock.dt_elastic
+
+
+# ---------------------------------------------------------------- closed forms
+
+def _elastic(bulkm, rmu, s, de11, de22, de12):
+    s11, s22, s33, s12 = (C.c_double(v) for v in s)
+    ol.lib().gfo_elastic(C.c_double(bulkm), C.c_double(rmu), C.byref(s11), C.byref(s22),
+                         C.byref(s33), C.byref(s12), C.c_double(de11), C.c_double(de22),
+                         C.c_double(de12))
+    return s11.value, s22.value, s33.value, s12.value
+
+
+def test_elastic_bar_effective_modulus():
+    # plane strain bar free in z: E_eff = 4 mu (lam + mu) / (lam + 2 mu)
+    # (examples/tutorial01-elastic/tutorial.md:131-142)
+    lam, mu = 4e10, 4e10
+    bulkm = lam + 2 * mu / 3
+    de11 = -1e-5
+    de22 = -lam / (lam + 2 * mu) * de11          # makes s22 = 0
+    s11, s22, s33, s12 = _elastic(bulkm, mu, (0, 0, 0, 0), de11, de22, 0.0)
+    assert abs(s22) < 1e-6 * abs(s11)
+    e_eff = 4 * mu * (lam + mu) / (lam + 2 * mu)
+    assert s11 / de11 == pytest.approx(e_eff, rel=1e-12)
+    assert s33 == pytest.approx(lam * (de11 + de22), rel=1e-12)
+
+
+def _maxwell(bulkm, rmu, vis, s, de, dv, dt):
+    s11, s22, s33, s12 = (C.c_double(v) for v in s)
+    rc = ol.lib().gfo_maxwell(C.c_double(bulkm), C.c_double(rmu), C.c_double(vis), C.byref(s11),
+                              C.byref(s22), C.byref(s33), C.byref(s12), C.c_double(de[0]),
+                              C.c_double(de[1]), C.c_double(0.0), C.c_double(de[2]),
+                              C.c_double(dv), 2, C.c_double(dt))
+    return rc, (s11.value, s22.value, s33.value, s12.value)
+
+
+def test_maxwell_relaxation_factor_and_stability_stop():
+    mu, bulkm, vis = 3e10, 5e10, 1e21
+    dt = 0.2 * 2 * vis / mu           # theta = mu dt / (2 eta) = 0.2
+    s = (-10e6, 4e6, -3e6, 2e6)
+    rc, out = _maxwell(bulkm, mu, vis, s, (0, 0, 0), 0.0, dt)
+    assert rc == 0
+    theta = mu * dt / (2 * vis)
+    fac = (1 - theta) / (1 + theta)
+    s0 = 0.5 * (s[0] + s[1])
+    assert out[0] - s0 == pytest.approx((s[0] - s0) * fac, rel=1e-13)
+    assert out[3] == pytest.approx(s[3] * fac, rel=1e-13)
+    assert out[2] == pytest.approx(s0, rel=1e-13)     # s33 collapses to the 2-D mean
+    # volumetric part is elastic: dv adds bulkm*dv to the mean
+    rc, out2 = _maxwell(bulkm, mu, vis, s, (0, 0, 0), 1e-4, dt)
+    assert 0.5 * (out2[0] + out2[1]) - 0.5 * (out[0] + out[1]) == pytest.approx(bulkm * 1e-4, rel=1e-12)
+    # theta > 0.5 is the reference's STOP 22 (src/rh_maxwell.f90:24-30)
+    rc, _ = _maxwell(bulkm, mu, vis, s, (0, 0, 0), 0.0, 0.51 * 2 * vis / mu)
+    assert rc == abi.GF_STOP_MAXWELL
+    # Maxwell steady state under constant shear rate: s12 -> 2 eta edot (1 - ...) ~ 2*eta*edot
+    edot, ss = 1e-14, (0.0, 0.0, 0.0, 0.0)
+    dt = 0.1 * 2 * vis / mu
+    for _ in range(2000):
+        rc, ss = _maxwell(bulkm, mu, vis, ss, (0, 0, edot * dt), 0.0, dt)
+    assert ss[3] == pytest.approx(2 * vis * edot, rel=1e-9)
+
+
+def _plastic(bulkm, rmu, coh, phi, psi, s, de, ten_off=1e9, hardn=0.0):
+    s11, s22, s33, s12 = (C.c_double(v) for v in s)
+    depls, diss = C.c_double(), C.c_double()
+    ipls = C.c_int()
+    ol.lib().gfo_plastic(C.c_double(bulkm), C.c_double(rmu), C.c_double(coh), C.c_double(phi),
+                         C.c_double(psi), C.byref(depls), C.byref(ipls), C.byref(diss),
+                         C.c_double(hardn), C.byref(s11), C.byref(s22), C.byref(s33),
+                         C.byref(s12), C.c_double(de[0]), C.c_double(de[1]), C.c_double(0.0),
+                         C.c_double(de[2]), C.c_double(ten_off), 2)
+    return (s11.value, s22.value, s33.value, s12.value), depls.value, ipls.value
+
+
+def test_plastic_return_lies_on_mohr_coulomb_surface():
+    lam, mu = 3e10, 3e10
+    bulkm = lam + 2 * mu / 3
+    coh, phi = 20e6, 30.0
+    nphi = (1 + math.sin(math.radians(phi))) / (1 - math.sin(math.radians(phi)))
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        s = tuple(rng.uniform(-50e6, 0, 4))
+        de = tuple(rng.uniform(-2e-3, 2e-3, 3))
+        out, depls, ipls = _plastic(bulkm, mu, coh, phi, 0.0, s, de)
+        s11, s22, s33, s12 = out
+        c0, rad = 0.5 * (s11 + s22), 0.5 * math.hypot(s11 - s22, 2 * s12)
+        s1, s3 = c0 - rad, c0 + rad
+        f = s1 - s3 * nphi + 2 * coh * math.sqrt(nphi)
+        assert depls >= 0.0
+        if ipls == -2:
+            assert abs(f) <= 1e-9 * (abs(s1) + abs(s3) * nphi + 1.0)
+            # psi = 0: no plastic dilation; the mean in-plane stress follows the elastic trial
+        else:
+            assert f >= -1e-6 * abs(s1)
+
+
+def test_plastic_uniaxial_strength():
+    # UCS = 2 c tan(45 + phi/2) (examples/tutorial03-elastoplastic/tutorial.md:148-168):
+    # with s22 = 0 the material yields exactly when s11 passes -UCS
+    lam, mu = 3e10, 3e10
+    bulkm = lam + 2 * mu / 3
+    coh, phi = 20e6, 30.0
+    ucs = 2 * coh * math.tan(math.radians(45 + phi / 2))
+    out, depls, ipls = _plastic(bulkm, mu, coh, phi, 0.0, (-ucs * (1 - 1e-9), 0, 0, 0), (0, 0, 0))
+    assert ipls == 0 and depls == 0.0
+    out, depls, ipls = _plastic(bulkm, mu, coh, phi, 0.0, (-ucs * (1 + 1e-9), 0, 0, 0), (0, 0, 0))
+    assert ipls == -2 and depls > 0.0
+
+
+def test_plastic_tension_cutoff():
+    lam, mu = 3e10, 3e10
+    bulkm = lam + 2 * mu / 3
+    out, depls, ipls = _plastic(bulkm, mu, 20e6, 30.0, 0.0, (0, 0, 0, 0), (1e-3, 1e-3, 0.0), ten_off=5e6)
+    assert ipls == -5
+    assert out == (5e6, 5e6, 5e6, 0.0)
+
+
+def _uniform_state(nx=6, nz=4, dx=1000.0, dz=500.0, skew=0.0, phase=2):
+    p = ol.golden_params(CASE)
+    p.nx, p.nz = nx, nz
+    st = state.State(nx, nz, p.nphase)
+    xs = np.arange(nx) * dx
+    zs = -np.arange(nz) * dz
+    st.cord[:, :, 0] = xs[None, :] + skew * zs[:, None]
+    st.cord[:, :, 1] = zs[:, None]
+    st.iphase[...] = phase
+    st.phase_ratio[phase - 1] = 1.0
+    st.temp[...] = 500.0
+    ol.lib().gfo_init_areas(C.byref(p), C.byref(st.host_arrays()))
+    return p, st
+
+
+def test_linear_velocity_gives_exact_strain_rate():
+    p, st = _uniform_state(skew=0.3)
+    a, b, c, d = 1e-14, -3e-15, 2e-15, -7e-15
+    x, z = st.cord[:, :, 0], st.cord[:, :, 1]
+    st.vel[:, :, 0] = a * x + b * z
+    st.vel[:, :, 1] = c * x + d * z
+    st.se2sr[...] = 0.0
+    st.clock.dt = 1e9
+    st.clock.nloop = 1        # not a viscosity-update step
+    st.clock.nsrate = 0
+    st.visn[...] = 1e22
+    assert ol.call("gfo_fl_rheol", p, st) == 0
+    dt = 1e9
+    np.testing.assert_allclose(st.se2sr[:, :, 0], a * dt, rtol=1e-10)
+    np.testing.assert_allclose(st.se2sr[:, :, 1], d * dt, rtol=1e-10)
+    np.testing.assert_allclose(st.se2sr[:, :, 2], 0.5 * (b + c) * dt, rtol=1e-10)
+
+
+def test_uniform_stress_gives_zero_interior_force_and_gravity_balance():
+    p, st = _uniform_state(nx=7, nz=5, skew=0.2)
+    st.stress0[:, :, 0, :] = -3e7
+    st.stress0[:, :, 1, :] = -1e7
+    st.stress0[:, :, 2, :] = 2e6
+    st.rmass[...] = 0.0
+    st.amass[...] = 1.0
+    st.clock.dt, st.clock.dt_elastic = 1.0, 1.0
+    p.demf = 0.0
+    st.ncod[...] = 0
+    assert ol.call("gfo_fl_node", p, st) == 0
+    interior = st.force[1:-1, 1:-1, :]
+    assert np.max(np.abs(interior)) < 1e-6 * 3e7 * 1000
+    # boundary tractions of a uniform stress sum to zero over a closed boundary
+    assert abs(st.force[:, :, 0].sum()) < 1e-6 * 3e7 * 1000
+    assert abs(st.force[:, :, 1].sum()) < 1e-6 * 3e7 * 1000
+
+
+def test_rmasses_total_equals_density_times_area():
+    p, st = _uniform_state(nx=8, nz=5, skew=0.1)
+    assert ol.call("gfo_rmasses", p, st) == 0
+    quad = 0.25 * (1.0 / st.area).sum(axis=2)        # sum of the 4 half-doubled areas / 4
+    dens = np.array([[ol.lib().gfo_eff_dens(C.byref(p), C.byref(st.host_arrays()), j + 1, i + 1)
+                      for i in range(p.nx - 1)] for j in range(p.nz - 1)])
+    assert st.rmass.sum() == pytest.approx((dens * quad).sum(), rel=1e-13)
+
+
+def test_dt_mass_rectangular_dlmin():
+    p, st = _uniform_state(nx=5, nz=4, dx=2000.0, dz=500.0)
+    p.idt_scale = 2
+    st.clock.dt_elastic = 0
+    assert ol.call("gfo_dt_mass", p, st) == 0
+    dlmin = 2000.0 * 500.0 / math.hypot(2000.0, 500.0)
+    assert st.clock.dt_elastic == pytest.approx(dlmin * p.frac * p.strain_inert / p.vbc, rel=1e-14)
+    kappa = p.conduct[1] / p.den[1] / p.cp[1]
+    dt_maxwell = p.v_min / p.rm[1] * p.fracm
+    assert st.clock.dt == pytest.approx(min(st.clock.dt_elastic, dt_maxwell, dlmin ** 2 / kappa), rel=1e-14)
+
+
+def test_thermal_uniform_field_is_stationary():
+    # each triangle's diffusion matrix has zero row sums, so a uniform field with
+    # zero sources and matching boundary values does not change (boundary nodes too)
+    p, st = _uniform_state(nx=9, nz=7, dx=3000.0, dz=1000.0, skew=0.25)
+    p.t_top, p.itemp_bc, p.bot_bc = 800.0, 1, 800.0
+    st.temp[...] = 800.0
+    st.source[...] = 0.0
+    st.clock.dt = 1e11
+    assert ol.call("gfo_fl_therm", p, st) == 0
+    np.testing.assert_allclose(st.temp, 800.0, rtol=0, atol=1e-9)
+    np.testing.assert_array_equal(st.temp0, 800.0)
+
+
+def test_thermal_sine_mode_decays_at_the_analytic_rate():
+    # 1-D mode T = sin(pi z / H) with T=0 at both ends decays like exp(-kappa (pi/H)^2 t).
+    # The in-place (Gauss-Seidel) update of the reference speeds the decay by O(r),
+    # r = kappa dt / dz^2, so r is kept at 0.01 and the rate checked to 2%.
+    p, st = _uniform_state(nx=5, nz=41, dx=1000.0, dz=250.0)
+    p.t_top, p.itemp_bc, p.bot_bc = 0.0, 1, 0.0
+    H = 40 * 250.0
+    z = -st.cord[:, :, 1]
+    st.temp[...] = np.sin(np.pi * z / H)
+    st.source[...] = 0.0
+    kappa = p.conduct[1] / p.den[1] / p.cp[1]
+    dt = 0.01 * 250.0 ** 2 / kappa
+    st.clock.dt = dt
+    nstep = 1000
+    for _ in range(nstep):
+        assert ol.call("gfo_fl_therm", p, st) == 0
+    rate = -math.log(st.temp[20, 2]) / (dt * nstep)
+    assert rate / (kappa * (math.pi / H) ** 2) == pytest.approx(1.0, abs=0.02)
