@@ -1,0 +1,145 @@
+#!/usr/bin/env python3
+"""Throughput of the GeoFLAC explicit time step on one MI355X (element-steps/s).
+
+Workload: the 4000x1000-element four-layer box (geoflac_amd/configs/
+layers4000x1000.inp), synthetic new-start state built on the host, all arrays
+resident in HBM before timing.  A step is one iteration of the reference's inner
+loop (src/par.f90:81-91: flac; nloop += 1; time += dt).  Prints one JSON line.
+
+roofline: per-kernel durations are measured with HIP events recorded on the
+library's own stream (gf_set_timing / gf_kernel_time) over a separate profiled
+run; the dominant kernel's algorithmic bytes per launch (table ALG_BYTES) over its
+average duration is `achieved`.  cpu_baseline: the C oracle (serial restatement
+of the reference step) on a bounded sample of the same model.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "element-steps/s on 1 GPU: mixed/learned vs plain FP64, at stated rel-L2 error"
+
+
+def alg_bytes(kernel, E, N, nact, p):
+    """Algorithmic (compulsory) HBM bytes of one launch: each array the kernel must
+    read once and write once, FP64 = 8 B, int32 = 4 B."""
+    if kernel == "k_rheol":
+        per_e = (32 + 32 + 128 + 8 + 8 + 8 * nact + 4 + 24 + 8 + 24)  # area dvol stress0 aps visn pr iphase se2sr sshrheat strain
+        per_e += 128 + 8 + 24 + 24 + 8                                 # writes stress0 aps strain se2sr sshrheat
+        per_n = 16 + 16                                                # vel, cord
+        return E * per_e + N * per_n
+    if kernel == "k_therm_sweep":
+        return N * (8 + 8 + 8 + 16) + E * (16 + 4 + 8)                # told xin xout cord | area(1:2) iphase source
+    if kernel == "k_node":
+        return E * (96 + 32) + N * (16 + 16 + 16 + 8 + 16 + 16 + 16)  # stress0(1:3) area | cord rmass+amass vel ncod bc, vel+force out
+    if kernel == "k_move_elem":
+        return E * (32 + 96 + 24 + 32 + 32 + 96 + 24) + N * 32
+    if kernel == "k_move_cord":
+        return N * 48
+    if kernel == "k_therm_final":
+        return N * 16
+    if kernel == "k_copy":
+        return N * 16
+    return 0
+
+
+def build_model(cfg, precision):
+    from geoflac_amd import inp, model, gpu
+    p = inp.read_inp(os.path.join(ROOT, "geoflac_amd", "configs", cfg + ".inp"))
+    gp, st = model.build_box(p)
+    m = gpu.Model(gp, st, precision=precision)
+    model.device_setup(m)
+    return p, gp, st, m
+
+
+def cpu_baseline(gp, st, seconds_cap=20.0):
+    """The oracle (serial C restatement) on a copy of the same model, a few steps."""
+    sys.path.insert(0, ROOT)
+    from tests import oracle_lib as ol
+    s = st.copy()
+    E = (gp.nx - 1) * (gp.nz - 1)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        rc = ol.step(gp, s, 1)
+        if rc != 0:
+            break
+        n += 1
+        el = time.perf_counter() - t0
+        if el > seconds_cap / 2 or n >= 5:
+            break
+    el = time.perf_counter() - t0
+    return {"value": E * n / el, "unit": "element-steps/s", "cores": 1, "kind": "port",
+            "sample": "%d step(s) of the same %dx%d model by the oracle (serial C restatement "
+                      "of the reference step), %.1f s" % (n, gp.nx - 1, gp.nz - 1, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--config", default="layers4000x1000")
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (one model per device); "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from geoflac_amd import abi, gpu
+    p, gp, st, m = build_model(args.config, abi.PREC_FP64)
+    E = (gp.nx - 1) * (gp.nz - 1)
+    N = gp.nx * gp.nz
+    m.download()
+    host_state = st.copy()  # complete new-start state, for the CPU baseline
+
+    m.step(args.warmup)
+    m.sync()
+    t0 = time.perf_counter()
+    m.step(args.steps)       # returns after the device finished (clock read-back)
+    m.sync()
+    t1 = time.perf_counter()
+    el = t1 - t0
+    value = E * args.steps / el
+
+    # profiled run: HIP events around every launch on the library stream
+    m.set_timing(True)
+    m.step(args.profile_steps)
+    kernels = ["k_rheol", "k_therm_sweep", "k_node", "k_move_elem", "k_move_cord", "k_therm_final",
+               "k_copy", "k_dens", "k_rmass", "k_dt_reduce", "k_amass", "k_clock"]
+    times = {k: m.kernel_time(k) for k in kernels}
+    m.set_timing(False)
+    dom = max(times, key=lambda k: times[k][0])
+    ms_tot, nl = times[dom]
+    nact = int((host_state.phase_ratio >= 0.01).any(axis=(1, 2)).sum())
+    ab = alg_bytes(dom, E, N, nact, gp)
+    avg_s = ms_tot / 1e3 / max(nl, 1)
+    achieved = ab / avg_s / 1e9 if ab else 0.0
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "alg_bytes_per_launch": ab, "avg_launch_us": round(avg_s * 1e6, 2),
+                "kernel_ms_per_step": {k: round(v[0] / args.profile_steps, 4) for k, v in times.items() if v[1]}}
+    m.close()
+
+    out = {"metric": METRIC, "value": value, "unit": "element-steps/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic",
+           "config": {"workload": "%s: %dx%d-element four-layer extension box, plain FP64 drop-in path"
+                                  % (args.config, gp.nx - 1, gp.nz - 1),
+                      "elements": E, "nodes": N, "precision": "fp64", "parallelism": "single GPU"},
+           "roofline": roofline}
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(gp, host_state)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,194 @@
+// gf_device.h -- device-side parameter block and per-element physics of the
+// GeoFLAC explicit step, written for CDNA4 (gfx950).
+//
+// Every function keeps the reference's expression order (Fortran a+b+c is
+// (a+b)+c, x**2 is x*x) and the library is compiled with -ffp-contract=off, so the
+// FP64 path rounds exactly as the reference does.  Per-phase constants that the
+// reference recomputes per element ((0.75A)^(-1/n), bulk modulus, ...) are formed
+// once on the host with the same expressions.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/gf_abi.h"
+
+#define GF_DEV static __device__ __forceinline__
+
+// Per-phase constants (phase id k stored at index k-1).
+struct GfPhase {
+    int irheol, is_mantle, pad0, pad1;
+    double den, alfa, beta, pln, eactiv, vactiv, rl, rm;
+    double plstrain1, plstrain2, fric1, fric2, cohesion1, cohesion2, dilat1, dilat2;
+    double conduct, cp;
+    double bulkm;        // rl + 2*rm/3              (src/fl_rheol.f90:191)
+    double pwave;        // rl + 0.6666*rm           (src/dt_mass.f90:86)
+    double pw;           // 1/pln - 1                (src/matprops.f90:191)
+    double pw1;          // -1/pln                   (src/matprops.f90:192)
+    double acoef_pow;    // (0.75*acoef)**pw1, host libm pow
+    double pln_r;        // pln * 8.31448
+    double diff;         // conduct/den/cp           (src/fl_therm.f90:412)
+    double dt_m;         // maxwell dt candidate     (src/dt_mass.f90:135-147), 1e28 if none
+};
+
+struct GfDevParams {
+    int nx, nz, nphase, nactive;
+    int mix_strain, mix_stress, ndim, movegrid;
+    int itherm, istress_therm, ishearh, itemp_bc;
+    int nystressbc, nyhydro, i_prestress, idt_scale;
+    int i_rey, ivis_present, iphsub, inhom;
+    int active[GF_MAXPH];  // phase ids (1-based) present anywhere with ratio >= 0.01
+    int ix1[GF_MAXINH], ix2[GF_MAXINH], iy1[GF_MAXINH], iy2[GF_MAXINH], igeom[GF_MAXINH];
+    double t_top, t_bot, bot_bc, g, demf, ten_off, tau_heal, v_min, v_max;
+    double frac, fracm, strain_inert, dt_scale, xReyn, vbc, rxbo, rzbo, pisos, drosub;
+    double vbc_over_rxbo;
+    GfPhase ph[GF_MAXPH];
+};
+
+// Time-stepping scalars and reduction cells kept on the device.
+#define GF_MAX_SWEEPS 32
+struct GfDevClock {
+    double time, dt, dtavg, dt_elastic, dt_maxwell;
+    unsigned long long dlmin_bits, velmax_bits, dtmax_therm_bits, dtmaxwell_bits, dte_bits;
+    int error_code, pad;
+    long long error_index;      // smallest linear index (reference loop order) that failed
+    unsigned long long therm_unconverged;
+    unsigned int sweep_changed[GF_MAX_SWEEPS];
+};
+
+// ---------------------------------------------------------------- helpers
+GF_DEV double fsign(double a, double b) { return b < 0.0 ? -fabs(a) : fabs(a); }
+
+GF_DEV void raise_error(GfDevClock *ck, int code, long long index)
+{
+    atomicCAS(&ck->error_code, 0, code);
+    atomicMin(&ck->error_index, index);
+}
+
+// src/rh_elastic.f90:3-26
+GF_DEV void dev_elastic(double bulkm, double rmu, double &s11, double &s22, double &s33,
+                        double &s12, double de11, double de22, double de12)
+{
+    const double c4d3 = 4.0 / 3.0, c2d3 = 2.0 / 3.0;
+    double a1 = bulkm + c4d3 * rmu;
+    double a2 = bulkm - c2d3 * rmu;
+    s11 = s11 + a1 * de11 + a2 * de22;
+    s22 = s22 + a2 * de11 + a1 * de22;
+    s12 = s12 + 2 * rmu * de12;
+    s33 = s33 + a2 * (de11 + de22);
+}
+
+// src/rh_maxwell.f90:2-76 (ndim = 2 branch; de33 = 0 as passed by fl_rheol).
+// Returns false where the reference does STOP 22.
+GF_DEV bool dev_maxwell(double bulkm, double rmu0, double viscosity, double &s11, double &s22,
+                        double &s33, double &s12, double de11, double de22, double de12, double dv,
+                        double dt)
+{
+    const double visc_cut = 1.0e19;
+    double rmu = viscosity < visc_cut ? rmu0 * viscosity / visc_cut : rmu0;
+    double temp = rmu / (2 * viscosity) * dt;
+    if (temp > 0.5) return false;
+    double vic1 = 1 - temp;
+    double vic2 = 1.0 / (1.0 + temp);
+    double dev = de11 + de22;
+    double de11d = de11 - 0.5 * dev;
+    double de22d = de22 - 0.5 * dev;
+    double s0 = 0.5 * (s11 + s22);
+    double s11d = s11 - s0;
+    double s22d = s22 - s0;
+    double s33d = 0;
+    s11d = (s11d * vic1 + 2 * rmu * de11d) * vic2;
+    s22d = (s22d * vic1 + 2 * rmu * de22d) * vic2;
+    s33d = (s33d * vic1 + 2 * rmu * 0.0) * vic2;
+    s12 = (s12 * vic1 + 2 * rmu * de12) * vic2;
+    s0 = s0 + bulkm * dv;
+    s11 = s11d + s0;
+    s22 = s22d + s0;
+    s33 = s33d + s0;
+    return true;
+}
+
+// Per-element plastic constants of src/rh_plastic.f90:29-46: they depend on
+// (bulkm, rmu, coh, phi, psi) only, which are uniform over the 4 triangles.
+struct PlasticConst {
+    double anphi, anpsi, amc, e1, e2, x1, ten_max;
+};
+
+GF_DEV PlasticConst plastic_const(double bulkm, double rmu, double coh, double phi, double psi,
+                                  double ten_off)
+{
+    const double degrad = 3.14159265358979323846 / 180.0, c4d3 = 4.0 / 3.0, c2d3 = 2.0 / 3.0;
+    PlasticConst k;
+    double sphi = sin(phi * degrad);
+    double spsi = sin(psi * degrad);
+    k.anphi = (1.0 + sphi) / (1.0 - sphi);
+    k.anpsi = (1.0 + spsi) / (1.0 - spsi);
+    k.amc = 2.0 * coh * sqrt(k.anphi);
+    k.e1 = bulkm + c4d3 * rmu;
+    k.e2 = bulkm - c2d3 * rmu;
+    k.x1 = (k.e1 - k.e2 * k.anpsi + k.e1 * k.anphi * k.anpsi - k.e2 * k.anphi);
+    if (phi == 0.0)
+        k.ten_max = ten_off;
+    else
+        k.ten_max = fmin(ten_off, coh / (tan(phi * degrad)));
+    return k;
+}
+
+// src/rh_plastic.f90:48-224, ndim = 2, de33 = 0, press_add = 0.
+GF_DEV void dev_plastic(const PlasticConst &k, double rmu, double hardn, double &depls, double &s11,
+                        double &s22, double &s33, double &s12, double de11, double de22,
+                        double de12)
+{
+    const double de33 = 0.0, press_add = 0.0;
+    depls = 0.0;
+    double s11i = s11 + (de22 + de33) * k.e2 + de11 * k.e1 - press_add;
+    double s22i = s22 + (de11 + de33) * k.e2 + de22 * k.e1 - press_add;
+    double s12i = s12 + de12 * 2.0 * rmu;
+    double s33i = s33 + (de11 + de22) * k.e2 + de33 * k.e1 - press_add;
+    double sdif = s11i - s22i;
+    double s0 = 0.5 * (s11i + s22i);
+    double rad = 0.5 * sqrt(sdif * sdif + 4.0 * s12i * s12i);
+    double si = s0 - rad;
+    double sii = s0 + rad;
+    double psdif = si - sii;
+    double s1 = si, s2 = s33i, s3 = sii;
+    const double ten_max = k.ten_max;
+    if (s1 >= ten_max) {
+        s11 = ten_max + press_add; s22 = ten_max + press_add; s12 = 0.0; s33 = ten_max + press_add;
+        return;
+    }
+    if (s3 >= ten_max) s3 = ten_max;
+    double fs = s1 - s3 * k.anphi + k.amc;
+    if (fs < 0.0) {
+        double alams = fs / (k.x1 + hardn);
+        s1 = s1 - alams * (k.e1 - k.e2 * k.anpsi);
+        s2 = s2 - alams * k.e2 * (1.0 - k.anpsi);
+        s3 = s3 - alams * (k.e2 - k.e1 * k.anpsi);
+        double dep1 = alams;
+        double dep3 = -alams * k.anpsi;
+        depls = 0.5 * fabs(dep1 - dep3);
+    } else {
+        s11 = s11i + press_add; s22 = s22i + press_add; s33 = s33i + press_add; s12 = s12i;
+        return;
+    }
+    if (s1 >= ten_max) {
+        s11 = ten_max + press_add; s22 = ten_max + press_add; s12 = 0.0; s33 = ten_max + press_add;
+        return;
+    }
+    if (s3 >= ten_max) s3 = ten_max;
+    double cs2, si2;
+    if (psdif == 0.0) {
+        cs2 = 1.0; si2 = 0.0;
+    } else {
+        cs2 = sdif / psdif;
+        si2 = 2.0 * s12i / psdif;
+    }
+    double dc2 = (s1 - s3) * cs2;
+    double dss = s1 + s3;
+    s11 = 0.5 * (dss + dc2);
+    s22 = 0.5 * (dss - dc2);
+    s12 = 0.5 * (s1 - s3) * si2;
+    s33 = s2;
+    s11 = s11 + press_add;
+    s22 = s22 + press_add;
+    s33 = s33 + press_add;
+}

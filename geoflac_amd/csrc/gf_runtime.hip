@@ -1,0 +1,693 @@
+// gf_runtime.hip -- C-ABI of libgfhip.so: device state of one GeoFLAC model and the
+// ordering of the step's kernels on one HIP stream (include/gf_abi.h).
+//
+// The host keeps only what the reference's step decides on the host: nloop and
+// nsrate, whose values select which kernels run (src/flac.f90:29,74,83 and
+// src/fl_rheol.f90:395-421).  time, dt, dtavg, dt_elastic and dt_maxwell live in
+// device memory and are read back once per gf_step call.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "gf_kernels.h"
+
+struct gf_ctx {
+    gf_params p;
+    int precision;
+    GfDevParams hp;            // host copy of the device parameter block
+    GfDevParams *dp = nullptr;
+    GfDevClock *dck = nullptr;
+    GfDevArrays a{};
+    int *d_present = nullptr;  // phase ids present in iphase (for dt_mass minima)
+    int npresent = 0;
+    hipStream_t stream = nullptr;
+    gf_host_arrays host{};
+    bool bound = false;
+    uint32_t uploaded = 0;
+    int32_t nloop = 0, nsrate = 0;
+    int therm_sweeps = 12;
+    long ne = 0, nn = 0;
+    std::vector<double> phase_ratio_host;  // full nphase layout, as uploaded
+    int err_j = 0, err_i = 0;
+    char msg[512] = {0};
+    // per-kernel event timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::map<std::string, std::pair<double, long long>> totals;
+};
+
+static int set_err(gf_ctx *c, int code, const char *fmt, ...)
+{
+    if (c) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(c->msg, sizeof(c->msg), fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+#define HIPCHK(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return set_err(ctx, GF_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));   \
+    } while (0)
+
+static inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
+
+// ------------------------------------------------------------------ parameters
+static int is_mantle(int iph) { return iph == 4 || iph == 8 || iph == 9 || iph == 16 || iph == 18; }
+
+static int build_dev_params(gf_ctx *ctx)
+{
+    const gf_params &p = ctx->p;
+    GfDevParams &d = ctx->hp;
+    int nactive = d.nactive;
+    int active[GF_MAXPH];
+    memcpy(active, d.active, sizeof(active));
+    memset(&d, 0, sizeof(d));
+    d.nactive = nactive;
+    memcpy(d.active, active, sizeof(active));
+    d.nx = p.nx; d.nz = p.nz; d.nphase = p.nphase;
+    d.mix_strain = p.mix_strain; d.mix_stress = p.mix_stress; d.ndim = p.ndim; d.movegrid = p.movegrid;
+    d.itherm = p.itherm; d.istress_therm = p.istress_therm; d.ishearh = p.ishearh; d.itemp_bc = p.itemp_bc;
+    d.nystressbc = p.nystressbc; d.nyhydro = p.nyhydro; d.i_prestress = p.i_prestress; d.idt_scale = p.idt_scale;
+    d.i_rey = p.i_rey; d.ivis_present = p.ivis_present; d.iphsub = p.iphsub; d.inhom = p.inhom;
+    for (int k = 0; k < GF_MAXINH; k++) {
+        d.ix1[k] = p.ix1[k]; d.ix2[k] = p.ix2[k]; d.iy1[k] = p.iy1[k]; d.iy2[k] = p.iy2[k]; d.igeom[k] = p.igeom[k];
+    }
+    d.t_top = p.t_top; d.t_bot = p.t_bot; d.bot_bc = p.bot_bc; d.g = p.g; d.demf = p.demf;
+    d.ten_off = p.ten_off; d.tau_heal = p.tau_heal; d.v_min = p.v_min; d.v_max = p.v_max;
+    d.frac = p.frac; d.fracm = p.fracm; d.strain_inert = p.strain_inert; d.dt_scale = p.dt_scale;
+    d.xReyn = p.xReyn; d.vbc = p.vbc; d.rxbo = p.rxbo; d.rzbo = p.rzbo; d.pisos = p.pisos; d.drosub = p.drosub;
+    d.vbc_over_rxbo = p.vbc / p.rxbo;
+    for (int k = 0; k < GF_MAXPH; k++) {
+        GfPhase &ph = d.ph[k];
+        ph.irheol = p.irheol[k];
+        ph.is_mantle = is_mantle(k + 1);
+        ph.den = p.den[k]; ph.alfa = p.alfa[k]; ph.beta = p.beta[k]; ph.pln = p.pln[k];
+        ph.eactiv = p.eactiv[k]; ph.vactiv = p.vactiv[k]; ph.rl = p.rl[k]; ph.rm = p.rm[k];
+        ph.plstrain1 = p.plstrain1[k]; ph.plstrain2 = p.plstrain2[k];
+        ph.fric1 = p.fric1[k]; ph.fric2 = p.fric2[k];
+        ph.cohesion1 = p.cohesion1[k]; ph.cohesion2 = p.cohesion2[k];
+        ph.dilat1 = p.dilat1[k]; ph.dilat2 = p.dilat2[k];
+        ph.conduct = p.conduct[k]; ph.cp = p.cp[k];
+        // the expressions below are the reference's, evaluated once per phase
+        ph.bulkm = p.rl[k] + 2.0 * p.rm[k] / 3.0;               // src/fl_rheol.f90:191
+        ph.pwave = p.rl[k] + 0.6666 * p.rm[k];                  // src/dt_mass.f90:86
+        if (p.pln[k] != 0.0) {
+            ph.pw = 1.0 / p.pln[k] - 1.0;                       // src/matprops.f90:191
+            ph.pw1 = -1.0 / p.pln[k];                           // src/matprops.f90:192
+            ph.acoef_pow = pow(0.75 * p.acoef[k], ph.pw1);      // src/matprops.f90:194
+            ph.pln_r = p.pln[k] * 8.31448;                      // src/matprops.f90:160,195
+        }
+        if (p.den[k] != 0.0 && p.cp[k] != 0.0) ph.diff = p.conduct[k] / p.den[k] / p.cp[k];
+        ph.dt_m = 1.0e28;
+        if ((p.irheol[k] == 3 || p.irheol[k] == 12) && p.rm[k] < 1.0e11) {  // src/dt_mass.f90:138-146
+            const double visc_cut = 1.0e10;
+            double rmu = p.v_min < visc_cut ? p.rm[k] * p.v_min / visc_cut : p.rm[k];
+            ph.dt_m = p.v_min / rmu * p.fracm;
+        }
+    }
+    return GF_OK;
+}
+
+static int push_params(gf_ctx *ctx)
+{
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(ctx->dp, &ctx->hp, sizeof(GfDevParams), hipMemcpyHostToDevice));
+    return GF_OK;
+}
+
+extern "C" int gf_create(const gf_params *p, int32_t precision, gf_ctx **out)
+{
+    gf_ctx *ctx = nullptr;
+    if (!p || !out) return GF_ERR_ARG;
+    *out = nullptr;
+    if (p->nx < 3 || p->nz < 3 || p->nphase < 1 || p->nphase > GF_MAXPH) return GF_ERR_ARG;
+    if (precision != GF_PREC_FP64) return GF_ERR_UNSUPPORTED;
+    // paths of the reference this build does not provide (see the header)
+    if (p->itype_melting != 0 || p->nystressbc != 0 || p->ndim != 2) return GF_ERR_UNSUPPORTED;
+    if (p->inhom < 0 || p->inhom > GF_MAXINH) return GF_ERR_ARG;
+    // deeper than ~240 km, Eff_dens would rewrite the global dt fraction (src/matprops.f90:37-51)
+    if (p->rzbo < -240.0e3) return GF_ERR_UNSUPPORTED;
+    ctx = new gf_ctx();
+    ctx->p = *p;
+    ctx->precision = precision;
+    ctx->ne = (long)(p->nz - 1) * (p->nx - 1);
+    ctx->nn = (long)p->nz * p->nx;
+    ctx->hp.nactive = 0;
+    build_dev_params(ctx);
+    const long ne = ctx->ne, nn = ctx->nn;
+    int rc = GF_OK;
+#define ALLOC(ptr, bytes)                                                                    \
+    do {                                                                                     \
+        hipError_t e_ = hipMalloc((void **)&(ptr), (bytes));                                 \
+        if (e_ != hipSuccess) {                                                              \
+            rc = set_err(ctx, GF_ERR_HIP, "hipMalloc(%ld) failed: %s", (long)(bytes),        \
+                         hipGetErrorString(e_));                                             \
+            goto fail;                                                                       \
+        }                                                                                    \
+    } while (0)
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        rc = set_err(ctx, GF_ERR_HIP, "hipStreamCreate failed");
+        goto fail;
+    }
+    ALLOC(ctx->dp, sizeof(GfDevParams));
+    ALLOC(ctx->dck, sizeof(GfDevClock));
+    ALLOC(ctx->d_present, sizeof(int) * GF_MAXPH);
+    ALLOC(ctx->a.cord, 8 * nn * 2);
+    ALLOC(ctx->a.vel, 8 * nn * 2);
+    ALLOC(ctx->a.temp, 8 * nn);
+    ALLOC(ctx->a.temp0, 8 * nn);
+    ALLOC(ctx->a.force, 8 * nn * 2);
+    ALLOC(ctx->a.amass, 8 * nn);
+    ALLOC(ctx->a.rmass, 8 * nn);
+    ALLOC(ctx->a.bc, 8 * nn * 2);
+    ALLOC(ctx->a.ncod, 4 * nn * 2);
+    ALLOC(ctx->a.stress0, 8 * ne * 16);
+    ALLOC(ctx->a.area, 8 * ne * 4);
+    ALLOC(ctx->a.dvol, 8 * ne * 4);
+    ALLOC(ctx->a.strain, 8 * ne * 3);
+    ALLOC(ctx->a.se2sr, 8 * ne * 3);
+    ALLOC(ctx->a.aps, 8 * ne);
+    ALLOC(ctx->a.visn, 8 * ne);
+    ALLOC(ctx->a.e2sr, 8 * ne);
+    ALLOC(ctx->a.sshrheat, 8 * ne);
+    ALLOC(ctx->a.shrheat, 8 * ne);
+    ALLOC(ctx->a.source, 8 * ne);
+    ALLOC(ctx->a.iphase, 4 * ne);
+    ALLOC(ctx->a.pr, 8 * ne * GF_MAXACT);
+    ALLOC(ctx->a.xa, 8 * nn);
+    ALLOC(ctx->a.xb, 8 * nn);
+    ALLOC(ctx->a.dens, 8 * ne);
+#undef ALLOC
+    {
+        GfDevClock ck{};
+        ck.error_index = (long long)1 << 62;
+        if (hipMemcpy(ctx->dck, &ck, sizeof(ck), hipMemcpyHostToDevice) != hipSuccess) {
+            rc = set_err(ctx, GF_ERR_HIP, "clock init failed");
+            goto fail;
+        }
+    }
+    if ((rc = push_params(ctx)) != GF_OK) goto fail;
+    *out = ctx;
+    return GF_OK;
+fail:
+    gf_destroy(ctx);
+    return rc;
+}
+
+extern "C" void gf_destroy(gf_ctx *ctx)
+{
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+    void *bufs[] = {ctx->dp, ctx->dck, ctx->d_present, ctx->a.cord, ctx->a.vel, ctx->a.temp, ctx->a.temp0,
+                    ctx->a.force, ctx->a.amass, ctx->a.rmass, ctx->a.bc, ctx->a.ncod, ctx->a.stress0,
+                    ctx->a.area, ctx->a.dvol, ctx->a.strain, ctx->a.se2sr, ctx->a.aps, ctx->a.visn,
+                    ctx->a.e2sr, ctx->a.sshrheat, ctx->a.shrheat, ctx->a.source, ctx->a.iphase, ctx->a.pr,
+                    ctx->a.xa, ctx->a.xb, ctx->a.dens};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+extern "C" const char *gf_last_error(const gf_ctx *ctx) { return ctx ? ctx->msg : "null context"; }
+
+extern "C" int gf_error_site(const gf_ctx *ctx, int32_t *j, int32_t *i)
+{
+    if (!ctx) return GF_ERR_ARG;
+    if (j) *j = ctx->err_j;
+    if (i) *i = ctx->err_i;
+    return GF_OK;
+}
+
+extern "C" int gf_bind_host(gf_ctx *ctx, const gf_host_arrays *h)
+{
+    if (!ctx || !h) return GF_ERR_ARG;
+    ctx->host = *h;
+    ctx->bound = true;
+    return GF_OK;
+}
+
+// ------------------------------------------------------------------ transfers
+struct FieldDesc {
+    uint32_t bit;
+    const char *name;
+    size_t elem_bytes;
+    long count_mult;  // of nn or ne
+    bool node;
+};
+
+static const FieldDesc kFields[] = {
+    {GF_CORD, "cord", 8, 2, true},       {GF_VEL, "vel", 8, 2, true},         {GF_TEMP, "temp", 8, 1, true},
+    {GF_TEMP0, "temp0", 8, 1, true},     {GF_FORCE, "force", 8, 2, true},     {GF_AMASS, "amass", 8, 1, true},
+    {GF_RMASS, "rmass", 8, 1, true},     {GF_BC, "bc", 8, 2, true},           {GF_NCOD, "ncod", 4, 2, true},
+    {GF_STRESS0, "stress0", 8, 16, false}, {GF_AREA, "area", 8, 4, false},    {GF_DVOL, "dvol", 8, 4, false},
+    {GF_STRAIN, "strain", 8, 3, false},  {GF_SE2SR, "se2sr", 8, 3, false},    {GF_APS, "aps", 8, 1, false},
+    {GF_VISN, "visn", 8, 1, false},      {GF_E2SR, "e2sr", 8, 1, false},      {GF_SSHRHEAT, "sshrheat", 8, 1, false},
+    {GF_SHRHEAT, "shrheat", 8, 1, false}, {GF_SOURCE, "source", 8, 1, false}, {GF_IPHASE, "iphase", 4, 1, false},
+};
+
+static void *host_ptr(gf_ctx *ctx, uint32_t bit)
+{
+    const gf_host_arrays &h = ctx->host;
+    switch (bit) {
+    case GF_CORD: return h.cord;       case GF_VEL: return h.vel;           case GF_TEMP: return h.temp;
+    case GF_TEMP0: return h.temp0;     case GF_FORCE: return h.force;       case GF_AMASS: return h.amass;
+    case GF_RMASS: return h.rmass;     case GF_BC: return h.bc;             case GF_NCOD: return h.ncod;
+    case GF_STRESS0: return h.stress0; case GF_AREA: return h.area;         case GF_DVOL: return h.dvol;
+    case GF_STRAIN: return h.strain;   case GF_SE2SR: return h.se2sr;       case GF_APS: return h.aps;
+    case GF_VISN: return h.visn;       case GF_E2SR: return h.e2sr;         case GF_SSHRHEAT: return h.sshrheat;
+    case GF_SHRHEAT: return h.shrheat; case GF_SOURCE: return h.source;     case GF_IPHASE: return h.iphase;
+    case GF_PHASE_RATIO: return h.phase_ratio;
+    }
+    return nullptr;
+}
+
+static void *dev_ptr(gf_ctx *ctx, uint32_t bit)
+{
+    GfDevArrays &a = ctx->a;
+    switch (bit) {
+    case GF_CORD: return a.cord;       case GF_VEL: return a.vel;           case GF_TEMP: return a.temp;
+    case GF_TEMP0: return a.temp0;     case GF_FORCE: return a.force;       case GF_AMASS: return a.amass;
+    case GF_RMASS: return a.rmass;     case GF_BC: return a.bc;             case GF_NCOD: return a.ncod;
+    case GF_STRESS0: return a.stress0; case GF_AREA: return a.area;         case GF_DVOL: return a.dvol;
+    case GF_STRAIN: return a.strain;   case GF_SE2SR: return a.se2sr;       case GF_APS: return a.aps;
+    case GF_VISN: return a.visn;       case GF_E2SR: return a.e2sr;         case GF_SSHRHEAT: return a.sshrheat;
+    case GF_SHRHEAT: return a.shrheat; case GF_SOURCE: return a.source;     case GF_IPHASE: return a.iphase;
+    }
+    return nullptr;
+}
+
+// phase ratio: keep the phases whose ratio reaches 0.01 somewhere (the others are
+// skipped by every loop of the reference, e.g. src/rh_plastic.f90:250) as planes.
+static int upload_phase_ratio(gf_ctx *ctx)
+{
+    const int nph = ctx->p.nphase;
+    const long ne = ctx->ne;
+    const double *pr = ctx->host.phase_ratio;
+    ctx->phase_ratio_host.assign(pr, pr + (size_t)nph * ne);
+    int active[GF_MAXPH], nact = 0;
+    for (int k = 0; k < nph; k++) {
+        bool any = false;
+        for (long e = 0; e < ne && !any; e++) any = pr[k + (long)nph * e] >= 0.01;
+        if (any) {
+            if (nact == GF_MAXACT)
+                return set_err(ctx, GF_ERR_UNSUPPORTED, "more than %d phases present", GF_MAXACT);
+            active[nact++] = k + 1;
+        }
+    }
+    std::vector<double> planes((size_t)ne * (nact > 0 ? nact : 1));
+    for (int a = 0; a < nact; a++)
+        for (long e = 0; e < ne; e++) planes[(size_t)a * ne + e] = pr[(active[a] - 1) + (long)nph * e];
+    HIPCHK(hipMemcpy(ctx->a.pr, planes.data(), sizeof(double) * ne * nact, hipMemcpyHostToDevice));
+    ctx->hp.nactive = nact;
+    memset(ctx->hp.active, 0, sizeof(ctx->hp.active));
+    for (int a = 0; a < nact; a++) ctx->hp.active[a] = active[a];
+    return push_params(ctx);
+}
+
+static int upload_iphase_present(gf_ctx *ctx)
+{
+    bool seen[GF_MAXPH + 1] = {false};
+    const int32_t *ip = ctx->host.iphase;
+    for (long e = 0; e < ctx->ne; e++) {
+        int v = ip[e];
+        if (v < 1 || v > ctx->p.nphase) return set_err(ctx, GF_ERR_ARG, "iphase out of range at %ld", e);
+        seen[v] = true;
+    }
+    int present[GF_MAXPH], n = 0;
+    for (int k = 1; k <= GF_MAXPH; k++)
+        if (seen[k]) present[n++] = k;
+    ctx->npresent = n;
+    HIPCHK(hipMemcpy(ctx->d_present, present, sizeof(int) * n, hipMemcpyHostToDevice));
+    return GF_OK;
+}
+
+extern "C" int gf_upload(gf_ctx *ctx, uint32_t mask)
+{
+    if (!ctx) return GF_ERR_ARG;
+    if (!ctx->bound) return set_err(ctx, GF_ERR_STATE, "gf_upload before gf_bind_host");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (const FieldDesc &f : kFields) {
+        if (!(mask & f.bit)) continue;
+        void *h = host_ptr(ctx, f.bit);
+        if (!h) return set_err(ctx, GF_ERR_ARG, "host pointer for %s is null", f.name);
+        size_t bytes = f.elem_bytes * (size_t)f.count_mult * (f.node ? ctx->nn : ctx->ne);
+        HIPCHK(hipMemcpy(dev_ptr(ctx, f.bit), h, bytes, hipMemcpyHostToDevice));
+    }
+    if (mask & GF_IPHASE) {
+        int rc = upload_iphase_present(ctx);
+        if (rc) return rc;
+    }
+    if (mask & GF_PHASE_RATIO) {
+        if (!ctx->host.phase_ratio) return set_err(ctx, GF_ERR_ARG, "host pointer for phase_ratio is null");
+        int rc = upload_phase_ratio(ctx);
+        if (rc) return rc;
+    }
+    ctx->uploaded |= mask;
+    return GF_OK;
+}
+
+extern "C" int gf_download(gf_ctx *ctx, uint32_t mask)
+{
+    if (!ctx) return GF_ERR_ARG;
+    if (!ctx->bound) return set_err(ctx, GF_ERR_STATE, "gf_download before gf_bind_host");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (const FieldDesc &f : kFields) {
+        if (!(mask & f.bit)) continue;
+        void *h = host_ptr(ctx, f.bit);
+        if (!h) return set_err(ctx, GF_ERR_ARG, "host pointer for %s is null", f.name);
+        size_t bytes = f.elem_bytes * (size_t)f.count_mult * (f.node ? ctx->nn : ctx->ne);
+        HIPCHK(hipMemcpy(h, dev_ptr(ctx, f.bit), bytes, hipMemcpyDeviceToHost));
+    }
+    if ((mask & GF_PHASE_RATIO) && ctx->host.phase_ratio && !ctx->phase_ratio_host.empty())
+        memcpy(ctx->host.phase_ratio, ctx->phase_ratio_host.data(), sizeof(double) * ctx->phase_ratio_host.size());
+    return GF_OK;
+}
+
+extern "C" int gf_set_clock(gf_ctx *ctx, const gf_clock *c)
+{
+    if (!ctx || !c) return GF_ERR_ARG;
+    GfDevClock ck{};
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(&ck, ctx->dck, sizeof(ck), hipMemcpyDeviceToHost));
+    ck.time = c->time; ck.dt = c->dt; ck.dtavg = c->dtavg;
+    ck.dt_elastic = c->dt_elastic; ck.dt_maxwell = c->dt_maxwell;
+    HIPCHK(hipMemcpy(ctx->dck, &ck, sizeof(ck), hipMemcpyHostToDevice));
+    ctx->nloop = c->nloop;
+    ctx->nsrate = c->nsrate;
+    return GF_OK;
+}
+
+// read the clock back and turn a device-side STOP into its status code
+static int fetch_clock(gf_ctx *ctx, gf_clock *c)
+{
+    GfDevClock ck;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(&ck, ctx->dck, sizeof(ck), hipMemcpyDeviceToHost));
+    if (c) {
+        c->nloop = ctx->nloop; c->nsrate = ctx->nsrate;
+        c->time = ck.time; c->dt = ck.dt; c->dtavg = ck.dtavg;
+        c->dt_elastic = ck.dt_elastic; c->dt_maxwell = ck.dt_maxwell;
+    }
+    if (ck.error_code != 0) {
+        long long idx = ck.error_index;
+        long rows = ctx->p.nz - 1;
+        ctx->err_j = (int)(idx % rows) + 1;
+        ctx->err_i = (int)(idx / rows) + 1;
+        // clear, so that a caller may inspect and continue deliberately
+        GfDevClock z = ck;
+        z.error_code = 0;
+        z.error_index = (long long)1 << 62;
+        HIPCHK(hipMemcpy(ctx->dck, &z, sizeof(z), hipMemcpyHostToDevice));
+        return set_err(ctx, ck.error_code, "reference STOP %d at element (j,i)=(%d,%d)", ck.error_code,
+                       ctx->err_j, ctx->err_i);
+    }
+    return GF_OK;
+}
+
+extern "C" int gf_get_clock(gf_ctx *ctx, gf_clock *c)
+{
+    if (!ctx || !c) return GF_ERR_ARG;
+    return fetch_clock(ctx, c);
+}
+
+// ------------------------------------------------------------------ kernel sequences
+static int launch_check(gf_ctx *ctx, const char *what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_err(ctx, GF_ERR_HIP, "launch of %s failed: %s", what, hipGetErrorString(e));
+    return GF_OK;
+}
+
+static hipEvent_t next_event(gf_ctx *ctx)
+{
+    if (ctx->ev_used == ctx->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        ctx->ev_pool.push_back(e);
+    }
+    return ctx->ev_pool[ctx->ev_used++];
+}
+
+// fold the recorded event pairs into the per-kernel totals (after a stream sync)
+static int collect_timing(gf_ctx *ctx)
+{
+    if (ctx->pending.empty()) return GF_OK;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (auto &p : ctx->pending) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+        auto &t = ctx->totals[p.first];
+        t.first += ms;
+        t.second += 1;
+    }
+    ctx->pending.clear();
+    ctx->ev_used = 0;
+    return GF_OK;
+}
+
+#define LAUNCH(kern, n, ...)                                                               \
+    do {                                                                                   \
+        hipEvent_t e0_ = nullptr, e1_ = nullptr;                                           \
+        if (ctx->timing) {                                                                 \
+            e0_ = next_event(ctx);                                                         \
+            e1_ = next_event(ctx);                                                         \
+            if (e0_) (void)hipEventRecord(e0_, ctx->stream);                               \
+        }                                                                                  \
+        hipLaunchKernelGGL(kern, dim3(nblk(n)), dim3(256), 0, ctx->stream, __VA_ARGS__);   \
+        int rc_ = launch_check(ctx, #kern);                                                \
+        if (rc_) return rc_;                                                               \
+        if (ctx->timing && e0_ && e1_) {                                                   \
+            (void)hipEventRecord(e1_, ctx->stream);                                        \
+            ctx->pending.push_back({#kern, {e0_, e1_}});                                   \
+            if (ctx->pending.size() > 4096 && (rc_ = collect_timing(ctx))) return rc_;     \
+        }                                                                                  \
+    } while (0)
+
+static int enq_clock(gf_ctx *ctx, int op)
+{
+    hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, ctx->stream, ctx->dck, op);
+    return launch_check(ctx, "k_clock");
+}
+
+static int enq_therm(gf_ctx *ctx)
+{
+    const long nn = ctx->nn;
+    int rc;
+    LAUNCH(k_copy, nn, ctx->a.temp0, ctx->a.temp, nn);
+    if ((rc = enq_clock(ctx, CK_THERM_BEGIN))) return rc;
+    const int S = ctx->therm_sweeps;
+    for (int s = 0; s < S; s++) {
+        const double *in = s == 0 ? ctx->a.temp0 : ((s - 1) % 2 == 0 ? ctx->a.xa : ctx->a.xb);
+        double *out = (s % 2 == 0) ? ctx->a.xa : ctx->a.xb;
+        LAUNCH(k_therm_sweep, nn, ctx->dp, ctx->dck, ctx->a, ctx->a.temp0, in, out, s);
+    }
+    const double *last = ((S - 1) % 2 == 0) ? ctx->a.xa : ctx->a.xb;
+    const double *prev = S == 1 ? ctx->a.temp0 : (((S - 2) % 2 == 0) ? ctx->a.xa : ctx->a.xb);
+    LAUNCH(k_therm_final, nn, ctx->dp, ctx->dck, ctx->a, last, prev, S);
+    return GF_OK;
+}
+
+static int enq_rheol(gf_ctx *ctx)
+{
+    int rc;
+    const int do_visc = (ctx->nloop % ctx->p.ifreq_visc) == 0;
+    int avgsr = 0;
+    if (ctx->nsrate == ctx->p.ifreq_avgsr)
+        avgsr = 1;
+    else if (ctx->nsrate == -1)
+        avgsr = 2;
+    if ((rc = enq_clock(ctx, CK_DTAVG_ADD))) return rc;
+    LAUNCH(k_rheol, ctx->ne, ctx->dp, ctx->dck, ctx->a, do_visc, avgsr);
+    if (avgsr == 1) {
+        if ((rc = enq_clock(ctx, CK_DTAVG_RESET))) return rc;
+        ctx->nsrate = 0;
+    }
+    ctx->nsrate = ctx->nsrate + 1;
+    return GF_OK;
+}
+
+static int enq_node(gf_ctx *ctx)
+{
+    LAUNCH(k_node, ctx->nn, ctx->dp, ctx->dck, ctx->a);
+    return GF_OK;
+}
+
+static int enq_move(gf_ctx *ctx)
+{
+    if (ctx->p.movegrid == 0) return GF_OK;
+    LAUNCH(k_move_cord, ctx->nn, ctx->dp, ctx->dck, ctx->a);
+    LAUNCH(k_move_elem, ctx->ne, ctx->dp, ctx->dck, ctx->a);
+    return GF_OK;
+}
+
+static int enq_rmasses(gf_ctx *ctx)
+{
+    LAUNCH(k_dens, ctx->ne, ctx->dp, ctx->dck, ctx->a);
+    LAUNCH(k_rmass, ctx->nn, ctx->dp, ctx->dck, ctx->a);
+    return GF_OK;
+}
+
+static int enq_dt_mass(gf_ctx *ctx)
+{
+    int rc;
+    if ((rc = enq_clock(ctx, CK_DT_BEGIN))) return rc;
+    LAUNCH(k_dt_reduce, ctx->ne, ctx->dp, ctx->dck, ctx->a);
+    hipLaunchKernelGGL(k_dt_scalars, dim3(1), dim3(64), 0, ctx->stream, ctx->dp, ctx->dck, ctx->d_present,
+                       ctx->npresent);
+    if ((rc = launch_check(ctx, "k_dt_scalars"))) return rc;
+    LAUNCH(k_amass, ctx->nn, ctx->dp, ctx->dck, ctx->a);
+    return GF_OK;
+}
+
+static const uint32_t kRequired = GF_CORD | GF_VEL | GF_TEMP | GF_AMASS | GF_RMASS | GF_BC | GF_NCOD | GF_STRESS0 |
+                                  GF_AREA | GF_DVOL | GF_STRAIN | GF_SE2SR | GF_APS | GF_VISN | GF_E2SR |
+                                  GF_SSHRHEAT | GF_SHRHEAT | GF_SOURCE | GF_IPHASE | GF_PHASE_RATIO;
+
+static int check_ready(gf_ctx *ctx)
+{
+    if ((ctx->uploaded & kRequired) != kRequired)
+        return set_err(ctx, GF_ERR_STATE, "state not uploaded (mask 0x%x missing)", kRequired & ~ctx->uploaded);
+    return GF_OK;
+}
+
+// one call of subroutine flac, src/flac.f90:18-83
+static int enq_flac(gf_ctx *ctx)
+{
+    int rc;
+    if ((rc = enq_therm(ctx))) return rc;
+    if (ctx->p.itherm != 2) {
+        // change_phase (every ifreq_rmasses) has no rule for the phases this build accepts
+        if ((rc = enq_rheol(ctx))) return rc;
+        if ((rc = enq_node(ctx))) return rc;
+        if ((rc = enq_move(ctx))) return rc;
+        if (ctx->nloop % ctx->p.ifreq_rmasses == 0 && (rc = enq_rmasses(ctx))) return rc;
+        if (ctx->nloop % ctx->p.ifreq_imasses == 0 && (rc = enq_dt_mass(ctx))) return rc;
+    }
+    return GF_OK;
+}
+
+extern "C" int gf_step(gf_ctx *ctx, int32_t nsteps, gf_clock *clock)
+{
+    if (!ctx || nsteps < 0) return GF_ERR_ARG;
+    int rc = check_ready(ctx);
+    if (rc) return rc;
+    for (int32_t n = 0; n < nsteps; n++) {
+        if ((rc = enq_flac(ctx))) return rc;
+        ctx->nloop = ctx->nloop + 1;
+        if ((rc = enq_clock(ctx, CK_TIME_ADVANCE))) return rc;
+    }
+    return fetch_clock(ctx, clock);
+}
+
+extern "C" int gf_flac(gf_ctx *ctx, gf_clock *clock)
+{
+    if (!ctx || !clock) return GF_ERR_ARG;
+    int rc = check_ready(ctx);
+    if (rc) return rc;
+    // the caller owns nloop and time between calls (src/par.f90:83-86)
+    ctx->nloop = clock->nloop;
+    ctx->nsrate = clock->nsrate;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(&ctx->dck->time, &clock->time, sizeof(double), hipMemcpyHostToDevice));
+    if ((rc = enq_flac(ctx))) return rc;
+    return fetch_clock(ctx, clock);
+}
+
+#define SINGLE(name, body)                                                \
+    extern "C" int name(gf_ctx *ctx, gf_clock *clock)                     \
+    {                                                                     \
+        if (!ctx) return GF_ERR_ARG;                                      \
+        int rc = check_ready(ctx);                                        \
+        if (rc) return rc;                                                \
+        if (clock) {                                                      \
+            ctx->nloop = clock->nloop;                                    \
+            ctx->nsrate = clock->nsrate;                                  \
+        }                                                                 \
+        if ((rc = (body))) return rc;                                     \
+        return fetch_clock(ctx, clock);                                   \
+    }
+
+SINGLE(gf_fl_therm, enq_therm(ctx))
+SINGLE(gf_fl_rheol, enq_rheol(ctx))
+SINGLE(gf_fl_node, enq_node(ctx))
+SINGLE(gf_fl_move, enq_move(ctx))
+SINGLE(gf_rmasses, enq_rmasses(ctx))
+SINGLE(gf_dt_mass, enq_dt_mass(ctx))
+
+extern "C" int gf_init_areas(gf_ctx *ctx)
+{
+    if (!ctx) return GF_ERR_ARG;
+    LAUNCH(k_init_areas, ctx->ne, ctx->dp, ctx->dck, ctx->a);
+    return fetch_clock(ctx, nullptr);
+}
+
+extern "C" int gf_init_visc(gf_ctx *ctx)
+{
+    if (!ctx) return GF_ERR_ARG;
+    LAUNCH(k_init_visc, ctx->ne, ctx->dp, ctx->dck, ctx->a);
+    return fetch_clock(ctx, nullptr);
+}
+
+extern "C" int gf_set_therm_sweeps(gf_ctx *ctx, int32_t nsweeps)
+{
+    if (!ctx || nsweeps < 1 || nsweeps > GF_MAX_SWEEPS) return GF_ERR_ARG;
+    ctx->therm_sweeps = nsweeps;
+    return GF_OK;
+}
+
+extern "C" int gf_therm_unconverged(gf_ctx *ctx, int64_t *count)
+{
+    if (!ctx || !count) return GF_ERR_ARG;
+    GfDevClock ck;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(&ck, ctx->dck, sizeof(ck), hipMemcpyDeviceToHost));
+    *count = ck.therm_unconverged;
+    unsigned long long z = 0;
+    HIPCHK(hipMemcpy(&ctx->dck->therm_unconverged, &z, sizeof(z), hipMemcpyHostToDevice));
+    return GF_OK;
+}
+
+extern "C" int gf_sync(gf_ctx *ctx)
+{
+    if (!ctx) return GF_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return GF_OK;
+}
+
+extern "C" int gf_set_timing(gf_ctx *ctx, int32_t on)
+{
+    if (!ctx) return GF_ERR_ARG;
+    int rc = collect_timing(ctx);
+    if (rc) return rc;
+    ctx->timing = on != 0;
+    ctx->totals.clear();
+    return GF_OK;
+}
+
+extern "C" int gf_kernel_time(gf_ctx *ctx, const char *name, double *total_ms, int64_t *launches)
+{
+    if (!ctx || !name) return GF_ERR_ARG;
+    int rc = collect_timing(ctx);
+    if (rc) return rc;
+    auto it = ctx->totals.find(name);
+    if (total_ms) *total_ms = it == ctx->totals.end() ? 0.0 : it->second.first;
+    if (launches) *launches = it == ctx->totals.end() ? 0 : it->second.second;
+    return GF_OK;
+}

@@ -1,0 +1,154 @@
+"""ctypes binding of libgfhip.so (include/gf_abi.h) for Python hosts and tests.
+
+This is a thin caller of the C-ABI: every computation runs in the HIP kernels of
+the library.  Importing it without the built library, or calling it without a
+HIP device, raises at once; there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgfhip.so")
+_lib = None
+
+EXPORTS = [
+    "gf_create", "gf_destroy", "gf_last_error", "gf_error_site", "gf_bind_host", "gf_upload",
+    "gf_download", "gf_set_clock", "gf_get_clock", "gf_step", "gf_flac", "gf_fl_therm",
+    "gf_fl_rheol", "gf_fl_node", "gf_fl_move", "gf_rmasses", "gf_dt_mass", "gf_init_areas",
+    "gf_init_visc", "gf_set_therm_sweeps", "gf_therm_unconverged", "gf_sync",
+    "gf_set_timing", "gf_kernel_time",
+]
+
+
+class GfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("libgfhip status %d: %s" % (code, msg))
+        self.code = code
+
+
+def lib():
+    """Load libgfhip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libgfhip.so is not built; run geoflac_amd/build.py (or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.gf_create.argtypes = [C.POINTER(abi.GfParams), C.c_int32, C.POINTER(vp)]
+        L.gf_destroy.argtypes = [vp]
+        L.gf_destroy.restype = None
+        L.gf_last_error.argtypes = [vp]
+        L.gf_last_error.restype = C.c_char_p
+        L.gf_error_site.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.gf_bind_host.argtypes = [vp, C.POINTER(abi.GfHostArrays)]
+        L.gf_upload.argtypes = [vp, C.c_uint32]
+        L.gf_download.argtypes = [vp, C.c_uint32]
+        L.gf_set_clock.argtypes = [vp, C.POINTER(abi.GfClock)]
+        L.gf_get_clock.argtypes = [vp, C.POINTER(abi.GfClock)]
+        L.gf_step.argtypes = [vp, C.c_int32, C.POINTER(abi.GfClock)]
+        L.gf_flac.argtypes = [vp, C.POINTER(abi.GfClock)]
+        for n in ("gf_fl_therm", "gf_fl_rheol", "gf_fl_node", "gf_fl_move", "gf_rmasses", "gf_dt_mass"):
+            getattr(L, n).argtypes = [vp, C.POINTER(abi.GfClock)]
+        L.gf_init_areas.argtypes = [vp]
+        L.gf_init_visc.argtypes = [vp]
+        L.gf_set_therm_sweeps.argtypes = [vp, C.c_int32]
+        L.gf_therm_unconverged.argtypes = [vp, C.POINTER(C.c_int64)]
+        L.gf_sync.argtypes = [vp]
+        L.gf_set_timing.argtypes = [vp, C.c_int32]
+        L.gf_kernel_time.argtypes = [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        _lib = L
+    return _lib
+
+
+class Model:
+    """One model on the device, bound to a geoflac_amd.state.State's host arrays."""
+
+    def __init__(self, params, state, precision=abi.PREC_FP64, upload=True):
+        self.params = params
+        self.state = state
+        self._ctx = C.c_void_p()
+        L = lib()
+        rc = L.gf_create(C.byref(params), int(precision), C.byref(self._ctx))
+        if rc != 0:
+            raise GfError(rc, "gf_create rejected the parameters")
+        self._host = state.host_arrays()
+        self._check(L.gf_bind_host(self._ctx, C.byref(self._host)))
+        if upload:
+            self.upload()
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = lib().gf_last_error(self._ctx)
+            raise GfError(rc, msg.decode() if msg else "")
+        return rc
+
+    def upload(self, mask=abi.GF_ALL):
+        self._check(lib().gf_upload(self._ctx, mask))
+        self._check(lib().gf_set_clock(self._ctx, C.byref(self.state.clock)))
+
+    def set_clock(self):
+        self._check(lib().gf_set_clock(self._ctx, C.byref(self.state.clock)))
+
+    def download(self, mask=abi.GF_ALL):
+        self._check(lib().gf_download(self._ctx, mask))
+        self._check(lib().gf_get_clock(self._ctx, C.byref(self.state.clock)))
+
+    def step(self, nsteps):
+        """nsteps iterations of the reference's inner loop (flac; nloop+=1; time+=dt)."""
+        return self._check(lib().gf_step(self._ctx, int(nsteps), C.byref(self.state.clock)))
+
+    def call(self, name):
+        """One reference subroutine: 'fl_therm', 'fl_rheol', 'fl_node', 'fl_move',
+        'rmasses', 'dt_mass', 'flac'."""
+        fn = getattr(lib(), "gf_" + name)
+        return self._check(fn(self._ctx, C.byref(self.state.clock)))
+
+    def init_areas(self):
+        return self._check(lib().gf_init_areas(self._ctx))
+
+    def init_visc(self):
+        return self._check(lib().gf_init_visc(self._ctx))
+
+    def set_therm_sweeps(self, n):
+        return self._check(lib().gf_set_therm_sweeps(self._ctx, int(n)))
+
+    def therm_unconverged(self):
+        v = C.c_int64()
+        self._check(lib().gf_therm_unconverged(self._ctx, C.byref(v)))
+        return v.value
+
+    def error_site(self):
+        j, i = C.c_int32(), C.c_int32()
+        lib().gf_error_site(self._ctx, C.byref(j), C.byref(i))
+        return j.value, i.value
+
+    def set_timing(self, on):
+        return self._check(lib().gf_set_timing(self._ctx, int(bool(on))))
+
+    def kernel_time(self, name):
+        """(total ms, launches) of one kernel since timing was switched on."""
+        ms, n = C.c_double(), C.c_int64()
+        self._check(lib().gf_kernel_time(self._ctx, name.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def sync(self):
+        return self._check(lib().gf_sync(self._ctx))
+
+    def close(self):
+        if self._ctx:
+            lib().gf_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -153,6 +153,13 @@ int gf_therm_unconverged(gf_ctx *ctx, int64_t *count);
 /* Synchronise the context's stream (all queued steps finished). */
 int gf_sync(gf_ctx *ctx);
 
+/* Per-kernel timing with HIP events recorded on the context's stream around each
+ * launch (off by default).  gf_kernel_time returns the summed duration in ms and the
+ * launch count of kernel `name` (e.g. "k_rheol") since timing was switched on.
+ * Instrumentation of this build; the reference has none. */
+int gf_set_timing(gf_ctx *ctx, int32_t on);
+int gf_kernel_time(gf_ctx *ctx, const char *name, double *total_ms, int64_t *launches);
+
 #ifdef __cplusplus
 }
 #endif
