@@ -11,6 +11,7 @@
 #include <stdint.h>
 
 #include "../../include/gf_abi.h"
+#include "gf_ddmath.h"
 
 #define GF_DEV static __device__ __forceinline__
 
@@ -118,8 +119,8 @@ GF_DEV PlasticConst plastic_const(double bulkm, double rmu, double coh, double p
 {
     const double degrad = 3.14159265358979323846 / 180.0, c4d3 = 4.0 / 3.0, c2d3 = 2.0 / 3.0;
     PlasticConst k;
-    double sphi = sin(phi * degrad);
-    double spsi = sin(psi * degrad);
+    double sphi = cr_sin(phi * degrad);
+    double spsi = cr_sin(psi * degrad);
     k.anphi = (1.0 + sphi) / (1.0 - sphi);
     k.anpsi = (1.0 + spsi) / (1.0 - spsi);
     k.amc = 2.0 * coh * sqrt(k.anphi);
@@ -129,7 +130,7 @@ GF_DEV PlasticConst plastic_const(double bulkm, double rmu, double coh, double p
     if (phi == 0.0)
         k.ten_max = ten_off;
     else
-        k.ten_max = fmin(ten_off, coh / (tan(phi * degrad)));
+        k.ten_max = fmin(ten_off, coh / (cr_tan(phi * degrad)));
     return k;
 }
 

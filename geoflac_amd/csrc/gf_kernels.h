@@ -55,6 +55,20 @@ __global__ void k_clock(GfDevClock *ck, int op)
     }
 }
 
+// correctly rounded math of the FP64 path, evaluated for a list of arguments
+// (fn: 0 sin, 1 tan, 2 exp, 3 pow with y = args2)
+__global__ void k_math_check(const double *x, const double *y, double *out, long n, int fn)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double v = 0.0;
+    if (fn == 0) v = cr_sin(x[t]);
+    if (fn == 1) v = cr_tan(x[t]);
+    if (fn == 2) v = cr_exp(x[t]);
+    if (fn == 3) v = cr_pow(x[t], y[t]);
+    out[t] = v;
+}
+
 // ------------------------------------------------------------------ fl_therm
 // src/fl_therm.f90:393-571.  The reference updates temp in place in the serial
 // order (i outer, j inner): node (j,i) reads the NEW values of (j-1,i), (j,i-1),
@@ -233,10 +247,10 @@ GF_DEV double dev_eff_visc(const GfDevParams *P, const GfDevArrays &A, int j, in
     double deptmpr410 = -101.58 * (tmpr + 273.0) - 216828.0;
     double deptmpr660 = 66.55 * (tmpr + 273.0) - 788494.0;
     if (zcord < deptmpr660) {
-        ev = 1e22 * exp(-0.0115 * (tmpr - P->t_bot));
+        ev = 1e22 * cr_exp(-0.0115 * (tmpr - P->t_bot));
         return fmin(P->v_max, fmax(P->v_min, ev));
     } else if (zcord < deptmpr410) {
-        ev = 1e21 * exp(-0.0115 * (tmpr - P->t_bot));
+        ev = 1e21 * cr_exp(-0.0115 * (tmpr - P->t_bot));
         return fmin(P->v_max, fmax(P->v_min, ev));
     }
     double srat = A.e2sr[IE(j, i)];
@@ -245,8 +259,8 @@ GF_DEV double dev_eff_visc(const GfDevParams *P, const GfDevArrays &A, int j, in
         double r = prv[a];
         if (r < 0.01) continue;
         const GfPhase &ph = PHASE(P->active[a]);
-        double vis = 0.25 * pow(srat, ph.pw) * ph.acoef_pow *
-                     exp((ph.eactiv + ph.vactiv * pres) / (ph.pln_r * (tmpr + 273.0))) * 1.0e+6;
+        double vis = 0.25 * cr_pow(srat, ph.pw) * ph.acoef_pow *
+                     cr_exp((ph.eactiv + ph.vactiv * pres) / (ph.pln_r * (tmpr + 273.0))) * 1.0e+6;
         if (vis < P->v_min) vis = P->v_min;
         if (vis > P->v_max) vis = P->v_max;
         ev = ev + r / vis;

@@ -691,3 +691,29 @@ extern "C" int gf_kernel_time(gf_ctx *ctx, const char *name, double *total_ms, i
     if (launches) *launches = it == ctx->totals.end() ? 0 : it->second.second;
     return GF_OK;
 }
+
+// Device evaluation of the correctly rounded functions of the FP64 path for n host
+// arguments (fn: 0 sin, 1 tan, 2 exp, 3 pow(x, y)); a check of this build, no
+// counterpart in the reference.
+extern "C" int gf_math_check(int32_t fn, const double *x, const double *y, double *out, int64_t n)
+{
+    gf_ctx *ctx = nullptr;
+    if (!x || !out || n <= 0 || fn < 0 || fn > 3 || (fn == 3 && !y)) return GF_ERR_ARG;
+    double *dx = nullptr, *dy = nullptr, *dout = nullptr;
+    int rc = GF_OK;
+    if (hipMalloc((void **)&dx, 8 * n) != hipSuccess || hipMalloc((void **)&dy, 8 * n) != hipSuccess ||
+        hipMalloc((void **)&dout, 8 * n) != hipSuccess)
+        rc = GF_ERR_HIP;
+    if (!rc && hipMemcpy(dx, x, 8 * n, hipMemcpyHostToDevice) != hipSuccess) rc = GF_ERR_HIP;
+    if (!rc && hipMemcpy(dy, fn == 3 ? y : x, 8 * n, hipMemcpyHostToDevice) != hipSuccess) rc = GF_ERR_HIP;
+    if (!rc) {
+        hipLaunchKernelGGL(k_math_check, dim3(nblk(n)), dim3(256), 0, 0, dx, dy, dout, (long)n, (int)fn);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = GF_ERR_HIP;
+    }
+    if (!rc && hipMemcpy(out, dout, 8 * n, hipMemcpyDeviceToHost) != hipSuccess) rc = GF_ERR_HIP;
+    if (dx) (void)hipFree(dx);
+    if (dy) (void)hipFree(dy);
+    if (dout) (void)hipFree(dout);
+    (void)ctx;
+    return rc;
+}

@@ -18,7 +18,7 @@ EXPORTS = [
     "gf_download", "gf_set_clock", "gf_get_clock", "gf_step", "gf_flac", "gf_fl_therm",
     "gf_fl_rheol", "gf_fl_node", "gf_fl_move", "gf_rmasses", "gf_dt_mass", "gf_init_areas",
     "gf_init_visc", "gf_set_therm_sweeps", "gf_therm_unconverged", "gf_sync",
-    "gf_set_timing", "gf_kernel_time",
+    "gf_set_timing", "gf_kernel_time", "gf_math_check",
 ]
 
 
@@ -58,6 +58,8 @@ def lib():
         L.gf_sync.argtypes = [vp]
         L.gf_set_timing.argtypes = [vp, C.c_int32]
         L.gf_kernel_time.argtypes = [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        dp = C.POINTER(C.c_double)
+        L.gf_math_check.argtypes = [C.c_int32, dp, dp, dp, C.c_int64]
         _lib = L
     return _lib
 
@@ -152,3 +154,18 @@ class Model:
 
     def __exit__(self, *a):
         self.close()
+
+
+def math_check(fn, x, y=None):
+    """Device values of the correctly rounded sin (0), tan (1), exp (2), pow (3)."""
+    import numpy as np
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty_like(x)
+    yy = None if y is None else np.ascontiguousarray(y, dtype=np.float64)
+    dp = C.POINTER(C.c_double)
+    rc = lib().gf_math_check(int(fn), x.ctypes.data_as(dp),
+                             None if yy is None else yy.ctypes.data_as(dp),
+                             out.ctypes.data_as(dp), x.size)
+    if rc != 0:
+        raise GfError(rc, "gf_math_check failed")
+    return out

@@ -158,6 +158,10 @@ int gf_sync(gf_ctx *ctx);
  * launch count of kernel `name` (e.g. "k_rheol") since timing was switched on.
  * Instrumentation of this build; the reference has none. */
 int gf_set_timing(gf_ctx *ctx, int32_t on);
+/* Evaluates on the device, for n host arguments, the correctly rounded functions the
+ * FP64 path uses in place of libm (fn: 0 sin, 1 tan, 2 exp, 3 pow(x, y)), so a test
+ * can compare them with the host libm the reference calls.  No reference counterpart. */
+int gf_math_check(int32_t fn, const double *x, const double *y, double *out, int64_t n);
 int gf_kernel_time(gf_ctx *ctx, const char *name, double *total_ms, int64_t *launches);
 
 #ifdef __cplusplus

@@ -19,6 +19,23 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Transcendental functions: the host libm (what the reference calls) by default;
+ * with GFO_CR_MATH the correctly rounded double-double versions the device uses
+ * (geoflac_amd/csrc/gf_ddmath.h), so the device can be checked bit for bit over long
+ * runs, past the point where glibc's few non-correctly-rounded results matter. */
+#ifdef GFO_CR_MATH
+#include "../geoflac_amd/csrc/gf_ddmath.h"
+static inline double gfo_sin(double x) { return cr_sin(x); }
+static inline double gfo_tan(double x) { return cr_tan(x); }
+static inline double gfo_exp(double x) { return cr_exp(x); }
+static inline double gfo_pow(double x, double y) { return cr_pow(x, y); }
+#else
+static inline double gfo_sin(double x) { return sin(x); }
+static inline double gfo_tan(double x) { return tan(x); }
+static inline double gfo_exp(double x) { return exp(x); }
+static inline double gfo_pow(double x, double y) { return pow(x, y); }
+#endif
+
 #define NZ (p->nz)
 #define NX (p->nx)
 #define NE ((long)(NZ - 1) * (NX - 1))
@@ -131,8 +148,8 @@ void gfo_plastic(double bulkm, double rmu, double coh, double phi, double psi, d
     *depls = 0.0;
     *diss = 0.0;
     *ipls = 0;
-    sphi = sin(phi * degrad);
-    spsi = sin(psi * degrad);
+    sphi = gfo_sin(phi * degrad);
+    spsi = gfo_sin(psi * degrad);
     anphi = (1.0 + sphi) / (1.0 - sphi);
     anpsi = (1.0 + spsi) / (1.0 - spsi);
     amc = 2.0 * coh * sqrt(anphi);
@@ -142,7 +159,7 @@ void gfo_plastic(double bulkm, double rmu, double coh, double phi, double psi, d
     if (phi == 0.0)
         ten_max = ten_off;
     else
-        ten_max = fmin(ten_off, coh / (tan(phi * degrad)));
+        ten_max = fmin(ten_off, coh / (gfo_tan(phi * degrad)));
 
     s11i = *s11 + (de22 + de33) * e2 + de11 * e1 - press_add;
     s22i = *s22 + (de11 + de33) * e2 + de22 * e1 - press_add;
@@ -296,10 +313,10 @@ double gfo_eff_visc(const gf_params *p, const gf_host_arrays *s, int j, int i)
     deptmpr410 = -101.58 * (tmpr + 273.0) - 216828.0;
     deptmpr660 = 66.55 * (tmpr + 273.0) - 788494.0;
     if (zcord < deptmpr660) {
-        ev = 1e22 * exp(-0.0115 * (tmpr - p->t_bot));
+        ev = 1e22 * gfo_exp(-0.0115 * (tmpr - p->t_bot));
         return fmin(p->v_max, fmax(p->v_min, ev));
     } else if (zcord < deptmpr410) {
-        ev = 1e21 * exp(-0.0115 * (tmpr - p->t_bot));
+        ev = 1e21 * gfo_exp(-0.0115 * (tmpr - p->t_bot));
         return fmin(p->v_max, fmax(p->v_min, ev));
     }
     srat = E2SR(j, i);
@@ -309,8 +326,8 @@ double gfo_eff_visc(const gf_params *p, const gf_host_arrays *s, int j, int i)
         if (PR(k, j, i) < 0.01) continue;
         pw = 1.0 / PH(pln, k) - 1.0;
         pw1 = -1.0 / PH(pln, k);
-        vis = 0.25 * pow(srat, pw) * pow(0.75 * PH(acoef, k), pw1) *
-              exp((PH(eactiv, k) + PH(vactiv, k) * pres) / (PH(pln, k) * r * (tmpr + 273.0))) *
+        vis = 0.25 * gfo_pow(srat, pw) * pow(0.75 * PH(acoef, k), pw1) *
+              gfo_exp((PH(eactiv, k) + PH(vactiv, k) * pres) / (PH(pln, k) * r * (tmpr + 273.0))) *
               1.0e+6;
         if (vis < p->v_min) vis = p->v_min;
         if (vis > p->v_max) vis = p->v_max;
@@ -1129,6 +1146,17 @@ int gfo_flac(const gf_params *p, gf_host_arrays *s, gf_clock *c)
         if (c->nloop % p->ifreq_imasses == 0 && (st = gfo_dt_mass(p, s, c))) return st;
     }
     return 0;
+}
+
+/* the transcendental functions of this build (fn: 0 sin, 1 tan, 2 exp, 3 pow) */
+double gfo_math(int fn, double x, double y)
+{
+    switch (fn) {
+    case 0: return gfo_sin(x);
+    case 1: return gfo_tan(x);
+    case 2: return gfo_exp(x);
+    default: return gfo_pow(x, y);
+    }
 }
 
 /* src/par.f90:81-91 */

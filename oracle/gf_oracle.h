@@ -28,6 +28,8 @@ int gfo_init_visc(const gf_params *p, gf_host_arrays *s);
 int gfo_flac(const gf_params *p, gf_host_arrays *s, gf_clock *c);
 /* nsteps iterations of src/par.f90:81-91 */
 int gfo_step(const gf_params *p, gf_host_arrays *s, gf_clock *c, int nsteps);
+/* the transcendental functions in use (fn: 0 sin, 1 tan, 2 exp, 3 pow) */
+double gfo_math(int fn, double x, double y);
 /* closed-form pieces, exposed for known-answer tests */
 double gfo_eff_visc(const gf_params *p, const gf_host_arrays *s, int j, int i);
 double gfo_eff_dens(const gf_params *p, const gf_host_arrays *s, int j, int i);

@@ -6,19 +6,24 @@ from geoflac_amd import abi, inp, state
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLDEN = os.path.join(ROOT, "tests", "golden")
-_LIB = None
+_LIBS = {}
 
 
-def lib():
-    global _LIB
-    if _LIB is None:
-        path = os.path.join(ROOT, "oracle", "liboracle.so")
+def lib(cr=False):
+    """The oracle with the host libm (cr=False: what the reference calls) or with the
+    correctly rounded functions the device uses (cr=True)."""
+    if cr not in _LIBS:
+        name = "liboracle_cr.so" if cr else "liboracle.so"
+        path = os.path.join(ROOT, "oracle", name)
         if not os.path.exists(path):
-            raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
-        _LIB = C.CDLL(path)
-        _LIB.gfo_eff_visc.restype = C.c_double
-        _LIB.gfo_eff_dens.restype = C.c_double
-    return _LIB
+            raise RuntimeError("oracle/%s is not built (make -C oracle)" % name)
+        L = C.CDLL(path)
+        L.gfo_eff_visc.restype = C.c_double
+        L.gfo_eff_dens.restype = C.c_double
+        L.gfo_math.restype = C.c_double
+        L.gfo_math.argtypes = [C.c_int, C.c_double, C.c_double]
+        _LIBS[cr] = L
+    return _LIBS[cr]
 
 
 def golden_params(case):
@@ -29,11 +34,11 @@ def golden_state(case, step):
     return state.load_npz(os.path.join(GOLDEN, "%s_s%d.npz" % (case, step)))
 
 
-def step(p, st, nsteps):
+def step(p, st, nsteps, cr=False):
     h = st.host_arrays()
-    return lib().gfo_step(C.byref(p), C.byref(h), C.byref(st.clock), int(nsteps))
+    return lib(cr).gfo_step(C.byref(p), C.byref(h), C.byref(st.clock), int(nsteps))
 
 
-def call(name, p, st):
+def call(name, p, st, cr=False):
     h = st.host_arrays()
-    return getattr(lib(), name)(C.byref(p), C.byref(h), C.byref(st.clock))
+    return getattr(lib(cr), name)(C.byref(p), C.byref(h), C.byref(st.clock))
