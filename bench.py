@@ -34,17 +34,17 @@ def alg_bytes(kernel, E, N, nact, p):
         per_e += 128 + 8 + 24 + 24 + 8                                 # writes stress0 aps strain se2sr sshrheat
         per_n = 16 + 16                                                # vel, cord
         return E * per_e + N * per_n
-    if kernel == "k_therm_sweep":
-        return N * (8 + 8 + 8 + 16) + E * (16 + 4 + 8)                # told xin xout cord | area(1:2) iphase source
+    if kernel == "k_therm_tile":
+        return N * (8 + 8 + 8 + 16) + E * (16 + 4 + 8)                # temp, temp0 out, x out, cord | area(1:2) iphase source
+    if kernel == "k_therm_verify":
+        return N * (8 + 8 + 8 + 16) + E * (16 + 4 + 8)                # temp0, x in, x out, cord | area(1:2) iphase source
     if kernel == "k_node":
         return E * (96 + 32) + N * (16 + 16 + 16 + 8 + 16 + 16 + 16)  # stress0(1:3) area | cord rmass+amass vel ncod bc, vel+force out
     if kernel == "k_move_elem":
         return E * (32 + 96 + 24 + 32 + 32 + 96 + 24) + N * 32
     if kernel == "k_move_cord":
         return N * 48
-    if kernel == "k_therm_final":
-        return N * 16
-    if kernel == "k_copy":
+    if kernel == "k_therm_bc":
         return N * 16
     return 0
 
@@ -112,8 +112,8 @@ def main():
     # profiled run: HIP events around every launch on the library stream
     m.set_timing(True)
     m.step(args.profile_steps)
-    kernels = ["k_rheol", "k_therm_sweep", "k_node", "k_move_elem", "k_move_cord", "k_therm_final",
-               "k_copy", "k_dens", "k_rmass", "k_dt_reduce", "k_amass", "k_clock"]
+    kernels = ["k_rheol", "k_therm_tile", "k_therm_verify", "k_node", "k_move_elem", "k_move_cord",
+               "k_therm_bc", "k_dens", "k_rmass", "k_dt_reduce", "k_amass", "k_clock"]
     times = {k: m.kernel_time(k) for k in kernels}
     m.set_timing(False)
     dom = max(times, key=lambda k: times[k][0])

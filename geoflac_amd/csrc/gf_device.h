@@ -53,7 +53,7 @@ struct GfDevClock {
     int error_code, pad;
     long long error_index;      // smallest linear index (reference loop order) that failed
     unsigned long long therm_unconverged;
-    unsigned int sweep_changed[GF_MAX_SWEEPS];
+    unsigned int therm_pass_changed[GF_MAX_SWEEPS];  // bit 0: first sweep changed a bit; bit 1: a tile did not settle
 };
 
 // ---------------------------------------------------------------- helpers

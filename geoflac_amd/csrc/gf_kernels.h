@@ -48,7 +48,7 @@ __global__ void k_clock(GfDevClock *ck, int op)
     if (op == CK_DTAVG_RESET) ck->dtavg = 0.0;
     if (op == CK_TIME_ADVANCE) ck->time = ck->time + ck->dt;
     if (op == CK_THERM_BEGIN)
-        for (int k = 0; k < GF_MAX_SWEEPS; k++) ck->sweep_changed[k] = 0u;
+        for (int k = 0; k < GF_MAX_SWEEPS; k++) ck->therm_pass_changed[k] = 0u;
     if (op == CK_DT_BEGIN) {
         ck->dlmin_bits = (unsigned long long)__double_as_longlong(1.0e28);
         ck->velmax_bits = 0ull;
@@ -72,137 +72,209 @@ __global__ void k_math_check(const double *x, const double *y, double *out, long
 // ------------------------------------------------------------------ fl_therm
 // src/fl_therm.f90:393-571.  The reference updates temp in place in the serial
 // order (i outer, j inner): node (j,i) reads the NEW values of (j-1,i), (j,i-1),
-// (j+1,i-1) and the old values of itself and (j+1,i), (j,i+1), (j-1,i+1).  Sweep s
-// evaluates that same formula with the new values taken from iterate X_{s-1}
-// (X_0 = old temp).  The map is lower-triangular in the serial order, so its only
-// fixed point is the serial result; a sweep that changes no bit proves it has
-// been reached, after which the remaining sweeps return at once.
-GF_DEV void therm_elem_coef(const GfDevParams *P, const GfDevArrays &A, int je, int ie,
-                            double &diff, double &dummye)
+// (j+1,i-1) and the OLD values of itself and of (j+1,i), (j,i+1), (j-1,i+1).
+//
+// A sweep evaluates that same formula at every node, taking the new values from
+// the previous iterate X (X_0 = old temp).  The map is lower-triangular in the
+// serial order, so its only fixed point is the serial result, and a sweep that
+// changes no bit proves it has been reached.  Sweeps run inside a workgroup on a
+// tile of 64 x 16 nodes held in LDS, with halos of TH_HJ rows above and below and
+// TH_HI columns to the left (the coupling to the previous column is ~(dz/dx)^2
+// weaker); the per-node coefficients of the six triangles are formed once, in the
+// reference's order, and kept in registers.  A tile iterates until it is still.
+// Pass 0 starts from the old temperature; each verification pass starts from the
+// previous pass's global result and records whether its first sweep changed any
+// bit anywhere; the first still pass proves the result is the serial one.
+#define TH_J 64
+#define TH_I 16
+#define TH_HJ 8
+#define TH_HI 4
+#define TH_OJ (TH_J - 2 * TH_HJ)  // output rows per tile
+#define TH_OI (TH_I - TH_HI)      // output columns per tile
+#define TH_LJ (TH_J + 2)          // LDS rows incl. a ring above and below
+#define TH_LI (TH_I + 2)          // LDS columns incl. a ring left and right
+#define TH_MAXPASS 4
+#define TH_MAXSWEEP 48
+
+struct ThermCoef {
+    // rhs = rhs + P_t - Q_t * S_t for the triangles t present, in the reference order:
+    // t1 B of (j-1,i-1), t2 A of (j-1,i), t3 B of (j-1,i), t4 A of (j,i-1), t5 B of (j,i-1), t6 A of (j,i)
+    double P[6], Q[6];
+    double k11, k12, c13;        // S1 = (X(j-1,i) k11 + X(j,i-1) k12) + T(j,i) k13
+    double k21, c22, c23;        // S2 = (X(j-1,i) k21 + T(j,i) k22) + T(j-1,i+1) k23
+    double s3;                   // S3 = only old values
+    double k41, k42, c43;        // S4 = (X(j,i-1) k41 + X(j+1,i-1) k42) + T(j,i) k43
+    double c51, k52, c53;        // S5 = (T(j,i) k51 + X(j+1,i-1) k52) + T(j+1,i) k53
+    double s6;                   // S6 = only old values
+    double area_n, told;
+    int mask;                    // bit t-1 set when triangle t exists
+};
+
+GF_DEV double therm_dummye(const GfDevParams *P, const GfDevArrays &A, int je, int ie)
 {
-    int iph = A.iphase[IE(je, ie)];
-    const GfPhase &ph = PHASE(iph);
+    const GfPhase &ph = PHASE(A.iphase[IE(je, ie)]);
     double dissip = 0;
     if (P->ishearh == 1 && P->itherm != 2) dissip = A.shrheat[IE(je, ie)];
-    diff = ph.conduct / ph.den / ph.cp;
-    dummye = (A.source[IE(je, ie)] + dissip / ph.den) / ph.cp;
+    return (A.source[IE(je, ie)] + dissip / ph.den) / ph.cp;
 }
 
-__global__ void __launch_bounds__(256) k_therm_sweep(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
-                                                     const double *__restrict__ told, const double *__restrict__ xin,
-                                                     double *__restrict__ xout, int sweep)
+// shape-function derivatives of thermal triangle A (vertices (je,ie),(je+1,ie),(je,ie+1))
+// or B (vertices (je+1,ie),(je,ie+1),(je+1,ie+1)) of element (je,ie), as in :440-560
+GF_DEV void therm_shape(const GfDevParams *P, const GfDevArrays &A, int je, int ie, int tri, double ar,
+                        double d[6])
 {
-    if (sweep > 0 && ck->sweep_changed[sweep - 1] == 0u) return;  // converged already
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    bool changed = false;
-    if (t < NN) {
-        const int j = (int)(t % NZ) + 1, i = (int)(t / NZ) + 1;
-        const double dt = ck->dt;
-#define TOLD(jj, ii) told[IN(jj, ii)]
-#define XNEW(jj, ii) xin[IN(jj, ii)]
-        double rhs = 0.0, area_n = 0.0;
-        double inv_area, ra13, x1, y1, x2, y2, x3, y3, x4, y4, d1x, d2x, d3x, d1z, d2z, d3z, ar, df, dm;
-        if (j != 1 && i != 1) {  // element (j-1,i-1), triangle B
-            therm_elem_coef(P, A, j - 1, i - 1, df, dm);
-            ar = AREA(j - 1, i - 1, 2);
-            inv_area = 1.0 / ar;
-            ra13 = inv_area / 6.0;
-            area_n = area_n + ra13;
-            y2 = CORD(j, i - 1, 2); y3 = CORD(j - 1, i, 2); y4 = CORD(j, i, 2);
-            x2 = CORD(j, i - 1, 1); x3 = CORD(j - 1, i, 1); x4 = CORD(j, i, 1);
-            d1x = (y2 - y4) * ar; d2x = (y4 - y3) * ar; d3x = (y3 - y2) * ar;
-            d1z = (x4 - x2) * ar; d2z = (x3 - x4) * ar; d3z = (x2 - x3) * ar;
-            rhs = rhs + dm * ra13 - 0.5 * df * inv_area *
-                  (XNEW(j - 1, i) * (d1x * d3x + d1z * d3z) + XNEW(j, i - 1) * (d2x * d3x + d2z * d3z) +
-                   TOLD(j, i) * (d3x * d3x + d3z * d3z));
-        }
-        if (j != 1 && i != NX) {  // element (j-1,i), triangles A and B
-            therm_elem_coef(P, A, j - 1, i, df, dm);
-            ar = AREA(j - 1, i, 1);
-            inv_area = 1.0 / ar;
-            ra13 = inv_area / 6.0;
-            area_n = area_n + ra13;
-            y1 = CORD(j - 1, i, 2); y2 = CORD(j, i, 2); y3 = CORD(j - 1, i + 1, 2);
-            x1 = CORD(j - 1, i, 1); x2 = CORD(j, i, 1); x3 = CORD(j - 1, i + 1, 1);
-            d1x = (y2 - y3) * ar; d2x = (y3 - y1) * ar; d3x = (y1 - y2) * ar;
-            d1z = (x3 - x2) * ar; d2z = (x1 - x3) * ar; d3z = (x2 - x1) * ar;
-            rhs = rhs + dm * ra13 - 0.5 * df * inv_area *
-                  (XNEW(j - 1, i) * (d1x * d2x + d1z * d2z) + TOLD(j, i) * (d2x * d2x + d2z * d2z) +
-                   TOLD(j - 1, i + 1) * (d3x * d2x + d3z * d2z));
-            ar = AREA(j - 1, i, 2);
-            inv_area = 1.0 / ar;
-            ra13 = inv_area / 6.0;
-            area_n = area_n + ra13;
-            y2 = CORD(j, i, 2); y3 = CORD(j - 1, i + 1, 2); y4 = CORD(j, i + 1, 2);
-            x2 = CORD(j, i, 1); x3 = CORD(j - 1, i + 1, 1); x4 = CORD(j, i + 1, 1);
-            d1x = (y2 - y4) * ar; d2x = (y4 - y3) * ar; d3x = (y3 - y2) * ar;
-            d1z = (x4 - x2) * ar; d2z = (x3 - x4) * ar; d3z = (x2 - x3) * ar;
-            rhs = rhs + dm * ra13 - 0.5 * df * inv_area *
-                  (TOLD(j - 1, i + 1) * (d1x * d2x + d1z * d2z) + TOLD(j, i) * (d2x * d2x + d2z * d2z) +
-                   TOLD(j, i + 1) * (d3x * d2x + d3z * d2z));
-        }
-        if (j != NZ && i != 1) {  // element (j,i-1), triangles A and B
-            therm_elem_coef(P, A, j, i - 1, df, dm);
-            ar = AREA(j, i - 1, 1);
-            inv_area = 1.0 / ar;
-            ra13 = inv_area / 6.0;
-            area_n = area_n + ra13;
-            y1 = CORD(j, i - 1, 2); y2 = CORD(j + 1, i - 1, 2); y3 = CORD(j, i, 2);
-            x1 = CORD(j, i - 1, 1); x2 = CORD(j + 1, i - 1, 1); x3 = CORD(j, i, 1);
-            d1x = (y2 - y3) * ar; d2x = (y3 - y1) * ar; d3x = (y1 - y2) * ar;
-            d1z = (x3 - x2) * ar; d2z = (x1 - x3) * ar; d3z = (x2 - x1) * ar;
-            rhs = rhs + dm * ra13 - 0.5 * df * inv_area *
-                  (XNEW(j, i - 1) * (d1x * d3x + d1z * d3z) + XNEW(j + 1, i - 1) * (d2x * d3x + d2z * d3z) +
-                   TOLD(j, i) * (d3x * d3x + d3z * d3z));
-            ar = AREA(j, i - 1, 2);
-            inv_area = 1.0 / ar;
-            ra13 = inv_area / 6.0;
-            area_n = area_n + ra13;
-            y2 = CORD(j + 1, i - 1, 2); y3 = CORD(j, i, 2); y4 = CORD(j + 1, i, 2);
-            x2 = CORD(j + 1, i - 1, 1); x3 = CORD(j, i, 1); x4 = CORD(j + 1, i, 1);
-            d1x = (y2 - y4) * ar; d2x = (y4 - y3) * ar; d3x = (y3 - y2) * ar;
-            d1z = (x4 - x2) * ar; d2z = (x3 - x4) * ar; d3z = (x2 - x3) * ar;
-            rhs = rhs + dm * ra13 - 0.5 * df * inv_area *
-                  (TOLD(j, i) * (d1x * d1x + d1z * d1z) + XNEW(j + 1, i - 1) * (d2x * d1x + d2z * d1z) +
-                   TOLD(j + 1, i) * (d3x * d1x + d3z * d1z));
-        }
-        if (j != NZ && i != NX) {  // element (j,i), triangle A
-            therm_elem_coef(P, A, j, i, df, dm);
-            ar = AREA(j, i, 1);
-            inv_area = 1.0 / ar;
-            ra13 = inv_area / 6.0;
-            area_n = area_n + ra13;
-            y1 = CORD(j, i, 2); y2 = CORD(j + 1, i, 2); y3 = CORD(j, i + 1, 2);
-            x1 = CORD(j, i, 1); x2 = CORD(j + 1, i, 1); x3 = CORD(j, i + 1, 1);
-            d1x = (y2 - y3) * ar; d2x = (y3 - y1) * ar; d3x = (y1 - y2) * ar;
-            d1z = (x3 - x2) * ar; d2z = (x1 - x3) * ar; d3z = (x2 - x1) * ar;
-            rhs = rhs + dm * ra13 - 0.5 * df * inv_area *
-                  (TOLD(j, i) * (d1x * d1x + d1z * d1z) + TOLD(j + 1, i) * (d2x * d1x + d2z * d1z) +
-                   TOLD(j, i + 1) * (d3x * d1x + d3z * d1z));
-        }
-        double tn = TOLD(j, i) + rhs * dt / area_n;
-        changed = __double_as_longlong(tn) != __double_as_longlong(xin[t]);
-        xout[t] = tn;
-#undef TOLD
-#undef XNEW
+    if (tri == 0) {
+        double y1 = CORD(je, ie, 2), y2 = CORD(je + 1, ie, 2), y3 = CORD(je, ie + 1, 2);
+        double x1 = CORD(je, ie, 1), x2 = CORD(je + 1, ie, 1), x3 = CORD(je, ie + 1, 1);
+        d[0] = (y2 - y3) * ar; d[1] = (y3 - y1) * ar; d[2] = (y1 - y2) * ar;
+        d[3] = (x3 - x2) * ar; d[4] = (x1 - x3) * ar; d[5] = (x2 - x1) * ar;
+    } else {
+        double y2 = CORD(je + 1, ie, 2), y3 = CORD(je, ie + 1, 2), y4 = CORD(je + 1, ie + 1, 2);
+        double x2 = CORD(je + 1, ie, 1), x3 = CORD(je, ie + 1, 1), x4 = CORD(je + 1, ie + 1, 1);
+        d[0] = (y2 - y4) * ar; d[1] = (y4 - y3) * ar; d[2] = (y3 - y2) * ar;
+        d[3] = (x4 - x2) * ar; d[4] = (x3 - x4) * ar; d[5] = (x2 - x3) * ar;
     }
-    unsigned long long any = __ballot(changed);
-    if ((threadIdx.x & 63) == 0 && any) atomicOr(&ck->sweep_changed[sweep], 1u);
+}
+// K(a,b) as the reference writes it: (dax*dbx + daz*dbz), a,b in 1..3
+#define KAB(d, a, b) ((d)[(a) - 1] * (d)[(b) - 1] + (d)[(a) + 2] * (d)[(b) + 2])
+
+// Coefficients of node (j,i); T(...) reads the old temperature.
+template <typename TOLDF>
+GF_DEV void therm_coef(const GfDevParams *P, const GfDevArrays &A, int j, int i, TOLDF T, ThermCoef &c)
+{
+    double d[6], ar, inv_area, ra13, df, dm;
+    c.mask = 0;
+    c.area_n = 0.0;
+    c.told = T(j, i);
+    auto tri_common = [&](int je, int ie, int tri, int t) {
+        ar = AREA(je, ie, tri + 1);
+        inv_area = 1.0 / ar;
+        ra13 = inv_area / 6.0;
+        c.area_n = c.area_n + ra13;
+        df = PHASE(A.iphase[IE(je, ie)]).diff;
+        dm = therm_dummye(P, A, je, ie);
+        c.P[t] = dm * ra13;
+        c.Q[t] = 0.5 * df * inv_area;
+        therm_shape(P, A, je, ie, tri, ar, d);
+        c.mask |= 1 << t;
+    };
+    if (j != 1 && i != 1) {  // element (j-1,i-1), triangle B: temps (j-1,i), (j,i-1), (j,i)
+        tri_common(j - 1, i - 1, 1, 0);
+        c.k11 = KAB(d, 1, 3); c.k12 = KAB(d, 2, 3); c.c13 = T(j, i) * KAB(d, 3, 3);
+    }
+    if (j != 1 && i != NX) {  // element (j-1,i), triangle A: (j-1,i), (j,i), (j-1,i+1)
+        tri_common(j - 1, i, 0, 1);
+        c.k21 = KAB(d, 1, 2); c.c22 = T(j, i) * KAB(d, 2, 2); c.c23 = T(j - 1, i + 1) * KAB(d, 3, 2);
+        // triangle B: (j-1,i+1), (j,i), (j,i+1) -- old values only
+        tri_common(j - 1, i, 1, 2);
+        c.s3 = T(j - 1, i + 1) * KAB(d, 1, 2) + T(j, i) * KAB(d, 2, 2) + T(j, i + 1) * KAB(d, 3, 2);
+    }
+    if (j != NZ && i != 1) {  // element (j,i-1), triangle A: (j,i-1), (j+1,i-1), (j,i)
+        tri_common(j, i - 1, 0, 3);
+        c.k41 = KAB(d, 1, 3); c.k42 = KAB(d, 2, 3); c.c43 = T(j, i) * KAB(d, 3, 3);
+        // triangle B: (j,i), (j+1,i-1), (j+1,i)
+        tri_common(j, i - 1, 1, 4);
+        c.c51 = T(j, i) * KAB(d, 1, 1); c.k52 = KAB(d, 2, 1); c.c53 = T(j + 1, i) * KAB(d, 3, 1);
+    }
+    if (j != NZ && i != NX) {  // element (j,i), triangle A: (j,i), (j+1,i), (j,i+1) -- old values only
+        tri_common(j, i, 0, 5);
+        c.s6 = T(j, i) * KAB(d, 1, 1) + T(j + 1, i) * KAB(d, 2, 1) + T(j, i + 1) * KAB(d, 3, 1);
+    }
 }
 
-// Boundary conditions src/fl_therm.f90:575-598 applied to the converged iterate.
-__global__ void __launch_bounds__(256) k_therm_final(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
-                                                     const double *__restrict__ x_last, const double *__restrict__ x_prev,
-                                                     int nsweeps)
+// one evaluation of the serial update at a node; xu = X(j-1,i), xl = X(j,i-1), xd = X(j+1,i-1)
+GF_DEV double therm_eval(const ThermCoef &c, double xu, double xl, double xd, double dt)
+{
+    double rhs = 0.0;
+    if (c.mask & 1) rhs = rhs + c.P[0] - c.Q[0] * ((xu * c.k11 + xl * c.k12) + c.c13);
+    if (c.mask & 2) rhs = rhs + c.P[1] - c.Q[1] * ((xu * c.k21 + c.c22) + c.c23);
+    if (c.mask & 4) rhs = rhs + c.P[2] - c.Q[2] * c.s3;
+    if (c.mask & 8) rhs = rhs + c.P[3] - c.Q[3] * ((xl * c.k41 + xd * c.k42) + c.c43);
+    if (c.mask & 16) rhs = rhs + c.P[4] - c.Q[4] * ((c.c51 + xd * c.k52) + c.c53);
+    if (c.mask & 32) rhs = rhs + c.P[5] - c.Q[5] * c.s6;
+    return c.told + rhs * dt / c.area_n;
+}
+
+// pass == 0: old temperature from A.temp (also copied to temp0 = src/fl_therm.f90:52);
+// pass > 0: old temperature from A.temp0 and the starting iterate from xin.
+__global__ void __launch_bounds__(1024) k_therm_tile(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
+                                                     const double *__restrict__ xin, double *__restrict__ xout, int pass)
+{
+    if (pass > 0 && ck->therm_pass_changed[pass - 1] == 0u) return;  // already proven
+    __shared__ double sT[TH_LI * TH_LJ];
+    __shared__ double sX[2][TH_LI * TH_LJ];
+    const int tj = threadIdx.x % TH_J, ti = threadIdx.x / TH_J;
+    const int j0 = blockIdx.x * TH_OJ + 1 - TH_HJ;  // 1-based row of tile row 0
+    const int i0 = blockIdx.y * TH_OI + 1 - TH_HI;  // 1-based column of tile column 0
+    const double *told_g = pass == 0 ? A.temp : A.temp0;
+    const double *x0_g = pass == 0 ? A.temp : xin;
+#define LIDX(lj, li) (((li) + 1) * TH_LJ + ((lj) + 1))
+    // load old temperature and starting iterate for the tile plus its ring
+    for (int q = threadIdx.x; q < TH_LI * TH_LJ; q += blockDim.x) {
+        int lj = q % TH_LJ - 1, li = q / TH_LJ - 1;
+        int j = j0 + lj, i = i0 + li;
+        double tv = 0.0, xv = 0.0;
+        if (j >= 1 && j <= NZ && i >= 1 && i <= NX) {
+            tv = told_g[IN(j, i)];
+            xv = x0_g[IN(j, i)];
+        }
+        sT[q] = tv;
+        sX[0][q] = xv;
+        sX[1][q] = xv;
+    }
+    __syncthreads();
+    const int j = j0 + tj, i = i0 + ti;
+    const bool valid = j >= 1 && j <= NZ && i >= 1 && i <= NX;
+    const bool is_out = valid && tj >= TH_HJ && tj < TH_J - TH_HJ && ti >= TH_HI;
+    ThermCoef c;
+    if (valid) {
+        auto T = [&](int jj, int ii) { return sT[LIDX(jj - j0, ii - i0)]; };
+        therm_coef(P, A, j, i, T, c);
+    }
+    const double dt = ck->dt;
+    const int me = LIDX(tj, ti), up = LIDX(tj - 1, ti), lf = LIDX(tj, ti - 1), dl = LIDX(tj + 1, ti - 1);
+    int cur = 0, first_changed = 0, nsweep = 0;
+    bool still = false;
+    while (nsweep < TH_MAXSWEEP) {
+        bool changed = false;
+        double tn = 0.0;
+        if (valid) {
+            tn = therm_eval(c, sX[cur][up], sX[cur][lf], sX[cur][dl], dt);
+            changed = __double_as_longlong(tn) != __double_as_longlong(sX[cur][me]);
+            sX[cur ^ 1][me] = tn;
+        }
+        int any = __syncthreads_or(changed);
+        if (nsweep == 0) first_changed = any;
+        cur ^= 1;
+        nsweep++;
+        if (!any) {
+            still = true;
+            break;
+        }
+    }
+    if (is_out) {
+        xout[IN(j, i)] = sX[cur][me];
+        if (pass == 0) A.temp0[IN(j, i)] = c.told;
+    }
+    if (threadIdx.x == 0) {
+        if (first_changed) atomicOr(&ck->therm_pass_changed[pass], 1u);
+        if (!still) atomicOr(&ck->therm_pass_changed[pass], 2u);
+    }
+#undef LIDX
+}
+
+// Boundary conditions src/fl_therm.f90:575-598 applied to the proven iterate.
+__global__ void __launch_bounds__(256) k_therm_bc(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
+                                                  const double *__restrict__ xa, const double *__restrict__ xb)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    // the first sweep that changed nothing holds the fixed point (so does its input)
-    int first_still = -1;
-    for (int k = 0; k < nsweeps; k++)
-        if (ck->sweep_changed[k] == 0u) { first_still = k; break; }
-    const double *x = x_last;
-    if (first_still >= 0) x = ((nsweeps - 1 - first_still) % 2 == 0) ? x_last : x_prev;
-    if (t == 0 && first_still < 0) atomicAdd(&ck->therm_unconverged, 1ull);
+    // pass p writes xa when p is even, xb when odd; the first pass whose first sweep
+    // changed nothing holds the fixed point (as does its input)
+    int proven = -1;
+    for (int k = 0; k < TH_MAXPASS; k++)
+        if (ck->therm_pass_changed[k] == 0u) { proven = k; break; }
+    const double *x = (proven < 0 ? (TH_MAXPASS - 1) : proven) % 2 == 0 ? xa : xb;
+    if (t == 0 && proven < 0) atomicAdd(&ck->therm_unconverged, 1ull);
     if (t >= NN) return;
     const int j = (int)(t % NZ) + 1, i = (int)(t / NZ) + 1;
     // value of node (jj, ii) after the top/bottom conditions (:575-588)
@@ -300,213 +372,214 @@ GF_DEV void dev_pre_plast(const GfDevParams *P, double aps, const double *prv, d
 }
 
 // ------------------------------------------------------------------ fl_rheol
-// src/fl_rheol.f90:69-421, one thread per element.  avgsr: 0 none, 1 refresh and
-// reset the averages (nsrate == ifreq_avgsr), 2 refresh only (nsrate == -1).
+// src/fl_rheol.f90:69-421.  Four lanes (a quad) per element, lane k owning triangle k
+// (A,B,C,D): each lane forms its triangle's strain rate and runs its constitutive
+// laws; the element-level sums the reference forms over the four triangles
+// (averages, the plastic/viscous invariants, the stress mixing of triangle pairs,
+// the plastic strain) gather the four lane values with quad shuffles and add them
+// in the reference's order, so every lane obtains the same bits.  Element-level
+// results are stored by lane 0.  avgsr: 0 none, 1 refresh and reset the averages
+// (nsrate == ifreq_avgsr), 2 refresh only (nsrate == -1).
 #define GF_MAXACT 8
-__global__ void __launch_bounds__(256) k_rheol(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
-                                               int do_visc, int avgsr)
+
+GF_DEV double qget(double v, int lane) { return __shfl(v, lane, 4); }
+
+template <bool SHEARH>
+GF_DEV void rheol_quad(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e, int k,
+                       int avgsr)
 {
-    long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= NE) return;
     const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
     const double dt = ck->dt;
-    double prv[GF_MAXACT];
-    for (int a = 0; a < P->nactive; a++) prv[a] = A.pr[a * NE + e];
-
-    const double vx1 = VEL(j, i, 1), vy1 = VEL(j, i, 2), vx2 = VEL(j + 1, i, 1), vy2 = VEL(j + 1, i, 2);
-    const double vx3 = VEL(j, i + 1, 1), vy3 = VEL(j, i + 1, 2), vx4 = VEL(j + 1, i + 1, 1), vy4 = VEL(j + 1, i + 1, 2);
-    const double x1 = CORD(j, i, 1), y1 = CORD(j, i, 2), x2 = CORD(j + 1, i, 1), y2 = CORD(j + 1, i, 2);
-    const double x3 = CORD(j, i + 1, 1), y3 = CORD(j, i + 1, 2), x4 = CORD(j + 1, i + 1, 1), y4 = CORD(j + 1, i + 1, 2);
-    const double ar1 = A.area[e], ar2 = A.area[e + NE], ar3 = A.area[e + 2 * NE], ar4 = A.area[e + 3 * NE];
-
-    double sr[4][3];
+    // corner k of the element: 1 (j,i), 2 (j+1,i), 3 (j,i+1), 4 (j+1,i+1)
+    const int cj = j + (k & 1), ci = i + (k >> 1);
+    const double mvx = VEL(cj, ci, 1), mvy = VEL(cj, ci, 2), mx = CORD(cj, ci, 1), my = CORD(cj, ci, 2);
+    const double ar = A.area[e + NE * k];
+    // vertices of triangle k: A (1,2,3), B (3,2,4), C (1,2,4), D (1,4,3)
+    const int va = k == 1 ? 2 : 0;
+    const int vb = k == 3 ? 3 : 1;
+    const int vc = (k == 0 || k == 3) ? 2 : 3;
+    const double vx1 = qget(mvx, va), vy1 = qget(mvy, va), x1 = qget(mx, va), y1 = qget(my, va);
+    const double vx2 = qget(mvx, vb), vy2 = qget(mvy, vb), x2 = qget(mx, vb), y2 = qget(my, vb);
+    const double vx3 = qget(mvx, vc), vy3 = qget(mvy, vc), x3 = qget(mx, vc), y3 = qget(my, vc);
+    double sr[3];
     {
-        double dx[3], dz[3];
-        // triangle A: vertices (1,2,3)
-        dx[0] = (y2 - y3) * ar1; dx[1] = (y3 - y1) * ar1; dx[2] = (y1 - y2) * ar1;
-        dz[0] = (x3 - x2) * ar1; dz[1] = (x1 - x3) * ar1; dz[2] = (x2 - x1) * ar1;
-        sr[0][0] = vx1 * dx[0] + vx2 * dx[1] + vx3 * dx[2];
-        sr[0][1] = vy1 * dz[0] + vy2 * dz[1] + vy3 * dz[2];
-        sr[0][2] = 0.5 * (vx1 * dz[0] + vx2 * dz[1] + vx3 * dz[2] + vy1 * dx[0] + vy2 * dx[1] + vy3 * dx[2]);
-        // triangle B: vertices (3,2,4)
-        dx[0] = (y2 - y4) * ar2; dx[1] = (y4 - y3) * ar2; dx[2] = (y3 - y2) * ar2;
-        dz[0] = (x4 - x2) * ar2; dz[1] = (x3 - x4) * ar2; dz[2] = (x2 - x3) * ar2;
-        sr[1][0] = vx3 * dx[0] + vx2 * dx[1] + vx4 * dx[2];
-        sr[1][1] = vy3 * dz[0] + vy2 * dz[1] + vy4 * dz[2];
-        sr[1][2] = 0.5 * (vx3 * dz[0] + vx2 * dz[1] + vx4 * dz[2] + vy3 * dx[0] + vy2 * dx[1] + vy4 * dx[2]);
-        // triangle C: vertices (1,2,4)
-        dx[0] = (y2 - y4) * ar3; dx[1] = (y4 - y1) * ar3; dx[2] = (y1 - y2) * ar3;
-        dz[0] = (x4 - x2) * ar3; dz[1] = (x1 - x4) * ar3; dz[2] = (x2 - x1) * ar3;
-        sr[2][0] = vx1 * dx[0] + vx2 * dx[1] + vx4 * dx[2];
-        sr[2][1] = vy1 * dz[0] + vy2 * dz[1] + vy4 * dz[2];
-        sr[2][2] = 0.5 * (vx1 * dz[0] + vx2 * dz[1] + vx4 * dz[2] + vy1 * dx[0] + vy2 * dx[1] + vy4 * dx[2]);
-        // triangle D: vertices (1,4,3)
-        dx[0] = (y4 - y3) * ar4; dx[1] = (y3 - y1) * ar4; dx[2] = (y1 - y4) * ar4;
-        dz[0] = (x3 - x4) * ar4; dz[1] = (x1 - x3) * ar4; dz[2] = (x4 - x1) * ar4;
-        sr[3][0] = vx1 * dx[0] + vx4 * dx[1] + vx3 * dx[2];
-        sr[3][1] = vy1 * dz[0] + vy4 * dz[1] + vy3 * dz[2];
-        sr[3][2] = 0.5 * (vx1 * dz[0] + vx4 * dz[1] + vx3 * dz[2] + vy1 * dx[0] + vy4 * dx[1] + vy3 * dx[2]);
+        // :98-132: for every triangle, with its vertices (v1,v2,v3) in the order the
+        // reference lists them, the derivatives are (y2-y3, y3-y1, y1-y2)*area and
+        // (x3-x2, x1-x3, x2-x1)*area
+        const double dx0 = (y2 - y3) * ar, dx1 = (y3 - y1) * ar, dx2 = (y1 - y2) * ar;
+        const double dz0 = (x3 - x2) * ar, dz1 = (x1 - x3) * ar, dz2 = (x2 - x1) * ar;
+        sr[0] = vx1 * dx0 + vx2 * dx1 + vx3 * dx2;
+        sr[1] = vy1 * dz0 + vy2 * dz1 + vy3 * dz2;
+        sr[2] = 0.5 * (vx1 * dz0 + vx2 * dz1 + vx3 * dz2 + vy1 * dx0 + vy2 * dx1 + vy3 * dx2);
     }
-    if (P->mix_strain == 1) {
-        for (int k = 0; k < 4; k += 2) {
-            double em = 0.5 * (sr[k][0] + sr[k][1] + sr[k + 1][0] + sr[k + 1][1]);
-            double eda = sr[k][0] - sr[k][1];
-            double edb = sr[k + 1][0] - sr[k + 1][1];
-            sr[k][0] = 0.5 * (em + eda);
-            sr[k][1] = 0.5 * (em - eda);
-            sr[k + 1][0] = 0.5 * (em + edb);
-            sr[k + 1][1] = 0.5 * (em - edb);
-        }
+    if (P->mix_strain == 1) {  // pairs (A,B) and (C,D), :158-168
+        const int ev = k & 2, od = ev + 1;
+        double e0 = qget(sr[0], ev), e1 = qget(sr[1], ev), o0 = qget(sr[0], od), o1 = qget(sr[1], od);
+        double em = 0.5 * (e0 + e1 + o0 + o1);
+        double ed = (k & 1) ? o0 - o1 : e0 - e1;
+        sr[0] = 0.5 * (em + ed);
+        sr[1] = 0.5 * (em - ed);
     }
-    double st[4][4];  // st[c][k]
-    for (int c = 0; c < 4; c++)
-        for (int k = 0; k < 4; k++) st[c][k] = A.stress0[e + NE * (c + 4 * k)];
+    double st[4];
+    for (int c = 0; c < 4; c++) st[c] = A.stress0[e + NE * (c + 4 * k)];
 
-    double s11 = 0.25 * (0.0 + sr[0][0] + sr[1][0] + sr[2][0] + sr[3][0]);
-    double s22 = 0.25 * (0.0 + sr[0][1] + sr[1][1] + sr[2][1] + sr[3][1]);
-    double s12 = 0.25 * (0.0 + sr[0][2] + sr[1][2] + sr[2][2] + sr[3][2]);
-    double srII = 0.5 * sqrt((s11 - s22) * (s11 - s22) + 4.0 * s12 * s12);
-    double srI = 0.5 * (s11 + s22);
-    double srs2 = (s11 - srI) * (s11 - srI) + (s22 - srI) * (s22 - srI) + 2.0 * s12 * s12;
-    double se1 = A.se2sr[e] + s11 * dt;
-    double se2 = A.se2sr[e + NE] + s22 * dt;
-    double se3 = A.se2sr[e + 2 * NE] + s12 * dt;
-
-    s11 = 0.25 * (0.0 + st[0][0] + st[0][1] + st[0][2] + st[0][3]);
-    s22 = 0.25 * (0.0 + st[1][0] + st[1][1] + st[1][2] + st[1][3]);
-    s12 = 0.25 * (0.0 + st[2][0] + st[2][1] + st[2][2] + st[2][3]);
-    double stII = 0.5 * sqrt((s11 - s22) * (s11 - s22) + 4.0 * s12 * s12);
-    double sshr = A.sshrheat[e];
-    if (srII != 0.0) sshr = sshr + stII / srII * srs2 * dt;
-
+    auto sum4 = [&](double v) {  // ((((0 + v_A) + v_B) + v_C) + v_D), Fortran SUM order
+        return 0.0 + qget(v, 0) + qget(v, 1) + qget(v, 2) + qget(v, 3);
+    };
+    double se1, se2, se3, sshr, s11, s22, s12;
+    {
+        double a11 = 0.25 * sum4(sr[0]);
+        double a22 = 0.25 * sum4(sr[1]);
+        double a12 = 0.25 * sum4(sr[2]);
+        double srII = 0.5 * sqrt((a11 - a22) * (a11 - a22) + 4.0 * a12 * a12);
+        double srI = 0.5 * (a11 + a22);
+        double srs2 = (a11 - srI) * (a11 - srI) + (a22 - srI) * (a22 - srI) + 2.0 * a12 * a12;
+        se1 = A.se2sr[e] + a11 * dt;
+        se2 = A.se2sr[e + NE] + a22 * dt;
+        se3 = A.se2sr[e + 2 * NE] + a12 * dt;
+        s11 = 0.25 * sum4(st[0]);
+        s22 = 0.25 * sum4(st[1]);
+        s12 = 0.25 * sum4(st[2]);
+        double stII = 0.5 * sqrt((s11 - s22) * (s11 - s22) + 4.0 * s12 * s12);
+        sshr = A.sshrheat[e];
+        if (srII != 0.0) sshr = sshr + stII / srII * srs2 * dt;
+    }
     const int iph = A.iphase[e];
     const GfPhase &ph = PHASE(iph);
     const int irh = ph.irheol;
     const double bulkm = ph.bulkm, rmu = ph.rm;
     double stherm = 0.0;
     if (P->istress_therm > 0) stherm = -ph.alfa * bulkm * (A.temp[IN(j, i)] - A.temp0[IN(j, i)]);
-    double coh = 0, phi = 0, psi = 0, hardn = 0;
     double aps = A.aps[e];
-    if (irh == 6 || irh >= 11) dev_pre_plast(P, aps, prv, coh, phi, psi, hardn);
-    double vis = A.visn[e];
-    if ((irh == 3 || irh >= 11) && do_visc) {
-        double a11 = 0.25 * (st[0][0] + st[0][1] + st[0][2] + st[0][3]);
-        double a22 = 0.25 * (st[1][0] + st[1][1] + st[1][2] + st[1][3]);
-        double a33 = 0.25 * (st[3][0] + st[3][1] + st[3][2] + st[3][3]);
-        vis = dev_eff_visc(P, A, j, i, a11, a22, a33, prv);
-        A.visn[e] = vis;
-    }
-    double s_start[4][4];
-    if (P->ishearh == 1)
-        for (int c = 0; c < 4; c++)
-            for (int k = 0; k < 4; k++) s_start[c][k] = st[c][k];
-
-    double depl[4] = {0, 0, 0, 0};
-    double sp[4][4], sv[4][4];  // [c][k]
-    PlasticConst pc;
-    if (irh == 6 || irh >= 11) pc = plastic_const(bulkm, rmu, coh, phi, psi, P->ten_off);
-    bool ok = true;
-    for (int k = 0; k < 4; k++) {
-        double de11 = sr[k][0] * dt, de22 = sr[k][1] * dt, de12 = sr[k][2] * dt;
-        double dv = A.dvol[e + NE * k] - 0.0;
-        sp[0][k] = st[0][k] + stherm;
-        sp[1][k] = st[1][k] + stherm;
-        sp[2][k] = st[2][k];
-        sp[3][k] = st[3][k] + stherm;
-        for (int c = 0; c < 4; c++) sv[c][k] = sp[c][k];
-        if (irh == 1) {
-            dev_elastic(bulkm, rmu, sp[0][k], sp[1][k], sp[3][k], sp[2][k], de11, de22, de12);
-            for (int c = 0; c < 4; c++) st[c][k] = sp[c][k];
-        } else if (irh == 3) {
-            ok &= dev_maxwell(bulkm, rmu, vis, sv[0][k], sv[1][k], sv[3][k], sv[2][k], de11, de22, de12, dv, dt);
-            for (int c = 0; c < 4; c++) st[c][k] = sv[c][k];
-        } else if (irh == 6) {
-            dev_plastic(pc, rmu, hardn, depl[k], sp[0][k], sp[1][k], sp[3][k], sp[2][k], de11, de22, de12);
-            for (int c = 0; c < 4; c++) st[c][k] = sp[c][k];
-        } else if (irh >= 11) {
-            dev_plastic(pc, rmu, hardn, depl[k], sp[0][k], sp[1][k], sp[3][k], sp[2][k], de11, de22, de12);
-            ok &= dev_maxwell(bulkm, rmu, vis, sv[0][k], sv[1][k], sv[3][k], sv[2][k], de11, de22, de12, dv, dt);
+    const double vis = A.visn[e];  // refreshed by k_visc on viscosity-update steps
+    const bool plas_law = irh == 6 || irh >= 11;
+    PlasticConst pc{};
+    double hardn = 0;
+    if (plas_law) {
+        double prv[GF_MAXACT], coh, phi, psi;
+        for (int a = 0; a < P->nactive; a++) prv[a] = A.pr[a * NE + e];
+        dev_pre_plast(P, aps, prv, coh, phi, psi, hardn);
+        // src/rh_plastic.f90:33-46: sin(phi), sin(psi) and tan(phi), one double-double
+        // sine/cosine per lane (lane 1 takes psi)
+        const double degrad = 3.14159265358979323846 / 180.0;
+        const double xarg = (k == 1 ? psi : phi) * degrad;
+        double sv, tv = 0.0;
+        if (dd_trig_range(xarg)) {
+            ddv s_, c_;
+            dd_sincos(xarg, s_, c_);
+            sv = s_.hi + s_.lo;
+            if (k == 0 && phi != 0.0 && xarg != 1.5707963267948966) {
+                ddv t_ = dd_div(s_, c_);
+                tv = t_.hi + t_.lo;
+            } else if (k == 0 && phi != 0.0) {
+                tv = tan(xarg);
+            }
+        } else {
+            sv = sin(xarg);
+            if (k == 0 && phi != 0.0) tv = tan(xarg);
         }
+        const double sphi = qget(sv, 0), spsi = qget(sv, 1), tphi = qget(tv, 0);
+        const double c4d3 = 4.0 / 3.0, c2d3 = 2.0 / 3.0;
+        pc.anphi = (1.0 + sphi) / (1.0 - sphi);
+        pc.anpsi = (1.0 + spsi) / (1.0 - spsi);
+        pc.amc = 2.0 * coh * sqrt(pc.anphi);
+        pc.e1 = bulkm + c4d3 * rmu;
+        pc.e2 = bulkm - c2d3 * rmu;
+        pc.x1 = (pc.e1 - pc.e2 * pc.anpsi + pc.e1 * pc.anphi * pc.anpsi - pc.e2 * pc.anphi);
+        pc.ten_max = phi == 0.0 ? P->ten_off : fmin(P->ten_off, coh / tphi);
     }
-    if (!ok) raise_error(ck, GF_STOP_MAXWELL, e);
-    if (irh >= 11) {
-        double a = sp[0][0] + sp[0][1] + sp[0][2] + sp[0][3] - sp[1][0] - sp[1][1] - sp[1][2] - sp[1][3];
-        double b = sp[2][0] + sp[2][1] + sp[2][2] + sp[2][3];
+    double s_start[SHEARH ? 4 : 1];
+    if (SHEARH)
+        for (int c = 0; c < 4; c++) s_start[c] = st[c];
+
+    const double de11 = sr[0] * dt, de22 = sr[1] * dt, de12 = sr[2] * dt;
+    const double dv = A.dvol[e + NE * k] - 0.0;
+    double sp[4] = {st[0] + stherm, st[1] + stherm, st[2], st[3] + stherm};
+    double depl = 0.0;
+    bool ok = true;
+    if (irh == 1) {
+        dev_elastic(bulkm, rmu, sp[0], sp[1], sp[3], sp[2], de11, de22, de12);
+        for (int c = 0; c < 4; c++) st[c] = sp[c];
+    } else if (irh == 3) {
+        ok = dev_maxwell(bulkm, rmu, vis, sp[0], sp[1], sp[3], sp[2], de11, de22, de12, dv, dt);
+        for (int c = 0; c < 4; c++) st[c] = sp[c];
+    } else if (irh == 6) {
+        dev_plastic(pc, rmu, hardn, depl, sp[0], sp[1], sp[3], sp[2], de11, de22, de12);
+        for (int c = 0; c < 4; c++) st[c] = sp[c];
+    } else if (irh >= 11) {
+        double sv[4] = {sp[0], sp[1], sp[2], sp[3]};
+        dev_plastic(pc, rmu, hardn, depl, sp[0], sp[1], sp[3], sp[2], de11, de22, de12);
+        ok = dev_maxwell(bulkm, rmu, vis, sv[0], sv[1], sv[3], sv[2], de11, de22, de12, dv, dt);
+        // :266-288 the branch with the smaller summed invariant, same bits on every lane
+        double a = qget(sp[0], 0) + qget(sp[0], 1) + qget(sp[0], 2) + qget(sp[0], 3) - qget(sp[1], 0) -
+                   qget(sp[1], 1) - qget(sp[1], 2) - qget(sp[1], 3);
+        double b = qget(sp[2], 0) + qget(sp[2], 1) + qget(sp[2], 2) + qget(sp[2], 3);
         double sII_plas = a * a + 4.0 * (b * b);
-        double av = sv[0][0] + sv[0][1] + sv[0][2] + sv[0][3] - sv[1][0] - sv[1][1] - sv[1][2] - sv[1][3];
-        double bv = sv[2][0] + sv[2][1] + sv[2][2] + sv[2][3];
+        double av = qget(sv[0], 0) + qget(sv[0], 1) + qget(sv[0], 2) + qget(sv[0], 3) - qget(sv[1], 0) -
+                    qget(sv[1], 1) - qget(sv[1], 2) - qget(sv[1], 3);
+        double bv = qget(sv[2], 0) + qget(sv[2], 1) + qget(sv[2], 2) + qget(sv[2], 3);
         double sII_visc = av * av + 4.0 * (bv * bv);
         const bool plas = sII_plas < sII_visc;
-        for (int c = 0; c < 4; c++)
-            for (int k = 0; k < 4; k++) st[c][k] = plas ? sp[c][k] : sv[c][k];
+        for (int c = 0; c < 4; c++) st[c] = plas ? sp[c] : sv[c];
     }
-    if (P->mix_stress == 1) {
-        double quad_area = 1.0 / ar1 + 1.0 / ar2;
-        double s0a = 0.5 * (st[0][0] + st[1][0]);
-        double s0b = 0.5 * (st[0][1] + st[1][1]);
-        double s0 = (s0a / ar2 + s0b / ar1) / quad_area;
-        st[0][0] = st[0][0] - s0a + s0;
-        st[1][0] = st[1][0] - s0a + s0;
-        st[0][1] = st[0][1] - s0b + s0;
-        st[1][1] = st[1][1] - s0b + s0;
-        quad_area = 1.0 / ar3 + 1.0 / ar4;
-        s0a = 0.5 * (st[0][2] + st[1][2]);
-        s0b = 0.5 * (st[0][3] + st[1][3]);
-        s0 = (s0a / ar4 + s0b / ar3) / quad_area;
-        st[0][2] = st[0][2] - s0a + s0;
-        st[1][2] = st[1][2] - s0a + s0;
-        st[0][3] = st[0][3] - s0b + s0;
-        st[1][3] = st[1][3] - s0b + s0;
+    if (!ok) raise_error(ck, GF_STOP_MAXWELL, e);
+    if (P->mix_stress == 1) {  // pairs (A,B), (C,D), :291-309
+        const int ev = k & 2, od = ev + 1;
+        double ar_e = qget(ar, ev), ar_o = qget(ar, od);
+        double e0 = qget(st[0], ev), e1 = qget(st[1], ev), o0 = qget(st[0], od), o1 = qget(st[1], od);
+        double quad_area = 1.0 / ar_e + 1.0 / ar_o;
+        double s0a = 0.5 * (e0 + e1);
+        double s0b = 0.5 * (o0 + o1);
+        double s0 = (s0a / ar_o + s0b / ar_e) / quad_area;
+        double sme = (k & 1) ? s0b : s0a;
+        st[0] = st[0] - sme + s0;
+        st[1] = st[1] - sme + s0;
     }
-    for (int k = 0; k < 4; k++) {
-        st[0][k] = st[0][k] - stherm;
-        st[1][k] = st[1][k] - stherm;
-        st[3][k] = st[3][k] - stherm;
-    }
-    if (irh == 6 || irh >= 11) {
-        double quad_area = ar1 + ar2 + ar3 + ar4;
-        aps = aps + (depl[0] * ar1 + depl[1] * ar2 + depl[2] * ar3 + depl[3] * ar4) / quad_area;
+    st[0] = st[0] - stherm;
+    st[1] = st[1] - stherm;
+    st[3] = st[3] - stherm;
+    for (int c = 0; c < 4; c++) A.stress0[e + NE * (c + 4 * k)] = st[c];
+
+    if (plas_law) {  // :319-328
+        double a0 = qget(ar, 0), a1 = qget(ar, 1), a2 = qget(ar, 2), a3 = qget(ar, 3);
+        double quad_area = a0 + a1 + a2 + a3;
+        aps = aps + (qget(depl, 0) * a0 + qget(depl, 1) * a1 + qget(depl, 2) * a2 + qget(depl, 3) * a3) / quad_area;
         if (aps < 0.0) aps = 0.0;
         if (P->tau_heal != 0.0) aps = aps / (1.0 + dt / P->tau_heal);
-        A.aps[e] = aps;
     }
-    // the reference adds the averaged (pre-update) STRESS times dt: s11..s12 were reused
+    if (SHEARH) {  // :336-386
+        double lambda = bulkm - 2.0 / 3.0 * rmu, mu = rmu;
+        double denom = 3.0 * lambda + 2.0 * mu;
+        if (fabs(denom) < 1.0e-30) denom = fsign(1.0e-30, denom);
+        const double *s0 = s_start;
+        double trs = s0[0] + s0[1] + s0[3];
+        double e11s = (s0[0] - (lambda / denom) * trs) / (2.0 * mu);
+        double e22s = (s0[1] - (lambda / denom) * trs) / (2.0 * mu);
+        double e33s = (s0[3] - (lambda / denom) * trs) / (2.0 * mu);
+        double e12s = s0[2] / (2.0 * mu);
+        double trf = st[0] + st[1] + st[3];
+        double e11f = (st[0] - (lambda / denom) * trf) / (2.0 * mu);
+        double e22f = (st[1] - (lambda / denom) * trf) / (2.0 * mu);
+        double e33f = (st[3] - (lambda / denom) * trf) / (2.0 * mu);
+        double e12f = st[2] / (2.0 * mu);
+        double d11n = sr[0] * dt - (e11f - e11s);
+        double d22n = sr[1] * dt - (e22f - e22s);
+        double d12n = sr[2] * dt - (e12f - e12s);
+        double d33n = 0.0 - (e33f - e33s);
+        double a11 = 0.5 * (s0[0] + st[0]);
+        double a22 = 0.5 * (s0[1] + st[1]);
+        double a12 = 0.5 * (s0[2] + st[2]);
+        double a33 = 0.5 * (s0[3] + st[3]);
+        double q = 0.25 * (a11 * d11n + a22 * d22n + 2.0 * a12 * d12n + a33 * d33n);
+        double q_shear_elem = 0.0 + qget(q, 0) + qget(q, 1) + qget(q, 2) + qget(q, 3);
+        sshr = sshr + q_shear_elem;
+    }
+    if (k != 0) return;
+    if (plas_law) A.aps[e] = aps;
+    // the reference adds the averaged (step-start) STRESS times dt: s11..s12 were reused
     A.strain[e] = A.strain[e] + s11 * dt;
     A.strain[e + NE] = A.strain[e + NE] + s22 * dt;
     A.strain[e + 2 * NE] = A.strain[e + 2 * NE] + s12 * dt;
-
-    if (P->ishearh == 1) {
-        double q_shear_elem = 0.0;
-        for (int k = 0; k < 4; k++) {
-            double lambda = bulkm - 2.0 / 3.0 * rmu, mu = rmu;
-            double denom = 3.0 * lambda + 2.0 * mu;
-            if (fabs(denom) < 1.0e-30) denom = fsign(1.0e-30, denom);
-            double trs = s_start[0][k] + s_start[1][k] + s_start[3][k];
-            double e11s = (s_start[0][k] - (lambda / denom) * trs) / (2.0 * mu);
-            double e22s = (s_start[1][k] - (lambda / denom) * trs) / (2.0 * mu);
-            double e33s = (s_start[3][k] - (lambda / denom) * trs) / (2.0 * mu);
-            double e12s = s_start[2][k] / (2.0 * mu);
-            double trf = st[0][k] + st[1][k] + st[3][k];
-            double e11f = (st[0][k] - (lambda / denom) * trf) / (2.0 * mu);
-            double e22f = (st[1][k] - (lambda / denom) * trf) / (2.0 * mu);
-            double e33f = (st[3][k] - (lambda / denom) * trf) / (2.0 * mu);
-            double e12f = st[2][k] / (2.0 * mu);
-            double d11n = sr[k][0] * dt - (e11f - e11s);
-            double d22n = sr[k][1] * dt - (e22f - e22s);
-            double d12n = sr[k][2] * dt - (e12f - e12s);
-            double d33n = 0.0 - (e33f - e33s);
-            double a11 = 0.5 * (s_start[0][k] + st[0][k]);
-            double a22 = 0.5 * (s_start[1][k] + st[1][k]);
-            double a12 = 0.5 * (s_start[2][k] + st[2][k]);
-            double a33 = 0.5 * (s_start[3][k] + st[3][k]);
-            double q = a11 * d11n + a22 * d22n + 2.0 * a12 * d12n + a33 * d33n;
-            q_shear_elem = q_shear_elem + 0.25 * q;
-        }
-        sshr = sshr + q_shear_elem;
-    }
-    for (int c = 0; c < 4; c++)
-        for (int k = 0; k < 4; k++) A.stress0[e + NE * (c + 4 * k)] = st[c][k];
-
     // periodic refresh of the time-averaged strain rate, src/fl_rheol.f90:395-421
     if (avgsr != 0) {
         double dtavg = ck->dtavg;
@@ -521,6 +594,36 @@ __global__ void __launch_bounds__(256) k_rheol(const GfDevParams *__restrict__ P
     A.se2sr[e + NE] = se2;
     A.se2sr[e + 2 * NE] = se3;
     A.sshrheat[e] = sshr;
+}
+
+// Eff_visc of src/fl_rheol.f90:203-206 on viscosity-update steps (mod(nloop,
+// ifreq_visc) == 0), for the Maxwell rheologies, from the step-start stress, before
+// k_rheol: the same inputs the reference's in-loop call sees.
+__global__ void __launch_bounds__(256) k_visc(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+{
+    long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= NE) return;
+    const int irh = PHASE(A.iphase[e]).irheol;
+    if (!(irh == 3 || irh >= 11)) return;
+    const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
+    double prv[GF_MAXACT];
+    for (int a = 0; a < P->nactive; a++) prv[a] = A.pr[a * NE + e];
+    double a11 = 0.25 * (STRESS(j, i, 1, 1) + STRESS(j, i, 1, 2) + STRESS(j, i, 1, 3) + STRESS(j, i, 1, 4));
+    double a22 = 0.25 * (STRESS(j, i, 2, 1) + STRESS(j, i, 2, 2) + STRESS(j, i, 2, 3) + STRESS(j, i, 2, 4));
+    double a33 = 0.25 * (STRESS(j, i, 4, 1) + STRESS(j, i, 4, 2) + STRESS(j, i, 4, 3) + STRESS(j, i, 4, 4));
+    A.visn[e] = dev_eff_visc(P, A, j, i, a11, a22, a33, prv);
+}
+
+__global__ void __launch_bounds__(256) k_rheol(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
+                                               int avgsr)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long e = t >> 2;
+    if (e >= NE) return;  // whole quads leave together (4 divides the block size)
+    if (P->ishearh == 1)
+        rheol_quad<true>(P, ck, A, e, (int)(t & 3), avgsr);
+    else
+        rheol_quad<false>(P, ck, A, e, (int)(t & 3), avgsr);
 }
 
 // ------------------------------------------------------------------ fl_node
@@ -851,41 +954,59 @@ GF_DEV double side_len(const GfDevParams *P, const GfDevArrays &A, int ja, int i
     return sqrt(dx * dx + dz * dz);
 }
 
-// dlmin_prop (src/dt_mass.f90:168-232) and vel_max (:55-62); min/max of positive
-// doubles are exact in any order, and their bit patterns order like the values.
+// dlmin_prop (src/dt_mass.f90:168-232) and vel_max (:55-62): grid-stride loops, a
+// wave then block reduction, one atomic per block; min/max of positive doubles are
+// exact in any order, and their bit patterns order like the values.
+GF_DEV double elem_dlmin(const GfDevParams *P, const GfDevArrays &A, long t)
+{
+    const int j = (int)(t % (NZ - 1)) + 1, i = (int)(t / (NZ - 1)) + 1;
+    double dlmin = 1.0e28, dl;
+    dl = side_len(P, A, j, i, j + 1, i);
+    dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 1) * dl));
+    dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 3) * dl));
+    dl = side_len(P, A, j + 1, i, j + 1, i + 1);
+    dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 2) * dl));
+    dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 3) * dl));
+    dl = side_len(P, A, j, i + 1, j + 1, i + 1);
+    dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 2) * dl));
+    dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 4) * dl));
+    dl = side_len(P, A, j, i, j, i + 1);
+    dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 1) * dl));
+    dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 4) * dl));
+    dl = side_len(P, A, j, i, j + 1, i + 1);
+    dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 3) * dl));
+    dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 4) * dl));
+    dl = side_len(P, A, j, i + 1, j + 1, i);
+    dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 1) * dl));
+    dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 2) * dl));
+    return dlmin;
+}
+
+#define DT_RED_BLOCKS 1024
 __global__ void __launch_bounds__(256) k_dt_reduce(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
 {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double s_min[4], s_max[4];
+    const long stride = (long)gridDim.x * blockDim.x;
     double dlmin = 1.0e28, vmax = 0.0;
-    if (t < NE) {
-        const int j = (int)(t % (NZ - 1)) + 1, i = (int)(t / (NZ - 1)) + 1;
-        double dl;
-        dl = side_len(P, A, j, i, j + 1, i);
-        dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 1) * dl));
-        dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 3) * dl));
-        dl = side_len(P, A, j + 1, i, j + 1, i + 1);
-        dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 2) * dl));
-        dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 3) * dl));
-        dl = side_len(P, A, j, i + 1, j + 1, i + 1);
-        dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 2) * dl));
-        dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 4) * dl));
-        dl = side_len(P, A, j, i, j, i + 1);
-        dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 1) * dl));
-        dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 4) * dl));
-        dl = side_len(P, A, j, i, j + 1, i + 1);
-        dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 3) * dl));
-        dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 4) * dl));
-        dl = side_len(P, A, j, i + 1, j + 1, i);
-        dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 1) * dl));
-        dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 2) * dl));
-    }
-    for (long n = t; n < 2 * NN; n += (long)gridDim.x * blockDim.x) vmax = fmax(vmax, fabs(A.vel[n]));
-    // wave reduction, then one atomic per wave
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < NE; t += stride)
+        dlmin = fmin(dlmin, elem_dlmin(P, A, t));
+    for (long n = (long)blockIdx.x * blockDim.x + threadIdx.x; n < 2 * NN; n += stride)
+        vmax = fmax(vmax, fabs(A.vel[n]));
     for (int off = 32; off > 0; off >>= 1) {
         dlmin = fmin(dlmin, __shfl_xor(dlmin, off));
         vmax = fmax(vmax, __shfl_xor(vmax, off));
     }
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
+        s_min[w] = dlmin;
+        s_max[w] = vmax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < (int)(blockDim.x >> 6); q++) {
+            dlmin = fmin(dlmin, s_min[q]);
+            vmax = fmax(vmax, s_max[q]);
+        }
         atomicMin(&ck->dlmin_bits, (unsigned long long)__double_as_longlong(dlmin));
         atomicMax(&ck->velmax_bits, (unsigned long long)__double_as_longlong(vmax));
     }

@@ -33,7 +33,6 @@ struct gf_ctx {
     bool bound = false;
     uint32_t uploaded = 0;
     int32_t nloop = 0, nsrate = 0;
-    int therm_sweeps = 12;
     long ne = 0, nn = 0;
     std::vector<double> phase_ratio_host;  // full nphase layout, as uploaded
     int err_j = 0, err_i = 0;
@@ -488,19 +487,27 @@ static int enq_clock(gf_ctx *ctx, int op)
 
 static int enq_therm(gf_ctx *ctx)
 {
-    const long nn = ctx->nn;
     int rc;
-    LAUNCH(k_copy, nn, ctx->a.temp0, ctx->a.temp, nn);
     if ((rc = enq_clock(ctx, CK_THERM_BEGIN))) return rc;
-    const int S = ctx->therm_sweeps;
-    for (int s = 0; s < S; s++) {
-        const double *in = s == 0 ? ctx->a.temp0 : ((s - 1) % 2 == 0 ? ctx->a.xa : ctx->a.xb);
-        double *out = (s % 2 == 0) ? ctx->a.xa : ctx->a.xb;
-        LAUNCH(k_therm_sweep, nn, ctx->dp, ctx->dck, ctx->a, ctx->a.temp0, in, out, s);
+    const dim3 grid((unsigned)((ctx->p.nz + TH_OJ - 1) / TH_OJ), (unsigned)((ctx->p.nx + TH_OI - 1) / TH_OI));
+    for (int pass = 0; pass < TH_MAXPASS; pass++) {
+        const double *in = (pass % 2 == 1) ? ctx->a.xa : ctx->a.xb;
+        double *out = (pass % 2 == 0) ? ctx->a.xa : ctx->a.xb;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (ctx->timing) {
+            e0 = next_event(ctx);
+            e1 = next_event(ctx);
+            if (e0) (void)hipEventRecord(e0, ctx->stream);
+        }
+        hipLaunchKernelGGL(k_therm_tile, grid, dim3(TH_J * TH_I), 0, ctx->stream, ctx->dp, ctx->dck, ctx->a, in, out,
+                           pass);
+        if ((rc = launch_check(ctx, "k_therm_tile"))) return rc;
+        if (ctx->timing && e0 && e1) {
+            (void)hipEventRecord(e1, ctx->stream);
+            ctx->pending.push_back({pass == 0 ? "k_therm_tile" : "k_therm_verify", {e0, e1}});
+        }
     }
-    const double *last = ((S - 1) % 2 == 0) ? ctx->a.xa : ctx->a.xb;
-    const double *prev = S == 1 ? ctx->a.temp0 : (((S - 2) % 2 == 0) ? ctx->a.xa : ctx->a.xb);
-    LAUNCH(k_therm_final, nn, ctx->dp, ctx->dck, ctx->a, last, prev, S);
+    LAUNCH(k_therm_bc, ctx->nn, ctx->dp, ctx->dck, ctx->a, ctx->a.xa, ctx->a.xb);
     return GF_OK;
 }
 
@@ -514,7 +521,8 @@ static int enq_rheol(gf_ctx *ctx)
     else if (ctx->nsrate == -1)
         avgsr = 2;
     if ((rc = enq_clock(ctx, CK_DTAVG_ADD))) return rc;
-    LAUNCH(k_rheol, ctx->ne, ctx->dp, ctx->dck, ctx->a, do_visc, avgsr);
+    if (do_visc) LAUNCH(k_visc, ctx->ne, ctx->dp, ctx->dck, ctx->a);
+    LAUNCH(k_rheol, 4 * ctx->ne, ctx->dp, ctx->dck, ctx->a, avgsr);
     if (avgsr == 1) {
         if ((rc = enq_clock(ctx, CK_DTAVG_RESET))) return rc;
         ctx->nsrate = 0;
@@ -548,7 +556,7 @@ static int enq_dt_mass(gf_ctx *ctx)
 {
     int rc;
     if ((rc = enq_clock(ctx, CK_DT_BEGIN))) return rc;
-    LAUNCH(k_dt_reduce, ctx->ne, ctx->dp, ctx->dck, ctx->a);
+    LAUNCH(k_dt_reduce, (long)DT_RED_BLOCKS * 256, ctx->dp, ctx->dck, ctx->a);
     hipLaunchKernelGGL(k_dt_scalars, dim3(1), dim3(64), 0, ctx->stream, ctx->dp, ctx->dck, ctx->d_present,
                        ctx->npresent);
     if ((rc = launch_check(ctx, "k_dt_scalars"))) return rc;
@@ -643,13 +651,6 @@ extern "C" int gf_init_visc(gf_ctx *ctx)
     if (!ctx) return GF_ERR_ARG;
     LAUNCH(k_init_visc, ctx->ne, ctx->dp, ctx->dck, ctx->a);
     return fetch_clock(ctx, nullptr);
-}
-
-extern "C" int gf_set_therm_sweeps(gf_ctx *ctx, int32_t nsweeps)
-{
-    if (!ctx || nsweeps < 1 || nsweeps > GF_MAX_SWEEPS) return GF_ERR_ARG;
-    ctx->therm_sweeps = nsweeps;
-    return GF_OK;
 }
 
 extern "C" int gf_therm_unconverged(gf_ctx *ctx, int64_t *count)

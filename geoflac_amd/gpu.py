@@ -17,7 +17,7 @@ EXPORTS = [
     "gf_create", "gf_destroy", "gf_last_error", "gf_error_site", "gf_bind_host", "gf_upload",
     "gf_download", "gf_set_clock", "gf_get_clock", "gf_step", "gf_flac", "gf_fl_therm",
     "gf_fl_rheol", "gf_fl_node", "gf_fl_move", "gf_rmasses", "gf_dt_mass", "gf_init_areas",
-    "gf_init_visc", "gf_set_therm_sweeps", "gf_therm_unconverged", "gf_sync",
+    "gf_init_visc", "gf_therm_unconverged", "gf_sync",
     "gf_set_timing", "gf_kernel_time", "gf_math_check",
 ]
 
@@ -53,7 +53,6 @@ def lib():
             getattr(L, n).argtypes = [vp, C.POINTER(abi.GfClock)]
         L.gf_init_areas.argtypes = [vp]
         L.gf_init_visc.argtypes = [vp]
-        L.gf_set_therm_sweeps.argtypes = [vp, C.c_int32]
         L.gf_therm_unconverged.argtypes = [vp, C.POINTER(C.c_int64)]
         L.gf_sync.argtypes = [vp]
         L.gf_set_timing.argtypes = [vp, C.c_int32]
@@ -112,9 +111,6 @@ class Model:
 
     def init_visc(self):
         return self._check(lib().gf_init_visc(self._ctx))
-
-    def set_therm_sweeps(self, n):
-        return self._check(lib().gf_set_therm_sweeps(self._ctx, int(n)))
 
     def therm_unconverged(self):
         v = C.c_int64()

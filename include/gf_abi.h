@@ -144,10 +144,9 @@ int gf_dt_mass(gf_ctx *ctx, gf_clock *clock);    /* src/dt_mass.f90:16-160 */
 int gf_init_areas(gf_ctx *ctx);                  /* src/init_areas.f90:7-55 */
 int gf_init_visc(gf_ctx *ctx);                   /* src/init_visc.f90:1-18  */
 
-/* Thermal update: number of fixed-point sweeps that reproduce the reference's
- * in-place (Gauss-Seidel) order, and the count of nodes found unconverged by the
- * verification sweep since the last call (0 means bitwise equal to serial order). */
-int gf_set_therm_sweeps(gf_ctx *ctx, int32_t nsweeps);
+/* Thermal update: number of thermal calls since the last query whose fixed-point
+ * iteration was not proven equal to the reference's in-place serial order (the
+ * result is bitwise the serial one whenever this stays 0). */
 int gf_therm_unconverged(gf_ctx *ctx, int64_t *count);
 
 /* Synchronise the context's stream (all queued steps finished). */
