@@ -29,7 +29,7 @@ METRIC = "element-steps/s on 1 GPU: mixed/learned vs plain FP64, at stated rel-L
 def alg_bytes(kernel, E, N, nact, p):
     """Algorithmic (compulsory) HBM bytes of one launch: each array the kernel must
     read once and write once, FP64 = 8 B, int32 = 4 B."""
-    if kernel == "k_rheol":
+    if kernel.startswith("k_rheol"):
         per_e = (32 + 32 + 128 + 8 + 8 + 8 * nact + 4 + 24 + 8 + 24)  # area dvol stress0 aps visn pr iphase se2sr sshrheat strain
         per_e += 128 + 8 + 24 + 24 + 8                                 # writes stress0 aps strain se2sr sshrheat
         per_n = 16 + 16                                                # vel, cord
@@ -87,6 +87,8 @@ def main():
     ap.add_argument("--config", default="layers4000x1000")
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 1 GiB device copy (torch) to calibrate HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (one model per device); "
@@ -110,9 +112,10 @@ def main():
     value = E * args.steps / el
 
     # profiled run: HIP events around every launch on the library stream
+    args.profile_steps = max(args.profile_steps, 1)
     m.set_timing(True)
     m.step(args.profile_steps)
-    kernels = ["k_rheol", "k_therm_tile", "k_therm_verify", "k_node", "k_move_elem", "k_move_cord",
+    kernels = ["k_rheol1<false>", "k_rheol1<true>", "k_rheol", "k_visc", "k_therm_tile", "k_therm_verify", "k_node", "k_move_elem", "k_move_cord",
                "k_therm_bc", "k_dens", "k_rmass", "k_dt_reduce", "k_amass", "k_clock"]
     times = {k: m.kernel_time(k) for k in kernels}
     m.set_timing(False)
@@ -127,6 +130,13 @@ def main():
                 "alg_bytes_per_launch": ab, "avg_launch_us": round(avg_s * 1e6, 2),
                 "kernel_ms_per_step": {k: round(v[0] / args.profile_steps, 4) for k, v in times.items() if v[1]}}
     m.close()
+    if args.copy_calibration:
+        import torch
+        x = torch.empty(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
 
     out = {"metric": METRIC, "value": value, "unit": "element-steps/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,

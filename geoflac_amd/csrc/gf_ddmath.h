@@ -14,12 +14,10 @@
 #pragma once
 #ifdef __HIPCC__
 #include <hip/hip_runtime.h>
-#define DD_DEV static __device__ __forceinline__
-#define DD_TABLE __device__ static const
-#else  // host build of the same code, used by the CPU unit test of these functions
+#define DD_DEV static __host__ __device__ __forceinline__
+#else  // host-only build of the same code (the CPU oracle variant and its unit test)
 #include <math.h>
 #define DD_DEV static inline
-#define DD_TABLE static const
 #endif
 
 struct ddv {
@@ -27,40 +25,44 @@ struct ddv {
 };
 
 // 1/n! as exact double-double pairs, n = 0..31
-DD_TABLE double kInvFact[32][2] = {
-    {0x1.0000000000000p+0, 0x0.0p+0},
-    {0x1.0000000000000p+0, 0x0.0p+0},
-    {0x1.0000000000000p-1, 0x0.0p+0},
-    {0x1.5555555555555p-3, 0x1.5555555555555p-57},
-    {0x1.5555555555555p-5, 0x1.5555555555555p-59},
-    {0x1.1111111111111p-7, 0x1.1111111111111p-63},
-    {0x1.6c16c16c16c17p-10, -0x1.f49f49f49f49fp-65},
-    {0x1.a01a01a01a01ap-13, 0x1.a01a01a01a01ap-73},
-    {0x1.a01a01a01a01ap-16, 0x1.a01a01a01a01ap-76},
-    {0x1.71de3a556c734p-19, -0x1.c154f8ddc6c00p-73},
-    {0x1.27e4fb7789f5cp-22, 0x1.cbbc05b4fa99ap-76},
-    {0x1.ae64567f544e4p-26, -0x1.c062e06d1f209p-80},
-    {0x1.1eed8eff8d898p-29, -0x1.2aec959e14c06p-83},
-    {0x1.6124613a86d09p-33, 0x1.f28e0cc748ebep-87},
-    {0x1.93974a8c07c9dp-37, 0x1.05d6f8a2efd1fp-92},
-    {0x1.ae7f3e733b81fp-41, 0x1.1d8656b0ee8cbp-97},
-    {0x1.ae7f3e733b81fp-45, 0x1.1d8656b0ee8cbp-101},
-    {0x1.952c77030ad4ap-49, 0x1.ac981465ddc6cp-103},
-    {0x1.6827863b97d97p-53, 0x1.eec01221a8b0bp-107},
-    {0x1.2f49b46814157p-57, 0x1.2650f61dbdcb4p-112},
-    {0x1.e542ba4020225p-62, 0x1.ea72b4afe3c2fp-120},
-    {0x1.71b8ef6dcf572p-66, -0x1.d043ae40c4647p-120},
-    {0x1.0ce396db7f853p-70, -0x1.aebcdbd20331cp-124},
-    {0x1.761b41316381ap-75, -0x1.3423c7d91404fp-130},
-    {0x1.f2cf01972f578p-80, -0x1.9ada5fcc1ab14p-135},
-    {0x1.3f3ccdd165fa9p-84, -0x1.58ddadf344487p-139},
-    {0x1.88e85fc6a4e5ap-89, -0x1.71c37ebd16540p-143},
-    {0x1.d1ab1c2dccea3p-94, 0x1.054d0c78aea14p-149},
-    {0x1.0a18a2635085dp-98, 0x1.b9e2e28e1aa54p-153},
-    {0x1.259f98b4358adp-103, 0x1.eaf8c39dd9bc5p-157},
-    {0x1.3932c5047d60ep-108, 0x1.832b7b530a627p-162},
-    {0x1.434d2e783f5bcp-113, 0x1.0b87b91be9affp-167},
-};
+#define GF_KINVFACT_DATA \
+    {0x1.0000000000000p+0, 0x0.0p+0}, \
+    {0x1.0000000000000p+0, 0x0.0p+0}, \
+    {0x1.0000000000000p-1, 0x0.0p+0}, \
+    {0x1.5555555555555p-3, 0x1.5555555555555p-57}, \
+    {0x1.5555555555555p-5, 0x1.5555555555555p-59}, \
+    {0x1.1111111111111p-7, 0x1.1111111111111p-63}, \
+    {0x1.6c16c16c16c17p-10, -0x1.f49f49f49f49fp-65}, \
+    {0x1.a01a01a01a01ap-13, 0x1.a01a01a01a01ap-73}, \
+    {0x1.a01a01a01a01ap-16, 0x1.a01a01a01a01ap-76}, \
+    {0x1.71de3a556c734p-19, -0x1.c154f8ddc6c00p-73}, \
+    {0x1.27e4fb7789f5cp-22, 0x1.cbbc05b4fa99ap-76}, \
+    {0x1.ae64567f544e4p-26, -0x1.c062e06d1f209p-80}, \
+    {0x1.1eed8eff8d898p-29, -0x1.2aec959e14c06p-83}, \
+    {0x1.6124613a86d09p-33, 0x1.f28e0cc748ebep-87}, \
+    {0x1.93974a8c07c9dp-37, 0x1.05d6f8a2efd1fp-92}, \
+    {0x1.ae7f3e733b81fp-41, 0x1.1d8656b0ee8cbp-97}, \
+    {0x1.ae7f3e733b81fp-45, 0x1.1d8656b0ee8cbp-101}, \
+    {0x1.952c77030ad4ap-49, 0x1.ac981465ddc6cp-103}, \
+    {0x1.6827863b97d97p-53, 0x1.eec01221a8b0bp-107}, \
+    {0x1.2f49b46814157p-57, 0x1.2650f61dbdcb4p-112}, \
+    {0x1.e542ba4020225p-62, 0x1.ea72b4afe3c2fp-120}, \
+    {0x1.71b8ef6dcf572p-66, -0x1.d043ae40c4647p-120}, \
+    {0x1.0ce396db7f853p-70, -0x1.aebcdbd20331cp-124}, \
+    {0x1.761b41316381ap-75, -0x1.3423c7d91404fp-130}, \
+    {0x1.f2cf01972f578p-80, -0x1.9ada5fcc1ab14p-135}, \
+    {0x1.3f3ccdd165fa9p-84, -0x1.58ddadf344487p-139}, \
+    {0x1.88e85fc6a4e5ap-89, -0x1.71c37ebd16540p-143}, \
+    {0x1.d1ab1c2dccea3p-94, 0x1.054d0c78aea14p-149}, \
+    {0x1.0a18a2635085dp-98, 0x1.b9e2e28e1aa54p-153}, \
+    {0x1.259f98b4358adp-103, 0x1.eaf8c39dd9bc5p-157}, \
+    {0x1.3932c5047d60ep-108, 0x1.832b7b530a627p-162}, \
+    {0x1.434d2e783f5bcp-113, 0x1.0b87b91be9affp-167}, \
+
+#ifdef __HIPCC__
+__device__ static const double kInvFactD[32][2] = {GF_KINVFACT_DATA};
+#endif
+static const double kInvFactH[32][2] = {GF_KINVFACT_DATA};
 
 DD_DEV ddv dd_two_sum(double a, double b)
 {
@@ -126,7 +128,14 @@ DD_DEV ddv dd_div(ddv a, ddv b)
     return dd_add_d(q, q3);
 }
 
-DD_DEV ddv dd_fact(int n) { return {kInvFact[n][0], kInvFact[n][1]}; }
+DD_DEV ddv dd_fact(int n)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return {kInvFactD[n][0], kInvFactD[n][1]};
+#else
+    return {kInvFactH[n][0], kInvFactH[n][1]};
+#endif
+}
 
 // sin and cos of |x| <= pi/4 by Taylor series in x^2 (terms to x^29 / x^28)
 DD_DEV ddv dd_sin_small(ddv x)

@@ -29,6 +29,9 @@ struct GfPhase {
     double pln_r;        // pln * 8.31448
     double diff;         // conduct/den/cp           (src/fl_therm.f90:412)
     double dt_m;         // maxwell dt candidate     (src/dt_mass.f90:135-147), 1e28 if none
+    // friction and dilation angles of a one-phase element before softening starts,
+    // and their correctly rounded sin(phi), sin(psi), tan(phi) (src/rh_plastic.f90:33-46)
+    double phi0, psi0, sphi0, spsi0, tphi0;
 };
 
 struct GfDevParams {
@@ -55,6 +58,34 @@ struct GfDevClock {
     unsigned long long therm_unconverged;
     unsigned int therm_pass_changed[GF_MAX_SWEEPS];  // bit 0: first sweep changed a bit; bit 1: a tile did not settle
 };
+
+// ---------------------------------------------------------------- buffer access
+// Raw buffer loads and stores (CDNA buffer instructions): the array base lives in an
+// SGPR descriptor, the per-thread 32-bit byte offset in one VGPR shared by every
+// plane of the same shape, and the plane offset in an SGPR, so a kernel touching
+// twenty planes does not hold twenty 64-bit addresses in vector registers.  Accesses
+// past the descriptor's size read 0 and drop writes.  Arrays are limited to 4 GiB.
+typedef unsigned int gf_u2 __attribute__((ext_vector_type(2)));
+struct Buf {
+    __amdgpu_buffer_rsrc_t r;
+};
+GF_DEV Buf mkbuf(const void *p, unsigned long long bytes)
+{
+    const int n = bytes >= 0xffffffffull ? (int)0xffffffff : (int)bytes;
+    return {__builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, n, 0x00020000)};
+}
+GF_DEV double bld(Buf b, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(b.r, (int)voff, (int)soff, 0));
+}
+GF_DEV int bldi(Buf b, unsigned voff, unsigned soff)
+{
+    return (int)__builtin_amdgcn_raw_buffer_load_b32(b.r, (int)voff, (int)soff, 0);
+}
+GF_DEV void bst(Buf b, unsigned voff, unsigned soff, double v)
+{
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(gf_u2, v), b.r, (int)voff, (int)soff, 0);
+}
 
 // ---------------------------------------------------------------- helpers
 GF_DEV double fsign(double a, double b) { return b < 0.0 ? -fabs(a) : fabs(a); }

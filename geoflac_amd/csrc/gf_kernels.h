@@ -382,25 +382,50 @@ GF_DEV void dev_pre_plast(const GfDevParams *P, double aps, const double *prv, d
 // (nsrate == ifreq_avgsr), 2 refresh only (nsrate == -1).
 #define GF_MAXACT 8
 
-GF_DEV double qget(double v, int lane) { return __shfl(v, lane, 4); }
+// quad permutation with DPP (one VALU move per 32-bit half): lane q of every quad
+// receives the value of lane QP(...)[q] of the same quad
+#define QP(a, b, c, d) ((a) | ((b) << 2) | ((c) << 4) | ((d) << 6))
+template <int CTRL>
+GF_DEV double qperm(double v)
+{
+    long long b = __double_as_longlong(v);
+    int lo = (int)(b & 0xffffffffll), hi = (int)(b >> 32);
+    lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
+    hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+#define QL(v, L) qperm<QP(L, L, L, L)>(v)
 
 template <bool SHEARH>
 GF_DEV void rheol_quad(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e, int k,
                        int avgsr)
 {
-    const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
+    const unsigned nzm = (unsigned)(NZ - 1), ne = (unsigned)NE, nn = (unsigned)NN, nzu = (unsigned)NZ;
+    const int j = (int)((unsigned)e % nzm) + 1, i = (int)((unsigned)e / nzm) + 1;
+    const unsigned pe = ne * 8u, pn = nn * 8u;
+    const unsigned eo = (unsigned)e * 8u;          // element byte offset
+    const unsigned ek = eo + (unsigned)k * pe;     // this lane's triangle plane (area, dvol)
+    const unsigned es = eo + 4u * (unsigned)k * pe; // this lane's stress planes
+    const Buf bS = mkbuf(A.stress0, 16ull * pe), bA = mkbuf(A.area, 4ull * pe), bD = mkbuf(A.dvol, 4ull * pe);
+    const Buf bSE = mkbuf(A.se2sr, 3ull * pe), bST = mkbuf(A.strain, 3ull * pe);
+    const Buf bAPS = mkbuf(A.aps, pe), bVIS = mkbuf(A.visn, pe), bSH = mkbuf(A.sshrheat, pe);
+    const Buf bV = mkbuf(A.vel, 2ull * pn), bC = mkbuf(A.cord, 2ull * pn);
     const double dt = ck->dt;
     // corner k of the element: 1 (j,i), 2 (j+1,i), 3 (j,i+1), 4 (j+1,i+1)
-    const int cj = j + (k & 1), ci = i + (k >> 1);
-    const double mvx = VEL(cj, ci, 1), mvy = VEL(cj, ci, 2), mx = CORD(cj, ci, 1), my = CORD(cj, ci, 2);
-    const double ar = A.area[e + NE * k];
-    // vertices of triangle k: A (1,2,3), B (3,2,4), C (1,2,4), D (1,4,3)
-    const int va = k == 1 ? 2 : 0;
-    const int vb = k == 3 ? 3 : 1;
-    const int vc = (k == 0 || k == 3) ? 2 : 3;
-    const double vx1 = qget(mvx, va), vy1 = qget(mvy, va), x1 = qget(mx, va), y1 = qget(my, va);
-    const double vx2 = qget(mvx, vb), vy2 = qget(mvy, vb), x2 = qget(mx, vb), y2 = qget(my, vb);
-    const double vx3 = qget(mvx, vc), vy3 = qget(mvy, vc), x3 = qget(mx, vc), y3 = qget(my, vc);
+    const unsigned cno = ((unsigned)(j - 1 + (k & 1)) + nzu * (unsigned)(i - 1 + (k >> 1))) * 8u;
+    const double mvx = bld(bV, cno, 0), mvy = bld(bV, cno, pn), mx = bld(bC, cno, 0), my = bld(bC, cno, pn);
+    const double ar = bld(bA, ek, 0);
+    // vertices of triangle k: A (1,2,3), B (3,2,4), C (1,2,4), D (1,4,3): lane k takes
+    // its first vertex from lane (0,2,0,0)[k], second from (1,1,1,3)[k], third (2,3,3,2)[k]
+#define QV1(v) qperm<QP(0, 2, 0, 0)>(v)
+#define QV2(v) qperm<QP(1, 1, 1, 3)>(v)
+#define QV3(v) qperm<QP(2, 3, 3, 2)>(v)
+    const double vx1 = QV1(mvx), vy1 = QV1(mvy), x1 = QV1(mx), y1 = QV1(my);
+    const double vx2 = QV2(mvx), vy2 = QV2(mvy), x2 = QV2(mx), y2 = QV2(my);
+    const double vx3 = QV3(mvx), vy3 = QV3(mvy), x3 = QV3(mx), y3 = QV3(my);
+#undef QV1
+#undef QV2
+#undef QV3
     double sr[3];
     {
         // :98-132: for every triangle, with its vertices (v1,v2,v3) in the order the
@@ -413,18 +438,19 @@ GF_DEV void rheol_quad(const GfDevParams *__restrict__ P, GfDevClock *ck, const 
         sr[2] = 0.5 * (vx1 * dz0 + vx2 * dz1 + vx3 * dz2 + vy1 * dx0 + vy2 * dx1 + vy3 * dx2);
     }
     if (P->mix_strain == 1) {  // pairs (A,B) and (C,D), :158-168
-        const int ev = k & 2, od = ev + 1;
-        double e0 = qget(sr[0], ev), e1 = qget(sr[1], ev), o0 = qget(sr[0], od), o1 = qget(sr[1], od);
+        // even partner (0,0,2,2)[k], odd partner (1,1,3,3)[k]
+        double e0 = qperm<QP(0, 0, 2, 2)>(sr[0]), e1 = qperm<QP(0, 0, 2, 2)>(sr[1]);
+        double o0 = qperm<QP(1, 1, 3, 3)>(sr[0]), o1 = qperm<QP(1, 1, 3, 3)>(sr[1]);
         double em = 0.5 * (e0 + e1 + o0 + o1);
         double ed = (k & 1) ? o0 - o1 : e0 - e1;
         sr[0] = 0.5 * (em + ed);
         sr[1] = 0.5 * (em - ed);
     }
     double st[4];
-    for (int c = 0; c < 4; c++) st[c] = A.stress0[e + NE * (c + 4 * k)];
+    for (int c = 0; c < 4; c++) st[c] = bld(bS, es, c * pe);
 
     auto sum4 = [&](double v) {  // ((((0 + v_A) + v_B) + v_C) + v_D), Fortran SUM order
-        return 0.0 + qget(v, 0) + qget(v, 1) + qget(v, 2) + qget(v, 3);
+        return 0.0 + QL(v, 0) + QL(v, 1) + QL(v, 2) + QL(v, 3);
     };
     double se1, se2, se3, sshr, s11, s22, s12;
     {
@@ -434,14 +460,14 @@ GF_DEV void rheol_quad(const GfDevParams *__restrict__ P, GfDevClock *ck, const 
         double srII = 0.5 * sqrt((a11 - a22) * (a11 - a22) + 4.0 * a12 * a12);
         double srI = 0.5 * (a11 + a22);
         double srs2 = (a11 - srI) * (a11 - srI) + (a22 - srI) * (a22 - srI) + 2.0 * a12 * a12;
-        se1 = A.se2sr[e] + a11 * dt;
-        se2 = A.se2sr[e + NE] + a22 * dt;
-        se3 = A.se2sr[e + 2 * NE] + a12 * dt;
+        se1 = bld(bSE, eo, 0) + a11 * dt;
+        se2 = bld(bSE, eo, pe) + a22 * dt;
+        se3 = bld(bSE, eo, 2 * pe) + a12 * dt;
         s11 = 0.25 * sum4(st[0]);
         s22 = 0.25 * sum4(st[1]);
         s12 = 0.25 * sum4(st[2]);
         double stII = 0.5 * sqrt((s11 - s22) * (s11 - s22) + 4.0 * s12 * s12);
-        sshr = A.sshrheat[e];
+        sshr = bld(bSH, eo, 0);
         if (srII != 0.0) sshr = sshr + stII / srII * srs2 * dt;
     }
     const int iph = A.iphase[e];
@@ -450,35 +476,46 @@ GF_DEV void rheol_quad(const GfDevParams *__restrict__ P, GfDevClock *ck, const 
     const double bulkm = ph.bulkm, rmu = ph.rm;
     double stherm = 0.0;
     if (P->istress_therm > 0) stherm = -ph.alfa * bulkm * (A.temp[IN(j, i)] - A.temp0[IN(j, i)]);
-    double aps = A.aps[e];
-    const double vis = A.visn[e];  // refreshed by k_visc on viscosity-update steps
+    double aps = bld(bAPS, eo, 0);
+    const double vis = bld(bVIS, eo, 0);  // refreshed by k_visc on viscosity-update steps
     const bool plas_law = irh == 6 || irh >= 11;
     PlasticConst pc{};
     double hardn = 0;
     if (plas_law) {
         double prv[GF_MAXACT], coh, phi, psi;
-        for (int a = 0; a < P->nactive; a++) prv[a] = A.pr[a * NE + e];
+        const Buf bPR = mkbuf(A.pr, (unsigned long long)P->nactive * pe);
+        for (int a = 0; a < P->nactive; a++) prv[a] = bld(bPR, eo, a * pe);
         dev_pre_plast(P, aps, prv, coh, phi, psi, hardn);
-        // src/rh_plastic.f90:33-46: sin(phi), sin(psi) and tan(phi), one double-double
-        // sine/cosine per lane (lane 1 takes psi)
-        const double degrad = 3.14159265358979323846 / 180.0;
-        const double xarg = (k == 1 ? psi : phi) * degrad;
-        double sv, tv = 0.0;
-        if (dd_trig_range(xarg)) {
-            ddv s_, c_;
-            dd_sincos(xarg, s_, c_);
-            sv = s_.hi + s_.lo;
-            if (k == 0 && phi != 0.0 && xarg != 1.5707963267948966) {
-                ddv t_ = dd_div(s_, c_);
-                tv = t_.hi + t_.lo;
-            } else if (k == 0 && phi != 0.0) {
-                tv = tan(xarg);
-            }
+        // src/rh_plastic.f90:33-46: sin(phi), sin(psi) and tan(phi).  Elements whose
+        // angles equal the phase's unsoftened ones reuse its correctly rounded values;
+        // the others take one double-double sine/cosine per lane (lane 1 takes psi).
+        double sphi, spsi, tphi;
+        if (phi == ph.phi0 && psi == ph.psi0) {
+            sphi = ph.sphi0;
+            spsi = ph.spsi0;
+            tphi = ph.tphi0;
         } else {
-            sv = sin(xarg);
-            if (k == 0 && phi != 0.0) tv = tan(xarg);
+            const double degrad = 3.14159265358979323846 / 180.0;
+            const double xarg = (k == 1 ? psi : phi) * degrad;
+            double sv, tv = 0.0;
+            if (dd_trig_range(xarg)) {
+                ddv s_, c_;
+                dd_sincos(xarg, s_, c_);
+                sv = s_.hi + s_.lo;
+                if (k == 0 && phi != 0.0 && xarg != 1.5707963267948966) {
+                    ddv t_ = dd_div(s_, c_);
+                    tv = t_.hi + t_.lo;
+                } else if (k == 0 && phi != 0.0) {
+                    tv = tan(xarg);
+                }
+            } else {
+                sv = sin(xarg);
+                if (k == 0 && phi != 0.0) tv = tan(xarg);
+            }
+            sphi = QL(sv, 0);
+            spsi = QL(sv, 1);
+            tphi = QL(tv, 0);
         }
-        const double sphi = qget(sv, 0), spsi = qget(sv, 1), tphi = qget(tv, 0);
         const double c4d3 = 4.0 / 3.0, c2d3 = 2.0 / 3.0;
         pc.anphi = (1.0 + sphi) / (1.0 - sphi);
         pc.anpsi = (1.0 + spsi) / (1.0 - spsi);
@@ -493,7 +530,7 @@ GF_DEV void rheol_quad(const GfDevParams *__restrict__ P, GfDevClock *ck, const 
         for (int c = 0; c < 4; c++) s_start[c] = st[c];
 
     const double de11 = sr[0] * dt, de22 = sr[1] * dt, de12 = sr[2] * dt;
-    const double dv = A.dvol[e + NE * k] - 0.0;
+    const double dv = bld(bD, ek, 0) - 0.0;
     double sp[4] = {st[0] + stherm, st[1] + stherm, st[2], st[3] + stherm};
     double depl = 0.0;
     bool ok = true;
@@ -511,22 +548,22 @@ GF_DEV void rheol_quad(const GfDevParams *__restrict__ P, GfDevClock *ck, const 
         dev_plastic(pc, rmu, hardn, depl, sp[0], sp[1], sp[3], sp[2], de11, de22, de12);
         ok = dev_maxwell(bulkm, rmu, vis, sv[0], sv[1], sv[3], sv[2], de11, de22, de12, dv, dt);
         // :266-288 the branch with the smaller summed invariant, same bits on every lane
-        double a = qget(sp[0], 0) + qget(sp[0], 1) + qget(sp[0], 2) + qget(sp[0], 3) - qget(sp[1], 0) -
-                   qget(sp[1], 1) - qget(sp[1], 2) - qget(sp[1], 3);
-        double b = qget(sp[2], 0) + qget(sp[2], 1) + qget(sp[2], 2) + qget(sp[2], 3);
+        double a = QL(sp[0], 0) + QL(sp[0], 1) + QL(sp[0], 2) + QL(sp[0], 3) - QL(sp[1], 0) - QL(sp[1], 1) -
+                   QL(sp[1], 2) - QL(sp[1], 3);
+        double b = QL(sp[2], 0) + QL(sp[2], 1) + QL(sp[2], 2) + QL(sp[2], 3);
         double sII_plas = a * a + 4.0 * (b * b);
-        double av = qget(sv[0], 0) + qget(sv[0], 1) + qget(sv[0], 2) + qget(sv[0], 3) - qget(sv[1], 0) -
-                    qget(sv[1], 1) - qget(sv[1], 2) - qget(sv[1], 3);
-        double bv = qget(sv[2], 0) + qget(sv[2], 1) + qget(sv[2], 2) + qget(sv[2], 3);
+        double av = QL(sv[0], 0) + QL(sv[0], 1) + QL(sv[0], 2) + QL(sv[0], 3) - QL(sv[1], 0) - QL(sv[1], 1) -
+                    QL(sv[1], 2) - QL(sv[1], 3);
+        double bv = QL(sv[2], 0) + QL(sv[2], 1) + QL(sv[2], 2) + QL(sv[2], 3);
         double sII_visc = av * av + 4.0 * (bv * bv);
         const bool plas = sII_plas < sII_visc;
         for (int c = 0; c < 4; c++) st[c] = plas ? sp[c] : sv[c];
     }
     if (!ok) raise_error(ck, GF_STOP_MAXWELL, e);
     if (P->mix_stress == 1) {  // pairs (A,B), (C,D), :291-309
-        const int ev = k & 2, od = ev + 1;
-        double ar_e = qget(ar, ev), ar_o = qget(ar, od);
-        double e0 = qget(st[0], ev), e1 = qget(st[1], ev), o0 = qget(st[0], od), o1 = qget(st[1], od);
+        double ar_e = qperm<QP(0, 0, 2, 2)>(ar), ar_o = qperm<QP(1, 1, 3, 3)>(ar);
+        double e0 = qperm<QP(0, 0, 2, 2)>(st[0]), e1 = qperm<QP(0, 0, 2, 2)>(st[1]);
+        double o0 = qperm<QP(1, 1, 3, 3)>(st[0]), o1 = qperm<QP(1, 1, 3, 3)>(st[1]);
         double quad_area = 1.0 / ar_e + 1.0 / ar_o;
         double s0a = 0.5 * (e0 + e1);
         double s0b = 0.5 * (o0 + o1);
@@ -538,12 +575,12 @@ GF_DEV void rheol_quad(const GfDevParams *__restrict__ P, GfDevClock *ck, const 
     st[0] = st[0] - stherm;
     st[1] = st[1] - stherm;
     st[3] = st[3] - stherm;
-    for (int c = 0; c < 4; c++) A.stress0[e + NE * (c + 4 * k)] = st[c];
+    for (int c = 0; c < 4; c++) bst(bS, es, c * pe, st[c]);
 
     if (plas_law) {  // :319-328
-        double a0 = qget(ar, 0), a1 = qget(ar, 1), a2 = qget(ar, 2), a3 = qget(ar, 3);
+        double a0 = QL(ar, 0), a1 = QL(ar, 1), a2 = QL(ar, 2), a3 = QL(ar, 3);
         double quad_area = a0 + a1 + a2 + a3;
-        aps = aps + (qget(depl, 0) * a0 + qget(depl, 1) * a1 + qget(depl, 2) * a2 + qget(depl, 3) * a3) / quad_area;
+        aps = aps + (QL(depl, 0) * a0 + QL(depl, 1) * a1 + QL(depl, 2) * a2 + QL(depl, 3) * a3) / quad_area;
         if (aps < 0.0) aps = 0.0;
         if (P->tau_heal != 0.0) aps = aps / (1.0 + dt / P->tau_heal);
     }
@@ -571,15 +608,15 @@ GF_DEV void rheol_quad(const GfDevParams *__restrict__ P, GfDevClock *ck, const 
         double a12 = 0.5 * (s0[2] + st[2]);
         double a33 = 0.5 * (s0[3] + st[3]);
         double q = 0.25 * (a11 * d11n + a22 * d22n + 2.0 * a12 * d12n + a33 * d33n);
-        double q_shear_elem = 0.0 + qget(q, 0) + qget(q, 1) + qget(q, 2) + qget(q, 3);
+        double q_shear_elem = 0.0 + QL(q, 0) + QL(q, 1) + QL(q, 2) + QL(q, 3);
         sshr = sshr + q_shear_elem;
     }
     if (k != 0) return;
-    if (plas_law) A.aps[e] = aps;
+    if (plas_law) bst(bAPS, eo, 0, aps);
     // the reference adds the averaged (step-start) STRESS times dt: s11..s12 were reused
-    A.strain[e] = A.strain[e] + s11 * dt;
-    A.strain[e + NE] = A.strain[e + NE] + s22 * dt;
-    A.strain[e + 2 * NE] = A.strain[e + 2 * NE] + s12 * dt;
+    bst(bST, eo, 0, bld(bST, eo, 0) + s11 * dt);
+    bst(bST, eo, pe, bld(bST, eo, pe) + s22 * dt);
+    bst(bST, eo, 2 * pe, bld(bST, eo, 2 * pe) + s12 * dt);
     // periodic refresh of the time-averaged strain rate, src/fl_rheol.f90:395-421
     if (avgsr != 0) {
         double dtavg = ck->dtavg;
@@ -590,10 +627,281 @@ GF_DEV void rheol_quad(const GfDevParams *__restrict__ P, GfDevClock *ck, const 
             se1 = 0.0; se2 = 0.0; se3 = 0.0; sshr = 0.0;
         }
     }
-    A.se2sr[e] = se1;
-    A.se2sr[e + NE] = se2;
-    A.se2sr[e + 2 * NE] = se3;
-    A.sshrheat[e] = sshr;
+    bst(bSE, eo, 0, se1);
+    bst(bSE, eo, pe, se2);
+    bst(bSE, eo, 2 * pe, se3);
+    bst(bSH, eo, 0, sshr);
+}
+
+// correctly rounded sin(phi), sin(psi), tan(phi) of src/rh_plastic.f90:33-46 (degrees);
+// out of line: one copy of the double-double code, off the caller's register budget
+static __device__ __noinline__ void cr_sin_sin_tan(double phi, double psi, double *sphi, double *spsi, double *tphi)
+{
+    const double degrad = 3.14159265358979323846 / 180.0;
+    *sphi = cr_sin(phi * degrad);
+    *spsi = cr_sin(psi * degrad);
+    *tphi = phi == 0.0 ? 0.0 : cr_tan(phi * degrad);
+}
+
+// the constitutive update of one triangle for rheology irh (:216-264); returns false on STOP 22
+GF_DEV bool rheol_tri(int irh, bool plastic_branch, const PlasticConst &pc, double bulkm, double rmu,
+                      double hardn, double vis, double dt, double stherm, const double st_k[4],
+                      const double sr_k[3], double dv, double out[4], double &depl)
+{
+    double de11 = sr_k[0] * dt, de22 = sr_k[1] * dt, de12 = sr_k[2] * dt;
+    out[0] = st_k[0] + stherm;
+    out[1] = st_k[1] + stherm;
+    out[2] = st_k[2];
+    out[3] = st_k[3] + stherm;
+    bool ok = true;
+    if (irh == 1) {
+        dev_elastic(bulkm, rmu, out[0], out[1], out[3], out[2], de11, de22, de12);
+    } else if (irh == 3 || (irh >= 11 && !plastic_branch)) {
+        ok = dev_maxwell(bulkm, rmu, vis, out[0], out[1], out[3], out[2], de11, de22, de12, dv, dt);
+    } else if (irh == 6 || irh >= 11) {
+        dev_plastic(pc, rmu, hardn, depl, out[0], out[1], out[3], out[2], de11, de22, de12);
+    }
+    return ok;
+}
+
+// Single-lane variant of the same update (one thread per element), used where a
+// quad's repeated element-level work costs more than the occupancy it buys.
+// The invariant loop accumulates both branches' sums without keeping either stress
+// set, and the chosen branch is recomputed; scheduling barriers bound live ranges.
+#define SB() __builtin_amdgcn_sched_barrier(0)
+template <bool SHEARH>
+GF_DEV void rheol_single(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e, int avgsr)
+{
+    const unsigned nzm = (unsigned)(NZ - 1);
+    const unsigned ne = (unsigned)NE, nn = (unsigned)NN, nzu = (unsigned)NZ;
+    const int j = (int)((unsigned)e % nzm) + 1, i = (int)((unsigned)e / nzm) + 1;
+    const unsigned eo = (unsigned)e * 8u;                             // element byte offset
+    const unsigned no = ((unsigned)(j - 1) + nzu * (unsigned)(i - 1)) * 8u;  // corner 1 byte offset
+    const unsigned pe = ne * 8u, pn = nn * 8u, col = nzu * 8u;        // plane and column strides
+    const Buf bS = mkbuf(A.stress0, 16ull * pe), bA = mkbuf(A.area, 4ull * pe), bD = mkbuf(A.dvol, 4ull * pe);
+    const Buf bSE = mkbuf(A.se2sr, 3ull * pe), bST = mkbuf(A.strain, 3ull * pe);
+    const Buf bAPS = mkbuf(A.aps, pe), bVIS = mkbuf(A.visn, pe), bSH = mkbuf(A.sshrheat, pe);
+    const Buf bV = mkbuf(A.vel, 2ull * pn), bC = mkbuf(A.cord, 2ull * pn);
+    const double dt = ck->dt;
+    double ar[4];
+    for (int k = 0; k < 4; k++) ar[k] = bld(bA, eo, k * pe);
+    double sr[4][3];
+    {
+        // corners 1 (j,i), 2 (j+1,i), 3 (j,i+1), 4 (j+1,i+1)
+        const double vx1 = bld(bV, no, 0), vy1 = bld(bV, no, pn), vx2 = bld(bV, no, 8), vy2 = bld(bV, no, pn + 8);
+        const double vx3 = bld(bV, no, col), vy3 = bld(bV, no, pn + col), vx4 = bld(bV, no, col + 8),
+                     vy4 = bld(bV, no, pn + col + 8);
+        const double x1 = bld(bC, no, 0), y1 = bld(bC, no, pn), x2 = bld(bC, no, 8), y2 = bld(bC, no, pn + 8);
+        const double x3 = bld(bC, no, col), y3 = bld(bC, no, pn + col), x4 = bld(bC, no, col + 8),
+                     y4 = bld(bC, no, pn + col + 8);
+        // per triangle, vertices (v1,v2,v3) in the reference's order: derivatives
+        // (y2-y3, y3-y1, y1-y2)*area and (x3-x2, x1-x3, x2-x1)*area (:98-156)
+        auto tri = [&](int k, double ax, double ay, double avx, double avy, double bx, double by, double bvx,
+                       double bvy, double cx, double cy, double cvx, double cvy) {
+            const double a = ar[k];
+            const double dx0 = (by - cy) * a, dx1 = (cy - ay) * a, dx2 = (ay - by) * a;
+            const double dz0 = (cx - bx) * a, dz1 = (ax - cx) * a, dz2 = (bx - ax) * a;
+            sr[k][0] = avx * dx0 + bvx * dx1 + cvx * dx2;
+            sr[k][1] = avy * dz0 + bvy * dz1 + cvy * dz2;
+            sr[k][2] = 0.5 * (avx * dz0 + bvx * dz1 + cvx * dz2 + avy * dx0 + bvy * dx1 + cvy * dx2);
+        };
+        tri(0, x1, y1, vx1, vy1, x2, y2, vx2, vy2, x3, y3, vx3, vy3);
+        tri(1, x3, y3, vx3, vy3, x2, y2, vx2, vy2, x4, y4, vx4, vy4);
+        tri(2, x1, y1, vx1, vy1, x2, y2, vx2, vy2, x4, y4, vx4, vy4);
+        tri(3, x1, y1, vx1, vy1, x4, y4, vx4, vy4, x3, y3, vx3, vy3);
+    }
+    if (P->mix_strain == 1) {
+        for (int k = 0; k < 4; k += 2) {
+            double em = 0.5 * (sr[k][0] + sr[k][1] + sr[k + 1][0] + sr[k + 1][1]);
+            double eda = sr[k][0] - sr[k][1];
+            double edb = sr[k + 1][0] - sr[k + 1][1];
+            sr[k][0] = 0.5 * (em + eda);
+            sr[k][1] = 0.5 * (em - eda);
+            sr[k + 1][0] = 0.5 * (em + edb);
+            sr[k + 1][1] = 0.5 * (em - edb);
+        }
+    }
+    double st[4][4];
+    for (int k = 0; k < 4; k++)
+        for (int c = 0; c < 4; c++) st[k][c] = bld(bS, eo, (c + 4 * k) * pe);
+    double se1, se2, se3, sshr, s11, s22, s12;
+    {
+        double a11 = 0.25 * (0.0 + sr[0][0] + sr[1][0] + sr[2][0] + sr[3][0]);
+        double a22 = 0.25 * (0.0 + sr[0][1] + sr[1][1] + sr[2][1] + sr[3][1]);
+        double a12 = 0.25 * (0.0 + sr[0][2] + sr[1][2] + sr[2][2] + sr[3][2]);
+        double srII = 0.5 * sqrt((a11 - a22) * (a11 - a22) + 4.0 * a12 * a12);
+        double srI = 0.5 * (a11 + a22);
+        double srs2 = (a11 - srI) * (a11 - srI) + (a22 - srI) * (a22 - srI) + 2.0 * a12 * a12;
+        se1 = bld(bSE, eo, 0) + a11 * dt;
+        se2 = bld(bSE, eo, pe) + a22 * dt;
+        se3 = bld(bSE, eo, 2 * pe) + a12 * dt;
+        s11 = 0.25 * (0.0 + st[0][0] + st[1][0] + st[2][0] + st[3][0]);
+        s22 = 0.25 * (0.0 + st[0][1] + st[1][1] + st[2][1] + st[3][1]);
+        s12 = 0.25 * (0.0 + st[0][2] + st[1][2] + st[2][2] + st[3][2]);
+        double stII = 0.5 * sqrt((s11 - s22) * (s11 - s22) + 4.0 * s12 * s12);
+        sshr = bld(bSH, eo, 0);
+        if (srII != 0.0) sshr = sshr + stII / srII * srs2 * dt;
+    }
+    const int iph = A.iphase[e];
+    const GfPhase &ph = PHASE(iph);
+    const int irh = ph.irheol;
+    const double bulkm = ph.bulkm, rmu = ph.rm;
+    double stherm = 0.0;
+    if (P->istress_therm > 0) stherm = -ph.alfa * bulkm * (A.temp[IN(j, i)] - A.temp0[IN(j, i)]);
+    double aps = bld(bAPS, eo, 0);
+    const double vis = bld(bVIS, eo, 0);  // refreshed by k_visc on viscosity-update steps
+    const bool plas_law = irh == 6 || irh >= 11;
+    PlasticConst pc{};
+    double hardn = 0;
+    if (plas_law) {
+        double prv[GF_MAXACT], coh, phi, psi;
+        const Buf bPR = mkbuf(A.pr, (unsigned long long)P->nactive * pe);
+        for (int a = 0; a < P->nactive; a++) prv[a] = bld(bPR, eo, a * pe);
+        dev_pre_plast(P, aps, prv, coh, phi, psi, hardn);
+        double sphi, spsi, tphi;
+        if (phi == ph.phi0 && psi == ph.psi0) {
+            sphi = ph.sphi0;
+            spsi = ph.spsi0;
+            tphi = ph.tphi0;
+        } else {
+            cr_sin_sin_tan(phi, psi, &sphi, &spsi, &tphi);
+        }
+        const double c4d3 = 4.0 / 3.0, c2d3 = 2.0 / 3.0;
+        pc.anphi = (1.0 + sphi) / (1.0 - sphi);
+        pc.anpsi = (1.0 + spsi) / (1.0 - spsi);
+        pc.amc = 2.0 * coh * sqrt(pc.anphi);
+        pc.e1 = bulkm + c4d3 * rmu;
+        pc.e2 = bulkm - c2d3 * rmu;
+        pc.x1 = (pc.e1 - pc.e2 * pc.anpsi + pc.e1 * pc.anphi * pc.anpsi - pc.e2 * pc.anphi);
+        pc.ten_max = phi == 0.0 ? P->ten_off : fmin(P->ten_off, coh / tphi);
+    }
+    double s_start[SHEARH ? 4 : 1][4];
+    if (SHEARH)
+        for (int k = 0; k < 4; k++)
+            for (int c = 0; c < 4; c++) s_start[k][c] = st[k][c];
+    double depl[4] = {0, 0, 0, 0};
+    bool ok = true, plastic_branch = true;
+    if (irh >= 11) {
+        double s11p = 0, s12p = 0, s11v = 0, s12v = 0, s22p[4], s22v[4];
+        for (int k = 0; k < 4; k++) {
+            double o[4];
+            const double dv = bld(bD, eo, k * pe) - 0.0;
+            rheol_tri(irh, true, pc, bulkm, rmu, hardn, vis, dt, stherm, st[k], sr[k], dv, o, depl[k]);
+            s11p = k == 0 ? o[0] : s11p + o[0];
+            s12p = k == 0 ? o[2] : s12p + o[2];
+            s22p[k] = o[1];
+            double dm;
+            ok &= rheol_tri(irh, false, pc, bulkm, rmu, hardn, vis, dt, stherm, st[k], sr[k], dv, o, dm);
+            s11v = k == 0 ? o[0] : s11v + o[0];
+            s12v = k == 0 ? o[2] : s12v + o[2];
+            s22v[k] = o[1];
+        }
+        double a = s11p - s22p[0] - s22p[1] - s22p[2] - s22p[3];
+        double sII_plas = a * a + 4.0 * (s12p * s12p);
+        double av = s11v - s22v[0] - s22v[1] - s22v[2] - s22v[3];
+        double sII_visc = av * av + 4.0 * (s12v * s12v);
+        plastic_branch = sII_plas < sII_visc;
+    }
+    if (irh == 1 || irh == 3 || irh == 6 || irh >= 11) {
+        for (int k = 0; k < 4; k++) {
+            double o[4], dm;
+            const double dv = bld(bD, eo, k * pe) - 0.0;
+            bool okk = rheol_tri(irh, plastic_branch, pc, bulkm, rmu, hardn, vis, dt, stherm, st[k], sr[k], dv, o,
+                                 irh == 6 ? depl[k] : dm);
+            if (irh == 3) ok &= okk;
+            for (int c = 0; c < 4; c++) st[k][c] = o[c];
+        }
+    }
+    if (!ok) raise_error(ck, GF_STOP_MAXWELL, e);
+    if (P->mix_stress == 1) {
+        double quad_area = 1.0 / ar[0] + 1.0 / ar[1];
+        double s0a = 0.5 * (st[0][0] + st[0][1]);
+        double s0b = 0.5 * (st[1][0] + st[1][1]);
+        double s0 = (s0a / ar[1] + s0b / ar[0]) / quad_area;
+        st[0][0] = st[0][0] - s0a + s0;
+        st[0][1] = st[0][1] - s0a + s0;
+        st[1][0] = st[1][0] - s0b + s0;
+        st[1][1] = st[1][1] - s0b + s0;
+        quad_area = 1.0 / ar[2] + 1.0 / ar[3];
+        s0a = 0.5 * (st[2][0] + st[2][1]);
+        s0b = 0.5 * (st[3][0] + st[3][1]);
+        s0 = (s0a / ar[3] + s0b / ar[2]) / quad_area;
+        st[2][0] = st[2][0] - s0a + s0;
+        st[2][1] = st[2][1] - s0a + s0;
+        st[3][0] = st[3][0] - s0b + s0;
+        st[3][1] = st[3][1] - s0b + s0;
+    }
+    for (int k = 0; k < 4; k++) {
+        st[k][0] = st[k][0] - stherm;
+        st[k][1] = st[k][1] - stherm;
+        st[k][3] = st[k][3] - stherm;
+    }
+    for (int k = 0; k < 4; k++)
+        for (int c = 0; c < 4; c++) bst(bS, eo, (c + 4 * k) * pe, st[k][c]);
+    if (plas_law) {
+        double quad_area = ar[0] + ar[1] + ar[2] + ar[3];
+        aps = aps + (depl[0] * ar[0] + depl[1] * ar[1] + depl[2] * ar[2] + depl[3] * ar[3]) / quad_area;
+        if (aps < 0.0) aps = 0.0;
+        if (P->tau_heal != 0.0) aps = aps / (1.0 + dt / P->tau_heal);
+        bst(bAPS, eo, 0, aps);
+    }
+    // the reference adds the averaged (step-start) STRESS times dt: s11..s12 were reused
+    bst(bST, eo, 0, bld(bST, eo, 0) + s11 * dt);
+    bst(bST, eo, pe, bld(bST, eo, pe) + s22 * dt);
+    bst(bST, eo, 2 * pe, bld(bST, eo, 2 * pe) + s12 * dt);
+    if (SHEARH) {
+        double q_shear_elem = 0.0;
+        for (int k = 0; k < 4; k++) {
+            double lambda = bulkm - 2.0 / 3.0 * rmu, mu = rmu;
+            double denom = 3.0 * lambda + 2.0 * mu;
+            if (fabs(denom) < 1.0e-30) denom = fsign(1.0e-30, denom);
+            const double *s0 = s_start[SHEARH ? k : 0];
+            double trs = s0[0] + s0[1] + s0[3];
+            double e11s = (s0[0] - (lambda / denom) * trs) / (2.0 * mu);
+            double e22s = (s0[1] - (lambda / denom) * trs) / (2.0 * mu);
+            double e33s = (s0[3] - (lambda / denom) * trs) / (2.0 * mu);
+            double e12s = s0[2] / (2.0 * mu);
+            double trf = st[k][0] + st[k][1] + st[k][3];
+            double e11f = (st[k][0] - (lambda / denom) * trf) / (2.0 * mu);
+            double e22f = (st[k][1] - (lambda / denom) * trf) / (2.0 * mu);
+            double e33f = (st[k][3] - (lambda / denom) * trf) / (2.0 * mu);
+            double e12f = st[k][2] / (2.0 * mu);
+            double d11n = sr[k][0] * dt - (e11f - e11s);
+            double d22n = sr[k][1] * dt - (e22f - e22s);
+            double d12n = sr[k][2] * dt - (e12f - e12s);
+            double d33n = 0.0 - (e33f - e33s);
+            double a11 = 0.5 * (s0[0] + st[k][0]);
+            double a22 = 0.5 * (s0[1] + st[k][1]);
+            double a12 = 0.5 * (s0[2] + st[k][2]);
+            double a33 = 0.5 * (s0[3] + st[k][3]);
+            double q = a11 * d11n + a22 * d22n + 2.0 * a12 * d12n + a33 * d33n;
+            q_shear_elem = q_shear_elem + 0.25 * q;
+        }
+        sshr = sshr + q_shear_elem;
+    }
+    // periodic refresh of the time-averaged strain rate, src/fl_rheol.f90:395-421
+    if (avgsr != 0) {
+        double dtavg = ck->dtavg;
+        double a = se1 - se2;
+        A.e2sr[e] = (0.5 * sqrt(a * a + 4.0 * se3 * se3)) / dtavg;
+        A.shrheat[e] = sshr / dtavg;
+        if (avgsr == 1) {
+            se1 = 0.0; se2 = 0.0; se3 = 0.0; sshr = 0.0;
+        }
+    }
+    bst(bSE, eo, 0, se1);
+    bst(bSE, eo, pe, se2);
+    bst(bSE, eo, 2 * pe, se3);
+    bst(bSH, eo, 0, sshr);
+}
+
+template <bool SHEARH>
+__global__ void __launch_bounds__(256) k_rheol1(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
+                                                int avgsr)
+{
+    long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= NE) return;
+    rheol_single<SHEARH>(P, ck, A, e, avgsr);
 }
 
 // Eff_visc of src/fl_rheol.f90:203-206 on viscosity-update steps (mod(nloop,

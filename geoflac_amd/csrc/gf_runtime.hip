@@ -37,6 +37,7 @@ struct gf_ctx {
     std::vector<double> phase_ratio_host;  // full nphase layout, as uploaded
     int err_j = 0, err_i = 0;
     char msg[512] = {0};
+    bool rheol_quad = false;  // four lanes per element (GF_RHEOL=quad) instead of one
     // per-kernel event timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -112,6 +113,26 @@ static int build_dev_params(gf_ctx *ctx)
             ph.pln_r = p.pln[k] * 8.31448;                      // src/matprops.f90:160,195
         }
         if (p.den[k] != 0.0 && p.cp[k] != 0.0) ph.diff = p.conduct[k] / p.den[k] / p.cp[k];
+        // pre_plast (src/rh_plastic.f90:249-284) of a one-phase element at aps = 0
+        {
+            const double aps0 = 0.0;
+            double f, dd;
+            if (aps0 < p.plstrain1[k]) {
+                f = p.fric1[k]; dd = p.dilat1[k];
+            } else if (aps0 < p.plstrain2[k]) {
+                double dpl = (aps0 - p.plstrain1[k]) / (p.plstrain2[k] - p.plstrain1[k]);
+                f = p.fric1[k] + (p.fric2[k] - p.fric1[k]) * dpl;
+                dd = p.dilat1[k] + (p.dilat2[k] - p.dilat1[k]) * dpl;
+            } else {
+                f = p.fric2[k]; dd = p.dilat2[k];
+            }
+            ph.phi0 = 1 / (0 + 1.0 / f);
+            ph.psi0 = 0 + 1.0 * dd;
+            const double degrad = 3.14159265358979323846 / 180.0;
+            ph.sphi0 = cr_sin(ph.phi0 * degrad);
+            ph.spsi0 = cr_sin(ph.psi0 * degrad);
+            ph.tphi0 = ph.phi0 != 0.0 ? cr_tan(ph.phi0 * degrad) : 0.0;
+        }
         ph.dt_m = 1.0e28;
         if ((p.irheol[k] == 3 || p.irheol[k] == 12) && p.rm[k] < 1.0e11) {  // src/dt_mass.f90:138-146
             const double visc_cut = 1.0e10;
@@ -148,6 +169,10 @@ extern "C" int gf_create(const gf_params *p, int32_t precision, gf_ctx **out)
     ctx->nn = (long)p->nz * p->nx;
     ctx->hp.nactive = 0;
     build_dev_params(ctx);
+    {
+        const char *rv = getenv("GF_RHEOL");
+        ctx->rheol_quad = rv && strcmp(rv, "quad") == 0;
+    }
     const long ne = ctx->ne, nn = ctx->nn;
     int rc = GF_OK;
 #define ALLOC(ptr, bytes)                                                                    \
@@ -522,7 +547,12 @@ static int enq_rheol(gf_ctx *ctx)
         avgsr = 2;
     if ((rc = enq_clock(ctx, CK_DTAVG_ADD))) return rc;
     if (do_visc) LAUNCH(k_visc, ctx->ne, ctx->dp, ctx->dck, ctx->a);
-    LAUNCH(k_rheol, 4 * ctx->ne, ctx->dp, ctx->dck, ctx->a, avgsr);
+    if (ctx->rheol_quad)
+        LAUNCH(k_rheol, 4 * ctx->ne, ctx->dp, ctx->dck, ctx->a, avgsr);
+    else if (ctx->p.ishearh == 1)
+        LAUNCH(k_rheol1<true>, ctx->ne, ctx->dp, ctx->dck, ctx->a, avgsr);
+    else
+        LAUNCH(k_rheol1<false>, ctx->ne, ctx->dp, ctx->dck, ctx->a, avgsr);
     if (avgsr == 1) {
         if ((rc = enq_clock(ctx, CK_DTAVG_RESET))) return rc;
         ctx->nsrate = 0;
