@@ -36,7 +36,7 @@ def alg_bytes(kernel, E, N, nact, p):
         return E * per_e + N * per_n
     if kernel == "k_therm_tile":
         return N * (8 + 8 + 8 + 16) + E * (16 + 4 + 8)                # temp, temp0 out, x out, cord | area(1:2) iphase source
-    if kernel == "k_therm_verify":
+    if kernel == "k_therm_tile_extra":
         return N * (8 + 8 + 8 + 16) + E * (16 + 4 + 8)                # temp0, x in, x out, cord | area(1:2) iphase source
     if kernel == "k_node":
         return E * (96 + 32) + N * (16 + 16 + 16 + 8 + 16 + 16 + 16)  # stress0(1:3) area | cord rmass+amass vel ncod bc, vel+force out
@@ -88,7 +88,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="also run a 1 GiB device copy (torch) to calibrate HBM byte counters")
+                    help="also run a streaming kernel over 1 GiB (read 1 GiB, write 1 GiB; torch.mul) "
+                         "to calibrate the HBM byte counters of a profiler")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (one model per device); "
@@ -113,12 +114,15 @@ def main():
 
     # profiled run: HIP events around every launch on the library stream
     args.profile_steps = max(args.profile_steps, 1)
+    m.therm_stats()
     m.set_timing(True)
     m.step(args.profile_steps)
-    kernels = ["k_rheol1<false>", "k_rheol1<true>", "k_rheol", "k_visc", "k_therm_tile", "k_therm_verify", "k_node", "k_move_elem", "k_move_cord",
-               "k_therm_bc", "k_dens", "k_rmass", "k_dt_reduce", "k_amass", "k_clock"]
+    kernels = ["k_rheol1<false>", "k_rheol1<true>", "k_rheol", "k_visc", "k_therm_tile", "k_therm_tile_extra",
+               "k_therm_check", "k_node", "k_move_elem", "k_move_cord", "k_therm_bc", "k_dens", "k_rmass",
+               "k_dt_reduce", "k_amass", "k_clock"]
     times = {k: m.kernel_time(k) for k in kernels}
     m.set_timing(False)
+    th = m.therm_stats()
     dom = max(times, key=lambda k: times[k][0])
     ms_tot, nl = times[dom]
     nact = int((host_state.phase_ratio >= 0.01).any(axis=(1, 2)).sum())
@@ -128,14 +132,16 @@ def main():
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "alg_bytes_per_launch": ab, "avg_launch_us": round(avg_s * 1e6, 2),
-                "kernel_ms_per_step": {k: round(v[0] / args.profile_steps, 4) for k, v in times.items() if v[1]}}
+                "kernel_ms_per_step": {k: round(v[0] / args.profile_steps, 4) for k, v in times.items() if v[1]},
+                "therm_extra_passes_per_step": th[1] / args.profile_steps,
+                "therm_border_mismatches": th[2], "therm_mismatch_max_ulp": th[3]}
     m.close()
     if args.copy_calibration:
         import torch
-        x = torch.empty(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
         y = torch.empty_like(x)
         for _ in range(3):
-            y.copy_(x)
+            torch.mul(x, 1.0, out=y)
         torch.cuda.synchronize()
 
     out = {"metric": METRIC, "value": value, "unit": "element-steps/s", "n_gpus": 1,

@@ -55,8 +55,8 @@ struct GfDevClock {
     unsigned long long dlmin_bits, velmax_bits, dtmax_therm_bits, dtmaxwell_bits, dte_bits;
     int error_code, pad;
     long long error_index;      // smallest linear index (reference loop order) that failed
-    unsigned long long therm_unconverged;
-    unsigned int therm_pass_changed[GF_MAX_SWEEPS];  // bit 0: first sweep changed a bit; bit 1: a tile did not settle
+    unsigned long long therm_unconverged, therm_extra_passes, therm_mismatch, therm_mismatch_ulp;
+    unsigned int therm_pass_changed[GF_MAX_SWEEPS];  // bit 0: a tile border differs from its owner; bit 1: a tile did not settle
 };
 
 // ---------------------------------------------------------------- buffer access

@@ -82,9 +82,16 @@ __global__ void k_math_check(const double *x, const double *y, double *out, long
 // TH_HI columns to the left (the coupling to the previous column is ~(dz/dx)^2
 // weaker); the per-node coefficients of the six triangles are formed once, in the
 // reference's order, and kept in registers.  A tile iterates until it is still.
-// Pass 0 starts from the old temperature; each verification pass starts from the
-// previous pass's global result and records whether its first sweep changed any
-// bit anywhere; the first still pass proves the result is the serial one.
+// Proof without a second full pass: when a tile is still, every node of its output
+// region is a fixed point of the map evaluated with the tile's own values of its
+// neighbours.  The output nodes' neighbours outside the output region (the row above,
+// the column to the left and the row below it, 2*TH_OI + TH_OJ nodes) are halo nodes
+// whose tile values are saved to a border buffer; k_therm_check compares them with
+// the values their owning tiles wrote.  If all are bit-equal, the assembled global
+// iterate is a fixed point of the global map at every node, hence the serial result.
+// Otherwise another pass starts from the assembled iterate (halos now near exact).
+// Tiles are numbered so that each XCD runs a contiguous range of column-major tiles,
+// which keeps the halo re-reads of neighbouring tiles in the same L2.
 #define TH_J 64
 #define TH_I 16
 #define TH_HJ 8
@@ -95,6 +102,32 @@ __global__ void k_math_check(const double *x, const double *y, double *out, long
 #define TH_LI (TH_I + 2)          // LDS columns incl. a ring left and right
 #define TH_MAXPASS 4
 #define TH_MAXSWEEP 48
+#define TH_NB (2 * TH_OI + TH_OJ)  // border entries per tile
+
+// logical tile of workgroup `wg` when consecutive workgroups go to the 8 XCDs in turn:
+// XCD x runs tiles [x*q, (x+1)*q); -1 for the padding workgroups of the last range
+GF_DEV int xcd_tile(int wg, int ntiles)
+{
+    const int q = (ntiles + 7) / 8;
+    const int t = (wg % 8) * q + wg / 8;
+    return t < ntiles ? t : -1;
+}
+GF_DEV int therm_ntj(const GfDevParams *P) { return (P->nz + TH_OJ - 1) / TH_OJ; }
+GF_DEV int therm_nti(const GfDevParams *P) { return (P->nx + TH_OI - 1) / TH_OI; }
+// tile-local (lj, li) of border entry k: the row above the output region (TH_OI),
+// the row below it up to the last column but one (TH_OI - 1, left-down neighbours of
+// the last output row), the column to its left from the first output row to the row
+// below the region (TH_OJ + 1)
+GF_DEV void therm_border_pos(int k, int &lj, int &li)
+{
+    if (k < TH_OI) {
+        lj = TH_HJ - 1; li = TH_HI + k;
+    } else if (k < 2 * TH_OI - 1) {
+        lj = TH_J - TH_HJ; li = TH_HI + (k - TH_OI);
+    } else {
+        lj = TH_HJ + (k - (2 * TH_OI - 1)); li = TH_HI - 1;
+    }
+}
 
 struct ThermCoef {
     // rhs = rhs + P_t - Q_t * S_t for the triangles t present, in the reference order:
@@ -198,14 +231,18 @@ GF_DEV double therm_eval(const ThermCoef &c, double xu, double xl, double xd, do
 // pass == 0: old temperature from A.temp (also copied to temp0 = src/fl_therm.f90:52);
 // pass > 0: old temperature from A.temp0 and the starting iterate from xin.
 __global__ void __launch_bounds__(1024) k_therm_tile(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
-                                                     const double *__restrict__ xin, double *__restrict__ xout, int pass)
+                                                     const double *__restrict__ xin, double *__restrict__ xout,
+                                                     double *__restrict__ bord, int pass)
 {
     if (pass > 0 && ck->therm_pass_changed[pass - 1] == 0u) return;  // already proven
+    const int ntj = therm_ntj(P);
+    const int tile = xcd_tile(blockIdx.x, ntj * therm_nti(P));
+    if (tile < 0) return;
     __shared__ double sT[TH_LI * TH_LJ];
     __shared__ double sX[2][TH_LI * TH_LJ];
     const int tj = threadIdx.x % TH_J, ti = threadIdx.x / TH_J;
-    const int j0 = blockIdx.x * TH_OJ + 1 - TH_HJ;  // 1-based row of tile row 0
-    const int i0 = blockIdx.y * TH_OI + 1 - TH_HI;  // 1-based column of tile column 0
+    const int j0 = (tile % ntj) * TH_OJ + 1 - TH_HJ;  // 1-based row of tile row 0
+    const int i0 = (tile / ntj) * TH_OI + 1 - TH_HI;  // 1-based column of tile column 0
     const double *told_g = pass == 0 ? A.temp : A.temp0;
     const double *x0_g = pass == 0 ? A.temp : xin;
 #define LIDX(lj, li) (((li) + 1) * TH_LJ + ((lj) + 1))
@@ -233,7 +270,7 @@ __global__ void __launch_bounds__(1024) k_therm_tile(const GfDevParams *__restri
     }
     const double dt = ck->dt;
     const int me = LIDX(tj, ti), up = LIDX(tj - 1, ti), lf = LIDX(tj, ti - 1), dl = LIDX(tj + 1, ti - 1);
-    int cur = 0, first_changed = 0, nsweep = 0;
+    int cur = 0, nsweep = 0;
     bool still = false;
     while (nsweep < TH_MAXSWEEP) {
         bool changed = false;
@@ -244,7 +281,6 @@ __global__ void __launch_bounds__(1024) k_therm_tile(const GfDevParams *__restri
             sX[cur ^ 1][me] = tn;
         }
         int any = __syncthreads_or(changed);
-        if (nsweep == 0) first_changed = any;
         cur ^= 1;
         nsweep++;
         if (!any) {
@@ -256,11 +292,58 @@ __global__ void __launch_bounds__(1024) k_therm_tile(const GfDevParams *__restri
         xout[IN(j, i)] = sX[cur][me];
         if (pass == 0) A.temp0[IN(j, i)] = c.told;
     }
-    if (threadIdx.x == 0) {
-        if (first_changed) atomicOr(&ck->therm_pass_changed[pass], 1u);
-        if (!still) atomicOr(&ck->therm_pass_changed[pass], 2u);
+    if (threadIdx.x < TH_NB) {  // save the tile's values of the output region's outer neighbours
+        int lj, li;
+        therm_border_pos(threadIdx.x, lj, li);
+        bord[(long)tile * TH_NB + threadIdx.x] = sX[cur][LIDX(lj, li)];
     }
+    if (threadIdx.x == 0 && !still) atomicOr(&ck->therm_pass_changed[pass], 2u);
 #undef LIDX
+}
+
+// Compares each tile's border values with the global iterate written by the owning
+// tiles.  A border node whose tile value differs is harmless when every output node
+// that reads it still satisfies x = F(x) with the global values: those (at most three)
+// nodes are re-evaluated here.  This matters on the top row, whose pre-boundary-
+// condition temperature is close to zero, so that differences far below the other
+// rows' rounding still show up as a few ulp there.  Any remaining difference (or a
+// tile that did not settle) sends the next pass.
+__global__ void __launch_bounds__(256) k_therm_check(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
+                                                     const double *__restrict__ x, const double *__restrict__ bord,
+                                                     int pass)
+{
+    if (pass > 0 && ck->therm_pass_changed[pass - 1] == 0u) return;
+    const int ntj = therm_ntj(P);
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)ntj * therm_nti(P) * TH_NB) return;
+    const int tile = (int)(t / TH_NB), k = (int)(t % TH_NB);
+    int lj, li;
+    therm_border_pos(k, lj, li);
+    const int j0 = (tile % ntj) * TH_OJ + 1 - TH_HJ, i0 = (tile / ntj) * TH_OI + 1 - TH_HI;
+    const int j = j0 + lj, i = i0 + li;
+    if (j < 1 || j > NZ || i < 1 || i > NX) return;
+    const long long a = __double_as_longlong(bord[t]), b = __double_as_longlong(x[IN(j, i)]);
+    if (a == b) return;
+    atomicAdd(&ck->therm_mismatch, 1ull);
+    atomicMax(&ck->therm_mismatch_ulp, (unsigned long long)(a > b ? a - b : b - a));
+    // output nodes of this tile that read (j,i): as up (j+1,i), left (j,i+1), left-down (j-1,i+1)
+    const int dj[3] = {1, 0, -1}, di[3] = {0, 1, 1};
+    const double dt = ck->dt;
+    auto T = [&](int jj, int ii) { return A.temp0[IN(jj, ii)]; };
+    for (int q = 0; q < 3; q++) {
+        const int mj = j + dj[q], mi = i + di[q];
+        const int mlj = mj - j0, mli = mi - i0;
+        if (mlj < TH_HJ || mlj >= TH_J - TH_HJ || mli < TH_HI || mli >= TH_I) continue;
+        if (mj < 1 || mj > NZ || mi < 1 || mi > NX) continue;
+        ThermCoef c;
+        therm_coef(P, A, mj, mi, T, c);
+        const double xu = mj > 1 ? x[IN(mj - 1, mi)] : 0.0;
+        const double xl = mi > 1 ? x[IN(mj, mi - 1)] : 0.0;
+        const double xd = (mi > 1 && mj < NZ) ? x[IN(mj + 1, mi - 1)] : 0.0;
+        const double f = therm_eval(c, xu, xl, xd, dt);
+        if (__double_as_longlong(f) != __double_as_longlong(x[IN(mj, mi)]))
+            atomicOr(&ck->therm_pass_changed[pass], 1u);
+    }
 }
 
 // Boundary conditions src/fl_therm.f90:575-598 applied to the proven iterate.
@@ -268,13 +351,14 @@ __global__ void __launch_bounds__(256) k_therm_bc(const GfDevParams *__restrict_
                                                   const double *__restrict__ xa, const double *__restrict__ xb)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    // pass p writes xa when p is even, xb when odd; the first pass whose first sweep
-    // changed nothing holds the fixed point (as does its input)
+    // pass p writes xa when p is even, xb when odd; the first pass whose border check
+    // found no difference holds the fixed point
     int proven = -1;
     for (int k = 0; k < TH_MAXPASS; k++)
         if (ck->therm_pass_changed[k] == 0u) { proven = k; break; }
     const double *x = (proven < 0 ? (TH_MAXPASS - 1) : proven) % 2 == 0 ? xa : xb;
     if (t == 0 && proven < 0) atomicAdd(&ck->therm_unconverged, 1ull);
+    if (t == 0 && proven > 0) atomicAdd(&ck->therm_extra_passes, (unsigned long long)proven);
     if (t >= NN) return;
     const int j = (int)(t % NZ) + 1, i = (int)(t / NZ) + 1;
     // value of node (jj, ii) after the top/bottom conditions (:575-588)

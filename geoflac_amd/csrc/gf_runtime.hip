@@ -27,6 +27,7 @@ struct gf_ctx {
     GfDevClock *dck = nullptr;
     GfDevArrays a{};
     int *d_present = nullptr;  // phase ids present in iphase (for dt_mass minima)
+    double *therm_border = nullptr;  // per thermal tile, its values of the output region's outer neighbours
     int npresent = 0;
     hipStream_t stream = nullptr;
     gf_host_arrays host{};
@@ -215,6 +216,7 @@ extern "C" int gf_create(const gf_params *p, int32_t precision, gf_ctx **out)
     ALLOC(ctx->a.pr, 8 * ne * GF_MAXACT);
     ALLOC(ctx->a.xa, 8 * nn);
     ALLOC(ctx->a.xb, 8 * nn);
+    ALLOC(ctx->therm_border, 8 * (size_t)TH_NB * ((p->nz + TH_OJ - 1) / TH_OJ) * ((p->nx + TH_OI - 1) / TH_OI));
     ALLOC(ctx->a.dens, 8 * ne);
 #undef ALLOC
     {
@@ -242,7 +244,7 @@ extern "C" void gf_destroy(gf_ctx *ctx)
                     ctx->a.force, ctx->a.amass, ctx->a.rmass, ctx->a.bc, ctx->a.ncod, ctx->a.stress0,
                     ctx->a.area, ctx->a.dvol, ctx->a.strain, ctx->a.se2sr, ctx->a.aps, ctx->a.visn,
                     ctx->a.e2sr, ctx->a.sshrheat, ctx->a.shrheat, ctx->a.source, ctx->a.iphase, ctx->a.pr,
-                    ctx->a.xa, ctx->a.xb, ctx->a.dens};
+                    ctx->a.xa, ctx->a.xb, ctx->a.dens, ctx->therm_border};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -514,7 +516,8 @@ static int enq_therm(gf_ctx *ctx)
 {
     int rc;
     if ((rc = enq_clock(ctx, CK_THERM_BEGIN))) return rc;
-    const dim3 grid((unsigned)((ctx->p.nz + TH_OJ - 1) / TH_OJ), (unsigned)((ctx->p.nx + TH_OI - 1) / TH_OI));
+    const int ntiles = ((ctx->p.nz + TH_OJ - 1) / TH_OJ) * ((ctx->p.nx + TH_OI - 1) / TH_OI);
+    const dim3 grid((unsigned)(8 * ((ntiles + 7) / 8)));
     for (int pass = 0; pass < TH_MAXPASS; pass++) {
         const double *in = (pass % 2 == 1) ? ctx->a.xa : ctx->a.xb;
         double *out = (pass % 2 == 0) ? ctx->a.xa : ctx->a.xb;
@@ -525,12 +528,13 @@ static int enq_therm(gf_ctx *ctx)
             if (e0) (void)hipEventRecord(e0, ctx->stream);
         }
         hipLaunchKernelGGL(k_therm_tile, grid, dim3(TH_J * TH_I), 0, ctx->stream, ctx->dp, ctx->dck, ctx->a, in, out,
-                           pass);
+                           ctx->therm_border, pass);
         if ((rc = launch_check(ctx, "k_therm_tile"))) return rc;
         if (ctx->timing && e0 && e1) {
             (void)hipEventRecord(e1, ctx->stream);
-            ctx->pending.push_back({pass == 0 ? "k_therm_tile" : "k_therm_verify", {e0, e1}});
+            ctx->pending.push_back({pass == 0 ? "k_therm_tile" : "k_therm_tile_extra", {e0, e1}});
         }
+        LAUNCH(k_therm_check, (long)ntiles * TH_NB, ctx->dp, ctx->dck, ctx->a, out, ctx->therm_border, pass);
     }
     LAUNCH(k_therm_bc, ctx->nn, ctx->dp, ctx->dck, ctx->a, ctx->a.xa, ctx->a.xb);
     return GF_OK;
@@ -683,15 +687,19 @@ extern "C" int gf_init_visc(gf_ctx *ctx)
     return fetch_clock(ctx, nullptr);
 }
 
-extern "C" int gf_therm_unconverged(gf_ctx *ctx, int64_t *count)
+
+extern "C" int gf_therm_stats(gf_ctx *ctx, int64_t *stats)
 {
-    if (!ctx || !count) return GF_ERR_ARG;
+    if (!ctx || !stats) return GF_ERR_ARG;
     GfDevClock ck;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipMemcpy(&ck, ctx->dck, sizeof(ck), hipMemcpyDeviceToHost));
-    *count = ck.therm_unconverged;
-    unsigned long long z = 0;
-    HIPCHK(hipMemcpy(&ctx->dck->therm_unconverged, &z, sizeof(z), hipMemcpyHostToDevice));
+    stats[0] = (int64_t)ck.therm_unconverged;
+    stats[1] = (int64_t)ck.therm_extra_passes;
+    stats[2] = (int64_t)ck.therm_mismatch;
+    stats[3] = (int64_t)ck.therm_mismatch_ulp;
+    unsigned long long z[4] = {0, 0, 0, 0};
+    HIPCHK(hipMemcpy(&ctx->dck->therm_unconverged, z, sizeof(z), hipMemcpyHostToDevice));
     return GF_OK;
 }
 

