@@ -33,6 +33,15 @@ struct GfDevArrays {
 #define PHASE(iph) (P->ph[(iph) - 1])
 
 // ------------------------------------------------------------------ utilities
+// Consecutive workgroups of a launch go to the 8 XCDs in turn, each with its own L2.
+// Launches of the stencil kernels round their grid up to a multiple of 8 and renumber
+// workgroups so that XCD x handles the x-th contiguous eighth of the index range:
+// neighbouring columns, which read each other's nodes or elements, then share an L2.
+GF_DEV long xcd_lin()
+{
+    const unsigned q = gridDim.x / 8;
+    return (long)((blockIdx.x % 8) * q + blockIdx.x / 8) * blockDim.x + threadIdx.x;
+}
 __global__ void k_copy(double *__restrict__ dst, const double *__restrict__ src, long n)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -94,7 +103,7 @@ __global__ void k_math_check(const double *x, const double *y, double *out, long
 // which keeps the halo re-reads of neighbouring tiles in the same L2.
 #define TH_J 64
 #define TH_I 16
-#define TH_HJ 8
+#define TH_HJ 6
 #define TH_HI 4
 #define TH_OJ (TH_J - 2 * TH_HJ)  // output rows per tile
 #define TH_OI (TH_I - TH_HI)      // output columns per tile
@@ -215,6 +224,13 @@ GF_DEV void therm_coef(const GfDevParams *P, const GfDevArrays &A, int j, int i,
     }
 }
 
+// Element table of the tile kernel: elements (ej, ei) of tile rows -1 .. TH_J-1 and
+// columns -1 .. TH_I-1 (the elements around the tile's nodes).
+#define TH_EJ (TH_J + 1)             // element rows of the table (tile rows -1 .. TH_J-1)
+#define TH_NE ((TH_J + 1) * (TH_I + 1))
+#define TH_NF 6                      // values per element and round
+#define EIDX(ej, ei) (((ei) + 1) * TH_EJ + ((ej) + 1))
+
 // one evaluation of the serial update at a node; xu = X(j-1,i), xl = X(j,i-1), xd = X(j+1,i-1)
 GF_DEV double therm_eval(const ThermCoef &c, double xu, double xl, double xd, double dt)
 {
@@ -230,44 +246,175 @@ GF_DEV double therm_eval(const ThermCoef &c, double xu, double xl, double xd, do
 
 // pass == 0: old temperature from A.temp (also copied to temp0 = src/fl_therm.f90:52);
 // pass > 0: old temperature from A.temp0 and the starting iterate from xin.
+// A tile's global reads (its node footprint: temperature and coordinates; its 65 x 17
+// elements: areas, phase, source) are issued together into LDS, so that each tile pays
+// the memory latency once; everything after that reads LDS.
 __global__ void __launch_bounds__(1024) k_therm_tile(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
                                                      const double *__restrict__ xin, double *__restrict__ xout,
                                                      double *__restrict__ bord, int pass)
 {
     if (pass > 0 && ck->therm_pass_changed[pass - 1] == 0u) return;  // already proven
     const int ntj = therm_ntj(P);
-    const int tile = xcd_tile(blockIdx.x, ntj * therm_nti(P));
-    if (tile < 0) return;
-    __shared__ double sT[TH_LI * TH_LJ];
-    __shared__ double sX[2][TH_LI * TH_LJ];
+    const int ntiles = ntj * therm_nti(P);
+    const int nwg = 8 * ((ntiles + 7) / 8);
+    constexpr int NL = TH_LI * TH_LJ;
+    __shared__ double sT[NL], sCx[NL], sCy[NL];     // old temperature, coordinates
+    __shared__ double sAr[2][TH_NE];                // area of triangles A and B
+    __shared__ double sDm[TH_NE], sDf[TH_NE];       // dummye and diffusivity per element
+    __shared__ double sU[TH_NF * TH_NE];            // element table by rounds, then the two iterates
+    __shared__ double sPh[3][GF_MAXPH];             // den, cp, diff per phase
+    double(*sX)[NL] = reinterpret_cast<double(*)[NL]>(sU);
+    for (int q = threadIdx.x; q < GF_MAXPH; q += blockDim.x) {
+        sPh[0][q] = P->ph[q].den;
+        sPh[1][q] = P->ph[q].cp;
+        sPh[2][q] = P->ph[q].diff;
+    }
+    const bool with_dissip = P->ishearh == 1 && P->itherm != 2;
+    // pass 0 runs one workgroup per tile; later passes, which are rarely needed, run a
+    // smaller grid that loops (so that a pass that is not needed costs little)
+    for (int wg = blockIdx.x; wg < nwg; wg += gridDim.x) {
+    const int tile = xcd_tile(wg, ntiles);
+    if (tile < 0) continue;
+    __syncthreads();  // the previous tile's LDS reads are done (and the phase table is in)
     const int tj = threadIdx.x % TH_J, ti = threadIdx.x / TH_J;
     const int j0 = (tile % ntj) * TH_OJ + 1 - TH_HJ;  // 1-based row of tile row 0
     const int i0 = (tile / ntj) * TH_OI + 1 - TH_HI;  // 1-based column of tile column 0
     const double *told_g = pass == 0 ? A.temp : A.temp0;
-    const double *x0_g = pass == 0 ? A.temp : xin;
 #define LIDX(lj, li) (((li) + 1) * TH_LJ + ((lj) + 1))
-    // load old temperature and starting iterate for the tile plus its ring
-    for (int q = threadIdx.x; q < TH_LI * TH_LJ; q += blockDim.x) {
-        int lj = q % TH_LJ - 1, li = q / TH_LJ - 1;
-        int j = j0 + lj, i = i0 + li;
-        double tv = 0.0, xv = 0.0;
-        if (j >= 1 && j <= NZ && i >= 1 && i <= NX) {
-            tv = told_g[IN(j, i)];
-            xv = x0_g[IN(j, i)];
-        }
-        sT[q] = tv;
-        sX[0][q] = xv;
-        sX[1][q] = xv;
+    for (int q = threadIdx.x; q < NL; q += blockDim.x) {
+        const int j = j0 + q % TH_LJ - 1, i = i0 + q / TH_LJ - 1;
+        const bool in = j >= 1 && j <= NZ && i >= 1 && i <= NX;
+        sT[q] = in ? told_g[IN(j, i)] : 0.0;
+        sCx[q] = in ? CORD(j, i, 1) : 0.0;
+        sCy[q] = in ? CORD(j, i, 2) : 0.0;
     }
+    const int e0 = threadIdx.x, e1 = threadIdx.x + TH_J * TH_I;  // this thread's table elements
+    auto elem_ok = [&](int e, int &je, int &ie) {
+        je = j0 + e % TH_EJ - 1;
+        ie = i0 + e / TH_EJ - 1;
+        return e < TH_NE && je >= 1 && je <= NZ - 1 && ie >= 1 && ie <= NX - 1;
+    };
+    auto elem_load = [&](int e) {
+        int je, ie;
+        if (!elem_ok(e, je, ie)) return;
+        const long ix = IE(je, ie);
+        const int ph = A.iphase[ix] - 1;
+        const double dissip = with_dissip ? A.shrheat[ix] : 0.0;
+        sAr[0][e] = A.area[ix];
+        sAr[1][e] = A.area[ix + NE];
+        sDm[e] = (A.source[ix] + dissip / sPh[0][ph]) / sPh[1][ph];  // dummye, src/fl_therm.f90:405-414
+        sDf[e] = sPh[2][ph];
+    };
+    elem_load(e0);
+    elem_load(e1);
     __syncthreads();
     const int j = j0 + tj, i = i0 + ti;
     const bool valid = j >= 1 && j <= NZ && i >= 1 && i <= NX;
     const bool is_out = valid && tj >= TH_HJ && tj < TH_J - TH_HJ && ti >= TH_HI;
+    // Node coefficients, as therm_coef forms them, from per-element data built once per
+    // element in three rounds through the table sU: round 0 ra13, P = dummye*ra13 and
+    // Q = 0.5*diff*inv_area of triangles A and B; round 1 the symmetric products
+    // K(a,b) of A; round 2 those of B (a product commutes exactly, so K(a,b) and K(b,a)
+    // are the same bits).
+    auto elem_store = [&](int e, int round) {
+        int je, ie;
+        if (!elem_ok(e, je, ie)) return;
+        if (round == 0) {
+            for (int tri = 0; tri < 2; tri++) {
+                const double inv_area = 1.0 / sAr[tri][e];
+                const double ra13 = inv_area / 6.0;
+                sU[(3 * tri + 0) * TH_NE + e] = ra13;
+                sU[(3 * tri + 1) * TH_NE + e] = sDm[e] * ra13;
+                sU[(3 * tri + 2) * TH_NE + e] = 0.5 * sDf[e] * inv_area;
+            }
+            return;
+        }
+        const double ar = sAr[round - 1][e];
+        const int ej = e % TH_EJ - 1, ei = e / TH_EJ - 1;
+        double d[6];
+        if (round == 1) {  // A: (ej,ei), (ej+1,ei), (ej,ei+1)
+            const int n1 = LIDX(ej, ei), n2 = LIDX(ej + 1, ei), n3 = LIDX(ej, ei + 1);
+            const double y1 = sCy[n1], y2 = sCy[n2], y3 = sCy[n3], x1 = sCx[n1], x2 = sCx[n2], x3 = sCx[n3];
+            d[0] = (y2 - y3) * ar; d[1] = (y3 - y1) * ar; d[2] = (y1 - y2) * ar;
+            d[3] = (x3 - x2) * ar; d[4] = (x1 - x3) * ar; d[5] = (x2 - x1) * ar;
+        } else {           // B: (ej+1,ei), (ej,ei+1), (ej+1,ei+1)
+            const int n2 = LIDX(ej + 1, ei), n3 = LIDX(ej, ei + 1), n4 = LIDX(ej + 1, ei + 1);
+            const double y2 = sCy[n2], y3 = sCy[n3], y4 = sCy[n4], x2 = sCx[n2], x3 = sCx[n3], x4 = sCx[n4];
+            d[0] = (y2 - y4) * ar; d[1] = (y4 - y3) * ar; d[2] = (y3 - y2) * ar;
+            d[3] = (x4 - x2) * ar; d[4] = (x3 - x4) * ar; d[5] = (x2 - x3) * ar;
+        }
+        sU[0 * TH_NE + e] = KAB(d, 1, 1);
+        sU[1 * TH_NE + e] = KAB(d, 1, 2);
+        sU[2 * TH_NE + e] = KAB(d, 1, 3);
+        sU[3 * TH_NE + e] = KAB(d, 2, 2);
+        sU[4 * TH_NE + e] = KAB(d, 2, 3);
+        sU[5 * TH_NE + e] = KAB(d, 3, 3);
+    };
     ThermCoef c;
+    c.mask = 0;
     if (valid) {
-        auto T = [&](int jj, int ii) { return sT[LIDX(jj - j0, ii - i0)]; };
-        therm_coef(P, A, j, i, T, c);
+        if (j != 1 && i != 1) c.mask |= 1;
+        if (j != 1 && i != NX) c.mask |= 2 | 4;
+        if (j != NZ && i != 1) c.mask |= 8 | 16;
+        if (j != NZ && i != NX) c.mask |= 32;
     }
+    c.area_n = 0.0;
+    // the element and triangle (0 A, 1 B) of each of the node's six triangles, in order
+    const int te[6] = {EIDX(tj - 1, ti - 1), EIDX(tj - 1, ti), EIDX(tj - 1, ti), EIDX(tj, ti - 1), EIDX(tj, ti - 1),
+                       EIDX(tj, ti)};
+    for (int round = 0; round < 3; round++) {
+        elem_store(e0, round);
+        elem_store(e1, round);
+        __syncthreads();
+        if (valid) {
+            auto T = [&](int jj, int ii) { return sT[LIDX(jj - j0, ii - i0)]; };
+            auto F = [&](int t, int f) { return sU[f * TH_NE + te[t]]; };
+            const double tme = T(j, i);
+            if (round == 0) {
+#pragma unroll
+                for (int t = 0; t < 6; t++) {
+                    if (!(c.mask & (1 << t))) continue;
+                    const int tri = (t == 0 || t == 2 || t == 4) ? 1 : 0;
+                    c.area_n = c.area_n + F(t, 3 * tri);
+                    c.P[t] = F(t, 3 * tri + 1);
+                    c.Q[t] = F(t, 3 * tri + 2);
+                }
+            } else if (round == 1) {  // triangles A: K11 K12 K13 K22 K23 K33 at 0..5
+                if (c.mask & 2) {  // element (j-1,i), node is vertex 2
+                    c.k21 = F(1, 1); c.c22 = tme * F(1, 3); c.c23 = T(j - 1, i + 1) * F(1, 4);
+                }
+                if (c.mask & 8) {  // element (j,i-1), node is vertex 3
+                    c.k41 = F(3, 2); c.k42 = F(3, 4); c.c43 = tme * F(3, 5);
+                }
+                if (c.mask & 32)  // element (j,i), node is vertex 1
+                    c.s6 = tme * F(5, 0) + T(j + 1, i) * F(5, 1) + T(j, i + 1) * F(5, 2);
+            } else {  // triangles B
+                if (c.mask & 1) {  // element (j-1,i-1), node is vertex 3
+                    c.k11 = F(0, 2); c.k12 = F(0, 4); c.c13 = tme * F(0, 5);
+                }
+                if (c.mask & 4)  // element (j-1,i), node is vertex 2
+                    c.s3 = T(j - 1, i + 1) * F(2, 1) + tme * F(2, 3) + T(j, i + 1) * F(2, 4);
+                if (c.mask & 16) {  // element (j,i-1), node is vertex 1
+                    c.c51 = tme * F(4, 0); c.k52 = F(4, 1); c.c53 = T(j + 1, i) * F(4, 2);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    c.told = valid ? sT[LIDX(tj, ti)] : 0.0;
+    // starting iterate (the element table is no longer needed): the old temperature in
+    // pass 0, the previous pass's result later
+    for (int q = threadIdx.x; q < NL; q += blockDim.x) {
+        double xv = sT[q];
+        if (pass > 0) {
+            const int jj = j0 + q % TH_LJ - 1, ii = i0 + q / TH_LJ - 1;
+            xv = (jj >= 1 && jj <= NZ && ii >= 1 && ii <= NX) ? xin[IN(jj, ii)] : 0.0;
+        }
+        sX[0][q] = xv;
+        sX[1][q] = xv;
+    }
+    __syncthreads();
+
     const double dt = ck->dt;
     const int me = LIDX(tj, ti), up = LIDX(tj - 1, ti), lf = LIDX(tj, ti - 1), dl = LIDX(tj + 1, ti - 1);
     int cur = 0, nsweep = 0;
@@ -298,6 +445,7 @@ __global__ void __launch_bounds__(1024) k_therm_tile(const GfDevParams *__restri
         bord[(long)tile * TH_NB + threadIdx.x] = sX[cur][LIDX(lj, li)];
     }
     if (threadIdx.x == 0 && !still) atomicOr(&ck->therm_pass_changed[pass], 2u);
+    }
 #undef LIDX
 }
 
@@ -350,7 +498,7 @@ __global__ void __launch_bounds__(256) k_therm_check(const GfDevParams *__restri
 __global__ void __launch_bounds__(256) k_therm_bc(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
                                                   const double *__restrict__ xa, const double *__restrict__ xb)
 {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long t = xcd_lin();
     // pass p writes xa when p is even, xb when odd; the first pass whose border check
     // found no difference holds the fixed point
     int proven = -1;
@@ -983,7 +1131,7 @@ template <bool SHEARH>
 __global__ void __launch_bounds__(256) k_rheol1(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
                                                 int avgsr)
 {
-    long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long e = xcd_lin();
     if (e >= NE) return;
     rheol_single<SHEARH>(P, ck, A, e, avgsr);
 }
@@ -993,7 +1141,7 @@ __global__ void __launch_bounds__(256) k_rheol1(const GfDevParams *__restrict__ 
 // k_rheol: the same inputs the reference's in-loop call sees.
 __global__ void __launch_bounds__(256) k_visc(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
 {
-    long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long e = xcd_lin();
     if (e >= NE) return;
     const int irh = PHASE(A.iphase[e]).irheol;
     if (!(irh == 3 || irh >= 11)) return;
@@ -1009,7 +1157,7 @@ __global__ void __launch_bounds__(256) k_visc(const GfDevParams *__restrict__ P,
 __global__ void __launch_bounds__(256) k_rheol(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
                                                int avgsr)
 {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long t = xcd_lin();
     long e = t >> 2;
     if (e >= NE) return;  // whole quads leave together (4 divides the block size)
     if (P->ishearh == 1)
@@ -1060,7 +1208,7 @@ GF_DEV void dev_tri_force(const GfDevParams *P, const GfDevArrays &A, int je, in
 // damping and the velocity update.
 __global__ void __launch_bounds__(256) k_node(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
 {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long t = xcd_lin();
     if (t >= NN) return;
     const int j = (int)(t % NZ) + 1, i = (int)(t / NZ) + 1;
     const double dt = ck->dt;
@@ -1168,7 +1316,7 @@ __global__ void __launch_bounds__(256) k_node(const GfDevParams *__restrict__ P,
 // ------------------------------------------------------------------ fl_move
 __global__ void __launch_bounds__(256) k_move_cord(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
 {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long t = xcd_lin();
     if (t >= NN) return;
     const double dt = ck->dt;
     A.cord[t] = A.cord[t] + A.vel[t] * dt;
@@ -1178,7 +1326,7 @@ __global__ void __launch_bounds__(256) k_move_cord(const GfDevParams *__restrict
 // src/fl_move.f90:47-128
 __global__ void __launch_bounds__(256) k_move_elem(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
 {
-    long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long e = xcd_lin();
     if (e >= NE) return;
     const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
     const double dt = ck->dt;
@@ -1263,7 +1411,7 @@ __global__ void __launch_bounds__(256) k_init_visc(const GfDevParams *__restrict
 // src/matprops.f90:3-83 (Eff_dens) per element into a scratch plane
 __global__ void __launch_bounds__(256) k_dens(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
 {
-    long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long e = xcd_lin();
     if (e >= NE) return;
     const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
     const double adi_grad = 3.0e-1, den_ramp_dist = 30.0e3, amp410 = 0.05, amp660 = 0.085;
@@ -1330,7 +1478,7 @@ GF_DEV double gather_corners(const GfDevParams *P, int j, int i, F contrib)
 
 __global__ void __launch_bounds__(256) k_rmass(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
 {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long t = xcd_lin();
     if (t >= NN) return;
     const int j = (int)(t % NZ) + 1, i = (int)(t / NZ) + 1;
     const double c1d12 = 1.0 / 12.0;
@@ -1437,7 +1585,7 @@ __global__ void k_dt_scalars(const GfDevParams *__restrict__ P, GfDevClock *ck, 
 // inertial masses, src/dt_mass.f90:64-120 (gather form of the coloured scatter)
 __global__ void __launch_bounds__(256) k_amass(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
 {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long t = xcd_lin();
     if (t >= NN) return;
     if (P->idt_scale == 0) {
         A.amass[t] = A.rmass[t];

@@ -65,7 +65,9 @@ static int set_err(gf_ctx *c, int code, const char *fmt, ...)
             return set_err(ctx, GF_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));   \
     } while (0)
 
-static inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
+// workgroups of 256 for n threads, rounded up to a multiple of the 8 XCDs (kernels
+// renumber workgroups per XCD and return for indices past n)
+static inline unsigned nblk(long n) { return (unsigned)(((n + 255) / 256 + 7) / 8 * 8); }
 
 // ------------------------------------------------------------------ parameters
 static int is_mantle(int iph) { return iph == 4 || iph == 8 || iph == 9 || iph == 16 || iph == 18; }
@@ -517,8 +519,9 @@ static int enq_therm(gf_ctx *ctx)
     int rc;
     if ((rc = enq_clock(ctx, CK_THERM_BEGIN))) return rc;
     const int ntiles = ((ctx->p.nz + TH_OJ - 1) / TH_OJ) * ((ctx->p.nx + TH_OI - 1) / TH_OI);
-    const dim3 grid((unsigned)(8 * ((ntiles + 7) / 8)));
+    const unsigned nwg = (unsigned)(8 * ((ntiles + 7) / 8));
     for (int pass = 0; pass < TH_MAXPASS; pass++) {
+        const dim3 grid(pass == 0 ? nwg : std::min(nwg, 512u));
         const double *in = (pass % 2 == 1) ? ctx->a.xa : ctx->a.xb;
         double *out = (pass % 2 == 0) ? ctx->a.xa : ctx->a.xb;
         hipEvent_t e0 = nullptr, e1 = nullptr;
