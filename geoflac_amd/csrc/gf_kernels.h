@@ -111,6 +111,7 @@ __global__ void k_math_check(const double *x, const double *y, double *out, long
 #define TH_LI (TH_I + 2)          // LDS columns incl. a ring left and right
 #define TH_MAXPASS 4
 #define TH_MAXSWEEP 48
+#define TH_SWEEP0 3   // sweeps run before the first stillness test
 #define TH_NB (2 * TH_OI + TH_OJ)  // border entries per tile
 
 // logical tile of workgroup `wg` when consecutive workgroups go to the 8 XCDs in turn:
@@ -281,32 +282,52 @@ __global__ void __launch_bounds__(1024) k_therm_tile(const GfDevParams *__restri
     const int i0 = (tile / ntj) * TH_OI + 1 - TH_HI;  // 1-based column of tile column 0
     const double *told_g = pass == 0 ? A.temp : A.temp0;
 #define LIDX(lj, li) (((li) + 1) * TH_LJ + ((lj) + 1))
-    for (int q = threadIdx.x; q < NL; q += blockDim.x) {
-        const int j = j0 + q % TH_LJ - 1, i = i0 + q / TH_LJ - 1;
-        const bool in = j >= 1 && j <= NZ && i >= 1 && i <= NX;
-        sT[q] = in ? told_g[IN(j, i)] : 0.0;
-        sCx[q] = in ? CORD(j, i, 1) : 0.0;
-        sCy[q] = in ? CORD(j, i, 2) : 0.0;
-    }
-    const int e0 = threadIdx.x, e1 = threadIdx.x + TH_J * TH_I;  // this thread's table elements
+    // Each thread has two node slots (q and q + 1024 of the 66 x 18 footprint) and two
+    // element slots (e and e + 1024 of the 65 x 17 table).  All loads are issued before
+    // the first LDS store so that the tile waits for memory once.
+    constexpr int NT = TH_J * TH_I;
+    const int e0 = threadIdx.x, e1 = threadIdx.x + NT;
+    auto node_at = [&](int q, long &ix) {
+        const int jj = j0 + q % TH_LJ - 1, ii = i0 + q / TH_LJ - 1;
+        ix = IN(jj, ii);
+        return q < NL && jj >= 1 && jj <= NZ && ii >= 1 && ii <= NX;
+    };
     auto elem_ok = [&](int e, int &je, int &ie) {
         je = j0 + e % TH_EJ - 1;
         ie = i0 + e / TH_EJ - 1;
         return e < TH_NE && je >= 1 && je <= NZ - 1 && ie >= 1 && ie <= NX - 1;
     };
-    auto elem_load = [&](int e) {
-        int je, ie;
-        if (!elem_ok(e, je, ie)) return;
-        const long ix = IE(je, ie);
-        const int ph = A.iphase[ix] - 1;
-        const double dissip = with_dissip ? A.shrheat[ix] : 0.0;
-        sAr[0][e] = A.area[ix];
-        sAr[1][e] = A.area[ix + NE];
-        sDm[e] = (A.source[ix] + dissip / sPh[0][ph]) / sPh[1][ph];  // dummye, src/fl_therm.f90:405-414
-        sDf[e] = sPh[2][ph];
-    };
-    elem_load(e0);
-    elem_load(e1);
+    long nix0, nix1;
+    const bool nin0 = node_at(e0, nix0), nin1 = node_at(e1, nix1);
+    int je0, ie0, je1, ie1;
+    const bool ein0 = elem_ok(e0, je0, ie0), ein1 = elem_ok(e1, je1, ie1);
+    const long eix0 = ein0 ? IE(je0, ie0) : 0, eix1 = ein1 ? IE(je1, ie1) : 0;
+    double tv0 = 0.0, cx0 = 0.0, cy0 = 0.0, tv1 = 0.0, cx1 = 0.0, cy1 = 0.0;
+    if (nin0) { tv0 = told_g[nix0]; cx0 = A.cord[nix0]; cy0 = A.cord[nix0 + NN]; }
+    if (nin1) { tv1 = told_g[nix1]; cx1 = A.cord[nix1]; cy1 = A.cord[nix1 + NN]; }
+    double a10 = 0.0, a20 = 0.0, sr0 = 0.0, sh0 = 0.0, a11 = 0.0, a21 = 0.0, sr1 = 0.0, sh1 = 0.0;
+    int ph0 = 1, ph1 = 1;
+    if (ein0) {
+        a10 = A.area[eix0]; a20 = A.area[eix0 + NE]; ph0 = A.iphase[eix0]; sr0 = A.source[eix0];
+        if (with_dissip) sh0 = A.shrheat[eix0];
+    }
+    if (ein1) {
+        a11 = A.area[eix1]; a21 = A.area[eix1 + NE]; ph1 = A.iphase[eix1]; sr1 = A.source[eix1];
+        if (with_dissip) sh1 = A.shrheat[eix1];
+    }
+    sT[e0] = tv0; sCx[e0] = cx0; sCy[e0] = cy0;
+    if (e1 < NL) { sT[e1] = tv1; sCx[e1] = cx1; sCy[e1] = cy1; }
+    // dummye = (source + dissip/den)/cp (src/fl_therm.f90:405-414; dissip 0 unless shear heating)
+    if (ein0) {
+        sAr[0][e0] = a10; sAr[1][e0] = a20;
+        sDm[e0] = (sr0 + sh0 / sPh[0][ph0 - 1]) / sPh[1][ph0 - 1];
+        sDf[e0] = sPh[2][ph0 - 1];
+    }
+    if (ein1) {
+        sAr[0][e1] = a11; sAr[1][e1] = a21;
+        sDm[e1] = (sr1 + sh1 / sPh[0][ph1 - 1]) / sPh[1][ph1 - 1];
+        sDf[e1] = sPh[2][ph1 - 1];
+    }
     __syncthreads();
     const int j = j0 + tj, i = i0 + ti;
     const bool valid = j >= 1 && j <= NZ && i >= 1 && i <= NX;
@@ -419,6 +440,13 @@ __global__ void __launch_bounds__(1024) k_therm_tile(const GfDevParams *__restri
     const int me = LIDX(tj, ti), up = LIDX(tj - 1, ti), lf = LIDX(tj, ti - 1), dl = LIDX(tj + 1, ti - 1);
     int cur = 0, nsweep = 0;
     bool still = false;
+    // the first TH_SWEEP0 sweeps are never the last one: no stillness test for them
+    // (a sweep at the fixed point changes nothing, so more sweeps cannot change the result)
+    for (; nsweep < TH_SWEEP0; nsweep++) {
+        if (valid) sX[cur ^ 1][me] = therm_eval(c, sX[cur][up], sX[cur][lf], sX[cur][dl], dt);
+        __syncthreads();
+        cur ^= 1;
+    }
     while (nsweep < TH_MAXSWEEP) {
         bool changed = false;
         double tn = 0.0;

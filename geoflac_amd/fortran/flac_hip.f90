@@ -1,0 +1,183 @@
+! flac_hip.f90 -- drop-in replacement of subroutine flac (src/flac.f90:5-91).
+!
+! Same name, no arguments, same place in the driver's loop (src/par.f90:83): each call
+! runs one explicit step (fl_therm, fl_rheol, fl_node, fl_move, and rmasses / dt_mass
+! at their frequencies) on the GPU through libgfhip.so.  The first call copies the
+! parameters of module params and every array of module arrays to the device; after
+! that the device state is authoritative and the Fortran arrays are refreshed only by
+! flac_hip_sync_host, which the driver calls where it reads them (output, restart
+! dumps, remeshing: src/par.f90:94-127).  After the host has changed arrays itself
+! (remeshing, src/remesh.f90) it calls flac_hip_sync_device.  nloop and time stay
+! owned by the driver; nsrate, dt, dtavg, dt_elastic and dt_maxwell come back from
+! every call, as the reference's flac leaves them in module params.
+!
+! Errors: the reference's STOP codes (22, src/rh_maxwell.f90:29; 40,
+! src/fl_move.f90:125) are raised the same way, after SysMsg; ABI errors (>= 1000)
+! stop with their code and the library's message.
+module flac_hip_state
+use iso_c_binding
+implicit none
+type(c_ptr), save :: gf_ctx_handle = c_null_ptr
+end module flac_hip_state
+
+subroutine flac
+use iso_c_binding
+use gf_hip
+use flac_hip_state
+use params
+implicit none
+type(gf_clock_t) :: ck
+integer(c_int) :: rc
+
+if (.not. c_associated(gf_ctx_handle)) call flac_hip_init
+ck%nloop = nloop
+ck%nsrate = nsrate
+ck%time = time
+ck%dt = dt
+ck%dtavg = dtavg
+ck%dt_elastic = dt_elastic
+ck%dt_maxwell = dt_maxwell
+rc = gf_flac(gf_ctx_handle, ck)
+if (rc /= GF_OK) call flac_hip_fail(rc)
+nsrate = ck%nsrate
+dt = ck%dt
+dtavg = ck%dtavg
+dt_elastic = ck%dt_elastic
+dt_maxwell = ck%dt_maxwell
+end subroutine flac
+
+! Parameters and arrays to the device (the state setflac/rsflac left on the host).
+subroutine flac_hip_init
+use iso_c_binding
+use gf_hip
+use flac_hip_state
+use arrays
+use params
+implicit none
+type(gf_params_t) :: p
+type(gf_host_arrays_t) :: h
+type(gf_clock_t) :: ck
+integer(c_int) :: rc
+integer :: k, n
+
+p%nx = nx; p%nz = nz; p%nphase = nphase
+p%mix_strain = mix_strain; p%mix_stress = mix_stress; p%ndim = ndim; p%movegrid = movegrid
+p%itherm = itherm; p%istress_therm = istress_therm; p%ishearh = ishearh; p%itemp_bc = itemp_bc
+p%itype_melting = itype_melting; p%nystressbc = nystressbc; p%nydrsides = nydrsides
+p%nyhydro = nyhydro; p%i_prestress = i_prestress; p%idt_scale = idt_scale; p%i_rey = i_rey
+p%ivis_present = ivis_present; p%iphsub = iphsub; p%ifreq_rmasses = ifreq_rmasses
+p%ifreq_imasses = ifreq_imasses; p%ifreq_visc = ifreq_visc; p%ifreq_avgsr = ifreq_avgsr
+p%inhom = inhom
+p%irheol = 0; p%ix1 = 0; p%ix2 = 0; p%iy1 = 0; p%iy2 = 0; p%igeom = 0; p%pad0 = 0
+n = min(maxph, GF_MAXPH)
+p%irheol(1:n) = irheol(1:n)
+n = min(maxinh, GF_MAXINH)
+p%ix1(1:n) = ix1(1:n); p%ix2(1:n) = ix2(1:n); p%iy1(1:n) = iy1(1:n); p%iy2(1:n) = iy2(1:n)
+p%igeom(1:n) = igeom(1:n)
+p%t_top = t_top; p%t_bot = t_bot; p%bot_bc = bot_bc; p%g = g; p%demf = demf
+p%ten_off = ten_off; p%tau_heal = tau_heal; p%v_min = v_min; p%v_max = v_max
+p%frac = frac; p%fracm = fracm; p%strain_inert = strain_inert; p%dt_scale = dt_scale
+p%xReyn = xReyn; p%vbc = vbc; p%rxbo = rxbo; p%rzbo = rzbo; p%pisos = pisos; p%drosub = drosub
+p%fmagma_max = fmagma_max; p%weaken_ratio_plastic = weaken_ratio_plastic
+p%weaken_ratio_viscous = weaken_ratio_viscous; p%rho_magma = rho_magma
+do k = 1, GF_MAXPH
+    if (k > maxph) exit
+    p%den(k) = den(k); p%alfa(k) = alfa(k); p%beta(k) = beta(k); p%pln(k) = pln(k)
+    p%acoef(k) = acoef(k); p%eactiv(k) = eactiv(k); p%vactiv(k) = vactiv(k)
+    p%rl(k) = rl(k); p%rm(k) = rm(k); p%plstrain1(k) = plstrain1(k); p%plstrain2(k) = plstrain2(k)
+    p%fric1(k) = fric1(k); p%fric2(k) = fric2(k); p%cohesion1(k) = cohesion1(k)
+    p%cohesion2(k) = cohesion2(k); p%dilat1(k) = dilat1(k); p%dilat2(k) = dilat2(k)
+    p%conduct(k) = conduct(k); p%cp(k) = cp(k)
+enddo
+
+rc = gf_create(p, GF_PREC_FP64, gf_ctx_handle)
+if (rc /= GF_OK) call flac_hip_fail(rc)
+
+h%cord = gf_addr8(cord); h%vel = gf_addr8(vel); h%temp = gf_addr8(temp); h%temp0 = gf_addr8(temp0)
+h%force = gf_addr8(force); h%amass = gf_addr8(amass); h%rmass = gf_addr8(rmass); h%bc = gf_addr8(bc)
+h%ncod = gf_addr4(ncod)
+h%stress0 = gf_addr8(stress0); h%area = gf_addr8(area); h%dvol = gf_addr8(dvol)
+h%strain = gf_addr8(strain); h%se2sr = gf_addr8(se2sr)
+h%aps = gf_addr8(aps); h%visn = gf_addr8(visn); h%e2sr = gf_addr8(e2sr)
+h%sshrheat = gf_addr8(sshrheat); h%shrheat = gf_addr8(shrheat); h%source = gf_addr8(source)
+h%iphase = gf_addr4(iphase)
+h%phase_ratio = gf_addr8(phase_ratio)
+rc = gf_bind_host(gf_ctx_handle, h)
+if (rc /= GF_OK) call flac_hip_fail(rc)
+rc = gf_upload(gf_ctx_handle, GF_ALL)
+if (rc /= GF_OK) call flac_hip_fail(rc)
+ck%nloop = nloop; ck%nsrate = nsrate; ck%time = time; ck%dt = dt; ck%dtavg = dtavg
+ck%dt_elastic = dt_elastic; ck%dt_maxwell = dt_maxwell
+rc = gf_set_clock(gf_ctx_handle, ck)
+if (rc /= GF_OK) call flac_hip_fail(rc)
+end subroutine flac_hip_init
+
+! Device -> Fortran arrays (before outflac / saveflac / remeshing read them).
+subroutine flac_hip_sync_host
+use iso_c_binding
+use gf_hip
+use flac_hip_state
+implicit none
+integer(c_int) :: rc
+if (.not. c_associated(gf_ctx_handle)) return
+rc = gf_download(gf_ctx_handle, GF_ALL)
+if (rc /= GF_OK) call flac_hip_fail(rc)
+end subroutine flac_hip_sync_host
+
+! Fortran arrays -> device (after the host changed them, e.g. remeshing).
+subroutine flac_hip_sync_device
+use iso_c_binding
+use gf_hip
+use flac_hip_state
+use params
+implicit none
+integer(c_int) :: rc
+type(gf_clock_t) :: ck
+if (.not. c_associated(gf_ctx_handle)) return
+rc = gf_upload(gf_ctx_handle, GF_ALL)
+if (rc /= GF_OK) call flac_hip_fail(rc)
+ck%nloop = nloop; ck%nsrate = nsrate; ck%time = time; ck%dt = dt; ck%dtavg = dtavg
+ck%dt_elastic = dt_elastic; ck%dt_maxwell = dt_maxwell
+rc = gf_set_clock(gf_ctx_handle, ck)
+if (rc /= GF_OK) call flac_hip_fail(rc)
+end subroutine flac_hip_sync_device
+
+subroutine flac_hip_finalize
+use iso_c_binding
+use gf_hip
+use flac_hip_state
+implicit none
+if (.not. c_associated(gf_ctx_handle)) return
+call gf_destroy(gf_ctx_handle)
+gf_ctx_handle = c_null_ptr
+end subroutine flac_hip_finalize
+
+subroutine flac_hip_fail(rc)
+use iso_c_binding
+use gf_hip
+use flac_hip_state
+implicit none
+integer(c_int), intent(in) :: rc
+integer(c_int32_t) :: j, i
+integer(c_int) :: r2
+character(len=600) :: msg
+j = 0; i = 0
+if (c_associated(gf_ctx_handle)) r2 = gf_error_site(gf_ctx_handle, j, i)
+if (rc == GF_STOP_MAXWELL) then
+    write(msg, '(A,2I8)') 'Maxwell: time step!!! (element j, i)', j, i
+    call SysMsg(trim(msg))
+    stop 22
+else if (rc == GF_STOP_NEGATIVE_AREA) then
+    write(msg, '(A,2I8)') 'FL_MOVE: NEGATIVE AREA! (element j, i)', j, i
+    call SysMsg(trim(msg))
+    stop 40
+endif
+if (c_associated(gf_ctx_handle)) then
+    msg = 'libgfhip: '//trim(gf_message(gf_ctx_handle))
+else
+    msg = 'libgfhip: context creation failed'
+endif
+call SysMsg(trim(msg))
+write(*, '(A,I6)') trim(msg)//' code', rc
+error stop 1
+end subroutine flac_hip_fail

@@ -50,6 +50,12 @@ force = 0.d0
 dike_released = 0.d0
 call setflac
 call dump_params(outdir)
+! every dump directory is made here, before the first step: once the drop-in has
+! initialised the GPU this process starts no other program
+do k = 1, nd
+    write(arg, '(A,A,I7.7)') trim(outdir), '/step_', dumps(k)
+    call execute_command_line('mkdir -p '//trim(arg))
+enddo
 if (any(dumps(1:nd) == 0)) call dump_state(outdir, 0)
 
 do istep = 1, nsteps
@@ -96,7 +102,6 @@ subroutine dump_state(base, step)
     character(len=600) :: dir
     integer :: u, ne, nn
     write(dir, '(A,A,I7.7)') trim(base), '/step_', step
-    call execute_command_line('mkdir -p '//trim(dir))
     ne = (nz-1)*(nx-1)
     nn = nz*nx
     call wr8(dir, 'cord', cord, nn*2)

@@ -1,0 +1,77 @@
+"""The Fortran drop-in: the reference's own driver code (read_params, setflac, its
+arrays and parameters; compiled from the reference sources by oracle/Makefile) calling
+geoflac_amd/fortran/flac_hip.f90 in place of src/flac.f90, through the ISO_C_BINDING
+module gf_hip.f90 and the C-ABI.  Its dumps must match the reference's goldens to the
+same bar as the Python-driven runs.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from geoflac_amd import state
+from tests import oracle_lib as ol
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "oracle", "_ref", "dropin_harness")
+CASE = "box60x20"
+TOL = 1e-10
+
+
+def test_fortran_interface_mirrors_c_structs():
+    """gf_hip.f90 declares the C structs' fields in the header's order (no GPU)."""
+    import re
+    hdr = open(os.path.join(ROOT, "include", "gf_abi.h")).read()
+    f90 = open(os.path.join(ROOT, "geoflac_amd", "fortran", "gf_hip.f90")).read()
+
+    def c_fields(struct):
+        body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (struct, struct), hdr, re.S).group(1)
+        body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+        out = []
+        for decl in body.split(";"):
+            decl = decl.strip()
+            if not decl:
+                continue
+            names = decl.split(None, 1)[1]
+            out += [n.strip().lstrip("*").split("[")[0] for n in names.split(",")]
+        return out
+
+    def f_fields(typ):
+        body = re.search(r"type, bind\(C\) :: %s\n(.*?)end type" % typ, f90, re.S).group(1)
+        out = []
+        for line in body.splitlines():
+            if "::" in line:
+                names = line.split("::")[1]
+                out += [re.sub(r"\(.*?\)", "", n).strip() for n in re.split(r",(?![^(]*\))", names)]
+        return out
+
+    assert c_fields("gf_params") == f_fields("gf_params_t")
+    assert c_fields("gf_host_arrays") == f_fields("gf_host_arrays_t")
+    assert c_fields("gf_clock") == f_fields("gf_clock_t")
+
+
+@pytest.mark.gpu
+def test_fortran_dropin_matches_reference(tmp_path):
+    if not os.path.exists(EXE):
+        pytest.skip("oracle/_ref/dropin_harness is built only where the reference sources are present")
+    inp = os.path.join(ROOT, "tests", "golden", "inputs", CASE + ".inp")
+    out = str(tmp_path / "out")
+    print("case: Fortran drop-in, 100 steps of %s (dumps at 10 and 100)" % CASE, flush=True)
+    r = subprocess.run([EXE, inp, out, "100", "10", "100"], cwd=str(tmp_path), capture_output=True,
+                       text=True, timeout=300)
+    sys.stdout.write(r.stdout[-2000:])
+    assert r.returncode == 0, r.stderr[-2000:]
+    for step in (10, 100):
+        st = state.load_dump(os.path.join(out, "step_%07d" % step))
+        ref = ol.golden_state(CASE, step)
+        worst = {}
+        for f in ("cord", "vel", "temp", "stress0", "aps", "strain", "area", "visn", "e2sr", "se2sr",
+                  "rmass", "amass", "temp0"):
+            a, b = getattr(st, f), getattr(ref, f)
+            worst[f] = 0.0 if np.array_equal(a, b) else state.rel_l2(a, b)
+        print("  step %d: %s" % (step, {k: "%.1e" % v for k, v in worst.items() if v}), flush=True)
+        assert max(worst.values()) <= TOL, worst
+        assert st.clock.nloop == ref.clock.nloop
+        assert abs(st.clock.dt - ref.clock.dt) <= 1e-12 * ref.clock.dt
