@@ -29,6 +29,7 @@ METRIC = "element-steps/s on 1 GPU: mixed/learned vs plain FP64, at stated rel-L
 def alg_bytes(kernel, E, N, nact, p):
     """Algorithmic (compulsory) HBM bytes of one launch: each array the kernel must
     read once and write once, FP64 = 8 B, int32 = 4 B."""
+    kernel = kernel.replace("<double>", "")
     if kernel.startswith("k_rheol"):
         per_e = (32 + 32 + 128 + 8 + 8 + 8 * nact + 4 + 24 + 8 + 24)  # area dvol stress0 aps visn pr iphase se2sr sshrheat strain
         per_e += 128 + 8 + 24 + 24 + 8                                 # writes stress0 aps strain se2sr sshrheat
@@ -79,6 +80,31 @@ def cpu_baseline(gp, st, seconds_cap=20.0):
                       "of the reference step), %.1f s" % (n, gp.nx - 1, gp.nz - 1, el)}
 
 
+def mixed_run(gp, host_state, fp64_state, args, E, precision):
+    """The same model and step count in a mixed-precision mode (FP32 planes for stress,
+    velocity and force; FP32 or FP64 arithmetic): throughput and per-field error
+    against the FP64 run."""
+    import numpy as np
+    from geoflac_amd import abi, gpu, state
+    st = host_state.copy()
+    m = gpu.Model(gp, st, precision=precision)
+    m.step(args.warmup)
+    m.sync()
+    t0 = time.perf_counter()
+    m.step(args.steps)
+    m.sync()
+    el = time.perf_counter() - t0
+    m.download()
+    m.close()
+    err = {}
+    for f in ("vel", "stress0", "temp", "cord", "aps"):
+        a, b = getattr(st, f), getattr(fp64_state, f)
+        err[f] = {"rel_l2": float("%.3e" % state.rel_l2(a, b)), "max_abs": float("%.3e" % np.max(np.abs(a - b)))}
+    dtype = "f32 arithmetic, f32 planes" if precision == abi.PREC_MIXED else "f64 arithmetic, f32 planes"
+    return {"value": E * args.steps / el, "ms_per_step": el / args.steps * 1e3, "dtype": dtype,
+            "steps_compared": args.warmup + args.steps, "error_vs_fp64": err}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +113,8 @@ def main():
     ap.add_argument("--config", default="layers4000x1000")
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-precision run (FP32 stress/strain rate/force/velocity)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a streaming kernel over 1 GiB (read 1 GiB, write 1 GiB; torch.mul) "
                          "to calibrate the HBM byte counters of a profiler")
@@ -111,15 +139,18 @@ def main():
     t1 = time.perf_counter()
     el = t1 - t0
     value = E * args.steps / el
+    m.download()
+    fp64_state = st.copy()   # after warmup + steps, for the mixed-precision comparison
 
     # profiled run: HIP events around every launch on the library stream
     args.profile_steps = max(args.profile_steps, 1)
     m.therm_stats()
     m.set_timing(True)
     m.step(args.profile_steps)
-    kernels = ["k_rheol1<false>", "k_rheol1<true>", "k_rheol", "k_visc", "k_therm_tile", "k_therm_tile_extra",
-               "k_therm_check", "k_node", "k_move_elem", "k_move_cord", "k_therm_bc", "k_dens", "k_rmass",
-               "k_dt_reduce", "k_amass", "k_clock"]
+    kernels = ["k_rheol1<false>", "k_rheol1<true>", "k_rheol", "k_visc<double>", "k_therm_tile",
+               "k_therm_tile_extra", "k_therm_check", "k_node<double>", "k_move_elem<double>",
+               "k_move_cord<double>", "k_therm_bc", "k_dens<double>", "k_rmass", "k_dt_reduce<double>", "k_amass",
+               "k_clock"]
     times = {k: m.kernel_time(k) for k in kernels}
     m.set_timing(False)
     th = m.therm_stats()
@@ -136,6 +167,10 @@ def main():
                 "therm_extra_passes_per_step": th[1] / args.profile_steps,
                 "therm_border_mismatches": th[2], "therm_mismatch_max_ulp": th[3]}
     m.close()
+    mixed = None
+    if not args.no_mixed:
+        mixed = {"fp32_arithmetic": mixed_run(gp, host_state, fp64_state, args, E, abi.PREC_MIXED),
+                 "fp32_storage": mixed_run(gp, host_state, fp64_state, args, E, abi.PREC_MIXED_STORAGE)}
     if args.copy_calibration:
         import torch
         x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
@@ -152,6 +187,10 @@ def main():
                                   % (args.config, gp.nx - 1, gp.nz - 1),
                       "elements": E, "nodes": N, "precision": "fp64", "parallelism": "single GPU"},
            "roofline": roofline}
+    if mixed:
+        for v in mixed.values():
+            v["speedup_vs_fp64"] = round(v["value"] / value, 3)
+        out["mixed"] = mixed
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(gp, host_state)
     print(json.dumps(out), flush=True)

@@ -20,6 +20,7 @@ GF_ERR_THERM_SWEEPS = 1004
 
 PREC_FP64 = 0
 PREC_MIXED = 1
+PREC_MIXED_STORAGE = 2
 
 _I = C.c_int32
 _D = C.c_double

@@ -86,9 +86,24 @@ GF_DEV void bst(Buf b, unsigned voff, unsigned soff, double v)
 {
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(gf_u2, v), b.r, (int)voff, (int)soff, 0);
 }
+GF_DEV float bldf(Buf b, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(b.r, (int)voff, (int)soff, 0));
+}
+GF_DEV void bstf(Buf b, unsigned voff, unsigned soff, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), b.r, (int)voff, (int)soff, 0);
+}
+// loads and stores of the precision T of a plane (FP64 path: double; mixed: float)
+template <typename T> GF_DEV T bldt(Buf b, unsigned voff, unsigned soff);
+template <> GF_DEV double bldt<double>(Buf b, unsigned voff, unsigned soff) { return bld(b, voff, soff); }
+template <> GF_DEV float bldt<float>(Buf b, unsigned voff, unsigned soff) { return bldf(b, voff, soff); }
+GF_DEV void bstt(Buf b, unsigned voff, unsigned soff, double v) { bst(b, voff, soff, v); }
+GF_DEV void bstt(Buf b, unsigned voff, unsigned soff, float v) { bstf(b, voff, soff, v); }
 
 // ---------------------------------------------------------------- helpers
 GF_DEV double fsign(double a, double b) { return b < 0.0 ? -fabs(a) : fabs(a); }
+GF_DEV float fsign(float a, float b) { return b < 0.0f ? -fabsf(a) : fabsf(a); }
 
 GF_DEV void raise_error(GfDevClock *ck, int code, long long index)
 {
@@ -96,129 +111,185 @@ GF_DEV void raise_error(GfDevClock *ck, int code, long long index)
     atomicMin(&ck->error_index, index);
 }
 
+// The constitutive updates are templates on the arithmetic type T of stress and
+// strain rate: double for the FP64 path (the reference's arithmetic, bit for bit),
+// float for the mixed-precision path.  Constants are written T(...) so that the
+// double instantiation keeps the reference's double constants.
+
 // src/rh_elastic.f90:3-26
-GF_DEV void dev_elastic(double bulkm, double rmu, double &s11, double &s22, double &s33,
-                        double &s12, double de11, double de22, double de12)
+template <typename T>
+GF_DEV void dev_elastic(T bulkm, T rmu, T &s11, T &s22, T &s33, T &s12, T de11, T de22, T de12)
 {
-    const double c4d3 = 4.0 / 3.0, c2d3 = 2.0 / 3.0;
-    double a1 = bulkm + c4d3 * rmu;
-    double a2 = bulkm - c2d3 * rmu;
+    const T c4d3 = T(4.0) / T(3.0), c2d3 = T(2.0) / T(3.0);
+    T a1 = bulkm + c4d3 * rmu;
+    T a2 = bulkm - c2d3 * rmu;
     s11 = s11 + a1 * de11 + a2 * de22;
     s22 = s22 + a2 * de11 + a1 * de22;
-    s12 = s12 + 2 * rmu * de12;
+    s12 = s12 + T(2) * rmu * de12;
     s33 = s33 + a2 * (de11 + de22);
 }
 
-// src/rh_maxwell.f90:2-76 (ndim = 2 branch; de33 = 0 as passed by fl_rheol).
-// Returns false where the reference does STOP 22.
-GF_DEV bool dev_maxwell(double bulkm, double rmu0, double viscosity, double &s11, double &s22,
-                        double &s33, double &s12, double de11, double de22, double de12, double dv,
-                        double dt)
+// Per-element factors of src/rh_maxwell.f90:2-76 (rmu0, viscosity and dt are the same
+// for the four triangles), computed in double for both paths.  ok is false where the
+// reference does STOP 22.
+struct MaxwellConst {
+    double rmu, vic1, vic2;
+    bool ok;
+};
+GF_DEV MaxwellConst maxwell_const(double rmu0, double viscosity, double dt)
 {
     const double visc_cut = 1.0e19;
-    double rmu = viscosity < visc_cut ? rmu0 * viscosity / visc_cut : rmu0;
-    double temp = rmu / (2 * viscosity) * dt;
-    if (temp > 0.5) return false;
-    double vic1 = 1 - temp;
-    double vic2 = 1.0 / (1.0 + temp);
-    double dev = de11 + de22;
-    double de11d = de11 - 0.5 * dev;
-    double de22d = de22 - 0.5 * dev;
-    double s0 = 0.5 * (s11 + s22);
-    double s11d = s11 - s0;
-    double s22d = s22 - s0;
-    double s33d = 0;
-    s11d = (s11d * vic1 + 2 * rmu * de11d) * vic2;
-    s22d = (s22d * vic1 + 2 * rmu * de22d) * vic2;
-    s33d = (s33d * vic1 + 2 * rmu * 0.0) * vic2;
-    s12 = (s12 * vic1 + 2 * rmu * de12) * vic2;
+    MaxwellConst m;
+    m.rmu = viscosity < visc_cut ? rmu0 * viscosity / visc_cut : rmu0;
+    double temp = m.rmu / (2 * viscosity) * dt;
+    m.ok = !(temp > 0.5);
+    m.vic1 = 1 - temp;
+    m.vic2 = 1.0 / (1.0 + temp);
+    return m;
+}
+
+// src/rh_maxwell.f90:2-76, ndim = 2 branch; de33 = 0 as passed by fl_rheol.
+template <typename T>
+GF_DEV void dev_maxwell(T bulkm, T rmu, T vic1, T vic2, T &s11, T &s22, T &s33, T &s12, T de11, T de22, T de12,
+                        T dv)
+{
+    T dev = de11 + de22;
+    T de11d = de11 - T(0.5) * dev;
+    T de22d = de22 - T(0.5) * dev;
+    T s0 = T(0.5) * (s11 + s22);
+    T s11d = s11 - s0;
+    T s22d = s22 - s0;
+    T s33d = 0;
+    s11d = (s11d * vic1 + T(2) * rmu * de11d) * vic2;
+    s22d = (s22d * vic1 + T(2) * rmu * de22d) * vic2;
+    s33d = (s33d * vic1 + T(2) * rmu * T(0.0)) * vic2;
+    s12 = (s12 * vic1 + T(2) * rmu * de12) * vic2;
     s0 = s0 + bulkm * dv;
     s11 = s11d + s0;
     s22 = s22d + s0;
     s33 = s33d + s0;
-    return true;
 }
 
 // Per-element plastic constants of src/rh_plastic.f90:29-46: they depend on
 // (bulkm, rmu, coh, phi, psi) only, which are uniform over the 4 triangles.
-struct PlasticConst {
-    double anphi, anpsi, amc, e1, e2, x1, ten_max;
+template <typename T>
+struct PlasticConstT {
+    T anphi, anpsi, amc, e1, e2, x1, ten_max;
 };
+typedef PlasticConstT<double> PlasticConst;
 
-GF_DEV PlasticConst plastic_const(double bulkm, double rmu, double coh, double phi, double psi,
-                                  double ten_off)
+// The same return in FP32 on the deviation d = sigma + pref*I of the stress from an
+// isotropic reference -pref*I (pref in FP64): every step of src/rh_plastic.f90:48-224
+// is linear in sigma and commutes with adding an isotropic tensor, except the yield
+// function and the tension tests, which are evaluated on the absolute principal
+// stresses in FP64.
+GF_DEV void dev_plastic_dev(const PlasticConstT<float> &k, double anphi, double amc, double ten_max, double pref,
+                            float rmu, float hardn, float &depls, float &s11, float &s22, float &s33, float &s12,
+                            float de11, float de22, float de12)
 {
-    const double degrad = 3.14159265358979323846 / 180.0, c4d3 = 4.0 / 3.0, c2d3 = 2.0 / 3.0;
-    PlasticConst k;
-    double sphi = cr_sin(phi * degrad);
-    double spsi = cr_sin(psi * degrad);
-    k.anphi = (1.0 + sphi) / (1.0 - sphi);
-    k.anpsi = (1.0 + spsi) / (1.0 - spsi);
-    k.amc = 2.0 * coh * sqrt(k.anphi);
-    k.e1 = bulkm + c4d3 * rmu;
-    k.e2 = bulkm - c2d3 * rmu;
-    k.x1 = (k.e1 - k.e2 * k.anpsi + k.e1 * k.anphi * k.anpsi - k.e2 * k.anphi);
-    if (phi == 0.0)
-        k.ten_max = ten_off;
-    else
-        k.ten_max = fmin(ten_off, coh / (cr_tan(phi * degrad)));
-    return k;
+    depls = 0.0f;
+    float s11i = s11 + de22 * k.e2 + de11 * k.e1;
+    float s22i = s22 + de11 * k.e2 + de22 * k.e1;
+    float s12i = s12 + de12 * 2.0f * rmu;
+    float s33i = s33 + (de11 + de22) * k.e2;
+    float sdif = s11i - s22i;
+    float s0 = 0.5f * (s11i + s22i);
+    float rad = 0.5f * sqrtf(sdif * sdif + 4.0f * s12i * s12i);
+    float si = s0 - rad, sii = s0 + rad;
+    float psdif = si - sii;
+    float s1 = si, s2 = s33i, s3 = sii;
+    const float ten_d = (float)(ten_max + pref);  // ten_max as a deviation
+    if ((double)s1 - pref >= ten_max) {
+        s11 = ten_d; s22 = ten_d; s12 = 0.0f; s33 = ten_d;
+        return;
+    }
+    if ((double)s3 - pref >= ten_max) s3 = ten_d;
+    const double fs = ((double)s1 - pref) - ((double)s3 - pref) * anphi + amc;
+    if (fs < 0.0) {
+        float alams = (float)(fs / ((double)k.x1 + (double)hardn));
+        s1 = s1 - alams * (k.e1 - k.e2 * k.anpsi);
+        s2 = s2 - alams * k.e2 * (1.0f - k.anpsi);
+        s3 = s3 - alams * (k.e2 - k.e1 * k.anpsi);
+        float dep3 = -alams * k.anpsi;
+        depls = 0.5f * fabsf(alams - dep3);
+    } else {
+        s11 = s11i; s22 = s22i; s33 = s33i; s12 = s12i;
+        return;
+    }
+    if ((double)s1 - pref >= ten_max) {
+        s11 = ten_d; s22 = ten_d; s12 = 0.0f; s33 = ten_d;
+        return;
+    }
+    if ((double)s3 - pref >= ten_max) s3 = ten_d;
+    float cs2, si2;
+    if (psdif == 0.0f) {
+        cs2 = 1.0f; si2 = 0.0f;
+    } else {
+        cs2 = sdif / psdif;
+        si2 = 2.0f * s12i / psdif;
+    }
+    float dc2 = (s1 - s3) * cs2;
+    float dss = s1 + s3;
+    s11 = 0.5f * (dss + dc2);
+    s22 = 0.5f * (dss - dc2);
+    s12 = 0.5f * (s1 - s3) * si2;
+    s33 = s2;
 }
 
 // src/rh_plastic.f90:48-224, ndim = 2, de33 = 0, press_add = 0.
-GF_DEV void dev_plastic(const PlasticConst &k, double rmu, double hardn, double &depls, double &s11,
-                        double &s22, double &s33, double &s12, double de11, double de22,
-                        double de12)
+template <typename T>
+GF_DEV void dev_plastic(const PlasticConstT<T> &k, T rmu, T hardn, T &depls, T &s11, T &s22, T &s33, T &s12,
+                        T de11, T de22, T de12)
 {
-    const double de33 = 0.0, press_add = 0.0;
-    depls = 0.0;
-    double s11i = s11 + (de22 + de33) * k.e2 + de11 * k.e1 - press_add;
-    double s22i = s22 + (de11 + de33) * k.e2 + de22 * k.e1 - press_add;
-    double s12i = s12 + de12 * 2.0 * rmu;
-    double s33i = s33 + (de11 + de22) * k.e2 + de33 * k.e1 - press_add;
-    double sdif = s11i - s22i;
-    double s0 = 0.5 * (s11i + s22i);
-    double rad = 0.5 * sqrt(sdif * sdif + 4.0 * s12i * s12i);
-    double si = s0 - rad;
-    double sii = s0 + rad;
-    double psdif = si - sii;
-    double s1 = si, s2 = s33i, s3 = sii;
-    const double ten_max = k.ten_max;
+    const T de33 = T(0.0), press_add = T(0.0);
+    depls = T(0.0);
+    T s11i = s11 + (de22 + de33) * k.e2 + de11 * k.e1 - press_add;
+    T s22i = s22 + (de11 + de33) * k.e2 + de22 * k.e1 - press_add;
+    T s12i = s12 + de12 * T(2.0) * rmu;
+    T s33i = s33 + (de11 + de22) * k.e2 + de33 * k.e1 - press_add;
+    T sdif = s11i - s22i;
+    T s0 = T(0.5) * (s11i + s22i);
+    T rad = T(0.5) * sqrt(sdif * sdif + T(4.0) * s12i * s12i);
+    T si = s0 - rad;
+    T sii = s0 + rad;
+    T psdif = si - sii;
+    T s1 = si, s2 = s33i, s3 = sii;
+    const T ten_max = k.ten_max;
     if (s1 >= ten_max) {
-        s11 = ten_max + press_add; s22 = ten_max + press_add; s12 = 0.0; s33 = ten_max + press_add;
+        s11 = ten_max + press_add; s22 = ten_max + press_add; s12 = T(0.0); s33 = ten_max + press_add;
         return;
     }
     if (s3 >= ten_max) s3 = ten_max;
-    double fs = s1 - s3 * k.anphi + k.amc;
-    if (fs < 0.0) {
-        double alams = fs / (k.x1 + hardn);
+    T fs = s1 - s3 * k.anphi + k.amc;
+    if (fs < T(0.0)) {
+        T alams = fs / (k.x1 + hardn);
         s1 = s1 - alams * (k.e1 - k.e2 * k.anpsi);
-        s2 = s2 - alams * k.e2 * (1.0 - k.anpsi);
+        s2 = s2 - alams * k.e2 * (T(1.0) - k.anpsi);
         s3 = s3 - alams * (k.e2 - k.e1 * k.anpsi);
-        double dep1 = alams;
-        double dep3 = -alams * k.anpsi;
-        depls = 0.5 * fabs(dep1 - dep3);
+        T dep1 = alams;
+        T dep3 = -alams * k.anpsi;
+        depls = T(0.5) * fabs(dep1 - dep3);
     } else {
         s11 = s11i + press_add; s22 = s22i + press_add; s33 = s33i + press_add; s12 = s12i;
         return;
     }
     if (s1 >= ten_max) {
-        s11 = ten_max + press_add; s22 = ten_max + press_add; s12 = 0.0; s33 = ten_max + press_add;
+        s11 = ten_max + press_add; s22 = ten_max + press_add; s12 = T(0.0); s33 = ten_max + press_add;
         return;
     }
     if (s3 >= ten_max) s3 = ten_max;
-    double cs2, si2;
-    if (psdif == 0.0) {
-        cs2 = 1.0; si2 = 0.0;
+    T cs2, si2;
+    if (psdif == T(0.0)) {
+        cs2 = T(1.0); si2 = T(0.0);
     } else {
         cs2 = sdif / psdif;
-        si2 = 2.0 * s12i / psdif;
+        si2 = T(2.0) * s12i / psdif;
     }
-    double dc2 = (s1 - s3) * cs2;
-    double dss = s1 + s3;
-    s11 = 0.5 * (dss + dc2);
-    s22 = 0.5 * (dss - dc2);
-    s12 = 0.5 * (s1 - s3) * si2;
+    T dc2 = (s1 - s3) * cs2;
+    T dss = s1 + s3;
+    s11 = T(0.5) * (dss + dc2);
+    s22 = T(0.5) * (dss - dc2);
+    s12 = T(0.5) * (s1 - s3) * si2;
     s33 = s2;
     s11 = s11 + press_add;
     s22 = s22 + press_add;

@@ -17,7 +17,31 @@ struct GfDevArrays {
     double *pr;           // phase ratio planes of the active phases: pr[a*ne + e]
     double *xa, *xb;      // thermal fixed-point iterates
     double *dens;         // per-element density scratch (rmasses)
+    // Mixed precision: velocity and force in FP32, and the stress as an FP32 deviation
+    // from a per-element isotropic reference -pref*I kept in FP64 (the element's mean
+    // normal stress when it was uploaded), so that the FP32 planes carry the tectonic
+    // part without the lithostatic one.  The double planes above are not allocated.
+    float *stress0f, *velf, *forcef;
+    double *pref;
 };
+
+// the isotropic reference added back to an SV stress (FP64 path: none)
+template <typename SV> GF_DEV double ref_press(const GfDevArrays &A, long e);
+template <> GF_DEV double ref_press<double>(const GfDevArrays &A, long e) { return 0.0; }
+template <> GF_DEV double ref_press<float>(const GfDevArrays &A, long e) { return A.pref[e]; }
+
+// Precision modes of the step: storage type SV of the stress/velocity/force planes and
+// arithmetic type AT of strain rate, stress and force (GF_PREC_FP64, GF_PREC_MIXED,
+// GF_PREC_MIXED_STORAGE of the C-ABI)
+template <int M> struct Prec;
+template <> struct Prec<0> { typedef double SV; typedef double AT; };
+template <> struct Prec<1> { typedef float SV; typedef float AT; };
+template <> struct Prec<2> { typedef float SV; typedef double AT; };
+
+// the plane of a stress/velocity/force field in the precision SV of the path
+template <typename SV> GF_DEV SV *sv_ptr(double *d, float *f);
+template <> GF_DEV double *sv_ptr<double>(double *d, float *f) { return d; }
+template <> GF_DEV float *sv_ptr<float>(double *d, float *f) { return f; }
 
 #define NZ (P->nz)
 #define NX (P->nx)
@@ -798,7 +822,9 @@ GF_DEV void rheol_quad(const GfDevParams *__restrict__ P, GfDevClock *ck, const 
         dev_elastic(bulkm, rmu, sp[0], sp[1], sp[3], sp[2], de11, de22, de12);
         for (int c = 0; c < 4; c++) st[c] = sp[c];
     } else if (irh == 3) {
-        ok = dev_maxwell(bulkm, rmu, vis, sp[0], sp[1], sp[3], sp[2], de11, de22, de12, dv, dt);
+        const MaxwellConst mc = maxwell_const(rmu, vis, dt);
+        ok = mc.ok;
+        dev_maxwell(bulkm, mc.rmu, mc.vic1, mc.vic2, sp[0], sp[1], sp[3], sp[2], de11, de22, de12, dv);
         for (int c = 0; c < 4; c++) st[c] = sp[c];
     } else if (irh == 6) {
         dev_plastic(pc, rmu, hardn, depl, sp[0], sp[1], sp[3], sp[2], de11, de22, de12);
@@ -806,7 +832,9 @@ GF_DEV void rheol_quad(const GfDevParams *__restrict__ P, GfDevClock *ck, const 
     } else if (irh >= 11) {
         double sv[4] = {sp[0], sp[1], sp[2], sp[3]};
         dev_plastic(pc, rmu, hardn, depl, sp[0], sp[1], sp[3], sp[2], de11, de22, de12);
-        ok = dev_maxwell(bulkm, rmu, vis, sv[0], sv[1], sv[3], sv[2], de11, de22, de12, dv, dt);
+        const MaxwellConst mc = maxwell_const(rmu, vis, dt);
+        ok = mc.ok;
+        dev_maxwell(bulkm, mc.rmu, mc.vic1, mc.vic2, sv[0], sv[1], sv[3], sv[2], de11, de22, de12, dv);
         // :266-288 the branch with the smaller summed invariant, same bits on every lane
         double a = QL(sp[0], 0) + QL(sp[0], 1) + QL(sp[0], 2) + QL(sp[0], 3) - QL(sp[1], 0) - QL(sp[1], 1) -
                    QL(sp[1], 2) - QL(sp[1], 3);
@@ -903,12 +931,19 @@ static __device__ __noinline__ void cr_sin_sin_tan(double phi, double psi, doubl
     *tphi = phi == 0.0 ? 0.0 : cr_tan(phi * degrad);
 }
 
-// the constitutive update of one triangle for rheology irh (:216-264); returns false on STOP 22
-GF_DEV bool rheol_tri(int irh, bool plastic_branch, const PlasticConst &pc, double bulkm, double rmu,
-                      double hardn, double vis, double dt, double stherm, const double st_k[4],
-                      const double sr_k[3], double dv, double out[4], double &depl)
+// the constitutive update of one triangle for rheology irh (:216-264), in arithmetic T;
+// returns false on STOP 22
+// FP64 values the FP32 plastic return needs (mixed precision only)
+struct PlasticDbl {
+    double anphi, amc, ten_max, pref;
+};
+
+template <typename T>
+GF_DEV bool rheol_tri(int irh, bool plastic_branch, const PlasticConstT<T> &pc, const PlasticDbl &pd, T bulkm,
+                      T rmu, T hardn, double rmu0, double vis, double dt_d, T dt, T stherm, const T st_k[4],
+                      const T sr_k[3], T dv, T out[4], T &depl)
 {
-    double de11 = sr_k[0] * dt, de22 = sr_k[1] * dt, de12 = sr_k[2] * dt;
+    T de11 = sr_k[0] * dt, de22 = sr_k[1] * dt, de12 = sr_k[2] * dt;
     out[0] = st_k[0] + stherm;
     out[1] = st_k[1] + stherm;
     out[2] = st_k[2];
@@ -917,9 +952,15 @@ GF_DEV bool rheol_tri(int irh, bool plastic_branch, const PlasticConst &pc, doub
     if (irh == 1) {
         dev_elastic(bulkm, rmu, out[0], out[1], out[3], out[2], de11, de22, de12);
     } else if (irh == 3 || (irh >= 11 && !plastic_branch)) {
-        ok = dev_maxwell(bulkm, rmu, vis, out[0], out[1], out[3], out[2], de11, de22, de12, dv, dt);
+        const MaxwellConst mc = maxwell_const(rmu0, vis, dt_d);
+        ok = mc.ok;
+        dev_maxwell(bulkm, T(mc.rmu), T(mc.vic1), T(mc.vic2), out[0], out[1], out[3], out[2], de11, de22, de12, dv);
     } else if (irh == 6 || irh >= 11) {
-        dev_plastic(pc, rmu, hardn, depl, out[0], out[1], out[3], out[2], de11, de22, de12);
+        if constexpr (sizeof(T) == 4)
+            dev_plastic_dev(pc, pd.anphi, pd.amc, pd.ten_max, pd.pref, rmu, hardn, depl, out[0], out[1], out[3],
+                            out[2], de11, de22, de12);
+        else
+            dev_plastic(pc, rmu, hardn, depl, out[0], out[1], out[3], out[2], de11, de22, de12);
     }
     return ok;
 }
@@ -929,7 +970,11 @@ GF_DEV bool rheol_tri(int irh, bool plastic_branch, const PlasticConst &pc, doub
 // The invariant loop accumulates both branches' sums without keeping either stress
 // set, and the chosen branch is recomputed; scheduling barriers bound live ranges.
 #define SB() __builtin_amdgcn_sched_barrier(0)
-template <bool SHEARH>
+// SV: precision of the stored stress/velocity planes; AT: arithmetic of strain rate
+// and stress.  (double, double) is the FP64 path; (float, float) FP32 arithmetic on
+// the stress deviation; (float, double) FP32 storage with FP64 arithmetic on the
+// absolute stress (the deviation is converted on load and store).
+template <bool SHEARH, typename SV, typename AT>
 GF_DEV void rheol_single(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e, int avgsr)
 {
     const unsigned nzm = (unsigned)(NZ - 1);
@@ -938,32 +983,42 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, GfDevClock *ck, cons
     const unsigned eo = (unsigned)e * 8u;                             // element byte offset
     const unsigned no = ((unsigned)(j - 1) + nzu * (unsigned)(i - 1)) * 8u;  // corner 1 byte offset
     const unsigned pe = ne * 8u, pn = nn * 8u, col = nzu * 8u;        // plane and column strides
-    const Buf bS = mkbuf(A.stress0, 16ull * pe), bA = mkbuf(A.area, 4ull * pe), bD = mkbuf(A.dvol, 4ull * pe);
+    // the same offsets for the planes kept in AT (stress, velocity)
+    constexpr unsigned W = sizeof(SV);
+    constexpr bool dev_form = sizeof(AT) == 4;            // arithmetic on the deviation
+    constexpr bool conv = sizeof(SV) == 4 && !dev_form;  // deviation <-> absolute at load/store
+    const unsigned so = (unsigned)e * W, ps = ne * W;
+    const unsigned nov = ((unsigned)(j - 1) + nzu * (unsigned)(i - 1)) * W, pnv = nn * W, colv = nzu * W;
+    const Buf bS = mkbuf(sv_ptr<SV>(A.stress0, A.stress0f), 16ull * ps), bA = mkbuf(A.area, 4ull * pe),
+              bD = mkbuf(A.dvol, 4ull * pe);
     const Buf bSE = mkbuf(A.se2sr, 3ull * pe), bST = mkbuf(A.strain, 3ull * pe);
     const Buf bAPS = mkbuf(A.aps, pe), bVIS = mkbuf(A.visn, pe), bSH = mkbuf(A.sshrheat, pe);
-    const Buf bV = mkbuf(A.vel, 2ull * pn), bC = mkbuf(A.cord, 2ull * pn);
+    const Buf bV = mkbuf(sv_ptr<SV>(A.vel, A.velf), 2ull * pnv), bC = mkbuf(A.cord, 2ull * pn);
     const double dt = ck->dt;
+    const AT dtS = AT(dt);
     double ar[4];
     for (int k = 0; k < 4; k++) ar[k] = bld(bA, eo, k * pe);
-    double sr[4][3];
+    AT sr[4][3];
     {
         // corners 1 (j,i), 2 (j+1,i), 3 (j,i+1), 4 (j+1,i+1)
-        const double vx1 = bld(bV, no, 0), vy1 = bld(bV, no, pn), vx2 = bld(bV, no, 8), vy2 = bld(bV, no, pn + 8);
-        const double vx3 = bld(bV, no, col), vy3 = bld(bV, no, pn + col), vx4 = bld(bV, no, col + 8),
-                     vy4 = bld(bV, no, pn + col + 8);
+        const AT vx1 = AT(bldt<SV>(bV, nov, 0)), vy1 = AT(bldt<SV>(bV, nov, pnv)), vx2 = AT(bldt<SV>(bV, nov, W)),
+                 vy2 = AT(bldt<SV>(bV, nov, pnv + W));
+        const AT vx3 = AT(bldt<SV>(bV, nov, colv)), vy3 = AT(bldt<SV>(bV, nov, pnv + colv)),
+                 vx4 = AT(bldt<SV>(bV, nov, colv + W)), vy4 = AT(bldt<SV>(bV, nov, pnv + colv + W));
         const double x1 = bld(bC, no, 0), y1 = bld(bC, no, pn), x2 = bld(bC, no, 8), y2 = bld(bC, no, pn + 8);
         const double x3 = bld(bC, no, col), y3 = bld(bC, no, pn + col), x4 = bld(bC, no, col + 8),
                      y4 = bld(bC, no, pn + col + 8);
         // per triangle, vertices (v1,v2,v3) in the reference's order: derivatives
-        // (y2-y3, y3-y1, y1-y2)*area and (x3-x2, x1-x3, x2-x1)*area (:98-156)
-        auto tri = [&](int k, double ax, double ay, double avx, double avy, double bx, double by, double bvx,
-                       double bvy, double cx, double cy, double cvx, double cvy) {
+        // (y2-y3, y3-y1, y1-y2)*area and (x3-x2, x1-x3, x2-x1)*area (:98-156), formed
+        // from the FP64 coordinates
+        auto tri = [&](int k, double ax, double ay, AT avx, AT avy, double bx, double by, AT bvx, AT bvy, double cx,
+                       double cy, AT cvx, AT cvy) {
             const double a = ar[k];
-            const double dx0 = (by - cy) * a, dx1 = (cy - ay) * a, dx2 = (ay - by) * a;
-            const double dz0 = (cx - bx) * a, dz1 = (ax - cx) * a, dz2 = (bx - ax) * a;
+            const AT dx0 = AT((by - cy) * a), dx1 = AT((cy - ay) * a), dx2 = AT((ay - by) * a);
+            const AT dz0 = AT((cx - bx) * a), dz1 = AT((ax - cx) * a), dz2 = AT((bx - ax) * a);
             sr[k][0] = avx * dx0 + bvx * dx1 + cvx * dx2;
             sr[k][1] = avy * dz0 + bvy * dz1 + cvy * dz2;
-            sr[k][2] = 0.5 * (avx * dz0 + bvx * dz1 + cvx * dz2 + avy * dx0 + bvy * dx1 + cvy * dx2);
+            sr[k][2] = AT(0.5) * (avx * dz0 + bvx * dz1 + cvx * dz2 + avy * dx0 + bvy * dx1 + cvy * dx2);
         };
         tri(0, x1, y1, vx1, vy1, x2, y2, vx2, vy2, x3, y3, vx3, vy3);
         tri(1, x3, y3, vx3, vy3, x2, y2, vx2, vy2, x4, y4, vx4, vy4);
@@ -972,120 +1027,146 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, GfDevClock *ck, cons
     }
     if (P->mix_strain == 1) {
         for (int k = 0; k < 4; k += 2) {
-            double em = 0.5 * (sr[k][0] + sr[k][1] + sr[k + 1][0] + sr[k + 1][1]);
-            double eda = sr[k][0] - sr[k][1];
-            double edb = sr[k + 1][0] - sr[k + 1][1];
-            sr[k][0] = 0.5 * (em + eda);
-            sr[k][1] = 0.5 * (em - eda);
-            sr[k + 1][0] = 0.5 * (em + edb);
-            sr[k + 1][1] = 0.5 * (em - edb);
+            AT em = AT(0.5) * (sr[k][0] + sr[k][1] + sr[k + 1][0] + sr[k + 1][1]);
+            AT eda = sr[k][0] - sr[k][1];
+            AT edb = sr[k + 1][0] - sr[k + 1][1];
+            sr[k][0] = AT(0.5) * (em + eda);
+            sr[k][1] = AT(0.5) * (em - eda);
+            sr[k + 1][0] = AT(0.5) * (em + edb);
+            sr[k + 1][1] = AT(0.5) * (em - edb);
         }
     }
-    double st[4][4];
+    AT st[4][4];
     for (int k = 0; k < 4; k++)
-        for (int c = 0; c < 4; c++) st[k][c] = bld(bS, eo, (c + 4 * k) * pe);
-    double se1, se2, se3, sshr, s11, s22, s12;
+        for (int c = 0; c < 4; c++) st[k][c] = AT(bldt<SV>(bS, so, (c + 4 * k) * ps));
+    const double pref = ref_press<SV>(A, e);
+    if (conv)
+        for (int k = 0; k < 4; k++) {
+            st[k][0] = st[k][0] - pref;
+            st[k][1] = st[k][1] - pref;
+            st[k][3] = st[k][3] - pref;
+        }
+    double se1, se2, se3, sshr;
+    AT s11, s22, s12;
     {
-        double a11 = 0.25 * (0.0 + sr[0][0] + sr[1][0] + sr[2][0] + sr[3][0]);
-        double a22 = 0.25 * (0.0 + sr[0][1] + sr[1][1] + sr[2][1] + sr[3][1]);
-        double a12 = 0.25 * (0.0 + sr[0][2] + sr[1][2] + sr[2][2] + sr[3][2]);
-        double srII = 0.5 * sqrt((a11 - a22) * (a11 - a22) + 4.0 * a12 * a12);
-        double srI = 0.5 * (a11 + a22);
-        double srs2 = (a11 - srI) * (a11 - srI) + (a22 - srI) * (a22 - srI) + 2.0 * a12 * a12;
+        AT a11 = AT(0.25) * (AT(0.0) + sr[0][0] + sr[1][0] + sr[2][0] + sr[3][0]);
+        AT a22 = AT(0.25) * (AT(0.0) + sr[0][1] + sr[1][1] + sr[2][1] + sr[3][1]);
+        AT a12 = AT(0.25) * (AT(0.0) + sr[0][2] + sr[1][2] + sr[2][2] + sr[3][2]);
+        AT srII = AT(0.5) * sqrt((a11 - a22) * (a11 - a22) + AT(4.0) * a12 * a12);
+        AT srI = AT(0.5) * (a11 + a22);
+        AT srs2 = (a11 - srI) * (a11 - srI) + (a22 - srI) * (a22 - srI) + AT(2.0) * a12 * a12;
+        // the time integrals stay FP64 on both paths
         se1 = bld(bSE, eo, 0) + a11 * dt;
         se2 = bld(bSE, eo, pe) + a22 * dt;
         se3 = bld(bSE, eo, 2 * pe) + a12 * dt;
-        s11 = 0.25 * (0.0 + st[0][0] + st[1][0] + st[2][0] + st[3][0]);
-        s22 = 0.25 * (0.0 + st[0][1] + st[1][1] + st[2][1] + st[3][1]);
-        s12 = 0.25 * (0.0 + st[0][2] + st[1][2] + st[2][2] + st[3][2]);
-        double stII = 0.5 * sqrt((s11 - s22) * (s11 - s22) + 4.0 * s12 * s12);
+        s11 = AT(0.25) * (AT(0.0) + st[0][0] + st[1][0] + st[2][0] + st[3][0]);
+        s22 = AT(0.25) * (AT(0.0) + st[0][1] + st[1][1] + st[2][1] + st[3][1]);
+        s12 = AT(0.25) * (AT(0.0) + st[0][2] + st[1][2] + st[2][2] + st[3][2]);
+        AT stII = AT(0.5) * sqrt((s11 - s22) * (s11 - s22) + AT(4.0) * s12 * s12);
         sshr = bld(bSH, eo, 0);
-        if (srII != 0.0) sshr = sshr + stII / srII * srs2 * dt;
+        if (srII != AT(0.0)) sshr = sshr + stII / srII * srs2 * dt;
     }
     const int iph = A.iphase[e];
     const GfPhase &ph = PHASE(iph);
     const int irh = ph.irheol;
-    const double bulkm = ph.bulkm, rmu = ph.rm;
-    double stherm = 0.0;
-    if (P->istress_therm > 0) stherm = -ph.alfa * bulkm * (A.temp[IN(j, i)] - A.temp0[IN(j, i)]);
+    const AT bulkm = AT(ph.bulkm), rmu = AT(ph.rm);
+    AT stherm = AT(0.0);
+    if (P->istress_therm > 0) stherm = AT(-ph.alfa * ph.bulkm * (A.temp[IN(j, i)] - A.temp0[IN(j, i)]));
     double aps = bld(bAPS, eo, 0);
     const double vis = bld(bVIS, eo, 0);  // refreshed by k_visc on viscosity-update steps
     const bool plas_law = irh == 6 || irh >= 11;
-    PlasticConst pc{};
-    double hardn = 0;
+    PlasticConstT<AT> pc{};
+    PlasticDbl pd{0.0, 0.0, 0.0, dev_form ? pref : 0.0};
+    AT hardn = AT(0);
     if (plas_law) {
-        double prv[GF_MAXACT], coh, phi, psi;
+        double prv[GF_MAXACT], coh, phi, psi, hd;
         const Buf bPR = mkbuf(A.pr, (unsigned long long)P->nactive * pe);
         for (int a = 0; a < P->nactive; a++) prv[a] = bld(bPR, eo, a * pe);
-        dev_pre_plast(P, aps, prv, coh, phi, psi, hardn);
+        dev_pre_plast(P, aps, prv, coh, phi, psi, hd);
         double sphi, spsi, tphi;
         if (phi == ph.phi0 && psi == ph.psi0) {
             sphi = ph.sphi0;
             spsi = ph.spsi0;
             tphi = ph.tphi0;
-        } else {
+        } else if (!dev_form) {
             cr_sin_sin_tan(phi, psi, &sphi, &spsi, &tphi);
+        } else {  // mixed precision: the device libm's double sin/tan
+            const double degrad = 3.14159265358979323846 / 180.0;
+            sphi = sin(phi * degrad);
+            spsi = sin(psi * degrad);
+            tphi = phi == 0.0 ? 0.0 : tan(phi * degrad);
         }
+        // src/rh_plastic.f90:29-46 in double, then in the stress arithmetic's precision
         const double c4d3 = 4.0 / 3.0, c2d3 = 2.0 / 3.0;
-        pc.anphi = (1.0 + sphi) / (1.0 - sphi);
-        pc.anpsi = (1.0 + spsi) / (1.0 - spsi);
-        pc.amc = 2.0 * coh * sqrt(pc.anphi);
-        pc.e1 = bulkm + c4d3 * rmu;
-        pc.e2 = bulkm - c2d3 * rmu;
-        pc.x1 = (pc.e1 - pc.e2 * pc.anpsi + pc.e1 * pc.anphi * pc.anpsi - pc.e2 * pc.anphi);
-        pc.ten_max = phi == 0.0 ? P->ten_off : fmin(P->ten_off, coh / tphi);
+        const double anphi = (1.0 + sphi) / (1.0 - sphi);
+        const double anpsi = (1.0 + spsi) / (1.0 - spsi);
+        const double e1 = ph.bulkm + c4d3 * ph.rm;
+        const double e2 = ph.bulkm - c2d3 * ph.rm;
+        pc.anphi = AT(anphi);
+        pc.anpsi = AT(anpsi);
+        pc.amc = AT(2.0 * coh * sqrt(anphi));
+        pc.e1 = AT(e1);
+        pc.e2 = AT(e2);
+        pc.x1 = AT((e1 - e2 * anpsi + e1 * anphi * anpsi - e2 * anphi));
+        pc.ten_max = AT(phi == 0.0 ? P->ten_off : fmin(P->ten_off, coh / tphi));
+        pd.anphi = anphi;
+        pd.amc = 2.0 * coh * sqrt(anphi);
+        pd.ten_max = phi == 0.0 ? P->ten_off : fmin(P->ten_off, coh / tphi);
+        hardn = AT(hd);
     }
-    double s_start[SHEARH ? 4 : 1][4];
+    AT s_start[SHEARH ? 4 : 1][4];
     if (SHEARH)
         for (int k = 0; k < 4; k++)
             for (int c = 0; c < 4; c++) s_start[k][c] = st[k][c];
-    double depl[4] = {0, 0, 0, 0};
+    AT depl[4] = {AT(0), AT(0), AT(0), AT(0)};
     bool ok = true, plastic_branch = true;
     if (irh >= 11) {
-        double s11p = 0, s12p = 0, s11v = 0, s12v = 0, s22p[4], s22v[4];
+        AT s11p = 0, s12p = 0, s11v = 0, s12v = 0, s22p[4], s22v[4];
         for (int k = 0; k < 4; k++) {
-            double o[4];
-            const double dv = bld(bD, eo, k * pe) - 0.0;
-            rheol_tri(irh, true, pc, bulkm, rmu, hardn, vis, dt, stherm, st[k], sr[k], dv, o, depl[k]);
+            AT o[4];
+            const AT dv = AT(bld(bD, eo, k * pe) - 0.0);
+            rheol_tri<AT>(irh, true, pc, pd, bulkm, rmu, hardn, ph.rm, vis, dt, dtS, stherm, st[k], sr[k], dv, o, depl[k]);
             s11p = k == 0 ? o[0] : s11p + o[0];
             s12p = k == 0 ? o[2] : s12p + o[2];
             s22p[k] = o[1];
-            double dm;
-            ok &= rheol_tri(irh, false, pc, bulkm, rmu, hardn, vis, dt, stherm, st[k], sr[k], dv, o, dm);
+            AT dm;
+            ok &= rheol_tri<AT>(irh, false, pc, pd, bulkm, rmu, hardn, ph.rm, vis, dt, dtS, stherm, st[k], sr[k], dv, o, dm);
             s11v = k == 0 ? o[0] : s11v + o[0];
             s12v = k == 0 ? o[2] : s12v + o[2];
             s22v[k] = o[1];
         }
-        double a = s11p - s22p[0] - s22p[1] - s22p[2] - s22p[3];
-        double sII_plas = a * a + 4.0 * (s12p * s12p);
-        double av = s11v - s22v[0] - s22v[1] - s22v[2] - s22v[3];
-        double sII_visc = av * av + 4.0 * (s12v * s12v);
+        AT a = s11p - s22p[0] - s22p[1] - s22p[2] - s22p[3];
+        AT sII_plas = a * a + AT(4.0) * (s12p * s12p);
+        AT av = s11v - s22v[0] - s22v[1] - s22v[2] - s22v[3];
+        AT sII_visc = av * av + AT(4.0) * (s12v * s12v);
         plastic_branch = sII_plas < sII_visc;
     }
     if (irh == 1 || irh == 3 || irh == 6 || irh >= 11) {
         for (int k = 0; k < 4; k++) {
-            double o[4], dm;
-            const double dv = bld(bD, eo, k * pe) - 0.0;
-            bool okk = rheol_tri(irh, plastic_branch, pc, bulkm, rmu, hardn, vis, dt, stherm, st[k], sr[k], dv, o,
-                                 irh == 6 ? depl[k] : dm);
+            AT o[4], dm = AT(0);
+            const AT dv = AT(bld(bD, eo, k * pe) - 0.0);
+            bool okk = rheol_tri<AT>(irh, plastic_branch, pc, pd, bulkm, rmu, hardn, ph.rm, vis, dt, dtS, stherm, st[k],
+                                     sr[k], dv, o, dm);
+            if (irh == 6) depl[k] = dm;
             if (irh == 3) ok &= okk;
             for (int c = 0; c < 4; c++) st[k][c] = o[c];
         }
     }
     if (!ok) raise_error(ck, GF_STOP_MAXWELL, e);
     if (P->mix_stress == 1) {
-        double quad_area = 1.0 / ar[0] + 1.0 / ar[1];
-        double s0a = 0.5 * (st[0][0] + st[0][1]);
-        double s0b = 0.5 * (st[1][0] + st[1][1]);
-        double s0 = (s0a / ar[1] + s0b / ar[0]) / quad_area;
+        const AT a0 = AT(ar[0]), a1 = AT(ar[1]), a2 = AT(ar[2]), a3 = AT(ar[3]);
+        AT quad_area = AT(1.0) / a0 + AT(1.0) / a1;
+        AT s0a = AT(0.5) * (st[0][0] + st[0][1]);
+        AT s0b = AT(0.5) * (st[1][0] + st[1][1]);
+        AT s0 = (s0a / a1 + s0b / a0) / quad_area;
         st[0][0] = st[0][0] - s0a + s0;
         st[0][1] = st[0][1] - s0a + s0;
         st[1][0] = st[1][0] - s0b + s0;
         st[1][1] = st[1][1] - s0b + s0;
-        quad_area = 1.0 / ar[2] + 1.0 / ar[3];
-        s0a = 0.5 * (st[2][0] + st[2][1]);
-        s0b = 0.5 * (st[3][0] + st[3][1]);
-        s0 = (s0a / ar[3] + s0b / ar[2]) / quad_area;
+        quad_area = AT(1.0) / a2 + AT(1.0) / a3;
+        s0a = AT(0.5) * (st[2][0] + st[2][1]);
+        s0b = AT(0.5) * (st[3][0] + st[3][1]);
+        s0 = (s0a / a3 + s0b / a2) / quad_area;
         st[2][0] = st[2][0] - s0a + s0;
         st[2][1] = st[2][1] - s0a + s0;
         st[3][0] = st[3][0] - s0b + s0;
@@ -1097,7 +1178,11 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, GfDevClock *ck, cons
         st[k][3] = st[k][3] - stherm;
     }
     for (int k = 0; k < 4; k++)
-        for (int c = 0; c < 4; c++) bst(bS, eo, (c + 4 * k) * pe, st[k][c]);
+        for (int c = 0; c < 4; c++) {
+            AT v = st[k][c];
+            if (conv && c != 2) v = v + pref;
+            bstt(bS, so, (c + 4 * k) * ps, SV(v));
+        }
     if (plas_law) {
         double quad_area = ar[0] + ar[1] + ar[2] + ar[3];
         aps = aps + (depl[0] * ar[0] + depl[1] * ar[1] + depl[2] * ar[2] + depl[3] * ar[3]) / quad_area;
@@ -1106,38 +1191,43 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, GfDevClock *ck, cons
         bst(bAPS, eo, 0, aps);
     }
     // the reference adds the averaged (step-start) STRESS times dt: s11..s12 were reused
-    bst(bST, eo, 0, bld(bST, eo, 0) + s11 * dt);
-    bst(bST, eo, pe, bld(bST, eo, pe) + s22 * dt);
+    // (mixed precision: the absolute stress, deviation - pref)
+    bst(bST, eo, 0, bld(bST, eo, 0) + (dev_form ? (double)s11 - pd.pref : (double)s11) * dt);
+    bst(bST, eo, pe, bld(bST, eo, pe) + (dev_form ? (double)s22 - pd.pref : (double)s22) * dt);
     bst(bST, eo, 2 * pe, bld(bST, eo, 2 * pe) + s12 * dt);
     if (SHEARH) {
-        double q_shear_elem = 0.0;
+        AT q_shear_elem = AT(0.0);
+        double dsum = 0.0;
         for (int k = 0; k < 4; k++) {
-            double lambda = bulkm - 2.0 / 3.0 * rmu, mu = rmu;
-            double denom = 3.0 * lambda + 2.0 * mu;
-            if (fabs(denom) < 1.0e-30) denom = fsign(1.0e-30, denom);
-            const double *s0 = s_start[SHEARH ? k : 0];
-            double trs = s0[0] + s0[1] + s0[3];
-            double e11s = (s0[0] - (lambda / denom) * trs) / (2.0 * mu);
-            double e22s = (s0[1] - (lambda / denom) * trs) / (2.0 * mu);
-            double e33s = (s0[3] - (lambda / denom) * trs) / (2.0 * mu);
-            double e12s = s0[2] / (2.0 * mu);
-            double trf = st[k][0] + st[k][1] + st[k][3];
-            double e11f = (st[k][0] - (lambda / denom) * trf) / (2.0 * mu);
-            double e22f = (st[k][1] - (lambda / denom) * trf) / (2.0 * mu);
-            double e33f = (st[k][3] - (lambda / denom) * trf) / (2.0 * mu);
-            double e12f = st[k][2] / (2.0 * mu);
-            double d11n = sr[k][0] * dt - (e11f - e11s);
-            double d22n = sr[k][1] * dt - (e22f - e22s);
-            double d12n = sr[k][2] * dt - (e12f - e12s);
-            double d33n = 0.0 - (e33f - e33s);
-            double a11 = 0.5 * (s0[0] + st[k][0]);
-            double a22 = 0.5 * (s0[1] + st[k][1]);
-            double a12 = 0.5 * (s0[2] + st[k][2]);
-            double a33 = 0.5 * (s0[3] + st[k][3]);
-            double q = a11 * d11n + a22 * d22n + 2.0 * a12 * d12n + a33 * d33n;
-            q_shear_elem = q_shear_elem + 0.25 * q;
+            AT lambda = bulkm - AT(2.0) / AT(3.0) * rmu, mu = rmu;
+            AT denom = AT(3.0) * lambda + AT(2.0) * mu;
+            if (fabs(denom) < AT(1.0e-30)) denom = denom < AT(0.0) ? -AT(1.0e-30) : AT(1.0e-30);
+            const AT *s0 = s_start[SHEARH ? k : 0];
+            AT trs = s0[0] + s0[1] + s0[3];
+            AT e11s = (s0[0] - (lambda / denom) * trs) / (AT(2.0) * mu);
+            AT e22s = (s0[1] - (lambda / denom) * trs) / (AT(2.0) * mu);
+            AT e33s = (s0[3] - (lambda / denom) * trs) / (AT(2.0) * mu);
+            AT e12s = s0[2] / (AT(2.0) * mu);
+            AT trf = st[k][0] + st[k][1] + st[k][3];
+            AT e11f = (st[k][0] - (lambda / denom) * trf) / (AT(2.0) * mu);
+            AT e22f = (st[k][1] - (lambda / denom) * trf) / (AT(2.0) * mu);
+            AT e33f = (st[k][3] - (lambda / denom) * trf) / (AT(2.0) * mu);
+            AT e12f = st[k][2] / (AT(2.0) * mu);
+            AT d11n = sr[k][0] * dtS - (e11f - e11s);
+            AT d22n = sr[k][1] * dtS - (e22f - e22s);
+            AT d12n = sr[k][2] * dtS - (e12f - e12s);
+            AT d33n = AT(0.0) - (e33f - e33s);
+            AT a11 = AT(0.5) * (s0[0] + st[k][0]);
+            AT a22 = AT(0.5) * (s0[1] + st[k][1]);
+            AT a12 = AT(0.5) * (s0[2] + st[k][2]);
+            AT a33 = AT(0.5) * (s0[3] + st[k][3]);
+            AT q = a11 * d11n + a22 * d22n + AT(2.0) * a12 * d12n + a33 * d33n;
+            q_shear_elem = q_shear_elem + AT(0.25) * q;
+            if (dev_form) dsum = dsum + (double)(d11n + d22n + d33n);
         }
         sshr = sshr + q_shear_elem;
+        // mixed precision: the isotropic reference's share, -pref * (d11 + d22 + d33)
+        if (dev_form) sshr = sshr - 0.25 * pd.pref * dsum;
     }
     // periodic refresh of the time-averaged strain rate, src/fl_rheol.f90:395-421
     if (avgsr != 0) {
@@ -1161,12 +1251,49 @@ __global__ void __launch_bounds__(256) k_rheol1(const GfDevParams *__restrict__ 
 {
     long e = xcd_lin();
     if (e >= NE) return;
-    rheol_single<SHEARH>(P, ck, A, e, avgsr);
+    rheol_single<SHEARH, double, double>(P, ck, A, e, avgsr);
+}
+// mixed precision: FP32 strain rate and stress
+template <bool SHEARH>
+__global__ void __launch_bounds__(256) k_rheol1m(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
+                                                 int avgsr)
+{
+    long e = xcd_lin();
+    if (e >= NE) return;
+    rheol_single<SHEARH, float, float>(P, ck, A, e, avgsr);
+}
+// FP32 storage, FP64 arithmetic
+template <bool SHEARH>
+__global__ void __launch_bounds__(256) k_rheol1s(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
+                                                 int avgsr)
+{
+    long e = xcd_lin();
+    if (e >= NE) return;
+    rheol_single<SHEARH, float, double>(P, ck, A, e, avgsr);
 }
 
 // Eff_visc of src/fl_rheol.f90:203-206 on viscosity-update steps (mod(nloop,
 // ifreq_visc) == 0), for the Maxwell rheologies, from the step-start stress, before
 // k_rheol: the same inputs the reference's in-loop call sees.
+// triangle-averaged stress components 1, 2 and 4 of element e, as Eff_visc forms them
+// (src/fl_rheol.f90:203-206), from the stress planes of precision SV
+template <typename SV>
+GF_DEV void stress_avg(const GfDevParams *P, const GfDevArrays &A, long e, double &a11, double &a22, double &a33)
+{
+    const SV *s = sv_ptr<SV>(A.stress0, A.stress0f);
+    auto S = [&](int c, int k) { return (double)s[e + NE * ((c - 1) + 4 * (k - 1))]; };
+    a11 = 0.25 * (S(1, 1) + S(1, 2) + S(1, 3) + S(1, 4));
+    a22 = 0.25 * (S(2, 1) + S(2, 2) + S(2, 3) + S(2, 4));
+    a33 = 0.25 * (S(4, 1) + S(4, 2) + S(4, 3) + S(4, 4));
+    if (sizeof(SV) == 4) {  // absolute stress = deviation - pref (normal components)
+        const double pr = ref_press<SV>(A, e);
+        a11 = a11 - pr;
+        a22 = a22 - pr;
+        a33 = a33 - pr;
+    }
+}
+
+template <typename SV>
 __global__ void __launch_bounds__(256) k_visc(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
 {
     long e = xcd_lin();
@@ -1176,9 +1303,8 @@ __global__ void __launch_bounds__(256) k_visc(const GfDevParams *__restrict__ P,
     const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
     double prv[GF_MAXACT];
     for (int a = 0; a < P->nactive; a++) prv[a] = A.pr[a * NE + e];
-    double a11 = 0.25 * (STRESS(j, i, 1, 1) + STRESS(j, i, 1, 2) + STRESS(j, i, 1, 3) + STRESS(j, i, 1, 4));
-    double a22 = 0.25 * (STRESS(j, i, 2, 1) + STRESS(j, i, 2, 2) + STRESS(j, i, 2, 3) + STRESS(j, i, 2, 4));
-    double a33 = 0.25 * (STRESS(j, i, 4, 1) + STRESS(j, i, 4, 2) + STRESS(j, i, 4, 3) + STRESS(j, i, 4, 4));
+    double a11, a22, a33;
+    stress_avg<SV>(P, A, e, a11, a22, a33);
     A.visn[e] = dev_eff_visc(P, A, j, i, a11, a22, a33, prv);
 }
 
@@ -1196,8 +1322,12 @@ __global__ void __launch_bounds__(256) k_rheol(const GfDevParams *__restrict__ P
 
 // ------------------------------------------------------------------ fl_node
 // force of triangle k (1..4) of element (je,ie) on its local vertex m (1..3)
+// Mode 1 (FP32 arithmetic): fx, fy are the FP32 forces of the stress deviation, and
+// rx, ry (FP64) add those of the element's isotropic reference -pref*I.  Mode 2: the
+// stored deviation is turned back into the absolute stress, in FP64.
+template <int M>
 GF_DEV void dev_tri_force(const GfDevParams *P, const GfDevArrays &A, int je, int ie, int k, int m,
-                          double &fx, double &fy)
+                          typename Prec<M>::AT &fx, typename Prec<M>::AT &fy, double &rx, double &ry)
 {
     const double x1 = CORD(je, ie, 1), y1 = CORD(je, ie, 2), x2 = CORD(je + 1, ie, 1), y2 = CORD(je + 1, ie, 2);
     const double x3 = CORD(je, ie + 1, 1), y3 = CORD(je, ie + 1, 2), x4 = CORD(je + 1, ie + 1, 1),
@@ -1226,26 +1356,48 @@ GF_DEV void dev_tri_force(const GfDevParams *P, const GfDevArrays &A, int je, in
         if (m == 3) { dx = (y1 - y4) * a; dz = (x4 - x1) * a; }
         break;
     }
-    const double factor = 0.25 / a;
-    fx = factor * (STRESS(je, ie, 1, k) * dx + STRESS(je, ie, 3, k) * dz);
-    fy = factor * (STRESS(je, ie, 3, k) * dx + STRESS(je, ie, 2, k) * dz);
+    typedef typename Prec<M>::SV SV;
+    typedef typename Prec<M>::AT AT;
+    const AT factor = AT(0.25 / a);
+    const SV *s = sv_ptr<SV>(A.stress0, A.stress0f);
+    const long ix = IE(je, ie) + NE * 4 * (k - 1);
+    AT s1 = s[ix], s2 = s[ix + NE], s3 = s[ix + 2 * NE];
+    if (M == 2) {
+        const double pr = ref_press<SV>(A, IE(je, ie));
+        s1 = s1 - pr;
+        s2 = s2 - pr;
+    }
+    fx = factor * (s1 * AT(dx) + s3 * AT(dz));
+    fy = factor * (s3 * AT(dx) + s2 * AT(dz));
+    if (M == 1) {
+        const double pr = ref_press<SV>(A, IE(je, ie));
+        rx = -(0.25 / a) * pr * dx;
+        ry = -(0.25 / a) * pr * dz;
+    }
 }
 
 // src/fl_node.f90:36-391, one thread per node: gather of the 12 triangle forces in
 // the reference's order (kk/mm tables of :7-8), gravity, optional boundary loads,
 // damping and the velocity update.
+// The force and velocity arithmetic runs in SV (double: the reference's, bit for bit;
+// float: mixed precision); geometry is formed in double from the FP64 coordinates.
+template <int M>
 __global__ void __launch_bounds__(256) k_node(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
 {
+    typedef typename Prec<M>::SV SV;
+    typedef typename Prec<M>::AT AT;
     long t = xcd_lin();
     if (t >= NN) return;
     const int j = (int)(t % NZ) + 1, i = (int)(t / NZ) + 1;
+    SV *force = sv_ptr<SV>(A.force, A.forcef), *vel = sv_ptr<SV>(A.vel, A.velf);
     const double dt = ck->dt;
     double drat = dt / ck->dt_elastic;
     if (drat < 1.0) drat = 1.0;
-    double f1 = 0, f2 = 0;
+    AT g1 = 0, g2 = 0;      // triangle forces in AT
+    double r1 = 0, r2 = 0;  // mixed precision: forces of the isotropic references
     if (P->nystressbc != 0) {
-        f1 = A.force[t];
-        f2 = A.force[t + NN];
+        g1 = force[t];
+        g2 = force[t + NN];
     }
     const int kk[4][3] = {{2, 3, 4}, {1, 2, 3}, {1, 2, 4}, {1, 3, 4}};
     const int mm[4][3] = {{3, 3, 2}, {2, 2, 2}, {3, 1, 3}, {1, 1, 1}};
@@ -1256,11 +1408,22 @@ __global__ void __launch_bounds__(256) k_node(const GfDevParams *__restrict__ P,
         if (!okk[el]) continue;
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            double fx, fy;
-            dev_tri_force(P, A, ej[el], ei[el], kk[el][k], mm[el][k], fx, fy);
-            f1 = f1 - fx;
-            f2 = f2 - fy;
+            AT fx, fy;
+            double rx = 0, ry = 0;
+            dev_tri_force<M>(P, A, ej[el], ei[el], kk[el][k], mm[el][k], fx, fy, rx, ry);
+            g1 = g1 - fx;
+            g2 = g2 - fy;
+            if (M == 1) {
+                r1 = r1 - rx;
+                r2 = r2 - ry;
+            }
         }
+    }
+    // the rest in double on both paths (for FP64, f = the sums above unchanged)
+    double f1 = g1, f2 = g2;
+    if (M == 1) {
+        f1 = f1 + r1;
+        f2 = f2 + r2;
     }
     f2 = f2 - A.rmass[t] * P->g;
     if (P->nyhydro > 0 && (j == 1 || j == NZ)) {
@@ -1319,7 +1482,7 @@ __global__ void __launch_bounds__(256) k_node(const GfDevParams *__restrict__ P,
         }
     }
     const int nc1 = A.ncod[t], nc2 = A.ncod[t + NN];
-    double v1 = A.vel[t], v2 = A.vel[t + NN];
+    double v1 = vel[t], v2 = vel[t + NN];
     if ((nc1 & 1) != 1 && fabs(v1) > 1.0e-13) f1 = f1 - P->demf * fsign(f1, v1);
     if ((nc2 & 2) != 2 && fabs(v2) > 1.0e-13) f2 = f2 - P->demf * fsign(f2, v2);
     const double am = A.amass[t];
@@ -1335,27 +1498,37 @@ __global__ void __launch_bounds__(256) k_node(const GfDevParams *__restrict__ P,
         v1 = 0.0;
         v2 = 0.0;
     }
-    A.force[t] = f1;
-    A.force[t + NN] = f2;
-    A.vel[t] = v1;
-    A.vel[t + NN] = v2;
+    force[t] = SV(f1);
+    force[t + NN] = SV(f2);
+    vel[t] = SV(v1);
+    vel[t + NN] = SV(v2);
 }
 
 // ------------------------------------------------------------------ fl_move
+// coordinates stay FP64 on both paths
+template <typename SV>
 __global__ void __launch_bounds__(256) k_move_cord(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
 {
     long t = xcd_lin();
     if (t >= NN) return;
+    const SV *vel = sv_ptr<SV>(A.vel, A.velf);
     const double dt = ck->dt;
-    A.cord[t] = A.cord[t] + A.vel[t] * dt;
-    A.cord[t + NN] = A.cord[t + NN] + A.vel[t + NN] * dt;
+    A.cord[t] = A.cord[t] + (double)vel[t] * dt;
+    A.cord[t + NN] = A.cord[t + NN] + (double)vel[t + NN] * dt;
 }
 
 // src/fl_move.f90:47-128
+// areas, dvol and strain stay FP64; the stress rotation runs in AT (it leaves the
+// isotropic reference unchanged, so the stored deviation rotates by itself)
+template <int M>
 __global__ void __launch_bounds__(256) k_move_elem(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
 {
+    typedef typename Prec<M>::SV SV;
+    typedef typename Prec<M>::AT AT;
     long e = xcd_lin();
     if (e >= NE) return;
+    const SV *vel = sv_ptr<SV>(A.vel, A.velf);
+    SV *stress = sv_ptr<SV>(A.stress0, A.stress0f);
     const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
     const double dt = ck->dt;
     double xc[5], yc[5], xv[5], yv[5];
@@ -1363,10 +1536,12 @@ __global__ void __launch_bounds__(256) k_move_elem(const GfDevParams *__restrict
     xc[2] = CORD(j + 1, i, 1); yc[2] = CORD(j + 1, i, 2);
     xc[3] = CORD(j, i + 1, 1); yc[3] = CORD(j, i + 1, 2);
     xc[4] = CORD(j + 1, i + 1, 1); yc[4] = CORD(j + 1, i + 1, 2);
-    xv[1] = VEL(j, i, 1); yv[1] = VEL(j, i, 2);
-    xv[2] = VEL(j + 1, i, 1); yv[2] = VEL(j + 1, i, 2);
-    xv[3] = VEL(j, i + 1, 1); yv[3] = VEL(j, i + 1, 2);
-    xv[4] = VEL(j + 1, i + 1, 1); yv[4] = VEL(j + 1, i + 1, 2);
+#define VELS(jj, ii, d) (double)vel[IN(jj, ii) + NN * ((d) - 1)]
+    xv[1] = VELS(j, i, 1); yv[1] = VELS(j, i, 2);
+    xv[2] = VELS(j + 1, i, 1); yv[2] = VELS(j + 1, i, 2);
+    xv[3] = VELS(j, i + 1, 1); yv[3] = VELS(j, i + 1, 2);
+    xv[4] = VELS(j + 1, i + 1, 1); yv[4] = VELS(j + 1, i + 1, 2);
+#undef VELS
     const int nd1[4] = {1, 3, 1, 1}, nd2[4] = {2, 2, 2, 4}, nd3[4] = {3, 4, 4, 3};
     bool bad = false;
 #pragma unroll
@@ -1382,16 +1557,17 @@ __global__ void __launch_bounds__(256) k_move_elem(const GfDevParams *__restrict
                       (xv[a] * (xc[c] - xc[b]) + xv[b] * (xc[a] - xc[c]) + xv[c] * (xc[b] - xc[a]) +
                        yv[a] * (yc[c] - yc[b]) + yv[b] * (yc[a] - yc[c]) + yv[c] * (yc[b] - yc[a])) /
                       det * dt;
-        double s11 = A.stress0[e + NE * (0 + 4 * k)];
-        double s22 = A.stress0[e + NE * (1 + 4 * k)];
-        double s12 = A.stress0[e + NE * (2 + 4 * k)];
-        A.stress0[e + NE * (0 + 4 * k)] = s11 + s12 * 2 * dw12;
-        A.stress0[e + NE * (1 + 4 * k)] = s22 - s12 * 2 * dw12;
-        A.stress0[e + NE * (2 + 4 * k)] = s12 + dw12 * (s22 - s11);
+        const AT w = AT(dw12);
+        const AT t11 = stress[e + NE * (0 + 4 * k)];
+        const AT t22 = stress[e + NE * (1 + 4 * k)];
+        const AT t12 = stress[e + NE * (2 + 4 * k)];
+        stress[e + NE * (0 + 4 * k)] = SV(t11 + t12 * 2 * w);
+        stress[e + NE * (1 + 4 * k)] = SV(t22 - t12 * 2 * w);
+        stress[e + NE * (2 + 4 * k)] = SV(t12 + w * (t22 - t11));
         if (k == 0) {
-            s11 = A.strain[e];
-            s22 = A.strain[e + NE];
-            s12 = A.strain[e + 2 * NE];
+            const double s11 = A.strain[e];
+            const double s22 = A.strain[e + NE];
+            const double s12 = A.strain[e + 2 * NE];
             A.strain[e] = s11 + s12 * 2 * dw12;
             A.strain[e + NE] = s22 - s12 * 2 * dw12;
             A.strain[e + 2 * NE] = s12 + dw12 * (s22 - s11);
@@ -1422,6 +1598,7 @@ __global__ void __launch_bounds__(256) k_init_areas(const GfDevParams *__restric
 }
 
 // src/init_visc.f90:1-18
+template <typename SV>
 __global__ void __launch_bounds__(256) k_init_visc(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
 {
     long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1429,14 +1606,14 @@ __global__ void __launch_bounds__(256) k_init_visc(const GfDevParams *__restrict
     const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
     double prv[GF_MAXACT];
     for (int a = 0; a < P->nactive; a++) prv[a] = A.pr[a * NE + e];
-    double a11 = 0.25 * (STRESS(j, i, 1, 1) + STRESS(j, i, 1, 2) + STRESS(j, i, 1, 3) + STRESS(j, i, 1, 4));
-    double a22 = 0.25 * (STRESS(j, i, 2, 1) + STRESS(j, i, 2, 2) + STRESS(j, i, 2, 3) + STRESS(j, i, 2, 4));
-    double a33 = 0.25 * (STRESS(j, i, 4, 1) + STRESS(j, i, 4, 2) + STRESS(j, i, 4, 3) + STRESS(j, i, 4, 4));
+    double a11, a22, a33;
+    stress_avg<SV>(P, A, e, a11, a22, a33);
     A.visn[e] = dev_eff_visc(P, A, j, i, a11, a22, a33, prv);
 }
 
 // ------------------------------------------------------------------ rmasses
 // src/matprops.f90:3-83 (Eff_dens) per element into a scratch plane
+template <typename SV>
 __global__ void __launch_bounds__(256) k_dens(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
 {
     long e = xcd_lin();
@@ -1464,7 +1641,11 @@ __global__ void __launch_bounds__(256) k_dens(const GfDevParams *__restrict__ P,
         den_amp = 1.05 + amp660 * fr;
     }
     double press = 0;
-    for (int k = 1; k <= 4; k++) press = press - (STRESS(j, i, 1, k) + STRESS(j, i, 2, k) + STRESS(j, i, 4, k));
+    const SV *sg = sv_ptr<SV>(A.stress0, A.stress0f);
+    for (int k = 1; k <= 4; k++)
+        press = press - ((double)sg[e + NE * (4 * (k - 1))] + (double)sg[e + NE * (1 + 4 * (k - 1))] +
+                         (double)sg[e + NE * (3 + 4 * (k - 1))]);
+    if (sizeof(SV) == 4) press = press + 12.0 * ref_press<SV>(A, e);
     press = press / 12;
     double ed = 0.0;
     for (int a = 0; a < P->nactive; a++) {
@@ -1551,15 +1732,17 @@ GF_DEV double elem_dlmin(const GfDevParams *P, const GfDevArrays &A, long t)
 }
 
 #define DT_RED_BLOCKS 1024
+template <typename SV>
 __global__ void __launch_bounds__(256) k_dt_reduce(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
 {
+    const SV *vel = sv_ptr<SV>(A.vel, A.velf);
     __shared__ double s_min[4], s_max[4];
     const long stride = (long)gridDim.x * blockDim.x;
     double dlmin = 1.0e28, vmax = 0.0;
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < NE; t += stride)
         dlmin = fmin(dlmin, elem_dlmin(P, A, t));
     for (long n = (long)blockIdx.x * blockDim.x + threadIdx.x; n < 2 * NN; n += stride)
-        vmax = fmax(vmax, fabs(A.vel[n]));
+        vmax = fmax(vmax, fabs((double)vel[n]));
     for (int off = 32; off > 0; off >>= 1) {
         dlmin = fmin(dlmin, __shfl_xor(dlmin, off));
         vmax = fmax(vmax, __shfl_xor(vmax, off));

@@ -36,6 +36,7 @@ struct gf_ctx {
     int32_t nloop = 0, nsrate = 0;
     long ne = 0, nn = 0;
     std::vector<double> phase_ratio_host;  // full nphase layout, as uploaded
+    std::vector<double> pref_host;         // mixed precision: the stress references, as uploaded
     int err_j = 0, err_i = 0;
     char msg[512] = {0};
     bool rheol_quad = false;  // four lanes per element (GF_RHEOL=quad) instead of one
@@ -159,7 +160,8 @@ extern "C" int gf_create(const gf_params *p, int32_t precision, gf_ctx **out)
     if (!p || !out) return GF_ERR_ARG;
     *out = nullptr;
     if (p->nx < 3 || p->nz < 3 || p->nphase < 1 || p->nphase > GF_MAXPH) return GF_ERR_ARG;
-    if (precision != GF_PREC_FP64) return GF_ERR_UNSUPPORTED;
+    if (precision != GF_PREC_FP64 && precision != GF_PREC_MIXED && precision != GF_PREC_MIXED_STORAGE)
+        return GF_ERR_UNSUPPORTED;
     // paths of the reference this build does not provide (see the header)
     if (p->itype_melting != 0 || p->nystressbc != 0 || p->ndim != 2) return GF_ERR_UNSUPPORTED;
     if (p->inhom < 0 || p->inhom > GF_MAXINH) return GF_ERR_ARG;
@@ -195,15 +197,23 @@ extern "C" int gf_create(const gf_params *p, int32_t precision, gf_ctx **out)
     ALLOC(ctx->dck, sizeof(GfDevClock));
     ALLOC(ctx->d_present, sizeof(int) * GF_MAXPH);
     ALLOC(ctx->a.cord, 8 * nn * 2);
-    ALLOC(ctx->a.vel, 8 * nn * 2);
+    // stress, velocity and force in the path's precision (FP32 planes when mixed)
+    if (precision != GF_PREC_FP64) {
+        ALLOC(ctx->a.velf, 4 * nn * 2);
+        ALLOC(ctx->a.forcef, 4 * nn * 2);
+        ALLOC(ctx->a.stress0f, 4 * ne * 16);
+        ALLOC(ctx->a.pref, 8 * ne);
+    } else {
+        ALLOC(ctx->a.vel, 8 * nn * 2);
+        ALLOC(ctx->a.force, 8 * nn * 2);
+        ALLOC(ctx->a.stress0, 8 * ne * 16);
+    }
     ALLOC(ctx->a.temp, 8 * nn);
     ALLOC(ctx->a.temp0, 8 * nn);
-    ALLOC(ctx->a.force, 8 * nn * 2);
     ALLOC(ctx->a.amass, 8 * nn);
     ALLOC(ctx->a.rmass, 8 * nn);
     ALLOC(ctx->a.bc, 8 * nn * 2);
     ALLOC(ctx->a.ncod, 4 * nn * 2);
-    ALLOC(ctx->a.stress0, 8 * ne * 16);
     ALLOC(ctx->a.area, 8 * ne * 4);
     ALLOC(ctx->a.dvol, 8 * ne * 4);
     ALLOC(ctx->a.strain, 8 * ne * 3);
@@ -246,7 +256,8 @@ extern "C" void gf_destroy(gf_ctx *ctx)
                     ctx->a.force, ctx->a.amass, ctx->a.rmass, ctx->a.bc, ctx->a.ncod, ctx->a.stress0,
                     ctx->a.area, ctx->a.dvol, ctx->a.strain, ctx->a.se2sr, ctx->a.aps, ctx->a.visn,
                     ctx->a.e2sr, ctx->a.sshrheat, ctx->a.shrheat, ctx->a.source, ctx->a.iphase, ctx->a.pr,
-                    ctx->a.xa, ctx->a.xb, ctx->a.dens, ctx->therm_border};
+                    ctx->a.xa, ctx->a.xb, ctx->a.dens, ctx->therm_border, ctx->a.stress0f, ctx->a.velf,
+                    ctx->a.forcef, ctx->a.pref};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -306,9 +317,16 @@ static void *host_ptr(gf_ctx *ctx, uint32_t bit)
     return nullptr;
 }
 
+// fields kept in FP32 on the device by the mixed-precision path
+static bool is_sv_field(const gf_ctx *ctx, uint32_t bit)
+{
+    return ctx->precision != GF_PREC_FP64 && (bit == GF_VEL || bit == GF_FORCE || bit == GF_STRESS0);
+}
+
 static void *dev_ptr(gf_ctx *ctx, uint32_t bit)
 {
     GfDevArrays &a = ctx->a;
+    if (is_sv_field(ctx, bit)) return bit == GF_VEL ? (void *)a.velf : bit == GF_FORCE ? (void *)a.forcef : (void *)a.stress0f;
     switch (bit) {
     case GF_CORD: return a.cord;       case GF_VEL: return a.vel;           case GF_TEMP: return a.temp;
     case GF_TEMP0: return a.temp0;     case GF_FORCE: return a.force;       case GF_AMASS: return a.amass;
@@ -375,8 +393,35 @@ extern "C" int gf_upload(gf_ctx *ctx, uint32_t mask)
         if (!(mask & f.bit)) continue;
         void *h = host_ptr(ctx, f.bit);
         if (!h) return set_err(ctx, GF_ERR_ARG, "host pointer for %s is null", f.name);
-        size_t bytes = f.elem_bytes * (size_t)f.count_mult * (f.node ? ctx->nn : ctx->ne);
-        HIPCHK(hipMemcpy(dev_ptr(ctx, f.bit), h, bytes, hipMemcpyHostToDevice));
+        const size_t count = (size_t)f.count_mult * (f.node ? ctx->nn : ctx->ne);
+        if (is_sv_field(ctx, f.bit)) {  // FP32 plane on the device: round the host's doubles
+            std::vector<float> tmp(count);
+            const double *hd = (const double *)h;
+            if (f.bit == GF_STRESS0) {
+                // the stress as a deviation from -pref*I, pref = minus the element's mean
+                // normal stress over its four triangles (kept in FP64)
+                const long ne = ctx->ne;
+                std::vector<double> pref(ne);
+                for (long e = 0; e < ne; e++) {
+                    double m = 0.0;
+                    for (int k = 0; k < 4; k++)
+                        m += hd[e + ne * (0 + 4 * k)] + hd[e + ne * (1 + 4 * k)] + hd[e + ne * (3 + 4 * k)];
+                    pref[e] = -m / 12.0;
+                    for (int k = 0; k < 4; k++)
+                        for (int c = 0; c < 4; c++) {
+                            const size_t q = (size_t)e + (size_t)ne * (c + 4 * k);
+                            tmp[q] = (float)(c == 2 ? hd[q] : hd[q] + pref[e]);
+                        }
+                }
+                HIPCHK(hipMemcpy(ctx->a.pref, pref.data(), sizeof(double) * ne, hipMemcpyHostToDevice));
+                ctx->pref_host = pref;
+            } else {
+                for (size_t q = 0; q < count; q++) tmp[q] = (float)hd[q];
+            }
+            HIPCHK(hipMemcpy(dev_ptr(ctx, f.bit), tmp.data(), count * sizeof(float), hipMemcpyHostToDevice));
+            continue;
+        }
+        HIPCHK(hipMemcpy(dev_ptr(ctx, f.bit), h, f.elem_bytes * count, hipMemcpyHostToDevice));
     }
     if (mask & GF_IPHASE) {
         int rc = upload_iphase_present(ctx);
@@ -400,8 +445,26 @@ extern "C" int gf_download(gf_ctx *ctx, uint32_t mask)
         if (!(mask & f.bit)) continue;
         void *h = host_ptr(ctx, f.bit);
         if (!h) return set_err(ctx, GF_ERR_ARG, "host pointer for %s is null", f.name);
-        size_t bytes = f.elem_bytes * (size_t)f.count_mult * (f.node ? ctx->nn : ctx->ne);
-        HIPCHK(hipMemcpy(h, dev_ptr(ctx, f.bit), bytes, hipMemcpyDeviceToHost));
+        const size_t count = (size_t)f.count_mult * (f.node ? ctx->nn : ctx->ne);
+        if (is_sv_field(ctx, f.bit)) {
+            std::vector<float> tmp(count);
+            HIPCHK(hipMemcpy(tmp.data(), dev_ptr(ctx, f.bit), count * sizeof(float), hipMemcpyDeviceToHost));
+            double *hd = (double *)h;
+            if (f.bit == GF_STRESS0) {
+                const long ne = ctx->ne;
+                for (long e = 0; e < ne; e++)
+                    for (int k = 0; k < 4; k++)
+                        for (int c = 0; c < 4; c++) {
+                            const size_t q = (size_t)e + (size_t)ne * (c + 4 * k);
+                            const double pr = (size_t)e < ctx->pref_host.size() ? ctx->pref_host[e] : 0.0;
+                            hd[q] = c == 2 ? (double)tmp[q] : (double)tmp[q] - pr;
+                        }
+            } else {
+                for (size_t q = 0; q < count; q++) hd[q] = (double)tmp[q];
+            }
+            continue;
+        }
+        HIPCHK(hipMemcpy(h, dev_ptr(ctx, f.bit), f.elem_bytes * count, hipMemcpyDeviceToHost));
     }
     if ((mask & GF_PHASE_RATIO) && ctx->host.phase_ratio && !ctx->phase_ratio_host.empty())
         memcpy(ctx->host.phase_ratio, ctx->phase_ratio_host.data(), sizeof(double) * ctx->phase_ratio_host.size());
@@ -508,6 +571,25 @@ static int collect_timing(gf_ctx *ctx)
         }                                                                                  \
     } while (0)
 
+// kernels templated on the storage precision of stress, velocity and force
+#define LAUNCH_SV(kern, n, ...)                                                           \
+    do {                                                                                   \
+        if (ctx->precision != GF_PREC_FP64)                                                \
+            LAUNCH(kern<float>, n, __VA_ARGS__);                                           \
+        else                                                                               \
+            LAUNCH(kern<double>, n, __VA_ARGS__);                                          \
+    } while (0)
+// kernels templated on the precision mode (storage and arithmetic)
+#define LAUNCH_M(kern, n, ...)                                                            \
+    do {                                                                                   \
+        if (ctx->precision == GF_PREC_MIXED)                                               \
+            LAUNCH(kern<1>, n, __VA_ARGS__);                                               \
+        else if (ctx->precision == GF_PREC_MIXED_STORAGE)                                  \
+            LAUNCH(kern<2>, n, __VA_ARGS__);                                               \
+        else                                                                               \
+            LAUNCH(kern<0>, n, __VA_ARGS__);                                               \
+    } while (0)
+
 static int enq_clock(gf_ctx *ctx, int op)
 {
     hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, ctx->stream, ctx->dck, op);
@@ -552,10 +634,19 @@ static int enq_rheol(gf_ctx *ctx)
         avgsr = 1;
     else if (ctx->nsrate == -1)
         avgsr = 2;
+    const bool mixed = ctx->precision == GF_PREC_MIXED, mstore = ctx->precision == GF_PREC_MIXED_STORAGE;
     if ((rc = enq_clock(ctx, CK_DTAVG_ADD))) return rc;
-    if (do_visc) LAUNCH(k_visc, ctx->ne, ctx->dp, ctx->dck, ctx->a);
-    if (ctx->rheol_quad)
+    if (do_visc) LAUNCH_SV(k_visc, ctx->ne, ctx->dp, ctx->dck, ctx->a);
+    if (ctx->rheol_quad && !mixed && !mstore)
         LAUNCH(k_rheol, 4 * ctx->ne, ctx->dp, ctx->dck, ctx->a, avgsr);
+    else if (mstore && ctx->p.ishearh == 1)
+        LAUNCH(k_rheol1s<true>, ctx->ne, ctx->dp, ctx->dck, ctx->a, avgsr);
+    else if (mstore)
+        LAUNCH(k_rheol1s<false>, ctx->ne, ctx->dp, ctx->dck, ctx->a, avgsr);
+    else if (mixed && ctx->p.ishearh == 1)
+        LAUNCH(k_rheol1m<true>, ctx->ne, ctx->dp, ctx->dck, ctx->a, avgsr);
+    else if (mixed)
+        LAUNCH(k_rheol1m<false>, ctx->ne, ctx->dp, ctx->dck, ctx->a, avgsr);
     else if (ctx->p.ishearh == 1)
         LAUNCH(k_rheol1<true>, ctx->ne, ctx->dp, ctx->dck, ctx->a, avgsr);
     else
@@ -570,21 +661,21 @@ static int enq_rheol(gf_ctx *ctx)
 
 static int enq_node(gf_ctx *ctx)
 {
-    LAUNCH(k_node, ctx->nn, ctx->dp, ctx->dck, ctx->a);
+    LAUNCH_M(k_node, ctx->nn, ctx->dp, ctx->dck, ctx->a);
     return GF_OK;
 }
 
 static int enq_move(gf_ctx *ctx)
 {
     if (ctx->p.movegrid == 0) return GF_OK;
-    LAUNCH(k_move_cord, ctx->nn, ctx->dp, ctx->dck, ctx->a);
-    LAUNCH(k_move_elem, ctx->ne, ctx->dp, ctx->dck, ctx->a);
+    LAUNCH_SV(k_move_cord, ctx->nn, ctx->dp, ctx->dck, ctx->a);
+    LAUNCH_M(k_move_elem, ctx->ne, ctx->dp, ctx->dck, ctx->a);
     return GF_OK;
 }
 
 static int enq_rmasses(gf_ctx *ctx)
 {
-    LAUNCH(k_dens, ctx->ne, ctx->dp, ctx->dck, ctx->a);
+    LAUNCH_SV(k_dens, ctx->ne, ctx->dp, ctx->dck, ctx->a);
     LAUNCH(k_rmass, ctx->nn, ctx->dp, ctx->dck, ctx->a);
     return GF_OK;
 }
@@ -593,7 +684,7 @@ static int enq_dt_mass(gf_ctx *ctx)
 {
     int rc;
     if ((rc = enq_clock(ctx, CK_DT_BEGIN))) return rc;
-    LAUNCH(k_dt_reduce, (long)DT_RED_BLOCKS * 256, ctx->dp, ctx->dck, ctx->a);
+    LAUNCH_SV(k_dt_reduce, (long)DT_RED_BLOCKS * 256, ctx->dp, ctx->dck, ctx->a);
     hipLaunchKernelGGL(k_dt_scalars, dim3(1), dim3(64), 0, ctx->stream, ctx->dp, ctx->dck, ctx->d_present,
                        ctx->npresent);
     if ((rc = launch_check(ctx, "k_dt_scalars"))) return rc;
@@ -686,7 +777,7 @@ extern "C" int gf_init_areas(gf_ctx *ctx)
 extern "C" int gf_init_visc(gf_ctx *ctx)
 {
     if (!ctx) return GF_ERR_ARG;
-    LAUNCH(k_init_visc, ctx->ne, ctx->dp, ctx->dck, ctx->a);
+    LAUNCH_SV(k_init_visc, ctx->ne, ctx->dp, ctx->dck, ctx->a);
     return fetch_clock(ctx, nullptr);
 }
 

@@ -9,7 +9,7 @@ use iso_c_binding
 implicit none
 
 integer, parameter :: GF_MAXPH = 20, GF_MAXINH = 100
-integer(c_int32_t), parameter :: GF_PREC_FP64 = 0, GF_PREC_MIXED = 1
+integer(c_int32_t), parameter :: GF_PREC_FP64 = 0, GF_PREC_MIXED = 1, GF_PREC_MIXED_STORAGE = 2
 integer(c_int32_t), parameter :: GF_ALL = int(z'3FFFFF', c_int32_t)
 integer, parameter :: GF_OK = 0, GF_STOP_MAXWELL = 22, GF_STOP_NEGATIVE_AREA = 40
 

@@ -101,10 +101,14 @@ typedef struct gf_clock {
 } gf_clock;
 
 /* Arithmetic of the step.  GF_PREC_FP64 is the plain drop-in (bitwise order of the
- * reference).  GF_PREC_MIXED keeps stress, strain rate, force and velocity in FP32
- * and coordinates, time, temperature and plastic strain in FP64. */
+ * reference).  GF_PREC_MIXED computes strain rate, stress and force in FP32 and stores
+ * velocity, force and stress in FP32 (the stress as a deviation from a per-element
+ * isotropic reference kept in FP64); coordinates, time, temperature, plastic strain
+ * and the other time integrals stay FP64.  GF_PREC_MIXED_STORAGE keeps the same FP32
+ * planes but does all arithmetic in FP64 (no reference counterpart for either). */
 #define GF_PREC_FP64 0
 #define GF_PREC_MIXED 1
+#define GF_PREC_MIXED_STORAGE 2
 
 typedef struct gf_ctx gf_ctx;
 
