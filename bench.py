@@ -80,7 +80,7 @@ def cpu_baseline(gp, st, seconds_cap=20.0):
                       "of the reference step), %.1f s" % (n, gp.nx - 1, gp.nz - 1, el)}
 
 
-def mixed_run(gp, host_state, fp64_state, args, E, precision):
+def mixed_run(gp, host_state, fp64_state, args, E, precision, learned=None):
     """The same model and step count in a mixed-precision mode (FP32 planes for stress,
     velocity and force; FP32 or FP64 arithmetic): throughput and per-field error
     against the FP64 run."""
@@ -88,6 +88,9 @@ def mixed_run(gp, host_state, fp64_state, args, E, precision):
     from geoflac_amd import abi, gpu, state
     st = host_state.copy()
     m = gpu.Model(gp, st, precision=precision)
+    if learned is not None:  # the surrogate trained on the FP64 run's samples
+        m.surrogate_weights(learned[0], learned[1])
+        m.surrogate_enable(True, tol=0.5, audit_stride=64)
     m.step(args.warmup)
     m.sync()
     t0 = time.perf_counter()
@@ -95,14 +98,21 @@ def mixed_run(gp, host_state, fp64_state, args, E, precision):
     m.sync()
     el = time.perf_counter() - t0
     m.download()
+    sur = m.surrogate_stats() if learned is not None else None
     m.close()
     err = {}
     for f in ("vel", "stress0", "temp", "cord", "aps"):
         a, b = getattr(st, f), getattr(fp64_state, f)
         err[f] = {"rel_l2": float("%.3e" % state.rel_l2(a, b)), "max_abs": float("%.3e" % np.max(np.abs(a - b)))}
     dtype = "f32 arithmetic, f32 planes" if precision == abi.PREC_MIXED else "f64 arithmetic, f32 planes"
-    return {"value": E * args.steps / el, "ms_per_step": el / args.steps * 1e3, "dtype": dtype,
-            "steps_compared": args.warmup + args.steps, "error_vs_fp64": err}
+    out = {"value": E * args.steps / el, "ms_per_step": el / args.steps * 1e3, "dtype": dtype,
+           "steps_compared": args.warmup + args.steps, "error_vs_fp64": err}
+    if learned is not None:
+        tot = max(1, sur["evals"] + sur["fallbacks"])
+        out["surrogate"] = dict(learned[2], fallback_rate=round(sur["fallbacks"] / tot, 4),
+                                audits=sur["audits"], max_audit_log10_err=round(sur["max_audit_err"], 5),
+                                worst_element=sur["worst_element"], switched_off=sur["off"])
+    return out
 
 
 def main():
@@ -114,7 +124,10 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true",
-                    help="skip the mixed-precision run (FP32 stress/strain rate/force/velocity)")
+                    help="skip the mixed-precision runs (FP32 stress/strain rate/force/velocity)")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-viscosity run (surrogate trained on this model's samples)")
+    ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a streaming kernel over 1 GiB (read 1 GiB, write 1 GiB; torch.mul) "
                          "to calibrate the HBM byte counters of a profiler")
@@ -131,7 +144,16 @@ def main():
     m.download()
     host_state = st.copy()  # complete new-start state, for the CPU baseline
 
-    m.step(args.warmup)
+    # warm-up; the viscosity samples of the surrogate are logged along the way
+    n_samples = 0
+    if args.no_learned:
+        m.step(args.warmup)
+    else:
+        snaps = 3  # three snapshots in the warm-up; a fourth after the timed steps
+        for k in range(snaps):
+            m.step(max(1, args.warmup // snaps))
+            n_samples = m.surrogate_sample((snaps + 1) * E)
+        m.step(max(0, args.warmup - snaps * max(1, args.warmup // snaps)))
     m.sync()
     t0 = time.perf_counter()
     m.step(args.steps)       # returns after the device finished (clock read-back)
@@ -141,6 +163,15 @@ def main():
     value = E * args.steps / el
     m.download()
     fp64_state = st.copy()   # after warmup + steps, for the mixed-precision comparison
+    learned_w = None
+    if not args.no_learned:
+        n_samples = m.surrogate_sample(4 * E)
+        t_tr = time.perf_counter()
+        loss0, loss1 = m.surrogate_train(args.train_steps, batch=8192, lr=2e-3, seed=1)
+        m.sync()
+        learned_w = m.surrogate_weights() + ({"samples": n_samples, "train_steps": args.train_steps, "batch": 8192,
+                                              "loss_first": loss0, "loss_last": loss1,
+                                              "train_s": round(time.perf_counter() - t_tr, 3)},)
 
     # profiled run: HIP events around every launch on the library stream
     args.profile_steps = max(args.profile_steps, 1)
@@ -171,6 +202,10 @@ def main():
     if not args.no_mixed:
         mixed = {"fp32_arithmetic": mixed_run(gp, host_state, fp64_state, args, E, abi.PREC_MIXED),
                  "fp32_storage": mixed_run(gp, host_state, fp64_state, args, E, abi.PREC_MIXED_STORAGE)}
+    if learned_w is not None:
+        mixed = mixed or {}
+        mixed["fp32_arithmetic_learned_visc"] = mixed_run(gp, host_state, fp64_state, args, E, abi.PREC_MIXED,
+                                                          learned=learned_w)
     if args.copy_calibration:
         import torch
         x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB

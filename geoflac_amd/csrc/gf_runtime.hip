@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "gf_kernels.h"
+#include "gf_mlp.h"
 
 struct gf_ctx {
     gf_params p;
@@ -37,6 +38,14 @@ struct gf_ctx {
     long ne = 0, nn = 0;
     std::vector<double> phase_ratio_host;  // full nphase layout, as uploaded
     std::vector<double> pref_host;         // mixed precision: the stress references, as uploaded
+    // viscosity surrogate: solver samples, weights, optimiser state, configuration
+    float *sur_samples = nullptr;
+    long sur_cap = 0, sur_count = 0;
+    float *sur_w = nullptr, *sur_m = nullptr, *sur_v = nullptr, *sur_grad = nullptr, *sur_partial = nullptr;
+    MlpCfg *sur_cfg = nullptr;
+    MlpStats *sur_stats = nullptr;
+    MlpCfg sur_cfg_host{};
+    bool sur_trained = false, sur_on = false;
     int err_j = 0, err_i = 0;
     char msg[512] = {0};
     bool rheol_quad = false;  // four lanes per element (GF_RHEOL=quad) instead of one
@@ -257,7 +266,8 @@ extern "C" void gf_destroy(gf_ctx *ctx)
                     ctx->a.area, ctx->a.dvol, ctx->a.strain, ctx->a.se2sr, ctx->a.aps, ctx->a.visn,
                     ctx->a.e2sr, ctx->a.sshrheat, ctx->a.shrheat, ctx->a.source, ctx->a.iphase, ctx->a.pr,
                     ctx->a.xa, ctx->a.xb, ctx->a.dens, ctx->therm_border, ctx->a.stress0f, ctx->a.velf,
-                    ctx->a.forcef, ctx->a.pref};
+                    ctx->a.forcef, ctx->a.pref, ctx->sur_samples, ctx->sur_w, ctx->sur_m, ctx->sur_v,
+                    ctx->sur_grad, ctx->sur_partial, ctx->sur_cfg, ctx->sur_stats};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -636,7 +646,10 @@ static int enq_rheol(gf_ctx *ctx)
         avgsr = 2;
     const bool mixed = ctx->precision == GF_PREC_MIXED, mstore = ctx->precision == GF_PREC_MIXED_STORAGE;
     if ((rc = enq_clock(ctx, CK_DTAVG_ADD))) return rc;
-    if (do_visc) LAUNCH_SV(k_visc, ctx->ne, ctx->dp, ctx->dck, ctx->a);
+    if (do_visc && ctx->sur_on)
+        LAUNCH_SV(k_visc_mlp, ctx->ne, ctx->dp, ctx->dck, ctx->a, ctx->sur_w, ctx->sur_cfg, ctx->sur_stats);
+    else if (do_visc)
+        LAUNCH_SV(k_visc, ctx->ne, ctx->dp, ctx->dck, ctx->a);
     if (ctx->rheol_quad && !mixed && !mstore)
         LAUNCH(k_rheol, 4 * ctx->ne, ctx->dp, ctx->dck, ctx->a, avgsr);
     else if (mstore && ctx->p.ishearh == 1)
@@ -848,5 +861,306 @@ extern "C" int gf_math_check(int32_t fn, const double *x, const double *y, doubl
     if (dy) (void)hipFree(dy);
     if (dout) (void)hipFree(dout);
     (void)ctx;
+    return rc;
+}
+
+// ------------------------------------------------------------------ viscosity surrogate
+#define MLP_GRAD_BLOCKS 256  // most workgroups of a gradient launch (one 64-sample tile each or more)
+
+extern "C" int gf_surrogate_sample(gf_ctx *ctx, int64_t capacity, int64_t *count)
+{
+    if (!ctx || capacity <= 0) return GF_ERR_ARG;
+    int rc = check_ready(ctx);
+    if (rc) return rc;
+    if (!ctx->sur_samples) {
+        HIPCHK(hipMalloc((void **)&ctx->sur_samples, sizeof(float) * MLP_SW * (size_t)capacity));
+        ctx->sur_cap = capacity;
+        ctx->sur_count = 0;
+    }
+    if (ctx->sur_count < ctx->sur_cap) {
+        LAUNCH_SV(k_visc_sample, ctx->ne, ctx->dp, ctx->dck, ctx->a, ctx->sur_samples, (long)ctx->sur_count,
+                  (long)ctx->sur_cap);
+        ctx->sur_count = std::min(ctx->sur_cap, ctx->sur_count + ctx->ne);
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (count) *count = ctx->sur_count;
+    return GF_OK;
+}
+
+static int sur_alloc(gf_ctx *ctx)
+{
+    if (ctx->sur_w) return GF_OK;
+    HIPCHK(hipMalloc((void **)&ctx->sur_w, sizeof(float) * MLP_NP));
+    HIPCHK(hipMalloc((void **)&ctx->sur_m, sizeof(float) * MLP_NP));
+    HIPCHK(hipMalloc((void **)&ctx->sur_v, sizeof(float) * MLP_NP));
+    HIPCHK(hipMalloc((void **)&ctx->sur_grad, sizeof(float) * (MLP_NP + 1)));
+    HIPCHK(hipMalloc((void **)&ctx->sur_partial, sizeof(float) * (MLP_NP + 1) * MLP_GRAD_BLOCKS));
+    HIPCHK(hipMalloc((void **)&ctx->sur_cfg, sizeof(MlpCfg)));
+    HIPCHK(hipMalloc((void **)&ctx->sur_stats, sizeof(MlpStats)));
+    MlpStats z{};
+    HIPCHK(hipMemcpy(ctx->sur_stats, &z, sizeof(z), hipMemcpyHostToDevice));
+    return GF_OK;
+}
+
+// one Adam step on a minibatch of `batch` samples starting at permuted position `first`
+// workgroups and samples per workgroup of a gradient launch over `batch` samples
+static void grad_grid(long batch, int &nblocks, int &per_block)
+{
+    const long tiles = (batch + MLP_TS - 1) / MLP_TS;
+    nblocks = (int)std::min<long>(tiles, MLP_GRAD_BLOCKS);
+    per_block = (int)((tiles + nblocks - 1) / nblocks) * MLP_TS;
+}
+
+static int sur_train_step(gf_ctx *ctx, long nsamples, long first, long batch, long pa, long pc, float lr, int step,
+                          hipStream_t s)
+{
+    int nb, per_block;
+    grad_grid(batch, nb, per_block);
+    hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(256), 0, s, ctx->sur_w, ctx->sur_samples, nsamples, first, batch,
+                       pa, pc, ctx->sur_cfg, per_block, ctx->sur_partial);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_NP + 1 + 255) / 256), dim3(256), 0, s, ctx->sur_partial, nb,
+                       ctx->sur_grad);
+    hipLaunchKernelGGL(k_mlp_adam, dim3((MLP_NP + 255) / 256), dim3(256), 0, s, ctx->sur_w, ctx->sur_m, ctx->sur_v,
+                       ctx->sur_grad, lr, 0.9f, 0.999f, 1.0e-8f, step);
+    return launch_check(ctx, "k_mlp_grad");
+}
+
+static uint64_t splitmix(uint64_t &x)
+{
+    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+static long gcdl(long a, long b) { return b == 0 ? a : gcdl(b, a % b); }
+
+extern "C" int gf_surrogate_train(gf_ctx *ctx, int32_t steps, int32_t batch, float lr, uint64_t seed,
+                                  float *loss_first, float *loss_last)
+{
+    if (!ctx || steps < 1 || batch < 64) return GF_ERR_ARG;
+    if (ctx->sur_count < 1024) return set_err(ctx, GF_ERR_STATE, "gf_surrogate_train: too few samples");
+    int rc = sur_alloc(ctx);
+    if (rc) return rc;
+    const long ns = ctx->sur_count;
+    // normalisation and gate box from a strided subsample (at most 2^20 samples)
+    const long stride = std::max(1L, ns / (1L << 20));
+    std::vector<float> sub;
+    {
+        std::vector<float> all((size_t)MLP_SW * ((ns + stride - 1) / stride));
+        HIPCHK(hipMemcpy2D(all.data(), MLP_SW * sizeof(float), ctx->sur_samples, stride * MLP_SW * sizeof(float),
+                           MLP_SW * sizeof(float), (ns + stride - 1) / stride, hipMemcpyDeviceToHost));
+        sub.swap(all);
+    }
+    const long m = (long)sub.size() / MLP_SW;
+    double mean[MLP_SW] = {0}, sq[MLP_SW] = {0}, lo[MLP_SW], hi[MLP_SW];
+    for (int f = 0; f < MLP_SW; f++) { lo[f] = 1e300; hi[f] = -1e300; }
+    long valid = 0;
+    for (long q = 0; q < m; q++) {
+        const float *x = &sub[q * MLP_SW];
+        if (x[MLP_NIN] != x[MLP_NIN]) continue;
+        valid++;
+        for (int f = 0; f < MLP_SW; f++) {
+            mean[f] += x[f];
+            sq[f] += (double)x[f] * x[f];
+            lo[f] = std::min(lo[f], (double)x[f]);
+            hi[f] = std::max(hi[f], (double)x[f]);
+        }
+    }
+    if (valid < 64) return set_err(ctx, GF_ERR_STATE, "gf_surrogate_train: no sample inside the surrogate's domain");
+    MlpCfg &c = ctx->sur_cfg_host;
+    for (int f = 0; f < MLP_SW; f++) {
+        mean[f] /= valid;
+        double var = sq[f] / valid - mean[f] * mean[f];
+        double sd = var > 0 ? sqrt(var) : 0.0;
+        if (f < MLP_NIN) {
+            c.xmean[f] = (float)mean[f];
+            c.xstd[f] = (float)(sd > 1e-12 * (fabs(mean[f]) + 1.0) ? sd : 1.0);
+            const double margin = 0.02 * (hi[f] - lo[f]);
+            c.xlo[f] = (float)(lo[f] - margin);
+            c.xhi[f] = (float)(hi[f] + margin);
+        } else {
+            c.ymean = (float)mean[f];
+            c.ystd = (float)(sd > 1e-12 ? sd : 1.0);
+        }
+    }
+    c.xmean[7] = 0; c.xstd[7] = 1; c.xlo[7] = 0; c.xhi[7] = 0;
+    {   // occupancy of the (feature 0, feature 1) plane: cells with at least 8 samples
+        std::vector<int> cnt(MLP_OCC * MLP_OCC, 0);
+        for (long q = 0; q < m; q++) {
+            const float *x = &sub[q * MLP_SW];
+            if (x[MLP_NIN] != x[MLP_NIN]) continue;
+            const int r = std::min(MLP_OCC - 1, std::max(0, (int)((x[0] - c.xlo[0]) / (c.xhi[0] - c.xlo[0]) * MLP_OCC)));
+            const int cc = std::min(MLP_OCC - 1, std::max(0, (int)((x[1] - c.xlo[1]) / (c.xhi[1] - c.xlo[1]) * MLP_OCC)));
+            cnt[r * MLP_OCC + cc]++;
+        }
+        for (int r = 0; r < MLP_OCC; r++) {
+            c.occ[r] = 0u;
+            for (int cc = 0; cc < MLP_OCC; cc++)
+                if (cnt[r * MLP_OCC + cc] >= 8) c.occ[r] |= 1u << cc;
+        }
+    }
+    if (c.tol == 0.0f) c.tol = 0.05f;
+    HIPCHK(hipMemcpy(ctx->sur_cfg, &c, sizeof(c), hipMemcpyHostToDevice));
+    // Glorot-uniform initial weights, zero biases; Adam moments zero
+    uint64_t st = seed;
+    std::vector<float> w(MLP_NP, 0.0f), z(MLP_NP, 0.0f);
+    auto uni = [&](double lim) { return (float)(((splitmix(st) >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0) * lim); };
+    for (int q = 0; q < MLP_NH * MLP_NIN; q++) w[MLP_OW1 + q] = uni(sqrt(6.0 / (MLP_NIN + MLP_NH)));
+    for (int q = 0; q < MLP_NH * MLP_NH; q++) w[MLP_OW2 + q] = uni(sqrt(6.0 / (MLP_NH + MLP_NH)));
+    for (int q = 0; q < MLP_NH; q++) w[MLP_OW3 + q] = uni(sqrt(6.0 / (MLP_NH + 1)));
+    HIPCHK(hipMemcpy(ctx->sur_w, w.data(), sizeof(float) * MLP_NP, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->sur_m, z.data(), sizeof(float) * MLP_NP, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->sur_v, z.data(), sizeof(float) * MLP_NP, hipMemcpyHostToDevice));
+    // sample order: i -> (pa*i + pc) mod ns with pa coprime to ns
+    long pa = (long)(splitmix(st) % (uint64_t)ns) | 1;
+    while (gcdl(pa, ns) != 1) pa += 2;
+    const long pc = (long)(splitmix(st) % (uint64_t)ns);
+    float lfirst = 0, llast = 0;
+    for (int k = 1; k <= steps; k++) {
+        // cosine decay of the learning rate to 1/20 of its start
+        const float lrk = lr * (0.05f + 0.95f * 0.5f * (1.0f + cosf(3.14159265f * (float)(k - 1) / (float)steps)));
+        if ((rc = sur_train_step(ctx, ns, (long)(k - 1) * batch, batch, pa, pc, lrk, k, ctx->stream))) return rc;
+        if (k == 1 || k == steps) {
+            float l;
+            HIPCHK(hipMemcpyAsync(&l, ctx->sur_grad + MLP_NP, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            (k == 1 ? lfirst : llast) = l / (float)batch;
+        }
+    }
+    ctx->sur_trained = true;
+    if (loss_first) *loss_first = lfirst;
+    if (loss_last) *loss_last = llast;
+    return GF_OK;
+}
+
+extern "C" int gf_surrogate_enable(gf_ctx *ctx, int32_t on, float tol, int32_t audit_stride)
+{
+    if (!ctx || tol < 0 || audit_stride < 0) return GF_ERR_ARG;
+    if (on && !ctx->sur_trained) return set_err(ctx, GF_ERR_STATE, "gf_surrogate_enable: not trained");
+    if (on) {
+        ctx->sur_cfg_host.tol = tol;
+        ctx->sur_cfg_host.audit_stride = audit_stride;
+        ctx->sur_cfg_host.audit_phase = audit_stride > 1 ? (int)(ctx->nloop % audit_stride) : 0;
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        HIPCHK(hipMemcpy(ctx->sur_cfg, &ctx->sur_cfg_host, sizeof(MlpCfg), hipMemcpyHostToDevice));
+        MlpStats z{};
+        HIPCHK(hipMemcpy(ctx->sur_stats, &z, sizeof(z), hipMemcpyHostToDevice));
+    }
+    ctx->sur_on = on != 0;
+    return GF_OK;
+}
+
+extern "C" int gf_surrogate_stats(gf_ctx *ctx, double stats[6])
+{
+    if (!ctx || !stats) return GF_ERR_ARG;
+    if (!ctx->sur_stats) {
+        for (int q = 0; q < 6; q++) stats[q] = 0;
+        return GF_OK;
+    }
+    MlpStats s;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(&s, ctx->sur_stats, sizeof(s), hipMemcpyDeviceToHost));
+    float err;
+    memcpy(&err, &s.audit_maxerr_bits, sizeof(err));
+    stats[0] = (double)s.evals;
+    stats[1] = (double)s.fallbacks;
+    stats[2] = (double)s.audits;
+    stats[3] = (double)err;
+    stats[4] = (double)s.off;
+    stats[5] = s.audits ? (double)(s.worst & 0xffffffffull) : -1.0;
+    return GF_OK;
+}
+
+extern "C" int gf_surrogate_weights(gf_ctx *ctx, float *w, float *cfg, int32_t get)
+{
+    if (!ctx || !w) return GF_ERR_ARG;
+    int rc = sur_alloc(ctx);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    // cfg[66]: xmean[8], xstd[8], xlo[8], xhi[8], ymean, ystd, then the 32 occupancy
+    // words (as float bit patterns)
+    if (get) {
+        HIPCHK(hipMemcpy(w, ctx->sur_w, sizeof(float) * MLP_NP, hipMemcpyDeviceToHost));
+        if (cfg) {
+            memcpy(cfg, &ctx->sur_cfg_host, 34 * sizeof(float));
+            memcpy(cfg + 34, ctx->sur_cfg_host.occ, sizeof(ctx->sur_cfg_host.occ));
+        }
+    } else {
+        HIPCHK(hipMemcpy(ctx->sur_w, w, sizeof(float) * MLP_NP, hipMemcpyHostToDevice));
+        if (cfg) {
+            memcpy(&ctx->sur_cfg_host, cfg, 34 * sizeof(float));
+            memcpy(ctx->sur_cfg_host.occ, cfg + 34, sizeof(ctx->sur_cfg_host.occ));
+            HIPCHK(hipMemcpy(ctx->sur_cfg, &ctx->sur_cfg_host, sizeof(MlpCfg), hipMemcpyHostToDevice));
+        }
+        ctx->sur_trained = true;
+    }
+    return GF_OK;
+}
+
+// Standalone forward/backward of the network on host data (no model), for tests:
+// x[n][7] normalised inputs, t[n] targets, w the 4737 parameters.
+extern "C" int gf_mlp_grad(const float *x, const float *t, int64_t n, const float *w, float *grad, float *loss)
+{
+    if (!x || !t || !w || !grad || n < 1) return GF_ERR_ARG;
+    float *ds = nullptr, *dw = nullptr, *dp = nullptr, *dg = nullptr;
+    MlpCfg *dc = nullptr;
+    int rc = GF_OK;
+    std::vector<float> smp((size_t)n * MLP_SW);
+    for (int64_t q = 0; q < n; q++) {
+        for (int f = 0; f < MLP_NIN; f++) smp[q * MLP_SW + f] = x[q * MLP_NIN + f];
+        smp[q * MLP_SW + MLP_NIN] = t[q];
+    }
+    MlpCfg c{};
+    for (int f = 0; f < 8; f++) { c.xmean[f] = 0; c.xstd[f] = 1; }
+    c.ymean = 0; c.ystd = 1;
+    int nb, per_block;
+    grad_grid((long)n, nb, per_block);
+    std::vector<float> out(MLP_NP + 1);
+    if (hipMalloc((void **)&ds, sizeof(float) * smp.size()) != hipSuccess ||
+        hipMalloc((void **)&dw, sizeof(float) * MLP_NP) != hipSuccess ||
+        hipMalloc((void **)&dp, sizeof(float) * (MLP_NP + 1) * MLP_GRAD_BLOCKS) != hipSuccess ||
+        hipMalloc((void **)&dg, sizeof(float) * (MLP_NP + 1)) != hipSuccess ||
+        hipMalloc((void **)&dc, sizeof(MlpCfg)) != hipSuccess ||
+        hipMemcpy(ds, smp.data(), sizeof(float) * smp.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dw, w, sizeof(float) * MLP_NP, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dc, &c, sizeof(c), hipMemcpyHostToDevice) != hipSuccess) {
+        rc = GF_ERR_HIP;
+    } else {
+        hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(256), 0, 0, dw, ds, (long)n, 0L, (long)n, 1L, 0L, dc, per_block,
+                           dp);
+        hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_NP + 1 + 255) / 256), dim3(256), 0, 0, dp, nb, dg);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+            hipMemcpy(out.data(), dg, sizeof(float) * (MLP_NP + 1), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = GF_ERR_HIP;
+    }
+    if (rc == GF_OK) {
+        memcpy(grad, out.data(), sizeof(float) * MLP_NP);
+        if (loss) *loss = out[MLP_NP] / (float)n;
+    }
+    for (void *b : {(void *)ds, (void *)dw, (void *)dp, (void *)dg, (void *)dc})
+        if (b) (void)hipFree(b);
+    return rc;
+}
+
+extern "C" int gf_mlp_infer(const float *x, int64_t n, const float *w, float *y)
+{
+    if (!x || !w || !y || n < 1) return GF_ERR_ARG;
+    float *dx = nullptr, *dw = nullptr, *dy = nullptr;
+    int rc = GF_OK;
+    if (hipMalloc((void **)&dx, sizeof(float) * MLP_NIN * n) != hipSuccess ||
+        hipMalloc((void **)&dw, sizeof(float) * MLP_NP) != hipSuccess ||
+        hipMalloc((void **)&dy, sizeof(float) * n) != hipSuccess ||
+        hipMemcpy(dx, x, sizeof(float) * MLP_NIN * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dw, w, sizeof(float) * MLP_NP, hipMemcpyHostToDevice) != hipSuccess) {
+        rc = GF_ERR_HIP;
+    } else {
+        hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, dw, dx, (long)n, dy);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+            hipMemcpy(y, dy, sizeof(float) * n, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = GF_ERR_HIP;
+    }
+    for (void *b : {(void *)dx, (void *)dw, (void *)dy})
+        if (b) (void)hipFree(b);
     return rc;
 }

@@ -19,7 +19,10 @@ EXPORTS = [
     "gf_fl_rheol", "gf_fl_node", "gf_fl_move", "gf_rmasses", "gf_dt_mass", "gf_init_areas",
     "gf_init_visc", "gf_therm_stats", "gf_sync",
     "gf_set_timing", "gf_kernel_time", "gf_math_check",
+    "gf_surrogate_sample", "gf_surrogate_train", "gf_surrogate_enable", "gf_surrogate_stats",
+    "gf_surrogate_weights", "gf_mlp_grad", "gf_mlp_infer",
 ]
+MLP_NPARAM = 4737
 
 
 class GfError(RuntimeError):
@@ -59,6 +62,14 @@ def lib():
         L.gf_kernel_time.argtypes = [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         dp = C.POINTER(C.c_double)
         L.gf_math_check.argtypes = [C.c_int32, dp, dp, dp, C.c_int64]
+        fp = C.POINTER(C.c_float)
+        L.gf_surrogate_sample.argtypes = [vp, C.c_int64, C.POINTER(C.c_int64)]
+        L.gf_surrogate_train.argtypes = [vp, C.c_int32, C.c_int32, C.c_float, C.c_uint64, fp, fp]
+        L.gf_surrogate_enable.argtypes = [vp, C.c_int32, C.c_float, C.c_int32]
+        L.gf_surrogate_stats.argtypes = [vp, dp]
+        L.gf_surrogate_weights.argtypes = [vp, fp, fp, C.c_int32]
+        L.gf_mlp_grad.argtypes = [fp, fp, C.c_int64, fp, fp, fp]
+        L.gf_mlp_infer.argtypes = [fp, C.c_int64, fp, fp]
         _lib = L
     return _lib
 
@@ -122,6 +133,44 @@ class Model:
     def therm_unconverged(self):
         return self.therm_stats()[0]
 
+    # -------- viscosity surrogate
+    def surrogate_sample(self, capacity):
+        """Append one sample per element of the current state; returns the count."""
+        n = C.c_int64()
+        self._check(lib().gf_surrogate_sample(self._ctx, int(capacity), C.byref(n)))
+        return n.value
+
+    def surrogate_train(self, steps, batch=1024, lr=1e-3, seed=0):
+        """Train on the stored samples; returns (first, last) minibatch losses."""
+        a, b = C.c_float(), C.c_float()
+        self._check(lib().gf_surrogate_train(self._ctx, int(steps), int(batch), float(lr), int(seed),
+                                             C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def surrogate_enable(self, on=True, tol=0.05, audit_stride=64):
+        self._check(lib().gf_surrogate_enable(self._ctx, int(on), float(tol), int(audit_stride)))
+
+    def surrogate_stats(self):
+        """dict(evals, fallbacks, audits, max_audit_err, off) since the last enable."""
+        v = (C.c_double * 6)()
+        self._check(lib().gf_surrogate_stats(self._ctx, v))
+        return {"evals": int(v[0]), "fallbacks": int(v[1]), "audits": int(v[2]), "max_audit_err": v[3],
+                "off": int(v[4]), "worst_element": int(v[5])}
+
+    def surrogate_weights(self, w=None, cfg=None):
+        """Without arguments: (weights[4737], cfg[66]); with them: load them."""
+        import numpy as np
+        fp = C.POINTER(C.c_float)
+        if w is None:
+            w = np.zeros(MLP_NPARAM, dtype=np.float32)
+            cfg = np.zeros(66, dtype=np.float32)
+            self._check(lib().gf_surrogate_weights(self._ctx, w.ctypes.data_as(fp), cfg.ctypes.data_as(fp), 1))
+            return w, cfg
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        cp = None if cfg is None else np.ascontiguousarray(cfg, dtype=np.float32)
+        self._check(lib().gf_surrogate_weights(self._ctx, w.ctypes.data_as(fp),
+                                               None if cp is None else cp.ctypes.data_as(fp), 0))
+
     def error_site(self):
         j, i = C.c_int32(), C.c_int32()
         lib().gf_error_site(self._ctx, C.byref(j), C.byref(i))
@@ -170,3 +219,32 @@ def math_check(fn, x, y=None):
     if rc != 0:
         raise GfError(rc, "gf_math_check failed")
     return out
+
+
+def mlp_grad(x, t, w):
+    """Loss and gradient of the surrogate network by the device's training kernels."""
+    import numpy as np
+    fp = C.POINTER(C.c_float)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    t = np.ascontiguousarray(t, dtype=np.float32)
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    g = np.zeros(MLP_NPARAM, dtype=np.float32)
+    loss = C.c_float()
+    rc = lib().gf_mlp_grad(x.ctypes.data_as(fp), t.ctypes.data_as(fp), x.shape[0], w.ctypes.data_as(fp),
+                           g.ctypes.data_as(fp), C.byref(loss))
+    if rc != 0:
+        raise RuntimeError("gf_mlp_grad failed (%d)" % rc)
+    return loss.value, g
+
+
+def mlp_infer(x, w):
+    """The surrogate network's BF16 matrix-core forward pass on host inputs."""
+    import numpy as np
+    fp = C.POINTER(C.c_float)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    y = np.zeros(x.shape[0], dtype=np.float32)
+    rc = lib().gf_mlp_infer(x.ctypes.data_as(fp), x.shape[0], w.ctypes.data_as(fp), y.ctypes.data_as(fp))
+    if rc != 0:
+        raise RuntimeError("gf_mlp_infer failed (%d)" % rc)
+    return y

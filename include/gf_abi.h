@@ -170,6 +170,38 @@ int gf_set_timing(gf_ctx *ctx, int32_t on);
 int gf_math_check(int32_t fn, const double *x, const double *y, double *out, int64_t n);
 int gf_kernel_time(gf_ctx *ctx, const char *name, double *total_ms, int64_t *launches);
 
+/* ---- Learned surrogate of Eff_visc (src/matprops.f90:136-215); no reference counterpart.
+ * A 7 -> 64 -> 64 -> 1 SiLU perceptron predicts log10 of the effective viscosity from
+ * (adiabat-corrected temperature, log10 strain-rate invariant, pressure, ratios of the
+ * first four active phases).  Parameters (4737 floats): W1[64][7] b1[64] W2[64][64]
+ * b2[64] W3[64] b3[1]. */
+#define GF_MLP_NPARAM 4737
+/* Appends one sample per element of the current device state (features and the exact
+ * log10 viscosity) to the context's sample store of `capacity` samples (allocated on
+ * the first call). */
+int gf_surrogate_sample(gf_ctx *ctx, int64_t capacity, int64_t *count);
+/* Normalises the stored samples, initialises the network from `seed` and runs `steps`
+ * Adam steps on minibatches of `batch` samples with the device kernels. */
+int gf_surrogate_train(gf_ctx *ctx, int32_t steps, int32_t batch, float lr, uint64_t seed, float *loss_first,
+                       float *loss_last);
+/* Uses the surrogate in the viscosity updates of the step (BF16 matrix-core inference)
+ * with the out-of-distribution gate; every audit_stride-th element is checked against
+ * the exact formula and an audited |log10 error| above tol switches it off. */
+int gf_surrogate_enable(gf_ctx *ctx, int32_t on, float tol, int32_t audit_stride);
+/* stats: elements on the surrogate, fallbacks to the exact formula, audits, largest
+ * audited |log10 error|, switched off (0/1), element of the largest audited error
+ * (-1: none); since the last enable. */
+int gf_surrogate_stats(gf_ctx *ctx, double stats[6]);
+/* get = 1: copy out, get = 0: copy in the parameters w[4737] and, if cfg is not NULL,
+ * the gate and normalisation cfg[66] = xmean[8] xstd[8] xlo[8] xhi[8] ymean ystd and
+ * 32 words of the (1/T, log10 strain rate) occupancy grid as float bit patterns. */
+int gf_surrogate_weights(gf_ctx *ctx, float *w, float *cfg, int32_t get);
+/* The network's device kernels on host data, without a model (for tests): loss and
+ * gradient (FP32 training kernels) of the mean squared error over x[n][7], t[n]; and
+ * the BF16 matrix-core forward pass. */
+int gf_mlp_grad(const float *x, const float *t, int64_t n, const float *w, float *grad, float *loss);
+int gf_mlp_infer(const float *x, int64_t n, const float *w, float *y);
+
 #ifdef __cplusplus
 }
 #endif
