@@ -15,7 +15,8 @@ struct GfDevArrays {
     double *aps, *visn, *e2sr, *sshrheat, *shrheat, *source;
     int *iphase;
     double *pr;           // phase ratio planes of the active phases: pr[a*ne + e]
-    double *xa, *xb;      // thermal fixed-point iterates
+    double *xa, *xb;      // thermal fixed-point iterates (pass p writes xa if p is even)
+    double *tbord;        // thermal tiles' border values
     double *dens;         // per-element density scratch (rmasses)
     // Mixed precision: velocity and force in FP32, and the stress as an FP32 deviation
     // from a per-element isotropic reference -pref*I kept in FP64 (the element's mean
@@ -74,9 +75,8 @@ __global__ void k_copy(double *__restrict__ dst, const double *__restrict__ src,
 
 enum { CK_DTAVG_ADD = 1, CK_DTAVG_RESET = 2, CK_TIME_ADVANCE = 3, CK_THERM_BEGIN = 4, CK_DT_BEGIN = 5 };
 
-__global__ void k_clock(GfDevClock *ck, int op)
+GF_DEV void clock_op(GfDevClock *ck, int op)
 {
-    if (threadIdx.x != 0) return;
     if (op == CK_DTAVG_ADD) ck->dtavg = ck->dtavg + ck->dt;
     if (op == CK_DTAVG_RESET) ck->dtavg = 0.0;
     if (op == CK_TIME_ADVANCE) ck->time = ck->time + ck->dt;
@@ -86,6 +86,11 @@ __global__ void k_clock(GfDevClock *ck, int op)
         ck->dlmin_bits = (unsigned long long)__double_as_longlong(1.0e28);
         ck->velmax_bits = 0ull;
     }
+}
+
+__global__ void k_clock(GfDevClock *ck, int op)
+{
+    if (threadIdx.x == 0) clock_op(ck, op);
 }
 
 // correctly rounded math of the FP64 path, evaluated for a list of arguments
@@ -274,14 +279,13 @@ GF_DEV double therm_eval(const ThermCoef &c, double xu, double xl, double xd, do
 // A tile's global reads (its node footprint: temperature and coordinates; its 65 x 17
 // elements: areas, phase, source) are issued together into LDS, so that each tile pays
 // the memory latency once; everything after that reads LDS.
-__global__ void __launch_bounds__(1024) k_therm_tile(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
-                                                     const double *__restrict__ xin, double *__restrict__ xout,
-                                                     double *__restrict__ bord, int pass)
+GF_DEV void therm_tile_one(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, int pass,
+                           int tile)
 {
-    if (pass > 0 && ck->therm_pass_changed[pass - 1] == 0u) return;  // already proven
+    const double *__restrict__ xin = (pass % 2 == 1) ? A.xa : A.xb;
+    double *__restrict__ xout = (pass % 2 == 0) ? A.xa : A.xb;
+    double *__restrict__ bord = A.tbord;
     const int ntj = therm_ntj(P);
-    const int ntiles = ntj * therm_nti(P);
-    const int nwg = 8 * ((ntiles + 7) / 8);
     constexpr int NL = TH_LI * TH_LJ;
     __shared__ double sT[NL], sCx[NL], sCy[NL];     // old temperature, coordinates
     __shared__ double sAr[2][TH_NE];                // area of triangles A and B
@@ -295,11 +299,6 @@ __global__ void __launch_bounds__(1024) k_therm_tile(const GfDevParams *__restri
         sPh[2][q] = P->ph[q].diff;
     }
     const bool with_dissip = P->ishearh == 1 && P->itherm != 2;
-    // pass 0 runs one workgroup per tile; later passes, which are rarely needed, run a
-    // smaller grid that loops (so that a pass that is not needed costs little)
-    for (int wg = blockIdx.x; wg < nwg; wg += gridDim.x) {
-    const int tile = xcd_tile(wg, ntiles);
-    if (tile < 0) continue;
     __syncthreads();  // the previous tile's LDS reads are done (and the phase table is in)
     const int tj = threadIdx.x % TH_J, ti = threadIdx.x / TH_J;
     const int j0 = (tile % ntj) * TH_OJ + 1 - TH_HJ;  // 1-based row of tile row 0
@@ -497,8 +496,21 @@ __global__ void __launch_bounds__(1024) k_therm_tile(const GfDevParams *__restri
         bord[(long)tile * TH_NB + threadIdx.x] = sX[cur][LIDX(lj, li)];
     }
     if (threadIdx.x == 0 && !still) atomicOr(&ck->therm_pass_changed[pass], 2u);
-    }
 #undef LIDX
+}
+
+// pass 0 runs one workgroup per tile; later passes, which are rarely needed, run a
+// smaller grid that loops (so that a pass that is not needed costs little)
+__global__ void __launch_bounds__(1024) k_therm_tile(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
+                                                     int pass)
+{
+    if (pass > 0 && ck->therm_pass_changed[pass - 1] == 0u) return;  // already proven
+    const int ntiles = therm_ntj(P) * therm_nti(P);
+    const int nwg = 8 * ((ntiles + 7) / 8);
+    for (int wg = blockIdx.x; wg < nwg; wg += gridDim.x) {
+        const int tile = xcd_tile(wg, ntiles);
+        if (tile >= 0) therm_tile_one(P, ck, A, pass, tile);
+    }
 }
 
 // Compares each tile's border values with the global iterate written by the owning
@@ -508,13 +520,13 @@ __global__ void __launch_bounds__(1024) k_therm_tile(const GfDevParams *__restri
 // condition temperature is close to zero, so that differences far below the other
 // rows' rounding still show up as a few ulp there.  Any remaining difference (or a
 // tile that did not settle) sends the next pass.
-__global__ void __launch_bounds__(256) k_therm_check(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
-                                                     const double *__restrict__ x, const double *__restrict__ bord,
-                                                     int pass)
+GF_DEV void therm_check_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, int pass,
+                             long t)
 {
     if (pass > 0 && ck->therm_pass_changed[pass - 1] == 0u) return;
+    const double *__restrict__ x = (pass % 2 == 0) ? A.xa : A.xb;
+    const double *__restrict__ bord = A.tbord;
     const int ntj = therm_ntj(P);
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)ntj * therm_nti(P) * TH_NB) return;
     const int tile = (int)(t / TH_NB), k = (int)(t % TH_NB);
     int lj, li;
@@ -546,11 +558,16 @@ __global__ void __launch_bounds__(256) k_therm_check(const GfDevParams *__restri
     }
 }
 
-// Boundary conditions src/fl_therm.f90:575-598 applied to the proven iterate.
-__global__ void __launch_bounds__(256) k_therm_bc(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
-                                                  const double *__restrict__ xa, const double *__restrict__ xb)
+__global__ void __launch_bounds__(256) k_therm_check(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
+                                                     int pass)
 {
-    long t = xcd_lin();
+    therm_check_body(P, ck, A, pass, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Boundary conditions src/fl_therm.f90:575-598 applied to the proven iterate.
+GF_DEV void therm_bc_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long t)
+{
+    const double *__restrict__ xa = A.xa, *__restrict__ xb = A.xb;
     // pass p writes xa when p is even, xb when odd; the first pass whose border check
     // found no difference holds the fixed point
     int proven = -1;
@@ -582,6 +599,11 @@ __global__ void __launch_bounds__(256) k_therm_bc(const GfDevParams *__restrict_
     else
         v = tb(j, i);
     A.temp[t] = v;
+}
+
+__global__ void __launch_bounds__(256) k_therm_bc(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+{
+    therm_bc_body(P, ck, A, xcd_lin());
 }
 
 // ------------------------------------------------------------------ material properties
@@ -1294,10 +1316,8 @@ GF_DEV void stress_avg(const GfDevParams *P, const GfDevArrays &A, long e, doubl
 }
 
 template <typename SV>
-__global__ void __launch_bounds__(256) k_visc(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+GF_DEV void visc_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e)
 {
-    long e = xcd_lin();
-    if (e >= NE) return;
     const int irh = PHASE(A.iphase[e]).irheol;
     if (!(irh == 3 || irh >= 11)) return;
     const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
@@ -1306,6 +1326,13 @@ __global__ void __launch_bounds__(256) k_visc(const GfDevParams *__restrict__ P,
     double a11, a22, a33;
     stress_avg<SV>(P, A, e, a11, a22, a33);
     A.visn[e] = dev_eff_visc(P, A, j, i, a11, a22, a33, prv);
+}
+template <typename SV>
+__global__ void __launch_bounds__(256) k_visc(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+{
+    long e = xcd_lin();
+    if (e >= NE) return;
+    visc_body<SV>(P, ck, A, e);
 }
 
 __global__ void __launch_bounds__(256) k_rheol(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
@@ -1382,12 +1409,10 @@ GF_DEV void dev_tri_force(const GfDevParams *P, const GfDevArrays &A, int je, in
 // The force and velocity arithmetic runs in SV (double: the reference's, bit for bit;
 // float: mixed precision); geometry is formed in double from the FP64 coordinates.
 template <int M>
-__global__ void __launch_bounds__(256) k_node(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+GF_DEV void node_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long t)
 {
     typedef typename Prec<M>::SV SV;
     typedef typename Prec<M>::AT AT;
-    long t = xcd_lin();
-    if (t >= NN) return;
     const int j = (int)(t % NZ) + 1, i = (int)(t / NZ) + 1;
     SV *force = sv_ptr<SV>(A.force, A.forcef), *vel = sv_ptr<SV>(A.vel, A.velf);
     const double dt = ck->dt;
@@ -1503,30 +1528,40 @@ __global__ void __launch_bounds__(256) k_node(const GfDevParams *__restrict__ P,
     vel[t] = SV(v1);
     vel[t + NN] = SV(v2);
 }
+template <int M>
+__global__ void __launch_bounds__(256) k_node(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+{
+    long t = xcd_lin();
+    if (t >= NN) return;
+    node_body<M>(P, ck, A, t);
+}
 
 // ------------------------------------------------------------------ fl_move
 // coordinates stay FP64 on both paths
+template <typename SV>
+GF_DEV void move_cord_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long t)
+{
+    const SV *vel = sv_ptr<SV>(A.vel, A.velf);
+    const double dt = ck->dt;
+    A.cord[t] = A.cord[t] + (double)vel[t] * dt;
+    A.cord[t + NN] = A.cord[t + NN] + (double)vel[t + NN] * dt;
+}
 template <typename SV>
 __global__ void __launch_bounds__(256) k_move_cord(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
 {
     long t = xcd_lin();
     if (t >= NN) return;
-    const SV *vel = sv_ptr<SV>(A.vel, A.velf);
-    const double dt = ck->dt;
-    A.cord[t] = A.cord[t] + (double)vel[t] * dt;
-    A.cord[t + NN] = A.cord[t + NN] + (double)vel[t + NN] * dt;
+    move_cord_body<SV>(P, ck, A, t);
 }
 
 // src/fl_move.f90:47-128
 // areas, dvol and strain stay FP64; the stress rotation runs in AT (it leaves the
 // isotropic reference unchanged, so the stored deviation rotates by itself)
 template <int M>
-__global__ void __launch_bounds__(256) k_move_elem(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+GF_DEV void move_elem_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e)
 {
     typedef typename Prec<M>::SV SV;
     typedef typename Prec<M>::AT AT;
-    long e = xcd_lin();
-    if (e >= NE) return;
     const SV *vel = sv_ptr<SV>(A.vel, A.velf);
     SV *stress = sv_ptr<SV>(A.stress0, A.stress0f);
     const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
@@ -1575,6 +1610,13 @@ __global__ void __launch_bounds__(256) k_move_elem(const GfDevParams *__restrict
     }
     if (bad) raise_error(ck, GF_STOP_NEGATIVE_AREA, e);
 }
+template <int M>
+__global__ void __launch_bounds__(256) k_move_elem(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+{
+    long e = xcd_lin();
+    if (e >= NE) return;
+    move_elem_body<M>(P, ck, A, e);
+}
 
 // src/init_areas.f90:7-55
 __global__ void __launch_bounds__(256) k_init_areas(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
@@ -1614,10 +1656,8 @@ __global__ void __launch_bounds__(256) k_init_visc(const GfDevParams *__restrict
 // ------------------------------------------------------------------ rmasses
 // src/matprops.f90:3-83 (Eff_dens) per element into a scratch plane
 template <typename SV>
-__global__ void __launch_bounds__(256) k_dens(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+GF_DEV void dens_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e)
 {
-    long e = xcd_lin();
-    if (e >= NE) return;
     const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
     const double adi_grad = 3.0e-1, den_ramp_dist = 30.0e3, amp410 = 0.05, amp660 = 0.085;
     const int iph = A.iphase[e];
@@ -1660,6 +1700,13 @@ __global__ void __launch_bounds__(256) k_dens(const GfDevParams *__restrict__ P,
     if (PHASE(iph).is_mantle) ed = ed * den_amp;
     A.dens[e] = ed;
 }
+template <typename SV>
+__global__ void __launch_bounds__(256) k_dens(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+{
+    long e = xcd_lin();
+    if (e >= NE) return;
+    dens_body<SV>(P, ck, A, e);
+}
 
 // The reference scatters each element's 4 triangles to their nodes in 2x2 colour
 // order (src/rmasses.f90:25-60, src/dt_mass.f90:74-154).  Node (j,i) receives, from
@@ -1685,15 +1732,19 @@ GF_DEV double gather_corners(const GfDevParams *P, int j, int i, F contrib)
     return sum;
 }
 
-__global__ void __launch_bounds__(256) k_rmass(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+GF_DEV void rmass_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long t)
 {
-    long t = xcd_lin();
-    if (t >= NN) return;
     const int j = (int)(t % NZ) + 1, i = (int)(t / NZ) + 1;
     const double c1d12 = 1.0 / 12.0;
     A.rmass[t] = gather_corners(P, j, i, [&](int je, int ie, int k) {
         return c1d12 / AREA(je, ie, k) * A.dens[IE(je, ie)];
     });
+}
+__global__ void __launch_bounds__(256) k_rmass(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+{
+    long t = xcd_lin();
+    if (t >= NN) return;
+    rmass_body(P, ck, A, t);
 }
 
 // ------------------------------------------------------------------ dt_mass
@@ -1732,16 +1783,18 @@ GF_DEV double elem_dlmin(const GfDevParams *P, const GfDevArrays &A, long t)
 }
 
 #define DT_RED_BLOCKS 1024
+// block `blk` of `nblk` workgroups of 256 threads working on one model
 template <typename SV>
-__global__ void __launch_bounds__(256) k_dt_reduce(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+GF_DEV void dt_reduce_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, int blk,
+                           int nblk)
 {
     const SV *vel = sv_ptr<SV>(A.vel, A.velf);
     __shared__ double s_min[4], s_max[4];
-    const long stride = (long)gridDim.x * blockDim.x;
+    const long stride = (long)nblk * blockDim.x;
     double dlmin = 1.0e28, vmax = 0.0;
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < NE; t += stride)
+    for (long t = (long)blk * blockDim.x + threadIdx.x; t < NE; t += stride)
         dlmin = fmin(dlmin, elem_dlmin(P, A, t));
-    for (long n = (long)blockIdx.x * blockDim.x + threadIdx.x; n < 2 * NN; n += stride)
+    for (long n = (long)blk * blockDim.x + threadIdx.x; n < 2 * NN; n += stride)
         vmax = fmax(vmax, fabs((double)vel[n]));
     for (int off = 32; off > 0; off >>= 1) {
         dlmin = fmin(dlmin, __shfl_xor(dlmin, off));
@@ -1763,11 +1816,16 @@ __global__ void __launch_bounds__(256) k_dt_reduce(const GfDevParams *__restrict
     }
 }
 
+template <typename SV>
+__global__ void __launch_bounds__(256) k_dt_reduce(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+{
+    dt_reduce_body<SV>(P, ck, A, blockIdx.x, gridDim.x);
+}
+
 // Scalars of src/dt_mass.f90:31-51,122-158.  dtmax_therm and dt_maxwell depend on
 // the element only through its phase, so their minima run over the phases present.
-__global__ void k_dt_scalars(const GfDevParams *__restrict__ P, GfDevClock *ck, const int *present, int npresent)
+GF_DEV void dt_scalars_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const int *present, int npresent)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const double sec_year = 3.1558e+7;
     double dlmin = __longlong_as_double((long long)ck->dlmin_bits);
     if (P->idt_scale == 1) {
@@ -1793,11 +1851,14 @@ __global__ void k_dt_scalars(const GfDevParams *__restrict__ P, GfDevClock *ck, 
     ck->dt = fmin(fmin(ck->dt_elastic, ck->dt_maxwell), dtmax_therm);
 }
 
-// inertial masses, src/dt_mass.f90:64-120 (gather form of the coloured scatter)
-__global__ void __launch_bounds__(256) k_amass(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+__global__ void k_dt_scalars(const GfDevParams *__restrict__ P, GfDevClock *ck, const int *present, int npresent)
 {
-    long t = xcd_lin();
-    if (t >= NN) return;
+    if (threadIdx.x == 0 && blockIdx.x == 0) dt_scalars_body(P, ck, present, npresent);
+}
+
+// inertial masses, src/dt_mass.f90:64-120 (gather form of the coloured scatter)
+GF_DEV void amass_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long t)
+{
     if (P->idt_scale == 0) {
         A.amass[t] = A.rmass[t];
         return;
@@ -1817,4 +1878,10 @@ __global__ void __launch_bounds__(256) k_amass(const GfDevParams *__restrict__ P
         }
         return c1d12 * rho_inert / AREA(je, ie, k);
     });
+}
+__global__ void __launch_bounds__(256) k_amass(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+{
+    long t = xcd_lin();
+    if (t >= NN) return;
+    amass_body(P, ck, A, t);
 }

@@ -19,6 +19,7 @@
 
 #include "gf_kernels.h"
 #include "gf_mlp.h"
+#include "gf_batch.h"
 
 struct gf_ctx {
     gf_params p;
@@ -238,6 +239,7 @@ extern "C" int gf_create(const gf_params *p, int32_t precision, gf_ctx **out)
     ALLOC(ctx->a.xa, 8 * nn);
     ALLOC(ctx->a.xb, 8 * nn);
     ALLOC(ctx->therm_border, 8 * (size_t)TH_NB * ((p->nz + TH_OJ - 1) / TH_OJ) * ((p->nx + TH_OI - 1) / TH_OI));
+    ctx->a.tbord = ctx->therm_border;
     ALLOC(ctx->a.dens, 8 * ne);
 #undef ALLOC
     {
@@ -614,24 +616,21 @@ static int enq_therm(gf_ctx *ctx)
     const unsigned nwg = (unsigned)(8 * ((ntiles + 7) / 8));
     for (int pass = 0; pass < TH_MAXPASS; pass++) {
         const dim3 grid(pass == 0 ? nwg : std::min(nwg, 512u));
-        const double *in = (pass % 2 == 1) ? ctx->a.xa : ctx->a.xb;
-        double *out = (pass % 2 == 0) ? ctx->a.xa : ctx->a.xb;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (ctx->timing) {
             e0 = next_event(ctx);
             e1 = next_event(ctx);
             if (e0) (void)hipEventRecord(e0, ctx->stream);
         }
-        hipLaunchKernelGGL(k_therm_tile, grid, dim3(TH_J * TH_I), 0, ctx->stream, ctx->dp, ctx->dck, ctx->a, in, out,
-                           ctx->therm_border, pass);
+        hipLaunchKernelGGL(k_therm_tile, grid, dim3(TH_J * TH_I), 0, ctx->stream, ctx->dp, ctx->dck, ctx->a, pass);
         if ((rc = launch_check(ctx, "k_therm_tile"))) return rc;
         if (ctx->timing && e0 && e1) {
             (void)hipEventRecord(e1, ctx->stream);
             ctx->pending.push_back({pass == 0 ? "k_therm_tile" : "k_therm_tile_extra", {e0, e1}});
         }
-        LAUNCH(k_therm_check, (long)ntiles * TH_NB, ctx->dp, ctx->dck, ctx->a, out, ctx->therm_border, pass);
+        LAUNCH(k_therm_check, (long)ntiles * TH_NB, ctx->dp, ctx->dck, ctx->a, pass);
     }
-    LAUNCH(k_therm_bc, ctx->nn, ctx->dp, ctx->dck, ctx->a, ctx->a.xa, ctx->a.xb);
+    LAUNCH(k_therm_bc, ctx->nn, ctx->dp, ctx->dck, ctx->a);
     return GF_OK;
 }
 
@@ -1163,4 +1162,295 @@ extern "C" int gf_mlp_infer(const float *x, int64_t n, const float *w, float *y)
     for (void *b : {(void *)dx, (void *)dw, (void *)dy})
         if (b) (void)hipFree(b);
     return rc;
+}
+
+// ------------------------------------------------------------------ batches of models
+struct gf_batch {
+    std::vector<gf_ctx *> m;
+    int precision = GF_PREC_FP64;
+    bool shearh = false, no_mech = false;
+    gf_params p0{};  // frequencies and switches shared by all models
+    BatchEntry *dtab = nullptr;
+    int nm = 0;
+    long tot_e = 0, tot_n = 0, tot_t = 0, tot_b = 0;
+    hipStream_t stream = nullptr;
+    int32_t nloop = 0, nsrate = 0;
+    int bs[3] = {256, 256, 256};  // threads per workgroup of the rheology, node and element-move kernels
+    bool use_graph = false;
+    hipGraphExec_t gexec = nullptr;
+    int32_t g_nsrate0 = -99, g_nsrate1 = 0;  // nsrate before and after the captured 10 steps
+    // optional per-kernel timing (autotuner)
+    bool timing = false;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    double t_ms[3] = {0, 0, 0};
+    char msg[256] = {0};
+};
+
+extern "C" const char *gf_batch_last_error(const gf_batch *b) { return b ? b->msg : "null batch"; }
+
+static int bset_err(gf_batch *b, int code, const char *what)
+{
+    snprintf(b->msg, sizeof(b->msg), "%s", what);
+    return code;
+}
+
+#define BCHK(call)                                                                            \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess) {                                                               \
+            snprintf(b->msg, sizeof(b->msg), "%s: %s", #call, hipGetErrorString(e_));         \
+            return GF_ERR_HIP;                                                                \
+        }                                                                                     \
+    } while (0)
+
+static unsigned bgrid(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+extern "C" void gf_batch_destroy(gf_batch *b)
+{
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    for (hipEvent_t e : b->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (b->dtab) (void)hipFree(b->dtab);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+extern "C" int gf_batch_create(gf_ctx *const *models, int32_t n, gf_batch **out)
+{
+    if (!models || n < 1 || !out) return GF_ERR_ARG;
+    *out = nullptr;
+    gf_batch *b = new gf_batch();
+    auto fail = [&](int code, const char *what) {
+        gf_batch_destroy(b);
+        return code == GF_OK ? GF_ERR_ARG : code;
+        (void)what;
+    };
+    const gf_ctx *c0 = models[0];
+    if (!c0) return fail(GF_ERR_ARG, "null model");
+    std::vector<BatchEntry> tab(n + 1);
+    long pe = 0, pn = 0, pt = 0;
+    for (int k = 0; k < n; k++) {
+        gf_ctx *c = models[k];
+        if (!c || check_ready(c) != GF_OK) return fail(GF_ERR_STATE, "model not ready");
+        const gf_params &q = c->p, &q0 = c0->p;
+        if (c->precision != c0->precision || q.ishearh != q0.ishearh || q.itherm != q0.itherm ||
+            q.ifreq_visc != q0.ifreq_visc || q.ifreq_avgsr != q0.ifreq_avgsr || q.ifreq_rmasses != q0.ifreq_rmasses ||
+            q.ifreq_imasses != q0.ifreq_imasses || c->nloop != c0->nloop || c->nsrate != c0->nsrate || c->sur_on ||
+            c->rheol_quad)
+            return fail(GF_ERR_ARG, "models of a batch share precision, switches, frequencies and loop counters");
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(GF_ERR_HIP, "sync");
+        const long ntiles = (long)((q.nz + TH_OJ - 1) / TH_OJ) * ((q.nx + TH_OI - 1) / TH_OI);
+        tab[k] = BatchEntry{c->dp, c->dck, c->a, pe, pn, pt, pt * TH_NB, c->d_present, c->npresent, 0};
+        pe += c->ne;
+        pn += c->nn;
+        pt += ntiles;
+        b->m.push_back(c);
+    }
+    tab[n] = BatchEntry{nullptr, nullptr, GfDevArrays{}, pe, pn, pt, pt * TH_NB, nullptr, 0, 0};
+    b->nm = n;
+    b->tot_e = pe; b->tot_n = pn; b->tot_t = pt; b->tot_b = pt * TH_NB;
+    b->precision = c0->precision;
+    b->shearh = c0->p.ishearh == 1;
+    b->no_mech = c0->p.itherm == 2;
+    b->p0 = c0->p;
+    b->nloop = c0->nloop;
+    b->nsrate = c0->nsrate;
+    if (hipMalloc((void **)&b->dtab, sizeof(BatchEntry) * (n + 1)) != hipSuccess ||
+        hipMemcpy(b->dtab, tab.data(), sizeof(BatchEntry) * (n + 1), hipMemcpyHostToDevice) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&b->ev[0]) != hipSuccess || hipEventCreate(&b->ev[1]) != hipSuccess)
+        return fail(GF_ERR_HIP, "allocation");
+    *out = b;
+    return GF_OK;
+}
+
+#define BLAUNCH(kern, grid, block, ...)                                                     \
+    do {                                                                                    \
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, b->stream, __VA_ARGS__);       \
+        hipError_t e_ = hipGetLastError();                                                  \
+        if (e_ != hipSuccess) {                                                             \
+            snprintf(b->msg, sizeof(b->msg), "launch %s: %s", #kern, hipGetErrorString(e_)); \
+            return GF_ERR_HIP;                                                              \
+        }                                                                                   \
+    } while (0)
+
+// a kernel timed for the autotuner (slot q), when timing is on
+#define BTIMED(q, ...)                                                                      \
+    do {                                                                                    \
+        if (b->timing) BCHK(hipEventRecord(b->ev[0], b->stream));                           \
+        __VA_ARGS__;                                                                        \
+        if (b->timing) {                                                                    \
+            BCHK(hipEventRecord(b->ev[1], b->stream));                                      \
+            BCHK(hipEventSynchronize(b->ev[1]));                                            \
+            float ms_ = 0;                                                                  \
+            BCHK(hipEventElapsedTime(&ms_, b->ev[0], b->ev[1]));                            \
+            b->t_ms[q] += ms_;                                                              \
+        }                                                                                   \
+    } while (0)
+
+template <int M, bool SH>
+static int enq_batch_flac(gf_batch *b)
+{
+    typedef typename Prec<M>::SV SV;
+    const int nm = b->nm;
+    const unsigned gm = bgrid(nm, 64);
+    BLAUNCH(kb_clock, gm, 64, b->dtab, nm, (int)CK_THERM_BEGIN);
+    for (int pass = 0; pass < TH_MAXPASS; pass++) {
+        const long g = pass == 0 ? b->tot_t : std::min<long>(b->tot_t, 512);
+        BLAUNCH(kb_therm_tile, (unsigned)g, TH_J * TH_I, b->dtab, nm, pass);
+        BLAUNCH(kb_therm_check, bgrid(b->tot_b, 256), 256, b->dtab, nm, pass);
+    }
+    BLAUNCH(kb_therm_bc, bgrid(b->tot_n, 256), 256, b->dtab, nm);
+    if (!b->no_mech) {
+        const gf_params &p = b->p0;
+        const int do_visc = (b->nloop % p.ifreq_visc) == 0;
+        int avgsr = 0;
+        if (b->nsrate == p.ifreq_avgsr)
+            avgsr = 1;
+        else if (b->nsrate == -1)
+            avgsr = 2;
+        BLAUNCH(kb_clock, gm, 64, b->dtab, nm, (int)CK_DTAVG_ADD);
+        if (do_visc) BLAUNCH(kb_visc<SV>, bgrid(b->tot_e, 256), 256, b->dtab, nm);
+        BTIMED(0, BLAUNCH((kb_rheol<SH, M>), bgrid(b->tot_e, b->bs[0]), b->bs[0], b->dtab, nm, avgsr));
+        if (avgsr == 1) {
+            BLAUNCH(kb_clock, gm, 64, b->dtab, nm, (int)CK_DTAVG_RESET);
+            b->nsrate = 0;
+        }
+        b->nsrate = b->nsrate + 1;
+        BTIMED(1, BLAUNCH(kb_node<M>, bgrid(b->tot_n, b->bs[1]), b->bs[1], b->dtab, nm));
+        BLAUNCH(kb_move_cord<SV>, bgrid(b->tot_n, 256), 256, b->dtab, nm);
+        BTIMED(2, BLAUNCH(kb_move_elem<M>, bgrid(b->tot_e, b->bs[2]), b->bs[2], b->dtab, nm));
+        if (b->nloop % p.ifreq_rmasses == 0) {
+            BLAUNCH(kb_dens<SV>, bgrid(b->tot_e, 256), 256, b->dtab, nm);
+            BLAUNCH(kb_rmass, bgrid(b->tot_n, 256), 256, b->dtab, nm);
+        }
+        if (b->nloop % p.ifreq_imasses == 0) {
+            BLAUNCH(kb_clock, gm, 64, b->dtab, nm, (int)CK_DT_BEGIN);
+            BLAUNCH(kb_dt_reduce<SV>, (unsigned)(nm * BATCH_DT_BLOCKS), 256, b->dtab, nm);
+            BLAUNCH(kb_dt_scalars, gm, 64, b->dtab, nm);
+            BLAUNCH(kb_amass, bgrid(b->tot_n, 256), 256, b->dtab, nm);
+        }
+    }
+    b->nloop = b->nloop + 1;
+    BLAUNCH(kb_clock, gm, 64, b->dtab, nm, (int)CK_TIME_ADVANCE);
+    return GF_OK;
+}
+
+static int enq_batch_step(gf_batch *b)
+{
+    const bool sh = b->shearh;
+    switch (b->precision) {
+    case GF_PREC_MIXED: return sh ? enq_batch_flac<1, true>(b) : enq_batch_flac<1, false>(b);
+    case GF_PREC_MIXED_STORAGE: return sh ? enq_batch_flac<2, true>(b) : enq_batch_flac<2, false>(b);
+    default: return sh ? enq_batch_flac<0, true>(b) : enq_batch_flac<0, false>(b);
+    }
+}
+
+extern "C" int gf_batch_set_graph(gf_batch *b, int32_t on)
+{
+    if (!b) return GF_ERR_ARG;
+    b->use_graph = on != 0;
+    return GF_OK;
+}
+
+// Steps in cycles of 10 starting at a multiple of 10 repeat the same launches when
+// every frequency divides 10 and the strain-rate average counter starts the cycle
+// at the same value: such cycles replay one captured graph.
+static bool graph_cycle_ok(const gf_batch *b)
+{
+    const gf_params &p = b->p0;
+    return b->nloop % 10 == 0 && b->nsrate >= 0 && 10 % p.ifreq_visc == 0 && 10 % p.ifreq_rmasses == 0 &&
+           10 % p.ifreq_imasses == 0 && p.ifreq_avgsr == 10;
+}
+
+extern "C" int gf_batch_step(gf_batch *b, int32_t nsteps)
+{
+    if (!b || nsteps < 0) return GF_ERR_ARG;
+    int rc;
+    for (int32_t n = 0; n < nsteps;) {
+        if (b->use_graph && !b->timing && nsteps - n >= 10 && graph_cycle_ok(b)) {
+            if (!b->gexec || b->g_nsrate0 != b->nsrate) {
+                if (b->gexec) {
+                    BCHK(hipGraphExecDestroy(b->gexec));
+                    b->gexec = nullptr;
+                }
+                const int32_t ns0 = b->nsrate, nl0 = b->nloop;
+                hipGraph_t g = nullptr;
+                BCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+                for (int k = 0; k < 10; k++)
+                    if ((rc = enq_batch_step(b))) {
+                        (void)hipStreamEndCapture(b->stream, &g);
+                        if (g) (void)hipGraphDestroy(g);
+                        return rc;
+                    }
+                BCHK(hipStreamEndCapture(b->stream, &g));
+                hipError_t ie = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (ie != hipSuccess) return bset_err(b, GF_ERR_HIP, "hipGraphInstantiate failed");
+                b->g_nsrate0 = ns0;
+                b->g_nsrate1 = b->nsrate;
+                b->nloop = nl0;  // the capture only recorded the launches
+                b->nsrate = ns0;
+            }
+            BCHK(hipGraphLaunch(b->gexec, b->stream));
+            b->nloop += 10;
+            b->nsrate = b->g_nsrate1;
+            n += 10;
+            continue;
+        }
+        if ((rc = enq_batch_step(b))) return rc;
+        n++;
+    }
+    BCHK(hipStreamSynchronize(b->stream));
+    int first = GF_OK;
+    for (gf_ctx *c : b->m) {
+        c->nloop = b->nloop;
+        c->nsrate = b->nsrate;
+        int r = fetch_clock(c, nullptr);
+        if (r && first == GF_OK) first = r;
+    }
+    if (first) snprintf(b->msg, sizeof(b->msg), "a model stopped with code %d (see its context)", first);
+    return first;
+}
+
+// Picks the workgroup size of the three heaviest kernels from {64, 128, 256} by timing
+// `steps` real steps per candidate (the models advance by 3 * steps).
+extern "C" int gf_batch_autotune(gf_batch *b, int32_t steps, int32_t chosen[3])
+{
+    if (!b || steps < 1) return GF_ERR_ARG;
+    const int cand[3] = {64, 128, 256};
+    double best[3] = {1e30, 1e30, 1e30};
+    int pick[3] = {256, 256, 256};
+    for (int c = 0; c < 3; c++) {
+        for (int q = 0; q < 3; q++) {
+            b->bs[q] = cand[c];
+            b->t_ms[q] = 0.0;
+        }
+        b->timing = true;
+        int rc = GF_OK;
+        for (int k = 0; k < steps && rc == GF_OK; k++) rc = enq_batch_step(b);
+        b->timing = false;
+        if (rc) return rc;
+        for (int q = 0; q < 3; q++)
+            if (b->t_ms[q] < best[q]) {
+                best[q] = b->t_ms[q];
+                pick[q] = cand[c];
+            }
+    }
+    BCHK(hipStreamSynchronize(b->stream));
+    for (gf_ctx *c : b->m) {
+        c->nloop = b->nloop;
+        c->nsrate = b->nsrate;
+    }
+    for (int q = 0; q < 3; q++) {
+        b->bs[q] = pick[q];
+        if (chosen) chosen[q] = pick[q];
+    }
+    if (b->gexec) {  // captured with the old sizes
+        BCHK(hipGraphExecDestroy(b->gexec));
+        b->gexec = nullptr;
+    }
+    return GF_OK;
 }

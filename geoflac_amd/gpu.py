@@ -21,6 +21,8 @@ EXPORTS = [
     "gf_set_timing", "gf_kernel_time", "gf_math_check",
     "gf_surrogate_sample", "gf_surrogate_train", "gf_surrogate_enable", "gf_surrogate_stats",
     "gf_surrogate_weights", "gf_mlp_grad", "gf_mlp_infer",
+    "gf_batch_create", "gf_batch_destroy", "gf_batch_last_error", "gf_batch_step", "gf_batch_set_graph",
+    "gf_batch_autotune",
 ]
 MLP_NPARAM = 4737
 
@@ -70,6 +72,14 @@ def lib():
         L.gf_surrogate_weights.argtypes = [vp, fp, fp, C.c_int32]
         L.gf_mlp_grad.argtypes = [fp, fp, C.c_int64, fp, fp, fp]
         L.gf_mlp_infer.argtypes = [fp, C.c_int64, fp, fp]
+        L.gf_batch_create.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(vp)]
+        L.gf_batch_destroy.argtypes = [vp]
+        L.gf_batch_destroy.restype = None
+        L.gf_batch_last_error.argtypes = [vp]
+        L.gf_batch_last_error.restype = C.c_char_p
+        L.gf_batch_step.argtypes = [vp, C.c_int32]
+        L.gf_batch_set_graph.argtypes = [vp, C.c_int32]
+        L.gf_batch_autotune.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
         _lib = L
     return _lib
 

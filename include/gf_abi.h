@@ -202,6 +202,25 @@ int gf_surrogate_weights(gf_ctx *ctx, float *w, float *cfg, int32_t get);
 int gf_mlp_grad(const float *x, const float *t, int64_t n, const float *w, float *grad, float *loss);
 int gf_mlp_infer(const float *x, int64_t n, const float *w, float *y);
 
+/* ---- Batches of independent models (parameter sweeps); no reference counterpart.
+ * The models are contexts created, bound and uploaded as usual, with the same
+ * precision, switches, ifreq_* frequencies and loop counters; they stay owned by the
+ * caller (destroy the batch first).  One launch per kernel of the step covers all of
+ * them; each keeps its own parameters, clock and arrays. */
+typedef struct gf_batch gf_batch;
+int gf_batch_create(gf_ctx *const *models, int32_t n, gf_batch **out);
+void gf_batch_destroy(gf_batch *b);
+const char *gf_batch_last_error(const gf_batch *b);
+/* nsteps iterations of src/par.f90:81-91 for every model; the clock of each model is
+ * read back at the end (gf_get_clock).  Returns the first model's STOP code, if any. */
+int gf_batch_step(gf_batch *b, int32_t nsteps);
+/* on: cycles of 10 steps (from a multiple of 10, when every frequency divides 10) are
+ * captured once into a HIP graph and replayed. */
+int gf_batch_set_graph(gf_batch *b, int32_t on);
+/* Times `steps` real steps for each workgroup size in {64, 128, 256} and keeps the
+ * fastest for the rheology, node and element-move kernels (chosen[3]). */
+int gf_batch_autotune(gf_batch *b, int32_t steps, int32_t chosen[3]);
+
 #ifdef __cplusplus
 }
 #endif

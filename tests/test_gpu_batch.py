@@ -1,0 +1,90 @@
+"""Batches of independent models (gf_batch_*): every model of a batch ends bit for bit
+where it ends when run alone (the batched launches run the single-model kernel bodies
+on each model's own parameters, clock and arrays), with and without graph replay and
+after the workgroup-size autotuner; plus a seeded sweep's variants on the CPU."""
+import sys
+
+import numpy as np
+import pytest
+
+from geoflac_amd import abi, inp, state, sweep
+
+BASE = "tests/golden/inputs/box60x20.inp"
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _variants():
+    import os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    p = inp.read_inp(os.path.join(root, BASE))
+    return [sweep.variant(p, 60, 20), sweep.variant(p, 37, 11, 22.0, 3.0e7, 2.0e-10),
+            sweep.variant(p, 45, 27, 31.0, 5.5e7, 0.7e-10)]
+
+
+def test_sweep_variants_are_consistent():
+    import os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    p = inp.read_inp(os.path.join(root, BASE))
+    ps = sweep.sweep_params(p, n=16, seed=3)
+    for q in ps:
+        assert 100 <= q["nelx"] <= 600 and 40 <= q["nelz"] <= 150
+        assert q["nx"] == q["nelx"] + 1 and all(b["nbc2"] in (q["nx"], q["nz"]) for b in q["bcs"])
+        assert all(10.0 <= ph["fric1"] <= 35.0 and 10e6 <= ph["cohesion1"] <= 60e6 for ph in q["phases"])
+    assert [q["nelx"] for q in ps] == [q["nelx"] for q in sweep.sweep_params(p, n=16, seed=3)]
+
+
+def _run(precision, nsteps, graph=False, tune=False):
+    models, states = [], []
+    for q in _variants():
+        gp, st, m = sweep.make_model(q, precision)
+        models.append(m)
+    singles = []
+    for q in _variants():  # the same models, each alone
+        gp, st, m = sweep.make_model(q, precision)
+        m.step(nsteps)
+        m.download()
+        singles.append(st)
+        m.close()
+    with sweep.Batch(models) as b:
+        if graph:
+            b.set_graph(True)
+        done = 0
+        if tune:
+            sizes = b.autotune(2)
+            _say("  autotuned workgroup sizes (rheology, node, element move): %s" % sizes)
+            done = 6
+        b.step(nsteps - done)
+    for m in models:
+        m.download()
+        states.append(m.state)
+    for m in models:
+        m.close()
+    return states, singles
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("graph,tune", [(False, False), (True, False), (True, True)])
+def test_batch_equals_single_runs(graph, tune):
+    n = 40
+    _say("case: batch of 3 models (60x20, 37x11, 45x27), %d steps, graph %s, autotune %s" % (n, graph, tune))
+    batch, single = _run(abi.PREC_FP64, n, graph, tune)
+    for k, (a, b) in enumerate(zip(batch, single)):
+        diff = [f for f in state.State.FIELDS if not np.array_equal(getattr(a, f), getattr(b, f))]
+        _say("  model %d: nloop %d, differing fields %s" % (k, a.clock.nloop, diff))
+        assert not diff
+        assert a.clock.nloop == b.clock.nloop == n
+        assert a.clock.time == b.clock.time and a.clock.dt == b.clock.dt
+
+
+@pytest.mark.gpu
+def test_batch_mixed_precision_equals_single_runs():
+    n = 20
+    _say("case: batch of 3 models in mixed precision, %d steps with graph replay" % n)
+    batch, single = _run(abi.PREC_MIXED, n, graph=True)
+    for a, b in zip(batch, single):
+        diff = [f for f in state.State.FIELDS if not np.array_equal(getattr(a, f), getattr(b, f))]
+        assert not diff, diff
