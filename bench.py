@@ -29,7 +29,7 @@ METRIC = "element-steps/s on 1 GPU: mixed/learned vs plain FP64, at stated rel-L
 def alg_bytes(kernel, E, N, nact, p):
     """Algorithmic (compulsory) HBM bytes of one launch: each array the kernel must
     read once and write once, FP64 = 8 B, int32 = 4 B."""
-    kernel = kernel.replace("<double>", "")
+    kernel = kernel.replace("<double>", "").replace("<0>", "")
     if kernel.startswith("k_rheol"):
         per_e = (32 + 32 + 128 + 8 + 8 + 8 * nact + 4 + 24 + 8 + 24)  # area dvol stress0 aps visn pr iphase se2sr sshrheat strain
         per_e += 128 + 8 + 24 + 24 + 8                                 # writes stress0 aps strain se2sr sshrheat
@@ -48,6 +48,22 @@ def alg_bytes(kernel, E, N, nact, p):
     if kernel == "k_therm_bc":
         return N * 16
     return 0
+
+
+def traffic_from_profile(kernel):
+    """HBM bytes per launch of `kernel` measured with profiler byte counters
+    (profiles/traffic.json: FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the
+    same counter's reading for the copy kernel of known size), when that file holds
+    this kernel; otherwise traffic stays null."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        t = json.load(f)
+    k = t.get("kernels", {}).get(kernel)
+    if not k:
+        return {}
+    return {"traffic": k["bytes_per_launch"], "traffic_source": "profiles/traffic.json (%s)" % t.get("command", "")}
 
 
 def build_model(cfg, precision):
@@ -176,8 +192,8 @@ def main():
     ap.add_argument("--sweep-check", type=int, default=4,
                     help="models of the sweep also run alone in FP64 for the error report")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="also run a streaming kernel over 1 GiB (read 1 GiB, write 1 GiB; torch.mul) "
-                         "to calibrate the HBM byte counters of a profiler")
+                    help="short copy probe only (for profiler counter runs, where k_copy over 1 GiB is the "
+                         "kernel of known traffic that scales FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (one model per device); "
@@ -226,7 +242,7 @@ def main():
     m.set_timing(True)
     m.step(args.profile_steps)
     kernels = ["k_rheol1<false>", "k_rheol1<true>", "k_rheol", "k_visc<double>", "k_therm_tile",
-               "k_therm_tile_extra", "k_therm_check", "k_node<double>", "k_move_elem<double>",
+               "k_therm_tile_extra", "k_therm_check", "k_node<0>", "k_move_elem<0>",
                "k_move_cord<double>", "k_therm_bc", "k_dens<double>", "k_rmass", "k_dt_reduce<double>", "k_amass",
                "k_clock"]
     times = {k: m.kernel_time(k) for k in kernels}
@@ -253,13 +269,11 @@ def main():
         mixed = mixed or {}
         mixed["fp32_arithmetic_learned_visc"] = mixed_run(gp, host_state, fp64_state, args, E, abi.PREC_MIXED,
                                                           learned=learned_w)
-    if args.copy_calibration:
-        import torch
-        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
-        y = torch.empty_like(x)
-        for _ in range(3):
-            torch.mul(x, 1.0, out=y)
-        torch.cuda.synchronize()
+    # measured copy bandwidth (k_copy over 1 GiB: 2 GiB of traffic per launch)
+    copy_ms = gpu.stream_probe(1 << 30, 10 if not args.copy_calibration else 3)
+    roofline["copy_bw_GBs"] = round(2.0 * (1 << 30) / (copy_ms * 1e-3) / 1e9, 1)
+    roofline["frac_of_copy_bw"] = round(achieved / roofline["copy_bw_GBs"], 4)
+    roofline.update(traffic_from_profile(dom))
 
     out = {"metric": METRIC, "value": value, "unit": "element-steps/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,

@@ -1454,3 +1454,39 @@ extern "C" int gf_batch_autotune(gf_batch *b, int32_t steps, int32_t chosen[3])
     }
     return GF_OK;
 }
+
+// ------------------------------------------------------------------ bandwidth reference
+// A plain streaming copy (k_copy, 8-byte elements) of `bytes` bytes, `reps` times:
+// the measured copy bandwidth of the device, and a kernel of known traffic (read
+// `bytes`, write `bytes` per launch) against which a profiler's byte counters are
+// scaled.  Instrumentation; no reference counterpart.
+extern "C" int gf_stream_probe(int64_t bytes, int32_t reps, double *ms_per_launch)
+{
+    if (bytes < 8 || reps < 1 || !ms_per_launch) return GF_ERR_ARG;
+    const long n = (long)(bytes / 8);
+    double *a = nullptr, *c = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = GF_OK;
+    if (hipMalloc((void **)&a, 8 * n) != hipSuccess || hipMalloc((void **)&c, 8 * n) != hipSuccess ||
+        hipMemset(a, 0, 8 * n) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess) {
+        rc = GF_ERR_HIP;
+    } else {
+        const unsigned g = (unsigned)((n + 255) / 256);
+        hipLaunchKernelGGL(k_copy, dim3(g), dim3(256), 0, 0, c, a, n);  // warm-up
+        (void)hipEventRecord(e0, 0);
+        for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy, dim3(g), dim3(256), 0, 0, c, a, n);
+        (void)hipEventRecord(e1, 0);
+        float ms = 0;
+        if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            rc = GF_ERR_HIP;
+        else
+            *ms_per_launch = ms / reps;
+    }
+    for (hipEvent_t e : {e0, e1})
+        if (e) (void)hipEventDestroy(e);
+    for (void *b : {(void *)a, (void *)c})
+        if (b) (void)hipFree(b);
+    return rc;
+}

@@ -22,7 +22,7 @@ EXPORTS = [
     "gf_surrogate_sample", "gf_surrogate_train", "gf_surrogate_enable", "gf_surrogate_stats",
     "gf_surrogate_weights", "gf_mlp_grad", "gf_mlp_infer",
     "gf_batch_create", "gf_batch_destroy", "gf_batch_last_error", "gf_batch_step", "gf_batch_set_graph",
-    "gf_batch_autotune",
+    "gf_batch_autotune", "gf_stream_probe",
 ]
 MLP_NPARAM = 4737
 
@@ -80,6 +80,7 @@ def lib():
         L.gf_batch_step.argtypes = [vp, C.c_int32]
         L.gf_batch_set_graph.argtypes = [vp, C.c_int32]
         L.gf_batch_autotune.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
+        L.gf_stream_probe.argtypes = [C.c_int64, C.c_int32, C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -258,3 +259,12 @@ def mlp_infer(x, w):
     if rc != 0:
         raise RuntimeError("gf_mlp_infer failed (%d)" % rc)
     return y
+
+
+def stream_probe(nbytes=1 << 30, reps=10):
+    """Average duration (ms) of one plain device copy of nbytes bytes (read + write)."""
+    ms = C.c_double()
+    rc = lib().gf_stream_probe(int(nbytes), int(reps), C.byref(ms))
+    if rc != 0:
+        raise RuntimeError("gf_stream_probe failed (%d)" % rc)
+    return ms.value

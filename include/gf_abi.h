@@ -164,6 +164,10 @@ int gf_sync(gf_ctx *ctx);
  * launch count of kernel `name` (e.g. "k_rheol") since timing was switched on.
  * Instrumentation of this build; the reference has none. */
 int gf_set_timing(gf_ctx *ctx, int32_t on);
+/* A plain device copy of `bytes` bytes, `reps` times: the average duration of one
+ * launch (the measured copy bandwidth; a kernel of known traffic, k_copy, to scale a
+ * profiler's byte counters).  Instrumentation of this build; no reference counterpart. */
+int gf_stream_probe(int64_t bytes, int32_t reps, double *ms_per_launch);
 /* Evaluates on the device, for n host arguments, the correctly rounded functions the
  * FP64 path uses in place of libm (fn: 0 sin, 1 tan, 2 exp, 3 pow(x, y)), so a test
  * can compare them with the host libm the reference calls.  No reference counterpart. */
