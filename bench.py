@@ -43,6 +43,20 @@ def alg_bytes(kernel, E, N, nact, p):
         return E * (96 + 32) + N * (16 + 16 + 16 + 8 + 16 + 16 + 16)  # stress0(1:3) area | cord rmass+amass vel ncod bc, vel+force out
     if kernel == "k_move_elem":
         return E * (32 + 96 + 24 + 32 + 32 + 96 + 24) + N * 32
+    if kernel == "k_move_area":
+        return E * (32 + 32 + 32) + N * 16                             # area in, area dvol out | cord
+    if kernel == "k_move_rot":
+        return E * (96 + 24 + 96 + 24) + N * 32                        # stress0(1:3) strain in and out | cord vel
+    if kernel == "k_visc":
+        return E * (96 + 8 + 8 * nact + 4 + 8) + N * 8                 # stress0(1,2,4) e2sr pr iphase, visn out | temp
+    if kernel == "k_dens":
+        return E * (96 + 8 * nact + 4 + 8) + N * 8                     # stress0(1,2,4) pr iphase, dens out | temp
+    if kernel == "k_rmass":
+        return E * (8 + 32) + N * 8                                    # dens area | rmass out
+    if kernel == "k_amass":
+        return E * (32 + 4) + N * 8                                    # area iphase | amass out
+    if kernel == "k_dt_reduce":
+        return E * (32 + 8 + 4) + N * 32                               # area visn iphase | cord vel
     if kernel == "k_move_cord":
         return N * 48
     if kernel == "k_therm_bc":
@@ -243,7 +257,7 @@ def main():
     m.step(args.profile_steps)
     kernels = ["k_rheol1<false>", "k_rheol1<true>", "k_rheol", "k_visc<double>", "k_therm_tile",
                "k_therm_tile_extra", "k_therm_check", "k_node<0>", "k_move_elem<0>",
-               "k_move_cord<double>", "k_therm_bc", "k_dens<double>", "k_rmass", "k_dt_reduce<double>", "k_amass",
+               "k_move_area", "k_move_rot<0>", "k_move_cord<double>", "k_therm_bc", "k_dens<double>", "k_rmass", "k_dt_reduce<double>", "k_amass",
                "k_clock"]
     times = {k: m.kernel_time(k) for k in kernels}
     m.set_timing(False)
@@ -252,6 +266,8 @@ def main():
     ms_tot, nl = times[dom]
     nact = int((host_state.phase_ratio >= 0.01).any(axis=(1, 2)).sum())
     ab = alg_bytes(dom, E, N, nact, gp)
+    step_bytes = sum(alg_bytes(k, E, N, nact, gp) * v[1] for k, v in times.items()) // args.profile_steps
+    step_ms = sum(v[0] for v in times.values()) / args.profile_steps
     avg_s = ms_tot / 1e3 / max(nl, 1)
     achieved = ab / avg_s / 1e9 if ab else 0.0
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -259,6 +275,9 @@ def main():
                 "alg_bytes_per_launch": ab, "avg_launch_us": round(avg_s * 1e6, 2),
                 "kernel_ms_per_step": {k: round(v[0] / args.profile_steps, 4) for k, v in times.items() if v[1]},
                 "therm_extra_passes_per_step": th[1] / args.profile_steps,
+                # the whole step: algorithmic bytes of every launch over the kernels' time
+                "step_alg_bytes": step_bytes, "step_kernel_ms": round(step_ms, 4),
+                "step_achieved": round(step_bytes / (step_ms * 1e-3) / 1e9, 1) if step_ms else None,
                 "therm_border_mismatches": th[2], "therm_mismatch_max_ulp": th[3]}
     m.close()
     mixed = None

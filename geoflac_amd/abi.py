@@ -50,9 +50,10 @@ class GfParams(C.Structure):
     _fields_ = ([(n, _I) for n in _INT_FIELDS]
                 + [("irheol", _I * MAXPH)]
                 + [(n, _I * MAXINH) for n in ("ix1", "ix2", "iy1", "iy2", "igeom")]
-                + [("pad0", _I)]
+                + [("flexure_base", _I)]
                 + [(n, _D) for n in _REAL_FIELDS]
-                + [(n, _D * MAXPH) for n in PHASE_TABLES])
+                + [(n, _D * MAXPH) for n in PHASE_TABLES]
+                + [("topo_kappa", _D)])
 
 
 _DP = C.POINTER(C.c_double)

@@ -88,7 +88,7 @@ template <bool SHEARH, int M>
 __global__ void __launch_bounds__(256) kb_rheol(const BatchEntry *__restrict__ tab, int nm, int avgsr)
 {
     batch_apply<BK_ELEM>(tab, nm, [&](const BatchEntry &b, long e) {
-        rheol_single<SHEARH, typename Prec<M>::SV, typename Prec<M>::AT>(b.P, b.ck, b.A, e, avgsr);
+        rheol_single<SHEARH, typename Prec<M>::SV, typename Prec<M>::AT>(b.P, b.P->ph, b.ck, b.A, e, avgsr, 0);
     });
 }
 

@@ -57,6 +57,7 @@ struct GfDevClock {
     long long error_index;      // smallest linear index (reference loop order) that failed
     unsigned long long therm_unconverged, therm_extra_passes, therm_mismatch, therm_mismatch_ulp;
     unsigned int therm_pass_changed[GF_MAX_SWEEPS];  // bit 0: a tile border differs from its owner; bit 1: a tile did not settle
+    double dt_move;             // dt of the last element move (its stress rotation may still be pending)
 };
 
 // ---------------------------------------------------------------- buffer access

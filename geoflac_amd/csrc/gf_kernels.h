@@ -62,10 +62,13 @@ template <> GF_DEV float *sv_ptr<float>(double *d, float *f) { return f; }
 // Launches of the stencil kernels round their grid up to a multiple of 8 and renumber
 // workgroups so that XCD x handles the x-th contiguous eighth of the index range:
 // neighbouring columns, which read each other's nodes or elements, then share an L2.
+// Kernels indexed this way are launched with GF_BLOCK threads per workgroup (a
+// compile-time constant: reading blockDim.x costs a memory round trip per wave).
+#define GF_BLOCK 256
 GF_DEV long xcd_lin()
 {
     const unsigned q = gridDim.x / 8;
-    return (long)((blockIdx.x % 8) * q + blockIdx.x / 8) * blockDim.x + threadIdx.x;
+    return (long)((blockIdx.x % 8) * q + blockIdx.x / 8) * GF_BLOCK + threadIdx.x;
 }
 __global__ void k_copy(double *__restrict__ dst, const double *__restrict__ src, long n)
 {
@@ -648,14 +651,14 @@ GF_DEV double dev_eff_visc(const GfDevParams *P, const GfDevArrays &A, int j, in
 }
 
 // src/rh_plastic.f90:231-291
-GF_DEV void dev_pre_plast(const GfDevParams *P, double aps, const double *prv, double &coh, double &phi,
-                          double &psi, double &hardn)
+GF_DEV void dev_pre_plast(const GfDevParams *P, const GfPhase *phs, double aps, const double *prv, double &coh,
+                          double &phi, double &psi, double &hardn)
 {
     phi = 0; coh = 0; psi = 0; hardn = 0;
     for (int a = 0; a < P->nactive; a++) {
         double r = prv[a];
         if (r < 0.01) continue;
-        const GfPhase &ph = PHASE(P->active[a]);
+        const GfPhase &ph = phs[P->active[a] - 1];
         double f, c, d, h;
         if (aps < ph.plstrain1) {
             f = ph.fric1; c = ph.cohesion1; d = ph.dilat1; h = 0;
@@ -791,7 +794,7 @@ GF_DEV void rheol_quad(const GfDevParams *__restrict__ P, GfDevClock *ck, const 
         double prv[GF_MAXACT], coh, phi, psi;
         const Buf bPR = mkbuf(A.pr, (unsigned long long)P->nactive * pe);
         for (int a = 0; a < P->nactive; a++) prv[a] = bld(bPR, eo, a * pe);
-        dev_pre_plast(P, aps, prv, coh, phi, psi, hardn);
+        dev_pre_plast(P, P->ph, aps, prv, coh, phi, psi, hardn);
         // src/rh_plastic.f90:33-46: sin(phi), sin(psi) and tan(phi).  Elements whose
         // angles equal the phase's unsoftened ones reuse its correctly rounded values;
         // the others take one double-double sine/cosine per lane (lane 1 takes psi).
@@ -945,12 +948,30 @@ GF_DEV void rheol_quad(const GfDevParams *__restrict__ P, GfDevClock *ck, const 
 
 // correctly rounded sin(phi), sin(psi), tan(phi) of src/rh_plastic.f90:33-46 (degrees);
 // out of line: one copy of the double-double code, off the caller's register budget
-static __device__ __noinline__ void cr_sin_sin_tan(double phi, double psi, double *sphi, double *spsi, double *tphi)
+// (returned in registers, not through memory)
+struct SinSinTan {
+    double sphi, spsi, tphi;
+};
+static __device__ __noinline__ SinSinTan cr_sin_sin_tan(double phi, double psi)
 {
     const double degrad = 3.14159265358979323846 / 180.0;
-    *sphi = cr_sin(phi * degrad);
-    *spsi = cr_sin(psi * degrad);
-    *tphi = phi == 0.0 ? 0.0 : cr_tan(phi * degrad);
+    SinSinTan r;
+    r.sphi = cr_sin(phi * degrad);
+    r.spsi = cr_sin(psi * degrad);
+    r.tphi = phi == 0.0 ? 0.0 : cr_tan(phi * degrad);
+    return r;
+}
+
+// the phase constants fl_rheol reads, fetched together as soon as the phase index
+// is known (one cache round trip instead of one per use)
+struct RheolPhase {
+    int irheol;
+    double bulkm, rm, phi0, psi0, sphi0, spsi0, tphi0;
+};
+GF_DEV RheolPhase rheol_phase(const GfPhase *__restrict__ phs, int iph)
+{
+    const GfPhase &ph = phs[iph - 1];
+    return {ph.irheol, ph.bulkm, ph.rm, ph.phi0, ph.psi0, ph.sphi0, ph.spsi0, ph.tphi0};
 }
 
 // the constitutive update of one triangle for rheology irh (:216-264), in arithmetic T;
@@ -987,6 +1008,42 @@ GF_DEV bool rheol_tri(int irh, bool plastic_branch, const PlasticConstT<T> &pc, 
     return ok;
 }
 
+// Element move formulas of src/fl_move.f90 (shared by k_move_* and k_rheol1).
+// determinant of triangle k (0..3; corners nd1/nd2/nd3 of :7-9) from the corner
+// coordinates xc/yc[1..4], :76-78
+GF_DEV void move_tri_corners(int k, int &a, int &b, int &c)
+{
+    a = k == 1 ? 3 : 1;                // nd1 = (1, 3, 1, 1)
+    b = k == 3 ? 4 : 2;                // nd2 = (2, 2, 2, 4)
+    c = (k == 1 || k == 2) ? 4 : 3;    // nd3 = (3, 4, 4, 3)
+}
+GF_DEV double move_det(const double *xc, const double *yc, int k)
+{
+    int a, b, c;
+    move_tri_corners(k, a, b, c);
+    return (xc[b] * yc[c] - yc[b] * xc[c]) - (xc[a] * yc[c] - yc[a] * xc[c]) + (xc[a] * yc[b] - yc[a] * xc[b]);
+}
+// rotation increment of triangle k, :84-90
+GF_DEV double move_dw12(const double *xc, const double *yc, const double *xv, const double *yv, int k, double det,
+                        double dt)
+{
+    int a, b, c;
+    move_tri_corners(k, a, b, c);
+    return 0.5 *
+           (xv[a] * (xc[c] - xc[b]) + xv[b] * (xc[a] - xc[c]) + xv[c] * (xc[b] - xc[a]) + yv[a] * (yc[c] - yc[b]) +
+            yv[b] * (yc[a] - yc[c]) + yv[c] * (yc[b] - yc[a])) /
+           det * dt;
+}
+// :92-97 and :100-106 on one component triple (stress in AT, strain in double)
+template <typename T>
+GF_DEV void move_rotate(T &s11, T &s22, T &s12, T w)
+{
+    const T a = s11, b = s22, c = s12;
+    s11 = a + c * 2 * w;
+    s22 = b - c * 2 * w;
+    s12 = c + w * (b - a);
+}
+
 // Single-lane variant of the same update (one thread per element), used where a
 // quad's repeated element-level work costs more than the occupancy it buys.
 // The invariant loop accumulates both branches' sums without keeping either stress
@@ -996,8 +1053,17 @@ GF_DEV bool rheol_tri(int irh, bool plastic_branch, const PlasticConstT<T> &pc, 
 // and stress.  (double, double) is the FP64 path; (float, float) FP32 arithmetic on
 // the stress deviation; (float, double) FP32 storage with FP64 arithmetic on the
 // absolute stress (the deviation is converted on load and store).
+//
+// rot != 0: the rotation of the previous element move is still pending (k_move_area
+// ran without it); it is applied here to the loaded stress and strain, with the
+// move's dt, exactly as k_move_rot would have stored them.
+//
+// phs: the phase table (a workgroup's LDS copy of P->ph, or P->ph itself).  Every
+// per-element load is issued before the first use, so that a wave pays one memory
+// round trip for its inputs instead of a chain of them.
 template <bool SHEARH, typename SV, typename AT>
-GF_DEV void rheol_single(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e, int avgsr)
+GF_DEV void rheol_single(const GfDevParams *__restrict__ P, const GfPhase *__restrict__ phs, GfDevClock *ck,
+                         const GfDevArrays &A, long e, int avgsr, int rot)
 {
     const unsigned nzm = (unsigned)(NZ - 1);
     const unsigned ne = (unsigned)NE, nn = (unsigned)NN, nzu = (unsigned)NZ;
@@ -1016,11 +1082,31 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, GfDevClock *ck, cons
     const Buf bSE = mkbuf(A.se2sr, 3ull * pe), bST = mkbuf(A.strain, 3ull * pe);
     const Buf bAPS = mkbuf(A.aps, pe), bVIS = mkbuf(A.visn, pe), bSH = mkbuf(A.sshrheat, pe);
     const Buf bV = mkbuf(sv_ptr<SV>(A.vel, A.velf), 2ull * pnv), bC = mkbuf(A.cord, 2ull * pn);
-    const double dt = ck->dt;
-    const AT dtS = AT(dt);
+    const Buf bIP = mkbuf(A.iphase, 4ull * ne);
+    // phase ratios: GF_MAXACT planes are read; those past nactive lie outside the
+    // buffer and read 0 without touching memory
+    const Buf bPR = mkbuf(A.pr, (unsigned long long)P->nactive * pe);
+    const int iph = bldi(bIP, (unsigned)e * 4u, 0);
+    double prv[GF_MAXACT];
+#pragma unroll
+    for (int a = 0; a < GF_MAXACT; a++) prv[a] = bld(bPR, eo, a * pe);
+    double aps = bld(bAPS, eo, 0);
+    const double vis = bld(bVIS, eo, 0);  // refreshed by k_visc on viscosity-update steps
+    double se_in[3], sshr_in, str_in[3];
+    for (int c = 0; c < 3; c++) se_in[c] = bld(bSE, eo, c * pe);
+    for (int c = 0; c < 3; c++) str_in[c] = bld(bST, eo, c * pe);
+    sshr_in = bld(bSH, eo, 0);
+    SV sv_in[16];
+    for (int q = 0; q < 16; q++) sv_in[q] = bldt<SV>(bS, so, q * ps);
+    double dvl[4];
+    for (int k = 0; k < 4; k++) dvl[k] = bld(bD, eo, k * pe);
     double ar[4];
     for (int k = 0; k < 4; k++) ar[k] = bld(bA, eo, k * pe);
+    const RheolPhase rp = rheol_phase(phs, iph);
+    const double dt = ck->dt;
+    const AT dtS = AT(dt);
     AT sr[4][3];
+    double dw[4] = {0.0, 0.0, 0.0, 0.0};  // pending rotation increments
     {
         // corners 1 (j,i), 2 (j+1,i), 3 (j,i+1), 4 (j+1,i+1)
         const AT vx1 = AT(bldt<SV>(bV, nov, 0)), vy1 = AT(bldt<SV>(bV, nov, pnv)), vx2 = AT(bldt<SV>(bV, nov, W)),
@@ -1046,6 +1132,15 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, GfDevClock *ck, cons
         tri(1, x3, y3, vx3, vy3, x2, y2, vx2, vy2, x4, y4, vx4, vy4);
         tri(2, x1, y1, vx1, vy1, x2, y2, vx2, vy2, x4, y4, vx4, vy4);
         tri(3, x1, y1, vx1, vy1, x4, y4, vx4, vy4, x3, y3, vx3, vy3);
+        if (rot) {
+            // the stored velocities, exactly (AT holds every SV value)
+            const double xc[5] = {0.0, x1, x2, x3, x4}, yc[5] = {0.0, y1, y2, y3, y4};
+            const double xv[5] = {0.0, (double)vx1, (double)vx2, (double)vx3, (double)vx4};
+            const double yv[5] = {0.0, (double)vy1, (double)vy2, (double)vy3, (double)vy4};
+            const double dtm = ck->dt_move;
+#pragma unroll
+            for (int k = 0; k < 4; k++) dw[k] = move_dw12(xc, yc, xv, yv, k, move_det(xc, yc, k), dtm);
+        }
     }
     if (P->mix_strain == 1) {
         for (int k = 0; k < 4; k += 2) {
@@ -1060,7 +1155,12 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, GfDevClock *ck, cons
     }
     AT st[4][4];
     for (int k = 0; k < 4; k++)
-        for (int c = 0; c < 4; c++) st[k][c] = AT(bldt<SV>(bS, so, (c + 4 * k) * ps));
+        for (int c = 0; c < 4; c++) st[k][c] = AT(sv_in[c + 4 * k]);
+    if (rot)
+        for (int k = 0; k < 4; k++) {
+            move_rotate<AT>(st[k][0], st[k][1], st[k][2], AT(dw[k]));
+            for (int c = 0; c < 3; c++) st[k][c] = AT(SV(st[k][c]));  // as k_move_rot stores them
+        }
     const double pref = ref_press<SV>(A, e);
     if (conv)
         for (int k = 0; k < 4; k++) {
@@ -1078,40 +1178,38 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, GfDevClock *ck, cons
         AT srI = AT(0.5) * (a11 + a22);
         AT srs2 = (a11 - srI) * (a11 - srI) + (a22 - srI) * (a22 - srI) + AT(2.0) * a12 * a12;
         // the time integrals stay FP64 on both paths
-        se1 = bld(bSE, eo, 0) + a11 * dt;
-        se2 = bld(bSE, eo, pe) + a22 * dt;
-        se3 = bld(bSE, eo, 2 * pe) + a12 * dt;
+        se1 = se_in[0] + a11 * dt;
+        se2 = se_in[1] + a22 * dt;
+        se3 = se_in[2] + a12 * dt;
         s11 = AT(0.25) * (AT(0.0) + st[0][0] + st[1][0] + st[2][0] + st[3][0]);
         s22 = AT(0.25) * (AT(0.0) + st[0][1] + st[1][1] + st[2][1] + st[3][1]);
         s12 = AT(0.25) * (AT(0.0) + st[0][2] + st[1][2] + st[2][2] + st[3][2]);
         AT stII = AT(0.5) * sqrt((s11 - s22) * (s11 - s22) + AT(4.0) * s12 * s12);
-        sshr = bld(bSH, eo, 0);
+        sshr = sshr_in;
         if (srII != AT(0.0)) sshr = sshr + stII / srII * srs2 * dt;
     }
-    const int iph = A.iphase[e];
-    const GfPhase &ph = PHASE(iph);
-    const int irh = ph.irheol;
-    const AT bulkm = AT(ph.bulkm), rmu = AT(ph.rm);
+    const GfPhase &ph = phs[iph - 1];
+    const int irh = rp.irheol;
+    const AT bulkm = AT(rp.bulkm), rmu = AT(rp.rm);
     AT stherm = AT(0.0);
     if (P->istress_therm > 0) stherm = AT(-ph.alfa * ph.bulkm * (A.temp[IN(j, i)] - A.temp0[IN(j, i)]));
-    double aps = bld(bAPS, eo, 0);
-    const double vis = bld(bVIS, eo, 0);  // refreshed by k_visc on viscosity-update steps
     const bool plas_law = irh == 6 || irh >= 11;
     PlasticConstT<AT> pc{};
     PlasticDbl pd{0.0, 0.0, 0.0, dev_form ? pref : 0.0};
     AT hardn = AT(0);
     if (plas_law) {
-        double prv[GF_MAXACT], coh, phi, psi, hd;
-        const Buf bPR = mkbuf(A.pr, (unsigned long long)P->nactive * pe);
-        for (int a = 0; a < P->nactive; a++) prv[a] = bld(bPR, eo, a * pe);
-        dev_pre_plast(P, aps, prv, coh, phi, psi, hd);
+        double coh, phi, psi, hd;
+        dev_pre_plast(P, phs, aps, prv, coh, phi, psi, hd);
         double sphi, spsi, tphi;
-        if (phi == ph.phi0 && psi == ph.psi0) {
-            sphi = ph.sphi0;
-            spsi = ph.spsi0;
-            tphi = ph.tphi0;
+        if (phi == rp.phi0 && psi == rp.psi0) {
+            sphi = rp.sphi0;
+            spsi = rp.spsi0;
+            tphi = rp.tphi0;
         } else if (!dev_form) {
-            cr_sin_sin_tan(phi, psi, &sphi, &spsi, &tphi);
+            const SinSinTan t = cr_sin_sin_tan(phi, psi);
+            sphi = t.sphi;
+            spsi = t.spsi;
+            tphi = t.tphi;
         } else {  // mixed precision: the device libm's double sin/tan
             const double degrad = 3.14159265358979323846 / 180.0;
             sphi = sin(phi * degrad);
@@ -1122,8 +1220,8 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, GfDevClock *ck, cons
         const double c4d3 = 4.0 / 3.0, c2d3 = 2.0 / 3.0;
         const double anphi = (1.0 + sphi) / (1.0 - sphi);
         const double anpsi = (1.0 + spsi) / (1.0 - spsi);
-        const double e1 = ph.bulkm + c4d3 * ph.rm;
-        const double e2 = ph.bulkm - c2d3 * ph.rm;
+        const double e1 = rp.bulkm + c4d3 * rp.rm;
+        const double e2 = rp.bulkm - c2d3 * rp.rm;
         pc.anphi = AT(anphi);
         pc.anpsi = AT(anpsi);
         pc.amc = AT(2.0 * coh * sqrt(anphi));
@@ -1146,13 +1244,13 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, GfDevClock *ck, cons
         AT s11p = 0, s12p = 0, s11v = 0, s12v = 0, s22p[4], s22v[4];
         for (int k = 0; k < 4; k++) {
             AT o[4];
-            const AT dv = AT(bld(bD, eo, k * pe) - 0.0);
-            rheol_tri<AT>(irh, true, pc, pd, bulkm, rmu, hardn, ph.rm, vis, dt, dtS, stherm, st[k], sr[k], dv, o, depl[k]);
+            const AT dv = AT(dvl[k] - 0.0);
+            rheol_tri<AT>(irh, true, pc, pd, bulkm, rmu, hardn, rp.rm, vis, dt, dtS, stherm, st[k], sr[k], dv, o, depl[k]);
             s11p = k == 0 ? o[0] : s11p + o[0];
             s12p = k == 0 ? o[2] : s12p + o[2];
             s22p[k] = o[1];
             AT dm;
-            ok &= rheol_tri<AT>(irh, false, pc, pd, bulkm, rmu, hardn, ph.rm, vis, dt, dtS, stherm, st[k], sr[k], dv, o, dm);
+            ok &= rheol_tri<AT>(irh, false, pc, pd, bulkm, rmu, hardn, rp.rm, vis, dt, dtS, stherm, st[k], sr[k], dv, o, dm);
             s11v = k == 0 ? o[0] : s11v + o[0];
             s12v = k == 0 ? o[2] : s12v + o[2];
             s22v[k] = o[1];
@@ -1166,8 +1264,8 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, GfDevClock *ck, cons
     if (irh == 1 || irh == 3 || irh == 6 || irh >= 11) {
         for (int k = 0; k < 4; k++) {
             AT o[4], dm = AT(0);
-            const AT dv = AT(bld(bD, eo, k * pe) - 0.0);
-            bool okk = rheol_tri<AT>(irh, plastic_branch, pc, pd, bulkm, rmu, hardn, ph.rm, vis, dt, dtS, stherm, st[k],
+            const AT dv = AT(dvl[k] - 0.0);
+            bool okk = rheol_tri<AT>(irh, plastic_branch, pc, pd, bulkm, rmu, hardn, rp.rm, vis, dt, dtS, stherm, st[k],
                                      sr[k], dv, o, dm);
             if (irh == 6) depl[k] = dm;
             if (irh == 3) ok &= okk;
@@ -1214,9 +1312,13 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, GfDevClock *ck, cons
     }
     // the reference adds the averaged (step-start) STRESS times dt: s11..s12 were reused
     // (mixed precision: the absolute stress, deviation - pref)
-    bst(bST, eo, 0, bld(bST, eo, 0) + (dev_form ? (double)s11 - pd.pref : (double)s11) * dt);
-    bst(bST, eo, pe, bld(bST, eo, pe) + (dev_form ? (double)s22 - pd.pref : (double)s22) * dt);
-    bst(bST, eo, 2 * pe, bld(bST, eo, 2 * pe) + s12 * dt);
+    {
+        double t11 = str_in[0], t22 = str_in[1], t12 = str_in[2];
+        if (rot) move_rotate<double>(t11, t22, t12, dw[0]);
+        bst(bST, eo, 0, t11 + (dev_form ? (double)s11 - pd.pref : (double)s11) * dt);
+        bst(bST, eo, pe, t22 + (dev_form ? (double)s22 - pd.pref : (double)s22) * dt);
+        bst(bST, eo, 2 * pe, t12 + s12 * dt);
+    }
     if (SHEARH) {
         AT q_shear_elem = AT(0.0);
         double dsum = 0.0;
@@ -1269,29 +1371,29 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, GfDevClock *ck, cons
 
 template <bool SHEARH>
 __global__ void __launch_bounds__(256) k_rheol1(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
-                                                int avgsr)
+                                                int avgsr, int rot)
 {
     long e = xcd_lin();
     if (e >= NE) return;
-    rheol_single<SHEARH, double, double>(P, ck, A, e, avgsr);
+    rheol_single<SHEARH, double, double>(P, P->ph, ck, A, e, avgsr, rot);
 }
 // mixed precision: FP32 strain rate and stress
 template <bool SHEARH>
 __global__ void __launch_bounds__(256) k_rheol1m(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
-                                                 int avgsr)
+                                                 int avgsr, int rot)
 {
     long e = xcd_lin();
     if (e >= NE) return;
-    rheol_single<SHEARH, float, float>(P, ck, A, e, avgsr);
+    rheol_single<SHEARH, float, float>(P, P->ph, ck, A, e, avgsr, rot);
 }
 // FP32 storage, FP64 arithmetic
 template <bool SHEARH>
 __global__ void __launch_bounds__(256) k_rheol1s(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
-                                                 int avgsr)
+                                                 int avgsr, int rot)
 {
     long e = xcd_lin();
     if (e >= NE) return;
-    rheol_single<SHEARH, float, double>(P, ck, A, e, avgsr);
+    rheol_single<SHEARH, float, double>(P, P->ph, ck, A, e, avgsr, rot);
 }
 
 // Eff_visc of src/fl_rheol.f90:203-206 on viscosity-update steps (mod(nloop,
@@ -1557,57 +1659,122 @@ __global__ void __launch_bounds__(256) k_move_cord(const GfDevParams *__restrict
 // src/fl_move.f90:47-128
 // areas, dvol and strain stay FP64; the stress rotation runs in AT (it leaves the
 // isotropic reference unchanged, so the stored deviation rotates by itself)
-template <int M>
-GF_DEV void move_elem_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e)
+//
+// The element move has two halves.  The areas and dvol (and the negative-area STOP)
+// are needed by the thermal step that follows; the rotation of stress and strain is
+// needed first by the next fl_rheol.  The single-model step therefore runs the first
+// half here (k_move_area) and leaves the rotation pending: k_rheol1 applies it to the
+// stress and strain it loads anyway, which saves a read and a write of 12 stress and
+// 3 strain planes per step.  k_move_rot applies a pending rotation on its own where
+// something else reads stress or strain first; k_move_elem does both halves at once.
+
+template <typename SV>
+GF_DEV void move_load_corners(const GfDevParams *__restrict__ P, const GfDevArrays &A, int j, int i, double *xc,
+                              double *yc, double *xv, double *yv)
 {
-    typedef typename Prec<M>::SV SV;
-    typedef typename Prec<M>::AT AT;
     const SV *vel = sv_ptr<SV>(A.vel, A.velf);
-    SV *stress = sv_ptr<SV>(A.stress0, A.stress0f);
-    const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
-    const double dt = ck->dt;
-    double xc[5], yc[5], xv[5], yv[5];
     xc[1] = CORD(j, i, 1); yc[1] = CORD(j, i, 2);
     xc[2] = CORD(j + 1, i, 1); yc[2] = CORD(j + 1, i, 2);
     xc[3] = CORD(j, i + 1, 1); yc[3] = CORD(j, i + 1, 2);
     xc[4] = CORD(j + 1, i + 1, 1); yc[4] = CORD(j + 1, i + 1, 2);
+    if (!xv) return;
 #define VELS(jj, ii, d) (double)vel[IN(jj, ii) + NN * ((d) - 1)]
     xv[1] = VELS(j, i, 1); yv[1] = VELS(j, i, 2);
     xv[2] = VELS(j + 1, i, 1); yv[2] = VELS(j + 1, i, 2);
     xv[3] = VELS(j, i + 1, 1); yv[3] = VELS(j, i + 1, 2);
     xv[4] = VELS(j + 1, i + 1, 1); yv[4] = VELS(j + 1, i + 1, 2);
 #undef VELS
-    const int nd1[4] = {1, 3, 1, 1}, nd2[4] = {2, 2, 2, 4}, nd3[4] = {3, 4, 4, 3};
+}
+
+// rotation of the 12 stress planes of element e by dw[k] (stored precision SV,
+// arithmetic AT) and of its strain by dw[0]
+template <int M>
+GF_DEV void move_rotate_elem(const GfDevParams *__restrict__ P, const GfDevArrays &A, long e, const double dw[4])
+{
+    typedef typename Prec<M>::SV SV;
+    typedef typename Prec<M>::AT AT;
+    SV *stress = sv_ptr<SV>(A.stress0, A.stress0f);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        AT t11 = stress[e + NE * (0 + 4 * k)], t22 = stress[e + NE * (1 + 4 * k)], t12 = stress[e + NE * (2 + 4 * k)];
+        move_rotate<AT>(t11, t22, t12, AT(dw[k]));
+        stress[e + NE * (0 + 4 * k)] = SV(t11);
+        stress[e + NE * (1 + 4 * k)] = SV(t22);
+        stress[e + NE * (2 + 4 * k)] = SV(t12);
+    }
+    double s11 = A.strain[e], s22 = A.strain[e + NE], s12 = A.strain[e + 2 * NE];
+    move_rotate<double>(s11, s22, s12, dw[0]);
+    A.strain[e] = s11;
+    A.strain[e + NE] = s22;
+    A.strain[e + 2 * NE] = s12;
+}
+
+// first half: new areas and dvol, negative-area STOP; records dt for the rotation
+GF_DEV void move_area_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e)
+{
+    const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
+    double xc[5], yc[5];
+    move_load_corners<double>(P, A, j, i, xc, yc, nullptr, nullptr);
     bool bad = false;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        const int a = nd1[k], b = nd2[k], c = nd3[k];
-        double det = (xc[b] * yc[c] - yc[b] * xc[c]) - (xc[a] * yc[c] - yc[a] * xc[c]) + (xc[a] * yc[b] - yc[a] * xc[b]);
-        double ar_old = A.area[e + NE * k];
+        const double det = move_det(xc, yc, k);
+        const double ar_old = A.area[e + NE * k];
         A.dvol[e + NE * k] = det * ar_old - 1.0;
-        double ar = 1.0 / det;
+        const double ar = 1.0 / det;
         A.area[e + NE * k] = ar;
         bad |= (ar <= 0);
-        double dw12 = 0.5 *
-                      (xv[a] * (xc[c] - xc[b]) + xv[b] * (xc[a] - xc[c]) + xv[c] * (xc[b] - xc[a]) +
-                       yv[a] * (yc[c] - yc[b]) + yv[b] * (yc[a] - yc[c]) + yv[c] * (yc[b] - yc[a])) /
-                      det * dt;
-        const AT w = AT(dw12);
-        const AT t11 = stress[e + NE * (0 + 4 * k)];
-        const AT t22 = stress[e + NE * (1 + 4 * k)];
-        const AT t12 = stress[e + NE * (2 + 4 * k)];
-        stress[e + NE * (0 + 4 * k)] = SV(t11 + t12 * 2 * w);
-        stress[e + NE * (1 + 4 * k)] = SV(t22 - t12 * 2 * w);
-        stress[e + NE * (2 + 4 * k)] = SV(t12 + w * (t22 - t11));
-        if (k == 0) {
-            const double s11 = A.strain[e];
-            const double s22 = A.strain[e + NE];
-            const double s12 = A.strain[e + 2 * NE];
-            A.strain[e] = s11 + s12 * 2 * dw12;
-            A.strain[e + NE] = s22 - s12 * 2 * dw12;
-            A.strain[e + 2 * NE] = s12 + dw12 * (s22 - s11);
-        }
     }
+    if (e == 0) ck->dt_move = ck->dt;
+    if (bad) raise_error(ck, GF_STOP_NEGATIVE_AREA, e);
+}
+__global__ void __launch_bounds__(256) k_move_area(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+{
+    long e = xcd_lin();
+    if (e >= NE) return;
+    move_area_body(P, ck, A, e);
+}
+
+// second half on its own: the pending rotation with the dt of its move
+template <int M>
+GF_DEV void move_rot_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e)
+{
+    const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
+    double xc[5], yc[5], xv[5], yv[5], dw[4];
+    move_load_corners<typename Prec<M>::SV>(P, A, j, i, xc, yc, xv, yv);
+    const double dt = ck->dt_move;
+#pragma unroll
+    for (int k = 0; k < 4; k++) dw[k] = move_dw12(xc, yc, xv, yv, k, move_det(xc, yc, k), dt);
+    move_rotate_elem<M>(P, A, e, dw);
+}
+template <int M>
+__global__ void __launch_bounds__(256) k_move_rot(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+{
+    long e = xcd_lin();
+    if (e >= NE) return;
+    move_rot_body<M>(P, ck, A, e);
+}
+
+// both halves
+template <int M>
+GF_DEV void move_elem_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e)
+{
+    const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
+    const double dt = ck->dt;
+    double xc[5], yc[5], xv[5], yv[5], dw[4];
+    move_load_corners<typename Prec<M>::SV>(P, A, j, i, xc, yc, xv, yv);
+    bool bad = false;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double det = move_det(xc, yc, k);
+        const double ar_old = A.area[e + NE * k];
+        A.dvol[e + NE * k] = det * ar_old - 1.0;
+        const double ar = 1.0 / det;
+        A.area[e + NE * k] = ar;
+        bad |= (ar <= 0);
+        dw[k] = move_dw12(xc, yc, xv, yv, k, det, dt);
+    }
+    move_rotate_elem<M>(P, A, e, dw);
     if (bad) raise_error(ck, GF_STOP_NEGATIVE_AREA, e);
 }
 template <int M>

@@ -50,6 +50,7 @@ struct gf_ctx {
     int err_j = 0, err_i = 0;
     char msg[512] = {0};
     bool rheol_quad = false;  // four lanes per element (GF_RHEOL=quad) instead of one
+    bool rot_pending = false; // the last element move's stress/strain rotation is not applied yet
     // per-kernel event timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -174,6 +175,8 @@ extern "C" int gf_create(const gf_params *p, int32_t precision, gf_ctx **out)
         return GF_ERR_UNSUPPORTED;
     // paths of the reference this build does not provide (see the header)
     if (p->itype_melting != 0 || p->nystressbc != 0 || p->ndim != 2) return GF_ERR_UNSUPPORTED;
+    // the flexural base and surface processes of fl_move (src/fl_move.f90:37-40)
+    if (p->flexure_base != 0 || p->topo_kappa > 0.0) return GF_ERR_UNSUPPORTED;
     if (p->inhom < 0 || p->inhom > GF_MAXINH) return GF_ERR_ARG;
     // deeper than ~240 km, Eff_dens would rewrite the global dt fraction (src/matprops.f90:37-51)
     if (p->rzbo < -240.0e3) return GF_ERR_UNSUPPORTED;
@@ -396,10 +399,14 @@ static int upload_iphase_present(gf_ctx *ctx)
     return GF_OK;
 }
 
+static int flush_rot(gf_ctx *ctx);
+
 extern "C" int gf_upload(gf_ctx *ctx, uint32_t mask)
 {
     if (!ctx) return GF_ERR_ARG;
     if (!ctx->bound) return set_err(ctx, GF_ERR_STATE, "gf_upload before gf_bind_host");
+    int frc = flush_rot(ctx);  // the upload overwrites the moved state
+    if (frc) return frc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     for (const FieldDesc &f : kFields) {
         if (!(mask & f.bit)) continue;
@@ -452,6 +459,8 @@ extern "C" int gf_download(gf_ctx *ctx, uint32_t mask)
 {
     if (!ctx) return GF_ERR_ARG;
     if (!ctx->bound) return set_err(ctx, GF_ERR_STATE, "gf_download before gf_bind_host");
+    int frc = flush_rot(ctx);
+    if (frc) return frc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     for (const FieldDesc &f : kFields) {
         if (!(mask & f.bit)) continue;
@@ -608,6 +617,16 @@ static int enq_clock(gf_ctx *ctx, int op)
     return launch_check(ctx, "k_clock");
 }
 
+// applies a pending stress/strain rotation (k_move_area ran without it) before
+// anything but k_rheol1 reads stress or strain
+static int flush_rot(gf_ctx *ctx)
+{
+    if (!ctx->rot_pending) return GF_OK;
+    ctx->rot_pending = false;
+    LAUNCH_M(k_move_rot, ctx->ne, ctx->dp, ctx->dck, ctx->a);
+    return GF_OK;
+}
+
 static int enq_therm(gf_ctx *ctx)
 {
     int rc;
@@ -645,6 +664,10 @@ static int enq_rheol(gf_ctx *ctx)
         avgsr = 2;
     const bool mixed = ctx->precision == GF_PREC_MIXED, mstore = ctx->precision == GF_PREC_MIXED_STORAGE;
     if ((rc = enq_clock(ctx, CK_DTAVG_ADD))) return rc;
+    // the viscosity update reads the rotated step-start stress
+    if ((do_visc || ctx->rheol_quad) && (rc = flush_rot(ctx))) return rc;
+    const int rot = ctx->rot_pending ? 1 : 0;
+    ctx->rot_pending = false;
     if (do_visc && ctx->sur_on)
         LAUNCH_SV(k_visc_mlp, ctx->ne, ctx->dp, ctx->dck, ctx->a, ctx->sur_w, ctx->sur_cfg, ctx->sur_stats);
     else if (do_visc)
@@ -652,17 +675,17 @@ static int enq_rheol(gf_ctx *ctx)
     if (ctx->rheol_quad && !mixed && !mstore)
         LAUNCH(k_rheol, 4 * ctx->ne, ctx->dp, ctx->dck, ctx->a, avgsr);
     else if (mstore && ctx->p.ishearh == 1)
-        LAUNCH(k_rheol1s<true>, ctx->ne, ctx->dp, ctx->dck, ctx->a, avgsr);
+        LAUNCH(k_rheol1s<true>, ctx->ne, ctx->dp, ctx->dck, ctx->a, avgsr, rot);
     else if (mstore)
-        LAUNCH(k_rheol1s<false>, ctx->ne, ctx->dp, ctx->dck, ctx->a, avgsr);
+        LAUNCH(k_rheol1s<false>, ctx->ne, ctx->dp, ctx->dck, ctx->a, avgsr, rot);
     else if (mixed && ctx->p.ishearh == 1)
-        LAUNCH(k_rheol1m<true>, ctx->ne, ctx->dp, ctx->dck, ctx->a, avgsr);
+        LAUNCH(k_rheol1m<true>, ctx->ne, ctx->dp, ctx->dck, ctx->a, avgsr, rot);
     else if (mixed)
-        LAUNCH(k_rheol1m<false>, ctx->ne, ctx->dp, ctx->dck, ctx->a, avgsr);
+        LAUNCH(k_rheol1m<false>, ctx->ne, ctx->dp, ctx->dck, ctx->a, avgsr, rot);
     else if (ctx->p.ishearh == 1)
-        LAUNCH(k_rheol1<true>, ctx->ne, ctx->dp, ctx->dck, ctx->a, avgsr);
+        LAUNCH(k_rheol1<true>, ctx->ne, ctx->dp, ctx->dck, ctx->a, avgsr, rot);
     else
-        LAUNCH(k_rheol1<false>, ctx->ne, ctx->dp, ctx->dck, ctx->a, avgsr);
+        LAUNCH(k_rheol1<false>, ctx->ne, ctx->dp, ctx->dck, ctx->a, avgsr, rot);
     if (avgsr == 1) {
         if ((rc = enq_clock(ctx, CK_DTAVG_RESET))) return rc;
         ctx->nsrate = 0;
@@ -673,6 +696,8 @@ static int enq_rheol(gf_ctx *ctx)
 
 static int enq_node(gf_ctx *ctx)
 {
+    int rc;
+    if ((rc = flush_rot(ctx))) return rc;
     LAUNCH_M(k_node, ctx->nn, ctx->dp, ctx->dck, ctx->a);
     return GF_OK;
 }
@@ -680,13 +705,22 @@ static int enq_node(gf_ctx *ctx)
 static int enq_move(gf_ctx *ctx)
 {
     if (ctx->p.movegrid == 0) return GF_OK;
+    int rc;
+    if ((rc = flush_rot(ctx))) return rc;  // a second move before any fl_rheol
     LAUNCH_SV(k_move_cord, ctx->nn, ctx->dp, ctx->dck, ctx->a);
-    LAUNCH_M(k_move_elem, ctx->ne, ctx->dp, ctx->dck, ctx->a);
+    if (ctx->rheol_quad) {
+        LAUNCH_M(k_move_elem, ctx->ne, ctx->dp, ctx->dck, ctx->a);
+    } else {
+        LAUNCH(k_move_area, ctx->ne, ctx->dp, ctx->dck, ctx->a);
+        ctx->rot_pending = true;
+    }
     return GF_OK;
 }
 
 static int enq_rmasses(gf_ctx *ctx)
 {
+    int rc;
+    if ((rc = flush_rot(ctx))) return rc;  // Eff_dens reads the stress
     LAUNCH_SV(k_dens, ctx->ne, ctx->dp, ctx->dck, ctx->a);
     LAUNCH(k_rmass, ctx->nn, ctx->dp, ctx->dck, ctx->a);
     return GF_OK;
@@ -789,6 +823,8 @@ extern "C" int gf_init_areas(gf_ctx *ctx)
 extern "C" int gf_init_visc(gf_ctx *ctx)
 {
     if (!ctx) return GF_ERR_ARG;
+    int rc = flush_rot(ctx);
+    if (rc) return rc;
     LAUNCH_SV(k_init_visc, ctx->ne, ctx->dp, ctx->dck, ctx->a);
     return fetch_clock(ctx, nullptr);
 }
@@ -876,6 +912,7 @@ extern "C" int gf_surrogate_sample(gf_ctx *ctx, int64_t capacity, int64_t *count
         ctx->sur_cap = capacity;
         ctx->sur_count = 0;
     }
+    if ((rc = flush_rot(ctx))) return rc;
     if (ctx->sur_count < ctx->sur_cap) {
         LAUNCH_SV(k_visc_sample, ctx->ne, ctx->dp, ctx->dck, ctx->a, ctx->sur_samples, (long)ctx->sur_count,
                   (long)ctx->sur_cap);
@@ -1365,10 +1402,24 @@ static bool graph_cycle_ok(const gf_batch *b)
            10 % p.ifreq_imasses == 0 && p.ifreq_avgsr == 10;
 }
 
+// a model stepped on its own may hold a pending rotation: apply it on the model's
+// stream before the batch's stream touches the arrays
+static int batch_flush(gf_batch *b)
+{
+    for (gf_ctx *c : b->m) {
+        if (!c->rot_pending) continue;
+        int rc = flush_rot(c);
+        if (rc == GF_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = GF_ERR_HIP;
+        if (rc) return bset_err(b, rc, c->msg);
+    }
+    return GF_OK;
+}
+
 extern "C" int gf_batch_step(gf_batch *b, int32_t nsteps)
 {
     if (!b || nsteps < 0) return GF_ERR_ARG;
     int rc;
+    if ((rc = batch_flush(b))) return rc;
     for (int32_t n = 0; n < nsteps;) {
         if (b->use_graph && !b->timing && nsteps - n >= 10 && graph_cycle_ok(b)) {
             if (!b->gexec || b->g_nsrate0 != b->nsrate) {
@@ -1420,6 +1471,7 @@ extern "C" int gf_batch_step(gf_batch *b, int32_t nsteps)
 extern "C" int gf_batch_autotune(gf_batch *b, int32_t steps, int32_t chosen[3])
 {
     if (!b || steps < 1) return GF_ERR_ARG;
+    if (int frc = batch_flush(b)) return frc;
     const int cand[3] = {64, 128, 256};
     double best[3] = {1e30, 1e30, 1e30};
     int pick[3] = {256, 256, 256};

@@ -68,7 +68,14 @@ p%nyhydro = nyhydro; p%i_prestress = i_prestress; p%idt_scale = idt_scale; p%i_r
 p%ivis_present = ivis_present; p%iphsub = iphsub; p%ifreq_rmasses = ifreq_rmasses
 p%ifreq_imasses = ifreq_imasses; p%ifreq_visc = ifreq_visc; p%ifreq_avgsr = ifreq_avgsr
 p%inhom = inhom
-p%irheol = 0; p%ix1 = 0; p%ix2 = 0; p%iy1 = 0; p%iy2 = 0; p%igeom = 0; p%pad0 = 0
+p%irheol = 0; p%ix1 = 0; p%ix2 = 0; p%iy1 = 0; p%iy2 = 0; p%igeom = 0
+! the flexural base (src/fl_flexure.f90:15-24) and surface diffusion (src/fl_move.f90:40)
+! are refused by gf_create
+p%flexure_base = 0
+do k = 1, nofbc
+    if (nbc(k) == 200 .and. nofside(k) == 2) p%flexure_base = 1
+enddo
+p%topo_kappa = topo_kappa
 n = min(maxph, GF_MAXPH)
 p%irheol(1:n) = irheol(1:n)
 n = min(maxinh, GF_MAXINH)

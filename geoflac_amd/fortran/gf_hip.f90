@@ -23,7 +23,7 @@ type, bind(C) :: gf_params_t
     integer(c_int32_t) :: inhom
     integer(c_int32_t) :: irheol(GF_MAXPH)
     integer(c_int32_t) :: ix1(GF_MAXINH), ix2(GF_MAXINH), iy1(GF_MAXINH), iy2(GF_MAXINH), igeom(GF_MAXINH)
-    integer(c_int32_t) :: pad0
+    integer(c_int32_t) :: flexure_base
     real(c_double) :: t_top, t_bot, bot_bc, g, demf, ten_off, tau_heal, v_min, v_max
     real(c_double) :: frac, fracm, strain_inert, dt_scale, xReyn, vbc, rxbo, rzbo
     real(c_double) :: pisos, drosub, fmagma_max, weaken_ratio_plastic, weaken_ratio_viscous, rho_magma
@@ -32,6 +32,7 @@ type, bind(C) :: gf_params_t
     real(c_double) :: plstrain1(GF_MAXPH), plstrain2(GF_MAXPH), fric1(GF_MAXPH), fric2(GF_MAXPH)
     real(c_double) :: cohesion1(GF_MAXPH), cohesion2(GF_MAXPH), dilat1(GF_MAXPH), dilat2(GF_MAXPH)
     real(c_double) :: conduct(GF_MAXPH), cp(GF_MAXPH)
+    real(c_double) :: topo_kappa
 end type
 
 type, bind(C) :: gf_host_arrays_t

@@ -235,6 +235,8 @@ def to_gf_params(p, vbc, ivis_present, pisos=None):
               "fmagma_max", "weaken_ratio_plastic", "weaken_ratio_viscous", "rho_magma"):
         setattr(g, k, float(p[k]))
     g.vbc = float(vbc)
+    g.topo_kappa = float(p.get("topo_kappa", 0.0))
+    g.flexure_base = int(any(b["nbc"] == 200 and b["nofside"] == 2 for b in p["bcs"]))
     g.pisos = float(p["pisos"] if pisos is None else pisos)
     if p["nphase"] > abi.MAXPH:
         raise InpError("nphase exceeds maxph")

@@ -60,7 +60,7 @@ typedef struct gf_params {
     int32_t inhom;
     int32_t irheol[GF_MAXPH];
     int32_t ix1[GF_MAXINH], ix2[GF_MAXINH], iy1[GF_MAXINH], iy2[GF_MAXINH], igeom[GF_MAXINH];
-    int32_t pad0;
+    int32_t flexure_base;                   /* 1 when side 2 carries nbc == 200 (src/fl_flexure.f90:15-24) */
     double t_top, t_bot, bot_bc, g, demf, ten_off, tau_heal, v_min, v_max;
     double frac, fracm, strain_inert, dt_scale, xReyn, vbc, rxbo, rzbo;
     double pisos, drosub, fmagma_max, weaken_ratio_plastic, weaken_ratio_viscous, rho_magma;
@@ -69,6 +69,7 @@ typedef struct gf_params {
     double plstrain1[GF_MAXPH], plstrain2[GF_MAXPH], fric1[GF_MAXPH], fric2[GF_MAXPH];
     double cohesion1[GF_MAXPH], cohesion2[GF_MAXPH], dilat1[GF_MAXPH], dilat2[GF_MAXPH];
     double conduct[GF_MAXPH], cp[GF_MAXPH];
+    double topo_kappa;                      /* surface diffusion, src/fl_move.f90:40 */
 } gf_params;
 
 /* Raw pointers to the host arrays (module arrays, src/arrays.f90:9-53).  Any

@@ -87,6 +87,39 @@ def test_subroutine_bitwise_equal_to_oracle(sub, step):
 
 
 @pytest.mark.gpu
+def test_interleaved_calls_bitwise_equal_to_oracle():
+    """The step leaves the element move's stress/strain rotation to the next fl_rheol.
+    Steps and single subroutines in an order that crosses each point where the
+    rotation must be applied first -- fl_node or rmasses right after fl_move, two
+    moves in a row, a viscosity-update step, a download in the middle -- stay bitwise
+    equal to the oracle."""
+    from geoflac_amd import gpu
+    seq = [("step", 3), ("fl_node", 0), ("fl_move", 0), ("rmasses", 0), ("fl_move", 0), ("fl_move", 0),
+           ("fl_rheol", 0), ("download", 0), ("step", 4), ("fl_move", 0), ("fl_node", 0), ("step", 12),
+           ("fl_move", 0)]
+    _say("case: interleaved steps and subroutines from the 10-step golden state of %s: %s" % (CASE, seq))
+    p = ol.golden_params(CASE)
+    st = golden_full(10)
+    ref = st.copy()
+    with gpu.Model(p, st) as m:
+        for what, n in seq:
+            if what == "step":
+                m.step(n)
+                assert ol.step(p, ref, n, cr=True) == 0
+            elif what == "download":
+                m.download()
+            else:
+                m.call(what)
+                assert ol.call("gfo_" + what, p, ref, cr=True) == 0
+        m.download()
+    bad, worst, nexact = compare(st, ref, tol=0.0)
+    _say("  exact fields %d, differing %s" % (nexact, worst))
+    assert not bad, bad
+    assert st.clock.nloop == ref.clock.nloop
+    assert st.clock.dt == ref.clock.dt
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("nsteps", [1, 10, 100, 1000])
 def test_steps_match_reference(nsteps):
     from geoflac_amd import gpu

@@ -25,8 +25,8 @@ def test_inp_reader_matches_reference_read_params():
 
 
 def test_params_layout_matches_header():
-    # 546 int32 (even count, no hidden padding) followed by 23 + 19*20 doubles
-    assert abi.SIZEOF_PARAMS == 546 * 4 + (23 + 19 * abi.MAXPH) * 8
+    # 546 int32 (even count, no hidden padding) followed by 23 + 19*20 + 1 doubles
+    assert abi.SIZEOF_PARAMS == 546 * 4 + (23 + 19 * abi.MAXPH + 1) * 8
     assert abi.SIZEOF_CLOCK == 8 + 5 * 8
 
 
