@@ -35,9 +35,9 @@ def alg_bytes(kernel, E, N, nact, p):
         per_e += 128 + 8 + 24 + 24 + 8                                 # writes stress0 aps strain se2sr sshrheat
         per_n = 16 + 16                                                # vel, cord
         return E * per_e + N * per_n
-    if kernel == "k_therm_tile":
+    if kernel in ("k_therm_tile", "k_therm_pipe"):
         return N * (8 + 8 + 8 + 16) + E * (16 + 4 + 8)                # temp, temp0 out, x out, cord | area(1:2) iphase source
-    if kernel == "k_therm_tile_extra":
+    if kernel in ("k_therm_tile_extra", "k_therm_pipe_extra"):
         return N * (8 + 8 + 8 + 16) + E * (16 + 4 + 8)                # temp0, x in, x out, cord | area(1:2) iphase source
     if kernel == "k_node":
         return E * (96 + 32) + N * (16 + 16 + 16 + 8 + 16 + 16 + 16)  # stress0(1:3) area | cord rmass+amass vel ncod bc, vel+force out
@@ -255,8 +255,8 @@ def main():
     m.therm_stats()
     m.set_timing(True)
     m.step(args.profile_steps)
-    kernels = ["k_rheol1<false>", "k_rheol1<true>", "k_rheol", "k_visc<double>", "k_therm_tile",
-               "k_therm_tile_extra", "k_therm_check", "k_node<0>", "k_move_elem<0>",
+    kernels = ["k_rheol1<false>", "k_rheol1<true>", "k_rheol", "k_visc<double>", "k_therm_pipe",
+               "k_therm_pipe_extra", "k_therm_check", "k_node<0>", "k_move_elem<0>",
                "k_move_area", "k_move_rot<0>", "k_move_cord<double>", "k_therm_bc", "k_dens<double>", "k_rmass", "k_dt_reduce<double>", "k_amass",
                "k_clock"]
     times = {k: m.kernel_time(k) for k in kernels}

@@ -516,6 +516,304 @@ __global__ void __launch_bounds__(1024) k_therm_tile(const GfDevParams *__restri
     }
 }
 
+// ---------------------------------------------------------------- pipelined thermal tiles
+// The single-model step runs the tiles above with one workgroup per CU walking a
+// list of tiles.  A tile's inputs are needed only while its coefficients are built;
+// as soon as that is done, each thread loads its share of the next tile's inputs into
+// registers and runs the current tile's sweeps meanwhile, then moves them to LDS.
+// So the memory latency of a tile hides behind the previous tile's sweeps instead of
+// being paid by every CU at the same moment.  Barriers inside the pipeline wait for
+// LDS operations only: the stock workgroup barrier would also wait for the loads in
+// flight.
+#define TR_NLP 1216   // node arrays: 66 x 18 = 1188 doubles (padded)
+#define TR_NEP 1120   // element arrays: 65 x 17 = 1105 (padded)
+static_assert(TH_LJ * TH_LI <= TR_NLP && TH_NE <= TR_NEP, "footprint arrays");
+static_assert(TH_LJ * TH_LI <= 2 * TH_J * TH_I && TH_NE <= 2 * TH_J * TH_I, "two slots per thread");
+static_assert(2 * TH_LJ * TH_LI <= TH_NF * TH_NE, "the iterates fit in the element table");
+
+struct ThermRaw {
+    double t[TR_NLP], cx[TR_NLP], cy[TR_NLP];     // old temperature, coordinates
+    double x[TR_NLP];                             // starting iterate (passes > 0)
+    double a0[TR_NEP], a1[TR_NEP], src[TR_NEP];   // areas of triangles A and B, heat source
+    double sh[TR_NEP];                            // shear heating (when it heats)
+    int iph[TR_NEP];                              // phase
+};
+// one thread's share of a tile's inputs: node slots q and q + 1024, element slots
+// e and e + 1024 (0 outside the mesh)
+struct ThermPf {
+    double t[2], cx[2], cy[2], x[2], a0[2], a1[2], src[2], sh[2];
+    int iph[2];
+};
+
+// workgroup barrier after this wave's LDS operations (outstanding global loads and
+// stores, the prefetch among them, are not waited for)
+GF_DEV void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+GF_DEV void therm_tile_origin(const GfDevParams *P, int tile, int &j0, int &i0)
+{
+    const int ntj = therm_ntj(P);
+    j0 = (tile % ntj) * TH_OJ + 1 - TH_HJ;  // 1-based row of tile row 0
+    i0 = (tile / ntj) * TH_OI + 1 - TH_HI;  // 1-based column of tile column 0
+}
+
+// node q = (q % 66, q / 66) - 1 of the footprint, element e = (e % 65, e / 65) - 1 of
+// the table
+GF_DEV ThermPf therm_prefetch(const GfDevParams *__restrict__ P, const GfDevArrays &A, int pass, int tile,
+                              bool with_dissip)
+{
+    int j0, i0;
+    therm_tile_origin(P, tile, j0, i0);
+    const double *__restrict__ told_g = pass == 0 ? A.temp : A.temp0;
+    const double *__restrict__ xin = (pass % 2 == 1) ? A.xa : A.xb;
+    ThermPf f;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const int q = threadIdx.x + k * TH_J * TH_I;
+        const int jj = j0 + q % TH_LJ - 1, ii = i0 + q / TH_LJ - 1;
+        f.t[k] = f.cx[k] = f.cy[k] = f.x[k] = 0.0;
+        if (q < TH_LJ * TH_LI && jj >= 1 && jj <= NZ && ii >= 1 && ii <= NX) {
+            const long ix = IN(jj, ii);
+            f.t[k] = told_g[ix];
+            f.cx[k] = A.cord[ix];
+            f.cy[k] = A.cord[ix + NN];
+            if (pass > 0) f.x[k] = xin[ix];
+        }
+        const int je = j0 + q % TH_EJ - 1, ie = i0 + q / TH_EJ - 1;
+        f.a0[k] = f.a1[k] = f.src[k] = f.sh[k] = 0.0;
+        f.iph[k] = 1;
+        if (q < TH_NE && je >= 1 && je <= NZ - 1 && ie >= 1 && ie <= NX - 1) {
+            const long ix = IE(je, ie);
+            f.a0[k] = A.area[ix];
+            f.a1[k] = A.area[ix + NE];
+            f.src[k] = A.source[ix];
+            f.iph[k] = A.iphase[ix];
+            if (with_dissip) f.sh[k] = A.shrheat[ix];
+        }
+    }
+    return f;
+}
+GF_DEV void therm_store_raw(const ThermPf &f, ThermRaw &R)
+{
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const int q = threadIdx.x + k * TH_J * TH_I;
+        if (q < TH_LJ * TH_LI) {
+            R.t[q] = f.t[k];
+            R.cx[q] = f.cx[k];
+            R.cy[q] = f.cy[k];
+            R.x[q] = f.x[k];
+        }
+        if (q < TH_NE) {
+            R.a0[q] = f.a0[k];
+            R.a1[q] = f.a1[k];
+            R.src[q] = f.src[k];
+            R.sh[q] = f.sh[k];
+            R.iph[q] = f.iph[k];
+        }
+    }
+}
+
+// One tile from its inputs in R (the arithmetic of therm_tile_one, bit for bit);
+// loads tile `next` (if >= 0) during the sweeps and leaves it in R.
+// sU: element table by rounds, then the two iterates.
+GF_DEV void therm_tile_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, int pass,
+                            int tile, int next, ThermRaw &R, double *sU, const double (*sPh)[GF_MAXPH], int *sFlag,
+                            bool with_dissip, double dt)
+{
+    double *__restrict__ xout = (pass % 2 == 0) ? A.xa : A.xb;
+    double *__restrict__ bord = A.tbord;
+    constexpr int NL = TH_LI * TH_LJ, NT = TH_J * TH_I;
+    double(*sX)[NL] = reinterpret_cast<double(*)[NL]>(sU);
+    int j0, i0;
+    therm_tile_origin(P, tile, j0, i0);
+    const int tj = threadIdx.x % TH_J, ti = threadIdx.x / TH_J;
+#define LIDX(lj, li) (((li) + 1) * TH_LJ + ((lj) + 1))
+    const int e0 = threadIdx.x, e1 = threadIdx.x + NT;
+    auto elem_ok = [&](int e) {
+        const int je = j0 + e % TH_EJ - 1, ie = i0 + e / TH_EJ - 1;
+        return e < TH_NE && je >= 1 && je <= NZ - 1 && ie >= 1 && ie <= NX - 1;
+    };
+    const bool ein0 = elem_ok(e0), ein1 = elem_ok(e1);
+    // round 0: ra13, P = dummye*ra13, Q = 0.5*diff*inv_area of triangles A and B;
+    // rounds 1 and 2: the K(a,b) products of A, of B (as in therm_tile_one)
+    auto elem_store = [&](int e, int round) {
+        if (round == 0) {
+            const int ph = R.iph[e];
+            const double dm = (R.src[e] + R.sh[e] / sPh[0][ph - 1]) / sPh[1][ph - 1];
+            const double df = sPh[2][ph - 1];
+            for (int tri = 0; tri < 2; tri++) {
+                const double inv_area = 1.0 / (tri == 0 ? R.a0[e] : R.a1[e]);
+                const double ra13 = inv_area / 6.0;
+                sU[(3 * tri + 0) * TH_NE + e] = ra13;
+                sU[(3 * tri + 1) * TH_NE + e] = dm * ra13;
+                sU[(3 * tri + 2) * TH_NE + e] = 0.5 * df * inv_area;
+            }
+            return;
+        }
+        const double ar = round == 1 ? R.a0[e] : R.a1[e];
+        const int ej = e % TH_EJ - 1, ei = e / TH_EJ - 1;
+        double d[6];
+        if (round == 1) {  // A: (ej,ei), (ej+1,ei), (ej,ei+1)
+            const int n1 = LIDX(ej, ei), n2 = LIDX(ej + 1, ei), n3 = LIDX(ej, ei + 1);
+            const double y1 = R.cy[n1], y2 = R.cy[n2], y3 = R.cy[n3], x1 = R.cx[n1], x2 = R.cx[n2], x3 = R.cx[n3];
+            d[0] = (y2 - y3) * ar; d[1] = (y3 - y1) * ar; d[2] = (y1 - y2) * ar;
+            d[3] = (x3 - x2) * ar; d[4] = (x1 - x3) * ar; d[5] = (x2 - x1) * ar;
+        } else {           // B: (ej+1,ei), (ej,ei+1), (ej+1,ei+1)
+            const int n2 = LIDX(ej + 1, ei), n3 = LIDX(ej, ei + 1), n4 = LIDX(ej + 1, ei + 1);
+            const double y2 = R.cy[n2], y3 = R.cy[n3], y4 = R.cy[n4], x2 = R.cx[n2], x3 = R.cx[n3], x4 = R.cx[n4];
+            d[0] = (y2 - y4) * ar; d[1] = (y4 - y3) * ar; d[2] = (y3 - y2) * ar;
+            d[3] = (x4 - x2) * ar; d[4] = (x3 - x4) * ar; d[5] = (x2 - x3) * ar;
+        }
+        sU[0 * TH_NE + e] = KAB(d, 1, 1);
+        sU[1 * TH_NE + e] = KAB(d, 1, 2);
+        sU[2 * TH_NE + e] = KAB(d, 1, 3);
+        sU[3 * TH_NE + e] = KAB(d, 2, 2);
+        sU[4 * TH_NE + e] = KAB(d, 2, 3);
+        sU[5 * TH_NE + e] = KAB(d, 3, 3);
+    };
+    const int j = j0 + tj, i = i0 + ti;
+    const bool valid = j >= 1 && j <= NZ && i >= 1 && i <= NX;
+    const bool is_out = valid && tj >= TH_HJ && tj < TH_J - TH_HJ && ti >= TH_HI;
+    ThermCoef c;
+    c.mask = 0;
+    if (valid) {
+        if (j != 1 && i != 1) c.mask |= 1;
+        if (j != 1 && i != NX) c.mask |= 2 | 4;
+        if (j != NZ && i != 1) c.mask |= 8 | 16;
+        if (j != NZ && i != NX) c.mask |= 32;
+    }
+    c.area_n = 0.0;
+    // the element of each of the node's six triangles (t1 B, t2 A, t3 B, t4 A, t5 B, t6 A):
+    // EIDX(tj - 1, ti - 1) plus a constant
+    const int tb = EIDX(tj - 1, ti - 1);
+    constexpr int toff[6] = {0, TH_EJ, TH_EJ, 1, 1, TH_EJ + 1};
+    const int nb = LIDX(tj, ti);  // this node in the footprint arrays
+    for (int round = 0; round < 3; round++) {
+        if (ein0) elem_store(e0, round);
+        if (ein1) elem_store(e1, round);
+        lds_barrier();
+        if (valid) {
+            // old temperature at (j + dj, i + di), element table value f of triangle t
+            auto T = [&](int dj, int di) { return R.t[nb + di * TH_LJ + dj]; };
+            auto F = [&](int t, int f) { return sU[f * TH_NE + tb + toff[t]]; };
+            const double tme = T(0, 0);
+            if (round == 0) {
+#pragma unroll
+                for (int t = 0; t < 6; t++) {
+                    if (!(c.mask & (1 << t))) continue;
+                    const int tri = (t == 0 || t == 2 || t == 4) ? 1 : 0;
+                    c.area_n = c.area_n + F(t, 3 * tri);
+                    c.P[t] = F(t, 3 * tri + 1);
+                    c.Q[t] = F(t, 3 * tri + 2);
+                }
+            } else if (round == 1) {  // triangles A: K11 K12 K13 K22 K23 K33 at 0..5
+                if (c.mask & 2) {
+                    c.k21 = F(1, 1); c.c22 = tme * F(1, 3); c.c23 = T(-1, 1) * F(1, 4);
+                }
+                if (c.mask & 8) {
+                    c.k41 = F(3, 2); c.k42 = F(3, 4); c.c43 = tme * F(3, 5);
+                }
+                if (c.mask & 32) c.s6 = tme * F(5, 0) + T(1, 0) * F(5, 1) + T(0, 1) * F(5, 2);
+            } else {  // triangles B
+                if (c.mask & 1) {
+                    c.k11 = F(0, 2); c.k12 = F(0, 4); c.c13 = tme * F(0, 5);
+                }
+                if (c.mask & 4) c.s3 = T(-1, 1) * F(2, 1) + tme * F(2, 3) + T(0, 1) * F(2, 4);
+                if (c.mask & 16) {
+                    c.c51 = tme * F(4, 0); c.k52 = F(4, 1); c.c53 = T(1, 0) * F(4, 2);
+                }
+            }
+        }
+        lds_barrier();
+    }
+    c.told = valid ? R.t[nb] : 0.0;
+    // starting iterate: the old temperature in pass 0, the previous pass's result later
+    {
+        const double v0 = pass > 0 ? R.x[e0] : R.t[e0];
+        sX[0][e0] = v0;
+        sX[1][e0] = v0;
+        if (e1 < NL) {
+            const double v1 = pass > 0 ? R.x[e1] : R.t[e1];
+            sX[0][e1] = v1;
+            sX[1][e1] = v1;
+        }
+        if (threadIdx.x < 3) sFlag[threadIdx.x] = 0;
+    }
+    lds_barrier();
+    ThermPf pf;
+    if (next >= 0) pf = therm_prefetch(P, A, pass, next, with_dissip);  // R is free now
+
+    const int me = nb, up = nb - 1, lf = nb - TH_LJ, dl = nb - TH_LJ + 1;
+    int cur = 0, nsweep = 0;
+    bool still = false;
+    for (; nsweep < TH_SWEEP0; nsweep++) {
+        if (valid) sX[cur ^ 1][me] = therm_eval(c, sX[cur][up], sX[cur][lf], sX[cur][dl], dt);
+        lds_barrier();
+        cur ^= 1;
+    }
+    // stillness: flag nsweep % 3 is raised before the barrier and read after it; the
+    // flag used two sweeps later is cleared meanwhile
+    while (nsweep < TH_MAXSWEEP) {
+        if (valid) {
+            const double tn = therm_eval(c, sX[cur][up], sX[cur][lf], sX[cur][dl], dt);
+            if (__double_as_longlong(tn) != __double_as_longlong(sX[cur][me])) sFlag[nsweep % 3] = 1;
+            sX[cur ^ 1][me] = tn;
+        }
+        lds_barrier();
+        const int any = sFlag[nsweep % 3];
+        if (threadIdx.x == 0) sFlag[(nsweep + 2) % 3] = 0;
+        cur ^= 1;
+        nsweep++;
+        if (!any) {
+            still = true;
+            break;
+        }
+    }
+    // the next tile's inputs go to R before this tile's outputs are stored, so that
+    // waiting for the loads does not wait for the stores
+    if (next >= 0) therm_store_raw(pf, R);
+    if (is_out) {
+        xout[IN(j, i)] = sX[cur][me];
+        if (pass == 0) A.temp0[IN(j, i)] = c.told;
+    }
+    if (threadIdx.x < TH_NB) {
+        int lj, li;
+        therm_border_pos(threadIdx.x, lj, li);
+        bord[(long)tile * TH_NB + threadIdx.x] = sX[cur][LIDX(lj, li)];
+    }
+    if (threadIdx.x == 0 && !still) atomicOr(&ck->therm_pass_changed[pass], 2u);
+#undef LIDX
+}
+
+// Workgroup w (of a grid that is a multiple of 8, one per CU) runs on XCD w % 8, which
+// owns the contiguous tile range [x q, (x+1) q); its r-th workgroup there takes tiles
+// x q + r, + R, + 2R, ... with R = grid / 8.
+__global__ void __launch_bounds__(1024) k_therm_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
+                                                     int pass)
+{
+    if (pass > 0 && ck->therm_pass_changed[pass - 1] == 0u) return;  // already proven
+    __shared__ ThermRaw raw;
+    __shared__ double sU[TH_NF * TH_NE];
+    __shared__ double sPh[3][GF_MAXPH];
+    __shared__ int sFlag[3];
+    for (int q = threadIdx.x; q < GF_MAXPH; q += blockDim.x) {
+        sPh[0][q] = P->ph[q].den;
+        sPh[1][q] = P->ph[q].cp;
+        sPh[2][q] = P->ph[q].diff;
+    }
+    const double dt = ck->dt;
+    const bool with_dissip = P->ishearh == 1 && P->itherm != 2;
+    const int ntiles = therm_ntj(P) * therm_nti(P);
+    const int q = (ntiles + 7) / 8, x = blockIdx.x % 8, r = blockIdx.x / 8, RW = gridDim.x / 8;
+    const int first = x * q + r, end = min((x + 1) * q, ntiles);
+    if (first >= end) return;
+    therm_store_raw(therm_prefetch(P, A, pass, first, with_dissip), raw);
+    for (int t = first; t < end; t += RW) {
+        lds_barrier();  // this tile's inputs are in; the previous tile is done with sU
+        therm_tile_pipe(P, ck, A, pass, t, t + RW < end ? t + RW : -1, raw, sU, sPh, sFlag, with_dissip, dt);
+    }
+}
+
 // Compares each tile's border values with the global iterate written by the owning
 // tiles.  A border node whose tile value differs is harmless when every output node
 // that reads it still satisfies x = F(x) with the global values: those (at most three)

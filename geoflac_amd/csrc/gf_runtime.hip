@@ -51,6 +51,7 @@ struct gf_ctx {
     char msg[512] = {0};
     bool rheol_quad = false;  // four lanes per element (GF_RHEOL=quad) instead of one
     bool rot_pending = false; // the last element move's stress/strain rotation is not applied yet
+    unsigned therm_grid = 8;  // workgroups of the pipelined thermal kernel: the CUs, a multiple of 8
     // per-kernel event timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -205,6 +206,15 @@ extern "C" int gf_create(const gf_params *p, int32_t precision, gf_ctx **out)
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         rc = set_err(ctx, GF_ERR_HIP, "hipStreamCreate failed");
         goto fail;
+    }
+    {
+        int dev = 0, ncu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+            rc = set_err(ctx, GF_ERR_HIP, "device query failed");
+            goto fail;
+        }
+        ctx->therm_grid = (unsigned)std::max(8, ncu / 8 * 8);
     }
     ALLOC(ctx->dp, sizeof(GfDevParams));
     ALLOC(ctx->dck, sizeof(GfDevClock));
@@ -632,20 +642,19 @@ static int enq_therm(gf_ctx *ctx)
     int rc;
     if ((rc = enq_clock(ctx, CK_THERM_BEGIN))) return rc;
     const int ntiles = ((ctx->p.nz + TH_OJ - 1) / TH_OJ) * ((ctx->p.nx + TH_OI - 1) / TH_OI);
-    const unsigned nwg = (unsigned)(8 * ((ntiles + 7) / 8));
     for (int pass = 0; pass < TH_MAXPASS; pass++) {
-        const dim3 grid(pass == 0 ? nwg : std::min(nwg, 512u));
+        const dim3 grid(ctx->therm_grid);
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (ctx->timing) {
             e0 = next_event(ctx);
             e1 = next_event(ctx);
             if (e0) (void)hipEventRecord(e0, ctx->stream);
         }
-        hipLaunchKernelGGL(k_therm_tile, grid, dim3(TH_J * TH_I), 0, ctx->stream, ctx->dp, ctx->dck, ctx->a, pass);
-        if ((rc = launch_check(ctx, "k_therm_tile"))) return rc;
+        hipLaunchKernelGGL(k_therm_pipe, grid, dim3(TH_J * TH_I), 0, ctx->stream, ctx->dp, ctx->dck, ctx->a, pass);
+        if ((rc = launch_check(ctx, "k_therm_pipe"))) return rc;
         if (ctx->timing && e0 && e1) {
             (void)hipEventRecord(e1, ctx->stream);
-            ctx->pending.push_back({pass == 0 ? "k_therm_tile" : "k_therm_tile_extra", {e0, e1}});
+            ctx->pending.push_back({pass == 0 ? "k_therm_pipe" : "k_therm_pipe_extra", {e0, e1}});
         }
         LAUNCH(k_therm_check, (long)ntiles * TH_NB, ctx->dp, ctx->dck, ctx->a, pass);
     }
