@@ -1748,18 +1748,20 @@ __global__ void __launch_bounds__(256) k_rheol(const GfDevParams *__restrict__ P
 }
 
 // ------------------------------------------------------------------ fl_node
-// force of triangle k (1..4) of element (je,ie) on its local vertex m (1..3)
+// Force of triangle k (1..4) of an element on its local vertex m (1..3), from values
+// already loaded: corners 1..4 of the element (x, y), the triangle's area a, its
+// stress components s1 s2 s3 in the stored precision and the element's isotropic
+// reference pr (mixed precision only).
 // Mode 1 (FP32 arithmetic): fx, fy are the FP32 forces of the stress deviation, and
 // rx, ry (FP64) add those of the element's isotropic reference -pref*I.  Mode 2: the
 // stored deviation is turned back into the absolute stress, in FP64.
 template <int M>
-GF_DEV void dev_tri_force(const GfDevParams *P, const GfDevArrays &A, int je, int ie, int k, int m,
-                          typename Prec<M>::AT &fx, typename Prec<M>::AT &fy, double &rx, double &ry)
+GF_DEV void tri_force_v(const double *x, const double *y, double a, typename Prec<M>::SV s1v,
+                        typename Prec<M>::SV s2v, typename Prec<M>::SV s3v, double pr, int k, int m,
+                        typename Prec<M>::AT &fx, typename Prec<M>::AT &fy, double &rx, double &ry)
 {
-    const double x1 = CORD(je, ie, 1), y1 = CORD(je, ie, 2), x2 = CORD(je + 1, ie, 1), y2 = CORD(je + 1, ie, 2);
-    const double x3 = CORD(je, ie + 1, 1), y3 = CORD(je, ie + 1, 2), x4 = CORD(je + 1, ie + 1, 1),
-                 y4 = CORD(je + 1, ie + 1, 2);
-    const double a = AREA(je, ie, k);
+    typedef typename Prec<M>::AT AT;
+    const double x1 = x[0], y1 = y[0], x2 = x[1], y2 = y[1], x3 = x[2], y3 = y[2], x4 = x[3], y4 = y[3];
     double dx = 0, dz = 0;
     switch (k) {
     case 1:
@@ -1783,21 +1785,15 @@ GF_DEV void dev_tri_force(const GfDevParams *P, const GfDevArrays &A, int je, in
         if (m == 3) { dx = (y1 - y4) * a; dz = (x4 - x1) * a; }
         break;
     }
-    typedef typename Prec<M>::SV SV;
-    typedef typename Prec<M>::AT AT;
     const AT factor = AT(0.25 / a);
-    const SV *s = sv_ptr<SV>(A.stress0, A.stress0f);
-    const long ix = IE(je, ie) + NE * 4 * (k - 1);
-    AT s1 = s[ix], s2 = s[ix + NE], s3 = s[ix + 2 * NE];
+    AT s1 = s1v, s2 = s2v, s3 = s3v;
     if (M == 2) {
-        const double pr = ref_press<SV>(A, IE(je, ie));
         s1 = s1 - pr;
         s2 = s2 - pr;
     }
     fx = factor * (s1 * AT(dx) + s3 * AT(dz));
     fy = factor * (s3 * AT(dx) + s2 * AT(dz));
     if (M == 1) {
-        const double pr = ref_press<SV>(A, IE(je, ie));
         rx = -(0.25 / a) * pr * dx;
         ry = -(0.25 / a) * pr * dz;
     }
@@ -1824,18 +1820,54 @@ GF_DEV void node_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const G
         g1 = force[t];
         g2 = force[t + NN];
     }
-    const int kk[4][3] = {{2, 3, 4}, {1, 2, 3}, {1, 2, 4}, {1, 3, 4}};
-    const int mm[4][3] = {{3, 3, 2}, {2, 2, 2}, {3, 1, 3}, {1, 1, 1}};
-    const int ej[4] = {j - 1, j - 1, j, j}, ei[4] = {i - 1, i, i - 1, i};
+    constexpr int kk[4][3] = {{2, 3, 4}, {1, 2, 3}, {1, 2, 4}, {1, 3, 4}};
+    constexpr int mm[4][3] = {{3, 3, 2}, {2, 2, 2}, {3, 1, 3}, {1, 1, 1}};
+    // elements (j-1,i-1) (j-1,i) (j,i-1) (j,i) as offsets from the node
+    constexpr int dje[4] = {-1, -1, 0, 0}, die[4] = {-1, 0, -1, 0};
     const bool okk[4] = {j != 1 && i != 1, j != 1 && i != NX, j != NZ && i != 1, j != NZ && i != NX};
+    // Every load is issued before the first use, through buffer descriptors: offsets
+    // of missing neighbours fall outside the arrays (read 0) or on values that are
+    // not used.
+    const unsigned nzu = (unsigned)NZ, nnu = (unsigned)NN, neu = (unsigned)NE;
+    const Buf bC = mkbuf(A.cord, 16ull * nnu), bA = mkbuf(A.area, 32ull * neu);
+    const Buf bS = mkbuf(sv_ptr<SV>(A.stress0, A.stress0f), 16ull * sizeof(SV) * neu);
+    double cx[3][3], cy[3][3];  // nodes (j + dj, i + di) at [dj + 1][di + 1]
+#pragma unroll
+    for (int dj = -1; dj <= 1; dj++)
+#pragma unroll
+        for (int di = -1; di <= 1; di++) {
+            const unsigned o = (unsigned)((j - 1 + dj) + (int)nzu * (i - 1 + di)) * 8u;
+            cx[dj + 1][di + 1] = bld(bC, o, 0);
+            cy[dj + 1][di + 1] = bld(bC, o, nnu * 8u);
+        }
+    double ar[4][3];
+    SV st[4][3][3];
+    double pr[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int el = 0; el < 4; el++) {
+        const unsigned e = (unsigned)((j - 1 + dje[el]) + ((int)nzu - 1) * (i - 1 + die[el]));
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const unsigned kb = (unsigned)(kk[el][k] - 1);
+            ar[el][k] = bld(bA, e * 8u, kb * neu * 8u);
+#pragma unroll
+            for (int c = 0; c < 3; c++) st[el][k][c] = bldt<SV>(bS, e * (unsigned)sizeof(SV), (c + 4 * kb) * neu * (unsigned)sizeof(SV));
+        }
+        if (M != 0 && okk[el]) pr[el] = ref_press<SV>(A, (long)e);
+    }
 #pragma unroll
     for (int el = 0; el < 4; el++) {
         if (!okk[el]) continue;
+        // corners 1 (je,ie), 2 (je+1,ie), 3 (je,ie+1), 4 (je+1,ie+1)
+        const int a0 = dje[el] + 1, b0 = die[el] + 1;
+        const double x[4] = {cx[a0][b0], cx[a0 + 1][b0], cx[a0][b0 + 1], cx[a0 + 1][b0 + 1]};
+        const double y[4] = {cy[a0][b0], cy[a0 + 1][b0], cy[a0][b0 + 1], cy[a0 + 1][b0 + 1]};
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             AT fx, fy;
             double rx = 0, ry = 0;
-            dev_tri_force<M>(P, A, ej[el], ei[el], kk[el][k], mm[el][k], fx, fy, rx, ry);
+            tri_force_v<M>(x, y, ar[el][k], st[el][k][0], st[el][k][1], st[el][k][2], pr[el], kk[el][k], mm[el][k],
+                           fx, fy, rx, ry);
             g1 = g1 - fx;
             g2 = g2 - fy;
             if (M == 1) {
