@@ -909,8 +909,21 @@ __global__ void __launch_bounds__(256) k_therm_bc(const GfDevParams *__restrict_
 
 // ------------------------------------------------------------------ material properties
 // src/matprops.f90:136-215
-GF_DEV double dev_eff_visc(const GfDevParams *P, const GfDevArrays &A, int j, int i, double s11,
-                           double s22, double s33, const double *prv)
+// the node and element values Eff_visc reads, loaded together
+struct ViscIn {
+    double y[4], ysurf[2], t[4], srat;
+};
+GF_DEV ViscIn visc_inputs(const GfDevParams *P, const GfDevArrays &A, int j, int i)
+{
+    ViscIn v;
+    v.y[0] = CORD(j, i, 2); v.y[1] = CORD(j + 1, i, 2); v.y[2] = CORD(j, i + 1, 2); v.y[3] = CORD(j + 1, i + 1, 2);
+    v.ysurf[0] = CORD(1, i, 2); v.ysurf[1] = CORD(1, i + 1, 2);
+    v.t[0] = TEMP(j, i); v.t[1] = TEMP(j + 1, i); v.t[2] = TEMP(j, i + 1); v.t[3] = TEMP(j + 1, i + 1);
+    v.srat = A.e2sr[IE(j, i)];
+    return v;
+}
+GF_DEV double dev_eff_visc_in(const GfDevParams *P, int j, int i, double s11, double s22, double s33,
+                              const double *prv, const ViscIn &v)
 {
     const double adi_grad = 3.0e-1;
     for (int k = 0; k < P->inhom; k++) {
@@ -918,9 +931,9 @@ GF_DEV double dev_eff_visc(const GfDevParams *P, const GfDevArrays &A, int j, in
             return P->v_min;
     }
     double ev = 0.0;
-    double zcord = 0.25 * (CORD(j, i, 2) + CORD(j + 1, i, 2) + CORD(j, i + 1, 2) + CORD(j + 1, i + 1, 2));
-    double tmpr0 = 0.25 * (TEMP(j, i) + TEMP(j + 1, i) + TEMP(j, i + 1) + TEMP(j + 1, i + 1));
-    double zsurf = 0.50 * (CORD(1, i, 2) + CORD(1, i + 1, 2));
+    double zcord = 0.25 * (v.y[0] + v.y[1] + v.y[2] + v.y[3]);
+    double tmpr0 = 0.25 * (v.t[0] + v.t[1] + v.t[2] + v.t[3]);
+    double zsurf = 0.50 * (v.ysurf[0] + v.ysurf[1]);
     double tmpr = (zsurf - zcord) * 1.0e-3 * adi_grad + tmpr0;
     double pres = -1 * (s11 + s22 + s33) / 3.0;
     double deptmpr410 = -101.58 * (tmpr + 273.0) - 216828.0;
@@ -932,7 +945,7 @@ GF_DEV double dev_eff_visc(const GfDevParams *P, const GfDevArrays &A, int j, in
         ev = 1e21 * cr_exp(-0.0115 * (tmpr - P->t_bot));
         return fmin(P->v_max, fmax(P->v_min, ev));
     }
-    double srat = A.e2sr[IE(j, i)];
+    double srat = v.srat;
     if (srat == 0) srat = P->vbc_over_rxbo;
     for (int a = 0; a < P->nactive; a++) {
         double r = prv[a];
@@ -946,6 +959,11 @@ GF_DEV double dev_eff_visc(const GfDevParams *P, const GfDevArrays &A, int j, in
     }
     ev = 1 / ev;
     return fmin(P->v_max, fmax(P->v_min, ev));
+}
+GF_DEV double dev_eff_visc(const GfDevParams *P, const GfDevArrays &A, int j, int i, double s11,
+                           double s22, double s33, const double *prv)
+{
+    return dev_eff_visc_in(P, j, i, s11, s22, s33, prv, visc_inputs(P, A, j, i));
 }
 
 // src/rh_plastic.f90:231-291
@@ -1716,16 +1734,38 @@ GF_DEV void stress_avg(const GfDevParams *P, const GfDevArrays &A, long e, doubl
 }
 
 template <typename SV>
+// every load issued before the first use (one memory round trip per element)
 GF_DEV void visc_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e)
 {
-    const int irh = PHASE(A.iphase[e]).irheol;
-    if (!(irh == 3 || irh >= 11)) return;
     const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
+    const unsigned ne = (unsigned)NE, eo = (unsigned)e;
+    const int iph = A.iphase[e];
+    // phase ratios past nactive lie outside the buffer and read 0
+    const Buf bPR = mkbuf(A.pr, (unsigned long long)P->nactive * ne * 8ull);
     double prv[GF_MAXACT];
-    for (int a = 0; a < P->nactive; a++) prv[a] = A.pr[a * NE + e];
-    double a11, a22, a33;
-    stress_avg<SV>(P, A, e, a11, a22, a33);
-    A.visn[e] = dev_eff_visc(P, A, j, i, a11, a22, a33, prv);
+#pragma unroll
+    for (int a = 0; a < GF_MAXACT; a++) prv[a] = bld(bPR, eo * 8u, (unsigned)a * ne * 8u);
+    const Buf bS = mkbuf(sv_ptr<SV>(A.stress0, A.stress0f), 16ull * sizeof(SV) * ne);
+    SV sv[3][4];  // components 1, 2, 4 of triangles 1..4
+    constexpr int comp[3] = {0, 1, 3};
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) sv[c][k] = bldt<SV>(bS, eo * (unsigned)sizeof(SV), (comp[c] + 4 * k) * ne * (unsigned)sizeof(SV));
+    const ViscIn vin = visc_inputs(P, A, j, i);
+    const int irh = PHASE(iph).irheol;
+    if (!(irh == 3 || irh >= 11)) return;
+    // stress_avg on the loaded values
+    double a11 = 0.25 * ((double)sv[0][0] + (double)sv[0][1] + (double)sv[0][2] + (double)sv[0][3]);
+    double a22 = 0.25 * ((double)sv[1][0] + (double)sv[1][1] + (double)sv[1][2] + (double)sv[1][3]);
+    double a33 = 0.25 * ((double)sv[2][0] + (double)sv[2][1] + (double)sv[2][2] + (double)sv[2][3]);
+    if (sizeof(SV) == 4) {
+        const double pr = ref_press<SV>(A, e);
+        a11 = a11 - pr;
+        a22 = a22 - pr;
+        a33 = a33 - pr;
+    }
+    A.visn[e] = dev_eff_visc_in(P, j, i, a11, a22, a33, prv, vin);
 }
 template <typename SV>
 __global__ void __launch_bounds__(256) k_visc(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
@@ -2158,9 +2198,24 @@ GF_DEV void dens_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const G
     const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
     const double adi_grad = 3.0e-1, den_ramp_dist = 30.0e3, amp410 = 0.05, amp660 = 0.085;
     const int iph = A.iphase[e];
-    double zcord = 0.25 * (CORD(j, i, 2) + CORD(j + 1, i, 2) + CORD(j, i + 1, 2) + CORD(j + 1, i + 1, 2));
-    double tmpr0 = 0.25 * (TEMP(j, i) + TEMP(j + 1, i) + TEMP(j, i + 1) + TEMP(j + 1, i + 1));
-    double zsurf = 0.50 * (CORD(1, i, 2) + CORD(1, i + 1, 2));
+    // every load issued before the first use
+    const unsigned ne = (unsigned)NE, eo = (unsigned)e;
+    const Buf bPR = mkbuf(A.pr, (unsigned long long)P->nactive * ne * 8ull);
+    double prv[GF_MAXACT];
+#pragma unroll
+    for (int a = 0; a < GF_MAXACT; a++) prv[a] = bld(bPR, eo * 8u, (unsigned)a * ne * 8u);
+    const Buf bS = mkbuf(sv_ptr<SV>(A.stress0, A.stress0f), 16ull * sizeof(SV) * ne);
+    SV sv[4][3];  // components 1, 2, 4 of triangles 1..4
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        sv[k][0] = bldt<SV>(bS, eo * (unsigned)sizeof(SV), (0 + 4 * k) * ne * (unsigned)sizeof(SV));
+        sv[k][1] = bldt<SV>(bS, eo * (unsigned)sizeof(SV), (1 + 4 * k) * ne * (unsigned)sizeof(SV));
+        sv[k][2] = bldt<SV>(bS, eo * (unsigned)sizeof(SV), (3 + 4 * k) * ne * (unsigned)sizeof(SV));
+    }
+    const ViscIn vin = visc_inputs(P, A, j, i);
+    double zcord = 0.25 * (vin.y[0] + vin.y[1] + vin.y[2] + vin.y[3]);
+    double tmpr0 = 0.25 * (vin.t[0] + vin.t[1] + vin.t[2] + vin.t[3]);
+    double zsurf = 0.50 * (vin.ysurf[0] + vin.ysurf[1]);
     double tmpr = (zsurf - zcord) * 1.0e-3 * adi_grad + tmpr0;
     double deptmpr410 = -101.58 * (tmpr + 273.0) - 216828.0;
     double deptmpr420 = deptmpr410 - den_ramp_dist;
@@ -2178,15 +2233,12 @@ GF_DEV void dens_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const G
         den_amp = 1.05 + amp660 * fr;
     }
     double press = 0;
-    const SV *sg = sv_ptr<SV>(A.stress0, A.stress0f);
-    for (int k = 1; k <= 4; k++)
-        press = press - ((double)sg[e + NE * (4 * (k - 1))] + (double)sg[e + NE * (1 + 4 * (k - 1))] +
-                         (double)sg[e + NE * (3 + 4 * (k - 1))]);
+    for (int k = 0; k < 4; k++) press = press - ((double)sv[k][0] + (double)sv[k][1] + (double)sv[k][2]);
     if (sizeof(SV) == 4) press = press + 12.0 * ref_press<SV>(A, e);
     press = press / 12;
     double ed = 0.0;
     for (int a = 0; a < P->nactive; a++) {
-        double ratio = A.pr[a * NE + e];
+        double ratio = prv[a];
         if (ratio < 0.01) continue;
         const GfPhase &ph = PHASE(P->active[a]);
         double dens = ph.den * (1 - ph.alfa * tmpr + ph.beta * press);
@@ -2210,32 +2262,58 @@ __global__ void __launch_bounds__(256) k_dens(const GfDevParams *__restrict__ P,
 // the element of which it is corner c, the triangles listed in kTriOfCorner[c] (in
 // A,B,C,D order); the 4 elements around a node have 4 different colours, so the
 // gather visits them by colour to keep the same summation order.
-GF_DEV int colour_of(int je, int ie) { return 2 * ((ie - 1) & 1) + ((je - 1) & 1); }
 
-template <typename F>
-GF_DEV double gather_corners(const GfDevParams *P, int j, int i, F contrib)
+// The four elements around node (j,i) in the order the coloured loops of the reference
+// visit them (colour 0..3, then triangle).  The element of colour s is corner
+// c = s ^ c0 of the node (c0 the colour of element (j,i)): element (j - (c & 1),
+// i - (c >> 1)), whose triangles through the node are tri[c] = {1,3,4}, {1,2,3},
+// {1,2,4}, {2,3,4}.  Missing elements keep their slot with ok = false; their offsets
+// fall outside the arrays or on values that are not used.
+struct CornerSlots {
+    unsigned e[4];      // element index
+    unsigned kp[4][3];  // triangle plane (0..3) of the three triangles through the node
+    bool ok[4];
+};
+GF_DEV CornerSlots corner_slots(const GfDevParams *P, int j, int i)
 {
-    // element of which (j,i) is corner 1..4: (j,i), (j-1,i), (j,i-1), (j-1,i-1)
-    int ej[4] = {j, j - 1, j, j - 1}, ei[4] = {i, i, i - 1, i - 1};
-    bool ok[4] = {j < NZ && i < NX, j > 1 && i < NX, j < NZ && i > 1, j > 1 && i > 1};
-    const int tri[4][3] = {{1, 3, 4}, {1, 2, 3}, {1, 2, 4}, {2, 3, 4}};
-    double sum = 0.0;
-    for (int col = 0; col < 4; col++) {
-        for (int c = 0; c < 4; c++) {
-            if (!ok[c] || colour_of(ej[c], ei[c]) != col) continue;
-            for (int q = 0; q < 3; q++) sum = sum + contrib(ej[c], ei[c], tri[c][q]);
-        }
+    CornerSlots cs;
+    const int c0 = 2 * ((i - 1) & 1) + ((j - 1) & 1);
+    // tri[c][q] - 1, two bits each, entry 3c + q
+    constexpr unsigned TRI = (0u << 0) | (2u << 2) | (3u << 4) | (0u << 6) | (1u << 8) | (2u << 10) | (0u << 12) |
+                             (1u << 14) | (3u << 16) | (1u << 18) | (2u << 20) | (3u << 22);
+#pragma unroll
+    for (int sl = 0; sl < 4; sl++) {
+        const int c = sl ^ c0;
+        const int je = j - (c & 1), ie = i - (c >> 1);
+        cs.ok[sl] = je >= 1 && je <= NZ - 1 && ie >= 1 && ie <= NX - 1;
+        cs.e[sl] = (unsigned)((je - 1) + (NZ - 1) * (ie - 1));
+#pragma unroll
+        for (int q = 0; q < 3; q++) cs.kp[sl][q] = (TRI >> (2 * (3 * c + q))) & 3u;
     }
-    return sum;
+    return cs;
 }
 
 GF_DEV void rmass_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long t)
 {
     const int j = (int)(t % NZ) + 1, i = (int)(t / NZ) + 1;
     const double c1d12 = 1.0 / 12.0;
-    A.rmass[t] = gather_corners(P, j, i, [&](int je, int ie, int k) {
-        return c1d12 / AREA(je, ie, k) * A.dens[IE(je, ie)];
-    });
+    const CornerSlots cs = corner_slots(P, j, i);
+    const unsigned ne = (unsigned)NE;
+    const Buf bA = mkbuf(A.area, 32ull * ne), bD = mkbuf(A.dens, 8ull * ne);
+    double ar[4][3], dn[4];
+#pragma unroll
+    for (int sl = 0; sl < 4; sl++) {
+        dn[sl] = bld(bD, cs.e[sl] * 8u, 0);
+#pragma unroll
+        for (int q = 0; q < 3; q++) ar[sl][q] = bld(bA, cs.e[sl] * 8u + cs.kp[sl][q] * ne * 8u, 0);
+    }
+    double sum = 0.0;
+#pragma unroll
+    for (int sl = 0; sl < 4; sl++)
+        if (cs.ok[sl])
+#pragma unroll
+            for (int q = 0; q < 3; q++) sum = sum + c1d12 / ar[sl][q] * dn[sl];
+    A.rmass[t] = sum;
 }
 __global__ void __launch_bounds__(256) k_rmass(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
 {
@@ -2289,10 +2367,23 @@ GF_DEV void dt_reduce_body(const GfDevParams *__restrict__ P, GfDevClock *ck, co
     __shared__ double s_min[4], s_max[4];
     const long stride = (long)nblk * blockDim.x;
     double dlmin = 1.0e28, vmax = 0.0;
-    for (long t = (long)blk * blockDim.x + threadIdx.x; t < NE; t += stride)
-        dlmin = fmin(dlmin, elem_dlmin(P, A, t));
-    for (long n = (long)blk * blockDim.x + threadIdx.x; n < 2 * NN; n += stride)
-        vmax = fmax(vmax, fabs((double)vel[n]));
+    // four elements (eight velocities) per iteration, so that their loads are in
+    // flight together; min and max are exact in any order
+    const long ne = NE, n2 = 2 * NN;
+    for (long t = (long)blk * blockDim.x + threadIdx.x; t < ne; t += 4 * stride) {
+        double d[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) d[u] = t + u * stride < ne ? elem_dlmin(P, A, t + u * stride) : 1.0e28;
+#pragma unroll
+        for (int u = 0; u < 4; u++) dlmin = fmin(dlmin, d[u]);
+    }
+    for (long n = (long)blk * blockDim.x + threadIdx.x; n < n2; n += 8 * stride) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = n + u * stride < n2 ? fabs((double)vel[n + u * stride]) : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; u++) vmax = fmax(vmax, v[u]);
+    }
     for (int off = 32; off > 0; off >>= 1) {
         dlmin = fmin(dlmin, __shfl_xor(dlmin, off));
         vmax = fmax(vmax, __shfl_xor(vmax, off));
@@ -2365,16 +2456,33 @@ GF_DEV void amass_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const 
     const double dlmin = __longlong_as_double((long long)ck->dlmin_bits);
     const double vel_max = __longlong_as_double((long long)ck->velmax_bits);
     const double dt_elastic = ck->dt_elastic;
-    A.amass[t] = gather_corners(P, j, i, [&](int je, int ie, int k) {
-        const GfPhase &ph = PHASE(A.iphase[IE(je, ie)]);
+    const CornerSlots cs = corner_slots(P, j, i);
+    const unsigned ne = (unsigned)NE;
+    const Buf bA = mkbuf(A.area, 32ull * ne), bI = mkbuf(A.iphase, 4ull * ne);
+    double ar[4][3], pw[4];
+    int ip[4];
+#pragma unroll
+    for (int sl = 0; sl < 4; sl++) {
+        ip[sl] = bldi(bI, cs.e[sl] * 4u, 0);
+#pragma unroll
+        for (int q = 0; q < 3; q++) ar[sl][q] = bld(bA, cs.e[sl] * 8u + cs.kp[sl][q] * ne * 8u, 0);
+    }
+#pragma unroll
+    for (int sl = 0; sl < 4; sl++) pw[sl] = cs.ok[sl] ? PHASE(ip[sl]).pwave : 0.0;
+    double sum = 0.0;
+#pragma unroll
+    for (int sl = 0; sl < 4; sl++) {
+        if (!cs.ok[sl]) continue;
         double vel_sound = dlmin * P->frac / dt_elastic;
-        double rho_inert = ph.pwave / (vel_sound * vel_sound);
+        double rho_inert = pw[sl] / (vel_sound * vel_sound);
         if (P->i_rey == 1 && vel_max > 0.0) {
             double rho_inert2 = (P->xReyn * P->v_min) / (vel_max * fabs(P->rzbo));
             if (rho_inert > rho_inert2) rho_inert = rho_inert2;
         }
-        return c1d12 * rho_inert / AREA(je, ie, k);
-    });
+#pragma unroll
+        for (int q = 0; q < 3; q++) sum = sum + c1d12 * rho_inert / ar[sl][q];
+    }
+    A.amass[t] = sum;
 }
 __global__ void __launch_bounds__(256) k_amass(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
 {
