@@ -3,10 +3,11 @@
 // A parameter sweep holds hundreds of small models of different mesh sizes.  One
 // launch per kernel of the step covers all of them: the launch's index range is the
 // concatenation of the models' elements (or nodes, thermal tiles, border entries),
-// and each thread finds its model in a device table by binary search over the
-// prefix sums.  Every model keeps its own parameters, clock (dt, time, reductions)
-// and arrays, so the models stay independent; the kernel bodies are the single-model
-// ones.
+// each model's range starting at a multiple of 256.  A workgroup therefore lies in
+// one model, which it reads from a table of the model of each 256-index chunk (one
+// load, the same for every lane, so the model's entry is read with scalar loads).
+// Every model keeps its own parameters, clock (dt, time, reductions) and arrays, so
+// the models stay independent; the kernel bodies are the single-model ones.
 #pragma once
 #include "gf_kernels.h"
 
@@ -14,131 +15,132 @@ struct BatchEntry {
     const GfDevParams *P;
     GfDevClock *ck;
     GfDevArrays A;
-    long pe, pn, pt, pb;  // prefix sums: elements, nodes, thermal tiles, border entries
+    long pe, pn, pt, pb;  // first global index: elements, nodes, thermal tiles, border entries
+    long ce, cn, ct, cb;  // counts of the same
     const int *present;   // phases present (dt_mass minima)
     int npresent, pad;
 };
 
 enum { BK_ELEM = 0, BK_NODE = 1, BK_TILE = 2, BK_BORD = 3 };
+#define BATCH_ALIGN 256  // each model's element, node and border ranges start at a multiple
+
+// model of each 256-index chunk of the element, node and border ranges, and of each tile
+struct BatchMap {
+    const int *chunk[4];
+    int nm, pad;
+};
 
 template <int F>
 GF_DEV long bprefix(const BatchEntry &b)
 {
     return F == BK_ELEM ? b.pe : F == BK_NODE ? b.pn : F == BK_TILE ? b.pt : b.pb;
 }
-
-// model of global index g (tab[nm] holds the totals)
 template <int F>
-GF_DEV int bfind(const BatchEntry *__restrict__ tab, int nm, long g)
+GF_DEV long bcount(const BatchEntry &b)
 {
-    int lo = 0, hi = nm - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (bprefix<F>(tab[mid]) <= g)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    return lo;
+    return F == BK_ELEM ? b.ce : F == BK_NODE ? b.cn : F == BK_TILE ? b.ct : b.cb;
 }
 
-// calls fn(entry, local index) for the global index of this thread
+// calls fn(entry, local index) for the global index of this thread (workgroups of at
+// most BATCH_ALIGN threads, dividing it)
 template <int F, typename Fn>
-GF_DEV void batch_apply(const BatchEntry *__restrict__ tab, int nm, Fn fn)
+GF_DEV void batch_apply(const BatchEntry *__restrict__ tab, const BatchMap &map, Fn fn)
 {
-    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= bprefix<F>(tab[nm])) return;
-    const BatchEntry &b = tab[bfind<F>(tab, nm, g)];
-    fn(b, g - bprefix<F>(b));
+    const long g0 = (long)blockIdx.x * blockDim.x;
+    const int m = __builtin_amdgcn_readfirstlane(map.chunk[F][g0 / BATCH_ALIGN]);
+    const BatchEntry &b = tab[m];
+    const long l = g0 + threadIdx.x - bprefix<F>(b);
+    if (l >= bcount<F>(b)) return;
+    fn(b, l);
 }
 
-__global__ void kb_clock(const BatchEntry *__restrict__ tab, int nm, int op)
+__global__ void kb_clock(const BatchEntry *__restrict__ tab, BatchMap map, int op)
 {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m < nm) clock_op(tab[m].ck, op);
+    if (m < map.nm) clock_op(tab[m].ck, op);
 }
 
-__global__ void __launch_bounds__(1024) kb_therm_tile(const BatchEntry *__restrict__ tab, int nm, int pass)
+__global__ void __launch_bounds__(1024) kb_therm_tile(const BatchEntry *__restrict__ tab, BatchMap map, long total,
+                                                      int pass)
 {
-    const long total = tab[nm].pt;
     for (long g = blockIdx.x; g < total; g += gridDim.x) {
-        const BatchEntry &b = tab[bfind<BK_TILE>(tab, nm, g)];
+        const BatchEntry &b = tab[__builtin_amdgcn_readfirstlane(map.chunk[BK_TILE][g])];
         if (pass > 0 && b.ck->therm_pass_changed[pass - 1] == 0u) continue;  // this model is proven
         therm_tile_one(b.P, b.ck, b.A, pass, (int)(g - b.pt));
     }
 }
 
-__global__ void __launch_bounds__(256) kb_therm_check(const BatchEntry *__restrict__ tab, int nm, int pass)
+__global__ void __launch_bounds__(256) kb_therm_check(const BatchEntry *__restrict__ tab, BatchMap map, int pass)
 {
-    batch_apply<BK_BORD>(tab, nm, [&](const BatchEntry &b, long t) { therm_check_body(b.P, b.ck, b.A, pass, t); });
+    batch_apply<BK_BORD>(tab, map, [&](const BatchEntry &b, long t) { therm_check_body(b.P, b.ck, b.A, pass, t); });
 }
 
-__global__ void __launch_bounds__(256) kb_therm_bc(const BatchEntry *__restrict__ tab, int nm)
+__global__ void __launch_bounds__(256) kb_therm_bc(const BatchEntry *__restrict__ tab, BatchMap map)
 {
-    batch_apply<BK_NODE>(tab, nm, [&](const BatchEntry &b, long t) { therm_bc_body(b.P, b.ck, b.A, t); });
+    batch_apply<BK_NODE>(tab, map, [&](const BatchEntry &b, long t) { therm_bc_body(b.P, b.ck, b.A, t); });
 }
 
 template <typename SV>
-__global__ void __launch_bounds__(256) kb_visc(const BatchEntry *__restrict__ tab, int nm)
+__global__ void __launch_bounds__(256) kb_visc(const BatchEntry *__restrict__ tab, BatchMap map)
 {
-    batch_apply<BK_ELEM>(tab, nm, [&](const BatchEntry &b, long e) { visc_body<SV>(b.P, b.ck, b.A, e); });
+    batch_apply<BK_ELEM>(tab, map, [&](const BatchEntry &b, long e) { visc_body<SV>(b.P, b.ck, b.A, e); });
 }
 
 template <bool SHEARH, int M>
-__global__ void __launch_bounds__(256) kb_rheol(const BatchEntry *__restrict__ tab, int nm, int avgsr)
+__global__ void __launch_bounds__(256) kb_rheol(const BatchEntry *__restrict__ tab, BatchMap map, int avgsr)
 {
-    batch_apply<BK_ELEM>(tab, nm, [&](const BatchEntry &b, long e) {
+    batch_apply<BK_ELEM>(tab, map, [&](const BatchEntry &b, long e) {
         rheol_single<SHEARH, typename Prec<M>::SV, typename Prec<M>::AT>(b.P, b.P->ph, b.ck, b.A, e, avgsr, 0);
     });
 }
 
 template <int M>
-__global__ void __launch_bounds__(256) kb_node(const BatchEntry *__restrict__ tab, int nm)
+__global__ void __launch_bounds__(256) kb_node(const BatchEntry *__restrict__ tab, BatchMap map)
 {
-    batch_apply<BK_NODE>(tab, nm, [&](const BatchEntry &b, long t) { node_body<M>(b.P, b.ck, b.A, t); });
+    batch_apply<BK_NODE>(tab, map, [&](const BatchEntry &b, long t) { node_body<M>(b.P, b.ck, b.A, t); });
 }
 
 template <typename SV>
-__global__ void __launch_bounds__(256) kb_move_cord(const BatchEntry *__restrict__ tab, int nm)
+__global__ void __launch_bounds__(256) kb_move_cord(const BatchEntry *__restrict__ tab, BatchMap map)
 {
-    batch_apply<BK_NODE>(tab, nm, [&](const BatchEntry &b, long t) { move_cord_body<SV>(b.P, b.ck, b.A, t); });
+    batch_apply<BK_NODE>(tab, map, [&](const BatchEntry &b, long t) { move_cord_body<SV>(b.P, b.ck, b.A, t); });
 }
 
 template <int M>
-__global__ void __launch_bounds__(256) kb_move_elem(const BatchEntry *__restrict__ tab, int nm)
+__global__ void __launch_bounds__(256) kb_move_elem(const BatchEntry *__restrict__ tab, BatchMap map)
 {
-    batch_apply<BK_ELEM>(tab, nm, [&](const BatchEntry &b, long e) { move_elem_body<M>(b.P, b.ck, b.A, e); });
+    batch_apply<BK_ELEM>(tab, map, [&](const BatchEntry &b, long e) { move_elem_body<M>(b.P, b.ck, b.A, e); });
 }
 
 template <typename SV>
-__global__ void __launch_bounds__(256) kb_dens(const BatchEntry *__restrict__ tab, int nm)
+__global__ void __launch_bounds__(256) kb_dens(const BatchEntry *__restrict__ tab, BatchMap map)
 {
-    batch_apply<BK_ELEM>(tab, nm, [&](const BatchEntry &b, long e) { dens_body<SV>(b.P, b.ck, b.A, e); });
+    batch_apply<BK_ELEM>(tab, map, [&](const BatchEntry &b, long e) { dens_body<SV>(b.P, b.ck, b.A, e); });
 }
 
-__global__ void __launch_bounds__(256) kb_rmass(const BatchEntry *__restrict__ tab, int nm)
+__global__ void __launch_bounds__(256) kb_rmass(const BatchEntry *__restrict__ tab, BatchMap map)
 {
-    batch_apply<BK_NODE>(tab, nm, [&](const BatchEntry &b, long t) { rmass_body(b.P, b.ck, b.A, t); });
+    batch_apply<BK_NODE>(tab, map, [&](const BatchEntry &b, long t) { rmass_body(b.P, b.ck, b.A, t); });
 }
 
-__global__ void __launch_bounds__(256) kb_amass(const BatchEntry *__restrict__ tab, int nm)
+__global__ void __launch_bounds__(256) kb_amass(const BatchEntry *__restrict__ tab, BatchMap map)
 {
-    batch_apply<BK_NODE>(tab, nm, [&](const BatchEntry &b, long t) { amass_body(b.P, b.ck, b.A, t); });
+    batch_apply<BK_NODE>(tab, map, [&](const BatchEntry &b, long t) { amass_body(b.P, b.ck, b.A, t); });
 }
 
 // dt minima: `per` workgroups of 256 threads per model
 #define BATCH_DT_BLOCKS 16
 template <typename SV>
-__global__ void __launch_bounds__(256) kb_dt_reduce(const BatchEntry *__restrict__ tab, int nm)
+__global__ void __launch_bounds__(256) kb_dt_reduce(const BatchEntry *__restrict__ tab, BatchMap map)
 {
     const int m = blockIdx.x / BATCH_DT_BLOCKS;
-    if (m >= nm) return;
+    if (m >= map.nm) return;
     const BatchEntry &b = tab[m];
     dt_reduce_body<SV>(b.P, b.ck, b.A, blockIdx.x % BATCH_DT_BLOCKS, BATCH_DT_BLOCKS);
 }
 
-__global__ void kb_dt_scalars(const BatchEntry *__restrict__ tab, int nm)
+__global__ void kb_dt_scalars(const BatchEntry *__restrict__ tab, BatchMap map)
 {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m < nm) dt_scalars_body(tab[m].P, tab[m].ck, tab[m].present, tab[m].npresent);
+    if (m < map.nm) dt_scalars_body(tab[m].P, tab[m].ck, tab[m].present, tab[m].npresent);
 }

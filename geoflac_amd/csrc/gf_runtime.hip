@@ -1217,6 +1217,8 @@ struct gf_batch {
     bool shearh = false, no_mech = false;
     gf_params p0{};  // frequencies and switches shared by all models
     BatchEntry *dtab = nullptr;
+    int *dmap = nullptr;  // chunk -> model tables (BatchMap)
+    BatchMap map{};
     int nm = 0;
     long tot_e = 0, tot_n = 0, tot_t = 0, tot_b = 0;
     hipStream_t stream = nullptr;
@@ -1259,6 +1261,7 @@ extern "C" void gf_batch_destroy(gf_batch *b)
     for (hipEvent_t e : b->ev)
         if (e) (void)hipEventDestroy(e);
     if (b->dtab) (void)hipFree(b->dtab);
+    if (b->dmap) (void)hipFree(b->dmap);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -1276,7 +1279,9 @@ extern "C" int gf_batch_create(gf_ctx *const *models, int32_t n, gf_batch **out)
     const gf_ctx *c0 = models[0];
     if (!c0) return fail(GF_ERR_ARG, "null model");
     std::vector<BatchEntry> tab(n + 1);
-    long pe = 0, pn = 0, pt = 0;
+    long pe = 0, pn = 0, pt = 0, pb = 0;
+    auto align = [](long v) { return (v + BATCH_ALIGN - 1) / BATCH_ALIGN * BATCH_ALIGN; };
+    std::vector<int> me, mn, mt, mb;  // model of each chunk (tiles: of each tile)
     for (int k = 0; k < n; k++) {
         gf_ctx *c = models[k];
         if (!c || check_ready(c) != GF_OK) return fail(GF_ERR_STATE, "model not ready");
@@ -1288,15 +1293,36 @@ extern "C" int gf_batch_create(gf_ctx *const *models, int32_t n, gf_batch **out)
             return fail(GF_ERR_ARG, "models of a batch share precision, switches, frequencies and loop counters");
         if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(GF_ERR_HIP, "sync");
         const long ntiles = (long)((q.nz + TH_OJ - 1) / TH_OJ) * ((q.nx + TH_OI - 1) / TH_OI);
-        tab[k] = BatchEntry{c->dp, c->dck, c->a, pe, pn, pt, pt * TH_NB, c->d_present, c->npresent, 0};
-        pe += c->ne;
-        pn += c->nn;
+        tab[k] = BatchEntry{c->dp, c->dck, c->a, pe, pn, pt, pb, c->ne, c->nn, ntiles, ntiles * TH_NB,
+                            c->d_present, c->npresent, 0};
+        me.insert(me.end(), align(c->ne) / BATCH_ALIGN, k);
+        mn.insert(mn.end(), align(c->nn) / BATCH_ALIGN, k);
+        mb.insert(mb.end(), align(ntiles * TH_NB) / BATCH_ALIGN, k);
+        mt.insert(mt.end(), ntiles, k);
+        pe += align(c->ne);
+        pn += align(c->nn);
         pt += ntiles;
+        pb += align(ntiles * TH_NB);
         b->m.push_back(c);
     }
-    tab[n] = BatchEntry{nullptr, nullptr, GfDevArrays{}, pe, pn, pt, pt * TH_NB, nullptr, 0, 0};
+    tab[n] = BatchEntry{nullptr, nullptr, GfDevArrays{}, pe, pn, pt, pb, 0, 0, 0, 0, nullptr, 0, 0};
     b->nm = n;
-    b->tot_e = pe; b->tot_n = pn; b->tot_t = pt; b->tot_b = pt * TH_NB;
+    b->tot_e = pe; b->tot_n = pn; b->tot_t = pt; b->tot_b = pb;
+    const size_t nmap = me.size() + mn.size() + mt.size() + mb.size();
+    if (hipMalloc((void **)&b->dmap, sizeof(int) * nmap) != hipSuccess) return fail(GF_ERR_HIP, "allocation");
+    {
+        std::vector<int> all;
+        all.reserve(nmap);
+        for (auto *v : {&me, &mn, &mt, &mb}) all.insert(all.end(), v->begin(), v->end());
+        if (hipMemcpy(b->dmap, all.data(), sizeof(int) * nmap, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(GF_ERR_HIP, "copy");
+        const int *d = b->dmap;
+        b->map.chunk[BK_ELEM] = d;
+        b->map.chunk[BK_NODE] = d + me.size();
+        b->map.chunk[BK_TILE] = d + me.size() + mn.size();
+        b->map.chunk[BK_BORD] = d + me.size() + mn.size() + mt.size();
+        b->map.nm = n;
+    }
     b->precision = c0->precision;
     b->shearh = c0->p.ishearh == 1;
     b->no_mech = c0->p.itherm == 2;
@@ -1341,14 +1367,15 @@ static int enq_batch_flac(gf_batch *b)
 {
     typedef typename Prec<M>::SV SV;
     const int nm = b->nm;
+    const BatchMap map = b->map;
     const unsigned gm = bgrid(nm, 64);
-    BLAUNCH(kb_clock, gm, 64, b->dtab, nm, (int)CK_THERM_BEGIN);
+    BLAUNCH(kb_clock, gm, 64, b->dtab, map, (int)CK_THERM_BEGIN);
     for (int pass = 0; pass < TH_MAXPASS; pass++) {
         const long g = pass == 0 ? b->tot_t : std::min<long>(b->tot_t, 512);
-        BLAUNCH(kb_therm_tile, (unsigned)g, TH_J * TH_I, b->dtab, nm, pass);
-        BLAUNCH(kb_therm_check, bgrid(b->tot_b, 256), 256, b->dtab, nm, pass);
+        BLAUNCH(kb_therm_tile, (unsigned)g, TH_J * TH_I, b->dtab, map, b->tot_t, pass);
+        BLAUNCH(kb_therm_check, bgrid(b->tot_b, 256), 256, b->dtab, map, pass);
     }
-    BLAUNCH(kb_therm_bc, bgrid(b->tot_n, 256), 256, b->dtab, nm);
+    BLAUNCH(kb_therm_bc, bgrid(b->tot_n, 256), 256, b->dtab, map);
     if (!b->no_mech) {
         const gf_params &p = b->p0;
         const int do_visc = (b->nloop % p.ifreq_visc) == 0;
@@ -1357,30 +1384,30 @@ static int enq_batch_flac(gf_batch *b)
             avgsr = 1;
         else if (b->nsrate == -1)
             avgsr = 2;
-        BLAUNCH(kb_clock, gm, 64, b->dtab, nm, (int)CK_DTAVG_ADD);
-        if (do_visc) BLAUNCH(kb_visc<SV>, bgrid(b->tot_e, 256), 256, b->dtab, nm);
-        BTIMED(0, BLAUNCH((kb_rheol<SH, M>), bgrid(b->tot_e, b->bs[0]), b->bs[0], b->dtab, nm, avgsr));
+        BLAUNCH(kb_clock, gm, 64, b->dtab, map, (int)CK_DTAVG_ADD);
+        if (do_visc) BLAUNCH(kb_visc<SV>, bgrid(b->tot_e, 256), 256, b->dtab, map);
+        BTIMED(0, BLAUNCH((kb_rheol<SH, M>), bgrid(b->tot_e, b->bs[0]), b->bs[0], b->dtab, map, avgsr));
         if (avgsr == 1) {
-            BLAUNCH(kb_clock, gm, 64, b->dtab, nm, (int)CK_DTAVG_RESET);
+            BLAUNCH(kb_clock, gm, 64, b->dtab, map, (int)CK_DTAVG_RESET);
             b->nsrate = 0;
         }
         b->nsrate = b->nsrate + 1;
-        BTIMED(1, BLAUNCH(kb_node<M>, bgrid(b->tot_n, b->bs[1]), b->bs[1], b->dtab, nm));
-        BLAUNCH(kb_move_cord<SV>, bgrid(b->tot_n, 256), 256, b->dtab, nm);
-        BTIMED(2, BLAUNCH(kb_move_elem<M>, bgrid(b->tot_e, b->bs[2]), b->bs[2], b->dtab, nm));
+        BTIMED(1, BLAUNCH(kb_node<M>, bgrid(b->tot_n, b->bs[1]), b->bs[1], b->dtab, map));
+        BLAUNCH(kb_move_cord<SV>, bgrid(b->tot_n, 256), 256, b->dtab, map);
+        BTIMED(2, BLAUNCH(kb_move_elem<M>, bgrid(b->tot_e, b->bs[2]), b->bs[2], b->dtab, map));
         if (b->nloop % p.ifreq_rmasses == 0) {
-            BLAUNCH(kb_dens<SV>, bgrid(b->tot_e, 256), 256, b->dtab, nm);
-            BLAUNCH(kb_rmass, bgrid(b->tot_n, 256), 256, b->dtab, nm);
+            BLAUNCH(kb_dens<SV>, bgrid(b->tot_e, 256), 256, b->dtab, map);
+            BLAUNCH(kb_rmass, bgrid(b->tot_n, 256), 256, b->dtab, map);
         }
         if (b->nloop % p.ifreq_imasses == 0) {
-            BLAUNCH(kb_clock, gm, 64, b->dtab, nm, (int)CK_DT_BEGIN);
-            BLAUNCH(kb_dt_reduce<SV>, (unsigned)(nm * BATCH_DT_BLOCKS), 256, b->dtab, nm);
-            BLAUNCH(kb_dt_scalars, gm, 64, b->dtab, nm);
-            BLAUNCH(kb_amass, bgrid(b->tot_n, 256), 256, b->dtab, nm);
+            BLAUNCH(kb_clock, gm, 64, b->dtab, map, (int)CK_DT_BEGIN);
+            BLAUNCH(kb_dt_reduce<SV>, (unsigned)(nm * BATCH_DT_BLOCKS), 256, b->dtab, map);
+            BLAUNCH(kb_dt_scalars, gm, 64, b->dtab, map);
+            BLAUNCH(kb_amass, bgrid(b->tot_n, 256), 256, b->dtab, map);
         }
     }
     b->nloop = b->nloop + 1;
-    BLAUNCH(kb_clock, gm, 64, b->dtab, nm, (int)CK_TIME_ADVANCE);
+    BLAUNCH(kb_clock, gm, 64, b->dtab, map, (int)CK_TIME_ADVANCE);
     return GF_OK;
 }
 
