@@ -70,14 +70,42 @@ __global__ void __launch_bounds__(1024) kb_therm_tile(const BatchEntry *__restri
     }
 }
 
+// the thermal pipeline over the tiles of all models (k_therm_pipe, one workgroup per
+// CU walking a list of tiles); a tile of a model that is already proven only passes
+// the prefetch on
+template <bool XIN, bool DIS>
+__global__ void __launch_bounds__(1024) kb_therm_pipe(const BatchEntry *__restrict__ tab, BatchMap map, long total,
+                                                      int pass)
+{
+    __shared__ ThermRaw raw;
+    __shared__ double sU[TH_NF * TH_NE];
+    __shared__ int sFlag[3];
+    const long q = (total + 7) / 8, x = blockIdx.x % 8, r = blockIdx.x / 8, RW = gridDim.x / 8;
+    const long first = x * q + r, end = min((x + 1) * q, total);
+    if (first >= end) return;
+    auto entry = [&](long t) -> const BatchEntry & { return tab[__builtin_amdgcn_readfirstlane(map.chunk[BK_TILE][t])]; };
+    {
+        const BatchEntry &b = entry(first);
+        therm_store_raw<XIN, DIS>(therm_prefetch<XIN, DIS>(b.P, b.ck, b.A, pass, (int)(first - b.pt)), raw);
+    }
+    for (long t = first; t < end; t += RW) {
+        lds_barrier();  // this tile's inputs are in; the previous tile is done with sU
+        const BatchEntry &b = entry(t);
+        const long tn = t + RW;
+        const BatchEntry &bn = entry(tn < end ? tn : t);
+        therm_tile_pipe<XIN, DIS>(b.P, b.ck, b.A, pass, (int)(t - b.pt), bn.P, bn.ck, bn.A,
+                                  tn < end ? (int)(tn - bn.pt) : -1, raw, sU, sFlag);
+    }
+}
+
 __global__ void __launch_bounds__(256) kb_therm_check(const BatchEntry *__restrict__ tab, BatchMap map, int pass)
 {
     batch_apply<BK_BORD>(tab, map, [&](const BatchEntry &b, long t) { therm_check_body(b.P, b.ck, b.A, pass, t); });
 }
 
-__global__ void __launch_bounds__(256) kb_therm_bc(const BatchEntry *__restrict__ tab, BatchMap map)
+__global__ void __launch_bounds__(256) kb_therm_bc(const BatchEntry *__restrict__ tab, BatchMap map, int passes)
 {
-    batch_apply<BK_NODE>(tab, map, [&](const BatchEntry &b, long t) { therm_bc_body(b.P, b.ck, b.A, t); });
+    batch_apply<BK_NODE>(tab, map, [&](const BatchEntry &b, long t) { therm_bc_body(b.P, b.ck, b.A, t, passes); });
 }
 
 template <typename SV>

@@ -537,13 +537,26 @@ struct ThermRaw {
     double a0[TR_NEP], a1[TR_NEP], src[TR_NEP];   // areas of triangles A and B, heat source
     double sh[TR_NEP];                            // shear heating (when it heats)
     int iph[TR_NEP];                              // phase
+    double ph[3][GF_MAXPH];                       // den, cp, diff per phase of the tile's model
+    double dt;                                    // the model's dt
+    int active, pad;                              // 0: the model is already proven (passes > 0)
 };
 // one thread's share of a tile's inputs: node slots q and q + 1024, element slots
-// e and e + 1024 (0 outside the mesh)
+// e and e + 1024 (0 outside the mesh); thread q < 3 GF_MAXPH: phase table entry q;
+// thread 0: dt and the activity flag
 struct ThermPf {
     double t[2], cx[2], cy[2], x[2], a0[2], a1[2], src[2], sh[2];
     int iph[2];
+    double phv, dtv;
+    int act;
 };
+// a load through a global-memory pointer (never the flat path, whose counter the LDS
+// barriers would also wait for)
+template <typename T>
+GF_DEV T gld(const T *p, long i)
+{
+    return ((const __attribute__((address_space(1))) T *)p)[i];
+}
 
 // workgroup barrier after this wave's LDS operations (outstanding global loads and
 // stores, the prefetch among them, are not waited for)
@@ -558,40 +571,59 @@ GF_DEV void therm_tile_origin(const GfDevParams *P, int tile, int &j0, int &i0)
 
 // node q = (q % 66, q / 66) - 1 of the footprint, element e = (e % 65, e / 65) - 1 of
 // the table
-GF_DEV ThermPf therm_prefetch(const GfDevParams *__restrict__ P, const GfDevArrays &A, int pass, int tile,
-                              bool with_dissip)
+// XIN: the starting iterate is loaded (passes > 0); DIS: the shear heating is loaded
+template <bool XIN, bool DIS>
+GF_DEV ThermPf therm_prefetch(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, int pass,
+                              int tile)
 {
     int j0, i0;
     therm_tile_origin(P, tile, j0, i0);
-    const double *__restrict__ told_g = pass == 0 ? A.temp : A.temp0;
-    const double *__restrict__ xin = (pass % 2 == 1) ? A.xa : A.xb;
+    const long nz = P->nz, nx = P->nx, nn = nz * nx, ne = (nz - 1) * (nx - 1);
+    const double *told_g = pass == 0 ? A.temp : A.temp0;
+    const double *xin = (pass % 2 == 1) ? A.xa : A.xb;
     ThermPf f;
+    // the model's scalars first, used at once: waiting for them later would also wait
+    // for the tile loads below
+    f.phv = 0.0;
+    if (threadIdx.x < 3 * GF_MAXPH) {
+        const GfPhase &ph = P->ph[threadIdx.x % GF_MAXPH];
+        const int w = threadIdx.x / GF_MAXPH;
+        f.phv = w == 0 ? ph.den : w == 1 ? ph.cp : ph.diff;
+    }
+    f.dtv = 0.0;
+    f.act = 1;
+    if (threadIdx.x == 0) {
+        f.dtv = ck->dt;
+        f.act = (pass > 0 && ck->therm_pass_changed[pass - 1] == 0u) ? 0 : 1;
+    }
+    asm volatile("" ::"v"(f.phv), "v"(f.dtv), "v"(f.act));
 #pragma unroll
     for (int k = 0; k < 2; k++) {
         const int q = threadIdx.x + k * TH_J * TH_I;
         const int jj = j0 + q % TH_LJ - 1, ii = i0 + q / TH_LJ - 1;
         f.t[k] = f.cx[k] = f.cy[k] = f.x[k] = 0.0;
-        if (q < TH_LJ * TH_LI && jj >= 1 && jj <= NZ && ii >= 1 && ii <= NX) {
-            const long ix = IN(jj, ii);
-            f.t[k] = told_g[ix];
-            f.cx[k] = A.cord[ix];
-            f.cy[k] = A.cord[ix + NN];
-            if (pass > 0) f.x[k] = xin[ix];
+        if (q < TH_LJ * TH_LI && jj >= 1 && jj <= nz && ii >= 1 && ii <= nx) {
+            const long ix = (jj - 1) + nz * (ii - 1);
+            f.t[k] = gld(told_g, ix);
+            f.cx[k] = gld(A.cord, ix);
+            f.cy[k] = gld(A.cord, ix + nn);
+            if (XIN) f.x[k] = gld(xin, ix);
         }
         const int je = j0 + q % TH_EJ - 1, ie = i0 + q / TH_EJ - 1;
         f.a0[k] = f.a1[k] = f.src[k] = f.sh[k] = 0.0;
         f.iph[k] = 1;
-        if (q < TH_NE && je >= 1 && je <= NZ - 1 && ie >= 1 && ie <= NX - 1) {
-            const long ix = IE(je, ie);
-            f.a0[k] = A.area[ix];
-            f.a1[k] = A.area[ix + NE];
-            f.src[k] = A.source[ix];
-            f.iph[k] = A.iphase[ix];
-            if (with_dissip) f.sh[k] = A.shrheat[ix];
+        if (q < TH_NE && je >= 1 && je <= nz - 1 && ie >= 1 && ie <= nx - 1) {
+            const long ix = (je - 1) + (nz - 1) * (ie - 1);
+            f.a0[k] = gld(A.area, ix);
+            f.a1[k] = gld(A.area, ix + ne);
+            f.src[k] = gld(A.source, ix);
+            f.iph[k] = gld(A.iphase, ix);
+            if (DIS) f.sh[k] = gld(A.shrheat, ix);
         }
     }
     return f;
 }
+template <bool XIN, bool DIS>
 GF_DEV void therm_store_raw(const ThermPf &f, ThermRaw &R)
 {
 #pragma unroll
@@ -601,37 +633,57 @@ GF_DEV void therm_store_raw(const ThermPf &f, ThermRaw &R)
             R.t[q] = f.t[k];
             R.cx[q] = f.cx[k];
             R.cy[q] = f.cy[k];
-            R.x[q] = f.x[k];
+            if (XIN) R.x[q] = f.x[k];
         }
         if (q < TH_NE) {
             R.a0[q] = f.a0[k];
             R.a1[q] = f.a1[k];
             R.src[q] = f.src[k];
-            R.sh[q] = f.sh[k];
+            R.sh[q] = DIS ? f.sh[k] : 0.0;
             R.iph[q] = f.iph[k];
         }
     }
+    if (threadIdx.x < 3 * GF_MAXPH) R.ph[threadIdx.x / GF_MAXPH][threadIdx.x % GF_MAXPH] = f.phv;
+    if (threadIdx.x == 0) {
+        R.dt = f.dtv;
+        R.active = f.act;
+    }
 }
 
-// One tile from its inputs in R (the arithmetic of therm_tile_one, bit for bit);
-// loads tile `next` (if >= 0) during the sweeps and leaves it in R.
-// sU: element table by rounds, then the two iterates.
+// One tile (P, ck, A, tile) from its inputs in R (the arithmetic of therm_tile_one,
+// bit for bit); loads tile `next` of (Pn, ckn, An) (if next >= 0) during the sweeps
+// and leaves it in R.  sU: element table by rounds, then the two iterates.  Nothing
+// after the prefetch is issued reads global memory (a wait for it would also wait
+// for the prefetch): the mesh size is held in registers, the phase table, dt and the
+// activity flag came with the tile's inputs.
+template <bool XIN, bool DIS>
 GF_DEV void therm_tile_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, int pass,
-                            int tile, int next, ThermRaw &R, double *sU, const double (*sPh)[GF_MAXPH], int *sFlag,
-                            bool with_dissip, double dt)
+                            int tile, const GfDevParams *__restrict__ Pn, GfDevClock *ckn, const GfDevArrays &An,
+                            int next, ThermRaw &R, double *sU, int *sFlag)
 {
     double *__restrict__ xout = (pass % 2 == 0) ? A.xa : A.xb;
     double *__restrict__ bord = A.tbord;
     constexpr int NL = TH_LI * TH_LJ, NT = TH_J * TH_I;
     double(*sX)[NL] = reinterpret_cast<double(*)[NL]>(sU);
+    const int nz = P->nz, nx = P->nx;
     int j0, i0;
-    therm_tile_origin(P, tile, j0, i0);
+    {
+        const int ntj = (nz + TH_OJ - 1) / TH_OJ;
+        j0 = (tile % ntj) * TH_OJ + 1 - TH_HJ;
+        i0 = (tile / ntj) * TH_OI + 1 - TH_HI;
+    }
+    if (!R.active) {  // a proven model (passes > 0): nothing to do but the prefetch
+        lds_barrier();
+        if (next >= 0) therm_store_raw<XIN, DIS>(therm_prefetch<XIN, DIS>(Pn, ckn, An, pass, next), R);
+        return;
+    }
+    const double(*sPh)[GF_MAXPH] = R.ph;
     const int tj = threadIdx.x % TH_J, ti = threadIdx.x / TH_J;
 #define LIDX(lj, li) (((li) + 1) * TH_LJ + ((lj) + 1))
     const int e0 = threadIdx.x, e1 = threadIdx.x + NT;
     auto elem_ok = [&](int e) {
         const int je = j0 + e % TH_EJ - 1, ie = i0 + e / TH_EJ - 1;
-        return e < TH_NE && je >= 1 && je <= NZ - 1 && ie >= 1 && ie <= NX - 1;
+        return e < TH_NE && je >= 1 && je <= nz - 1 && ie >= 1 && ie <= nx - 1;
     };
     const bool ein0 = elem_ok(e0), ein1 = elem_ok(e1);
     // round 0: ra13, P = dummye*ra13, Q = 0.5*diff*inv_area of triangles A and B;
@@ -672,15 +724,16 @@ GF_DEV void therm_tile_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, c
         sU[5 * TH_NE + e] = KAB(d, 3, 3);
     };
     const int j = j0 + tj, i = i0 + ti;
-    const bool valid = j >= 1 && j <= NZ && i >= 1 && i <= NX;
+    const bool valid = j >= 1 && j <= nz && i >= 1 && i <= nx;
+    const long oix = (long)(j - 1) + (long)nz * (i - 1);  // this node's global index
     const bool is_out = valid && tj >= TH_HJ && tj < TH_J - TH_HJ && ti >= TH_HI;
     ThermCoef c;
     c.mask = 0;
     if (valid) {
         if (j != 1 && i != 1) c.mask |= 1;
-        if (j != 1 && i != NX) c.mask |= 2 | 4;
-        if (j != NZ && i != 1) c.mask |= 8 | 16;
-        if (j != NZ && i != NX) c.mask |= 32;
+        if (j != 1 && i != nx) c.mask |= 2 | 4;
+        if (j != nz && i != 1) c.mask |= 8 | 16;
+        if (j != nz && i != nx) c.mask |= 32;
     }
     c.area_n = 0.0;
     // the element of each of the node's six triangles (t1 B, t2 A, t3 B, t4 A, t5 B, t6 A):
@@ -729,19 +782,21 @@ GF_DEV void therm_tile_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, c
     c.told = valid ? R.t[nb] : 0.0;
     // starting iterate: the old temperature in pass 0, the previous pass's result later
     {
-        const double v0 = pass > 0 ? R.x[e0] : R.t[e0];
+        const double v0 = XIN ? R.x[e0] : R.t[e0];
         sX[0][e0] = v0;
         sX[1][e0] = v0;
         if (e1 < NL) {
-            const double v1 = pass > 0 ? R.x[e1] : R.t[e1];
+            const double v1 = XIN ? R.x[e1] : R.t[e1];
             sX[0][e1] = v1;
             sX[1][e1] = v1;
         }
         if (threadIdx.x < 3) sFlag[threadIdx.x] = 0;
     }
     lds_barrier();
+    const double dt = R.dt;
+    asm volatile("" ::"v"(dt));
     ThermPf pf;
-    if (next >= 0) pf = therm_prefetch(P, A, pass, next, with_dissip);  // R is free now
+    if (next >= 0) pf = therm_prefetch<XIN, DIS>(Pn, ckn, An, pass, next);  // R is free now
 
     const int me = nb, up = nb - 1, lf = nb - TH_LJ, dl = nb - TH_LJ + 1;
     int cur = 0, nsweep = 0;
@@ -771,10 +826,10 @@ GF_DEV void therm_tile_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, c
     }
     // the next tile's inputs go to R before this tile's outputs are stored, so that
     // waiting for the loads does not wait for the stores
-    if (next >= 0) therm_store_raw(pf, R);
+    if (next >= 0) therm_store_raw<XIN, DIS>(pf, R);
     if (is_out) {
-        xout[IN(j, i)] = sX[cur][me];
-        if (pass == 0) A.temp0[IN(j, i)] = c.told;
+        xout[oix] = sX[cur][me];
+        if (pass == 0) A.temp0[oix] = c.told;
     }
     if (threadIdx.x < TH_NB) {
         int lj, li;
@@ -788,29 +843,22 @@ GF_DEV void therm_tile_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, c
 // Workgroup w (of a grid that is a multiple of 8, one per CU) runs on XCD w % 8, which
 // owns the contiguous tile range [x q, (x+1) q); its r-th workgroup there takes tiles
 // x q + r, + R, + 2R, ... with R = grid / 8.
+template <bool XIN, bool DIS>
 __global__ void __launch_bounds__(1024) k_therm_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
                                                      int pass)
 {
     if (pass > 0 && ck->therm_pass_changed[pass - 1] == 0u) return;  // already proven
     __shared__ ThermRaw raw;
     __shared__ double sU[TH_NF * TH_NE];
-    __shared__ double sPh[3][GF_MAXPH];
     __shared__ int sFlag[3];
-    for (int q = threadIdx.x; q < GF_MAXPH; q += blockDim.x) {
-        sPh[0][q] = P->ph[q].den;
-        sPh[1][q] = P->ph[q].cp;
-        sPh[2][q] = P->ph[q].diff;
-    }
-    const double dt = ck->dt;
-    const bool with_dissip = P->ishearh == 1 && P->itherm != 2;
     const int ntiles = therm_ntj(P) * therm_nti(P);
     const int q = (ntiles + 7) / 8, x = blockIdx.x % 8, r = blockIdx.x / 8, RW = gridDim.x / 8;
     const int first = x * q + r, end = min((x + 1) * q, ntiles);
     if (first >= end) return;
-    therm_store_raw(therm_prefetch(P, A, pass, first, with_dissip), raw);
+    therm_store_raw<XIN, DIS>(therm_prefetch<XIN, DIS>(P, ck, A, pass, first), raw);
     for (int t = first; t < end; t += RW) {
         lds_barrier();  // this tile's inputs are in; the previous tile is done with sU
-        therm_tile_pipe(P, ck, A, pass, t, t + RW < end ? t + RW : -1, raw, sU, sPh, sFlag, with_dissip, dt);
+        therm_tile_pipe<XIN, DIS>(P, ck, A, pass, t, P, ck, A, t + RW < end ? t + RW : -1, raw, sU, sFlag);
     }
 }
 
@@ -866,16 +914,21 @@ __global__ void __launch_bounds__(256) k_therm_check(const GfDevParams *__restri
 }
 
 // Boundary conditions src/fl_therm.f90:575-598 applied to the proven iterate.
-GF_DEV void therm_bc_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long t)
+// passes: TH_MAXPASS when the tiles' result is proven to be the serial one, 1 when
+// the single tile pass is taken as it is (FP32-arithmetic mode)
+GF_DEV void therm_bc_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long t,
+                          int passes)
 {
     const double *__restrict__ xa = A.xa, *__restrict__ xb = A.xb;
     // pass p writes xa when p is even, xb when odd; the first pass whose border check
     // found no difference holds the fixed point
     int proven = -1;
-    for (int k = 0; k < TH_MAXPASS; k++)
+    for (int k = 0; k < passes; k++)
         if (ck->therm_pass_changed[k] == 0u) { proven = k; break; }
+    if (passes == 1) proven = 0;
     const double *x = (proven < 0 ? (TH_MAXPASS - 1) : proven) % 2 == 0 ? xa : xb;
-    if (t == 0 && proven < 0) atomicAdd(&ck->therm_unconverged, 1ull);
+    if (t == 0 && (proven < 0 || (passes == 1 && ck->therm_pass_changed[0] != 0u)))
+        atomicAdd(&ck->therm_unconverged, 1ull);
     if (t == 0 && proven > 0) atomicAdd(&ck->therm_extra_passes, (unsigned long long)proven);
     if (t >= NN) return;
     const int j = (int)(t % NZ) + 1, i = (int)(t / NZ) + 1;
@@ -902,9 +955,10 @@ GF_DEV void therm_bc_body(const GfDevParams *__restrict__ P, GfDevClock *ck, con
     A.temp[t] = v;
 }
 
-__global__ void __launch_bounds__(256) k_therm_bc(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+__global__ void __launch_bounds__(256) k_therm_bc(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
+                                                  int passes)
 {
-    therm_bc_body(P, ck, A, xcd_lin());
+    therm_bc_body(P, ck, A, xcd_lin(), passes);
 }
 
 // ------------------------------------------------------------------ material properties

@@ -52,6 +52,9 @@ struct gf_ctx {
     bool rheol_quad = false;  // four lanes per element (GF_RHEOL=quad) instead of one
     bool rot_pending = false; // the last element move's stress/strain rotation is not applied yet
     unsigned therm_grid = 8;  // workgroups of the pipelined thermal kernel: the CUs, a multiple of 8
+    // TH_MAXPASS: the thermal result is proven to be the reference's serial one (fallback
+    // passes when needed); 1: FP32-arithmetic mode takes the single tile pass as it is
+    int therm_passes = TH_MAXPASS;
     // per-kernel event timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -184,6 +187,7 @@ extern "C" int gf_create(const gf_params *p, int32_t precision, gf_ctx **out)
     ctx = new gf_ctx();
     ctx->p = *p;
     ctx->precision = precision;
+    ctx->therm_passes = precision == GF_PREC_MIXED ? 1 : TH_MAXPASS;
     ctx->ne = (long)(p->nz - 1) * (p->nx - 1);
     ctx->nn = (long)p->nz * p->nx;
     ctx->hp.nactive = 0;
@@ -642,7 +646,8 @@ static int enq_therm(gf_ctx *ctx)
     int rc;
     if ((rc = enq_clock(ctx, CK_THERM_BEGIN))) return rc;
     const int ntiles = ((ctx->p.nz + TH_OJ - 1) / TH_OJ) * ((ctx->p.nx + TH_OI - 1) / TH_OI);
-    for (int pass = 0; pass < TH_MAXPASS; pass++) {
+    const int npass = ctx->therm_passes;
+    for (int pass = 0; pass < npass; pass++) {
         const dim3 grid(ctx->therm_grid);
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (ctx->timing) {
@@ -650,15 +655,18 @@ static int enq_therm(gf_ctx *ctx)
             e1 = next_event(ctx);
             if (e0) (void)hipEventRecord(e0, ctx->stream);
         }
-        hipLaunchKernelGGL(k_therm_pipe, grid, dim3(TH_J * TH_I), 0, ctx->stream, ctx->dp, ctx->dck, ctx->a, pass);
+        const bool dis = ctx->p.ishearh == 1 && ctx->p.itherm != 2;
+        auto kern = pass == 0 ? (dis ? k_therm_pipe<false, true> : k_therm_pipe<false, false>)
+                              : (dis ? k_therm_pipe<true, true> : k_therm_pipe<true, false>);
+        hipLaunchKernelGGL(kern, grid, dim3(TH_J * TH_I), 0, ctx->stream, ctx->dp, ctx->dck, ctx->a, pass);
         if ((rc = launch_check(ctx, "k_therm_pipe"))) return rc;
         if (ctx->timing && e0 && e1) {
             (void)hipEventRecord(e1, ctx->stream);
             ctx->pending.push_back({pass == 0 ? "k_therm_pipe" : "k_therm_pipe_extra", {e0, e1}});
         }
-        LAUNCH(k_therm_check, (long)ntiles * TH_NB, ctx->dp, ctx->dck, ctx->a, pass);
+        if (npass > 1) LAUNCH(k_therm_check, (long)ntiles * TH_NB, ctx->dp, ctx->dck, ctx->a, pass);
     }
-    LAUNCH(k_therm_bc, ctx->nn, ctx->dp, ctx->dck, ctx->a);
+    LAUNCH(k_therm_bc, ctx->nn, ctx->dp, ctx->dck, ctx->a, npass);
     return GF_OK;
 }
 
@@ -1218,6 +1226,7 @@ struct gf_batch {
     gf_params p0{};  // frequencies and switches shared by all models
     BatchEntry *dtab = nullptr;
     int *dmap = nullptr;  // chunk -> model tables (BatchMap)
+    unsigned therm_grid = 8;  // workgroups of the thermal pipeline (the CUs)
     BatchMap map{};
     int nm = 0;
     long tot_e = 0, tot_n = 0, tot_t = 0, tot_b = 0;
@@ -1308,6 +1317,7 @@ extern "C" int gf_batch_create(gf_ctx *const *models, int32_t n, gf_batch **out)
     tab[n] = BatchEntry{nullptr, nullptr, GfDevArrays{}, pe, pn, pt, pb, 0, 0, 0, 0, nullptr, 0, 0};
     b->nm = n;
     b->tot_e = pe; b->tot_n = pn; b->tot_t = pt; b->tot_b = pb;
+    b->therm_grid = c0->therm_grid;
     const size_t nmap = me.size() + mn.size() + mt.size() + mb.size();
     if (hipMalloc((void **)&b->dmap, sizeof(int) * nmap) != hipSuccess) return fail(GF_ERR_HIP, "allocation");
     {
@@ -1370,12 +1380,19 @@ static int enq_batch_flac(gf_batch *b)
     const BatchMap map = b->map;
     const unsigned gm = bgrid(nm, 64);
     BLAUNCH(kb_clock, gm, 64, b->dtab, map, (int)CK_THERM_BEGIN);
-    for (int pass = 0; pass < TH_MAXPASS; pass++) {
-        const long g = pass == 0 ? b->tot_t : std::min<long>(b->tot_t, 512);
-        BLAUNCH(kb_therm_tile, (unsigned)g, TH_J * TH_I, b->dtab, map, b->tot_t, pass);
-        BLAUNCH(kb_therm_check, bgrid(b->tot_b, 256), 256, b->dtab, map, pass);
+    const bool dis = b->p0.ishearh == 1 && b->p0.itherm != 2;
+    const int npass = b->m[0]->therm_passes;
+    for (int pass = 0; pass < npass; pass++) {
+        // pass 0 through the pipeline; a fallback pass, which only some models need,
+        // with one workgroup per tile that returns at once for a proven model
+        if (pass == 0)
+            BLAUNCH((dis ? kb_therm_pipe<false, true> : kb_therm_pipe<false, false>), b->therm_grid, TH_J * TH_I,
+                    b->dtab, map, b->tot_t, pass);
+        else
+            BLAUNCH(kb_therm_tile, (unsigned)std::min<long>(b->tot_t, 512), TH_J * TH_I, b->dtab, map, b->tot_t, pass);
+        if (npass > 1) BLAUNCH(kb_therm_check, bgrid(b->tot_b, 256), 256, b->dtab, map, pass);
     }
-    BLAUNCH(kb_therm_bc, bgrid(b->tot_n, 256), 256, b->dtab, map);
+    BLAUNCH(kb_therm_bc, bgrid(b->tot_n, 256), 256, b->dtab, map, npass);
     if (!b->no_mech) {
         const gf_params &p = b->p0;
         const int do_visc = (b->nloop % p.ifreq_visc) == 0;
