@@ -88,3 +88,58 @@ def test_batch_mixed_precision_equals_single_runs():
     for a, b in zip(batch, single):
         diff = [f for f in state.State.FIELDS if not np.array_equal(getattr(a, f), getattr(b, f))]
         assert not diff, diff
+
+
+@pytest.mark.gpu
+def test_thermal_fallback_passes_bitwise_equal_to_oracle():
+    """Cells 1.7 times taller than wide (200x40 elements over the 300x100 km box)
+    couple neighbouring columns strongly, so that the thermal tiles' halos do not
+    settle every output node and fallback passes run (the pipelined tile kernel from
+    the assembled iterate).  The run stays bitwise equal to the correctly rounded
+    oracle started from the same state."""
+    import os
+    from tests import oracle_lib as ol
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    q = sweep.variant(inp.read_inp(os.path.join(root, BASE)), 200, 40)
+    n = 30
+    _say("case: FP64 200x40 variant, %d steps, device vs oracle" % n)
+    gp, st, m = sweep.make_model(q, abi.PREC_FP64)
+    m.download()
+    ref = st.copy()
+    m.therm_stats()
+    m.step(n)
+    m.download()
+    unconv, extra, mism, mulp = m.therm_stats()
+    m.close()
+    assert ol.step(gp, ref, n, cr=True) == 0
+    diff = [f for f in state.State.FIELDS if not np.array_equal(getattr(st, f), getattr(ref, f))]
+    _say("  thermal: unproven %d, extra passes %d, border mismatches %d (max %d ulp); differing fields %s"
+         % (unconv, extra, mism, mulp, diff))
+    assert extra > 0, "the case no longer exercises the fallback passes"
+    assert unconv == 0
+    assert not diff
+
+
+@pytest.mark.gpu
+def test_fp32_arithmetic_single_thermal_pass():
+    """In FP32-arithmetic mode the thermal step takes the single tile pass without the
+    fixed-point proof.  On the 200x40 variant, whose proof needs fallback passes in
+    FP64, the temperature stays within 1e-9 relative L2 of the FP64 run after 30
+    steps."""
+    import os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    q = sweep.variant(inp.read_inp(os.path.join(root, BASE)), 200, 40)
+    n = 30
+    _say("case: 200x40 variant, %d steps, FP32 arithmetic vs FP64 temperature" % n)
+    out = {}
+    for prec in (abi.PREC_FP64, abi.PREC_MIXED):
+        gp, st, m = sweep.make_model(q, prec)
+        m.therm_stats()
+        m.step(n)
+        m.download()
+        out[prec] = (st, m.therm_stats())
+        m.close()
+    err = state.rel_l2(out[abi.PREC_MIXED][0].temp, out[abi.PREC_FP64][0].temp)
+    _say("  temp rel-L2 %.2e; FP32-mode thermal stats %s" % (err, out[abi.PREC_MIXED][1]))
+    assert out[abi.PREC_MIXED][1][0] == 0 and out[abi.PREC_MIXED][1][1] == 0
+    assert err <= 1e-9
