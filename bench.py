@@ -129,6 +129,14 @@ def mixed_run(gp, host_state, fp64_state, args, E, precision, learned=None):
     el = time.perf_counter() - t0
     m.download()
     sur = m.surrogate_stats() if learned is not None else None
+    # time of the viscosity update per call (every ifreq_visc steps), exact or learned
+    m.set_timing(True)
+    m.step(10)
+    vnames = ["k_visc_mlp<float>", "k_visc_list<float>"] if learned is not None \
+        else ["k_visc<float>"]
+    vt = {k: m.kernel_time(k) for k in vnames}
+    calls = max(1, vt[vnames[-1]][1])
+    m.set_timing(False)
     m.close()
     err = {}
     for f in ("vel", "stress0", "temp", "cord", "aps"):
@@ -136,12 +144,16 @@ def mixed_run(gp, host_state, fp64_state, args, E, precision, learned=None):
         err[f] = {"rel_l2": float("%.3e" % state.rel_l2(a, b)), "max_abs": float("%.3e" % np.max(np.abs(a - b)))}
     dtype = "f32 arithmetic, f32 planes" if precision == abi.PREC_MIXED else "f64 arithmetic, f32 planes"
     out = {"value": E * args.steps / el, "ms_per_step": el / args.steps * 1e3, "dtype": dtype,
-           "steps_compared": args.warmup + args.steps, "error_vs_fp64": err}
+           "steps_compared": args.warmup + args.steps, "error_vs_fp64": err,
+           "viscosity_update_ms_per_call": dict({k: round(v[0] / calls, 4) for k, v in vt.items()},
+                                                 total=round(sum(v[0] for v in vt.values()) / calls, 4))}
     if learned is not None:
-        tot = max(1, sur["evals"] + sur["fallbacks"])
+        tot = max(1, sur["evals"] + sur["fallbacks"] + sur["audits"])
         out["surrogate"] = dict(learned[2], fallback_rate=round(sur["fallbacks"] / tot, 4),
                                 audits=sur["audits"], max_audit_log10_err=round(sur["max_audit_err"], 5),
-                                worst_element=sur["worst_element"], switched_off=sur["off"])
+                                worst_element=sur["worst_element"], switched_off=sur["off"],
+                                failed_audits=sur["failed_audits"], cells_taken_off=sur["cells_off"],
+                                tolerance_log10=0.5)
     return out
 
 

@@ -45,25 +45,42 @@ struct MlpCfg {
 };
 
 struct MlpStats {
-    unsigned long long evals, fallbacks, audits;
-    unsigned int audit_maxerr_bits;  // float bits of the largest audited |log10 error|
-    int off;                         // set when an audit exceeded tol
-    unsigned long long worst;        // (error bits << 32) | element of the largest audited error
+    int off;                    // set when an audit exceeded MLP_OFF_FACTOR * tol
+    unsigned int bad[MLP_OCC];  // bit c of word r: an audit in cell (r, c) exceeded tol
 };
+// counts of one workgroup of the persistent viscosity grid, which alone updates them
+// (the host sums them: no contended global atomics)
+struct MlpWgStats {
+    unsigned long long evals, fallbacks, audits, failed;  // network values, fallbacks, audits, failed audits
+    unsigned long long worst;        // (error bits << 32) | element of the largest audited error
+    unsigned int audit_maxerr_bits;  // float bits of the largest audited |log10 error|
+    unsigned int pad;
+};
+// An audit above tol takes its (1/T, log10 strain rate) cell off the surrogate; one
+// above this multiple of tol takes the whole surrogate off.
+#define MLP_OFF_FACTOR 4.0f
 
 // the out-of-distribution gate: inside the box of every feature and in an occupied
 // cell of the (1/T, log10 strain rate) plane
-GF_DEV bool mlp_in_distribution(const MlpCfg *cfg, const float x[MLP_NIN])
+// (1/T, log10 strain rate) cell of a feature vector inside the training box
+GF_DEV void mlp_cell(const MlpCfg *cfg, const float x[MLP_NIN], int &r, int &c)
+{
+    r = min(MLP_OCC - 1, max(0, (int)((x[0] - cfg->xlo[0]) / (cfg->xhi[0] - cfg->xlo[0]) * MLP_OCC)));
+    c = min(MLP_OCC - 1, max(0, (int)((x[1] - cfg->xlo[1]) / (cfg->xhi[1] - cfg->xlo[1]) * MLP_OCC)));
+}
+GF_DEV bool mlp_in_distribution(const MlpCfg *cfg, const MlpStats *st, const float x[MLP_NIN])
 {
     for (int f = 0; f < MLP_NIN; f++)
         if (!(x[f] >= cfg->xlo[f] && x[f] <= cfg->xhi[f])) return false;
-    const int r = min(MLP_OCC - 1, (int)((x[0] - cfg->xlo[0]) / (cfg->xhi[0] - cfg->xlo[0]) * MLP_OCC));
-    const int c = min(MLP_OCC - 1, (int)((x[1] - cfg->xlo[1]) / (cfg->xhi[1] - cfg->xlo[1]) * MLP_OCC));
-    return (cfg->occ[r] >> c) & 1u;
+    int r, c;
+    mlp_cell(cfg, x, r, c);
+    return ((cfg->occ[r] & ~st->bad[r]) >> c) & 1u;
 }
 
 GF_DEV float mlp_sigmoid(float z) { return 1.0f / (1.0f + __expf(-z)); }
 GF_DEV float mlp_silu(float z) { return z * mlp_sigmoid(z); }
+// inference: the hardware reciprocal (1 ulp) in place of the rounded division
+GF_DEV float mlp_silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 GF_DEV float mlp_dsilu(float z)
 {
     const float s = mlp_sigmoid(z);
@@ -76,31 +93,37 @@ GF_DEV unsigned short f2bf(float f)  // round to nearest, ties to even
     return (unsigned short)(u >> 16);
 }
 GF_DEV float bf2f(unsigned short b) { return __uint_as_float((unsigned int)b << 16); }
+GF_DEV unsigned short f2bf_hw(float f) { return __builtin_bit_cast(unsigned short, (__bf16)f); }  // same rounding
 
 // ------------------------------------------------------------------ features
 // The inputs Eff_visc uses (src/matprops.f90:136-215).  Returns false where the exact
 // formula must be used: weak zones, the 410/660 km branches, more than 4 phases.
-GF_DEV bool visc_features(const GfDevParams *P, const GfDevArrays &A, int j, int i, double s11, double s22,
-                          double s33, const double *prv, float x[MLP_NIN])
+GF_DEV bool visc_features_in(const GfDevParams *P, int j, int i, double s11, double s22, double s33,
+                             const double *prv, const ViscIn &v, float x[MLP_NIN])
 {
     const double adi_grad = 3.0e-1;
     for (int k = 0; k < P->inhom; k++)
         if (P->igeom[k] == 100 && i >= P->ix1[k] && i <= P->ix2[k] && j >= P->iy1[k] && j <= P->iy2[k])
             return false;
     if (P->nactive > 4) return false;
-    double zcord = 0.25 * (CORD(j, i, 2) + CORD(j + 1, i, 2) + CORD(j, i + 1, 2) + CORD(j + 1, i + 1, 2));
-    double tmpr0 = 0.25 * (TEMP(j, i) + TEMP(j + 1, i) + TEMP(j, i + 1) + TEMP(j + 1, i + 1));
-    double zsurf = 0.50 * (CORD(1, i, 2) + CORD(1, i + 1, 2));
+    double zcord = 0.25 * (v.y[0] + v.y[1] + v.y[2] + v.y[3]);
+    double tmpr0 = 0.25 * (v.t[0] + v.t[1] + v.t[2] + v.t[3]);
+    double zsurf = 0.50 * (v.ysurf[0] + v.ysurf[1]);
     double tmpr = (zsurf - zcord) * 1.0e-3 * adi_grad + tmpr0;
     double pres = -1 * (s11 + s22 + s33) / 3.0;
     if (zcord < 66.55 * (tmpr + 273.0) - 788494.0 || zcord < -101.58 * (tmpr + 273.0) - 216828.0) return false;
-    double srat = A.e2sr[IE(j, i)];
+    double srat = v.srat;
     if (srat == 0) srat = P->vbc_over_rxbo;
-    x[0] = (float)(1.0e3 / (tmpr + 273.0));  // the Arrhenius factor is linear in 1/T
-    x[1] = (float)log10(srat);
+    x[0] = 1.0e3f / (float)(tmpr + 273.0);  // the Arrhenius factor is linear in 1/T
+    x[1] = log10f((float)srat);
     x[2] = (float)(pres * 1.0e-8);
     for (int a = 0; a < 4; a++) x[3 + a] = a < P->nactive ? (float)prv[a] : 0.0f;
     return true;
+}
+GF_DEV bool visc_features(const GfDevParams *P, const GfDevArrays &A, int j, int i, double s11, double s22,
+                          double s33, const double *prv, float x[MLP_NIN])
+{
+    return visc_features_in(P, j, i, s11, s22, s33, prv, visc_inputs(P, A, j, i), x);
 }
 
 // one sample per element: features and the exact log10 viscosity (NaN features when
@@ -275,13 +298,16 @@ __global__ void __launch_bounds__(256) k_mlp_adam(float *__restrict__ w, float *
 }
 
 // ------------------------------------------------------------------ inference on the matrix cores
-typedef short mlp_s4 __attribute__((ext_vector_type(4)));
-typedef float mlp_f16 __attribute__((ext_vector_type(16)));
-
-// BF16 weights in LDS, row-major by output unit: W1 padded to 8 inputs
-struct MlpLds {
-    unsigned short w1[MLP_NH][8], w2[MLP_NH][MLP_NH];
+// Weights in LDS as BF16 (biases and W3 as FP32, W3 rounded to BF16).  Rows are padded
+// so that the 32 rows one MFMA operand load touches fall in distinct banks.
+#define MLP_W1R 12            // W1 row: 7 inputs, 0, 4 of padding
+#define MLP_W2R (MLP_NH + 4)  // W2 row: 64 inputs, 4 of padding
+struct __align__(16) MlpLds {  // 8-byte operand rows, 16-byte bias quads
+    unsigned short w1[MLP_NH][MLP_W1R], w2[MLP_NH][MLP_W2R];
     float b1[MLP_NH], b2[MLP_NH], w3[MLP_NH], b3;
+};
+struct __align__(8) mlp_row {  // one sample's 8 BF16 inputs, padded like W1
+    unsigned short v[MLP_W1R];
 };
 
 GF_DEV void mlp_load_weights(const float *__restrict__ w, MlpLds &L)
@@ -299,64 +325,96 @@ GF_DEV void mlp_load_weights(const float *__restrict__ w, MlpLds &L)
     if (threadIdx.x == 0) L.b3 = w[MLP_OB3];
 }
 
-GF_DEV mlp_s4 lds4(const unsigned short *p)  // four consecutive BF16 values
+typedef short mlp_s4 __attribute__((ext_vector_type(4)));
+typedef float mlp_f16 __attribute__((ext_vector_type(16)));
+GF_DEV mlp_s4 lds4(const unsigned short *p)  // four consecutive BF16 values (8-byte aligned)
 {
-    mlp_s4 r;
-    r[0] = (short)p[0]; r[1] = (short)p[1]; r[2] = (short)p[2]; r[3] = (short)p[3];
-    return r;
+    return *reinterpret_cast<const mlp_s4 *>(p);
+}
+GF_DEV float4 lds_f4(const float *p) { return *reinterpret_cast<const float4 *>(p); }
+// two floats to BF16 (round to nearest even, the hardware conversion), packed
+GF_DEV unsigned int bf16x2(float a, float b)
+{
+    return (unsigned int)__builtin_bit_cast(unsigned short, (__bf16)a) |
+           ((unsigned int)__builtin_bit_cast(unsigned short, (__bf16)b) << 16);
 }
 
-// Forward of the 64 samples whose normalised inputs (BF16, 8 per row, the 8th 0) are
-// in sx[64][8]; y[64] receives the outputs.  Each wave of the workgroup calls it on
-// its own rows (all waves together: it synchronises the workgroup); sh is the wave's
-// [32][64] BF16 scratch for the hidden activations.  MFMA 32x32x8 operand layout: lane l holds
-// A[l % 32][4 (l / 32) + q] and B[4 (l / 32) + q][l % 32] (q = 0..3), and the result
-// D[8 (v / 4) + 4 (l / 32) + v % 4][l % 32] in accumulator v = 0..15.
-GF_DEV void mlp_wave64(const MlpLds &L, unsigned short (*sx)[8], unsigned short (*sh)[MLP_NH], float *y)
+// Orders the wave's LDS accesses around an exchange between its own lanes: a wave's
+// LDS operations complete in issue order, and the fences keep the compiler from
+// moving them across.
+GF_DEV void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Forward pass of the wave's 64 samples; lane l passes its sample's normalised inputs
+// xb[8] (BF16, the 8th 0) and receives its output.  sx is the wave's own [64] row
+// buffer in LDS.  Two tiles of 32 samples; every product is computed transposed
+// (hidden units x samples), so that the MFMA 32x32x8 result registers of one layer are
+// the next layer's operand registers.  MFMA operand layout: lane l holds A[l % 32][4 (l / 32) + q]
+// and B[4 (l / 32) + q][l % 32] (q = 0..3); the result D[8 (v / 4) + 4 (l / 32) + v % 4][l % 32]
+// is in accumulator v = 0..15.
+//   layer 1: Z1^T (hidden 32c.. x samples) = W1[32c..][0..8) X^T; lane l then holds, for
+//            its sample l % 32 of the tile, hidden units 32c + 8g + 4 (l / 32) + q in
+//            accumulator 4g + q -- the B fragment of K step s = 4c + g of layer 2;
+//   layer 2: Z2^T (hidden 32c2.. x samples) = sum_s W2[32c2..][8s..8s+8) H1^T[8s..8s+8);
+//   layer 3: y = w3 . silu(Z2 + b2) + b3, summed over each lane's 32 hidden units and
+//            over the two half-waves.
+GF_DEV float mlp_wave64(const MlpLds &L, mlp_row *sx, const unsigned short xb[8])
 {
     const int l = threadIdx.x & 63, r32 = l & 31, hi = l >> 5;
-    for (int tile = 0; tile < 2; tile++) {
-        unsigned short(*x)[8] = sx + 32 * tile;
-        // layer 1: Z1 (32 x 64) = X (32 x 8) W1^T
-        const mlp_s4 ax = lds4(&x[r32][4 * hi]);
-        for (int c = 0; c < 2; c++) {
-            mlp_f16 acc = {0};
-            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ax, lds4(&L.w1[32 * c + r32][4 * hi]), acc, 0, 0, 0);
-            const int col = 32 * c + r32;
-            for (int v = 0; v < 16; v++) {
-                const int row = 8 * (v / 4) + 4 * hi + (v % 4);
-                sh[row][col] = f2bf(mlp_silu(acc[v] + L.b1[col]));
-            }
-        }
-        __syncthreads();
-        // layer 2: Z2 (32 x 64) = H1 (32 x 64) W2^T, eight K steps of 8
-        mlp_f16 acc2[2] = {{0}, {0}};
-        for (int s = 0; s < 8; s++) {
-            const mlp_s4 ah = lds4(&sh[r32][8 * s + 4 * hi]);
-            for (int c = 0; c < 2; c++)
-                acc2[c] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, lds4(&L.w2[32 * c + r32][8 * s + 4 * hi]),
-                                                                   acc2[c], 0, 0, 0);
-        }
-        // layer 3: y = W3 . silu(Z2) + b3; each lane holds 16 rows of one column
-        float part[16];
-        for (int v = 0; v < 16; v++) part[v] = 0.0f;
-        for (int c = 0; c < 2; c++) {
-            const int col = 32 * c + r32;
-            for (int v = 0; v < 16; v++) {
-                const float h = bf2f(f2bf(mlp_silu(acc2[c][v] + L.b2[col])));
-                part[v] = part[v] + h * L.w3[col];
-            }
-        }
-        // sum over the 32 columns held by the lanes of each half-wave
-        for (int v = 0; v < 16; v++) {
-            float s = part[v];
-            for (int off = 16; off > 0; off >>= 1) s += __shfl_xor(s, off, 32);
-            part[v] = s;
-        }
-        if (r32 == 0)
-            for (int v = 0; v < 16; v++) y[32 * tile + 8 * (v / 4) + 4 * hi + (v % 4)] = part[v] + L.b3;
-        __syncthreads();  // sh is reused by the next tile; y is read after the call
+    wave_lds_sync();  // the previous call's reads of sx are done
+    {
+        uint2 *row = reinterpret_cast<uint2 *>(sx[l].v);
+        row[0] = make_uint2(xb[0] | ((unsigned int)xb[1] << 16), xb[2] | ((unsigned int)xb[3] << 16));
+        row[1] = make_uint2(xb[4] | ((unsigned int)xb[5] << 16), xb[6] | ((unsigned int)xb[7] << 16));
     }
+    wave_lds_sync();
+    float y[2];
+#pragma unroll 1
+    for (int t = 0; t < 2; t++) {
+        // the weight and bias loads stay inside the tile (hoisted, they would hold
+        // about 100 registers for the whole pass)
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const mlp_s4 bx = lds4(&sx[32 * t + r32].v[4 * hi]);
+        mlp_s4 hb[8];
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            mlp_f16 z = {0};
+            z = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(lds4(&L.w1[32 * c + r32][4 * hi]), bx, z, 0, 0, 0);
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const float4 b = lds_f4(&L.b1[32 * c + 8 * g + 4 * hi]);
+                const unsigned int p0 = bf16x2(mlp_silu_fast(z[4 * g] + b.x), mlp_silu_fast(z[4 * g + 1] + b.y));
+                const unsigned int p1 = bf16x2(mlp_silu_fast(z[4 * g + 2] + b.z), mlp_silu_fast(z[4 * g + 3] + b.w));
+                hb[4 * c + g] = __builtin_bit_cast(mlp_s4, make_uint2(p0, p1));
+            }
+        }
+        mlp_f16 z2[2] = {{0}, {0}};
+#pragma unroll
+        for (int st = 0; st < 8; st++)
+#pragma unroll
+            for (int c = 0; c < 2; c++)
+                z2[c] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(lds4(&L.w2[32 * c + r32][8 * st + 4 * hi]), hb[st],
+                                                                 z2[c], 0, 0, 0);
+        float part = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const float4 b = lds_f4(&L.b2[32 * c + 8 * g + 4 * hi]), w3 = lds_f4(&L.w3[32 * c + 8 * g + 4 * hi]);
+                const float zb[4] = {z2[c][4 * g] + b.x, z2[c][4 * g + 1] + b.y, z2[c][4 * g + 2] + b.z,
+                                     z2[c][4 * g + 3] + b.w};
+                const float wq[4] = {w3.x, w3.y, w3.z, w3.w};
+#pragma unroll
+                for (int q = 0; q < 4; q++) part = part + bf2f(f2bf_hw(mlp_silu_fast(zb[q]))) * wq[q];
+            }
+        part = part + __shfl_xor(part, 32);
+        y[t] = part + L.b3;
+    }
+    return hi ? y[1] : y[0];
 }
 
 // standalone inference for tests: xn[n][7] normalised inputs -> y[n]
@@ -364,73 +422,190 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const float *__restrict__ w, 
                                                    float *__restrict__ yout)
 {
     __shared__ MlpLds L;
-    __shared__ unsigned short sx[4][64][8];
-    __shared__ unsigned short sh[4][32][MLP_NH];
-    __shared__ float sy[4][64];
+    __shared__ mlp_row sx[4][64];
     mlp_load_weights(w, L);
     __syncthreads();
-    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    for (int f = 0; f < 8; f++) sx[wv][l][f] = (idx < n && f < MLP_NIN) ? f2bf(xn[idx * MLP_NIN + f]) : (unsigned short)0;
-    __syncthreads();
-    mlp_wave64(L, sx[wv], sh[wv], sy[wv]);
-    if (idx < n) yout[idx] = sy[wv][l];
+    unsigned short xb[8];
+    for (int f = 0; f < 8; f++) xb[f] = (idx < n && f < MLP_NIN) ? f2bf(xn[idx * MLP_NIN + f]) : (unsigned short)0;
+    const float y = mlp_wave64(L, sx[wv], xb);
+    if (idx < n) yout[idx] = y;
 }
 
-// Eff_visc with the surrogate (viscosity-update steps, like k_visc)
+// Eff_visc with the surrogate (viscosity-update steps, like k_visc), in two passes
+// over the same persistent grid of G workgroups (the network's weights go to LDS once
+// per workgroup).  k_visc_mlp runs the network on every element of the Maxwell
+// rheologies and writes the viscosity of those inside the gate; the others, and every
+// element of the audited waves, go to the workgroup's own segment of a list (cap
+// entries from g * cap, counted in LDS: no global atomics per wave).  k_visc_list has
+// the grid evaluate the exact formula on the segments' entries, spread evenly over
+// its workgroups (compacted, so that a few fallbacks do not hold whole waves on the
+// slow path; evenly, because fallbacks cluster in space and so in a few segments),
+// write it and judge the audited elements.  cnt[G]: the segment lengths; wgs[G]: each
+// workgroup's statistics.
+
+GF_DEV double mlp_visc(const GfDevParams *P, float lg)
+{
+    return fmin(P->v_max, fmax(P->v_min, (double)exp2f(lg * 3.32192809488736234787f)));
+}
+
+#define MLP_LIST_AUDIT 0x80000000u  // list entry flag: an audited element
+#define MLP_MAXG 2048               // workgroups of the persistent viscosity grid, at most
+
 template <typename SV>
-__global__ void __launch_bounds__(256) k_visc_mlp(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
+__global__ void __launch_bounds__(256, 4) k_visc_mlp(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
                                                   const float *__restrict__ w, const MlpCfg *__restrict__ cfg,
-                                                  MlpStats *st)
+                                                  const MlpStats *st, unsigned int *__restrict__ list,
+                                                  float *__restrict__ lval, unsigned int *__restrict__ cnt,
+                                                  MlpWgStats *__restrict__ wgs, long cap)
 {
     __shared__ MlpLds L;
-    __shared__ unsigned short sx[4][64][8];
-    __shared__ unsigned short sh[4][32][MLP_NH];
-    __shared__ float sy[4][64];
+    __shared__ mlp_row sx[4][64];
+    __shared__ unsigned int s_len, s_ev[4], s_fb[4];
     mlp_load_weights(w, L);
+    if (threadIdx.x == 0) s_len = 0u;
     __syncthreads();
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const long e = xcd_lin();
-    const bool in = e < NE;
-    int irh = 0, j = 1, i = 1;
-    double prv[GF_MAXACT], a11 = 0, a22 = 0, a33 = 0;
-    float x[MLP_NIN];
-    bool use = false;
-    if (in) {
-        irh = PHASE(A.iphase[e]).irheol;
-        j = (int)(e % (NZ - 1)) + 1;
-        i = (int)(e / (NZ - 1)) + 1;
-        for (int a = 0; a < P->nactive; a++) prv[a] = A.pr[a * NE + e];
-        stress_avg<SV>(P, A, e, a11, a22, a33);
-        if ((irh == 3 || irh >= 11) && !st->off) {
-            use = visc_features(P, A, j, i, a11, a22, a33, prv, x) && mlp_in_distribution(cfg, x);
+    const long ne = NE;
+    const int nzm = NZ - 1;
+    const bool off = st->off != 0;
+    unsigned int *seg = list + blockIdx.x * cap;
+    float *segv = lval + blockIdx.x * cap;
+    unsigned int nev = 0, nfb = 0;  // wave-uniform counts
+    for (long blk = blockIdx.x; blk * 256 < ne; blk += gridDim.x) {
+        const long e = blk * 256 + threadIdx.x;
+        const bool in = e < ne;
+        const int j = (int)(e % nzm) + 1, i = (int)(e / nzm) + 1;
+        ViscLoad v{};
+        if (in) v = visc_load<SV>(P, A, e, j, i);
+        const int irh = in ? PHASE(v.iph).irheol : 0;
+        const bool visc = in && visc_rheology(irh);
+        float x[MLP_NIN];
+        const bool use = visc && !off && visc_features_in(P, j, i, v.a11, v.a22, v.a33, v.prv, v.vin, x) &&
+                         mlp_in_distribution(cfg, st, x);
+        unsigned short xb[8];
+        for (int f = 0; f < 8; f++)
+            xb[f] = (use && f < MLP_NIN) ? f2bf((x[f] - cfg->xmean[f]) / cfg->xstd[f]) : (unsigned short)0;
+        const float lg = mlp_wave64(L, sx[wv], xb) * cfg->ystd + cfg->ymean;  // every lane takes part
+        const long gw = blk * 4 + wv;            // waves audited as a whole
+        const bool aud = cfg->audit_stride > 0 && gw % cfg->audit_stride == cfg->audit_phase;
+        if (use && !aud) A.visn[e] = mlp_visc(P, lg);
+        // the rest to the workgroup's segment, one LDS atomic per wave
+        const bool app = visc && (!use || aud);
+        const unsigned long long m = __ballot(app);
+        if (m) {
+            const int lead = __ffsll((long long)m) - 1;
+            unsigned int base = 0;
+            if (l == lead) base = atomicAdd(&s_len, (unsigned int)__popcll(m));
+            base = __shfl(base, lead);
+            if (app) {
+                const unsigned int k = base + (unsigned int)__popcll(m & ((1ull << l) - 1ull));
+                seg[k] = (unsigned int)e | (use ? MLP_LIST_AUDIT : 0u);
+                segv[k] = lg;
+            }
         }
+        nev += (unsigned int)__popcll(__ballot(use && !aud));
+        nfb += (unsigned int)__popcll(__ballot(visc && !use));
     }
-    for (int f = 0; f < 8; f++)
-        sx[wv][l][f] = (use && f < MLP_NIN) ? f2bf((x[f] - cfg->xmean[f]) / cfg->xstd[f]) : (unsigned short)0;
+    if (l == 0) {
+        s_ev[wv] = nev;
+        s_fb[wv] = nfb;
+    }
     __syncthreads();
-    mlp_wave64(L, sx[wv], sh[wv], sy[wv]);  // every lane of the wave takes part
-    if (!in || !(irh == 3 || irh >= 11)) return;
-    double vis;
-    if (use) {
-        const double lg = (double)sy[wv][l] * cfg->ystd + cfg->ymean;
-        vis = fmin(P->v_max, fmax(P->v_min, exp10(lg)));
-        if (cfg->audit_stride > 0 && e % cfg->audit_stride == cfg->audit_phase) {
-            const double ex = dev_eff_visc(P, A, j, i, a11, a22, a33, prv);
-            const float err = (float)fabs(log10(vis) - log10(ex));
-            atomicMax(&st->audit_maxerr_bits, __float_as_uint(err));
-            atomicMax(&st->worst, ((unsigned long long)__float_as_uint(err) << 32) | (unsigned long long)e);
-            atomicAdd(&st->audits, 1ull);
-            if (err > cfg->tol) atomicOr(&st->off, 1);
-        }
-    } else {
-        vis = dev_eff_visc(P, A, j, i, a11, a22, a33, prv);
+    if (threadIdx.x == 0) {
+        cnt[blockIdx.x] = s_len;
+        MlpWgStats &ws = wgs[blockIdx.x];
+        ws.evals += s_ev[0] + s_ev[1] + s_ev[2] + s_ev[3];
+        ws.fallbacks += s_fb[0] + s_fb[1] + s_fb[2] + s_fb[3];
     }
-    A.visn[e] = vis;
-    // one counter update per wave
-    const unsigned long long nu = __popcll(__ballot(use)), nf = __popcll(__ballot(!use));
-    if (l == __ffsll((long long)__ballot(1)) - 1) {
-        atomicAdd(&st->evals, nu);
-        atomicAdd(&st->fallbacks, nf);
+}
+
+template <typename SV>
+__global__ void __launch_bounds__(256, 4) k_visc_list(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
+                                                   const MlpCfg *__restrict__ cfg, MlpStats *st,
+                                                   const unsigned int *__restrict__ list,
+                                                   const float *__restrict__ lval,
+                                                   const unsigned int *__restrict__ cnt,
+                                                   MlpWgStats *__restrict__ wgs, long cap)
+{
+    __shared__ unsigned int s_aud, s_fail, s_max;
+    __shared__ unsigned long long s_worst;
+    __shared__ unsigned int s_pre[MLP_MAXG + 1];  // exclusive prefix sums of the segment lengths
+    __shared__ unsigned int s_wsum[4];
+    const unsigned int G = gridDim.x;
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (threadIdx.x == 0) {
+        s_aud = 0u;
+        s_fail = 0u;
+        s_max = 0u;
+        s_worst = 0ull;
+    }
+    {  // thread t scans segments 8t .. 8t+7
+        constexpr int PER = MLP_MAXG / 256;
+        unsigned int c[PER], run = 0;
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const unsigned int g = threadIdx.x * PER + q;
+            c[q] = g < G ? cnt[g] : 0u;
+            run += c[q];
+        }
+        unsigned int inc = run;  // inclusive scan over the wave
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned int y = __shfl_up(inc, o);
+            if (l >= o) inc += y;
+        }
+        if (l == 63) s_wsum[wv] = inc;
+        __syncthreads();
+        unsigned int base = inc - run;
+        for (int q = 0; q < wv; q++) base += s_wsum[q];
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            s_pre[threadIdx.x * PER + q] = base;
+            base += c[q];
+        }
+        if (threadIdx.x == 255) s_pre[MLP_MAXG] = base;
+        __syncthreads();
+    }
+    const unsigned int n = s_pre[MLP_MAXG];
+    const int nzm = NZ - 1;
+    for (unsigned int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += G * blockDim.x) {
+        // the segment of flattened entry k: the last g with s_pre[g] <= k
+        unsigned int lo = 0, hi = G - 1;
+        while (lo < hi) {
+            const unsigned int mid = (lo + hi + 1) >> 1;
+            if (s_pre[mid] <= k) lo = mid;
+            else hi = mid - 1;
+        }
+        const long at = (long)lo * cap + (k - s_pre[lo]);
+        const unsigned int item = list[at];
+        const long e = (long)(item & ~MLP_LIST_AUDIT);
+        const int j = (int)(e % nzm) + 1, i = (int)(e / nzm) + 1;
+        const ViscLoad v = visc_load<SV>(P, A, e, j, i);
+        const double ex = dev_eff_visc_in(P, j, i, v.a11, v.a22, v.a33, v.prv, v.vin);
+        A.visn[e] = ex;
+        if (item & MLP_LIST_AUDIT) {
+            const float err = (float)fabs(log10(mlp_visc(P, lval[at])) - log10(ex));
+            atomicMax(&s_max, __float_as_uint(err));
+            atomicMax(&s_worst, ((unsigned long long)__float_as_uint(err) << 32) | (unsigned long long)e);
+            atomicAdd(&s_aud, 1u);
+            if (err > cfg->tol) {  // take the element's cell off
+                float x[MLP_NIN];
+                visc_features_in(P, j, i, v.a11, v.a22, v.a33, v.prv, v.vin, x);
+                int r, c;
+                mlp_cell(cfg, x, r, c);
+                atomicOr(&st->bad[r], 1u << c);
+                atomicAdd(&s_fail, 1u);
+            }
+            if (err > MLP_OFF_FACTOR * cfg->tol) atomicOr(&st->off, 1);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        MlpWgStats &ws = wgs[blockIdx.x];
+        ws.audits += s_aud;
+        ws.failed += s_fail;
+        ws.audit_maxerr_bits = max(ws.audit_maxerr_bits, s_max);
+        ws.worst = max(ws.worst, s_worst);
     }
 }

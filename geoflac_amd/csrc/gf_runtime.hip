@@ -45,6 +45,14 @@ struct gf_ctx {
     float *sur_w = nullptr, *sur_m = nullptr, *sur_v = nullptr, *sur_grad = nullptr, *sur_partial = nullptr;
     MlpCfg *sur_cfg = nullptr;
     MlpStats *sur_stats = nullptr;
+    // elements left to the exact formula, in one segment of sur_seg entries per
+    // workgroup of the sur_grid persistent workgroups; their network value (audits);
+    // the segment lengths; each workgroup's statistics
+    unsigned int *sur_list = nullptr, *sur_cnt = nullptr;
+    MlpWgStats *sur_wgs = nullptr;
+    float *sur_lval = nullptr;
+    int sur_grid = 0;
+    long sur_seg = 0;
     MlpCfg sur_cfg_host{};
     bool sur_trained = false, sur_on = false;
     int err_j = 0, err_i = 0;
@@ -182,6 +190,9 @@ extern "C" int gf_create(const gf_params *p, int32_t precision, gf_ctx **out)
     // the flexural base and surface processes of fl_move (src/fl_move.f90:37-40)
     if (p->flexure_base != 0 || p->topo_kappa > 0.0) return GF_ERR_UNSUPPORTED;
     if (p->inhom < 0 || p->inhom > GF_MAXINH) return GF_ERR_ARG;
+    // the kernels address each array with 32-bit byte offsets: the 16 FP64 stress
+    // planes (128 bytes per element) must stay under 4 GiB
+    if ((long)(p->nz - 1) * (p->nx - 1) * 128L > 0xffffffffL) return GF_ERR_ARG;
     // deeper than ~240 km, Eff_dens would rewrite the global dt fraction (src/matprops.f90:37-51)
     if (p->rzbo < -240.0e3) return GF_ERR_UNSUPPORTED;
     ctx = new gf_ctx();
@@ -286,7 +297,8 @@ extern "C" void gf_destroy(gf_ctx *ctx)
                     ctx->a.e2sr, ctx->a.sshrheat, ctx->a.shrheat, ctx->a.source, ctx->a.iphase, ctx->a.pr,
                     ctx->a.xa, ctx->a.xb, ctx->a.dens, ctx->therm_border, ctx->a.stress0f, ctx->a.velf,
                     ctx->a.forcef, ctx->a.pref, ctx->sur_samples, ctx->sur_w, ctx->sur_m, ctx->sur_v,
-                    ctx->sur_grad, ctx->sur_partial, ctx->sur_cfg, ctx->sur_stats};
+                    ctx->sur_grad, ctx->sur_partial, ctx->sur_cfg, ctx->sur_stats, ctx->sur_list,
+                    ctx->sur_cnt, ctx->sur_lval, ctx->sur_wgs};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -685,9 +697,14 @@ static int enq_rheol(gf_ctx *ctx)
     if ((do_visc || ctx->rheol_quad) && (rc = flush_rot(ctx))) return rc;
     const int rot = ctx->rot_pending ? 1 : 0;
     ctx->rot_pending = false;
-    if (do_visc && ctx->sur_on)
-        LAUNCH_SV(k_visc_mlp, ctx->ne, ctx->dp, ctx->dck, ctx->a, ctx->sur_w, ctx->sur_cfg, ctx->sur_stats);
-    else if (do_visc)
+    if (do_visc && ctx->sur_on) {
+        // the same persistent grid for both passes (a few workgroups per CU)
+        const long pers = (long)ctx->sur_grid * 256;
+        LAUNCH_SV(k_visc_mlp, pers, ctx->dp, ctx->dck, ctx->a, ctx->sur_w, ctx->sur_cfg, ctx->sur_stats,
+                  ctx->sur_list, ctx->sur_lval, ctx->sur_cnt, ctx->sur_wgs, ctx->sur_seg);
+        LAUNCH_SV(k_visc_list, pers, ctx->dp, ctx->dck, ctx->a, ctx->sur_cfg, ctx->sur_stats, ctx->sur_list,
+                  ctx->sur_lval, ctx->sur_cnt, ctx->sur_wgs, ctx->sur_seg);
+    } else if (do_visc)
         LAUNCH_SV(k_visc, ctx->ne, ctx->dp, ctx->dck, ctx->a);
     if (ctx->rheol_quad && !mixed && !mstore)
         LAUNCH(k_rheol, 4 * ctx->ne, ctx->dp, ctx->dck, ctx->a, avgsr);
@@ -950,6 +967,16 @@ static int sur_alloc(gf_ctx *ctx)
     HIPCHK(hipMalloc((void **)&ctx->sur_partial, sizeof(float) * (MLP_NP + 1) * MLP_GRAD_BLOCKS));
     HIPCHK(hipMalloc((void **)&ctx->sur_cfg, sizeof(MlpCfg)));
     HIPCHK(hipMalloc((void **)&ctx->sur_stats, sizeof(MlpStats)));
+    {
+        const long nblocks = (ctx->ne + 255) / 256;
+        ctx->sur_grid = (int)std::min<long>(std::min<long>(nblocks, (long)ctx->therm_grid * 4), MLP_MAXG);
+        ctx->sur_seg = (nblocks + ctx->sur_grid - 1) / ctx->sur_grid * 256;  // a workgroup's elements at most
+    }
+    HIPCHK(hipMalloc((void **)&ctx->sur_list, sizeof(unsigned int) * ctx->sur_seg * ctx->sur_grid));
+    HIPCHK(hipMalloc((void **)&ctx->sur_lval, sizeof(float) * ctx->sur_seg * ctx->sur_grid));
+    HIPCHK(hipMalloc((void **)&ctx->sur_cnt, sizeof(unsigned int) * ctx->sur_grid));
+    HIPCHK(hipMalloc((void **)&ctx->sur_wgs, sizeof(MlpWgStats) * ctx->sur_grid));
+    HIPCHK(hipMemset(ctx->sur_wgs, 0, sizeof(MlpWgStats) * ctx->sur_grid));
     MlpStats z{};
     HIPCHK(hipMemcpy(ctx->sur_stats, &z, sizeof(z), hipMemcpyHostToDevice));
     return GF_OK;
@@ -1099,29 +1126,45 @@ extern "C" int gf_surrogate_enable(gf_ctx *ctx, int32_t on, float tol, int32_t a
         HIPCHK(hipMemcpy(ctx->sur_cfg, &ctx->sur_cfg_host, sizeof(MlpCfg), hipMemcpyHostToDevice));
         MlpStats z{};
         HIPCHK(hipMemcpy(ctx->sur_stats, &z, sizeof(z), hipMemcpyHostToDevice));
+        HIPCHK(hipMemset(ctx->sur_wgs, 0, sizeof(MlpWgStats) * ctx->sur_grid));
     }
     ctx->sur_on = on != 0;
     return GF_OK;
 }
 
-extern "C" int gf_surrogate_stats(gf_ctx *ctx, double stats[6])
+extern "C" int gf_surrogate_stats(gf_ctx *ctx, double stats[GF_SUR_NSTATS])
 {
     if (!ctx || !stats) return GF_ERR_ARG;
     if (!ctx->sur_stats) {
-        for (int q = 0; q < 6; q++) stats[q] = 0;
+        for (int q = 0; q < GF_SUR_NSTATS; q++) stats[q] = 0;
         return GF_OK;
     }
     MlpStats s;
+    std::vector<MlpWgStats> wg(ctx->sur_grid);
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipMemcpy(&s, ctx->sur_stats, sizeof(s), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(wg.data(), ctx->sur_wgs, sizeof(MlpWgStats) * wg.size(), hipMemcpyDeviceToHost));
+    MlpWgStats t{};
+    for (const MlpWgStats &g : wg) {
+        t.evals += g.evals;
+        t.fallbacks += g.fallbacks;
+        t.audits += g.audits;
+        t.failed += g.failed;
+        t.worst = std::max(t.worst, g.worst);
+        t.audit_maxerr_bits = std::max(t.audit_maxerr_bits, g.audit_maxerr_bits);
+    }
     float err;
-    memcpy(&err, &s.audit_maxerr_bits, sizeof(err));
-    stats[0] = (double)s.evals;
-    stats[1] = (double)s.fallbacks;
-    stats[2] = (double)s.audits;
+    memcpy(&err, &t.audit_maxerr_bits, sizeof(err));
+    stats[0] = (double)t.evals;
+    stats[1] = (double)t.fallbacks;
+    stats[2] = (double)t.audits;
     stats[3] = (double)err;
     stats[4] = (double)s.off;
-    stats[5] = s.audits ? (double)(s.worst & 0xffffffffull) : -1.0;
+    stats[5] = t.audits ? (double)(t.worst & 0xffffffffull) : -1.0;
+    stats[6] = (double)t.failed;
+    int nbad = 0;
+    for (int r = 0; r < MLP_OCC; r++) nbad += __builtin_popcount(s.bad[r]);
+    stats[7] = (double)nbad;
     return GF_OK;
 }
 

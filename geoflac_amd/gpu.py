@@ -162,11 +162,12 @@ class Model:
         self._check(lib().gf_surrogate_enable(self._ctx, int(on), float(tol), int(audit_stride)))
 
     def surrogate_stats(self):
-        """dict(evals, fallbacks, audits, max_audit_err, off) since the last enable."""
-        v = (C.c_double * 6)()
+        """dict(evals, fallbacks, audits, max_audit_err, off, worst_element, failed_audits,
+        cells_off) since the last enable."""
+        v = (C.c_double * 8)()
         self._check(lib().gf_surrogate_stats(self._ctx, v))
         return {"evals": int(v[0]), "fallbacks": int(v[1]), "audits": int(v[2]), "max_audit_err": v[3],
-                "off": int(v[4]), "worst_element": int(v[5])}
+                "off": int(v[4]), "worst_element": int(v[5]), "failed_audits": int(v[6]), "cells_off": int(v[7])}
 
     def surrogate_weights(self, w=None, cfg=None):
         """Without arguments: (weights[4737], cfg[66]); with them: load them."""

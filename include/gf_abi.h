@@ -190,13 +190,19 @@ int gf_surrogate_sample(gf_ctx *ctx, int64_t capacity, int64_t *count);
 int gf_surrogate_train(gf_ctx *ctx, int32_t steps, int32_t batch, float lr, uint64_t seed, float *loss_first,
                        float *loss_last);
 /* Uses the surrogate in the viscosity updates of the step (BF16 matrix-core inference)
- * with the out-of-distribution gate; every audit_stride-th element is checked against
- * the exact formula and an audited |log10 error| above tol switches it off. */
+ * with the out-of-distribution gate.  Audits go by runs of 64 consecutive elements
+ * (element e = (i-1)*(nz-1) + j-1): in a call at loop counter nloop, the runs
+ * r = e / 64 with r % audit_stride == nloop % audit_stride at enable time take the
+ * exact formula and compare it with the network (audit_stride 0: no audits). */
 int gf_surrogate_enable(gf_ctx *ctx, int32_t on, float tol, int32_t audit_stride);
-/* stats: elements on the surrogate, fallbacks to the exact formula, audits, largest
+/* stats: elements given the network's value, elements outside the gate (exact
+ * formula), audited elements inside the gate (exact formula), largest
  * audited |log10 error|, switched off (0/1), element of the largest audited error
- * (-1: none); since the last enable. */
-int gf_surrogate_stats(gf_ctx *ctx, double stats[6]);
+ * (-1: none), audits above tol, (1/T, log10 strain rate) cells taken off the surrogate
+ * by such audits; since the last enable.  An audited element always gets the exact
+ * value; an audit above tol takes its cell off, one above 4 tol the whole surrogate. */
+#define GF_SUR_NSTATS 8
+int gf_surrogate_stats(gf_ctx *ctx, double stats[GF_SUR_NSTATS]);
 /* get = 1: copy out, get = 0: copy in the parameters w[4737] and, if cfg is not NULL,
  * the gate and normalisation cfg[66] = xmean[8] xstd[8] xlo[8] xhi[8] ymean ystd and
  * 32 words of the (1/T, log10 strain rate) occupancy grid as float bit patterns. */
