@@ -61,6 +61,11 @@ def alg_bytes(kernel, E, N, nact, p):
         return N * 48
     if kernel == "k_therm_bc":
         return N * 16
+    if kernel == "k_therm_check":
+        # the border check: 76 border values of each 64x16-node tile (52x12 outputs)
+        # against the assembled iterate; fallback passes are added where they ran
+        ntiles = -(-p.nz // 52) * -(-p.nx // 12)
+        return ntiles * 76 * 16
     return 0
 
 
@@ -278,7 +283,13 @@ def main():
     ms_tot, nl = times[dom]
     nact = int((host_state.phase_ratio >= 0.01).any(axis=(1, 2)).sum())
     ab = alg_bytes(dom, E, N, nact, gp)
-    step_bytes = sum(alg_bytes(k, E, N, nact, gp) * v[1] for k, v in times.items()) // args.profile_steps
+    # a fallback pass and its check count only where the pass ran (the launches after a
+    # proven pass exit at once)
+    step_bytes = sum(alg_bytes(k, E, N, nact, gp) * v[1] for k, v in times.items()
+                     if k not in ("k_therm_pipe_extra", "k_therm_check"))
+    step_bytes += alg_bytes("k_therm_pipe_extra", E, N, nact, gp) * th[1]
+    step_bytes += alg_bytes("k_therm_check", E, N, nact, gp) * (args.profile_steps + th[1])
+    step_bytes //= args.profile_steps
     step_ms = sum(v[0] for v in times.values()) / args.profile_steps
     avg_s = ms_tot / 1e3 / max(nl, 1)
     achieved = ab / avg_s / 1e9 if ab else 0.0
