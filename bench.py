@@ -39,7 +39,7 @@ def alg_bytes(kernel, E, N, nact, p):
         return N * (8 + 8 + 8 + 16) + E * (16 + 4 + 8)                # temp, temp0 out, x out, cord | area(1:2) iphase source
     if kernel in ("k_therm_tile_extra", "k_therm_pipe_extra"):
         return N * (8 + 8 + 8 + 16) + E * (16 + 4 + 8)                # temp0, x in, x out, cord | area(1:2) iphase source
-    if kernel == "k_node":
+    if kernel in ("k_node", "k_node_t"):
         return E * (96 + 32) + N * (16 + 16 + 16 + 8 + 16 + 16 + 16)  # stress0(1:3) area | cord rmass+amass vel ncod bc, vel+force out
     if kernel == "k_move_elem":
         return E * (32 + 96 + 24 + 32 + 32 + 96 + 24) + N * 32
@@ -273,7 +273,7 @@ def main():
     m.set_timing(True)
     m.step(args.profile_steps)
     kernels = ["k_rheol1<false>", "k_rheol1<true>", "k_rheol", "k_visc<double>", "k_therm_pipe",
-               "k_therm_pipe_extra", "k_therm_check", "k_node<0>", "k_move_elem<0>",
+               "k_therm_pipe_extra", "k_therm_check", "k_node<0>", "k_node_t<0>", "k_move_elem<0>",
                "k_move_area", "k_move_rot<0>", "k_move_cord<double>", "k_therm_bc", "k_dens<double>", "k_rmass", "k_dt_reduce<double>", "k_amass",
                "k_clock"]
     times = {k: m.kernel_time(k) for k in kernels}

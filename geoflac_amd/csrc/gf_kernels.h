@@ -1906,20 +1906,44 @@ GF_DEV void tri_force_v(const double *x, const double *y, double a, typename Pre
     }
 }
 
-// src/fl_node.f90:36-391, one thread per node: gather of the 12 triangle forces in
-// the reference's order (kk/mm tables of :7-8), gravity, optional boundary loads,
-// damping and the velocity update.
-// The force and velocity arithmetic runs in SV (double: the reference's, bit for bit;
-// float: mixed precision); geometry is formed in double from the FP64 coordinates.
-template <int M>
-GF_DEV void node_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long t)
+// the clock values the node update reads
+struct NodeClk {
+    double dt, dt_elastic, time;
+};
+GF_DEV NodeClk node_clk(const GfDevClock *ck) { return NodeClk{ck->dt, ck->dt_elastic, ck->time}; }
+// node t's own inputs of the update
+struct NodeOwn {
+    double rm, am, v1, v2, bc1, bc2;
+    int nc1, nc2;
+};
+template <typename SV>
+GF_DEV NodeOwn node_own(const GfDevParams *__restrict__ P, const GfDevArrays &A, long t)
+{
+    const SV *vel = sv_ptr<SV>(A.vel, A.velf);
+    NodeOwn o;
+    o.rm = A.rmass[t];
+    o.am = A.amass[t];
+    o.v1 = vel[t];
+    o.v2 = vel[t + NN];
+    o.bc1 = A.bc[t];
+    o.bc2 = A.bc[t + NN];
+    o.nc1 = A.ncod[t];
+    o.nc2 = A.ncod[t + NN];
+    return o;
+}
+
+// The rest of node (j,i) = node t from the values around it: cx, cy of nodes
+// (j + dj, i + di) at [dj + 1][di + 1]; the areas, stresses and references of its
+// elements (j-1,i-1) (j-1,i) (j,i-1) (j,i), triangle kk[el][k] at [el][k].
+template <int M, typename Src, typename Own>
+GF_DEV void node_update(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long t, int j,
+                        int i, Src src, Own ownf, const NodeClk &kc)
 {
     typedef typename Prec<M>::SV SV;
     typedef typename Prec<M>::AT AT;
-    const int j = (int)(t % NZ) + 1, i = (int)(t / NZ) + 1;
     SV *force = sv_ptr<SV>(A.force, A.forcef), *vel = sv_ptr<SV>(A.vel, A.velf);
-    const double dt = ck->dt;
-    double drat = dt / ck->dt_elastic;
+    const double dt = kc.dt;
+    double drat = dt / kc.dt_elastic;
     if (drat < 1.0) drat = 1.0;
     AT g1 = 0, g2 = 0;      // triangle forces in AT
     double r1 = 0, r2 = 0;  // mixed precision: forces of the isotropic references
@@ -1929,52 +1953,21 @@ GF_DEV void node_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const G
     }
     constexpr int kk[4][3] = {{2, 3, 4}, {1, 2, 3}, {1, 2, 4}, {1, 3, 4}};
     constexpr int mm[4][3] = {{3, 3, 2}, {2, 2, 2}, {3, 1, 3}, {1, 1, 1}};
-    // elements (j-1,i-1) (j-1,i) (j,i-1) (j,i) as offsets from the node
     constexpr int dje[4] = {-1, -1, 0, 0}, die[4] = {-1, 0, -1, 0};
     const bool okk[4] = {j != 1 && i != 1, j != 1 && i != NX, j != NZ && i != 1, j != NZ && i != NX};
-    // Every load is issued before the first use, through buffer descriptors: offsets
-    // of missing neighbours fall outside the arrays (read 0) or on values that are
-    // not used.
-    const unsigned nzu = (unsigned)NZ, nnu = (unsigned)NN, neu = (unsigned)NE;
-    const Buf bC = mkbuf(A.cord, 16ull * nnu), bA = mkbuf(A.area, 32ull * neu);
-    const Buf bS = mkbuf(sv_ptr<SV>(A.stress0, A.stress0f), 16ull * sizeof(SV) * neu);
-    double cx[3][3], cy[3][3];  // nodes (j + dj, i + di) at [dj + 1][di + 1]
-#pragma unroll
-    for (int dj = -1; dj <= 1; dj++)
-#pragma unroll
-        for (int di = -1; di <= 1; di++) {
-            const unsigned o = (unsigned)((j - 1 + dj) + (int)nzu * (i - 1 + di)) * 8u;
-            cx[dj + 1][di + 1] = bld(bC, o, 0);
-            cy[dj + 1][di + 1] = bld(bC, o, nnu * 8u);
-        }
-    double ar[4][3];
-    SV st[4][3][3];
-    double pr[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int el = 0; el < 4; el++) {
-        const unsigned e = (unsigned)((j - 1 + dje[el]) + ((int)nzu - 1) * (i - 1 + die[el]));
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const unsigned kb = (unsigned)(kk[el][k] - 1);
-            ar[el][k] = bld(bA, e * 8u, kb * neu * 8u);
-#pragma unroll
-            for (int c = 0; c < 3; c++) st[el][k][c] = bldt<SV>(bS, e * (unsigned)sizeof(SV), (c + 4 * kb) * neu * (unsigned)sizeof(SV));
-        }
-        if (M != 0 && okk[el]) pr[el] = ref_press<SV>(A, (long)e);
-    }
 #pragma unroll
     for (int el = 0; el < 4; el++) {
         if (!okk[el]) continue;
-        // corners 1 (je,ie), 2 (je+1,ie), 3 (je,ie+1), 4 (je+1,ie+1)
-        const int a0 = dje[el] + 1, b0 = die[el] + 1;
-        const double x[4] = {cx[a0][b0], cx[a0 + 1][b0], cx[a0][b0 + 1], cx[a0 + 1][b0 + 1]};
-        const double y[4] = {cy[a0][b0], cy[a0 + 1][b0], cy[a0][b0 + 1], cy[a0 + 1][b0 + 1]};
+        // corners 1 (je,ie), 2 (je+1,ie), 3 (je,ie+1), 4 (je+1,ie+1); triangle kk[el][k]'s
+        // area, stress and the element's reference
+        double x[4], y[4], ar[3], pr = 0.0;
+        SV st[3][3];
+        src(el, x, y, ar, st, pr);
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             AT fx, fy;
             double rx = 0, ry = 0;
-            tri_force_v<M>(x, y, ar[el][k], st[el][k][0], st[el][k][1], st[el][k][2], pr[el], kk[el][k], mm[el][k],
-                           fx, fy, rx, ry);
+            tri_force_v<M>(x, y, ar[k], st[k][0], st[k][1], st[k][2], pr, kk[el][k], mm[el][k], fx, fy, rx, ry);
             g1 = g1 - fx;
             g2 = g2 - fy;
             if (M == 1) {
@@ -1989,7 +1982,8 @@ GF_DEV void node_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const G
         f1 = f1 + r1;
         f2 = f2 + r2;
     }
-    f2 = f2 - A.rmass[t] * P->g;
+    const NodeOwn own = ownf();  // the node's own values, where they are first needed
+    f2 = f2 - own.rm * P->g;
     if (P->nyhydro > 0 && (j == 1 || j == NZ)) {
         double pl, dlxl, dlyl, pr, dlxr, dlyr;
         if (j == 1) {  // sea-water load on the surface, :260-297
@@ -2045,20 +2039,20 @@ GF_DEV void node_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const G
             f2 = f2 + 0.5 * pl * dlxl + 0.5 * pr * dlxr;
         }
     }
-    const int nc1 = A.ncod[t], nc2 = A.ncod[t + NN];
-    double v1 = vel[t], v2 = vel[t + NN];
+    const int nc1 = own.nc1, nc2 = own.nc2;
+    double v1 = own.v1, v2 = own.v2;
     if ((nc1 & 1) != 1 && fabs(v1) > 1.0e-13) f1 = f1 - P->demf * fsign(f1, v1);
     if ((nc2 & 2) != 2 && fabs(v2) > 1.0e-13) f2 = f2 - P->demf * fsign(f2, v2);
-    const double am = A.amass[t];
+    const double am = own.am;
     if (nc1 == 1)
-        v1 = A.bc[t];
+        v1 = own.bc1;
     else
         v1 = v1 + dt * f1 / (am * drat * drat);
     if (nc2 == 1)
-        v2 = A.bc[t + NN];
+        v2 = own.bc2;
     else
         v2 = v2 + dt * f2 / (am * drat * drat);
-    if (P->i_prestress == 1 && ck->time < 600.0e3 * 3.1558e+7 && (j == NZ || i == 1 || i == NX)) {
+    if (P->i_prestress == 1 && kc.time < 600.0e3 * 3.1558e+7 && (j == NZ || i == 1 || i == NX)) {
         v1 = 0.0;
         v2 = 0.0;
     }
@@ -2067,12 +2061,219 @@ GF_DEV void node_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const G
     vel[t] = SV(v1);
     vel[t + NN] = SV(v2);
 }
+// src/fl_node.f90:36-391, one thread per node: gather of the 12 triangle forces in
+// the reference's order (kk/mm tables of :7-8), gravity, optional boundary loads,
+// damping and the velocity update.
+// The force and velocity arithmetic runs in SV (double: the reference's, bit for bit;
+// float: mixed precision); geometry is formed in double from the FP64 coordinates.
+template <int M>
+GF_DEV void node_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long t)
+{
+    typedef typename Prec<M>::SV SV;
+    const int j = (int)(t % NZ) + 1, i = (int)(t / NZ) + 1;
+    constexpr int kk[4][3] = {{2, 3, 4}, {1, 2, 3}, {1, 2, 4}, {1, 3, 4}};
+    // elements (j-1,i-1) (j-1,i) (j,i-1) (j,i) as offsets from the node
+    constexpr int dje[4] = {-1, -1, 0, 0}, die[4] = {-1, 0, -1, 0};
+    const bool okk[4] = {j != 1 && i != 1, j != 1 && i != NX, j != NZ && i != 1, j != NZ && i != NX};
+    // Every load is issued before the first use, through buffer descriptors: offsets
+    // of missing neighbours fall outside the arrays (read 0) or on values that are
+    // not used.
+    const unsigned nzu = (unsigned)NZ, nnu = (unsigned)NN, neu = (unsigned)NE;
+    const Buf bC = mkbuf(A.cord, 16ull * nnu), bA = mkbuf(A.area, 32ull * neu);
+    const Buf bS = mkbuf(sv_ptr<SV>(A.stress0, A.stress0f), 16ull * sizeof(SV) * neu);
+    double cx[3][3], cy[3][3];  // nodes (j + dj, i + di) at [dj + 1][di + 1]
+#pragma unroll
+    for (int dj = -1; dj <= 1; dj++)
+#pragma unroll
+        for (int di = -1; di <= 1; di++) {
+            const unsigned o = (unsigned)((j - 1 + dj) + (int)nzu * (i - 1 + di)) * 8u;
+            cx[dj + 1][di + 1] = bld(bC, o, 0);
+            cy[dj + 1][di + 1] = bld(bC, o, nnu * 8u);
+        }
+    double ar[4][3];
+    SV st[4][3][3];
+    double pr[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int el = 0; el < 4; el++) {
+        const unsigned e = (unsigned)((j - 1 + dje[el]) + ((int)nzu - 1) * (i - 1 + die[el]));
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const unsigned kb = (unsigned)(kk[el][k] - 1);
+            ar[el][k] = bld(bA, e * 8u, kb * neu * 8u);
+#pragma unroll
+            for (int c = 0; c < 3; c++) st[el][k][c] = bldt<SV>(bS, e * (unsigned)sizeof(SV), (c + 4 * kb) * neu * (unsigned)sizeof(SV));
+        }
+        if (M != 0 && okk[el]) pr[el] = ref_press<SV>(A, (long)e);
+    }
+    auto src = [&](int el, double (&x)[4], double (&y)[4], double (&a)[3], SV (&sv)[3][3], double &p) {
+        const int a0 = dje[el] + 1, b0 = die[el] + 1;
+        x[0] = cx[a0][b0]; x[1] = cx[a0 + 1][b0]; x[2] = cx[a0][b0 + 1]; x[3] = cx[a0 + 1][b0 + 1];
+        y[0] = cy[a0][b0]; y[1] = cy[a0 + 1][b0]; y[2] = cy[a0][b0 + 1]; y[3] = cy[a0 + 1][b0 + 1];
+        for (int k = 0; k < 3; k++) {
+            a[k] = ar[el][k];
+            for (int c = 0; c < 3; c++) sv[k][c] = st[el][k][c];
+        }
+        p = pr[el];
+    };
+    node_update<M>(P, ck, A, t, j, i, src, [&]() { return node_own<SV>(P, A, t); }, node_clk(ck));
+}
+
 template <int M>
 __global__ void __launch_bounds__(256) k_node(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
 {
     long t = xcd_lin();
     if (t >= NN) return;
     node_body<M>(P, ck, A, t);
+}
+
+// The same node update with the inputs staged in LDS.  A workgroup takes a tile of
+// 64 x 4 nodes and loads, with coalesced loads, the 65 x 5 elements around it (12
+// stress planes, 4 areas, the isotropic reference in mixed precision) and the 66 x 6
+// nodes' coordinates: each element is loaded once per tile (1.27 per node) instead
+// of once for each of the four nodes that read it, so that a wave issues ~25 loads
+// instead of ~75.  Then each thread runs node_update on its node.
+#define ND_TJ 64
+#define ND_TI 4
+#define ND_EJ (ND_TJ + 1)        // element rows j0-1 .. j0+63
+#define ND_NEL (ND_EJ * (ND_TI + 1))
+#define ND_NJ (ND_TJ + 2)        // node rows j0-1 .. j0+64
+#define ND_NND (ND_NJ * (ND_TI + 2))
+template <int M>
+struct NodeLds {
+    typename Prec<M>::SV s[12][ND_NEL];  // stress component c of triangle kb at [c + 4 kb]
+    double a[4][ND_NEL];
+    double pr[M != 0 ? ND_NEL : 1];
+    double cx[ND_NND], cy[ND_NND];
+};
+GF_DEV int node_ntj(const GfDevParams *P) { return (P->nz + ND_TJ - 1) / ND_TJ; }
+GF_DEV int node_nti(const GfDevParams *P) { return (P->nx + ND_TI - 1) / ND_TI; }
+// one thread's share of a tile's inputs: element values v = tid + 256 r (plane p = v /
+// 325, slot v % 325 = row + 65 column; planes 0..11: stress component p % 3 of
+// triangle p / 3, 12..15: areas, 16: the isotropic reference), node coordinates v =
+// tid + 256 r (plane v / 396, slot row + 66 column), and the thread's own node's values
+template <int M>
+struct NodePf {
+    static constexpr int NV = (16 + (M != 0 ? 1 : 0)) * ND_NEL, NR = (NV + 255) / 256;
+    static constexpr int NC = (2 * ND_NND + 255) / 256;
+    double dv[NR];  // (FP32 stress values are held as doubles, exactly)
+    double cv[NC];
+    NodeOwn own;
+};
+GF_DEV void node_tile_origin(const GfDevParams *P, int tile, int &j0, int &i0)
+{
+    const int ntj = node_ntj(P);
+    j0 = (tile % ntj) * ND_TJ + 1;
+    i0 = (tile / ntj) * ND_TI + 1;
+}
+template <int M>
+GF_DEV NodePf<M> node_prefetch(const GfDevParams *__restrict__ P, const GfDevArrays &A, int tile)
+{
+    typedef typename Prec<M>::SV SV;
+    typedef NodePf<M> F;
+    int j0, i0;
+    node_tile_origin(P, tile, j0, i0);
+    const int nz = NZ, nx = NX;
+    const unsigned neu = (unsigned)NE, nnu = (unsigned)NN;
+    const Buf bA = mkbuf(A.area, 32ull * neu), bC = mkbuf(A.cord, 16ull * nnu);
+    const Buf bS = mkbuf(sv_ptr<SV>(A.stress0, A.stress0f), 16ull * sizeof(SV) * neu);
+    constexpr unsigned OOB = 0x80000000u;  // an offset past every array (reads 0)
+    F f;
+#pragma unroll
+    for (int r = 0; r < F::NR; r++) {
+        const int v = threadIdx.x + 256 * r, p = v / ND_NEL, sl = v % ND_NEL;
+        const int je = j0 - 1 + sl % ND_EJ, ie = i0 - 1 + sl / ND_EJ;
+        const bool ok = v < F::NV && je >= 1 && je <= nz - 1 && ie >= 1 && ie <= nx - 1;
+        const unsigned e = (unsigned)((je - 1) + (nz - 1) * (ie - 1));
+        f.dv[r] = 0.0;
+        if (p < 12)
+            f.dv[r] = (double)bldt<SV>(bS, ok ? e * (unsigned)sizeof(SV) : OOB,
+                                       (unsigned)(p % 3 + 4 * (p / 3)) * neu * (unsigned)sizeof(SV));
+        else if (p < 16)
+            f.dv[r] = bld(bA, ok ? e * 8u : OOB, (unsigned)(p - 12) * neu * 8u);
+        else if (M != 0 && ok)
+            f.dv[r] = ref_press<SV>(A, (long)e);
+    }
+#pragma unroll
+    for (int r = 0; r < F::NC; r++) {
+        const int v = threadIdx.x + 256 * r, p = v / ND_NND, sl = v % ND_NND;
+        const int jj = j0 - 1 + sl % ND_NJ, ii = i0 - 1 + sl / ND_NJ;
+        const bool ok = v < 2 * ND_NND && jj >= 1 && jj <= nz && ii >= 1 && ii <= nx;
+        f.cv[r] = bld(bC, ok ? (unsigned)((jj - 1) + nz * (ii - 1)) * 8u : OOB, (unsigned)(p & 1) * nnu * 8u);
+    }
+    const int j = j0 + (int)threadIdx.x % ND_TJ, i = i0 + (int)threadIdx.x / ND_TJ;
+    f.own = NodeOwn{};
+    if (j <= nz && i <= nx) f.own = node_own<SV>(P, A, (long)(j - 1) + (long)nz * (i - 1));
+    return f;
+}
+template <int M>
+GF_DEV void node_stage(const NodePf<M> &f, NodeLds<M> &L)
+{
+    typedef typename Prec<M>::SV SV;
+    typedef NodePf<M> F;
+#pragma unroll
+    for (int r = 0; r < F::NR; r++) {
+        const int v = threadIdx.x + 256 * r, p = v / ND_NEL, sl = v % ND_NEL;
+        if (v >= F::NV) continue;
+        if (p < 12)
+            L.s[p][sl] = SV(f.dv[r]);
+        else if (p < 16)
+            L.a[p - 12][sl] = f.dv[r];
+        else if (M != 0)
+            L.pr[sl] = f.dv[r];
+    }
+#pragma unroll
+    for (int r = 0; r < F::NC; r++) {
+        const int v = threadIdx.x + 256 * r, sl = v % ND_NND;
+        if (v < ND_NND)
+            L.cx[sl] = f.cv[r];
+        else if (v < 2 * ND_NND)
+            L.cy[sl] = f.cv[r];
+    }
+}
+// the update of the thread's node of `tile` from the staged values
+template <int M>
+GF_DEV void node_tile_update(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, int tile,
+                             const NodeLds<M> &L, const NodeOwn &own, const NodeClk &kc)
+{
+    typedef typename Prec<M>::SV SV;
+    int j0, i0;
+    node_tile_origin(P, tile, j0, i0);
+    const int tj = threadIdx.x % ND_TJ, ti = threadIdx.x / ND_TJ;
+    const int j = j0 + tj, i = i0 + ti;
+    if (j > NZ || i > NX) return;
+    constexpr int kk[4][3] = {{2, 3, 4}, {1, 2, 3}, {1, 2, 4}, {1, 3, 4}};
+    constexpr int dje[4] = {-1, -1, 0, 0}, die[4] = {-1, 0, -1, 0};
+    // one element's values at a time (the LDS reads are not hoisted over the others)
+    auto src = [&](int el, double (&x)[4], double (&y)[4], double (&a)[3], SV (&sv)[3][3], double &p) {
+        asm volatile("" ::: "memory");
+        const int n0 = (tj + dje[el] + 1) + ND_NJ * (ti + die[el] + 1);  // corner 1 of the element
+        x[0] = L.cx[n0]; x[1] = L.cx[n0 + 1]; x[2] = L.cx[n0 + ND_NJ]; x[3] = L.cx[n0 + ND_NJ + 1];
+        y[0] = L.cy[n0]; y[1] = L.cy[n0 + 1]; y[2] = L.cy[n0 + ND_NJ]; y[3] = L.cy[n0 + ND_NJ + 1];
+        const int sl = (tj + dje[el] + 1) + ND_EJ * (ti + die[el] + 1);
+        for (int k = 0; k < 3; k++) {
+            const int kb = kk[el][k] - 1;
+            a[k] = L.a[kb][sl];
+            for (int c = 0; c < 3; c++) sv[k][c] = L.s[c + 3 * kb][sl];
+        }
+        p = M != 0 ? L.pr[sl] : 0.0;
+    };
+    node_update<M>(P, ck, A, (long)(j - 1) + (long)NZ * (i - 1), j, i, src, [&]() { return own; }, kc);
+}
+// One tile per workgroup (XCD x takes the x-th contiguous eighth of the tiles, column
+// of tiles by column).  (Persistent workgroups that load the next tile into registers
+// while computing the current one ran slower: two workgroups per CU instead of three.)
+template <int M>
+__global__ void __launch_bounds__(256) k_node_t(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+{
+    __shared__ NodeLds<M> L;
+    const int ntiles = node_ntj(P) * node_nti(P);
+    const int q = (ntiles + 7) / 8, tile = (int)(blockIdx.x % 8) * q + (int)(blockIdx.x / 8);
+    if (tile >= ntiles) return;  // the whole workgroup
+    const NodeClk kc = node_clk(ck);
+    const NodePf<M> pf = node_prefetch<M>(P, A, tile);
+    node_stage<M>(pf, L);
+    __syncthreads();
+    node_tile_update<M>(P, ck, A, tile, L, pf.own, kc);
 }
 
 // ------------------------------------------------------------------ fl_move

@@ -57,7 +57,8 @@ struct gf_ctx {
     bool sur_trained = false, sur_on = false;
     int err_j = 0, err_i = 0;
     char msg[512] = {0};
-    bool rheol_quad = false;  // four lanes per element (GF_RHEOL=quad) instead of one
+    bool rheol_quad = false;   // four lanes per element (GF_RHEOL=quad) instead of one
+    bool node_direct = false;  // fl_node without the LDS tiles (GF_NODE=direct)
     bool rot_pending = false; // the last element move's stress/strain rotation is not applied yet
     unsigned therm_grid = 8;  // workgroups of the pipelined thermal kernel: the CUs, a multiple of 8
     // TH_MAXPASS: the thermal result is proven to be the reference's serial one (fallback
@@ -206,6 +207,8 @@ extern "C" int gf_create(const gf_params *p, int32_t precision, gf_ctx **out)
     {
         const char *rv = getenv("GF_RHEOL");
         ctx->rheol_quad = rv && strcmp(rv, "quad") == 0;
+        const char *nv = getenv("GF_NODE");
+        ctx->node_direct = nv && strcmp(nv, "direct") == 0;
     }
     const long ne = ctx->ne, nn = ctx->nn;
     int rc = GF_OK;
@@ -732,7 +735,12 @@ static int enq_node(gf_ctx *ctx)
 {
     int rc;
     if ((rc = flush_rot(ctx))) return rc;
-    LAUNCH_M(k_node, ctx->nn, ctx->dp, ctx->dck, ctx->a);
+    if (ctx->node_direct) {
+        LAUNCH_M(k_node, ctx->nn, ctx->dp, ctx->dck, ctx->a);
+    } else {  // tiles of ND_TJ x ND_TI nodes, a multiple of 8 workgroups
+        const long ntiles = (long)((ctx->p.nz + ND_TJ - 1) / ND_TJ) * ((ctx->p.nx + ND_TI - 1) / ND_TI);
+        LAUNCH_M(k_node_t, (ntiles + 7) / 8 * 8 * 256, ctx->dp, ctx->dck, ctx->a);
+    }
     return GF_OK;
 }
 

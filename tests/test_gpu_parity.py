@@ -87,6 +87,28 @@ def test_subroutine_bitwise_equal_to_oracle(sub, step):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("env", [("GF_NODE", "direct"), ("GF_RHEOL", "quad")])
+def test_kernel_variants_bitwise_equal_to_oracle(env, monkeypatch):
+    """The alternative kernels, chosen when a context is created: the node update
+    gathering straight from global memory (instead of the LDS tiles) and the rheology
+    with four lanes per element.  20 steps from the 10-step golden state stay bitwise
+    equal to the oracle."""
+    from geoflac_amd import gpu
+    monkeypatch.setenv(*env)
+    _say("case: %s=%s, 20 steps from the 10-step golden state of %s" % (env[0], env[1], CASE))
+    p = ol.golden_params(CASE)
+    st = golden_full(10)
+    ref = st.copy()
+    with gpu.Model(p, st) as m:
+        m.step(20)
+        m.download()
+    assert ol.step(p, ref, 20, cr=True) == 0
+    bad, worst, nexact = compare(st, ref, tol=0.0)
+    _say("  exact fields %d, differing %s" % (nexact, worst))
+    assert not bad, bad
+
+
+@pytest.mark.gpu
 def test_interleaved_calls_bitwise_equal_to_oracle():
     """The step leaves the element move's stress/strain rotation to the next fl_rheol.
     Steps and single subroutines in an order that crosses each point where the
