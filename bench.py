@@ -39,8 +39,10 @@ def alg_bytes(kernel, E, N, nact, p):
         return N * (8 + 8 + 8 + 16) + E * (16 + 4 + 8)                # temp, temp0 out, x out, cord | area(1:2) iphase source
     if kernel in ("k_therm_tile_extra", "k_therm_pipe_extra"):
         return N * (8 + 8 + 8 + 16) + E * (16 + 4 + 8)                # temp0, x in, x out, cord | area(1:2) iphase source
-    if kernel in ("k_node", "k_node_t"):
+    if kernel == "k_node":
         return E * (96 + 32) + N * (16 + 16 + 16 + 8 + 16 + 16 + 16)  # stress0(1:3) area | cord rmass+amass vel ncod bc, vel+force out
+    if kernel == "k_node_t":  # the step's fl_node also writes the moved coordinates
+        return E * (96 + 32) + N * (16 + 16 + 16 + 8 + 16 + 16 + 16 + 16)
     if kernel == "k_move_elem":
         return E * (32 + 96 + 24 + 32 + 32 + 96 + 24) + N * 32
     if kernel == "k_move_area":

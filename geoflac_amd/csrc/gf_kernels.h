@@ -1906,6 +1906,11 @@ GF_DEV void tri_force_v(const double *x, const double *y, double a, typename Pre
     }
 }
 
+// the velocity the node update stored
+template <typename SV>
+struct NodeVel {
+    SV v1, v2;
+};
 // the clock values the node update reads
 struct NodeClk {
     double dt, dt_elastic, time;
@@ -1936,8 +1941,9 @@ GF_DEV NodeOwn node_own(const GfDevParams *__restrict__ P, const GfDevArrays &A,
 // (j + dj, i + di) at [dj + 1][di + 1]; the areas, stresses and references of its
 // elements (j-1,i-1) (j-1,i) (j,i-1) (j,i), triangle kk[el][k] at [el][k].
 template <int M, typename Src, typename Own>
-GF_DEV void node_update(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long t, int j,
-                        int i, Src src, Own ownf, const NodeClk &kc)
+GF_DEV NodeVel<typename Prec<M>::SV> node_update(const GfDevParams *__restrict__ P, GfDevClock *ck,
+                                                 const GfDevArrays &A, long t, int j, int i, Src src, Own ownf,
+                                                 const NodeClk &kc)
 {
     typedef typename Prec<M>::SV SV;
     typedef typename Prec<M>::AT AT;
@@ -2060,6 +2066,7 @@ GF_DEV void node_update(const GfDevParams *__restrict__ P, GfDevClock *ck, const
     force[t + NN] = SV(f2);
     vel[t] = SV(v1);
     vel[t + NN] = SV(v2);
+    return NodeVel<SV>{SV(v1), SV(v2)};
 }
 // src/fl_node.f90:36-391, one thread per node: gather of the 12 triangle forces in
 // the reference's order (kk/mm tables of :7-8), gravity, optional boundary loads,
@@ -2231,9 +2238,12 @@ GF_DEV void node_stage(const NodePf<M> &f, NodeLds<M> &L)
     }
 }
 // the update of the thread's node of `tile` from the staged values
-template <int M>
+// MOVE: also the coordinate update of fl_move (src/fl_move.f90:21-28) with the new
+// velocity, into cord_out (the coordinates are read from LDS, so that no tile reads a
+// moved node)
+template <int M, bool MOVE>
 GF_DEV void node_tile_update(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, int tile,
-                             const NodeLds<M> &L, const NodeOwn &own, const NodeClk &kc)
+                             const NodeLds<M> &L, const NodeOwn &own, const NodeClk &kc, double *cord_out)
 {
     typedef typename Prec<M>::SV SV;
     int j0, i0;
@@ -2257,13 +2267,20 @@ GF_DEV void node_tile_update(const GfDevParams *__restrict__ P, GfDevClock *ck, 
         }
         p = M != 0 ? L.pr[sl] : 0.0;
     };
-    node_update<M>(P, ck, A, (long)(j - 1) + (long)NZ * (i - 1), j, i, src, [&]() { return own; }, kc);
+    const long t = (long)(j - 1) + (long)NZ * (i - 1);
+    const NodeVel<SV> nv = node_update<M>(P, ck, A, t, j, i, src, [&]() { return own; }, kc);
+    if (MOVE) {
+        const int me = (tj + 1) + ND_NJ * (ti + 1);
+        cord_out[t] = L.cx[me] + (double)nv.v1 * kc.dt;
+        cord_out[t + NN] = L.cy[me] + (double)nv.v2 * kc.dt;
+    }
 }
 // One tile per workgroup (XCD x takes the x-th contiguous eighth of the tiles, column
 // of tiles by column).  (Persistent workgroups that load the next tile into registers
 // while computing the current one ran slower: two workgroups per CU instead of three.)
-template <int M>
-__global__ void __launch_bounds__(256) k_node_t(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+template <int M, bool MOVE>
+__global__ void __launch_bounds__(256) k_node_t(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
+                                                double *cord_out)
 {
     __shared__ NodeLds<M> L;
     const int ntiles = node_ntj(P) * node_nti(P);
@@ -2273,7 +2290,7 @@ __global__ void __launch_bounds__(256) k_node_t(const GfDevParams *__restrict__ 
     const NodePf<M> pf = node_prefetch<M>(P, A, tile);
     node_stage<M>(pf, L);
     __syncthreads();
-    node_tile_update<M>(P, ck, A, tile, L, pf.own, kc);
+    node_tile_update<M, MOVE>(P, ck, A, tile, L, pf.own, kc, cord_out);
 }
 
 // ------------------------------------------------------------------ fl_move

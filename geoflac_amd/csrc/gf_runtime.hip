@@ -59,6 +59,12 @@ struct gf_ctx {
     char msg[512] = {0};
     bool rheol_quad = false;   // four lanes per element (GF_RHEOL=quad) instead of one
     bool node_direct = false;  // fl_node without the LDS tiles (GF_NODE=direct)
+    // The step's fl_node also moves the coordinates into cord_alt, which then becomes
+    // a.cord (the pointers swap), and fl_move skips its coordinate update
+    // (cord_moved).  Not while a batch holds the context's array pointers (in_batch).
+    double *cord_alt = nullptr;
+    bool cord_moved = false;
+    int in_batch = 0;
     bool rot_pending = false; // the last element move's stress/strain rotation is not applied yet
     unsigned therm_grid = 8;  // workgroups of the pipelined thermal kernel: the CUs, a multiple of 8
     // TH_MAXPASS: the thermal result is proven to be the reference's serial one (fallback
@@ -238,6 +244,7 @@ extern "C" int gf_create(const gf_params *p, int32_t precision, gf_ctx **out)
     ALLOC(ctx->dck, sizeof(GfDevClock));
     ALLOC(ctx->d_present, sizeof(int) * GF_MAXPH);
     ALLOC(ctx->a.cord, 8 * nn * 2);
+    ALLOC(ctx->cord_alt, 8 * nn * 2);
     // stress, velocity and force in the path's precision (FP32 planes when mixed)
     if (precision != GF_PREC_FP64) {
         ALLOC(ctx->a.velf, 4 * nn * 2);
@@ -301,7 +308,7 @@ extern "C" void gf_destroy(gf_ctx *ctx)
                     ctx->a.xa, ctx->a.xb, ctx->a.dens, ctx->therm_border, ctx->a.stress0f, ctx->a.velf,
                     ctx->a.forcef, ctx->a.pref, ctx->sur_samples, ctx->sur_w, ctx->sur_m, ctx->sur_v,
                     ctx->sur_grad, ctx->sur_partial, ctx->sur_cfg, ctx->sur_stats, ctx->sur_list,
-                    ctx->sur_cnt, ctx->sur_lval, ctx->sur_wgs};
+                    ctx->sur_cnt, ctx->sur_lval, ctx->sur_wgs, ctx->cord_alt};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -603,7 +610,8 @@ static int collect_timing(gf_ctx *ctx)
     return GF_OK;
 }
 
-#define LAUNCH(kern, n, ...)                                                               \
+#define LAUNCH(kern, n, ...) LAUNCH_NAMED(#kern, kern, n, __VA_ARGS__)
+#define LAUNCH_NAMED(name, kern, n, ...)                                                   \
     do {                                                                                   \
         hipEvent_t e0_ = nullptr, e1_ = nullptr;                                           \
         if (ctx->timing) {                                                                 \
@@ -612,11 +620,11 @@ static int collect_timing(gf_ctx *ctx)
             if (e0_) (void)hipEventRecord(e0_, ctx->stream);                               \
         }                                                                                  \
         hipLaunchKernelGGL(kern, dim3(nblk(n)), dim3(256), 0, ctx->stream, __VA_ARGS__);   \
-        int rc_ = launch_check(ctx, #kern);                                                \
+        int rc_ = launch_check(ctx, name);                                                 \
         if (rc_) return rc_;                                                               \
         if (ctx->timing && e0_ && e1_) {                                                   \
             (void)hipEventRecord(e1_, ctx->stream);                                        \
-            ctx->pending.push_back({#kern, {e0_, e1_}});                                   \
+            ctx->pending.push_back({name, {e0_, e1_}});                                    \
             if (ctx->pending.size() > 4096 && (rc_ = collect_timing(ctx))) return rc_;     \
         }                                                                                  \
     } while (0)
@@ -731,15 +739,31 @@ static int enq_rheol(gf_ctx *ctx)
     return GF_OK;
 }
 
-static int enq_node(gf_ctx *ctx)
+// move: the step's fl_move follows at once, so that the coordinate update may run here
+static int enq_node(gf_ctx *ctx, bool move)
 {
     int rc;
     if ((rc = flush_rot(ctx))) return rc;
     if (ctx->node_direct) {
         LAUNCH_M(k_node, ctx->nn, ctx->dp, ctx->dck, ctx->a);
-    } else {  // tiles of ND_TJ x ND_TI nodes, a multiple of 8 workgroups
-        const long ntiles = (long)((ctx->p.nz + ND_TJ - 1) / ND_TJ) * ((ctx->p.nx + ND_TI - 1) / ND_TI);
-        LAUNCH_M(k_node_t, (ntiles + 7) / 8 * 8 * 256, ctx->dp, ctx->dck, ctx->a);
+        return GF_OK;
+    }
+    // tiles of ND_TJ x ND_TI nodes, a multiple of 8 workgroups
+    const long ntiles = (long)((ctx->p.nz + ND_TJ - 1) / ND_TJ) * ((ctx->p.nx + ND_TI - 1) / ND_TI);
+    const long n = (ntiles + 7) / 8 * 8 * 256;
+    const bool fuse = move && ctx->p.movegrid != 0 && ctx->in_batch == 0;
+    const int M = ctx->precision == GF_PREC_MIXED ? 1 : ctx->precision == GF_PREC_MIXED_STORAGE ? 2 : 0;
+    double *out = fuse ? ctx->cord_alt : nullptr;
+    // (timed under one name per precision, with or without the move)
+    if (M == 1 && fuse) LAUNCH_NAMED("k_node_t<1>", (k_node_t<1, true>), n, ctx->dp, ctx->dck, ctx->a, out);
+    else if (M == 1) LAUNCH_NAMED("k_node_t<1>", (k_node_t<1, false>), n, ctx->dp, ctx->dck, ctx->a, out);
+    else if (M == 2 && fuse) LAUNCH_NAMED("k_node_t<2>", (k_node_t<2, true>), n, ctx->dp, ctx->dck, ctx->a, out);
+    else if (M == 2) LAUNCH_NAMED("k_node_t<2>", (k_node_t<2, false>), n, ctx->dp, ctx->dck, ctx->a, out);
+    else if (fuse) LAUNCH_NAMED("k_node_t<0>", (k_node_t<0, true>), n, ctx->dp, ctx->dck, ctx->a, out);
+    else LAUNCH_NAMED("k_node_t<0>", (k_node_t<0, false>), n, ctx->dp, ctx->dck, ctx->a, out);
+    if (fuse) {
+        std::swap(ctx->a.cord, ctx->cord_alt);
+        ctx->cord_moved = true;
     }
     return GF_OK;
 }
@@ -749,7 +773,8 @@ static int enq_move(gf_ctx *ctx)
     if (ctx->p.movegrid == 0) return GF_OK;
     int rc;
     if ((rc = flush_rot(ctx))) return rc;  // a second move before any fl_rheol
-    LAUNCH_SV(k_move_cord, ctx->nn, ctx->dp, ctx->dck, ctx->a);
+    if (!ctx->cord_moved) LAUNCH_SV(k_move_cord, ctx->nn, ctx->dp, ctx->dck, ctx->a);
+    ctx->cord_moved = false;
     if (ctx->rheol_quad) {
         LAUNCH_M(k_move_elem, ctx->ne, ctx->dp, ctx->dck, ctx->a);
     } else {
@@ -799,7 +824,7 @@ static int enq_flac(gf_ctx *ctx)
     if (ctx->p.itherm != 2) {
         // change_phase (every ifreq_rmasses) has no rule for the phases this build accepts
         if ((rc = enq_rheol(ctx))) return rc;
-        if ((rc = enq_node(ctx))) return rc;
+        if ((rc = enq_node(ctx, true))) return rc;
         if ((rc = enq_move(ctx))) return rc;
         if (ctx->nloop % ctx->p.ifreq_rmasses == 0 && (rc = enq_rmasses(ctx))) return rc;
         if (ctx->nloop % ctx->p.ifreq_imasses == 0 && (rc = enq_dt_mass(ctx))) return rc;
@@ -850,7 +875,7 @@ extern "C" int gf_flac(gf_ctx *ctx, gf_clock *clock)
 
 SINGLE(gf_fl_therm, enq_therm(ctx))
 SINGLE(gf_fl_rheol, enq_rheol(ctx))
-SINGLE(gf_fl_node, enq_node(ctx))
+SINGLE(gf_fl_node, enq_node(ctx, false))
 SINGLE(gf_fl_move, enq_move(ctx))
 SINGLE(gf_rmasses, enq_rmasses(ctx))
 SINGLE(gf_dt_mass, enq_dt_mass(ctx))
@@ -1272,6 +1297,7 @@ extern "C" int gf_mlp_infer(const float *x, int64_t n, const float *w, float *y)
 // ------------------------------------------------------------------ batches of models
 struct gf_batch {
     std::vector<gf_ctx *> m;
+    bool pinned = false;  // the models' in_batch counts include this batch
     int precision = GF_PREC_FP64;
     bool shearh = false, no_mech = false;
     gf_params p0{};  // frequencies and switches shared by all models
@@ -1323,6 +1349,8 @@ extern "C" void gf_batch_destroy(gf_batch *b)
     if (b->dtab) (void)hipFree(b->dtab);
     if (b->dmap) (void)hipFree(b->dmap);
     if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->pinned)
+        for (gf_ctx *c : b->m) c->in_batch--;
     delete b;
 }
 
@@ -1395,6 +1423,8 @@ extern "C" int gf_batch_create(gf_ctx *const *models, int32_t n, gf_batch **out)
         hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&b->ev[0]) != hipSuccess || hipEventCreate(&b->ev[1]) != hipSuccess)
         return fail(GF_ERR_HIP, "allocation");
+    for (gf_ctx *c : b->m) c->in_batch++;  // the table holds their array pointers
+    b->pinned = true;
     *out = b;
     return GF_OK;
 }
