@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(1024) kb_therm_tile(const BatchEntry *__restri
 // the thermal pipeline over the tiles of all models (k_therm_pipe, one workgroup per
 // CU walking a list of tiles); a tile of a model that is already proven only passes
 // the prefetch on
-template <bool XIN, bool DIS>
+template <int XIN, bool DIS>
 __global__ void __launch_bounds__(1024) kb_therm_pipe(const BatchEntry *__restrict__ tab, BatchMap map, long total,
                                                       int pass)
 {

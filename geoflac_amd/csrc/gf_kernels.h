@@ -571,8 +571,13 @@ GF_DEV void therm_tile_origin(const GfDevParams *P, int tile, int &j0, int &i0)
 
 // node q = (q % 66, q / 66) - 1 of the footprint, element e = (e % 65, e / 65) - 1 of
 // the table
-// XIN: the starting iterate is loaded (passes > 0); DIS: the shear heating is loaded
-template <bool XIN, bool DIS>
+// XIN: the starting iterate -- 0: the old temperature (pass 0), 1: loaded from the
+// previous pass's iterate (passes > 0), 2: extrapolated in time from the old
+// temperature and temp0, the temperature at the start of the previous step (pass 0;
+// temp0 of this tile's neighbours may already hold this step's value, and then the
+// extrapolation is the old temperature: only the number of sweeps changes, the fixed
+// point is the same).  DIS: the shear heating is loaded
+template <int XIN, bool DIS>
 GF_DEV ThermPf therm_prefetch(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, int pass,
                               int tile)
 {
@@ -607,7 +612,8 @@ GF_DEV ThermPf therm_prefetch(const GfDevParams *__restrict__ P, GfDevClock *ck,
             f.t[k] = gld(told_g, ix);
             f.cx[k] = gld(A.cord, ix);
             f.cy[k] = gld(A.cord, ix + nn);
-            if (XIN) f.x[k] = gld(xin, ix);
+            if (XIN == 1) f.x[k] = gld(xin, ix);
+            if (XIN == 2) f.x[k] = gld(A.temp0, ix);
         }
         const int je = j0 + q % TH_EJ - 1, ie = i0 + q / TH_EJ - 1;
         f.a0[k] = f.a1[k] = f.src[k] = f.sh[k] = 0.0;
@@ -623,7 +629,7 @@ GF_DEV ThermPf therm_prefetch(const GfDevParams *__restrict__ P, GfDevClock *ck,
     }
     return f;
 }
-template <bool XIN, bool DIS>
+template <int XIN, bool DIS>
 GF_DEV void therm_store_raw(const ThermPf &f, ThermRaw &R)
 {
 #pragma unroll
@@ -633,7 +639,7 @@ GF_DEV void therm_store_raw(const ThermPf &f, ThermRaw &R)
             R.t[q] = f.t[k];
             R.cx[q] = f.cx[k];
             R.cy[q] = f.cy[k];
-            if (XIN) R.x[q] = f.x[k];
+            if (XIN != 0) R.x[q] = f.x[k];
         }
         if (q < TH_NE) {
             R.a0[q] = f.a0[k];
@@ -656,7 +662,7 @@ GF_DEV void therm_store_raw(const ThermPf &f, ThermRaw &R)
 // after the prefetch is issued reads global memory (a wait for it would also wait
 // for the prefetch): the mesh size is held in registers, the phase table, dt and the
 // activity flag came with the tile's inputs.
-template <bool XIN, bool DIS>
+template <int XIN, bool DIS>
 GF_DEV void therm_tile_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, int pass,
                             int tile, const GfDevParams *__restrict__ Pn, GfDevClock *ckn, const GfDevArrays &An,
                             int next, ThermRaw &R, double *sU, int *sFlag)
@@ -782,11 +788,11 @@ GF_DEV void therm_tile_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, c
     c.told = valid ? R.t[nb] : 0.0;
     // starting iterate: the old temperature in pass 0, the previous pass's result later
     {
-        const double v0 = XIN ? R.x[e0] : R.t[e0];
+        const double v0 = XIN == 1 ? R.x[e0] : XIN == 2 ? 2.0 * R.t[e0] - R.x[e0] : R.t[e0];
         sX[0][e0] = v0;
         sX[1][e0] = v0;
         if (e1 < NL) {
-            const double v1 = XIN ? R.x[e1] : R.t[e1];
+            const double v1 = XIN == 1 ? R.x[e1] : XIN == 2 ? 2.0 * R.t[e1] - R.x[e1] : R.t[e1];
             sX[0][e1] = v1;
             sX[1][e1] = v1;
         }
@@ -843,7 +849,7 @@ GF_DEV void therm_tile_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, c
 // Workgroup w (of a grid that is a multiple of 8, one per CU) runs on XCD w % 8, which
 // owns the contiguous tile range [x q, (x+1) q); its r-th workgroup there takes tiles
 // x q + r, + R, + 2R, ... with R = grid / 8.
-template <bool XIN, bool DIS>
+template <int XIN, bool DIS>
 __global__ void __launch_bounds__(1024) k_therm_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
                                                      int pass)
 {

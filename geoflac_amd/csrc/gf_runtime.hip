@@ -64,6 +64,10 @@ struct gf_ctx {
     // (cord_moved).  Not while a batch holds the context's array pointers (in_batch).
     double *cord_alt = nullptr;
     bool cord_moved = false;
+    // temp0 holds the temperature at the start of the previous thermal step on the
+    // device (the thermal tiles then start from the extrapolation 2 temp - temp0)
+    bool temp0_prev = false;
+    bool therm_noguess = false;  // GF_THERM_GUESS=0: pass 0 starts from the old temperature
     int in_batch = 0;
     bool rot_pending = false; // the last element move's stress/strain rotation is not applied yet
     unsigned therm_grid = 8;  // workgroups of the pipelined thermal kernel: the CUs, a multiple of 8
@@ -215,6 +219,8 @@ extern "C" int gf_create(const gf_params *p, int32_t precision, gf_ctx **out)
         ctx->rheol_quad = rv && strcmp(rv, "quad") == 0;
         const char *nv = getenv("GF_NODE");
         ctx->node_direct = nv && strcmp(nv, "direct") == 0;
+        const char *gv = getenv("GF_THERM_GUESS");
+        ctx->therm_noguess = gv && strcmp(gv, "0") == 0;
     }
     const long ne = ctx->ne, nn = ctx->nn;
     int rc = GF_OK;
@@ -443,6 +449,7 @@ extern "C" int gf_upload(gf_ctx *ctx, uint32_t mask)
     if (!ctx->bound) return set_err(ctx, GF_ERR_STATE, "gf_upload before gf_bind_host");
     int frc = flush_rot(ctx);  // the upload overwrites the moved state
     if (frc) return frc;
+    ctx->temp0_prev = false;  // temp and temp0 may no longer be consecutive steps
     HIPCHK(hipStreamSynchronize(ctx->stream));
     for (const FieldDesc &f : kFields) {
         if (!(mask & f.bit)) continue;
@@ -679,8 +686,12 @@ static int enq_therm(gf_ctx *ctx)
             if (e0) (void)hipEventRecord(e0, ctx->stream);
         }
         const bool dis = ctx->p.ishearh == 1 && ctx->p.itherm != 2;
-        auto kern = pass == 0 ? (dis ? k_therm_pipe<false, true> : k_therm_pipe<false, false>)
-                              : (dis ? k_therm_pipe<true, true> : k_therm_pipe<true, false>);
+        // pass 0 starts from the extrapolation 2 temp - temp0 once temp0 holds the
+        // previous step's starting temperature
+        const bool ex = ctx->temp0_prev && !ctx->therm_noguess;
+        auto kern = pass > 0 ? (dis ? k_therm_pipe<1, true> : k_therm_pipe<1, false>)
+                    : ex     ? (dis ? k_therm_pipe<2, true> : k_therm_pipe<2, false>)
+                             : (dis ? k_therm_pipe<0, true> : k_therm_pipe<0, false>);
         hipLaunchKernelGGL(kern, grid, dim3(TH_J * TH_I), 0, ctx->stream, ctx->dp, ctx->dck, ctx->a, pass);
         if ((rc = launch_check(ctx, "k_therm_pipe"))) return rc;
         if (ctx->timing && e0 && e1) {
@@ -690,6 +701,7 @@ static int enq_therm(gf_ctx *ctx)
         if (npass > 1) LAUNCH(k_therm_check, (long)ntiles * TH_NB, ctx->dp, ctx->dck, ctx->a, pass);
     }
     LAUNCH(k_therm_bc, ctx->nn, ctx->dp, ctx->dck, ctx->a, npass);
+    ctx->temp0_prev = true;  // pass 0 wrote temp0 at every node
     return GF_OK;
 }
 
@@ -1467,7 +1479,7 @@ static int enq_batch_flac(gf_batch *b)
         // pass 0 through the pipeline; a fallback pass, which only some models need,
         // with one workgroup per tile that returns at once for a proven model
         if (pass == 0)
-            BLAUNCH((dis ? kb_therm_pipe<false, true> : kb_therm_pipe<false, false>), b->therm_grid, TH_J * TH_I,
+            BLAUNCH((dis ? kb_therm_pipe<0, true> : kb_therm_pipe<0, false>), b->therm_grid, TH_J * TH_I,
                     b->dtab, map, b->tot_t, pass);
         else
             BLAUNCH(kb_therm_tile, (unsigned)std::min<long>(b->tot_t, 512), TH_J * TH_I, b->dtab, map, b->tot_t, pass);
