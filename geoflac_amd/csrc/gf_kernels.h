@@ -76,16 +76,18 @@ __global__ void k_copy(double *__restrict__ dst, const double *__restrict__ src,
     if (t < n) dst[t] = src[t];
 }
 
-enum { CK_DTAVG_ADD = 1, CK_DTAVG_RESET = 2, CK_TIME_ADVANCE = 3, CK_THERM_BEGIN = 4, CK_DT_BEGIN = 5 };
+// clock updates, a bit each; several in one launch apply in this order (each writes
+// its own fields and reads only dt, which none of them changes)
+enum { CK_TIME_ADVANCE = 1, CK_THERM_BEGIN = 2, CK_DTAVG_ADD = 4, CK_DTAVG_RESET = 8, CK_DT_BEGIN = 16 };
 
 GF_DEV void clock_op(GfDevClock *ck, int op)
 {
-    if (op == CK_DTAVG_ADD) ck->dtavg = ck->dtavg + ck->dt;
-    if (op == CK_DTAVG_RESET) ck->dtavg = 0.0;
-    if (op == CK_TIME_ADVANCE) ck->time = ck->time + ck->dt;
-    if (op == CK_THERM_BEGIN)
+    if (op & CK_TIME_ADVANCE) ck->time = ck->time + ck->dt;
+    if (op & CK_THERM_BEGIN)
         for (int k = 0; k < GF_MAX_SWEEPS; k++) ck->therm_pass_changed[k] = 0u;
-    if (op == CK_DT_BEGIN) {
+    if (op & CK_DTAVG_ADD) ck->dtavg = ck->dtavg + ck->dt;
+    if (op & CK_DTAVG_RESET) ck->dtavg = 0.0;
+    if (op & CK_DT_BEGIN) {
         ck->dlmin_bits = (unsigned long long)__double_as_longlong(1.0e28);
         ck->velmax_bits = 0ull;
     }

@@ -671,10 +671,11 @@ static int flush_rot(gf_ctx *ctx)
     return GF_OK;
 }
 
-static int enq_therm(gf_ctx *ctx)
+// clock_done: the step launched the thermal step's clock update already
+static int enq_therm(gf_ctx *ctx, bool clock_done = false)
 {
     int rc;
-    if ((rc = enq_clock(ctx, CK_THERM_BEGIN))) return rc;
+    if (!clock_done && (rc = enq_clock(ctx, CK_THERM_BEGIN))) return rc;
     const int ntiles = ((ctx->p.nz + TH_OJ - 1) / TH_OJ) * ((ctx->p.nx + TH_OI - 1) / TH_OI);
     const int npass = ctx->therm_passes;
     for (int pass = 0; pass < npass; pass++) {
@@ -705,7 +706,7 @@ static int enq_therm(gf_ctx *ctx)
     return GF_OK;
 }
 
-static int enq_rheol(gf_ctx *ctx)
+static int enq_rheol(gf_ctx *ctx, bool clock_done = false)
 {
     int rc;
     const int do_visc = (ctx->nloop % ctx->p.ifreq_visc) == 0;
@@ -715,7 +716,7 @@ static int enq_rheol(gf_ctx *ctx)
     else if (ctx->nsrate == -1)
         avgsr = 2;
     const bool mixed = ctx->precision == GF_PREC_MIXED, mstore = ctx->precision == GF_PREC_MIXED_STORAGE;
-    if ((rc = enq_clock(ctx, CK_DTAVG_ADD))) return rc;
+    if (!clock_done && (rc = enq_clock(ctx, CK_DTAVG_ADD))) return rc;
     // the viscosity update reads the rotated step-start stress
     if ((do_visc || ctx->rheol_quad) && (rc = flush_rot(ctx))) return rc;
     const int rot = ctx->rot_pending ? 1 : 0;
@@ -828,14 +829,18 @@ static int check_ready(gf_ctx *ctx)
     return GF_OK;
 }
 
-// one call of subroutine flac, src/flac.f90:18-83
-static int enq_flac(gf_ctx *ctx)
+// one call of subroutine flac, src/flac.f90:18-83; pending: clock updates left by the
+// previous step (its time advance), launched together with this step's own
+static int enq_flac(gf_ctx *ctx, int pending = 0)
 {
     int rc;
-    if ((rc = enq_therm(ctx))) return rc;
-    if (ctx->p.itherm != 2) {
+    const bool mech = ctx->p.itherm != 2;
+    // fl_therm's and fl_rheol's clock updates in one launch (neither touches dt)
+    if ((rc = enq_clock(ctx, pending | CK_THERM_BEGIN | (mech ? CK_DTAVG_ADD : 0)))) return rc;
+    if ((rc = enq_therm(ctx, true))) return rc;
+    if (mech) {
         // change_phase (every ifreq_rmasses) has no rule for the phases this build accepts
-        if ((rc = enq_rheol(ctx))) return rc;
+        if ((rc = enq_rheol(ctx, true))) return rc;
         if ((rc = enq_node(ctx, true))) return rc;
         if ((rc = enq_move(ctx))) return rc;
         if (ctx->nloop % ctx->p.ifreq_rmasses == 0 && (rc = enq_rmasses(ctx))) return rc;
@@ -850,10 +855,11 @@ extern "C" int gf_step(gf_ctx *ctx, int32_t nsteps, gf_clock *clock)
     int rc = check_ready(ctx);
     if (rc) return rc;
     for (int32_t n = 0; n < nsteps; n++) {
-        if ((rc = enq_flac(ctx))) return rc;
+        // the previous step's time advance goes with this step's first clock update
+        if ((rc = enq_flac(ctx, n > 0 ? CK_TIME_ADVANCE : 0))) return rc;
         ctx->nloop = ctx->nloop + 1;
-        if ((rc = enq_clock(ctx, CK_TIME_ADVANCE))) return rc;
     }
+    if (nsteps > 0 && (rc = enq_clock(ctx, CK_TIME_ADVANCE))) return rc;
     return fetch_clock(ctx, clock);
 }
 
@@ -1472,7 +1478,8 @@ static int enq_batch_flac(gf_batch *b)
     const int nm = b->nm;
     const BatchMap map = b->map;
     const unsigned gm = bgrid(nm, 64);
-    BLAUNCH(kb_clock, gm, 64, b->dtab, map, (int)CK_THERM_BEGIN);
+    // fl_therm's and fl_rheol's clock updates in one launch
+    BLAUNCH(kb_clock, gm, 64, b->dtab, map, (int)(CK_THERM_BEGIN | (b->no_mech ? 0 : CK_DTAVG_ADD)));
     const bool dis = b->p0.ishearh == 1 && b->p0.itherm != 2;
     const int npass = b->m[0]->therm_passes;
     for (int pass = 0; pass < npass; pass++) {
@@ -1494,7 +1501,6 @@ static int enq_batch_flac(gf_batch *b)
             avgsr = 1;
         else if (b->nsrate == -1)
             avgsr = 2;
-        BLAUNCH(kb_clock, gm, 64, b->dtab, map, (int)CK_DTAVG_ADD);
         if (do_visc) BLAUNCH(kb_visc<SV>, bgrid(b->tot_e, 256), 256, b->dtab, map);
         BTIMED(0, BLAUNCH((kb_rheol<SH, M>), bgrid(b->tot_e, b->bs[0]), b->bs[0], b->dtab, map, avgsr));
         if (avgsr == 1) {
