@@ -184,8 +184,22 @@ struct ThermCoef {
     double c51, k52, c53;        // S5 = (T(j,i) k51 + X(j+1,i-1) k52) + T(j+1,i) k53
     double s6;                   // S6 = only old values
     double area_n, told;
+    double rarea;                // 1 / area_n, rounded (for div_fma)
     int mask;                    // bit t-1 set when triangle t exists
 };
+
+// x / d, correctly rounded, from rd = 1 / d correctly rounded: q0 = x rd is within
+// one ulp of x / d, the remainder x - q0 d is exact with an FMA, and q0 + (x - q0 d) rd
+// rounds to the correctly rounded quotient (Markstein's theorem, P. W. Markstein,
+// IBM J. Res. Develop. 34(1), 1990; valid without overflow or underflow).  Three
+// FP64 operations in place of a division, where the divisor repeats.
+GF_DEV double div_fma(double x, double d, double rd)
+{
+    const double q0 = x * rd;
+    const double r = __builtin_fma(-q0, d, x);
+    return __builtin_fma(r, rd, q0);
+}
+constexpr double GF_RSIX = 1.0 / 6.0;  // correctly rounded
 
 GF_DEV double therm_dummye(const GfDevParams *P, const GfDevArrays &A, int je, int ie)
 {
@@ -226,7 +240,7 @@ GF_DEV void therm_coef(const GfDevParams *P, const GfDevArrays &A, int j, int i,
     auto tri_common = [&](int je, int ie, int tri, int t) {
         ar = AREA(je, ie, tri + 1);
         inv_area = 1.0 / ar;
-        ra13 = inv_area / 6.0;
+        ra13 = div_fma(inv_area, 6.0, GF_RSIX);
         c.area_n = c.area_n + ra13;
         df = PHASE(A.iphase[IE(je, ie)]).diff;
         dm = therm_dummye(P, A, je, ie);
@@ -257,6 +271,7 @@ GF_DEV void therm_coef(const GfDevParams *P, const GfDevArrays &A, int j, int i,
         tri_common(j, i, 0, 5);
         c.s6 = T(j, i) * KAB(d, 1, 1) + T(j + 1, i) * KAB(d, 2, 1) + T(j, i + 1) * KAB(d, 3, 1);
     }
+    c.rarea = 1.0 / c.area_n;
 }
 
 // Element table of the tile kernel: elements (ej, ei) of tile rows -1 .. TH_J-1 and
@@ -276,7 +291,7 @@ GF_DEV double therm_eval(const ThermCoef &c, double xu, double xl, double xd, do
     if (c.mask & 8) rhs = rhs + c.P[3] - c.Q[3] * ((xl * c.k41 + xd * c.k42) + c.c43);
     if (c.mask & 16) rhs = rhs + c.P[4] - c.Q[4] * ((c.c51 + xd * c.k52) + c.c53);
     if (c.mask & 32) rhs = rhs + c.P[5] - c.Q[5] * c.s6;
-    return c.told + rhs * dt / c.area_n;
+    return c.told + div_fma(rhs * dt, c.area_n, c.rarea);
 }
 
 // pass == 0: old temperature from A.temp (also copied to temp0 = src/fl_therm.f90:52);
@@ -371,7 +386,7 @@ GF_DEV void therm_tile_one(const GfDevParams *__restrict__ P, GfDevClock *ck, co
         if (round == 0) {
             for (int tri = 0; tri < 2; tri++) {
                 const double inv_area = 1.0 / sAr[tri][e];
-                const double ra13 = inv_area / 6.0;
+                const double ra13 = div_fma(inv_area, 6.0, GF_RSIX);
                 sU[(3 * tri + 0) * TH_NE + e] = ra13;
                 sU[(3 * tri + 1) * TH_NE + e] = sDm[e] * ra13;
                 sU[(3 * tri + 2) * TH_NE + e] = 0.5 * sDf[e] * inv_area;
@@ -428,6 +443,7 @@ GF_DEV void therm_tile_one(const GfDevParams *__restrict__ P, GfDevClock *ck, co
                     c.P[t] = F(t, 3 * tri + 1);
                     c.Q[t] = F(t, 3 * tri + 2);
                 }
+                c.rarea = 1.0 / c.area_n;
             } else if (round == 1) {  // triangles A: K11 K12 K13 K22 K23 K33 at 0..5
                 if (c.mask & 2) {  // element (j-1,i), node is vertex 2
                     c.k21 = F(1, 1); c.c22 = tme * F(1, 3); c.c23 = T(j - 1, i + 1) * F(1, 4);
@@ -703,7 +719,7 @@ GF_DEV void therm_tile_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, c
             const double df = sPh[2][ph - 1];
             for (int tri = 0; tri < 2; tri++) {
                 const double inv_area = 1.0 / (tri == 0 ? R.a0[e] : R.a1[e]);
-                const double ra13 = inv_area / 6.0;
+                const double ra13 = div_fma(inv_area, 6.0, GF_RSIX);
                 sU[(3 * tri + 0) * TH_NE + e] = ra13;
                 sU[(3 * tri + 1) * TH_NE + e] = dm * ra13;
                 sU[(3 * tri + 2) * TH_NE + e] = 0.5 * df * inv_area;
@@ -767,6 +783,7 @@ GF_DEV void therm_tile_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, c
                     c.P[t] = F(t, 3 * tri + 1);
                     c.Q[t] = F(t, 3 * tri + 2);
                 }
+                c.rarea = 1.0 / c.area_n;
             } else if (round == 1) {  // triangles A: K11 K12 K13 K22 K23 K33 at 0..5
                 if (c.mask & 2) {
                     c.k21 = F(1, 1); c.c22 = tme * F(1, 3); c.c23 = T(-1, 1) * F(1, 4);
