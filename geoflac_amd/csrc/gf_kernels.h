@@ -1814,21 +1814,16 @@ GF_DEV void stress_avg(const GfDevParams *P, const GfDevArrays &A, long e, doubl
 
 // the inputs of element e's viscosity, every load issued before the first use (one
 // memory round trip per element)
-struct ViscLoad {
-    int iph;
-    double prv[GF_MAXACT], a11, a22, a33;  // phase ratios, stress_avg
-    ViscIn vin;
-};
 template <typename SV>
-GF_DEV ViscLoad visc_load(const GfDevParams *__restrict__ P, const GfDevArrays &A, long e, int j, int i)
+GF_DEV void visc_load(const GfDevParams *__restrict__ P, const GfDevArrays &A, long e, int j, int i, int &iph,
+                      double (&prv)[GF_MAXACT], double &a11, double &a22, double &a33, ViscIn &vin)
 {
-    ViscLoad v;
     const unsigned ne = (unsigned)NE, eo = (unsigned)e;
-    v.iph = A.iphase[e];
+    iph = A.iphase[e];
     // phase ratios past nactive lie outside the buffer and read 0
     const Buf bPR = mkbuf(A.pr, (unsigned long long)P->nactive * ne * 8ull);
 #pragma unroll
-    for (int a = 0; a < GF_MAXACT; a++) v.prv[a] = bld(bPR, eo * 8u, (unsigned)a * ne * 8u);
+    for (int a = 0; a < GF_MAXACT; a++) prv[a] = bld(bPR, eo * 8u, (unsigned)a * ne * 8u);
     const Buf bS = mkbuf(sv_ptr<SV>(A.stress0, A.stress0f), 16ull * sizeof(SV) * ne);
     SV sv[3][4];  // components 1, 2, 4 of triangles 1..4
     constexpr int comp[3] = {0, 1, 3};
@@ -1837,17 +1832,16 @@ GF_DEV ViscLoad visc_load(const GfDevParams *__restrict__ P, const GfDevArrays &
 #pragma unroll
         for (int k = 0; k < 4; k++) sv[c][k] = bldt<SV>(bS, eo * (unsigned)sizeof(SV), (comp[c] + 4 * k) * ne * (unsigned)sizeof(SV));
     const double pr = ref_press<SV>(A, e);
-    v.vin = visc_inputs(P, A, j, i);
+    vin = visc_inputs(P, A, j, i);
     // stress_avg on the loaded values
-    v.a11 = 0.25 * ((double)sv[0][0] + (double)sv[0][1] + (double)sv[0][2] + (double)sv[0][3]);
-    v.a22 = 0.25 * ((double)sv[1][0] + (double)sv[1][1] + (double)sv[1][2] + (double)sv[1][3]);
-    v.a33 = 0.25 * ((double)sv[2][0] + (double)sv[2][1] + (double)sv[2][2] + (double)sv[2][3]);
+    a11 = 0.25 * ((double)sv[0][0] + (double)sv[0][1] + (double)sv[0][2] + (double)sv[0][3]);
+    a22 = 0.25 * ((double)sv[1][0] + (double)sv[1][1] + (double)sv[1][2] + (double)sv[1][3]);
+    a33 = 0.25 * ((double)sv[2][0] + (double)sv[2][1] + (double)sv[2][2] + (double)sv[2][3]);
     if (sizeof(SV) == 4) {
-        v.a11 = v.a11 - pr;
-        v.a22 = v.a22 - pr;
-        v.a33 = v.a33 - pr;
+        a11 = a11 - pr;
+        a22 = a22 - pr;
+        a33 = a33 - pr;
     }
-    return v;
 }
 GF_DEV bool visc_rheology(int irh) { return irh == 3 || irh >= 11; }  // the Maxwell rheologies
 
@@ -1855,9 +1849,12 @@ template <typename SV>
 GF_DEV void visc_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e)
 {
     const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
-    const ViscLoad v = visc_load<SV>(P, A, e, j, i);
-    if (!visc_rheology(PHASE(v.iph).irheol)) return;
-    A.visn[e] = dev_eff_visc_in(P, j, i, v.a11, v.a22, v.a33, v.prv, v.vin);
+    int iph;
+    double prv[GF_MAXACT], a11, a22, a33;
+    ViscIn vin;
+    visc_load<SV>(P, A, e, j, i, iph, prv, a11, a22, a33, vin);
+    if (!visc_rheology(PHASE(iph).irheol)) return;
+    A.visn[e] = dev_eff_visc_in(P, j, i, a11, a22, a33, prv, vin);
 }
 template <typename SV>
 __global__ void __launch_bounds__(256) k_visc(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)

@@ -477,12 +477,14 @@ __global__ void __launch_bounds__(256, 4) k_visc_mlp(const GfDevParams *__restri
         const long e = blk * 256 + threadIdx.x;
         const bool in = e < ne;
         const int j = (int)(e % nzm) + 1, i = (int)(e / nzm) + 1;
-        ViscLoad v{};
-        if (in) v = visc_load<SV>(P, A, e, j, i);
-        const int irh = in ? PHASE(v.iph).irheol : 0;
+        int iph = 1;
+        double prv[GF_MAXACT] = {}, a11 = 0.0, a22 = 0.0, a33 = 0.0;
+        ViscIn vin{};
+        if (in) visc_load<SV>(P, A, e, j, i, iph, prv, a11, a22, a33, vin);
+        const int irh = in ? PHASE(iph).irheol : 0;
         const bool visc = in && visc_rheology(irh);
         float x[MLP_NIN];
-        const bool use = visc && !off && visc_features_in(P, j, i, v.a11, v.a22, v.a33, v.prv, v.vin, x) &&
+        const bool use = visc && !off && visc_features_in(P, j, i, a11, a22, a33, prv, vin, x) &&
                          mlp_in_distribution(cfg, st, x);
         unsigned short xb[8];
         for (int f = 0; f < 8; f++)
@@ -522,18 +524,18 @@ __global__ void __launch_bounds__(256, 4) k_visc_mlp(const GfDevParams *__restri
 }
 
 template <typename SV>
-__global__ void __launch_bounds__(256, 4) k_visc_list(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
+__global__ void __launch_bounds__(256) k_visc_list(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
                                                    const MlpCfg *__restrict__ cfg, MlpStats *st,
                                                    const unsigned int *__restrict__ list,
                                                    const float *__restrict__ lval,
                                                    const unsigned int *__restrict__ cnt,
-                                                   MlpWgStats *__restrict__ wgs, long cap)
+                                                   MlpWgStats *__restrict__ wgs, long cap, int nseg)
 {
     __shared__ unsigned int s_aud, s_fail, s_max;
     __shared__ unsigned long long s_worst;
     __shared__ unsigned int s_pre[MLP_MAXG + 1];  // exclusive prefix sums of the segment lengths
     __shared__ unsigned int s_wsum[4];
-    const unsigned int G = gridDim.x;
+    const unsigned int G = (unsigned int)nseg;  // the segments: the network pass's workgroups
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
     if (threadIdx.x == 0) {
         s_aud = 0u;
@@ -569,7 +571,7 @@ __global__ void __launch_bounds__(256, 4) k_visc_list(const GfDevParams *__restr
     }
     const unsigned int n = s_pre[MLP_MAXG];
     const int nzm = NZ - 1;
-    for (unsigned int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += G * blockDim.x) {
+    for (unsigned int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         // the segment of flattened entry k: the last g with s_pre[g] <= k
         unsigned int lo = 0, hi = G - 1;
         while (lo < hi) {
@@ -581,8 +583,11 @@ __global__ void __launch_bounds__(256, 4) k_visc_list(const GfDevParams *__restr
         const unsigned int item = list[at];
         const long e = (long)(item & ~MLP_LIST_AUDIT);
         const int j = (int)(e % nzm) + 1, i = (int)(e / nzm) + 1;
-        const ViscLoad v = visc_load<SV>(P, A, e, j, i);
-        const double ex = dev_eff_visc_in(P, j, i, v.a11, v.a22, v.a33, v.prv, v.vin);
+        int iph;
+        double prv[GF_MAXACT], a11, a22, a33;
+        ViscIn vin;
+        visc_load<SV>(P, A, e, j, i, iph, prv, a11, a22, a33, vin);
+        const double ex = dev_eff_visc_in(P, j, i, a11, a22, a33, prv, vin);
         A.visn[e] = ex;
         if (item & MLP_LIST_AUDIT) {
             const float err = (float)fabs(log10(mlp_visc(P, lval[at])) - log10(ex));
@@ -591,7 +596,7 @@ __global__ void __launch_bounds__(256, 4) k_visc_list(const GfDevParams *__restr
             atomicAdd(&s_aud, 1u);
             if (err > cfg->tol) {  // take the element's cell off
                 float x[MLP_NIN];
-                visc_features_in(P, j, i, v.a11, v.a22, v.a33, v.prv, v.vin, x);
+                visc_features_in(P, j, i, a11, a22, a33, prv, vin, x);
                 int r, c;
                 mlp_cell(cfg, x, r, c);
                 atomicOr(&st->bad[r], 1u << c);

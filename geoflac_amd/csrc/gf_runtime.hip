@@ -726,8 +726,10 @@ static int enq_rheol(gf_ctx *ctx, bool clock_done = false)
         const long pers = (long)ctx->sur_grid * 256;
         LAUNCH_SV(k_visc_mlp, pers, ctx->dp, ctx->dck, ctx->a, ctx->sur_w, ctx->sur_cfg, ctx->sur_stats,
                   ctx->sur_list, ctx->sur_lval, ctx->sur_cnt, ctx->sur_wgs, ctx->sur_seg);
-        LAUNCH_SV(k_visc_list, pers, ctx->dp, ctx->dck, ctx->a, ctx->sur_cfg, ctx->sur_stats, ctx->sur_list,
-                  ctx->sur_lval, ctx->sur_cnt, ctx->sur_wgs, ctx->sur_seg);
+        // the exact formula on the entries: three workgroups per CU (its register use)
+        LAUNCH_SV(k_visc_list, std::min<long>(pers, (long)ctx->therm_grid * 3 * 256), ctx->dp, ctx->dck, ctx->a,
+                  ctx->sur_cfg, ctx->sur_stats, ctx->sur_list, ctx->sur_lval, ctx->sur_cnt, ctx->sur_wgs,
+                  ctx->sur_seg, ctx->sur_grid);
     } else if (do_visc)
         LAUNCH_SV(k_visc, ctx->ne, ctx->dp, ctx->dck, ctx->a);
     if (ctx->rheol_quad && !mixed && !mstore)
