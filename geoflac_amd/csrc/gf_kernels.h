@@ -179,10 +179,10 @@ struct ThermCoef {
     double P[6], Q[6];
     double k11, k12, c13;        // S1 = (X(j-1,i) k11 + X(j,i-1) k12) + T(j,i) k13
     double k21, c22, c23;        // S2 = (X(j-1,i) k21 + T(j,i) k22) + T(j-1,i+1) k23
-    double s3;                   // S3 = only old values
+    double qs3;                  // Q3 S3, S3 = only old values
     double k41, k42, c43;        // S4 = (X(j,i-1) k41 + X(j+1,i-1) k42) + T(j,i) k43
     double c51, k52, c53;        // S5 = (T(j,i) k51 + X(j+1,i-1) k52) + T(j+1,i) k53
-    double s6;                   // S6 = only old values
+    double qs6;                  // Q6 S6, S6 = only old values
     double area_n, told;
     double rarea;                // 1 / area_n, rounded (for div_fma)
     int mask;                    // bit t-1 set when triangle t exists
@@ -258,7 +258,7 @@ GF_DEV void therm_coef(const GfDevParams *P, const GfDevArrays &A, int j, int i,
         c.k21 = KAB(d, 1, 2); c.c22 = T(j, i) * KAB(d, 2, 2); c.c23 = T(j - 1, i + 1) * KAB(d, 3, 2);
         // triangle B: (j-1,i+1), (j,i), (j,i+1) -- old values only
         tri_common(j - 1, i, 1, 2);
-        c.s3 = T(j - 1, i + 1) * KAB(d, 1, 2) + T(j, i) * KAB(d, 2, 2) + T(j, i + 1) * KAB(d, 3, 2);
+        c.qs3 = c.Q[2] * (T(j - 1, i + 1) * KAB(d, 1, 2) + T(j, i) * KAB(d, 2, 2) + T(j, i + 1) * KAB(d, 3, 2));
     }
     if (j != NZ && i != 1) {  // element (j,i-1), triangle A: (j,i-1), (j+1,i-1), (j,i)
         tri_common(j, i - 1, 0, 3);
@@ -269,7 +269,7 @@ GF_DEV void therm_coef(const GfDevParams *P, const GfDevArrays &A, int j, int i,
     }
     if (j != NZ && i != NX) {  // element (j,i), triangle A: (j,i), (j+1,i), (j,i+1) -- old values only
         tri_common(j, i, 0, 5);
-        c.s6 = T(j, i) * KAB(d, 1, 1) + T(j + 1, i) * KAB(d, 2, 1) + T(j, i + 1) * KAB(d, 3, 1);
+        c.qs6 = c.Q[5] * (T(j, i) * KAB(d, 1, 1) + T(j + 1, i) * KAB(d, 2, 1) + T(j, i + 1) * KAB(d, 3, 1));
     }
     c.rarea = 1.0 / c.area_n;
 }
@@ -287,10 +287,10 @@ GF_DEV double therm_eval(const ThermCoef &c, double xu, double xl, double xd, do
     double rhs = 0.0;
     if (c.mask & 1) rhs = rhs + c.P[0] - c.Q[0] * ((xu * c.k11 + xl * c.k12) + c.c13);
     if (c.mask & 2) rhs = rhs + c.P[1] - c.Q[1] * ((xu * c.k21 + c.c22) + c.c23);
-    if (c.mask & 4) rhs = rhs + c.P[2] - c.Q[2] * c.s3;
+    if (c.mask & 4) rhs = rhs + c.P[2] - c.qs3;
     if (c.mask & 8) rhs = rhs + c.P[3] - c.Q[3] * ((xl * c.k41 + xd * c.k42) + c.c43);
     if (c.mask & 16) rhs = rhs + c.P[4] - c.Q[4] * ((c.c51 + xd * c.k52) + c.c53);
-    if (c.mask & 32) rhs = rhs + c.P[5] - c.Q[5] * c.s6;
+    if (c.mask & 32) rhs = rhs + c.P[5] - c.qs6;
     return c.told + div_fma(rhs * dt, c.area_n, c.rarea);
 }
 
@@ -452,13 +452,13 @@ GF_DEV void therm_tile_one(const GfDevParams *__restrict__ P, GfDevClock *ck, co
                     c.k41 = F(3, 2); c.k42 = F(3, 4); c.c43 = tme * F(3, 5);
                 }
                 if (c.mask & 32)  // element (j,i), node is vertex 1
-                    c.s6 = tme * F(5, 0) + T(j + 1, i) * F(5, 1) + T(j, i + 1) * F(5, 2);
+                    c.qs6 = c.Q[5] * (tme * F(5, 0) + T(j + 1, i) * F(5, 1) + T(j, i + 1) * F(5, 2));
             } else {  // triangles B
                 if (c.mask & 1) {  // element (j-1,i-1), node is vertex 3
                     c.k11 = F(0, 2); c.k12 = F(0, 4); c.c13 = tme * F(0, 5);
                 }
                 if (c.mask & 4)  // element (j-1,i), node is vertex 2
-                    c.s3 = T(j - 1, i + 1) * F(2, 1) + tme * F(2, 3) + T(j, i + 1) * F(2, 4);
+                    c.qs3 = c.Q[2] * (T(j - 1, i + 1) * F(2, 1) + tme * F(2, 3) + T(j, i + 1) * F(2, 4));
                 if (c.mask & 16) {  // element (j,i-1), node is vertex 1
                     c.c51 = tme * F(4, 0); c.k52 = F(4, 1); c.c53 = T(j + 1, i) * F(4, 2);
                 }
@@ -791,12 +791,12 @@ GF_DEV void therm_tile_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, c
                 if (c.mask & 8) {
                     c.k41 = F(3, 2); c.k42 = F(3, 4); c.c43 = tme * F(3, 5);
                 }
-                if (c.mask & 32) c.s6 = tme * F(5, 0) + T(1, 0) * F(5, 1) + T(0, 1) * F(5, 2);
+                if (c.mask & 32) c.qs6 = c.Q[5] * (tme * F(5, 0) + T(1, 0) * F(5, 1) + T(0, 1) * F(5, 2));
             } else {  // triangles B
                 if (c.mask & 1) {
                     c.k11 = F(0, 2); c.k12 = F(0, 4); c.c13 = tme * F(0, 5);
                 }
-                if (c.mask & 4) c.s3 = T(-1, 1) * F(2, 1) + tme * F(2, 3) + T(0, 1) * F(2, 4);
+                if (c.mask & 4) c.qs3 = c.Q[2] * (T(-1, 1) * F(2, 1) + tme * F(2, 3) + T(0, 1) * F(2, 4));
                 if (c.mask & 16) {
                     c.c51 = tme * F(4, 0); c.k52 = F(4, 1); c.c53 = T(1, 0) * F(4, 2);
                 }
