@@ -75,7 +75,7 @@ __global__ void __launch_bounds__(1024) kb_therm_tile(const BatchEntry *__restri
 // the prefetch on
 template <int XIN, bool DIS>
 __global__ void __launch_bounds__(1024) kb_therm_pipe(const BatchEntry *__restrict__ tab, BatchMap map, long total,
-                                                      int pass)
+                                                      int pass, double stol)
 {
     __shared__ ThermRaw raw;
     __shared__ double sU[TH_NF * TH_NE];
@@ -94,7 +94,7 @@ __global__ void __launch_bounds__(1024) kb_therm_pipe(const BatchEntry *__restri
         const long tn = t + RW;
         const BatchEntry &bn = entry(tn < end ? tn : t);
         therm_tile_pipe<XIN, DIS>(b.P, b.ck, b.A, pass, (int)(t - b.pt), bn.P, bn.ck, bn.A,
-                                  tn < end ? (int)(tn - bn.pt) : -1, raw, sU, sFlag);
+                                  tn < end ? (int)(tn - bn.pt) : -1, raw, sU, sFlag, stol);
     }
 }
 

@@ -683,7 +683,7 @@ GF_DEV void therm_store_raw(const ThermPf &f, ThermRaw &R)
 template <int XIN, bool DIS>
 GF_DEV void therm_tile_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, int pass,
                             int tile, const GfDevParams *__restrict__ Pn, GfDevClock *ckn, const GfDevArrays &An,
-                            int next, ThermRaw &R, double *sU, int *sFlag)
+                            int next, ThermRaw &R, double *sU, int *sFlag, double stol)
 {
     double *__restrict__ xout = (pass % 2 == 0) ? A.xa : A.xb;
     double *__restrict__ bord = A.tbord;
@@ -836,7 +836,11 @@ GF_DEV void therm_tile_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, c
     while (nsweep < TH_MAXSWEEP) {
         if (valid) {
             const double tn = therm_eval(c, sX[cur][up], sX[cur][lf], sX[cur][dl], dt);
-            if (__double_as_longlong(tn) != __double_as_longlong(sX[cur][me])) sFlag[nsweep % 3] = 1;
+            // still: no bit changes (stol 0), or no change above stol kelvin (the
+            // single unproven pass of FP32-arithmetic mode)
+            const double old = sX[cur][me];
+            if (stol > 0.0 ? fabs(tn - old) > stol : __double_as_longlong(tn) != __double_as_longlong(old))
+                sFlag[nsweep % 3] = 1;
             sX[cur ^ 1][me] = tn;
         }
         lds_barrier();
@@ -870,7 +874,7 @@ GF_DEV void therm_tile_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, c
 // x q + r, + R, + 2R, ... with R = grid / 8.
 template <int XIN, bool DIS>
 __global__ void __launch_bounds__(1024) k_therm_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
-                                                     int pass)
+                                                     int pass, double stol)
 {
     if (pass > 0 && ck->therm_pass_changed[pass - 1] == 0u) return;  // already proven
     __shared__ ThermRaw raw;
@@ -883,7 +887,7 @@ __global__ void __launch_bounds__(1024) k_therm_pipe(const GfDevParams *__restri
     therm_store_raw<XIN, DIS>(therm_prefetch<XIN, DIS>(P, ck, A, pass, first), raw);
     for (int t = first; t < end; t += RW) {
         lds_barrier();  // this tile's inputs are in; the previous tile is done with sU
-        therm_tile_pipe<XIN, DIS>(P, ck, A, pass, t, P, ck, A, t + RW < end ? t + RW : -1, raw, sU, sFlag);
+        therm_tile_pipe<XIN, DIS>(P, ck, A, pass, t, P, ck, A, t + RW < end ? t + RW : -1, raw, sU, sFlag, stol);
     }
 }
 

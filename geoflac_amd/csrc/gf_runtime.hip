@@ -672,6 +672,10 @@ static int flush_rot(gf_ctx *ctx)
 }
 
 // clock_done: the step launched the thermal step's clock update already
+// FP32-arithmetic mode runs one thermal pass without the fixed-point proof; its tiles
+// stop sweeping once no node changes by more than this many kelvin (the GS iteration
+// contracts geometrically, so the remaining error is of this order)
+#define GF_THERM_STOL 1.0e-9
 static int enq_therm(gf_ctx *ctx, bool clock_done = false)
 {
     int rc;
@@ -693,7 +697,8 @@ static int enq_therm(gf_ctx *ctx, bool clock_done = false)
         auto kern = pass > 0 ? (dis ? k_therm_pipe<1, true> : k_therm_pipe<1, false>)
                     : ex     ? (dis ? k_therm_pipe<2, true> : k_therm_pipe<2, false>)
                              : (dis ? k_therm_pipe<0, true> : k_therm_pipe<0, false>);
-        hipLaunchKernelGGL(kern, grid, dim3(TH_J * TH_I), 0, ctx->stream, ctx->dp, ctx->dck, ctx->a, pass);
+        hipLaunchKernelGGL(kern, grid, dim3(TH_J * TH_I), 0, ctx->stream, ctx->dp, ctx->dck, ctx->a, pass,
+                           npass == 1 ? GF_THERM_STOL : 0.0);
         if ((rc = launch_check(ctx, "k_therm_pipe"))) return rc;
         if (ctx->timing && e0 && e1) {
             (void)hipEventRecord(e1, ctx->stream);
@@ -1489,7 +1494,7 @@ static int enq_batch_flac(gf_batch *b)
         // with one workgroup per tile that returns at once for a proven model
         if (pass == 0)
             BLAUNCH((dis ? kb_therm_pipe<0, true> : kb_therm_pipe<0, false>), b->therm_grid, TH_J * TH_I,
-                    b->dtab, map, b->tot_t, pass);
+                    b->dtab, map, b->tot_t, pass, npass == 1 ? GF_THERM_STOL : 0.0);
         else
             BLAUNCH(kb_therm_tile, (unsigned)std::min<long>(b->tot_t, 512), TH_J * TH_I, b->dtab, map, b->tot_t, pass);
         if (npass > 1) BLAUNCH(kb_therm_check, bgrid(b->tot_b, 256), 256, b->dtab, map, pass);
