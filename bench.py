@@ -26,7 +26,20 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "element-steps/s on 1 GPU: mixed/learned vs plain FP64, at stated rel-L2 error"
 
 
-def alg_bytes(kernel, E, N, nact, p):
+def area_pending_fraction(p, n0, nsteps):
+    """Fraction of the steps n0 .. n0+nsteps-1 whose fl_rheol forms the areas and dvol of
+    the previous step's move (the device defers them to it), assuming none pending at
+    n0: a viscosity-update step, and a step after rmasses or dt_mass, find them formed."""
+    pend, cnt = False, 0
+    for n in range(n0, n0 + nsteps):
+        if n % p.ifreq_visc == 0:
+            pend = False
+        cnt += pend
+        pend = not (n % p.ifreq_rmasses == 0 or n % p.ifreq_imasses == 0)
+    return cnt / max(nsteps, 1)
+
+
+def alg_bytes(kernel, E, N, nact, p, area_frac=0.0):
     """Algorithmic (compulsory) HBM bytes of one launch: each array the kernel must
     read once and write once, FP64 = 8 B, int32 = 4 B."""
     kernel = kernel.replace("<double>", "").replace("<0>", "")
@@ -34,7 +47,8 @@ def alg_bytes(kernel, E, N, nact, p):
         per_e = (32 + 32 + 128 + 8 + 8 + 8 * nact + 4 + 24 + 8 + 24)  # area dvol stress0 aps visn pr iphase se2sr sshrheat strain
         per_e += 128 + 8 + 24 + 24 + 8                                 # writes stress0 aps strain se2sr sshrheat
         per_n = 16 + 16                                                # vel, cord
-        return E * per_e + N * per_n
+        # with the move's areas pending: no dvol read, area and dvol written
+        return E * (per_e + 32 * area_frac) + N * per_n
     if kernel in ("k_therm_tile", "k_therm_pipe"):
         return N * (8 + 8 + 8 + 16) + E * (16 + 4 + 8)                # temp, temp0 out, x out, cord | area(1:2) iphase source
     if kernel in ("k_therm_tile_extra", "k_therm_pipe_extra"):
@@ -273,7 +287,9 @@ def main():
     args.profile_steps = max(args.profile_steps, 1)
     m.therm_stats()
     m.set_timing(True)
+    n0 = m.state.clock.nloop
     m.step(args.profile_steps)
+    afrac = area_pending_fraction(gp, n0, args.profile_steps)
     kernels = ["k_rheol1<false>", "k_rheol1<true>", "k_rheol", "k_visc<double>", "k_therm_pipe",
                "k_therm_pipe_extra", "k_therm_check", "k_node<0>", "k_node_t<0>", "k_move_elem<0>",
                "k_move_area", "k_move_rot<0>", "k_move_cord<double>", "k_therm_bc", "k_dens<double>", "k_rmass", "k_dt_reduce<double>", "k_amass",
@@ -284,10 +300,10 @@ def main():
     dom = max(times, key=lambda k: times[k][0])
     ms_tot, nl = times[dom]
     nact = int((host_state.phase_ratio >= 0.01).any(axis=(1, 2)).sum())
-    ab = alg_bytes(dom, E, N, nact, gp)
+    ab = alg_bytes(dom, E, N, nact, gp, afrac)
     # a fallback pass and its check count only where the pass ran (the launches after a
     # proven pass exit at once)
-    step_bytes = sum(alg_bytes(k, E, N, nact, gp) * v[1] for k, v in times.items()
+    step_bytes = sum(alg_bytes(k, E, N, nact, gp, afrac) * v[1] for k, v in times.items()
                      if k not in ("k_therm_pipe_extra", "k_therm_check"))
     step_bytes += alg_bytes("k_therm_pipe_extra", E, N, nact, gp) * th[1]
     step_bytes += alg_bytes("k_therm_check", E, N, nact, gp) * (args.profile_steps + th[1])

@@ -229,16 +229,26 @@ GF_DEV void therm_shape(const GfDevParams *P, const GfDevArrays &A, int je, int 
 // K(a,b) as the reference writes it: (dax*dbx + daz*dbz), a,b in 1..3
 #define KAB(d, a, b) ((d)[(a) - 1] * (d)[(b) - 1] + (d)[(a) + 2] * (d)[(b) + 2])
 
-// Coefficients of node (j,i); T(...) reads the old temperature.
+GF_DEV double move_det(const double *xc, const double *yc, int k);
+// area entry tri (0, 1) of element (je,ie) from the moved coordinates, as
+// move_area_body stores it, while that move's areas are pending
+GF_DEV double therm_moved_area(const GfDevParams *P, const GfDevArrays &A, int je, int ie, int tri)
+{
+    const double xc[5] = {0.0, CORD(je, ie, 1), CORD(je + 1, ie, 1), CORD(je, ie + 1, 1), CORD(je + 1, ie + 1, 1)};
+    const double yc[5] = {0.0, CORD(je, ie, 2), CORD(je + 1, ie, 2), CORD(je, ie + 1, 2), CORD(je + 1, ie + 1, 2)};
+    return 1.0 / move_det(xc, yc, tri);
+}
+// Coefficients of node (j,i); T(...) reads the old temperature; arp: the element
+// areas of the last move are pending (formed from the coordinates).
 template <typename TOLDF>
-GF_DEV void therm_coef(const GfDevParams *P, const GfDevArrays &A, int j, int i, TOLDF T, ThermCoef &c)
+GF_DEV void therm_coef(const GfDevParams *P, const GfDevArrays &A, int j, int i, TOLDF T, ThermCoef &c, int arp = 0)
 {
     double d[6], ar, inv_area, ra13, df, dm;
     c.mask = 0;
     c.area_n = 0.0;
     c.told = T(j, i);
     auto tri_common = [&](int je, int ie, int tri, int t) {
-        ar = AREA(je, ie, tri + 1);
+        ar = arp ? therm_moved_area(P, A, je, ie, tri) : AREA(je, ie, tri + 1);
         inv_area = 1.0 / ar;
         ra13 = div_fma(inv_area, 6.0, GF_RSIX);
         c.area_n = c.area_n + ra13;
@@ -597,7 +607,7 @@ GF_DEV void therm_tile_origin(const GfDevParams *P, int tile, int &j0, int &i0)
 // point is the same).  DIS: the shear heating is loaded
 template <int XIN, bool DIS>
 GF_DEV ThermPf therm_prefetch(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, int pass,
-                              int tile)
+                              int tile, int arp = 0)
 {
     int j0, i0;
     therm_tile_origin(P, tile, j0, i0);
@@ -638,8 +648,10 @@ GF_DEV ThermPf therm_prefetch(const GfDevParams *__restrict__ P, GfDevClock *ck,
         f.iph[k] = 1;
         if (q < TH_NE && je >= 1 && je <= nz - 1 && ie >= 1 && ie <= nx - 1) {
             const long ix = (je - 1) + (nz - 1) * (ie - 1);
-            f.a0[k] = gld(A.area, ix);
-            f.a1[k] = gld(A.area, ix + ne);
+            if (!arp) {  // (pending areas are formed from the coordinates)
+                f.a0[k] = gld(A.area, ix);
+                f.a1[k] = gld(A.area, ix + ne);
+            }
             f.src[k] = gld(A.source, ix);
             f.iph[k] = gld(A.iphase, ix);
             if (DIS) f.sh[k] = gld(A.shrheat, ix);
@@ -683,7 +695,7 @@ GF_DEV void therm_store_raw(const ThermPf &f, ThermRaw &R)
 template <int XIN, bool DIS>
 GF_DEV void therm_tile_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, int pass,
                             int tile, const GfDevParams *__restrict__ Pn, GfDevClock *ckn, const GfDevArrays &An,
-                            int next, ThermRaw &R, double *sU, int *sFlag, double stol)
+                            int next, ThermRaw &R, double *sU, int *sFlag, double stol, int arp = 0)
 {
     double *__restrict__ xout = (pass % 2 == 0) ? A.xa : A.xb;
     double *__restrict__ bord = A.tbord;
@@ -698,7 +710,7 @@ GF_DEV void therm_tile_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, c
     }
     if (!R.active) {  // a proven model (passes > 0): nothing to do but the prefetch
         lds_barrier();
-        if (next >= 0) therm_store_raw<XIN, DIS>(therm_prefetch<XIN, DIS>(Pn, ckn, An, pass, next), R);
+        if (next >= 0) therm_store_raw<XIN, DIS>(therm_prefetch<XIN, DIS>(Pn, ckn, An, pass, next, arp), R);
         return;
     }
     const double(*sPh)[GF_MAXPH] = R.ph;
@@ -710,6 +722,18 @@ GF_DEV void therm_tile_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, c
         return e < TH_NE && je >= 1 && je <= nz - 1 && ie >= 1 && ie <= nx - 1;
     };
     const bool ein0 = elem_ok(e0), ein1 = elem_ok(e1);
+    if (arp) {  // the pending areas of triangles A and B from the footprint's coordinates
+        auto moved_area = [&](int e) {
+            const int ej = e % TH_EJ - 1, ei = e / TH_EJ - 1;
+            const int n1 = LIDX(ej, ei), n2 = LIDX(ej + 1, ei), n3 = LIDX(ej, ei + 1), n4 = LIDX(ej + 1, ei + 1);
+            const double xc[5] = {0.0, R.cx[n1], R.cx[n2], R.cx[n3], R.cx[n4]};
+            const double yc[5] = {0.0, R.cy[n1], R.cy[n2], R.cy[n3], R.cy[n4]};
+            R.a0[e] = 1.0 / move_det(xc, yc, 0);
+            R.a1[e] = 1.0 / move_det(xc, yc, 1);
+        };
+        if (ein0) moved_area(e0);
+        if (ein1) moved_area(e1);
+    }
     // round 0: ra13, P = dummye*ra13, Q = 0.5*diff*inv_area of triangles A and B;
     // rounds 1 and 2: the K(a,b) products of A, of B (as in therm_tile_one)
     auto elem_store = [&](int e, int round) {
@@ -821,7 +845,7 @@ GF_DEV void therm_tile_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, c
     const double dt = R.dt;
     asm volatile("" ::"v"(dt));
     ThermPf pf;
-    if (next >= 0) pf = therm_prefetch<XIN, DIS>(Pn, ckn, An, pass, next);  // R is free now
+    if (next >= 0) pf = therm_prefetch<XIN, DIS>(Pn, ckn, An, pass, next, arp);  // R is free now
 
     const int me = nb, up = nb - 1, lf = nb - TH_LJ, dl = nb - TH_LJ + 1;
     int cur = 0, nsweep = 0;
@@ -873,8 +897,9 @@ GF_DEV void therm_tile_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, c
 // owns the contiguous tile range [x q, (x+1) q); its r-th workgroup there takes tiles
 // x q + r, + R, + 2R, ... with R = grid / 8.
 template <int XIN, bool DIS>
+// arp: the element areas of the last move are pending (formed from the coordinates)
 __global__ void __launch_bounds__(1024) k_therm_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
-                                                     int pass, double stol)
+                                                     int pass, double stol, int arp)
 {
     if (pass > 0 && ck->therm_pass_changed[pass - 1] == 0u) return;  // already proven
     __shared__ ThermRaw raw;
@@ -884,10 +909,10 @@ __global__ void __launch_bounds__(1024) k_therm_pipe(const GfDevParams *__restri
     const int q = (ntiles + 7) / 8, x = blockIdx.x % 8, r = blockIdx.x / 8, RW = gridDim.x / 8;
     const int first = x * q + r, end = min((x + 1) * q, ntiles);
     if (first >= end) return;
-    therm_store_raw<XIN, DIS>(therm_prefetch<XIN, DIS>(P, ck, A, pass, first), raw);
+    therm_store_raw<XIN, DIS>(therm_prefetch<XIN, DIS>(P, ck, A, pass, first, arp), raw);
     for (int t = first; t < end; t += RW) {
         lds_barrier();  // this tile's inputs are in; the previous tile is done with sU
-        therm_tile_pipe<XIN, DIS>(P, ck, A, pass, t, P, ck, A, t + RW < end ? t + RW : -1, raw, sU, sFlag, stol);
+        therm_tile_pipe<XIN, DIS>(P, ck, A, pass, t, P, ck, A, t + RW < end ? t + RW : -1, raw, sU, sFlag, stol, arp);
     }
 }
 
@@ -899,7 +924,7 @@ __global__ void __launch_bounds__(1024) k_therm_pipe(const GfDevParams *__restri
 // rows' rounding still show up as a few ulp there.  Any remaining difference (or a
 // tile that did not settle) sends the next pass.
 GF_DEV void therm_check_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, int pass,
-                             long t)
+                             long t, int arp = 0)
 {
     if (pass > 0 && ck->therm_pass_changed[pass - 1] == 0u) return;
     const double *__restrict__ x = (pass % 2 == 0) ? A.xa : A.xb;
@@ -926,7 +951,7 @@ GF_DEV void therm_check_body(const GfDevParams *__restrict__ P, GfDevClock *ck, 
         if (mlj < TH_HJ || mlj >= TH_J - TH_HJ || mli < TH_HI || mli >= TH_I) continue;
         if (mj < 1 || mj > NZ || mi < 1 || mi > NX) continue;
         ThermCoef c;
-        therm_coef(P, A, mj, mi, T, c);
+        therm_coef(P, A, mj, mi, T, c, arp);
         const double xu = mj > 1 ? x[IN(mj - 1, mi)] : 0.0;
         const double xl = mi > 1 ? x[IN(mj, mi - 1)] : 0.0;
         const double xd = (mi > 1 && mj < NZ) ? x[IN(mj + 1, mi - 1)] : 0.0;
@@ -937,9 +962,9 @@ GF_DEV void therm_check_body(const GfDevParams *__restrict__ P, GfDevClock *ck, 
 }
 
 __global__ void __launch_bounds__(256) k_therm_check(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
-                                                     int pass)
+                                                     int pass, int arp)
 {
-    therm_check_body(P, ck, A, pass, (long)blockIdx.x * blockDim.x + threadIdx.x);
+    therm_check_body(P, ck, A, pass, (long)blockIdx.x * blockDim.x + threadIdx.x, arp);
 }
 
 // Boundary conditions src/fl_therm.f90:575-598 applied to the proven iterate.
@@ -1453,13 +1478,18 @@ GF_DEV void move_rotate(T &s11, T &s22, T &s12, T w)
 // the stress deviation; (float, double) FP32 storage with FP64 arithmetic on the
 // absolute stress (the deviation is converted on load and store).
 //
-// rot != 0: the rotation of the previous element move is still pending (k_move_area
-// ran without it); it is applied here to the loaded stress and strain, with the
-// move's dt, exactly as k_move_rot would have stored them.
+// rot bit 0 (RH_ROT): the rotation of the previous element move is still pending
+// (k_move_area ran without it); it is applied here to the loaded stress and strain,
+// with the move's dt, exactly as k_move_rot would have stored them.
+// rot bit 1 (RH_AREA): the areas and dvol of that move are pending too (no
+// k_move_area ran): they are formed here from the moved coordinates the kernel loads
+// anyway, with k_move_area's arithmetic, used, and stored (src/fl_move.f90:108-126).
 //
 // phs: the phase table (a workgroup's LDS copy of P->ph, or P->ph itself).  Every
 // per-element load is issued before the first use, so that a wave pays one memory
 // round trip for its inputs instead of a chain of them.
+#define RH_ROT 1
+#define RH_AREA 2
 template <bool SHEARH, typename SV, typename AT>
 GF_DEV void rheol_single(const GfDevParams *__restrict__ P, const GfPhase *__restrict__ phs, GfDevClock *ck,
                          const GfDevArrays &A, long e, int avgsr, int rot)
@@ -1497,9 +1527,11 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, const GfPhase *__res
     sshr_in = bld(bSH, eo, 0);
     SV sv_in[16];
     for (int q = 0; q < 16; q++) sv_in[q] = bldt<SV>(bS, so, q * ps);
-    double dvl[4];
-    for (int k = 0; k < 4; k++) dvl[k] = bld(bD, eo, k * pe);
-    double ar[4];
+    const bool areas = (rot & RH_AREA) != 0;
+    double dvl[4] = {0.0, 0.0, 0.0, 0.0};
+    if (!areas)
+        for (int k = 0; k < 4; k++) dvl[k] = bld(bD, eo, k * pe);
+    double ar[4];  // (the areas before the move when they are pending)
     for (int k = 0; k < 4; k++) ar[k] = bld(bA, eo, k * pe);
     const RheolPhase rp = rheol_phase(phs, iph);
     const double dt = ck->dt;
@@ -1515,6 +1547,20 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, const GfPhase *__res
         const double x1 = bld(bC, no, 0), y1 = bld(bC, no, pn), x2 = bld(bC, no, 8), y2 = bld(bC, no, pn + 8);
         const double x3 = bld(bC, no, col), y3 = bld(bC, no, pn + col), x4 = bld(bC, no, col + 8),
                      y4 = bld(bC, no, pn + col + 8);
+        if (areas) {  // the move's areas and dvol (move_area_body)
+            const double xc[5] = {0.0, x1, x2, x3, x4}, yc[5] = {0.0, y1, y2, y3, y4};
+            bool bad = false;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const double det = move_det(xc, yc, k);
+                dvl[k] = det * ar[k] - 1.0;
+                ar[k] = 1.0 / det;
+                bad |= (ar[k] <= 0);
+                bst(bD, eo, k * pe, dvl[k]);
+                bst(bA, eo, k * pe, ar[k]);
+            }
+            if (bad) raise_error(ck, GF_STOP_NEGATIVE_AREA, e);
+        }
         // per triangle, vertices (v1,v2,v3) in the reference's order: derivatives
         // (y2-y3, y3-y1, y1-y2)*area and (x3-x2, x1-x3, x2-x1)*area (:98-156), formed
         // from the FP64 coordinates
@@ -1531,7 +1577,7 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, const GfPhase *__res
         tri(1, x3, y3, vx3, vy3, x2, y2, vx2, vy2, x4, y4, vx4, vy4);
         tri(2, x1, y1, vx1, vy1, x2, y2, vx2, vy2, x4, y4, vx4, vy4);
         tri(3, x1, y1, vx1, vy1, x4, y4, vx4, vy4, x3, y3, vx3, vy3);
-        if (rot) {
+        if (rot & RH_ROT) {
             // the stored velocities, exactly (AT holds every SV value)
             const double xc[5] = {0.0, x1, x2, x3, x4}, yc[5] = {0.0, y1, y2, y3, y4};
             const double xv[5] = {0.0, (double)vx1, (double)vx2, (double)vx3, (double)vx4};
@@ -1555,7 +1601,7 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, const GfPhase *__res
     AT st[4][4];
     for (int k = 0; k < 4; k++)
         for (int c = 0; c < 4; c++) st[k][c] = AT(sv_in[c + 4 * k]);
-    if (rot)
+    if (rot & RH_ROT)
         for (int k = 0; k < 4; k++) {
             move_rotate<AT>(st[k][0], st[k][1], st[k][2], AT(dw[k]));
             for (int c = 0; c < 3; c++) st[k][c] = AT(SV(st[k][c]));  // as k_move_rot stores them
@@ -1713,7 +1759,7 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, const GfPhase *__res
     // (mixed precision: the absolute stress, deviation - pref)
     {
         double t11 = str_in[0], t22 = str_in[1], t12 = str_in[2];
-        if (rot) move_rotate<double>(t11, t22, t12, dw[0]);
+        if (rot & RH_ROT) move_rotate<double>(t11, t22, t12, dw[0]);
         bst(bST, eo, 0, t11 + (dev_form ? (double)s11 - pd.pref : (double)s11) * dt);
         bst(bST, eo, pe, t22 + (dev_form ? (double)s22 - pd.pref : (double)s22) * dt);
         bst(bST, eo, 2 * pe, t12 + s12 * dt);
@@ -2313,6 +2359,7 @@ __global__ void __launch_bounds__(256) k_node_t(const GfDevParams *__restrict__ 
     const int q = (ntiles + 7) / 8, tile = (int)(blockIdx.x % 8) * q + (int)(blockIdx.x / 8);
     if (tile >= ntiles) return;  // the whole workgroup
     const NodeClk kc = node_clk(ck);
+    if (MOVE && blockIdx.x == 0 && threadIdx.x == 0) ck->dt_move = kc.dt;  // the move's dt (no other reader here)
     const NodePf<M> pf = node_prefetch<M>(P, A, tile);
     node_stage<M>(pf, L);
     __syncthreads();
@@ -2391,7 +2438,8 @@ GF_DEV void move_rotate_elem(const GfDevParams *__restrict__ P, const GfDevArray
 }
 
 // first half: new areas and dvol, negative-area STOP; records dt for the rotation
-GF_DEV void move_area_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e)
+GF_DEV void move_area_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e,
+                           int rec_dt = 1)
 {
     const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
     double xc[5], yc[5];
@@ -2406,14 +2454,16 @@ GF_DEV void move_area_body(const GfDevParams *__restrict__ P, GfDevClock *ck, co
         A.area[e + NE * k] = ar;
         bad |= (ar <= 0);
     }
-    if (e == 0) ck->dt_move = ck->dt;
+    if (rec_dt && e == 0) ck->dt_move = ck->dt;
     if (bad) raise_error(ck, GF_STOP_NEGATIVE_AREA, e);
 }
-__global__ void __launch_bounds__(256) k_move_area(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+// rec_dt: record the move's dt (0 when completing a move whose dt is recorded already)
+__global__ void __launch_bounds__(256) k_move_area(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
+                                                   int rec_dt)
 {
     long e = xcd_lin();
     if (e >= NE) return;
-    move_area_body(P, ck, A, e);
+    move_area_body(P, ck, A, e, rec_dt);
 }
 
 // second half on its own: the pending rotation with the dt of its move

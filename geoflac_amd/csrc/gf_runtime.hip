@@ -70,6 +70,9 @@ struct gf_ctx {
     bool therm_noguess = false;  // GF_THERM_GUESS=0: pass 0 starts from the old temperature
     int in_batch = 0;
     bool rot_pending = false; // the last element move's stress/strain rotation is not applied yet
+    // ... nor its areas and dvol (the step's fl_rheol forms them; the thermal step
+    // before it forms the areas it needs from the coordinates)
+    bool area_pending = false;
     unsigned therm_grid = 8;  // workgroups of the pipelined thermal kernel: the CUs, a multiple of 8
     // TH_MAXPASS: the thermal result is proven to be the reference's serial one (fallback
     // passes when needed); 1: FP32-arithmetic mode takes the single tile pass as it is
@@ -663,8 +666,13 @@ static int enq_clock(gf_ctx *ctx, int op)
 
 // applies a pending stress/strain rotation (k_move_area ran without it) before
 // anything but k_rheol1 reads stress or strain
+// the pending areas and dvol of the last move, and then its pending rotation
 static int flush_rot(gf_ctx *ctx)
 {
+    if (ctx->area_pending) {
+        ctx->area_pending = false;
+        LAUNCH(k_move_area, ctx->ne, ctx->dp, ctx->dck, ctx->a, 0);  // (the move's dt is recorded)
+    }
     if (!ctx->rot_pending) return GF_OK;
     ctx->rot_pending = false;
     LAUNCH_M(k_move_rot, ctx->ne, ctx->dp, ctx->dck, ctx->a);
@@ -698,13 +706,14 @@ static int enq_therm(gf_ctx *ctx, bool clock_done = false)
                     : ex     ? (dis ? k_therm_pipe<2, true> : k_therm_pipe<2, false>)
                              : (dis ? k_therm_pipe<0, true> : k_therm_pipe<0, false>);
         hipLaunchKernelGGL(kern, grid, dim3(TH_J * TH_I), 0, ctx->stream, ctx->dp, ctx->dck, ctx->a, pass,
-                           npass == 1 ? GF_THERM_STOL : 0.0);
+                           npass == 1 ? GF_THERM_STOL : 0.0, ctx->area_pending ? 1 : 0);
         if ((rc = launch_check(ctx, "k_therm_pipe"))) return rc;
         if (ctx->timing && e0 && e1) {
             (void)hipEventRecord(e1, ctx->stream);
             ctx->pending.push_back({pass == 0 ? "k_therm_pipe" : "k_therm_pipe_extra", {e0, e1}});
         }
-        if (npass > 1) LAUNCH(k_therm_check, (long)ntiles * TH_NB, ctx->dp, ctx->dck, ctx->a, pass);
+        if (npass > 1)
+            LAUNCH(k_therm_check, (long)ntiles * TH_NB, ctx->dp, ctx->dck, ctx->a, pass, ctx->area_pending ? 1 : 0);
     }
     LAUNCH(k_therm_bc, ctx->nn, ctx->dp, ctx->dck, ctx->a, npass);
     ctx->temp0_prev = true;  // pass 0 wrote temp0 at every node
@@ -724,8 +733,9 @@ static int enq_rheol(gf_ctx *ctx, bool clock_done = false)
     if (!clock_done && (rc = enq_clock(ctx, CK_DTAVG_ADD))) return rc;
     // the viscosity update reads the rotated step-start stress
     if ((do_visc || ctx->rheol_quad) && (rc = flush_rot(ctx))) return rc;
-    const int rot = ctx->rot_pending ? 1 : 0;
+    const int rot = (ctx->rot_pending ? RH_ROT : 0) | (ctx->area_pending ? RH_AREA : 0);
     ctx->rot_pending = false;
+    ctx->area_pending = false;
     if (do_visc && ctx->sur_on) {
         // the same persistent grid for both passes (a few workgroups per CU)
         const long pers = (long)ctx->sur_grid * 256;
@@ -793,12 +803,18 @@ static int enq_move(gf_ctx *ctx)
     if (ctx->p.movegrid == 0) return GF_OK;
     int rc;
     if ((rc = flush_rot(ctx))) return rc;  // a second move before any fl_rheol
-    if (!ctx->cord_moved) LAUNCH_SV(k_move_cord, ctx->nn, ctx->dp, ctx->dck, ctx->a);
+    const bool fused = ctx->cord_moved;   // the step's fl_node moved the coordinates
+    if (!fused) LAUNCH_SV(k_move_cord, ctx->nn, ctx->dp, ctx->dck, ctx->a);
     ctx->cord_moved = false;
     if (ctx->rheol_quad) {
         LAUNCH_M(k_move_elem, ctx->ne, ctx->dp, ctx->dck, ctx->a);
+    } else if (fused) {
+        // the areas and dvol wait for the next fl_rheol (or the first kernel that
+        // needs them); the fused fl_node recorded the move's dt
+        ctx->area_pending = true;
+        ctx->rot_pending = true;
     } else {
-        LAUNCH(k_move_area, ctx->ne, ctx->dp, ctx->dck, ctx->a);
+        LAUNCH(k_move_area, ctx->ne, ctx->dp, ctx->dck, ctx->a, 1);
         ctx->rot_pending = true;
     }
     return GF_OK;
@@ -816,6 +832,7 @@ static int enq_rmasses(gf_ctx *ctx)
 static int enq_dt_mass(gf_ctx *ctx)
 {
     int rc;
+    if ((rc = flush_rot(ctx))) return rc;  // the dt minima and inertial masses read the areas
     if ((rc = enq_clock(ctx, CK_DT_BEGIN))) return rc;
     LAUNCH_SV(k_dt_reduce, (long)DT_RED_BLOCKS * 256, ctx->dp, ctx->dck, ctx->a);
     hipLaunchKernelGGL(k_dt_scalars, dim3(1), dim3(64), 0, ctx->stream, ctx->dp, ctx->dck, ctx->d_present,
@@ -1566,7 +1583,7 @@ static bool graph_cycle_ok(const gf_batch *b)
 static int batch_flush(gf_batch *b)
 {
     for (gf_ctx *c : b->m) {
-        if (!c->rot_pending) continue;
+        if (!c->rot_pending && !c->area_pending) continue;
         int rc = flush_rot(c);
         if (rc == GF_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = GF_ERR_HIP;
         if (rc) return bset_err(b, rc, c->msg);
