@@ -853,6 +853,15 @@ static int check_ready(gf_ctx *ctx)
     return GF_OK;
 }
 
+// Before a call returns: a move whose areas are still pending is checked for the
+// negative-area STOP now (the areas come with the next fl_rheol), so that the call
+// that moved the grid reports it.
+static int enq_area_check(gf_ctx *ctx)
+{
+    if (ctx->area_pending) LAUNCH(k_area_check, ctx->ne, ctx->dp, ctx->dck, ctx->a);
+    return GF_OK;
+}
+
 // one call of subroutine flac, src/flac.f90:18-83; pending: clock updates left by the
 // previous step (its time advance), launched together with this step's own
 static int enq_flac(gf_ctx *ctx, int pending = 0)
@@ -884,6 +893,7 @@ extern "C" int gf_step(gf_ctx *ctx, int32_t nsteps, gf_clock *clock)
         ctx->nloop = ctx->nloop + 1;
     }
     if (nsteps > 0 && (rc = enq_clock(ctx, CK_TIME_ADVANCE))) return rc;
+    if ((rc = enq_area_check(ctx))) return rc;
     return fetch_clock(ctx, clock);
 }
 
@@ -898,6 +908,7 @@ extern "C" int gf_flac(gf_ctx *ctx, gf_clock *clock)
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipMemcpy(&ctx->dck->time, &clock->time, sizeof(double), hipMemcpyHostToDevice));
     if ((rc = enq_flac(ctx))) return rc;
+    if ((rc = enq_area_check(ctx))) return rc;
     return fetch_clock(ctx, clock);
 }
 

@@ -109,6 +109,54 @@ def test_kernel_variants_bitwise_equal_to_oracle(env, monkeypatch):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("nsteps", [1, 3])
+def test_negative_area_stop(nsteps):
+    """A node pushed far beyond its neighbours in one step turns its elements inside
+    out: the device returns STOP 40 (src/fl_move.f90:125) from the call whose move
+    did it, when the move's areas are still pending at the end of the call (nsteps 1)
+    and when the next step's fl_rheol forms them (nsteps 3).  With one step the error
+    site is the oracle's, the first inverted element in the reference's loop order;
+    a longer call keeps stepping past the stop, and the site is then the smallest
+    element index any of its steps found."""
+    from geoflac_amd import gpu
+    _say("case: STOP 40 after %d step(s) from the 10-step golden state of %s" % (nsteps, CASE))
+    p = ol.golden_params(CASE)
+    st = golden_full(10)
+    j, i = 10, 30                       # an interior node (1-based)
+    st.vel[j - 1, i - 1, 0] = 1.0e3     # m/s: moves it by far more than an element
+    ref = st.copy()
+    rc_ref = ol.step(p, ref, nsteps, cr=True)
+    assert rc_ref == 40
+    # the oracle stops inside the first step's fl_move: the first element (i outer,
+    # j inner) with a non-positive area
+    x, y = ref.cord[..., 0], ref.cord[..., 1]
+
+    def det(a, b, c):
+        (xa, ya), (xb, yb), (xc, yc) = a, b, c
+        return (xb * yc - yb * xc) - (xa * yc - ya * xc) + (xa * yb - ya * xb)
+    first = None
+    for ii in range(p.nx - 1):
+        for jj in range(p.nz - 1):
+            c = [(x[jj, ii], y[jj, ii]), (x[jj + 1, ii], y[jj + 1, ii]), (x[jj, ii + 1], y[jj, ii + 1]),
+                 (x[jj + 1, ii + 1], y[jj + 1, ii + 1])]
+            tris = [(0, 1, 2), (2, 1, 3), (0, 1, 3), (0, 3, 2)]
+            if any(1.0 / det(c[a], c[b], c[d]) <= 0 for a, b, d in tris):
+                first = (jj + 1, ii + 1)
+                break
+        if first:
+            break
+    assert first is not None
+    with gpu.Model(p, st) as m:
+        with pytest.raises(gpu.GfError) as exc:
+            m.step(nsteps)
+        assert exc.value.code == 40
+        site = m.error_site()
+    _say("  oracle rc %d, first inverted element %s, device site %s" % (rc_ref, first, site))
+    if nsteps == 1:
+        assert tuple(site) == first
+
+
+@pytest.mark.gpu
 def test_interleaved_calls_bitwise_equal_to_oracle():
     """The step leaves the element move's stress/strain rotation to the next fl_rheol.
     Steps and single subroutines in an order that crosses each point where the
