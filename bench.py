@@ -77,6 +77,8 @@ def alg_bytes(kernel, E, N, nact, p, area_frac=0.0):
         return N * 48
     if kernel == "k_therm_bc":
         return N * 16
+    if kernel == "k_area_check":
+        return N * 16                                                  # cord (the move's negative-area test)
     if kernel == "k_therm_check":
         # the border check: 76 border values of each 64x16-node tile (52x12 outputs)
         # against the assembled iterate; fallback passes are added where they ran
@@ -293,7 +295,7 @@ def main():
     kernels = ["k_rheol1<false>", "k_rheol1<true>", "k_rheol", "k_visc<double>", "k_therm_pipe",
                "k_therm_pipe_extra", "k_therm_check", "k_node<0>", "k_node_t<0>", "k_move_elem<0>",
                "k_move_area", "k_move_rot<0>", "k_move_cord<double>", "k_therm_bc", "k_dens<double>", "k_rmass", "k_dt_reduce<double>", "k_amass",
-               "k_clock"]
+               "k_area_check", "k_clock"]
     times = {k: m.kernel_time(k) for k in kernels}
     m.set_timing(False)
     th = m.therm_stats()
