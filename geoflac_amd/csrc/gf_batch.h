@@ -108,10 +108,10 @@ __global__ void __launch_bounds__(256) kb_therm_bc(const BatchEntry *__restrict_
     batch_apply<BK_NODE>(tab, map, [&](const BatchEntry &b, long t) { therm_bc_body(b.P, b.ck, b.A, t, passes); });
 }
 
-template <typename SV>
+template <int M>
 __global__ void __launch_bounds__(256) kb_visc(const BatchEntry *__restrict__ tab, BatchMap map)
 {
-    batch_apply<BK_ELEM>(tab, map, [&](const BatchEntry &b, long e) { visc_body<SV>(b.P, b.ck, b.A, e); });
+    batch_apply<BK_ELEM>(tab, map, [&](const BatchEntry &b, long e) { visc_body<M>(b.P, b.ck, b.A, e, 0); });
 }
 
 template <bool SHEARH, int M>

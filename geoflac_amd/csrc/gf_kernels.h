@@ -1864,9 +1864,12 @@ GF_DEV void stress_avg(const GfDevParams *P, const GfDevArrays &A, long e, doubl
 
 // the inputs of element e's viscosity, every load issued before the first use (one
 // memory round trip per element)
-template <typename SV>
+// AT, dw: the rotation of a pending element move (dw per triangle, in the arithmetic AT
+// of k_move_rot) is applied to the loaded stress first, as k_move_rot would store it
+template <typename SV, typename AT = SV>
 GF_DEV void visc_load(const GfDevParams *__restrict__ P, const GfDevArrays &A, long e, int j, int i, int &iph,
-                      double (&prv)[GF_MAXACT], double &a11, double &a22, double &a33, ViscIn &vin)
+                      double (&prv)[GF_MAXACT], double &a11, double &a22, double &a33, ViscIn &vin,
+                      const double *dw = nullptr)
 {
     const unsigned ne = (unsigned)NE, eo = (unsigned)e;
     iph = A.iphase[e];
@@ -1881,6 +1884,16 @@ GF_DEV void visc_load(const GfDevParams *__restrict__ P, const GfDevArrays &A, l
     for (int c = 0; c < 3; c++)
 #pragma unroll
         for (int k = 0; k < 4; k++) sv[c][k] = bldt<SV>(bS, eo * (unsigned)sizeof(SV), (comp[c] + 4 * k) * ne * (unsigned)sizeof(SV));
+    if (dw) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            AT t11 = sv[0][k], t22 = sv[1][k];
+            AT t12 = bldt<SV>(bS, eo * (unsigned)sizeof(SV), (2 + 4 * k) * ne * (unsigned)sizeof(SV));
+            move_rotate<AT>(t11, t22, t12, AT(dw[k]));
+            sv[0][k] = SV(t11);
+            sv[1][k] = SV(t22);
+        }
+    }
     const double pr = ref_press<SV>(A, e);
     vin = visc_inputs(P, A, j, i);
     // stress_avg on the loaded values
@@ -1895,23 +1908,35 @@ GF_DEV void visc_load(const GfDevParams *__restrict__ P, const GfDevArrays &A, l
 }
 GF_DEV bool visc_rheology(int irh) { return irh == 3 || irh >= 11; }  // the Maxwell rheologies
 
-template <typename SV>
-GF_DEV void visc_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e)
+// rot: the last element move's rotation is pending (it stays pending: the stress is
+// rotated here for the average only)
+template <int M>
+GF_DEV void visc_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e, int rot)
 {
+    typedef typename Prec<M>::SV SV;
+    typedef typename Prec<M>::AT AT;
     const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
     int iph;
     double prv[GF_MAXACT], a11, a22, a33;
     ViscIn vin;
-    visc_load<SV>(P, A, e, j, i, iph, prv, a11, a22, a33, vin);
+    double dw[4];
+    if (rot) {  // move_rot_body's increments
+        double xc[5], yc[5], xv[5], yv[5];
+        move_load_corners<SV>(P, A, j, i, xc, yc, xv, yv);
+        const double dt = ck->dt_move;
+#pragma unroll
+        for (int k = 0; k < 4; k++) dw[k] = move_dw12(xc, yc, xv, yv, k, move_det(xc, yc, k), dt);
+    }
+    visc_load<SV, AT>(P, A, e, j, i, iph, prv, a11, a22, a33, vin, rot ? dw : nullptr);
     if (!visc_rheology(PHASE(iph).irheol)) return;
     A.visn[e] = dev_eff_visc_in(P, j, i, a11, a22, a33, prv, vin);
 }
-template <typename SV>
-__global__ void __launch_bounds__(256) k_visc(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+template <int M>
+__global__ void __launch_bounds__(256) k_visc(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A, int rot)
 {
     long e = xcd_lin();
     if (e >= NE) return;
-    visc_body<SV>(P, ck, A, e);
+    visc_body<M>(P, ck, A, e, rot);
 }
 
 __global__ void __launch_bounds__(256) k_rheol(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,

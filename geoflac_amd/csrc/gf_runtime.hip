@@ -731,8 +731,10 @@ static int enq_rheol(gf_ctx *ctx, bool clock_done = false)
         avgsr = 2;
     const bool mixed = ctx->precision == GF_PREC_MIXED, mstore = ctx->precision == GF_PREC_MIXED_STORAGE;
     if (!clock_done && (rc = enq_clock(ctx, CK_DTAVG_ADD))) return rc;
-    // the viscosity update reads the rotated step-start stress
-    if ((do_visc || ctx->rheol_quad) && (rc = flush_rot(ctx))) return rc;
+    // the viscosity update reads the rotated step-start stress: k_visc rotates what it
+    // loads itself (the move stays pending for k_rheol1); the surrogate's passes and
+    // the quad rheology find the move completed
+    if (((do_visc && ctx->sur_on) || ctx->rheol_quad) && (rc = flush_rot(ctx))) return rc;
     const int rot = (ctx->rot_pending ? RH_ROT : 0) | (ctx->area_pending ? RH_AREA : 0);
     ctx->rot_pending = false;
     ctx->area_pending = false;
@@ -745,8 +747,12 @@ static int enq_rheol(gf_ctx *ctx, bool clock_done = false)
         LAUNCH_SV(k_visc_list, std::min<long>(pers, (long)ctx->therm_grid * 3 * 256), ctx->dp, ctx->dck, ctx->a,
                   ctx->sur_cfg, ctx->sur_stats, ctx->sur_list, ctx->sur_lval, ctx->sur_cnt, ctx->sur_wgs,
                   ctx->sur_seg, ctx->sur_grid);
-    } else if (do_visc)
-        LAUNCH_SV(k_visc, ctx->ne, ctx->dp, ctx->dck, ctx->a);
+    } else if (do_visc) {
+        const int vr = (rot & RH_ROT) ? 1 : 0;
+        if (mixed) LAUNCH_NAMED("k_visc<float>", k_visc<1>, ctx->ne, ctx->dp, ctx->dck, ctx->a, vr);
+        else if (mstore) LAUNCH_NAMED("k_visc<float>", k_visc<2>, ctx->ne, ctx->dp, ctx->dck, ctx->a, vr);
+        else LAUNCH_NAMED("k_visc<double>", k_visc<0>, ctx->ne, ctx->dp, ctx->dck, ctx->a, vr);
+    }
     if (ctx->rheol_quad && !mixed && !mstore)
         LAUNCH(k_rheol, 4 * ctx->ne, ctx->dp, ctx->dck, ctx->a, avgsr);
     else if (mstore && ctx->p.ishearh == 1)
@@ -1536,7 +1542,7 @@ static int enq_batch_flac(gf_batch *b)
             avgsr = 1;
         else if (b->nsrate == -1)
             avgsr = 2;
-        if (do_visc) BLAUNCH(kb_visc<SV>, bgrid(b->tot_e, 256), 256, b->dtab, map);
+        if (do_visc) BLAUNCH(kb_visc<M>, bgrid(b->tot_e, 256), 256, b->dtab, map);
         BTIMED(0, BLAUNCH((kb_rheol<SH, M>), bgrid(b->tot_e, b->bs[0]), b->bs[0], b->dtab, map, avgsr));
         if (avgsr == 1) {
             BLAUNCH(kb_clock, gm, 64, b->dtab, map, (int)CK_DTAVG_RESET);
