@@ -29,11 +29,9 @@ METRIC = "element-steps/s on 1 GPU: mixed/learned vs plain FP64, at stated rel-L
 def area_pending_fraction(p, n0, nsteps):
     """Fraction of the steps n0 .. n0+nsteps-1 whose fl_rheol forms the areas and dvol of
     the previous step's move (the device defers them to it), assuming none pending at
-    n0: a viscosity-update step, and a step after rmasses or dt_mass, find them formed."""
+    n0: a step after rmasses or dt_mass finds them formed."""
     pend, cnt = False, 0
     for n in range(n0, n0 + nsteps):
-        if n % p.ifreq_visc == 0:
-            pend = False
         cnt += pend
         pend = not (n % p.ifreq_rmasses == 0 or n % p.ifreq_imasses == 0)
     return cnt / max(nsteps, 1)

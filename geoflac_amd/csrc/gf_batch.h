@@ -140,10 +140,10 @@ __global__ void __launch_bounds__(256) kb_move_elem(const BatchEntry *__restrict
     batch_apply<BK_ELEM>(tab, map, [&](const BatchEntry &b, long e) { move_elem_body<M>(b.P, b.ck, b.A, e); });
 }
 
-template <typename SV>
+template <int M>
 __global__ void __launch_bounds__(256) kb_dens(const BatchEntry *__restrict__ tab, BatchMap map)
 {
-    batch_apply<BK_ELEM>(tab, map, [&](const BatchEntry &b, long e) { dens_body<SV>(b.P, b.ck, b.A, e); });
+    batch_apply<BK_ELEM>(tab, map, [&](const BatchEntry &b, long e) { dens_body<M>(b.P, b.ck, b.A, e, 0); });
 }
 
 __global__ void __launch_bounds__(256) kb_rmass(const BatchEntry *__restrict__ tab, BatchMap map)

@@ -2592,9 +2592,13 @@ __global__ void __launch_bounds__(256) k_init_visc(const GfDevParams *__restrict
 
 // ------------------------------------------------------------------ rmasses
 // src/matprops.f90:3-83 (Eff_dens) per element into a scratch plane
-template <typename SV>
-GF_DEV void dens_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e)
+// rot: the last element move's rotation is pending; the loaded stress is rotated as
+// k_move_rot would store it, for the pressure only (the move stays pending)
+template <int M>
+GF_DEV void dens_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e, int rot)
 {
+    typedef typename Prec<M>::SV SV;
+    typedef typename Prec<M>::AT AT;
     const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
     const double adi_grad = 3.0e-1, den_ramp_dist = 30.0e3, amp410 = 0.05, amp660 = 0.085;
     const int iph = A.iphase[e];
@@ -2611,6 +2615,20 @@ GF_DEV void dens_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const G
         sv[k][0] = bldt<SV>(bS, eo * (unsigned)sizeof(SV), (0 + 4 * k) * ne * (unsigned)sizeof(SV));
         sv[k][1] = bldt<SV>(bS, eo * (unsigned)sizeof(SV), (1 + 4 * k) * ne * (unsigned)sizeof(SV));
         sv[k][2] = bldt<SV>(bS, eo * (unsigned)sizeof(SV), (3 + 4 * k) * ne * (unsigned)sizeof(SV));
+    }
+    if (rot) {  // move_rot_body's increments and rotation
+        double xc[5], yc[5], xv[5], yv[5];
+        move_load_corners<SV>(P, A, j, i, xc, yc, xv, yv);
+        const double dtm = ck->dt_move;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const double w = move_dw12(xc, yc, xv, yv, k, move_det(xc, yc, k), dtm);
+            AT t11 = sv[k][0], t22 = sv[k][1];
+            AT t12 = bldt<SV>(bS, eo * (unsigned)sizeof(SV), (2 + 4 * k) * ne * (unsigned)sizeof(SV));
+            move_rotate<AT>(t11, t22, t12, AT(w));
+            sv[k][0] = SV(t11);
+            sv[k][1] = SV(t22);
+        }
     }
     const ViscIn vin = visc_inputs(P, A, j, i);
     double zcord = 0.25 * (vin.y[0] + vin.y[1] + vin.y[2] + vin.y[3]);
@@ -2649,12 +2667,12 @@ GF_DEV void dens_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const G
     if (PHASE(iph).is_mantle) ed = ed * den_amp;
     A.dens[e] = ed;
 }
-template <typename SV>
-__global__ void __launch_bounds__(256) k_dens(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+template <int M>
+__global__ void __launch_bounds__(256) k_dens(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A, int rot)
 {
     long e = xcd_lin();
     if (e >= NE) return;
-    dens_body<SV>(P, ck, A, e);
+    dens_body<M>(P, ck, A, e, rot);
 }
 
 // The reference scatters each element's 4 triangles to their nodes in 2x2 colour

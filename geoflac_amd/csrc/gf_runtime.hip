@@ -666,13 +666,19 @@ static int enq_clock(gf_ctx *ctx, int op)
 
 // applies a pending stress/strain rotation (k_move_area ran without it) before
 // anything but k_rheol1 reads stress or strain
-// the pending areas and dvol of the last move, and then its pending rotation
+// the pending areas and dvol of the last move
+static int flush_area(gf_ctx *ctx)
+{
+    if (!ctx->area_pending) return GF_OK;
+    ctx->area_pending = false;
+    LAUNCH(k_move_area, ctx->ne, ctx->dp, ctx->dck, ctx->a, 0);  // (the move's dt is recorded)
+    return GF_OK;
+}
+// ... and then its pending rotation
 static int flush_rot(gf_ctx *ctx)
 {
-    if (ctx->area_pending) {
-        ctx->area_pending = false;
-        LAUNCH(k_move_area, ctx->ne, ctx->dp, ctx->dck, ctx->a, 0);  // (the move's dt is recorded)
-    }
+    int rc;
+    if ((rc = flush_area(ctx))) return rc;
     if (!ctx->rot_pending) return GF_OK;
     ctx->rot_pending = false;
     LAUNCH_M(k_move_rot, ctx->ne, ctx->dp, ctx->dck, ctx->a);
@@ -816,8 +822,11 @@ static int enq_move(gf_ctx *ctx)
         LAUNCH_M(k_move_elem, ctx->ne, ctx->dp, ctx->dck, ctx->a);
     } else if (fused) {
         // the areas and dvol wait for the next fl_rheol (or the first kernel that
-        // needs them); the fused fl_node recorded the move's dt
-        ctx->area_pending = true;
+        // needs them) unless this step's rmasses or dt_mass read them next; the fused
+        // fl_node recorded the move's dt
+        const bool masses = ctx->nloop % ctx->p.ifreq_rmasses == 0 || ctx->nloop % ctx->p.ifreq_imasses == 0;
+        if (masses) LAUNCH(k_move_area, ctx->ne, ctx->dp, ctx->dck, ctx->a, 0);
+        ctx->area_pending = !masses;
         ctx->rot_pending = true;
     } else {
         LAUNCH(k_move_area, ctx->ne, ctx->dp, ctx->dck, ctx->a, 1);
@@ -829,8 +838,16 @@ static int enq_move(gf_ctx *ctx)
 static int enq_rmasses(gf_ctx *ctx)
 {
     int rc;
-    if ((rc = flush_rot(ctx))) return rc;  // Eff_dens reads the stress
-    LAUNCH_SV(k_dens, ctx->ne, ctx->dp, ctx->dck, ctx->a);
+    if ((rc = flush_area(ctx))) return rc;  // the masses read the areas
+    // Eff_dens rotates the stress it reads by a pending move itself (the move stays
+    // pending for the next fl_rheol)
+    const int rot = ctx->rot_pending ? 1 : 0;
+    if (ctx->precision == GF_PREC_MIXED)
+        LAUNCH_NAMED("k_dens<float>", k_dens<1>, ctx->ne, ctx->dp, ctx->dck, ctx->a, rot);
+    else if (ctx->precision == GF_PREC_MIXED_STORAGE)
+        LAUNCH_NAMED("k_dens<float>", k_dens<2>, ctx->ne, ctx->dp, ctx->dck, ctx->a, rot);
+    else
+        LAUNCH_NAMED("k_dens<double>", k_dens<0>, ctx->ne, ctx->dp, ctx->dck, ctx->a, rot);
     LAUNCH(k_rmass, ctx->nn, ctx->dp, ctx->dck, ctx->a);
     return GF_OK;
 }
@@ -838,7 +855,7 @@ static int enq_rmasses(gf_ctx *ctx)
 static int enq_dt_mass(gf_ctx *ctx)
 {
     int rc;
-    if ((rc = flush_rot(ctx))) return rc;  // the dt minima and inertial masses read the areas
+    if ((rc = flush_area(ctx))) return rc;  // the dt minima and inertial masses read the areas
     if ((rc = enq_clock(ctx, CK_DT_BEGIN))) return rc;
     LAUNCH_SV(k_dt_reduce, (long)DT_RED_BLOCKS * 256, ctx->dp, ctx->dck, ctx->a);
     hipLaunchKernelGGL(k_dt_scalars, dim3(1), dim3(64), 0, ctx->stream, ctx->dp, ctx->dck, ctx->d_present,
@@ -1553,7 +1570,7 @@ static int enq_batch_flac(gf_batch *b)
         BLAUNCH(kb_move_cord<SV>, bgrid(b->tot_n, 256), 256, b->dtab, map);
         BTIMED(2, BLAUNCH(kb_move_elem<M>, bgrid(b->tot_e, b->bs[2]), b->bs[2], b->dtab, map));
         if (b->nloop % p.ifreq_rmasses == 0) {
-            BLAUNCH(kb_dens<SV>, bgrid(b->tot_e, 256), 256, b->dtab, map);
+            BLAUNCH(kb_dens<M>, bgrid(b->tot_e, 256), 256, b->dtab, map);
             BLAUNCH(kb_rmass, bgrid(b->tot_n, 256), 256, b->dtab, map);
         }
         if (b->nloop % p.ifreq_imasses == 0) {
