@@ -1869,7 +1869,7 @@ GF_DEV void stress_avg(const GfDevParams *P, const GfDevArrays &A, long e, doubl
 template <typename SV, typename AT = SV>
 GF_DEV void visc_load(const GfDevParams *__restrict__ P, const GfDevArrays &A, long e, int j, int i, int &iph,
                       double (&prv)[GF_MAXACT], double &a11, double &a22, double &a33, ViscIn &vin,
-                      const double *dw = nullptr)
+                      bool rotate = false, const double (&dw)[4] = {0.0, 0.0, 0.0, 0.0})
 {
     const unsigned ne = (unsigned)NE, eo = (unsigned)e;
     iph = A.iphase[e];
@@ -1884,7 +1884,7 @@ GF_DEV void visc_load(const GfDevParams *__restrict__ P, const GfDevArrays &A, l
     for (int c = 0; c < 3; c++)
 #pragma unroll
         for (int k = 0; k < 4; k++) sv[c][k] = bldt<SV>(bS, eo * (unsigned)sizeof(SV), (comp[c] + 4 * k) * ne * (unsigned)sizeof(SV));
-    if (dw) {
+    if (rotate) {
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             AT t11 = sv[0][k], t22 = sv[1][k];
@@ -1908,6 +1908,17 @@ GF_DEV void visc_load(const GfDevParams *__restrict__ P, const GfDevArrays &A, l
 }
 GF_DEV bool visc_rheology(int irh) { return irh == 3 || irh >= 11; }  // the Maxwell rheologies
 
+// the rotation increments of a pending element move (move_rot_body's)
+template <int M>
+GF_DEV void move_dw_elem(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, int j, int i,
+                         double (&dw)[4])
+{
+    double xc[5], yc[5], xv[5], yv[5];
+    move_load_corners<typename Prec<M>::SV>(P, A, j, i, xc, yc, xv, yv);
+    const double dt = ck->dt_move;
+#pragma unroll
+    for (int k = 0; k < 4; k++) dw[k] = move_dw12(xc, yc, xv, yv, k, move_det(xc, yc, k), dt);
+}
 // rot: the last element move's rotation is pending (it stays pending: the stress is
 // rotated here for the average only)
 template <int M>
@@ -1919,15 +1930,9 @@ GF_DEV void visc_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const G
     int iph;
     double prv[GF_MAXACT], a11, a22, a33;
     ViscIn vin;
-    double dw[4];
-    if (rot) {  // move_rot_body's increments
-        double xc[5], yc[5], xv[5], yv[5];
-        move_load_corners<SV>(P, A, j, i, xc, yc, xv, yv);
-        const double dt = ck->dt_move;
-#pragma unroll
-        for (int k = 0; k < 4; k++) dw[k] = move_dw12(xc, yc, xv, yv, k, move_det(xc, yc, k), dt);
-    }
-    visc_load<SV, AT>(P, A, e, j, i, iph, prv, a11, a22, a33, vin, rot ? dw : nullptr);
+    double dw[4] = {0.0, 0.0, 0.0, 0.0};
+    if (rot) move_dw_elem<M>(P, ck, A, j, i, dw);
+    visc_load<SV, AT>(P, A, e, j, i, iph, prv, a11, a22, a33, vin, rot != 0, dw);
     if (!visc_rheology(PHASE(iph).irheol)) return;
     A.visn[e] = dev_eff_visc_in(P, j, i, a11, a22, a33, prv, vin);
 }

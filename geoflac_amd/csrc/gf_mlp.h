@@ -453,13 +453,15 @@ GF_DEV double mlp_visc(const GfDevParams *P, float lg)
 #define MLP_LIST_AUDIT 0x80000000u  // list entry flag: an audited element
 #define MLP_MAXG 2048               // workgroups of the persistent viscosity grid, at most
 
-template <typename SV>
+template <int M>
 __global__ void __launch_bounds__(256, 4) k_visc_mlp(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
                                                   const float *__restrict__ w, const MlpCfg *__restrict__ cfg,
                                                   const MlpStats *st, unsigned int *__restrict__ list,
                                                   float *__restrict__ lval, unsigned int *__restrict__ cnt,
-                                                  MlpWgStats *__restrict__ wgs, long cap)
+                                                  MlpWgStats *__restrict__ wgs, long cap, int rot)
 {
+    typedef typename Prec<M>::SV SV;
+    typedef typename Prec<M>::AT AT;
     __shared__ MlpLds L;
     __shared__ mlp_row sx[4][64];
     __shared__ unsigned int s_len, s_ev[4], s_fb[4];
@@ -480,7 +482,9 @@ __global__ void __launch_bounds__(256, 4) k_visc_mlp(const GfDevParams *__restri
         int iph = 1;
         double prv[GF_MAXACT] = {}, a11 = 0.0, a22 = 0.0, a33 = 0.0;
         ViscIn vin{};
-        if (in) visc_load<SV>(P, A, e, j, i, iph, prv, a11, a22, a33, vin);
+        double dw[4] = {0.0, 0.0, 0.0, 0.0};
+        if (in && rot) move_dw_elem<M>(P, ck, A, j, i, dw);  // a pending move's rotation
+        if (in) visc_load<SV, AT>(P, A, e, j, i, iph, prv, a11, a22, a33, vin, rot != 0, dw);
         const int irh = in ? PHASE(iph).irheol : 0;
         const bool visc = in && visc_rheology(irh);
         float x[MLP_NIN];
@@ -523,14 +527,16 @@ __global__ void __launch_bounds__(256, 4) k_visc_mlp(const GfDevParams *__restri
     }
 }
 
-template <typename SV>
+template <int M>
 __global__ void __launch_bounds__(256) k_visc_list(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
                                                    const MlpCfg *__restrict__ cfg, MlpStats *st,
                                                    const unsigned int *__restrict__ list,
                                                    const float *__restrict__ lval,
                                                    const unsigned int *__restrict__ cnt,
-                                                   MlpWgStats *__restrict__ wgs, long cap, int nseg)
+                                                   MlpWgStats *__restrict__ wgs, long cap, int nseg, int rot)
 {
+    typedef typename Prec<M>::SV SV;
+    typedef typename Prec<M>::AT AT;
     __shared__ unsigned int s_aud, s_fail, s_max;
     __shared__ unsigned long long s_worst;
     __shared__ unsigned int s_pre[MLP_MAXG + 1];  // exclusive prefix sums of the segment lengths
@@ -586,7 +592,9 @@ __global__ void __launch_bounds__(256) k_visc_list(const GfDevParams *__restrict
         int iph;
         double prv[GF_MAXACT], a11, a22, a33;
         ViscIn vin;
-        visc_load<SV>(P, A, e, j, i, iph, prv, a11, a22, a33, vin);
+        double dw[4] = {0.0, 0.0, 0.0, 0.0};
+        if (rot) move_dw_elem<M>(P, ck, A, j, i, dw);
+        visc_load<SV, AT>(P, A, e, j, i, iph, prv, a11, a22, a33, vin, rot != 0, dw);
         const double ex = dev_eff_visc_in(P, j, i, a11, a22, a33, prv, vin);
         A.visn[e] = ex;
         if (item & MLP_LIST_AUDIT) {

@@ -740,19 +740,29 @@ static int enq_rheol(gf_ctx *ctx, bool clock_done = false)
     // the viscosity update reads the rotated step-start stress: k_visc rotates what it
     // loads itself (the move stays pending for k_rheol1); the surrogate's passes and
     // the quad rheology find the move completed
-    if (((do_visc && ctx->sur_on) || ctx->rheol_quad) && (rc = flush_rot(ctx))) return rc;
+    if (ctx->rheol_quad && (rc = flush_rot(ctx))) return rc;
     const int rot = (ctx->rot_pending ? RH_ROT : 0) | (ctx->area_pending ? RH_AREA : 0);
     ctx->rot_pending = false;
     ctx->area_pending = false;
     if (do_visc && ctx->sur_on) {
         // the same persistent grid for both passes (a few workgroups per CU)
         const long pers = (long)ctx->sur_grid * 256;
-        LAUNCH_SV(k_visc_mlp, pers, ctx->dp, ctx->dck, ctx->a, ctx->sur_w, ctx->sur_cfg, ctx->sur_stats,
-                  ctx->sur_list, ctx->sur_lval, ctx->sur_cnt, ctx->sur_wgs, ctx->sur_seg);
-        // the exact formula on the entries: three workgroups per CU (its register use)
-        LAUNCH_SV(k_visc_list, std::min<long>(pers, (long)ctx->therm_grid * 3 * 256), ctx->dp, ctx->dck, ctx->a,
-                  ctx->sur_cfg, ctx->sur_stats, ctx->sur_list, ctx->sur_lval, ctx->sur_cnt, ctx->sur_wgs,
-                  ctx->sur_seg, ctx->sur_grid);
+        const int vr = (rot & RH_ROT) ? 1 : 0;
+        const long lgrid = std::min<long>(pers, (long)ctx->therm_grid * 3 * 256);
+#define SUR_LAUNCH(MM, NAME)                                                                                    \
+    do {                                                                                                        \
+        LAUNCH_NAMED("k_visc_mlp<" NAME ">", k_visc_mlp<MM>, pers, ctx->dp, ctx->dck, ctx->a, ctx->sur_w,       \
+                     ctx->sur_cfg, ctx->sur_stats, ctx->sur_list, ctx->sur_lval, ctx->sur_cnt, ctx->sur_wgs,     \
+                     ctx->sur_seg, vr);                                                                         \
+        /* the exact formula on the entries: three workgroups per CU (its register use) */                     \
+        LAUNCH_NAMED("k_visc_list<" NAME ">", k_visc_list<MM>, lgrid, ctx->dp, ctx->dck, ctx->a, ctx->sur_cfg,  \
+                     ctx->sur_stats, ctx->sur_list, ctx->sur_lval, ctx->sur_cnt, ctx->sur_wgs, ctx->sur_seg,    \
+                     ctx->sur_grid, vr);                                                                        \
+    } while (0)
+        if (mixed) SUR_LAUNCH(1, "float");
+        else if (mstore) SUR_LAUNCH(2, "float");
+        else SUR_LAUNCH(0, "double");
+#undef SUR_LAUNCH
     } else if (do_visc) {
         const int vr = (rot & RH_ROT) ? 1 : 0;
         if (mixed) LAUNCH_NAMED("k_visc<float>", k_visc<1>, ctx->ne, ctx->dp, ctx->dck, ctx->a, vr);
