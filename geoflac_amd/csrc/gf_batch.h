@@ -109,16 +109,18 @@ __global__ void __launch_bounds__(256) kb_therm_bc(const BatchEntry *__restrict_
 }
 
 template <int M>
-__global__ void __launch_bounds__(256) kb_visc(const BatchEntry *__restrict__ tab, BatchMap map)
+__global__ void __launch_bounds__(256) kb_visc(const BatchEntry *__restrict__ tab, BatchMap map, int rot)
 {
-    batch_apply<BK_ELEM>(tab, map, [&](const BatchEntry &b, long e) { visc_body<M>(b.P, b.ck, b.A, e, 0); });
+    batch_apply<BK_ELEM>(tab, map, [&](const BatchEntry &b, long e) { visc_body<M>(b.P, b.ck, b.A, e, rot); });
 }
 
+// rot: the batch's last element move left its stress/strain rotation to this kernel
 template <bool SHEARH, int M>
-__global__ void __launch_bounds__(256) kb_rheol(const BatchEntry *__restrict__ tab, BatchMap map, int avgsr)
+__global__ void __launch_bounds__(256) kb_rheol(const BatchEntry *__restrict__ tab, BatchMap map, int avgsr,
+                                                int rot)
 {
     batch_apply<BK_ELEM>(tab, map, [&](const BatchEntry &b, long e) {
-        rheol_single<SHEARH, typename Prec<M>::SV, typename Prec<M>::AT>(b.P, b.P->ph, b.ck, b.A, e, avgsr, 0);
+        rheol_single<SHEARH, typename Prec<M>::SV, typename Prec<M>::AT>(b.P, b.P->ph, b.ck, b.A, e, avgsr, rot);
     });
 }
 
@@ -139,11 +141,16 @@ __global__ void __launch_bounds__(256) kb_move_elem(const BatchEntry *__restrict
 {
     batch_apply<BK_ELEM>(tab, map, [&](const BatchEntry &b, long e) { move_elem_body<M>(b.P, b.ck, b.A, e); });
 }
+// the areas, dvol and negative-area STOP of the move (the rotation is left to kb_rheol)
+__global__ void __launch_bounds__(256) kb_move_area(const BatchEntry *__restrict__ tab, BatchMap map)
+{
+    batch_apply<BK_ELEM>(tab, map, [&](const BatchEntry &b, long e) { move_area_body(b.P, b.ck, b.A, e, 1); });
+}
 
 template <int M>
-__global__ void __launch_bounds__(256) kb_dens(const BatchEntry *__restrict__ tab, BatchMap map)
+__global__ void __launch_bounds__(256) kb_dens(const BatchEntry *__restrict__ tab, BatchMap map, int rot)
 {
-    batch_apply<BK_ELEM>(tab, map, [&](const BatchEntry &b, long e) { dens_body<M>(b.P, b.ck, b.A, e, 0); });
+    batch_apply<BK_ELEM>(tab, map, [&](const BatchEntry &b, long e) { dens_body<M>(b.P, b.ck, b.A, e, rot); });
 }
 
 __global__ void __launch_bounds__(256) kb_rmass(const BatchEntry *__restrict__ tab, BatchMap map)

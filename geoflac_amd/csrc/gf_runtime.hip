@@ -1395,8 +1395,12 @@ struct gf_batch {
     long tot_e = 0, tot_n = 0, tot_t = 0, tot_b = 0;
     hipStream_t stream = nullptr;
     int32_t nloop = 0, nsrate = 0;
-    int bs[3] = {256, 256, 256};  // threads per workgroup of the rheology, node and element-move kernels
+    int bs[3] = {256, 256, 256};  // threads per workgroup of the rheology, node and element-area kernels
     bool use_graph = false;
+    // the batch's last element move left its rotation to the next kb_rheol (kb_visc and
+    // kb_dens rotate what they read themselves); g_rot0: the same at the start of the
+    // captured cycle
+    bool rot_pending = false, g_rot0 = false;
     hipGraphExec_t gexec = nullptr;
     int32_t g_nsrate0 = -99, g_nsrate1 = 0;  // nsrate before and after the captured 10 steps
     // optional per-kernel timing (autotuner)
@@ -1569,8 +1573,10 @@ static int enq_batch_flac(gf_batch *b)
             avgsr = 1;
         else if (b->nsrate == -1)
             avgsr = 2;
-        if (do_visc) BLAUNCH(kb_visc<M>, bgrid(b->tot_e, 256), 256, b->dtab, map);
-        BTIMED(0, BLAUNCH((kb_rheol<SH, M>), bgrid(b->tot_e, b->bs[0]), b->bs[0], b->dtab, map, avgsr));
+        const int rot = b->rot_pending ? RH_ROT : 0;
+        if (do_visc) BLAUNCH(kb_visc<M>, bgrid(b->tot_e, 256), 256, b->dtab, map, rot);
+        BTIMED(0, BLAUNCH((kb_rheol<SH, M>), bgrid(b->tot_e, b->bs[0]), b->bs[0], b->dtab, map, avgsr, rot));
+        b->rot_pending = false;
         if (avgsr == 1) {
             BLAUNCH(kb_clock, gm, 64, b->dtab, map, (int)CK_DTAVG_RESET);
             b->nsrate = 0;
@@ -1578,9 +1584,10 @@ static int enq_batch_flac(gf_batch *b)
         b->nsrate = b->nsrate + 1;
         BTIMED(1, BLAUNCH(kb_node<M>, bgrid(b->tot_n, b->bs[1]), b->bs[1], b->dtab, map));
         BLAUNCH(kb_move_cord<SV>, bgrid(b->tot_n, 256), 256, b->dtab, map);
-        BTIMED(2, BLAUNCH(kb_move_elem<M>, bgrid(b->tot_e, b->bs[2]), b->bs[2], b->dtab, map));
+        BTIMED(2, BLAUNCH(kb_move_area, bgrid(b->tot_e, b->bs[2]), b->bs[2], b->dtab, map));
+        b->rot_pending = true;
         if (b->nloop % p.ifreq_rmasses == 0) {
-            BLAUNCH(kb_dens<M>, bgrid(b->tot_e, 256), 256, b->dtab, map);
+            BLAUNCH(kb_dens<M>, bgrid(b->tot_e, 256), 256, b->dtab, map, b->rot_pending ? 1 : 0);
             BLAUNCH(kb_rmass, bgrid(b->tot_n, 256), 256, b->dtab, map);
         }
         if (b->nloop % p.ifreq_imasses == 0) {
@@ -1622,8 +1629,8 @@ static bool graph_cycle_ok(const gf_batch *b)
            10 % p.ifreq_imasses == 0 && p.ifreq_avgsr == 10;
 }
 
-// a model stepped on its own may hold a pending rotation: apply it on the model's
-// stream before the batch's stream touches the arrays
+// a model may hold a pending move (its own, or the one a previous batch call left):
+// complete it on the model's stream before the batch's stream touches the arrays
 static int batch_flush(gf_batch *b)
 {
     for (gf_ctx *c : b->m) {
@@ -1632,6 +1639,7 @@ static int batch_flush(gf_batch *b)
         if (rc == GF_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = GF_ERR_HIP;
         if (rc) return bset_err(b, rc, c->msg);
     }
+    b->rot_pending = false;
     return GF_OK;
 }
 
@@ -1642,12 +1650,13 @@ extern "C" int gf_batch_step(gf_batch *b, int32_t nsteps)
     if ((rc = batch_flush(b))) return rc;
     for (int32_t n = 0; n < nsteps;) {
         if (b->use_graph && !b->timing && nsteps - n >= 10 && graph_cycle_ok(b)) {
-            if (!b->gexec || b->g_nsrate0 != b->nsrate) {
+            if (!b->gexec || b->g_nsrate0 != b->nsrate || b->g_rot0 != b->rot_pending) {
                 if (b->gexec) {
                     BCHK(hipGraphExecDestroy(b->gexec));
                     b->gexec = nullptr;
                 }
                 const int32_t ns0 = b->nsrate, nl0 = b->nloop;
+                const bool rot0 = b->rot_pending;
                 hipGraph_t g = nullptr;
                 BCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
                 for (int k = 0; k < 10; k++)
@@ -1662,12 +1671,15 @@ extern "C" int gf_batch_step(gf_batch *b, int32_t nsteps)
                 if (ie != hipSuccess) return bset_err(b, GF_ERR_HIP, "hipGraphInstantiate failed");
                 b->g_nsrate0 = ns0;
                 b->g_nsrate1 = b->nsrate;
+                b->g_rot0 = rot0;
                 b->nloop = nl0;  // the capture only recorded the launches
                 b->nsrate = ns0;
+                b->rot_pending = rot0;
             }
             BCHK(hipGraphLaunch(b->gexec, b->stream));
             b->nloop += 10;
             b->nsrate = b->g_nsrate1;
+            b->rot_pending = true;  // the cycle ends with a move
             n += 10;
             continue;
         }
@@ -1679,6 +1691,7 @@ extern "C" int gf_batch_step(gf_batch *b, int32_t nsteps)
     for (gf_ctx *c : b->m) {
         c->nloop = b->nloop;
         c->nsrate = b->nsrate;
+        c->rot_pending = b->rot_pending;  // completed by the model's own k_move_rot where needed
         int r = fetch_clock(c, nullptr);
         if (r && first == GF_OK) first = r;
     }
@@ -1715,6 +1728,7 @@ extern "C" int gf_batch_autotune(gf_batch *b, int32_t steps, int32_t chosen[3])
     for (gf_ctx *c : b->m) {
         c->nloop = b->nloop;
         c->nsrate = b->nsrate;
+        c->rot_pending = b->rot_pending;
     }
     for (int q = 0; q < 3; q++) {
         b->bs[q] = pick[q];

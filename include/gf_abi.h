@@ -229,7 +229,7 @@ int gf_batch_step(gf_batch *b, int32_t nsteps);
  * captured once into a HIP graph and replayed. */
 int gf_batch_set_graph(gf_batch *b, int32_t on);
 /* Times `steps` real steps for each workgroup size in {64, 128, 256} and keeps the
- * fastest for the rheology, node and element-move kernels (chosen[3]). */
+ * fastest for the rheology, node and element-area kernels (chosen[3]). */
 int gf_batch_autotune(gf_batch *b, int32_t steps, int32_t chosen[3]);
 
 #ifdef __cplusplus
