@@ -75,7 +75,7 @@ __global__ void __launch_bounds__(1024) kb_therm_tile(const BatchEntry *__restri
 // the prefetch on
 template <int XIN, bool DIS>
 __global__ void __launch_bounds__(1024) kb_therm_pipe(const BatchEntry *__restrict__ tab, BatchMap map, long total,
-                                                      int pass, double stol)
+                                                      int pass, double stol, int arp)
 {
     __shared__ ThermRaw raw;
     __shared__ double sU[TH_NF * TH_NE];
@@ -86,7 +86,7 @@ __global__ void __launch_bounds__(1024) kb_therm_pipe(const BatchEntry *__restri
     auto entry = [&](long t) -> const BatchEntry & { return tab[__builtin_amdgcn_readfirstlane(map.chunk[BK_TILE][t])]; };
     {
         const BatchEntry &b = entry(first);
-        therm_store_raw<XIN, DIS>(therm_prefetch<XIN, DIS>(b.P, b.ck, b.A, pass, (int)(first - b.pt)), raw);
+        therm_store_raw<XIN, DIS>(therm_prefetch<XIN, DIS>(b.P, b.ck, b.A, pass, (int)(first - b.pt), arp), raw);
     }
     for (long t = first; t < end; t += RW) {
         lds_barrier();  // this tile's inputs are in; the previous tile is done with sU
@@ -94,7 +94,7 @@ __global__ void __launch_bounds__(1024) kb_therm_pipe(const BatchEntry *__restri
         const long tn = t + RW;
         const BatchEntry &bn = entry(tn < end ? tn : t);
         therm_tile_pipe<XIN, DIS>(b.P, b.ck, b.A, pass, (int)(t - b.pt), bn.P, bn.ck, bn.A,
-                                  tn < end ? (int)(tn - bn.pt) : -1, raw, sU, sFlag, stol);
+                                  tn < end ? (int)(tn - bn.pt) : -1, raw, sU, sFlag, stol, arp);
     }
 }
 
@@ -130,10 +130,14 @@ __global__ void __launch_bounds__(256) kb_node(const BatchEntry *__restrict__ ta
     batch_apply<BK_NODE>(tab, map, [&](const BatchEntry &b, long t) { node_body<M>(b.P, b.ck, b.A, t); });
 }
 
+// also records each model's move dt (for a rotation and areas left pending)
 template <typename SV>
 __global__ void __launch_bounds__(256) kb_move_cord(const BatchEntry *__restrict__ tab, BatchMap map)
 {
-    batch_apply<BK_NODE>(tab, map, [&](const BatchEntry &b, long t) { move_cord_body<SV>(b.P, b.ck, b.A, t); });
+    batch_apply<BK_NODE>(tab, map, [&](const BatchEntry &b, long t) {
+        move_cord_body<SV>(b.P, b.ck, b.A, t);
+        if (t == 0) b.ck->dt_move = b.ck->dt;
+    });
 }
 
 template <int M>
@@ -144,7 +148,12 @@ __global__ void __launch_bounds__(256) kb_move_elem(const BatchEntry *__restrict
 // the areas, dvol and negative-area STOP of the move (the rotation is left to kb_rheol)
 __global__ void __launch_bounds__(256) kb_move_area(const BatchEntry *__restrict__ tab, BatchMap map)
 {
-    batch_apply<BK_ELEM>(tab, map, [&](const BatchEntry &b, long e) { move_area_body(b.P, b.ck, b.A, e, 1); });
+    batch_apply<BK_ELEM>(tab, map, [&](const BatchEntry &b, long e) { move_area_body(b.P, b.ck, b.A, e, 0); });
+}
+// the negative-area STOP of moves whose areas are still pending (coordinates only)
+__global__ void __launch_bounds__(256) kb_area_check(const BatchEntry *__restrict__ tab, BatchMap map)
+{
+    batch_apply<BK_ELEM>(tab, map, [&](const BatchEntry &b, long e) { area_check_body(b.P, b.ck, b.A, e); });
 }
 
 template <int M>

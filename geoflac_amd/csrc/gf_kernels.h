@@ -2489,10 +2489,8 @@ GF_DEV void move_area_body(const GfDevParams *__restrict__ P, GfDevClock *ck, co
 }
 // STOP 40 of a move whose areas are still pending, from the moved coordinates (the
 // areas themselves are formed later): the same test as move_area_body, no stores
-__global__ void __launch_bounds__(256) k_area_check(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+GF_DEV void area_check_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e)
 {
-    long e = xcd_lin();
-    if (e >= NE) return;
     const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
     double xc[5], yc[5];
     move_load_corners<double>(P, A, j, i, xc, yc, nullptr, nullptr);
@@ -2500,6 +2498,12 @@ __global__ void __launch_bounds__(256) k_area_check(const GfDevParams *__restric
 #pragma unroll
     for (int k = 0; k < 4; k++) bad |= (1.0 / move_det(xc, yc, k) <= 0);
     if (bad) raise_error(ck, GF_STOP_NEGATIVE_AREA, e);
+}
+__global__ void __launch_bounds__(256) k_area_check(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+{
+    long e = xcd_lin();
+    if (e >= NE) return;
+    area_check_body(P, ck, A, e);
 }
 // rec_dt: record the move's dt (0 when completing a move whose dt is recorded already)
 __global__ void __launch_bounds__(256) k_move_area(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,

@@ -1401,6 +1401,10 @@ struct gf_batch {
     // kb_dens rotate what they read themselves); g_rot0: the same at the start of the
     // captured cycle
     bool rot_pending = false, g_rot0 = false;
+    // ... and its areas and dvol to the next kb_rheol (the thermal tiles before it form
+    // the areas they need), in FP32-arithmetic mode, whose thermal step has no fallback
+    // passes; g_area0: the same at the start of the captured cycle
+    bool area_pending = false, g_area0 = false, g_area_end = false;
     hipGraphExec_t gexec = nullptr;
     int32_t g_nsrate0 = -99, g_nsrate1 = 0;  // nsrate before and after the captured 10 steps
     // optional per-kernel timing (autotuner)
@@ -1559,7 +1563,7 @@ static int enq_batch_flac(gf_batch *b)
         // with one workgroup per tile that returns at once for a proven model
         if (pass == 0)
             BLAUNCH((dis ? kb_therm_pipe<0, true> : kb_therm_pipe<0, false>), b->therm_grid, TH_J * TH_I,
-                    b->dtab, map, b->tot_t, pass, npass == 1 ? GF_THERM_STOL : 0.0);
+                    b->dtab, map, b->tot_t, pass, npass == 1 ? GF_THERM_STOL : 0.0, b->area_pending ? 1 : 0);
         else
             BLAUNCH(kb_therm_tile, (unsigned)std::min<long>(b->tot_t, 512), TH_J * TH_I, b->dtab, map, b->tot_t, pass);
         if (npass > 1) BLAUNCH(kb_therm_check, bgrid(b->tot_b, 256), 256, b->dtab, map, pass);
@@ -1573,10 +1577,11 @@ static int enq_batch_flac(gf_batch *b)
             avgsr = 1;
         else if (b->nsrate == -1)
             avgsr = 2;
-        const int rot = b->rot_pending ? RH_ROT : 0;
-        if (do_visc) BLAUNCH(kb_visc<M>, bgrid(b->tot_e, 256), 256, b->dtab, map, rot);
+        const int rot = (b->rot_pending ? RH_ROT : 0) | (b->area_pending ? RH_AREA : 0);
+        if (do_visc) BLAUNCH(kb_visc<M>, bgrid(b->tot_e, 256), 256, b->dtab, map, (rot & RH_ROT) ? 1 : 0);
         BTIMED(0, BLAUNCH((kb_rheol<SH, M>), bgrid(b->tot_e, b->bs[0]), b->bs[0], b->dtab, map, avgsr, rot));
         b->rot_pending = false;
+        b->area_pending = false;
         if (avgsr == 1) {
             BLAUNCH(kb_clock, gm, 64, b->dtab, map, (int)CK_DTAVG_RESET);
             b->nsrate = 0;
@@ -1584,7 +1589,11 @@ static int enq_batch_flac(gf_batch *b)
         b->nsrate = b->nsrate + 1;
         BTIMED(1, BLAUNCH(kb_node<M>, bgrid(b->tot_n, b->bs[1]), b->bs[1], b->dtab, map));
         BLAUNCH(kb_move_cord<SV>, bgrid(b->tot_n, 256), 256, b->dtab, map);
-        BTIMED(2, BLAUNCH(kb_move_area, bgrid(b->tot_e, b->bs[2]), b->bs[2], b->dtab, map));
+        const bool masses = b->nloop % p.ifreq_rmasses == 0 || b->nloop % p.ifreq_imasses == 0;
+        if (npass == 1 && !masses)
+            b->area_pending = true;
+        else
+            BTIMED(2, BLAUNCH(kb_move_area, bgrid(b->tot_e, b->bs[2]), b->bs[2], b->dtab, map));
         b->rot_pending = true;
         if (b->nloop % p.ifreq_rmasses == 0) {
             BLAUNCH(kb_dens<M>, bgrid(b->tot_e, 256), 256, b->dtab, map, b->rot_pending ? 1 : 0);
@@ -1640,6 +1649,7 @@ static int batch_flush(gf_batch *b)
         if (rc) return bset_err(b, rc, c->msg);
     }
     b->rot_pending = false;
+    b->area_pending = false;
     return GF_OK;
 }
 
@@ -1650,13 +1660,14 @@ extern "C" int gf_batch_step(gf_batch *b, int32_t nsteps)
     if ((rc = batch_flush(b))) return rc;
     for (int32_t n = 0; n < nsteps;) {
         if (b->use_graph && !b->timing && nsteps - n >= 10 && graph_cycle_ok(b)) {
-            if (!b->gexec || b->g_nsrate0 != b->nsrate || b->g_rot0 != b->rot_pending) {
+            if (!b->gexec || b->g_nsrate0 != b->nsrate || b->g_rot0 != b->rot_pending ||
+                b->g_area0 != b->area_pending) {
                 if (b->gexec) {
                     BCHK(hipGraphExecDestroy(b->gexec));
                     b->gexec = nullptr;
                 }
                 const int32_t ns0 = b->nsrate, nl0 = b->nloop;
-                const bool rot0 = b->rot_pending;
+                const bool rot0 = b->rot_pending, area0 = b->area_pending;
                 hipGraph_t g = nullptr;
                 BCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
                 for (int k = 0; k < 10; k++)
@@ -1671,27 +1682,34 @@ extern "C" int gf_batch_step(gf_batch *b, int32_t nsteps)
                 if (ie != hipSuccess) return bset_err(b, GF_ERR_HIP, "hipGraphInstantiate failed");
                 b->g_nsrate0 = ns0;
                 b->g_nsrate1 = b->nsrate;
+                b->g_area_end = b->area_pending;
                 b->g_rot0 = rot0;
+                b->g_area0 = area0;
                 b->nloop = nl0;  // the capture only recorded the launches
                 b->nsrate = ns0;
                 b->rot_pending = rot0;
+                b->area_pending = area0;
             }
             BCHK(hipGraphLaunch(b->gexec, b->stream));
             b->nloop += 10;
             b->nsrate = b->g_nsrate1;
             b->rot_pending = true;  // the cycle ends with a move
+            b->area_pending = b->g_area_end;
             n += 10;
             continue;
         }
         if ((rc = enq_batch_step(b))) return rc;
         n++;
     }
+    if (b->area_pending)  // the negative-area STOP of the last move, now
+        BLAUNCH(kb_area_check, bgrid(b->tot_e, 256), 256, b->dtab, b->map);
     BCHK(hipStreamSynchronize(b->stream));
     int first = GF_OK;
     for (gf_ctx *c : b->m) {
         c->nloop = b->nloop;
         c->nsrate = b->nsrate;
         c->rot_pending = b->rot_pending;  // completed by the model's own k_move_rot where needed
+        c->area_pending = b->area_pending;
         int r = fetch_clock(c, nullptr);
         if (r && first == GF_OK) first = r;
     }
@@ -1718,8 +1736,8 @@ extern "C" int gf_batch_autotune(gf_batch *b, int32_t steps, int32_t chosen[3])
         for (int k = 0; k < steps && rc == GF_OK; k++) rc = enq_batch_step(b);
         b->timing = false;
         if (rc) return rc;
-        for (int q = 0; q < 3; q++)
-            if (b->t_ms[q] < best[q]) {
+        for (int q = 0; q < 3; q++)  // (a kernel that did not run in these steps keeps 256)
+            if (b->t_ms[q] > 0.0 && b->t_ms[q] < best[q]) {
                 best[q] = b->t_ms[q];
                 pick[q] = cand[c];
             }
@@ -1729,6 +1747,7 @@ extern "C" int gf_batch_autotune(gf_batch *b, int32_t steps, int32_t chosen[3])
         c->nloop = b->nloop;
         c->nsrate = b->nsrate;
         c->rot_pending = b->rot_pending;
+        c->area_pending = b->area_pending;
     }
     for (int q = 0; q < 3; q++) {
         b->bs[q] = pick[q];
