@@ -109,6 +109,38 @@ def test_kernel_variants_bitwise_equal_to_oracle(env, monkeypatch):
 
 
 @pytest.mark.gpu
+def test_pending_move_completion_points_bitwise_equal_to_oracle():
+    """A step leaves its element move's areas, dvol and rotation to the next step's
+    fl_rheol (the thermal step before it forms the areas it needs itself; the
+    viscosity update and Eff_dens rotate what they read).  Single subroutines right
+    after a step -- fl_therm, fl_rheol, dt_mass, rmasses, fl_node, fl_move -- and steps
+    across viscosity and mass updates stay bitwise equal to the oracle."""
+    from geoflac_amd import gpu
+    seq = [("step", 1), ("fl_therm", 0), ("fl_rheol", 0), ("step", 2), ("dt_mass", 0), ("step", 1),
+           ("rmasses", 0), ("step", 1), ("fl_node", 0), ("step", 1), ("fl_move", 0), ("step", 13),
+           ("fl_therm", 0), ("step", 1)]
+    _say("case: single subroutines after steps with a pending move, from the 10-step golden state of %s: %s"
+         % (CASE, seq))
+    p = ol.golden_params(CASE)
+    st = golden_full(10)
+    ref = st.copy()
+    with gpu.Model(p, st) as m:
+        for what, n in seq:
+            if what == "step":
+                m.step(n)
+                assert ol.step(p, ref, n, cr=True) == 0
+            else:
+                m.call(what)
+                assert ol.call("gfo_" + what, p, ref, cr=True) == 0
+        m.download()
+    bad, worst, nexact = compare(st, ref, tol=0.0)
+    _say("  exact fields %d, differing %s" % (nexact, worst))
+    assert not bad, bad
+    assert st.clock.nloop == ref.clock.nloop
+    assert st.clock.dt == ref.clock.dt
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("nsteps", [1, 3])
 def test_negative_area_stop(nsteps):
     """A node pushed far beyond its neighbours in one step turns its elements inside
