@@ -91,6 +91,40 @@ def test_batch_mixed_precision_equals_single_runs():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("precision", [abi.PREC_FP64, abi.PREC_MIXED])
+def test_batch_then_single_steps_equal_single_runs(precision):
+    """A batch leaves its last element move pending on the models (the rotation, and in
+    FP32-arithmetic mode the areas); stepping the models on their own afterwards, and
+    batching them again, ends bit for bit where single runs end."""
+    n1, n2, n3 = 13, 4, 10
+    _say("case: precision %d, batch %d steps (graph), single %d, batch %d" % (precision, n1, n2, n3))
+    models, singles = [], []
+    for q in _variants():
+        gp, st, m = sweep.make_model(q, precision)
+        models.append(m)
+    for q in _variants():
+        gp, st, m = sweep.make_model(q, precision)
+        m.step(n1 + n2 + n3)
+        m.download()
+        singles.append(st)
+        m.close()
+    with sweep.Batch(models) as b:
+        b.set_graph(True)
+        b.step(n1)
+    for m in models:
+        m.step(n2)
+    with sweep.Batch(models) as b:
+        b.set_graph(True)
+        b.step(n3)
+    for m, ref in zip(models, singles):
+        m.download()
+        diff = [f for f in state.State.FIELDS if not np.array_equal(getattr(m.state, f), getattr(ref, f))]
+        _say("  nloop %d, differing fields %s" % (m.state.clock.nloop, diff))
+        assert not diff
+        m.close()
+
+
+@pytest.mark.gpu
 def test_thermal_fallback_passes_bitwise_equal_to_oracle():
     """Cells 1.7 times taller than wide (200x40 elements over the 300x100 km box)
     couple neighbouring columns strongly, so that the thermal tiles' halos do not
