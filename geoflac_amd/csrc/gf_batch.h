@@ -41,14 +41,34 @@ GF_DEV long bcount(const BatchEntry &b)
     return F == BK_ELEM ? b.ce : F == BK_NODE ? b.cn : F == BK_TILE ? b.ct : b.cb;
 }
 
+// the entry of model m (m uniform across the workgroup), read through the constant
+// address space: the table is never written while a kernel runs, so its fields load
+// with scalar loads and the buffer descriptors built from them stay uniform (read as
+// ordinary global memory, the compiler cannot prove them uniform once the kernel has
+// stored anything and wraps every buffer access in a loop over distinct values)
+typedef const __attribute__((address_space(4))) BatchEntry ConstBatchEntry;
+GF_DEV const BatchEntry &bentry(const BatchEntry *tab, int m)
+{
+    return *(const BatchEntry *)&((ConstBatchEntry *)tab)[m];
+}
+
+// workgroup k is dispatched to XCD k % 8: give each XCD one contiguous run of the
+// index range (the first G % 8 XCDs one workgroup more), so that the node rows two
+// neighbouring workgroups share stay in one L2 (a bijection for any grid size G)
+GF_DEV unsigned xcd_block()
+{
+    const unsigned G = gridDim.x, k = blockIdx.x, x = k % 8, q = G / 8, r = G % 8;
+    return x * q + min(x, r) + k / 8;
+}
+
 // calls fn(entry, local index) for the global index of this thread (workgroups of at
 // most BATCH_ALIGN threads, dividing it)
 template <int F, typename Fn>
 GF_DEV void batch_apply(const BatchEntry *__restrict__ tab, const BatchMap &map, Fn fn)
 {
-    const long g0 = (long)blockIdx.x * blockDim.x;
+    const long g0 = (long)xcd_block() * blockDim.x;
     const int m = __builtin_amdgcn_readfirstlane(map.chunk[F][g0 / BATCH_ALIGN]);
-    const BatchEntry &b = tab[m];
+    const BatchEntry &b = bentry(tab, m);
     const long l = g0 + threadIdx.x - bprefix<F>(b);
     if (l >= bcount<F>(b)) return;
     fn(b, l);
@@ -64,7 +84,7 @@ __global__ void __launch_bounds__(1024) kb_therm_tile(const BatchEntry *__restri
                                                       int pass)
 {
     for (long g = blockIdx.x; g < total; g += gridDim.x) {
-        const BatchEntry &b = tab[__builtin_amdgcn_readfirstlane(map.chunk[BK_TILE][g])];
+        const BatchEntry &b = bentry(tab, __builtin_amdgcn_readfirstlane(map.chunk[BK_TILE][g]));
         if (pass > 0 && b.ck->therm_pass_changed[pass - 1] == 0u) continue;  // this model is proven
         therm_tile_one(b.P, b.ck, b.A, pass, (int)(g - b.pt));
     }
@@ -83,7 +103,7 @@ __global__ void __launch_bounds__(1024) kb_therm_pipe(const BatchEntry *__restri
     const long q = (total + 7) / 8, x = blockIdx.x % 8, r = blockIdx.x / 8, RW = gridDim.x / 8;
     const long first = x * q + r, end = min((x + 1) * q, total);
     if (first >= end) return;
-    auto entry = [&](long t) -> const BatchEntry & { return tab[__builtin_amdgcn_readfirstlane(map.chunk[BK_TILE][t])]; };
+    auto entry = [&](long t) -> const BatchEntry & { return bentry(tab, __builtin_amdgcn_readfirstlane(map.chunk[BK_TILE][t])); };
     {
         const BatchEntry &b = entry(first);
         therm_store_raw<XIN, DIS>(therm_prefetch<XIN, DIS>(b.P, b.ck, b.A, pass, (int)(first - b.pt), arp), raw);
@@ -179,7 +199,7 @@ __global__ void __launch_bounds__(256) kb_dt_reduce(const BatchEntry *__restrict
 {
     const int m = blockIdx.x / BATCH_DT_BLOCKS;
     if (m >= map.nm) return;
-    const BatchEntry &b = tab[m];
+    const BatchEntry &b = bentry(tab, m);
     dt_reduce_body<SV>(b.P, b.ck, b.A, blockIdx.x % BATCH_DT_BLOCKS, BATCH_DT_BLOCKS);
 }
 
