@@ -269,15 +269,33 @@ __global__ void __launch_bounds__(256) k_mlp_grad(const float *__restrict__ w, c
     }
 }
 
-// gradient (and loss) = sum of the workgroups' partials, in workgroup order
+// gradient (and loss) = sum of the workgroups' partials, MLP_RED_PARAMS parameters per
+// workgroup: wave q adds the partials of workgroups q, q+4, q+8, ... and the four sums
+// are added in wave order (a fixed order: the result does not depend on timing)
+#define MLP_RED_PARAMS 64
 __global__ void __launch_bounds__(256) k_mlp_reduce(const float *__restrict__ partial, int nblocks,
                                                     float *__restrict__ grad)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p > MLP_NP) return;
+    __shared__ float part[4][MLP_RED_PARAMS];
+    const int l = threadIdx.x % MLP_RED_PARAMS, q = threadIdx.x / MLP_RED_PARAMS;
+    const int p = blockIdx.x * MLP_RED_PARAMS + l;
     float s = 0.0f;
-    for (int b = 0; b < nblocks; b++) s += partial[(long)b * (MLP_NP + 1) + p];
-    grad[p] = s;
+    if (p <= MLP_NP)
+        for (int b = q; b < nblocks; b += 4) s += partial[(long)b * (MLP_NP + 1) + p];
+    part[q][l] = s;
+    __syncthreads();
+    if (q == 0 && p <= MLP_NP) grad[p] = ((part[0][l] + part[1][l]) + part[2][l]) + part[3][l];
+}
+
+// every stride-th sample, packed (the training set-up's statistics read them on the
+// host in one contiguous copy): rows * MLP_SW floats out, row r from sample r * stride
+__global__ void __launch_bounds__(256) k_sample_stride(float *__restrict__ dst, const float *__restrict__ src,
+                                                       long rows, long stride)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= rows * MLP_SW) return;
+    const long r = t / MLP_SW;
+    dst[t] = src[r * stride * MLP_SW + (t - r * MLP_SW)];
 }
 
 // Adam (Kingma and Ba, 2015) on the FP32 master weights

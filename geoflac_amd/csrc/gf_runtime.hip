@@ -1117,7 +1117,7 @@ static int sur_train_step(gf_ctx *ctx, long nsamples, long first, long batch, lo
     grad_grid(batch, nb, per_block);
     hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(256), 0, s, ctx->sur_w, ctx->sur_samples, nsamples, first, batch,
                        pa, pc, ctx->sur_cfg, per_block, ctx->sur_partial);
-    hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_NP + 1 + 255) / 256), dim3(256), 0, s, ctx->sur_partial, nb,
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_NP + MLP_RED_PARAMS) / MLP_RED_PARAMS), dim3(256), 0, s, ctx->sur_partial, nb,
                        ctx->sur_grad);
     hipLaunchKernelGGL(k_mlp_adam, dim3((MLP_NP + 255) / 256), dim3(256), 0, s, ctx->sur_w, ctx->sur_m, ctx->sur_v,
                        ctx->sur_grad, lr, 0.9f, 0.999f, 1.0e-8f, step);
@@ -1145,10 +1145,19 @@ extern "C" int gf_surrogate_train(gf_ctx *ctx, int32_t steps, int32_t batch, flo
     // normalisation and gate box from a strided subsample (at most 2^20 samples)
     const long stride = std::max(1L, ns / (1L << 20));
     std::vector<float> sub;
-    {
-        std::vector<float> all((size_t)MLP_SW * ((ns + stride - 1) / stride));
-        HIPCHK(hipMemcpy2D(all.data(), MLP_SW * sizeof(float), ctx->sur_samples, stride * MLP_SW * sizeof(float),
-                           MLP_SW * sizeof(float), (ns + stride - 1) / stride, hipMemcpyDeviceToHost));
+    {   // packed on the device first: a pitched copy of 2^20 short rows to the host takes seconds
+        const long rows = (ns + stride - 1) / stride;  // the last row read is sample (rows-1)*stride < ns
+        std::vector<float> all((size_t)MLP_SW * rows);
+        float *packed = nullptr;
+        HIPCHK(hipMalloc((void **)&packed, sizeof(float) * MLP_SW * rows));
+        hipLaunchKernelGGL(k_sample_stride, dim3((unsigned)((rows * MLP_SW + 255) / 256)), dim3(256), 0, ctx->stream,
+                           packed, (const float *)ctx->sur_samples, rows, stride);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(all.data(), packed, sizeof(float) * MLP_SW * rows, hipMemcpyDeviceToHost,
+                                                ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        (void)hipFree(packed);
+        HIPCHK(e);
         sub.swap(all);
     }
     const long m = (long)sub.size() / MLP_SW;
@@ -1344,7 +1353,7 @@ extern "C" int gf_mlp_grad(const float *x, const float *t, int64_t n, const floa
     } else {
         hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(256), 0, 0, dw, ds, (long)n, 0L, (long)n, 1L, 0L, dc, per_block,
                            dp);
-        hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_NP + 1 + 255) / 256), dim3(256), 0, 0, dp, nb, dg);
+        hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_NP + MLP_RED_PARAMS) / MLP_RED_PARAMS), dim3(256), 0, 0, dp, nb, dg);
         if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
             hipMemcpy(out.data(), dg, sizeof(float) * (MLP_NP + 1), hipMemcpyDeviceToHost) != hipSuccess)
             rc = GF_ERR_HIP;
