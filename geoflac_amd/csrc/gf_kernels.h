@@ -1619,7 +1619,7 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, const GfPhase *__res
         AT a11 = AT(0.25) * (AT(0.0) + sr[0][0] + sr[1][0] + sr[2][0] + sr[3][0]);
         AT a22 = AT(0.25) * (AT(0.0) + sr[0][1] + sr[1][1] + sr[2][1] + sr[3][1]);
         AT a12 = AT(0.25) * (AT(0.0) + sr[0][2] + sr[1][2] + sr[2][2] + sr[3][2]);
-        AT srII = AT(0.5) * sqrt((a11 - a22) * (a11 - a22) + AT(4.0) * a12 * a12);
+        AT srII = AT(0.5) * asqrt((a11 - a22) * (a11 - a22) + AT(4.0) * a12 * a12);
         AT srI = AT(0.5) * (a11 + a22);
         AT srs2 = (a11 - srI) * (a11 - srI) + (a22 - srI) * (a22 - srI) + AT(2.0) * a12 * a12;
         // the time integrals stay FP64 on both paths
@@ -1629,9 +1629,9 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, const GfPhase *__res
         s11 = AT(0.25) * (AT(0.0) + st[0][0] + st[1][0] + st[2][0] + st[3][0]);
         s22 = AT(0.25) * (AT(0.0) + st[0][1] + st[1][1] + st[2][1] + st[3][1]);
         s12 = AT(0.25) * (AT(0.0) + st[0][2] + st[1][2] + st[2][2] + st[3][2]);
-        AT stII = AT(0.5) * sqrt((s11 - s22) * (s11 - s22) + AT(4.0) * s12 * s12);
+        AT stII = AT(0.5) * asqrt((s11 - s22) * (s11 - s22) + AT(4.0) * s12 * s12);
         sshr = sshr_in;
-        if (srII != AT(0.0)) sshr = sshr + stII / srII * srs2 * dt;
+        if (srII != AT(0.0)) sshr = sshr + adiv(stII, srII) * srs2 * dt;
     }
     const GfPhase &ph = phs[iph - 1];
     const int irh = rp.irheol;
@@ -1720,18 +1720,19 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, const GfPhase *__res
     if (!ok) raise_error(ck, GF_STOP_MAXWELL, e);
     if (P->mix_stress == 1) {
         const AT a0 = AT(ar[0]), a1 = AT(ar[1]), a2 = AT(ar[2]), a3 = AT(ar[3]);
-        AT quad_area = AT(1.0) / a0 + AT(1.0) / a1;
+        // FP32 arithmetic: the same area-weighted mean times a0 a1 / a0 a1, one division
+        AT quad_area = dev_form ? AT(0) : AT(1.0) / a0 + AT(1.0) / a1;
         AT s0a = AT(0.5) * (st[0][0] + st[0][1]);
         AT s0b = AT(0.5) * (st[1][0] + st[1][1]);
-        AT s0 = (s0a / a1 + s0b / a0) / quad_area;
+        AT s0 = dev_form ? adiv(s0a * a0 + s0b * a1, a0 + a1) : (s0a / a1 + s0b / a0) / quad_area;
         st[0][0] = st[0][0] - s0a + s0;
         st[0][1] = st[0][1] - s0a + s0;
         st[1][0] = st[1][0] - s0b + s0;
         st[1][1] = st[1][1] - s0b + s0;
-        quad_area = AT(1.0) / a2 + AT(1.0) / a3;
+        quad_area = dev_form ? AT(0) : AT(1.0) / a2 + AT(1.0) / a3;
         s0a = AT(0.5) * (st[2][0] + st[2][1]);
         s0b = AT(0.5) * (st[3][0] + st[3][1]);
-        s0 = (s0a / a3 + s0b / a2) / quad_area;
+        s0 = dev_form ? adiv(s0a * a2 + s0b * a3, a2 + a3) : (s0a / a3 + s0b / a2) / quad_area;
         st[2][0] = st[2][0] - s0a + s0;
         st[2][1] = st[2][1] - s0a + s0;
         st[3][0] = st[3][0] - s0b + s0;
