@@ -7,7 +7,8 @@ project) on the inputs in tests/golden/inputs, serially, and stores:
   <case>_s0.npz        the complete state after setflac (step 0)
   <case>_s<N>.npz      the fields the step changes, after N steps
 
-Only run in the build container (the GPU box has no reference).
+Only run where the reference sources and oracle/_ref are present; the committed
+fixtures are what the tests read elsewhere.
 usage: python tests/golden/make_golden.py [case ...]
 """
 import os

@@ -1,6 +1,6 @@
 """The HIP path (through the C-ABI) against the reference's golden states and the oracle.
 
-Bar: rel-L2 <= 1e-10 per field against the reference; the
+Bar: rel-L2 <= 1e-10 per field against the reference (the FP64 drop-in's bar); the
 tests also report how many fields are bit-identical.  All GPU tests print the
 case they run before touching the device.
 """
