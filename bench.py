@@ -243,7 +243,7 @@ def main():
                          "kernel of known traffic that scales FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (one model per device); "
+        print("bench.py: this benchmark runs on one GPU only (one model per device); "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 

@@ -199,7 +199,7 @@ extern "C" int gf_create(const gf_params *p, int32_t precision, gf_ctx **out)
     if (p->nx < 3 || p->nz < 3 || p->nphase < 1 || p->nphase > GF_MAXPH) return GF_ERR_ARG;
     if (precision != GF_PREC_FP64 && precision != GF_PREC_MIXED && precision != GF_PREC_MIXED_STORAGE)
         return GF_ERR_UNSUPPORTED;
-    // paths of the reference this build does not provide (see the header)
+    // paths of the reference this library does not provide (listed in gf_abi.h)
     if (p->itype_melting != 0 || p->nystressbc != 0 || p->ndim != 2) return GF_ERR_UNSUPPORTED;
     // the flexural base and surface processes of fl_move (src/fl_move.f90:37-40)
     if (p->flexure_base != 0 || p->topo_kappa > 0.0) return GF_ERR_UNSUPPORTED;
@@ -905,7 +905,7 @@ static int enq_flac(gf_ctx *ctx, int pending = 0)
     if ((rc = enq_clock(ctx, pending | CK_THERM_BEGIN | (mech ? CK_DTAVG_ADD : 0)))) return rc;
     if ((rc = enq_therm(ctx, true))) return rc;
     if (mech) {
-        // change_phase (every ifreq_rmasses) has no rule for the phases this build accepts
+        // change_phase (every ifreq_rmasses) has no rule for the phases this library accepts
         if ((rc = enq_rheol(ctx, true))) return rc;
         if ((rc = enq_node(ctx, true))) return rc;
         if ((rc = enq_move(ctx))) return rc;
@@ -1027,7 +1027,7 @@ extern "C" int gf_kernel_time(gf_ctx *ctx, const char *name, double *total_ms, i
 }
 
 // Device evaluation of the correctly rounded functions of the FP64 path for n host
-// arguments (fn: 0 sin, 1 tan, 2 exp, 3 pow(x, y)); a check of this build, no
+// arguments (fn: 0 sin, 1 tan, 2 exp, 3 pow(x, y)); a self-check of the library, no
 // counterpart in the reference.
 extern "C" int gf_math_check(int32_t fn, const double *x, const double *y, double *out, int64_t n)
 {

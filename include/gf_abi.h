@@ -43,7 +43,7 @@ extern "C" {
 #define GF_STOP_NEGATIVE_AREA 40  /* src/fl_move.f90:125   */
 #define GF_ERR_HIP 1000           /* a HIP runtime call failed */
 #define GF_ERR_ARG 1001           /* bad argument (null pointer, size) */
-#define GF_ERR_UNSUPPORTED 1002   /* parameter selects a path this build does not provide */
+#define GF_ERR_UNSUPPORTED 1002   /* parameter selects a path this library does not provide */
 #define GF_ERR_STATE 1003         /* call out of order (e.g. step before upload) */
 #define GF_ERR_THERM_SWEEPS 1004  /* thermal fixed-point sweeps did not converge bitwise */
 
@@ -163,11 +163,11 @@ int gf_sync(gf_ctx *ctx);
 /* Per-kernel timing with HIP events recorded on the context's stream around each
  * launch (off by default).  gf_kernel_time returns the summed duration in ms and the
  * launch count of kernel `name` (e.g. "k_rheol") since timing was switched on.
- * Instrumentation of this build; the reference has none. */
+ * Instrumentation of this library; the reference has none. */
 int gf_set_timing(gf_ctx *ctx, int32_t on);
 /* A plain device copy of `bytes` bytes, `reps` times: the average duration of one
  * launch (the measured copy bandwidth; a kernel of known traffic, k_copy, to scale a
- * profiler's byte counters).  Instrumentation of this build; no reference counterpart. */
+ * profiler's byte counters).  Instrumentation of this library; no reference counterpart. */
 int gf_stream_probe(int64_t bytes, int32_t reps, double *ms_per_launch);
 /* Evaluates on the device, for n host arguments, the correctly rounded functions the
  * FP64 path uses in place of libm (fn: 0 sin, 1 tan, 2 exp, 3 pow(x, y)), so a test

@@ -1148,7 +1148,7 @@ int gfo_flac(const gf_params *p, gf_host_arrays *s, gf_clock *c)
     return 0;
 }
 
-/* the transcendental functions of this build (fn: 0 sin, 1 tan, 2 exp, 3 pow) */
+/* the transcendental functions of the library (fn: 0 sin, 1 tan, 2 exp, 3 pow) */
 double gfo_math(int fn, double x, double y)
 {
     switch (fn) {
