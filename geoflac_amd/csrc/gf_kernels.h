@@ -2251,7 +2251,6 @@ __global__ void __launch_bounds__(256) k_node(const GfDevParams *__restrict__ P,
 template <int M>
 struct NodeLds {
     typename Prec<M>::SV s[12][ND_NEL];  // stress component c of triangle kb at [c + 4 kb]
-    double a[4][ND_NEL];
     double pr[M != 0 ? ND_NEL : 1];
     double cx[ND_NND], cy[ND_NND];
 };
@@ -2259,14 +2258,18 @@ GF_DEV int node_ntj(const GfDevParams *P) { return (P->nz + ND_TJ - 1) / ND_TJ; 
 GF_DEV int node_nti(const GfDevParams *P) { return (P->nx + ND_TI - 1) / ND_TI; }
 // one thread's share of a tile's inputs: element values v = tid + 256 r (plane p = v /
 // 325, slot v % 325 = row + 65 column; planes 0..11: stress component p % 3 of
-// triangle p / 3, 12..15: areas, 16: the isotropic reference), node coordinates v =
-// tid + 256 r (plane v / 396, slot row + 66 column), and the thread's own node's values
+// triangle p / 3, 12: the isotropic reference), node coordinates v = tid + 256 r
+// (plane v / 396, slot row + 66 column), and the thread's own node's values: its
+// mass, velocity and conditions, and the areas of the three triangles of each of its
+// four elements that act on it (registers, not LDS: the smaller tile lets four
+// workgroups share a CU)
 template <int M>
 struct NodePf {
-    static constexpr int NV = (16 + (M != 0 ? 1 : 0)) * ND_NEL, NR = (NV + 255) / 256;
+    static constexpr int NV = (12 + (M != 0 ? 1 : 0)) * ND_NEL, NR = (NV + 255) / 256;
     static constexpr int NC = (2 * ND_NND + 255) / 256;
     double dv[NR];  // (FP32 stress values are held as doubles, exactly)
     double cv[NC];
+    double ar[4][3];
     NodeOwn own;
 };
 GF_DEV void node_tile_origin(const GfDevParams *P, int tile, int &j0, int &i0)
@@ -2298,8 +2301,6 @@ GF_DEV NodePf<M> node_prefetch(const GfDevParams *__restrict__ P, const GfDevArr
         if (p < 12)
             f.dv[r] = (double)bldt<SV>(bS, ok ? e * (unsigned)sizeof(SV) : OOB,
                                        (unsigned)(p % 3 + 4 * (p / 3)) * neu * (unsigned)sizeof(SV));
-        else if (p < 16)
-            f.dv[r] = bld(bA, ok ? e * 8u : OOB, (unsigned)(p - 12) * neu * 8u);
         else if (M != 0 && ok)
             f.dv[r] = ref_press<SV>(A, (long)e);
     }
@@ -2311,6 +2312,18 @@ GF_DEV NodePf<M> node_prefetch(const GfDevParams *__restrict__ P, const GfDevArr
         f.cv[r] = bld(bC, ok ? (unsigned)((jj - 1) + nz * (ii - 1)) * 8u : OOB, (unsigned)(p & 1) * nnu * 8u);
     }
     const int j = j0 + (int)threadIdx.x % ND_TJ, i = i0 + (int)threadIdx.x / ND_TJ;
+    // the areas the node's update reads: triangles kk of element el at (j + dje, i + die),
+    // 0 outside the mesh (as the staged tile held them)
+    constexpr int kk[4][3] = {{2, 3, 4}, {1, 2, 3}, {1, 2, 4}, {1, 3, 4}};
+    constexpr int dje[4] = {-1, -1, 0, 0}, die[4] = {-1, 0, -1, 0};
+#pragma unroll
+    for (int el = 0; el < 4; el++) {
+        const int je = j + dje[el], ie = i + die[el];
+        const bool ok = je >= 1 && je <= nz - 1 && ie >= 1 && ie <= nx - 1;
+        const unsigned e = (unsigned)((je - 1) + (nz - 1) * (ie - 1));
+#pragma unroll
+        for (int k = 0; k < 3; k++) f.ar[el][k] = bld(bA, ok ? e * 8u : OOB, (unsigned)(kk[el][k] - 1) * neu * 8u);
+    }
     f.own = NodeOwn{};
     if (j <= nz && i <= nx) f.own = node_own<SV>(P, A, (long)(j - 1) + (long)nz * (i - 1));
     return f;
@@ -2326,8 +2339,6 @@ GF_DEV void node_stage(const NodePf<M> &f, NodeLds<M> &L)
         if (v >= F::NV) continue;
         if (p < 12)
             L.s[p][sl] = SV(f.dv[r]);
-        else if (p < 16)
-            L.a[p - 12][sl] = f.dv[r];
         else if (M != 0)
             L.pr[sl] = f.dv[r];
     }
@@ -2346,7 +2357,8 @@ GF_DEV void node_stage(const NodePf<M> &f, NodeLds<M> &L)
 // moved node)
 template <int M, bool MOVE>
 GF_DEV void node_tile_update(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, int tile,
-                             const NodeLds<M> &L, const NodeOwn &own, const NodeClk &kc, double *cord_out)
+                             const NodeLds<M> &L, const NodeOwn &own, const double (&ar)[4][3], const NodeClk &kc,
+                             double *cord_out)
 {
     typedef typename Prec<M>::SV SV;
     int j0, i0;
@@ -2365,7 +2377,7 @@ GF_DEV void node_tile_update(const GfDevParams *__restrict__ P, GfDevClock *ck, 
         const int sl = (tj + dje[el] + 1) + ND_EJ * (ti + die[el] + 1);
         for (int k = 0; k < 3; k++) {
             const int kb = kk[el][k] - 1;
-            a[k] = L.a[kb][sl];
+            a[k] = ar[el][k];
             for (int c = 0; c < 3; c++) sv[k][c] = L.s[c + 3 * kb][sl];
         }
         p = M != 0 ? L.pr[sl] : 0.0;
@@ -2394,7 +2406,7 @@ __global__ void __launch_bounds__(256) k_node_t(const GfDevParams *__restrict__ 
     const NodePf<M> pf = node_prefetch<M>(P, A, tile);
     node_stage<M>(pf, L);
     __syncthreads();
-    node_tile_update<M, MOVE>(P, ck, A, tile, L, pf.own, kc, cord_out);
+    node_tile_update<M, MOVE>(P, ck, A, tile, L, pf.own, pf.ar, kc, cord_out);
 }
 
 // ------------------------------------------------------------------ fl_move
