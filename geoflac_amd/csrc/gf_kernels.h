@@ -48,6 +48,8 @@ template <> GF_DEV float *sv_ptr<float>(double *d, float *f) { return f; }
 #define NX (P->nx)
 #define NE ((long)(NZ - 1) * (NX - 1))
 #define NN ((long)NZ * NX)
+// element and node indices fit in 32 bits (gf_create refuses a mesh whose element
+// planes pass 4 GiB), so (j, i) come from 32-bit divisions
 #define IN(j, i) ((long)(j) - 1 + (long)NZ * ((i) - 1))
 #define IE(j, i) ((long)(j) - 1 + (long)(NZ - 1) * ((i) - 1))
 #define CORD(j, i, d) A.cord[IN(j, i) + NN * ((d) - 1)]
@@ -985,7 +987,7 @@ GF_DEV void therm_bc_body(const GfDevParams *__restrict__ P, GfDevClock *ck, con
         atomicAdd(&ck->therm_unconverged, 1ull);
     if (t == 0 && proven > 0) atomicAdd(&ck->therm_extra_passes, (unsigned long long)proven);
     if (t >= NN) return;
-    const int j = (int)(t % NZ) + 1, i = (int)(t / NZ) + 1;
+    const int j = (int)((unsigned)t % (unsigned)NZ) + 1, i = (int)((unsigned)t / (unsigned)NZ) + 1;
     // value of node (jj, ii) after the top/bottom conditions (:575-588)
     auto tb = [&](int jj, int ii) -> double {
         if (jj == 1) return P->t_top;
@@ -1927,7 +1929,7 @@ GF_DEV void visc_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const G
 {
     typedef typename Prec<M>::SV SV;
     typedef typename Prec<M>::AT AT;
-    const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
+    const int j = (int)((unsigned)e % (unsigned)(NZ - 1)) + 1, i = (int)((unsigned)e / (unsigned)(NZ - 1)) + 1;
     int iph;
     double prv[GF_MAXACT], a11, a22, a33;
     ViscIn vin;
@@ -2180,7 +2182,7 @@ template <int M>
 GF_DEV void node_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long t)
 {
     typedef typename Prec<M>::SV SV;
-    const int j = (int)(t % NZ) + 1, i = (int)(t / NZ) + 1;
+    const int j = (int)((unsigned)t % (unsigned)NZ) + 1, i = (int)((unsigned)t / (unsigned)NZ) + 1;
     constexpr int kk[4][3] = {{2, 3, 4}, {1, 2, 3}, {1, 2, 4}, {1, 3, 4}};
     // elements (j-1,i-1) (j-1,i) (j,i-1) (j,i) as offsets from the node
     constexpr int dje[4] = {-1, -1, 0, 0}, die[4] = {-1, 0, -1, 0};
@@ -2484,7 +2486,7 @@ GF_DEV void move_rotate_elem(const GfDevParams *__restrict__ P, const GfDevArray
 GF_DEV void move_area_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e,
                            int rec_dt = 1)
 {
-    const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
+    const int j = (int)((unsigned)e % (unsigned)(NZ - 1)) + 1, i = (int)((unsigned)e / (unsigned)(NZ - 1)) + 1;
     double xc[5], yc[5];
     move_load_corners<double>(P, A, j, i, xc, yc, nullptr, nullptr);
     bool bad = false;
@@ -2504,7 +2506,7 @@ GF_DEV void move_area_body(const GfDevParams *__restrict__ P, GfDevClock *ck, co
 // areas themselves are formed later): the same test as move_area_body, no stores
 GF_DEV void area_check_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e)
 {
-    const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
+    const int j = (int)((unsigned)e % (unsigned)(NZ - 1)) + 1, i = (int)((unsigned)e / (unsigned)(NZ - 1)) + 1;
     double xc[5], yc[5];
     move_load_corners<double>(P, A, j, i, xc, yc, nullptr, nullptr);
     bool bad = false;
@@ -2531,7 +2533,7 @@ __global__ void __launch_bounds__(256) k_move_area(const GfDevParams *__restrict
 template <int M>
 GF_DEV void move_rot_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e)
 {
-    const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
+    const int j = (int)((unsigned)e % (unsigned)(NZ - 1)) + 1, i = (int)((unsigned)e / (unsigned)(NZ - 1)) + 1;
     double xc[5], yc[5], xv[5], yv[5], dw[4];
     move_load_corners<typename Prec<M>::SV>(P, A, j, i, xc, yc, xv, yv);
     const double dt = ck->dt_move;
@@ -2551,7 +2553,7 @@ __global__ void __launch_bounds__(256) k_move_rot(const GfDevParams *__restrict_
 template <int M>
 GF_DEV void move_elem_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e)
 {
-    const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
+    const int j = (int)((unsigned)e % (unsigned)(NZ - 1)) + 1, i = (int)((unsigned)e / (unsigned)(NZ - 1)) + 1;
     const double dt = ck->dt;
     double xc[5], yc[5], xv[5], yv[5], dw[4];
     move_load_corners<typename Prec<M>::SV>(P, A, j, i, xc, yc, xv, yv);
@@ -2582,7 +2584,7 @@ __global__ void __launch_bounds__(256) k_init_areas(const GfDevParams *__restric
 {
     long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= NE) return;
-    const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
+    const int j = (int)((unsigned)e % (unsigned)(NZ - 1)) + 1, i = (int)((unsigned)e / (unsigned)(NZ - 1)) + 1;
     const double x1 = CORD(j, i, 1), y1 = CORD(j, i, 2), x2 = CORD(j + 1, i, 1), y2 = CORD(j + 1, i, 2);
     const double x3 = CORD(j, i + 1, 1), y3 = CORD(j, i + 1, 2), x4 = CORD(j + 1, i + 1, 1), y4 = CORD(j + 1, i + 1, 2);
     double det, a[4];
@@ -2604,7 +2606,7 @@ __global__ void __launch_bounds__(256) k_init_visc(const GfDevParams *__restrict
 {
     long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= NE) return;
-    const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
+    const int j = (int)((unsigned)e % (unsigned)(NZ - 1)) + 1, i = (int)((unsigned)e / (unsigned)(NZ - 1)) + 1;
     double prv[GF_MAXACT];
     for (int a = 0; a < P->nactive; a++) prv[a] = A.pr[a * NE + e];
     double a11, a22, a33;
@@ -2621,7 +2623,7 @@ GF_DEV void dens_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const G
 {
     typedef typename Prec<M>::SV SV;
     typedef typename Prec<M>::AT AT;
-    const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
+    const int j = (int)((unsigned)e % (unsigned)(NZ - 1)) + 1, i = (int)((unsigned)e / (unsigned)(NZ - 1)) + 1;
     const double adi_grad = 3.0e-1, den_ramp_dist = 30.0e3, amp410 = 0.05, amp660 = 0.085;
     const int iph = A.iphase[e];
     // every load issued before the first use
@@ -2735,7 +2737,7 @@ GF_DEV CornerSlots corner_slots(const GfDevParams *P, int j, int i)
 
 GF_DEV void rmass_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long t)
 {
-    const int j = (int)(t % NZ) + 1, i = (int)(t / NZ) + 1;
+    const int j = (int)((unsigned)t % (unsigned)NZ) + 1, i = (int)((unsigned)t / (unsigned)NZ) + 1;
     const double c1d12 = 1.0 / 12.0;
     const CornerSlots cs = corner_slots(P, j, i);
     const unsigned ne = (unsigned)NE;
@@ -2774,7 +2776,7 @@ GF_DEV double side_len(const GfDevParams *P, const GfDevArrays &A, int ja, int i
 // exact in any order, and their bit patterns order like the values.
 GF_DEV double elem_dlmin(const GfDevParams *P, const GfDevArrays &A, long t)
 {
-    const int j = (int)(t % (NZ - 1)) + 1, i = (int)(t / (NZ - 1)) + 1;
+    const int j = (int)((unsigned)t % (unsigned)(NZ - 1)) + 1, i = (int)((unsigned)t / (unsigned)(NZ - 1)) + 1;
     double dlmin = 1.0e28, dl;
     dl = side_len(P, A, j, i, j + 1, i);
     dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 1) * dl));
@@ -2891,7 +2893,7 @@ GF_DEV void amass_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const 
         A.amass[t] = A.rmass[t];
         return;
     }
-    const int j = (int)(t % NZ) + 1, i = (int)(t / NZ) + 1;
+    const int j = (int)((unsigned)t % (unsigned)NZ) + 1, i = (int)((unsigned)t / (unsigned)NZ) + 1;
     const double c1d12 = 1.0 / 12.0;
     const double dlmin = __longlong_as_double((long long)ck->dlmin_bits);
     const double vel_max = __longlong_as_double((long long)ck->velmax_bits);

@@ -134,7 +134,7 @@ __global__ void __launch_bounds__(256) k_visc_sample(const GfDevParams *__restri
 {
     long e = xcd_lin();
     if (e >= NE || base + e >= cap) return;
-    const int j = (int)(e % (NZ - 1)) + 1, i = (int)(e / (NZ - 1)) + 1;
+    const int j = (int)((unsigned)e % (unsigned)(NZ - 1)) + 1, i = (int)((unsigned)e / (unsigned)(NZ - 1)) + 1;
     double prv[GF_MAXACT];
     for (int a = 0; a < P->nactive; a++) prv[a] = A.pr[a * NE + e];
     double a11, a22, a33;
