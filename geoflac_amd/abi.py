@@ -17,6 +17,7 @@ GF_ERR_ARG = 1001
 GF_ERR_UNSUPPORTED = 1002
 GF_ERR_STATE = 1003
 GF_ERR_THERM_SWEEPS = 1004
+GF_ERR_PHASES = 1005
 
 PREC_FP64 = 0
 PREC_MIXED = 1
@@ -50,7 +51,7 @@ class GfParams(C.Structure):
     _fields_ = ([(n, _I) for n in _INT_FIELDS]
                 + [("irheol", _I * MAXPH)]
                 + [(n, _I * MAXINH) for n in ("ix1", "ix2", "iy1", "iy2", "igeom")]
-                + [("flexure_base", _I)]
+                + [("flexure_base", _I), ("nelem_serp", _I), ("pad_i", _I)]
                 + [(n, _D) for n in _REAL_FIELDS]
                 + [(n, _D * MAXPH) for n in PHASE_TABLES]
                 + [("topo_kappa", _D)])
@@ -58,6 +59,8 @@ class GfParams(C.Structure):
 
 _DP = C.POINTER(C.c_double)
 _IP = C.POINTER(C.c_int32)
+_FP = C.POINTER(C.c_float)
+_BP = C.POINTER(C.c_int8)
 
 HOST_FIELDS = [
     ("cord", _DP), ("vel", _DP), ("temp", _DP), ("temp0", _DP), ("force", _DP),
@@ -69,7 +72,16 @@ HOST_FIELDS = [
 
 
 class GfHostArrays(C.Structure):
-    _fields_ = HOST_FIELDS
+    _fields_ = HOST_FIELDS + [("coh_scale", _FP)]
+
+
+MAX_MARKERS_PER_ELEM = 32  # src/marker_data.f90:5
+MARKER_FIELDS = [("mark_a1", _DP), ("mark_a2", _DP), ("mark_ntriag", _IP), ("mark_phase", _BP),
+                 ("mark_dead", _BP), ("mark_id_elem", _IP), ("nmark_elem", _IP)]
+
+
+class GfMarkerArrays(C.Structure):
+    _fields_ = [("nmarkers", _I), ("pad", _I)] + MARKER_FIELDS
 
 
 class GfClock(C.Structure):
@@ -80,6 +92,8 @@ class GfClock(C.Structure):
 # field bits, in the order of HOST_FIELDS
 FIELD_BIT = {name: 1 << k for k, (name, _) in enumerate(HOST_FIELDS)}
 GF_ALL = (1 << len(HOST_FIELDS)) - 1
+GF_MARKERS = 1 << 22
+GF_COH_SCALE = 1 << 23
 
 SIZEOF_PARAMS = C.sizeof(GfParams)
 SIZEOF_HOST = C.sizeof(GfHostArrays)

@@ -10,35 +10,34 @@
 // the models stay independent; the kernel bodies are the single-model ones.
 #pragma once
 #include "gf_kernels.h"
+#include "gf_phase.h"
 
 struct BatchEntry {
     const GfDevParams *P;
     GfDevClock *ck;
     GfDevArrays A;
-    long pe, pn, pt, pb;  // first global index: elements, nodes, thermal tiles, border entries
-    long ce, cn, ct, cb;  // counts of the same
-    const int *present;   // phases present (dt_mass minima)
-    int npresent, pad;
+    long pe, pn, pt, pb, pm;  // first global index: elements, nodes, thermal tiles, border entries, markers
+    long ce, cn, ct, cb, cm;  // counts of the same
 };
 
-enum { BK_ELEM = 0, BK_NODE = 1, BK_TILE = 2, BK_BORD = 3 };
+enum { BK_ELEM = 0, BK_NODE = 1, BK_TILE = 2, BK_BORD = 3, BK_MARK = 4 };
 #define BATCH_ALIGN 256  // each model's element, node and border ranges start at a multiple
 
 // model of each 256-index chunk of the element, node and border ranges, and of each tile
 struct BatchMap {
-    const int *chunk[4];
+    const int *chunk[5];
     int nm, pad;
 };
 
 template <int F>
 GF_DEV long bprefix(const BatchEntry &b)
 {
-    return F == BK_ELEM ? b.pe : F == BK_NODE ? b.pn : F == BK_TILE ? b.pt : b.pb;
+    return F == BK_ELEM ? b.pe : F == BK_NODE ? b.pn : F == BK_TILE ? b.pt : F == BK_BORD ? b.pb : b.pm;
 }
 template <int F>
 GF_DEV long bcount(const BatchEntry &b)
 {
-    return F == BK_ELEM ? b.ce : F == BK_NODE ? b.cn : F == BK_TILE ? b.ct : b.cb;
+    return F == BK_ELEM ? b.ce : F == BK_NODE ? b.cn : F == BK_TILE ? b.ct : F == BK_BORD ? b.cb : b.cm;
 }
 
 // the entry of model m (m uniform across the workgroup), read through the constant
@@ -123,9 +122,11 @@ __global__ void __launch_bounds__(256) kb_therm_check(const BatchEntry *__restri
     batch_apply<BK_BORD>(tab, map, [&](const BatchEntry &b, long t) { therm_check_body(b.P, b.ck, b.A, pass, t); });
 }
 
-__global__ void __launch_bounds__(256) kb_therm_bc(const BatchEntry *__restrict__ tab, BatchMap map, int passes)
+__global__ void __launch_bounds__(256) kb_therm_bc(const BatchEntry *__restrict__ tab, BatchMap map, int passes,
+                                                   int proof)
 {
-    batch_apply<BK_NODE>(tab, map, [&](const BatchEntry &b, long t) { therm_bc_body(b.P, b.ck, b.A, t, passes); });
+    batch_apply<BK_NODE>(tab, map,
+                         [&](const BatchEntry &b, long t) { therm_bc_body(b.P, b.ck, b.A, t, passes, proof); });
 }
 
 template <int M>
@@ -206,5 +207,29 @@ __global__ void __launch_bounds__(256) kb_dt_reduce(const BatchEntry *__restrict
 __global__ void kb_dt_scalars(const BatchEntry *__restrict__ tab, BatchMap map)
 {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m < map.nm) dt_scalars_body(tab[m].P, tab[m].ck, tab[m].present, tab[m].npresent);
+    if (m < map.nm) dt_scalars_body(tab[m].P, tab[m].ck);
+}
+
+// change_phase of every model with markers (gf_phase.h)
+__global__ void __launch_bounds__(256) kb_phase_rules(const BatchEntry *__restrict__ tab, BatchMap map)
+{
+    batch_apply<BK_MARK>(tab, map, [&](const BatchEntry &b, long kk) { phase_rules_body(b.P, b.ck, b.A, kk); });
+}
+// the plan writes the model's active planes into its parameter block
+__global__ void kb_phase_plan(const BatchEntry *__restrict__ tab, BatchMap map)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < map.nm && tab[m].A.phstamp) phase_plan_body(const_cast<GfDevParams *>(tab[m].P), tab[m].ck);
+}
+__global__ void __launch_bounds__(256) kb_phase_replane(const BatchEntry *__restrict__ tab, BatchMap map)
+{
+    batch_apply<BK_ELEM>(tab, map, [&](const BatchEntry &b, long e) {
+        if (b.A.phstamp && b.ck->ph_replan) phase_replane_body(b.P, b.ck, b.A, e);
+    });
+}
+__global__ void __launch_bounds__(256) kb_phase_recount(const BatchEntry *__restrict__ tab, BatchMap map)
+{
+    batch_apply<BK_ELEM>(tab, map, [&](const BatchEntry &b, long e) {
+        if (b.A.phstamp) phase_recount_body(b.P, b.ck, b.A, e);
+    });
 }

@@ -40,6 +40,7 @@ struct GfDevParams {
     int itherm, istress_therm, ishearh, itemp_bc;
     int nystressbc, nyhydro, i_prestress, idt_scale;
     int i_rey, ivis_present, iphsub, inhom;
+    int nelem_serp, pad_ns;
     int active[GF_MAXPH];  // phase ids (1-based) present anywhere with ratio >= 0.01
     int ix1[GF_MAXINH], ix2[GF_MAXINH], iy1[GF_MAXINH], iy2[GF_MAXINH], igeom[GF_MAXINH];
     double t_top, t_bot, bot_bc, g, demf, ten_off, tau_heal, v_min, v_max;
@@ -50,6 +51,8 @@ struct GfDevParams {
 
 // Time-stepping scalars and reduction cells kept on the device.
 #define GF_MAX_SWEEPS 32
+// phase-ratio planes kept per element: the phases present anywhere, in increasing id
+#define GF_MAXACT 8
 struct GfDevClock {
     double time, dt, dtavg, dt_elastic, dt_maxwell;
     unsigned long long dlmin_bits, velmax_bits, dtmax_therm_bits, dtmaxwell_bits, dte_bits;
@@ -58,6 +61,13 @@ struct GfDevClock {
     unsigned long long therm_unconverged, therm_extra_passes, therm_mismatch, therm_mismatch_ulp;
     unsigned int therm_pass_changed[GF_MAX_SWEEPS];  // bit 0: a tile border differs from its owner; bit 1: a tile did not settle
     double dt_move;             // dt of the last element move (its stress rotation may still be pending)
+    // change_phase (src/change_phase.f90:1-260) on the device
+    unsigned int ph_newmask;    // phases (bit id-1) that markers turned into in the current call
+    int ph_call;                // change_phase calls so far: the stamp of the current call's changed elements
+    int ph_replan, ph_nold;     // the phase-ratio planes were re-ordered in the current call; planes before
+    int ph_map[GF_MAXACT];      // new plane of each old plane (-1: dropped)
+    int ph_nz[GF_MAXACT];       // elements with a nonzero ratio, per plane
+    int ph_count[GF_MAXPH];     // elements whose iphase is id-1 (the dt_mass minima run over ids present)
 };
 
 // ---------------------------------------------------------------- buffer access

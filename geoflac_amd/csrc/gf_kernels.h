@@ -24,6 +24,16 @@ struct GfDevArrays {
     // part without the lithostatic one.  The double planes above are not allocated.
     float *stress0f, *velf, *forcef;
     double *pref;
+    // per-element factor on the mixed cohesion (seeded heterogeneity; null: none)
+    const float *coh_scale;
+    // markers (module marker_data, src/marker_data.f90:1-20); null when none are bound
+    double *mark_a1, *mark_a2;
+    int *mark_ntriag;
+    signed char *mark_phase, *mark_dead;
+    int *mark_id_elem;    // GF_MAX_MARKERS_PER_ELEM per element, element-major, 1-based
+    int *nmark_elem;
+    int *phstamp;         // per element: the change_phase call that last changed one of its markers
+    long nmarkers;
 };
 
 // the isotropic reference added back to an SV stress (FP64 path: none)
@@ -970,10 +980,12 @@ __global__ void __launch_bounds__(256) k_therm_check(const GfDevParams *__restri
 }
 
 // Boundary conditions src/fl_therm.f90:575-598 applied to the proven iterate.
-// passes: TH_MAXPASS when the tiles' result is proven to be the serial one, 1 when
-// the single tile pass is taken as it is (FP32-arithmetic mode)
+// passes: the fixed-point passes that ran; proof: their results carry the border
+// proof (FP64), and a call that no pass proved raises GF_ERR_THERM_SWEEPS (its last
+// pass is used, but the result is not known to be the serial one); without proof the
+// single tile pass is taken as it is (FP32-arithmetic mode)
 GF_DEV void therm_bc_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long t,
-                          int passes)
+                          int passes, int proof)
 {
     const double *__restrict__ xa = A.xa, *__restrict__ xb = A.xb;
     // pass p writes xa when p is even, xb when odd; the first pass whose border check
@@ -981,10 +993,11 @@ GF_DEV void therm_bc_body(const GfDevParams *__restrict__ P, GfDevClock *ck, con
     int proven = -1;
     for (int k = 0; k < passes; k++)
         if (ck->therm_pass_changed[k] == 0u) { proven = k; break; }
-    if (passes == 1) proven = 0;
-    const double *x = (proven < 0 ? (TH_MAXPASS - 1) : proven) % 2 == 0 ? xa : xb;
-    if (t == 0 && (proven < 0 || (passes == 1 && ck->therm_pass_changed[0] != 0u)))
+    if (!proof) proven = 0;
+    const double *x = (proven < 0 ? (passes - 1) : proven) % 2 == 0 ? xa : xb;
+    if (t == 0 && (proven < 0 || (!proof && ck->therm_pass_changed[0] != 0u)))
         atomicAdd(&ck->therm_unconverged, 1ull);
+    if (t == 0 && proof && proven < 0) raise_error(ck, GF_ERR_THERM_SWEEPS, 0);
     if (t == 0 && proven > 0) atomicAdd(&ck->therm_extra_passes, (unsigned long long)proven);
     if (t >= NN) return;
     const int j = (int)((unsigned)t % (unsigned)NZ) + 1, i = (int)((unsigned)t / (unsigned)NZ) + 1;
@@ -1012,9 +1025,9 @@ GF_DEV void therm_bc_body(const GfDevParams *__restrict__ P, GfDevClock *ck, con
 }
 
 __global__ void __launch_bounds__(256) k_therm_bc(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
-                                                  int passes)
+                                                  int passes, int proof)
 {
-    therm_bc_body(P, ck, A, xcd_lin(), passes);
+    therm_bc_body(P, ck, A, xcd_lin(), passes, proof);
 }
 
 // ------------------------------------------------------------------ material properties
@@ -1115,8 +1128,6 @@ GF_DEV void dev_pre_plast(const GfDevParams *P, const GfPhase *phs, double aps, 
 // in the reference's order, so every lane obtains the same bits.  Element-level
 // results are stored by lane 0.  avgsr: 0 none, 1 refresh and reset the averages
 // (nsrate == ifreq_avgsr), 2 refresh only (nsrate == -1).
-#define GF_MAXACT 8
-
 // quad permutation with DPP (one VALU move per 32-bit half): lane q of every quad
 // receives the value of lane QP(...)[q] of the same quad
 #define QP(a, b, c, d) ((a) | ((b) << 2) | ((c) << 4) | ((d) << 6))
@@ -1221,6 +1232,7 @@ GF_DEV void rheol_quad(const GfDevParams *__restrict__ P, GfDevClock *ck, const 
         const Buf bPR = mkbuf(A.pr, (unsigned long long)P->nactive * pe);
         for (int a = 0; a < P->nactive; a++) prv[a] = bld(bPR, eo, a * pe);
         dev_pre_plast(P, P->ph, aps, prv, coh, phi, psi, hardn);
+        if (A.coh_scale) coh = coh * (double)A.coh_scale[e];
         // src/rh_plastic.f90:33-46: sin(phi), sin(psi) and tan(phi).  Elements whose
         // angles equal the phase's unsoftened ones reuse its correctly rounded values;
         // the others take one double-double sine/cosine per lane (lane 1 takes psi).
@@ -1522,6 +1534,7 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, const GfPhase *__res
 #pragma unroll
     for (int a = 0; a < GF_MAXACT; a++) prv[a] = bld(bPR, eo, a * pe);
     double aps = bld(bAPS, eo, 0);
+    const float csc = A.coh_scale ? A.coh_scale[e] : 1.0f;
     const double vis = bld(bVIS, eo, 0);  // refreshed by k_visc on viscosity-update steps
     double se_in[3], sshr_in, str_in[3];
     for (int c = 0; c < 3; c++) se_in[c] = bld(bSE, eo, c * pe);
@@ -1647,6 +1660,7 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, const GfPhase *__res
     if (plas_law) {
         double coh, phi, psi, hd;
         dev_pre_plast(P, phs, aps, prv, coh, phi, psi, hd);
+        if (A.coh_scale) coh = coh * (double)csc;
         double sphi, spsi, tphi;
         if (phi == rp.phi0 && psi == rp.psi0) {
             sphi = rp.sphi0;
@@ -2854,7 +2868,7 @@ __global__ void __launch_bounds__(256) k_dt_reduce(const GfDevParams *__restrict
 
 // Scalars of src/dt_mass.f90:31-51,122-158.  dtmax_therm and dt_maxwell depend on
 // the element only through its phase, so their minima run over the phases present.
-GF_DEV void dt_scalars_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const int *present, int npresent)
+GF_DEV void dt_scalars_body(const GfDevParams *__restrict__ P, GfDevClock *ck)
 {
     const double sec_year = 3.1558e+7;
     double dlmin = __longlong_as_double((long long)ck->dlmin_bits);
@@ -2867,8 +2881,9 @@ GF_DEV void dt_scalars_body(const GfDevParams *__restrict__ P, GfDevClock *ck, c
             ck->dt_elastic = dlmin * P->frac * P->strain_inert / P->vbc;
     }
     double dtmax_therm = 1.0e28, dt_maxwell = 1.0e28;
-    for (int n = 0; n < npresent; n++) {
-        const GfPhase &ph = PHASE(present[n]);
+    for (int k = 1; k <= GF_MAXPH; k++) {
+        if (ck->ph_count[k - 1] <= 0) continue;  // no element has this iphase
+        const GfPhase &ph = PHASE(k);
         if (P->idt_scale == 0) {
             double dte = P->frac * dlmin * sqrt(ph.den / ph.pwave);
             ck->dt_elastic = fmin(ck->dt_elastic, dte);
@@ -2881,9 +2896,9 @@ GF_DEV void dt_scalars_body(const GfDevParams *__restrict__ P, GfDevClock *ck, c
     ck->dt = fmin(fmin(ck->dt_elastic, ck->dt_maxwell), dtmax_therm);
 }
 
-__global__ void k_dt_scalars(const GfDevParams *__restrict__ P, GfDevClock *ck, const int *present, int npresent)
+__global__ void k_dt_scalars(const GfDevParams *__restrict__ P, GfDevClock *ck)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) dt_scalars_body(P, ck, present, npresent);
+    if (threadIdx.x == 0 && blockIdx.x == 0) dt_scalars_body(P, ck);
 }
 
 // inertial masses, src/dt_mass.f90:64-120 (gather form of the coloured scatter)

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "gf_kernels.h"
+#include "gf_phase.h"
 #include "gf_mlp.h"
 #include "gf_batch.h"
 
@@ -28,12 +29,15 @@ struct gf_ctx {
     GfDevParams *dp = nullptr;
     GfDevClock *dck = nullptr;
     GfDevArrays a{};
-    int *d_present = nullptr;  // phase ids present in iphase (for dt_mass minima)
     double *therm_border = nullptr;  // per thermal tile, its values of the output region's outer neighbours
-    int npresent = 0;
     hipStream_t stream = nullptr;
     gf_host_arrays host{};
     bool bound = false;
+    // markers (module marker_data): bound host arrays, device capacity
+    gf_marker_arrays hmark{};
+    bool markers_bound = false, markers_uploaded = false;
+    long mark_cap = 0;
+    float *coh_scale_dev = nullptr;
     uint32_t uploaded = 0;
     int32_t nloop = 0, nsrate = 0;
     long ne = 0, nn = 0;
@@ -77,6 +81,7 @@ struct gf_ctx {
     // TH_MAXPASS: the thermal result is proven to be the reference's serial one (fallback
     // passes when needed); 1: FP32-arithmetic mode takes the single tile pass as it is
     int therm_passes = TH_MAXPASS;
+    bool therm_proof = true;  // false: FP32-arithmetic mode, the single pass is not proven
     // per-kernel event timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -125,6 +130,7 @@ static int build_dev_params(gf_ctx *ctx)
     d.itherm = p.itherm; d.istress_therm = p.istress_therm; d.ishearh = p.ishearh; d.itemp_bc = p.itemp_bc;
     d.nystressbc = p.nystressbc; d.nyhydro = p.nyhydro; d.i_prestress = p.i_prestress; d.idt_scale = p.idt_scale;
     d.i_rey = p.i_rey; d.ivis_present = p.ivis_present; d.iphsub = p.iphsub; d.inhom = p.inhom;
+    d.nelem_serp = p.nelem_serp;
     for (int k = 0; k < GF_MAXINH; k++) {
         d.ix1[k] = p.ix1[k]; d.ix2[k] = p.ix2[k]; d.iy1[k] = p.iy1[k]; d.iy2[k] = p.iy2[k]; d.igeom[k] = p.igeom[k];
     }
@@ -213,6 +219,7 @@ extern "C" int gf_create(const gf_params *p, int32_t precision, gf_ctx **out)
     ctx->p = *p;
     ctx->precision = precision;
     ctx->therm_passes = precision == GF_PREC_MIXED ? 1 : TH_MAXPASS;
+    ctx->therm_proof = precision != GF_PREC_MIXED;
     ctx->ne = (long)(p->nz - 1) * (p->nx - 1);
     ctx->nn = (long)p->nz * p->nx;
     ctx->hp.nactive = 0;
@@ -251,7 +258,6 @@ extern "C" int gf_create(const gf_params *p, int32_t precision, gf_ctx **out)
     }
     ALLOC(ctx->dp, sizeof(GfDevParams));
     ALLOC(ctx->dck, sizeof(GfDevClock));
-    ALLOC(ctx->d_present, sizeof(int) * GF_MAXPH);
     ALLOC(ctx->a.cord, 8 * nn * 2);
     ALLOC(ctx->cord_alt, 8 * nn * 2);
     // stress, velocity and force in the path's precision (FP32 planes when mixed)
@@ -310,7 +316,9 @@ extern "C" void gf_destroy(gf_ctx *ctx)
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
-    void *bufs[] = {ctx->dp, ctx->dck, ctx->d_present, ctx->a.cord, ctx->a.vel, ctx->a.temp, ctx->a.temp0,
+    void *bufs[] = {ctx->dp, ctx->dck, ctx->a.mark_a1, ctx->a.mark_a2, ctx->a.mark_ntriag, ctx->a.mark_phase,
+                    ctx->a.mark_dead, ctx->a.mark_id_elem, ctx->a.nmark_elem, ctx->a.phstamp, ctx->coh_scale_dev,
+                    ctx->a.cord, ctx->a.vel, ctx->a.temp, ctx->a.temp0,
                     ctx->a.force, ctx->a.amass, ctx->a.rmass, ctx->a.bc, ctx->a.ncod, ctx->a.stress0,
                     ctx->a.area, ctx->a.dvol, ctx->a.strain, ctx->a.se2sr, ctx->a.aps, ctx->a.visn,
                     ctx->a.e2sr, ctx->a.sshrheat, ctx->a.shrheat, ctx->a.source, ctx->a.iphase, ctx->a.pr,
@@ -339,6 +347,15 @@ extern "C" int gf_bind_host(gf_ctx *ctx, const gf_host_arrays *h)
     if (!ctx || !h) return GF_ERR_ARG;
     ctx->host = *h;
     ctx->bound = true;
+    return GF_OK;
+}
+
+extern "C" int gf_bind_markers(gf_ctx *ctx, const gf_marker_arrays *m)
+{
+    if (!ctx || !m) return GF_ERR_ARG;
+    if (m->nmarkers < 0) return set_err(ctx, GF_ERR_ARG, "negative marker count");
+    ctx->hmark = *m;
+    ctx->markers_bound = true;
     return GF_OK;
 }
 
@@ -408,10 +425,27 @@ static int upload_phase_ratio(gf_ctx *ctx)
     const double *pr = ctx->host.phase_ratio;
     ctx->phase_ratio_host.assign(pr, pr + (size_t)nph * ne);
     int active[GF_MAXPH], nact = 0;
+    // With markers, change_phase keeps the planes of the phases present anywhere
+    // (nonzero ratio; markers of a phase give it a ratio of at least 1/32); without,
+    // phases below 0.01 everywhere are skipped by every loop that reads them.
+    const double thr = ctx->markers_bound ? 0.0 : 0.01;
+    bool mark_ph[GF_MAXPH + 1] = {false};
+    if (ctx->markers_bound && ctx->hmark.mark_phase)
+        for (long q = 0; q < ctx->hmark.nmarkers; q++) {
+            const int v = ctx->hmark.mark_phase[q];
+            if (v >= 1 && v <= nph) mark_ph[v] = true;
+        }
+    int nz_count[GF_MAXACT] = {0};
     for (int k = 0; k < nph; k++) {
-        bool any = false;
-        for (long e = 0; e < ne && !any; e++) any = pr[k + (long)nph * e] >= 0.01;
+        bool any = mark_ph[k + 1];
+        long nz = 0;
+        for (long e = 0; e < ne; e++) {
+            const double r = pr[k + (long)nph * e];
+            if (thr == 0.0 ? r != 0.0 : r >= thr) any = true;
+            if (r != 0.0) nz++;
+        }
         if (any) {
+            if (nact < GF_MAXACT) nz_count[nact] = (int)nz;
             if (nact == GF_MAXACT)
                 return set_err(ctx, GF_ERR_UNSUPPORTED, "more than %d phases present", GF_MAXACT);
             active[nact++] = k + 1;
@@ -424,23 +458,104 @@ static int upload_phase_ratio(gf_ctx *ctx)
     ctx->hp.nactive = nact;
     memset(ctx->hp.active, 0, sizeof(ctx->hp.active));
     for (int a = 0; a < nact; a++) ctx->hp.active[a] = active[a];
+    HIPCHK(hipMemcpy(&ctx->dck->ph_nz[0], nz_count, sizeof(nz_count), hipMemcpyHostToDevice));
     return push_params(ctx);
 }
 
+// the active planes as the device holds them (change_phase re-plans on the device)
+static int pull_active(gf_ctx *ctx)
+{
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    int v[1 + GF_MAXPH];
+    HIPCHK(hipMemcpy(&v[0], &ctx->dp->nactive, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&v[1], &ctx->dp->active[0], sizeof(int) * GF_MAXPH, hipMemcpyDeviceToHost));
+    ctx->hp.nactive = v[0];
+    memcpy(ctx->hp.active, &v[1], sizeof(int) * GF_MAXPH);
+    return GF_OK;
+}
+
+static int upload_markers(gf_ctx *ctx)
+{
+    const gf_marker_arrays &m = ctx->hmark;
+    if (!ctx->markers_bound) return set_err(ctx, GF_ERR_STATE, "GF_MARKERS uploaded before gf_bind_markers");
+    if (!m.mark_a1 || !m.mark_a2 || !m.mark_ntriag || !m.mark_phase || !m.mark_dead || !m.mark_id_elem ||
+        !m.nmark_elem)
+        return set_err(ctx, GF_ERR_ARG, "a marker array pointer is null");
+    const long nmk = m.nmarkers, ne = ctx->ne;
+    // the triangles and element lists must address this mesh (the kernels index with them)
+    for (long q = 0; q < nmk; q++)
+        if (m.mark_ntriag[q] < 1 || m.mark_ntriag[q] > 2 * ne)
+            return set_err(ctx, GF_ERR_ARG, "marker %ld: triangle %d outside the mesh", q + 1, m.mark_ntriag[q]);
+    for (long e = 0; e < ne; e++) {
+        const int nm = m.nmark_elem[e];
+        if (nm < 0 || nm > GF_MAX_MARKERS_PER_ELEM)
+            return set_err(ctx, GF_ERR_ARG, "element %ld holds %d markers", e, nm);
+        for (int q = 0; q < nm; q++) {
+            const int id = m.mark_id_elem[q + (long)GF_MAX_MARKERS_PER_ELEM * e];
+            if (id < 1 || id > nmk) return set_err(ctx, GF_ERR_ARG, "element %ld lists marker %d", e, id);
+        }
+    }
+    if (nmk > ctx->mark_cap) {
+        for (void *q : {(void *)ctx->a.mark_a1, (void *)ctx->a.mark_a2, (void *)ctx->a.mark_ntriag,
+                        (void *)ctx->a.mark_phase, (void *)ctx->a.mark_dead})
+            if (q) (void)hipFree(q);
+        ctx->a.mark_a1 = ctx->a.mark_a2 = nullptr;
+        ctx->a.mark_ntriag = nullptr;
+        ctx->a.mark_phase = ctx->a.mark_dead = nullptr;
+        ctx->mark_cap = 0;
+        const size_t n = (size_t)std::max<long>(nmk, 1);
+        HIPCHK(hipMalloc((void **)&ctx->a.mark_a1, 8 * n));
+        HIPCHK(hipMalloc((void **)&ctx->a.mark_a2, 8 * n));
+        HIPCHK(hipMalloc((void **)&ctx->a.mark_ntriag, 4 * n));
+        HIPCHK(hipMalloc((void **)&ctx->a.mark_phase, n));
+        HIPCHK(hipMalloc((void **)&ctx->a.mark_dead, n));
+        ctx->mark_cap = nmk;
+    }
+    if (!ctx->a.mark_id_elem) {
+        HIPCHK(hipMalloc((void **)&ctx->a.mark_id_elem, 4 * (size_t)GF_MAX_MARKERS_PER_ELEM * ne));
+        HIPCHK(hipMalloc((void **)&ctx->a.nmark_elem, 4 * (size_t)ne));
+        HIPCHK(hipMalloc((void **)&ctx->a.phstamp, 4 * (size_t)ne));
+        HIPCHK(hipMemset(ctx->a.phstamp, 0, 4 * (size_t)ne));  // calls are numbered from 1
+    }
+    if (nmk > 0) {
+        HIPCHK(hipMemcpy(ctx->a.mark_a1, m.mark_a1, 8 * nmk, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->a.mark_a2, m.mark_a2, 8 * nmk, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->a.mark_ntriag, m.mark_ntriag, 4 * nmk, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->a.mark_phase, m.mark_phase, nmk, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->a.mark_dead, m.mark_dead, nmk, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(ctx->a.mark_id_elem, m.mark_id_elem, 4 * (size_t)GF_MAX_MARKERS_PER_ELEM * ne,
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->a.nmark_elem, m.nmark_elem, 4 * (size_t)ne, hipMemcpyHostToDevice));
+    ctx->a.nmarkers = nmk;
+    ctx->markers_uploaded = true;
+    return GF_OK;
+}
+
+static int upload_coh_scale(gf_ctx *ctx)
+{
+    if (!ctx->host.coh_scale) {  // none: factor 1
+        ctx->a.coh_scale = nullptr;
+        return GF_OK;
+    }
+    if (!ctx->coh_scale_dev) HIPCHK(hipMalloc((void **)&ctx->coh_scale_dev, 4 * (size_t)ctx->ne));
+    HIPCHK(hipMemcpy(ctx->coh_scale_dev, ctx->host.coh_scale, 4 * (size_t)ctx->ne, hipMemcpyHostToDevice));
+    ctx->a.coh_scale = ctx->coh_scale_dev;
+    return GF_OK;
+}
+
+// elements per iphase value: the dt_mass minima run over the values present, and
+// change_phase keeps the counts as it changes iphase
 static int upload_iphase_present(gf_ctx *ctx)
 {
-    bool seen[GF_MAXPH + 1] = {false};
+    int count[GF_MAXPH] = {0};
     const int32_t *ip = ctx->host.iphase;
     for (long e = 0; e < ctx->ne; e++) {
         int v = ip[e];
         if (v < 1 || v > ctx->p.nphase) return set_err(ctx, GF_ERR_ARG, "iphase out of range at %ld", e);
-        seen[v] = true;
+        count[v - 1]++;
     }
-    int present[GF_MAXPH], n = 0;
-    for (int k = 1; k <= GF_MAXPH; k++)
-        if (seen[k]) present[n++] = k;
-    ctx->npresent = n;
-    HIPCHK(hipMemcpy(ctx->d_present, present, sizeof(int) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(&ctx->dck->ph_count[0], count, sizeof(count), hipMemcpyHostToDevice));
     return GF_OK;
 }
 
@@ -492,6 +607,14 @@ extern "C" int gf_upload(gf_ctx *ctx, uint32_t mask)
         int rc = upload_iphase_present(ctx);
         if (rc) return rc;
     }
+    if (mask & GF_MARKERS) {
+        int rc = upload_markers(ctx);
+        if (rc) return rc;
+    }
+    if (mask & GF_COH_SCALE) {
+        int rc = upload_coh_scale(ctx);
+        if (rc) return rc;
+    }
     if (mask & GF_PHASE_RATIO) {
         if (!ctx->host.phase_ratio) return set_err(ctx, GF_ERR_ARG, "host pointer for phase_ratio is null");
         int rc = upload_phase_ratio(ctx);
@@ -533,8 +656,28 @@ extern "C" int gf_download(gf_ctx *ctx, uint32_t mask)
         }
         HIPCHK(hipMemcpy(h, dev_ptr(ctx, f.bit), f.elem_bytes * count, hipMemcpyDeviceToHost));
     }
-    if ((mask & GF_PHASE_RATIO) && ctx->host.phase_ratio && !ctx->phase_ratio_host.empty())
-        memcpy(ctx->host.phase_ratio, ctx->phase_ratio_host.data(), sizeof(double) * ctx->phase_ratio_host.size());
+    if ((mask & GF_PHASE_RATIO) && ctx->host.phase_ratio && !ctx->phase_ratio_host.empty()) {
+        if (ctx->markers_uploaded) {
+            // change_phase may have changed the ratios and the planes: read them back
+            int rc = pull_active(ctx);
+            if (rc) return rc;
+            const int nph = ctx->p.nphase, nact = ctx->hp.nactive;
+            const long ne = ctx->ne;
+            std::vector<double> planes((size_t)ne * std::max(nact, 1));
+            if (nact > 0)
+                HIPCHK(hipMemcpy(planes.data(), ctx->a.pr, sizeof(double) * ne * nact, hipMemcpyDeviceToHost));
+            double *h = ctx->host.phase_ratio;
+            memset(h, 0, sizeof(double) * (size_t)nph * ne);
+            for (int a = 0; a < nact; a++) {
+                const int k = ctx->hp.active[a] - 1;
+                for (long e = 0; e < ne; e++) h[k + (long)nph * e] = planes[(size_t)a * ne + e];
+            }
+        } else {
+            memcpy(ctx->host.phase_ratio, ctx->phase_ratio_host.data(), sizeof(double) * ctx->phase_ratio_host.size());
+        }
+    }
+    if ((mask & GF_MARKERS) && ctx->markers_uploaded && ctx->hmark.mark_phase && ctx->a.nmarkers > 0)
+        HIPCHK(hipMemcpy(ctx->hmark.mark_phase, ctx->a.mark_phase, ctx->a.nmarkers, hipMemcpyDeviceToHost));
     return GF_OK;
 }
 
@@ -712,16 +855,16 @@ static int enq_therm(gf_ctx *ctx, bool clock_done = false)
                     : ex     ? (dis ? k_therm_pipe<2, true> : k_therm_pipe<2, false>)
                              : (dis ? k_therm_pipe<0, true> : k_therm_pipe<0, false>);
         hipLaunchKernelGGL(kern, grid, dim3(TH_J * TH_I), 0, ctx->stream, ctx->dp, ctx->dck, ctx->a, pass,
-                           npass == 1 ? GF_THERM_STOL : 0.0, ctx->area_pending ? 1 : 0);
+                           ctx->therm_proof ? 0.0 : GF_THERM_STOL, ctx->area_pending ? 1 : 0);
         if ((rc = launch_check(ctx, "k_therm_pipe"))) return rc;
         if (ctx->timing && e0 && e1) {
             (void)hipEventRecord(e1, ctx->stream);
             ctx->pending.push_back({pass == 0 ? "k_therm_pipe" : "k_therm_pipe_extra", {e0, e1}});
         }
-        if (npass > 1)
+        if (ctx->therm_proof)
             LAUNCH(k_therm_check, (long)ntiles * TH_NB, ctx->dp, ctx->dck, ctx->a, pass, ctx->area_pending ? 1 : 0);
     }
-    LAUNCH(k_therm_bc, ctx->nn, ctx->dp, ctx->dck, ctx->a, npass);
+    LAUNCH(k_therm_bc, ctx->nn, ctx->dp, ctx->dck, ctx->a, npass, ctx->therm_proof ? 1 : 0);
     ctx->temp0_prev = true;  // pass 0 wrote temp0 at every node
     return GF_OK;
 }
@@ -868,8 +1011,7 @@ static int enq_dt_mass(gf_ctx *ctx)
     if ((rc = flush_area(ctx))) return rc;  // the dt minima and inertial masses read the areas
     if ((rc = enq_clock(ctx, CK_DT_BEGIN))) return rc;
     LAUNCH_SV(k_dt_reduce, (long)DT_RED_BLOCKS * 256, ctx->dp, ctx->dck, ctx->a);
-    hipLaunchKernelGGL(k_dt_scalars, dim3(1), dim3(64), 0, ctx->stream, ctx->dp, ctx->dck, ctx->d_present,
-                       ctx->npresent);
+    hipLaunchKernelGGL(k_dt_scalars, dim3(1), dim3(64), 0, ctx->stream, ctx->dp, ctx->dck);
     if ((rc = launch_check(ctx, "k_dt_scalars"))) return rc;
     LAUNCH(k_amass, ctx->nn, ctx->dp, ctx->dck, ctx->a);
     return GF_OK;
@@ -879,10 +1021,38 @@ static const uint32_t kRequired = GF_CORD | GF_VEL | GF_TEMP | GF_AMASS | GF_RMA
                                   GF_AREA | GF_DVOL | GF_STRAIN | GF_SE2SR | GF_APS | GF_VISN | GF_E2SR |
                                   GF_SSHRHEAT | GF_SHRHEAT | GF_SOURCE | GF_IPHASE | GF_PHASE_RATIO;
 
+// phases whose markers a change_phase rule can change (src/change_phase.f90:100-230)
+static bool has_phase_rule(int k)
+{
+    return k == 1 || k == 3 || k == 4 || k == 5 || k == 7 || k == 8 || k == 9 || k == 10 || k == 11 || k == 16 ||
+           k == 17;
+}
+
 static int check_ready(gf_ctx *ctx)
 {
     if ((ctx->uploaded & kRequired) != kRequired)
         return set_err(ctx, GF_ERR_STATE, "state not uploaded (mask 0x%x missing)", kRequired & ~ctx->uploaded);
+    // change_phase needs the markers as soon as one present phase has a rule
+    if (!ctx->markers_uploaded && ctx->p.itherm != 2)
+        for (int a = 0; a < ctx->hp.nactive; a++)
+            if (has_phase_rule(ctx->hp.active[a]))
+                return set_err(ctx, GF_ERR_STATE,
+                               "phase %d has change_phase rules (src/change_phase.f90): upload the markers (GF_MARKERS)",
+                               ctx->hp.active[a]);
+    return GF_OK;
+}
+
+// change_phase, src/change_phase.f90:1-260 (gf_phase.h); without markers no present
+// phase has a rule (check_ready) and it changes nothing
+static int enq_change_phase(gf_ctx *ctx)
+{
+    if (!ctx->markers_uploaded) return GF_OK;
+    if (ctx->a.nmarkers > 0) LAUNCH(k_phase_rules, ctx->a.nmarkers, ctx->dp, ctx->dck, ctx->a);
+    hipLaunchKernelGGL(k_phase_plan, dim3(1), dim3(64), 0, ctx->stream, ctx->dp, ctx->dck);
+    int rc = launch_check(ctx, "k_phase_plan");
+    if (rc) return rc;
+    LAUNCH(k_phase_replane, ctx->ne, ctx->dp, ctx->dck, ctx->a);
+    LAUNCH(k_phase_recount, ctx->ne, ctx->dp, ctx->dck, ctx->a);
     return GF_OK;
 }
 
@@ -905,7 +1075,7 @@ static int enq_flac(gf_ctx *ctx, int pending = 0)
     if ((rc = enq_clock(ctx, pending | CK_THERM_BEGIN | (mech ? CK_DTAVG_ADD : 0)))) return rc;
     if ((rc = enq_therm(ctx, true))) return rc;
     if (mech) {
-        // change_phase (every ifreq_rmasses) has no rule for the phases this library accepts
+        if (ctx->nloop % ctx->p.ifreq_rmasses == 0 && (rc = enq_change_phase(ctx))) return rc;
         if ((rc = enq_rheol(ctx, true))) return rc;
         if ((rc = enq_node(ctx, true))) return rc;
         if ((rc = enq_move(ctx))) return rc;
@@ -965,6 +1135,7 @@ SINGLE(gf_fl_node, enq_node(ctx, false))
 SINGLE(gf_fl_move, enq_move(ctx))
 SINGLE(gf_rmasses, enq_rmasses(ctx))
 SINGLE(gf_dt_mass, enq_dt_mass(ctx))
+SINGLE(gf_change_phase, enq_change_phase(ctx))
 
 extern "C" int gf_init_areas(gf_ctx *ctx)
 {
@@ -995,6 +1166,16 @@ extern "C" int gf_therm_stats(gf_ctx *ctx, int64_t *stats)
     stats[3] = (int64_t)ck.therm_mismatch_ulp;
     unsigned long long z[4] = {0, 0, 0, 0};
     HIPCHK(hipMemcpy(&ctx->dck->therm_unconverged, z, sizeof(z), hipMemcpyHostToDevice));
+    return GF_OK;
+}
+
+extern "C" int gf_set_therm_passes(gf_ctx *ctx, int32_t npass)
+{
+    if (!ctx) return GF_ERR_ARG;
+    if (npass < 1 || npass > TH_MAXPASS)
+        return set_err(ctx, GF_ERR_ARG, "thermal passes %d outside 1..%d", npass, TH_MAXPASS);
+    if (!ctx->therm_proof) return set_err(ctx, GF_ERR_UNSUPPORTED, "FP32-arithmetic mode runs one unproven pass");
+    ctx->therm_passes = npass;
     return GF_OK;
 }
 
@@ -1401,7 +1582,9 @@ struct gf_batch {
     unsigned therm_grid = 8;  // workgroups of the thermal pipeline (the CUs)
     BatchMap map{};
     int nm = 0;
-    long tot_e = 0, tot_n = 0, tot_t = 0, tot_b = 0;
+    long tot_e = 0, tot_n = 0, tot_t = 0, tot_b = 0, tot_m = 0;
+    bool markers = false;   // some model has markers (change_phase runs)
+    bool movegrid = true;   // the models move their grids (all or none)
     hipStream_t stream = nullptr;
     int32_t nloop = 0, nsrate = 0;
     int bs[3] = {256, 256, 256};  // threads per workgroup of the rheology, node and element-area kernels
@@ -1413,7 +1596,7 @@ struct gf_batch {
     // ... and its areas and dvol to the next kb_rheol (the thermal tiles before it form
     // the areas they need), in FP32-arithmetic mode, whose thermal step has no fallback
     // passes; g_area0: the same at the start of the captured cycle
-    bool area_pending = false, g_area0 = false, g_area_end = false;
+    bool area_pending = false, g_area0 = false, g_area_end = false, g_rot_end = false;
     hipGraphExec_t gexec = nullptr;
     int32_t g_nsrate0 = -99, g_nsrate1 = 0;  // nsrate before and after the captured 10 steps
     // optional per-kernel timing (autotuner)
@@ -1470,9 +1653,9 @@ extern "C" int gf_batch_create(gf_ctx *const *models, int32_t n, gf_batch **out)
     const gf_ctx *c0 = models[0];
     if (!c0) return fail(GF_ERR_ARG, "null model");
     std::vector<BatchEntry> tab(n + 1);
-    long pe = 0, pn = 0, pt = 0, pb = 0;
+    long pe = 0, pn = 0, pt = 0, pb = 0, pm = 0;
     auto align = [](long v) { return (v + BATCH_ALIGN - 1) / BATCH_ALIGN * BATCH_ALIGN; };
-    std::vector<int> me, mn, mt, mb;  // model of each chunk (tiles: of each tile)
+    std::vector<int> me, mn, mt, mb, mm;  // model of each chunk (tiles: of each tile)
     for (int k = 0; k < n; k++) {
         gf_ctx *c = models[k];
         if (!c || check_ready(c) != GF_OK) return fail(GF_ERR_STATE, "model not ready");
@@ -1480,12 +1663,16 @@ extern "C" int gf_batch_create(gf_ctx *const *models, int32_t n, gf_batch **out)
         if (c->precision != c0->precision || q.ishearh != q0.ishearh || q.itherm != q0.itherm ||
             q.ifreq_visc != q0.ifreq_visc || q.ifreq_avgsr != q0.ifreq_avgsr || q.ifreq_rmasses != q0.ifreq_rmasses ||
             q.ifreq_imasses != q0.ifreq_imasses || c->nloop != c0->nloop || c->nsrate != c0->nsrate || c->sur_on ||
-            c->rheol_quad)
-            return fail(GF_ERR_ARG, "models of a batch share precision, switches, frequencies and loop counters");
+            c->rheol_quad || (q.movegrid != 0) != (q0.movegrid != 0) || c->therm_passes != c0->therm_passes)
+            return fail(GF_ERR_ARG,
+                        "models of a batch share precision, switches, frequencies, movegrid and loop counters");
         if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(GF_ERR_HIP, "sync");
         const long ntiles = (long)((q.nz + TH_OJ - 1) / TH_OJ) * ((q.nx + TH_OI - 1) / TH_OI);
-        tab[k] = BatchEntry{c->dp, c->dck, c->a, pe, pn, pt, pb, c->ne, c->nn, ntiles, ntiles * TH_NB,
-                            c->d_present, c->npresent, 0};
+        const long nmk = c->markers_uploaded ? c->a.nmarkers : 0;
+        tab[k] = BatchEntry{c->dp, c->dck, c->a, pe, pn, pt, pb, pm, c->ne, c->nn, ntiles, ntiles * TH_NB, nmk};
+        if (nmk > 0) b->markers = true;
+        mm.insert(mm.end(), align(nmk) / BATCH_ALIGN, k);
+        pm += align(nmk);
         me.insert(me.end(), align(c->ne) / BATCH_ALIGN, k);
         mn.insert(mn.end(), align(c->nn) / BATCH_ALIGN, k);
         mb.insert(mb.end(), align(ntiles * TH_NB) / BATCH_ALIGN, k);
@@ -1496,16 +1683,18 @@ extern "C" int gf_batch_create(gf_ctx *const *models, int32_t n, gf_batch **out)
         pb += align(ntiles * TH_NB);
         b->m.push_back(c);
     }
-    tab[n] = BatchEntry{nullptr, nullptr, GfDevArrays{}, pe, pn, pt, pb, 0, 0, 0, 0, nullptr, 0, 0};
+    tab[n] = BatchEntry{nullptr, nullptr, GfDevArrays{}, pe, pn, pt, pb, pm, 0, 0, 0, 0, 0};
     b->nm = n;
-    b->tot_e = pe; b->tot_n = pn; b->tot_t = pt; b->tot_b = pb;
+    b->tot_e = pe; b->tot_n = pn; b->tot_t = pt; b->tot_b = pb; b->tot_m = pm;
+    b->movegrid = c0->p.movegrid != 0;
     b->therm_grid = c0->therm_grid;
-    const size_t nmap = me.size() + mn.size() + mt.size() + mb.size();
+    const size_t nmap = me.size() + mn.size() + mt.size() + mb.size() + mm.size() + 1;
     if (hipMalloc((void **)&b->dmap, sizeof(int) * nmap) != hipSuccess) return fail(GF_ERR_HIP, "allocation");
     {
         std::vector<int> all;
         all.reserve(nmap);
-        for (auto *v : {&me, &mn, &mt, &mb}) all.insert(all.end(), v->begin(), v->end());
+        for (auto *v : {&me, &mn, &mt, &mb, &mm}) all.insert(all.end(), v->begin(), v->end());
+        all.push_back(0);
         if (hipMemcpy(b->dmap, all.data(), sizeof(int) * nmap, hipMemcpyHostToDevice) != hipSuccess)
             return fail(GF_ERR_HIP, "copy");
         const int *d = b->dmap;
@@ -1513,6 +1702,7 @@ extern "C" int gf_batch_create(gf_ctx *const *models, int32_t n, gf_batch **out)
         b->map.chunk[BK_NODE] = d + me.size();
         b->map.chunk[BK_TILE] = d + me.size() + mn.size();
         b->map.chunk[BK_BORD] = d + me.size() + mn.size() + mt.size();
+        b->map.chunk[BK_MARK] = d + me.size() + mn.size() + mt.size() + mb.size();
         b->map.nm = n;
     }
     b->precision = c0->precision;
@@ -1567,17 +1757,18 @@ static int enq_batch_flac(gf_batch *b)
     BLAUNCH(kb_clock, gm, 64, b->dtab, map, (int)(CK_THERM_BEGIN | (b->no_mech ? 0 : CK_DTAVG_ADD)));
     const bool dis = b->p0.ishearh == 1 && b->p0.itherm != 2;
     const int npass = b->m[0]->therm_passes;
+    const bool proof = b->m[0]->therm_proof;
     for (int pass = 0; pass < npass; pass++) {
         // pass 0 through the pipeline; a fallback pass, which only some models need,
         // with one workgroup per tile that returns at once for a proven model
         if (pass == 0)
             BLAUNCH((dis ? kb_therm_pipe<0, true> : kb_therm_pipe<0, false>), b->therm_grid, TH_J * TH_I,
-                    b->dtab, map, b->tot_t, pass, npass == 1 ? GF_THERM_STOL : 0.0, b->area_pending ? 1 : 0);
+                    b->dtab, map, b->tot_t, pass, proof ? 0.0 : GF_THERM_STOL, b->area_pending ? 1 : 0);
         else
             BLAUNCH(kb_therm_tile, (unsigned)std::min<long>(b->tot_t, 512), TH_J * TH_I, b->dtab, map, b->tot_t, pass);
-        if (npass > 1) BLAUNCH(kb_therm_check, bgrid(b->tot_b, 256), 256, b->dtab, map, pass);
+        if (proof) BLAUNCH(kb_therm_check, bgrid(b->tot_b, 256), 256, b->dtab, map, pass);
     }
-    BLAUNCH(kb_therm_bc, bgrid(b->tot_n, 256), 256, b->dtab, map, npass);
+    BLAUNCH(kb_therm_bc, bgrid(b->tot_n, 256), 256, b->dtab, map, npass, proof ? 1 : 0);
     if (!b->no_mech) {
         const gf_params &p = b->p0;
         const int do_visc = (b->nloop % p.ifreq_visc) == 0;
@@ -1586,6 +1777,12 @@ static int enq_batch_flac(gf_batch *b)
             avgsr = 1;
         else if (b->nsrate == -1)
             avgsr = 2;
+        if (b->markers && b->nloop % p.ifreq_rmasses == 0) {  // change_phase
+            if (b->tot_m > 0) BLAUNCH(kb_phase_rules, bgrid(b->tot_m, 256), 256, b->dtab, map);
+            BLAUNCH(kb_phase_plan, gm, 64, b->dtab, map);
+            BLAUNCH(kb_phase_replane, bgrid(b->tot_e, 256), 256, b->dtab, map);
+            BLAUNCH(kb_phase_recount, bgrid(b->tot_e, 256), 256, b->dtab, map);
+        }
         const int rot = (b->rot_pending ? RH_ROT : 0) | (b->area_pending ? RH_AREA : 0);
         if (do_visc) BLAUNCH(kb_visc<M>, bgrid(b->tot_e, 256), 256, b->dtab, map, (rot & RH_ROT) ? 1 : 0);
         BTIMED(0, BLAUNCH((kb_rheol<SH, M>), bgrid(b->tot_e, b->bs[0]), b->bs[0], b->dtab, map, avgsr, rot));
@@ -1597,13 +1794,15 @@ static int enq_batch_flac(gf_batch *b)
         }
         b->nsrate = b->nsrate + 1;
         BTIMED(1, BLAUNCH(kb_node<M>, bgrid(b->tot_n, b->bs[1]), b->bs[1], b->dtab, map));
-        BLAUNCH(kb_move_cord<SV>, bgrid(b->tot_n, 256), 256, b->dtab, map);
-        const bool masses = b->nloop % p.ifreq_rmasses == 0 || b->nloop % p.ifreq_imasses == 0;
-        if (npass == 1 && !masses)
-            b->area_pending = true;
-        else
-            BTIMED(2, BLAUNCH(kb_move_area, bgrid(b->tot_e, b->bs[2]), b->bs[2], b->dtab, map));
-        b->rot_pending = true;
+        if (b->movegrid) {  // fl_move returns at once without movegrid (src/fl_move.f90:16)
+            BLAUNCH(kb_move_cord<SV>, bgrid(b->tot_n, 256), 256, b->dtab, map);
+            const bool masses = b->nloop % p.ifreq_rmasses == 0 || b->nloop % p.ifreq_imasses == 0;
+            if (npass == 1 && !masses)
+                b->area_pending = true;
+            else
+                BTIMED(2, BLAUNCH(kb_move_area, bgrid(b->tot_e, b->bs[2]), b->bs[2], b->dtab, map));
+            b->rot_pending = true;
+        }
         if (b->nloop % p.ifreq_rmasses == 0) {
             BLAUNCH(kb_dens<M>, bgrid(b->tot_e, 256), 256, b->dtab, map, b->rot_pending ? 1 : 0);
             BLAUNCH(kb_rmass, bgrid(b->tot_n, 256), 256, b->dtab, map);
@@ -1692,6 +1891,7 @@ extern "C" int gf_batch_step(gf_batch *b, int32_t nsteps)
                 b->g_nsrate0 = ns0;
                 b->g_nsrate1 = b->nsrate;
                 b->g_area_end = b->area_pending;
+                b->g_rot_end = b->rot_pending;
                 b->g_rot0 = rot0;
                 b->g_area0 = area0;
                 b->nloop = nl0;  // the capture only recorded the launches
@@ -1702,7 +1902,7 @@ extern "C" int gf_batch_step(gf_batch *b, int32_t nsteps)
             BCHK(hipGraphLaunch(b->gexec, b->stream));
             b->nloop += 10;
             b->nsrate = b->g_nsrate1;
-            b->rot_pending = true;  // the cycle ends with a move
+            b->rot_pending = b->g_rot_end;  // the cycle ends with a move when the models move
             b->area_pending = b->g_area_end;
             n += 10;
             continue;

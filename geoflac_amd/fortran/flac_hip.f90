@@ -1,8 +1,10 @@
 ! flac_hip.f90 -- drop-in replacement of subroutine flac (src/flac.f90:5-91).
 !
 ! Same name, no arguments, same place in the driver's loop (src/par.f90:83): each call
-! runs one explicit step (fl_therm, fl_rheol, fl_node, fl_move, and rmasses / dt_mass
-! at their frequencies) on the GPU through libgfhip.so.  The first call copies the
+! runs one explicit step (fl_therm, change_phase, fl_rheol, fl_node, fl_move, and
+! rmasses / dt_mass at their frequencies) on the GPU through libgfhip.so.  The markers
+! of module marker_data go to the device with the arrays; change_phase updates their
+! phases there.  The first call copies the
 ! parameters of module params and every array of module arrays to the device; after
 ! that the device state is authoritative and the Fortran arrays are refreshed only by
 ! flac_hip_sync_host, which the driver calls where it reads them (output, restart
@@ -68,6 +70,8 @@ p%nyhydro = nyhydro; p%i_prestress = i_prestress; p%idt_scale = idt_scale; p%i_r
 p%ivis_present = ivis_present; p%iphsub = iphsub; p%ifreq_rmasses = ifreq_rmasses
 p%ifreq_imasses = ifreq_imasses; p%ifreq_visc = ifreq_visc; p%ifreq_avgsr = ifreq_avgsr
 p%inhom = inhom
+p%nelem_serp = nelem_serp
+p%pad_i = 0
 p%irheol = 0; p%ix1 = 0; p%ix2 = 0; p%iy1 = 0; p%iy2 = 0; p%igeom = 0
 ! the flexural base (src/fl_flexure.f90:15-24) and surface diffusion (src/fl_move.f90:40)
 ! are refused by gf_create
@@ -109,15 +113,37 @@ h%aps = gf_addr8(aps); h%visn = gf_addr8(visn); h%e2sr = gf_addr8(e2sr)
 h%sshrheat = gf_addr8(sshrheat); h%shrheat = gf_addr8(shrheat); h%source = gf_addr8(source)
 h%iphase = gf_addr4(iphase)
 h%phase_ratio = gf_addr8(phase_ratio)
+h%coh_scale = c_null_ptr
 rc = gf_bind_host(gf_ctx_handle, h)
 if (rc /= GF_OK) call flac_hip_fail(rc)
-rc = gf_upload(gf_ctx_handle, GF_ALL)
+call flac_hip_bind_markers
+rc = gf_upload(gf_ctx_handle, ior(GF_ALL, GF_MARKERS))
 if (rc /= GF_OK) call flac_hip_fail(rc)
 ck%nloop = nloop; ck%nsrate = nsrate; ck%time = time; ck%dt = dt; ck%dtavg = dtavg
 ck%dt_elastic = dt_elastic; ck%dt_maxwell = dt_maxwell
 rc = gf_set_clock(gf_ctx_handle, ck)
 if (rc /= GF_OK) call flac_hip_fail(rc)
 end subroutine flac_hip_init
+
+! The marker arrays of module marker_data (their count changes with remeshing).
+subroutine flac_hip_bind_markers
+use iso_c_binding
+use gf_hip
+use flac_hip_state
+use marker_data
+use params
+implicit none
+type(gf_marker_arrays_t) :: m
+integer(c_int) :: rc
+m%nmarkers = nmarkers
+m%pad = 0
+m%mark_a1 = gf_addr8(mark_a1); m%mark_a2 = gf_addr8(mark_a2)
+m%mark_ntriag = gf_addr4(mark_ntriag)
+m%mark_phase = gf_addr1(mark_phase); m%mark_dead = gf_addr1(mark_dead)
+m%mark_id_elem = gf_addr4(mark_id_elem); m%nmark_elem = gf_addr4(nmark_elem)
+rc = gf_bind_markers(gf_ctx_handle, m)
+if (rc /= GF_OK) call flac_hip_fail(rc)
+end subroutine flac_hip_bind_markers
 
 ! Device -> Fortran arrays (before outflac / saveflac / remeshing read them).
 subroutine flac_hip_sync_host
@@ -127,7 +153,7 @@ use flac_hip_state
 implicit none
 integer(c_int) :: rc
 if (.not. c_associated(gf_ctx_handle)) return
-rc = gf_download(gf_ctx_handle, GF_ALL)
+rc = gf_download(gf_ctx_handle, ior(GF_ALL, GF_MARKERS))
 if (rc /= GF_OK) call flac_hip_fail(rc)
 end subroutine flac_hip_sync_host
 
@@ -141,7 +167,8 @@ implicit none
 integer(c_int) :: rc
 type(gf_clock_t) :: ck
 if (.not. c_associated(gf_ctx_handle)) return
-rc = gf_upload(gf_ctx_handle, GF_ALL)
+call flac_hip_bind_markers
+rc = gf_upload(gf_ctx_handle, ior(GF_ALL, GF_MARKERS))
 if (rc /= GF_OK) call flac_hip_fail(rc)
 ck%nloop = nloop; ck%nsrate = nsrate; ck%time = time; ck%dt = dt; ck%dtavg = dtavg
 ck%dt_elastic = dt_elastic; ck%dt_maxwell = dt_maxwell

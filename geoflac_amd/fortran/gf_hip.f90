@@ -11,6 +11,9 @@ implicit none
 integer, parameter :: GF_MAXPH = 20, GF_MAXINH = 100
 integer(c_int32_t), parameter :: GF_PREC_FP64 = 0, GF_PREC_MIXED = 1, GF_PREC_MIXED_STORAGE = 2
 integer(c_int32_t), parameter :: GF_ALL = int(z'3FFFFF', c_int32_t)
+integer(c_int32_t), parameter :: GF_MARKERS = int(z'400000', c_int32_t)
+integer(c_int32_t), parameter :: GF_PHASE_RATIO = int(z'200000', c_int32_t), GF_IPHASE = int(z'100000', c_int32_t)
+integer(c_int32_t), parameter :: GF_APS = int(z'4000', c_int32_t)
 integer, parameter :: GF_OK = 0, GF_STOP_MAXWELL = 22, GF_STOP_NEGATIVE_AREA = 40
 
 type, bind(C) :: gf_params_t
@@ -24,6 +27,7 @@ type, bind(C) :: gf_params_t
     integer(c_int32_t) :: irheol(GF_MAXPH)
     integer(c_int32_t) :: ix1(GF_MAXINH), ix2(GF_MAXINH), iy1(GF_MAXINH), iy2(GF_MAXINH), igeom(GF_MAXINH)
     integer(c_int32_t) :: flexure_base
+    integer(c_int32_t) :: nelem_serp, pad_i
     real(c_double) :: t_top, t_bot, bot_bc, g, demf, ten_off, tau_heal, v_min, v_max
     real(c_double) :: frac, fracm, strain_inert, dt_scale, xReyn, vbc, rxbo, rzbo
     real(c_double) :: pisos, drosub, fmagma_max, weaken_ratio_plastic, weaken_ratio_viscous, rho_magma
@@ -42,6 +46,12 @@ type, bind(C) :: gf_host_arrays_t
     type(c_ptr) :: aps, visn, e2sr, sshrheat, shrheat, source
     type(c_ptr) :: iphase
     type(c_ptr) :: phase_ratio
+    type(c_ptr) :: coh_scale
+end type
+
+type, bind(C) :: gf_marker_arrays_t
+    integer(c_int32_t) :: nmarkers, pad
+    type(c_ptr) :: mark_a1, mark_a2, mark_ntriag, mark_phase, mark_dead, mark_id_elem, nmark_elem
 end type
 
 type, bind(C) :: gf_clock_t
@@ -73,6 +83,11 @@ interface
         import :: c_int, c_ptr, gf_host_arrays_t
         type(c_ptr), value :: ctx
         type(gf_host_arrays_t), intent(in) :: h
+    end function
+    integer(c_int) function gf_bind_markers(ctx, m) bind(C, name='gf_bind_markers')
+        import :: c_int, c_ptr, gf_marker_arrays_t
+        type(c_ptr), value :: ctx
+        type(gf_marker_arrays_t), intent(in) :: m
     end function
     integer(c_int) function gf_upload(ctx, mask) bind(C, name='gf_upload')
         import :: c_int, c_int32_t, c_ptr
@@ -122,6 +137,12 @@ contains
 
     function gf_addr4(a) result(p)
         integer(c_int32_t), target, intent(in) :: a(*)
+        type(c_ptr) :: p
+        p = c_loc(a)
+    end function
+
+    function gf_addr1(a) result(p)
+        integer(c_int8_t), target, intent(in) :: a(*)
         type(c_ptr) :: p
         p = c_loc(a)
     end function

@@ -14,10 +14,10 @@ LIB_PATH = os.path.join(_HERE, "libgfhip.so")
 _lib = None
 
 EXPORTS = [
-    "gf_create", "gf_destroy", "gf_last_error", "gf_error_site", "gf_bind_host", "gf_upload",
+    "gf_create", "gf_destroy", "gf_last_error", "gf_error_site", "gf_bind_host", "gf_bind_markers", "gf_upload",
     "gf_download", "gf_set_clock", "gf_get_clock", "gf_step", "gf_flac", "gf_fl_therm",
-    "gf_fl_rheol", "gf_fl_node", "gf_fl_move", "gf_rmasses", "gf_dt_mass", "gf_init_areas",
-    "gf_init_visc", "gf_therm_stats", "gf_sync",
+    "gf_fl_rheol", "gf_fl_node", "gf_fl_move", "gf_rmasses", "gf_dt_mass", "gf_change_phase", "gf_init_areas",
+    "gf_init_visc", "gf_therm_stats", "gf_set_therm_passes", "gf_sync",
     "gf_set_timing", "gf_kernel_time", "gf_math_check",
     "gf_surrogate_sample", "gf_surrogate_train", "gf_surrogate_enable", "gf_surrogate_stats",
     "gf_surrogate_weights", "gf_mlp_grad", "gf_mlp_infer",
@@ -48,18 +48,21 @@ def lib():
         L.gf_last_error.restype = C.c_char_p
         L.gf_error_site.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
         L.gf_bind_host.argtypes = [vp, C.POINTER(abi.GfHostArrays)]
+        L.gf_bind_markers.argtypes = [vp, C.POINTER(abi.GfMarkerArrays)]
         L.gf_upload.argtypes = [vp, C.c_uint32]
         L.gf_download.argtypes = [vp, C.c_uint32]
         L.gf_set_clock.argtypes = [vp, C.POINTER(abi.GfClock)]
         L.gf_get_clock.argtypes = [vp, C.POINTER(abi.GfClock)]
         L.gf_step.argtypes = [vp, C.c_int32, C.POINTER(abi.GfClock)]
         L.gf_flac.argtypes = [vp, C.POINTER(abi.GfClock)]
-        for n in ("gf_fl_therm", "gf_fl_rheol", "gf_fl_node", "gf_fl_move", "gf_rmasses", "gf_dt_mass"):
+        for n in ("gf_fl_therm", "gf_fl_rheol", "gf_fl_node", "gf_fl_move", "gf_rmasses", "gf_dt_mass",
+                  "gf_change_phase"):
             getattr(L, n).argtypes = [vp, C.POINTER(abi.GfClock)]
         L.gf_init_areas.argtypes = [vp]
         L.gf_init_visc.argtypes = [vp]
         L.gf_therm_stats.argtypes = [vp, C.POINTER(C.c_int64)]
         L.gf_sync.argtypes = [vp]
+        L.gf_set_therm_passes.argtypes = [vp, C.c_int32]
         L.gf_set_timing.argtypes = [vp, C.c_int32]
         L.gf_kernel_time.argtypes = [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         dp = C.POINTER(C.c_double)
@@ -98,6 +101,10 @@ class Model:
             raise GfError(rc, "gf_create rejected the parameters")
         self._host = state.host_arrays()
         self._check(L.gf_bind_host(self._ctx, C.byref(self._host)))
+        self._marks = None
+        if state.has_markers:
+            self._marks = state.marker_arrays()
+            self._check(L.gf_bind_markers(self._ctx, C.byref(self._marks)))
         if upload:
             self.upload()
 
@@ -107,15 +114,24 @@ class Model:
             raise GfError(rc, msg.decode() if msg else "")
         return rc
 
+    def _mask(self, mask):
+        """GF_ALL also carries the markers and the cohesion factors the state holds."""
+        if mask == abi.GF_ALL:
+            if self.state.has_markers:
+                mask |= abi.GF_MARKERS
+            if self.state.coh_scale is not None:
+                mask |= abi.GF_COH_SCALE
+        return mask
+
     def upload(self, mask=abi.GF_ALL):
-        self._check(lib().gf_upload(self._ctx, mask))
+        self._check(lib().gf_upload(self._ctx, self._mask(mask)))
         self._check(lib().gf_set_clock(self._ctx, C.byref(self.state.clock)))
 
     def set_clock(self):
         self._check(lib().gf_set_clock(self._ctx, C.byref(self.state.clock)))
 
     def download(self, mask=abi.GF_ALL):
-        self._check(lib().gf_download(self._ctx, mask))
+        self._check(lib().gf_download(self._ctx, self._mask(mask) & ~abi.GF_COH_SCALE))
         self._check(lib().gf_get_clock(self._ctx, C.byref(self.state.clock)))
 
     def step(self, nsteps):
@@ -123,8 +139,8 @@ class Model:
         return self._check(lib().gf_step(self._ctx, int(nsteps), C.byref(self.state.clock)))
 
     def call(self, name):
-        """One reference subroutine: 'fl_therm', 'fl_rheol', 'fl_node', 'fl_move',
-        'rmasses', 'dt_mass', 'flac'."""
+        """One reference subroutine: 'fl_therm', 'change_phase', 'fl_rheol', 'fl_node',
+        'fl_move', 'rmasses', 'dt_mass', 'flac'."""
         fn = getattr(lib(), "gf_" + name)
         return self._check(fn(self._ctx, C.byref(self.state.clock)))
 
@@ -140,6 +156,10 @@ class Model:
         v = (C.c_int64 * 4)()
         self._check(lib().gf_therm_stats(self._ctx, v))
         return tuple(int(x) for x in v)
+
+    def set_therm_passes(self, n):
+        """Most thermal fixed-point passes per call; an unproven call raises GfError 1004."""
+        return self._check(lib().gf_set_therm_passes(self._ctx, int(n)))
 
     def therm_unconverged(self):
         return self.therm_stats()[0]

@@ -237,6 +237,7 @@ def to_gf_params(p, vbc, ivis_present, pisos=None):
     g.vbc = float(vbc)
     g.topo_kappa = float(p.get("topo_kappa", 0.0))
     g.flexure_base = int(any(b["nbc"] == 200 and b["nofside"] == 2 for b in p["bcs"]))
+    g.nelem_serp = int(p["nelem_serp"])
     g.pisos = float(p["pisos"] if pisos is None else pisos)
     if p["nphase"] > abi.MAXPH:
         raise InpError("nphase exceeds maxph")
@@ -271,6 +272,8 @@ def read_params_txt(path):
             elif t[0] == "flags":
                 for k, v in zip(iflags, t[1:]):
                     setattr(g, k, int(v))
+            elif t[0] == "flags2":
+                g.nelem_serp = int(t[1])
             elif t[0] == "reals":
                 for k, v in zip(reals, t[1:]):
                     if hasattr(g, k):

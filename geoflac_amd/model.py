@@ -2,14 +2,13 @@
 
 `build_box` mirrors src/setflac.f90:4-100 for the inputs this project runs
 (uniform mesh zones, horizontal phase layers, `ictherm` 21 geotherm, velocity
-boundary conditions): init_cord (src/init_cord.f90:6-80), init_temp
-(src/init_temp.f90:26-60,205-213), init_stress (src/init_stress.f90:5-60),
-init_bc/velbc/vbcal (src/init_bc.f90:4-330,376-417).  Phases come from the layer
-depths sampled at the 9 regular marker positions of each element
-(src/init_marker.f90:44-118) without the random jitter, and an optional seeded
-perturbation of the layer interfaces.  The device then computes areas, real and
-inertial masses, viscosity and dt (init_areas, rmasses, init_visc, dt_mass), as
-the reference does at the end of setflac.
+boundary conditions): init_cord (src/init_cord.f90:6-80), the markers and the
+phases they give (init_marker, src/init_marker.f90:1-169, with the serial build's
+seeded random offsets, and count_phase_ratio, src/marker_data.f90:140-174),
+init_temp (src/init_temp.f90:26-60,205-213), init_stress (src/init_stress.f90:5-60),
+init_bc/velbc/vbcal (src/init_bc.f90:4-330,376-417).  The device then computes
+areas, real and inertial masses, viscosity and dt (init_areas, rmasses, init_visc,
+dt_mass), as the reference does at the end of setflac.
 """
 import math
 
@@ -137,6 +136,211 @@ def layer_phases(p, st, interface_noise=0.0, seed=0):
     st.iphase[...] = (np.argmax(counts, axis=0) + 1).astype(np.int32)
 
 
+# ---------------------------------------------------------------- markers
+_LCG_M, _LCG_A, _LCG_C = 134456, 8121, 28411   # src/myrandom_mod.f90:9-16
+
+
+def _lcg(s):
+    """One draw of myrandom (src/myrandom_mod.f90:9-16): the new seed (r = seed / 134456)."""
+    return (_LCG_A * (s % _LCG_M) + _LCG_C) % _LCG_M
+
+
+def _marker_points(cord, j, i):
+    """The 9 regular points of elements (j, i) (0-based index arrays) and dx, dy
+    (src/init_marker.f90:44-82)."""
+    f6 = 0.8333333333333333333333
+    o6 = 0.1666666666666666666666
+    h = 0.5
+    c = lambda jj, ii, d: cord[jj, ii, d]
+    pts = []
+    for wa, wb in ((f6, o6), (h, h), (o6, f6)):
+        ax = c(j, i, 0) * wa + c(j + 1, i, 0) * wb
+        ay = c(j, i, 1) * wa + c(j + 1, i, 1) * wb
+        bx = c(j, i + 1, 0) * wa + c(j + 1, i + 1, 0) * wb
+        by = c(j, i + 1, 1) * wa + c(j + 1, i + 1, 1) * wb
+        for u, v in ((f6, o6), (h, h), (o6, f6)):
+            pts.append((ax * u + bx * v, ay * u + by * v))
+    dx = c(j, i + 1, 0) - c(j, i, 0)
+    dy = c(j + 1, i, 1) - c(j, i, 1)
+    return pts, dx, dy
+
+
+def _check_inside(cord, j, i, x, y):
+    """src/euler2bar.f90:102-161: triangle k (1 or 2, 0 if neither) of element (j, i)
+    holding (x, y) and its barycentric coordinates."""
+    c = cord
+    out_k = np.zeros(np.shape(x), dtype=np.int64)
+    out1 = np.zeros(np.shape(x))
+    out2 = np.zeros(np.shape(x))
+    for k in (2, 1):
+        if k == 1:
+            x1, x2, x3 = c[j, i, 0], c[j + 1, i, 0], c[j, i + 1, 0]
+            y1, y2, y3 = c[j, i, 1], c[j + 1, i, 1], c[j, i + 1, 1]
+        else:
+            x1, x2, x3 = c[j, i + 1, 0], c[j + 1, i, 0], c[j + 1, i + 1, 0]
+            y1, y2, y3 = c[j, i + 1, 1], c[j + 1, i, 1], c[j + 1, i + 1, 1]
+        det = ((x2 * y3 - y2 * x3) - (x1 * y3 - y1 * x3) + (x1 * y2 - y1 * x2))
+        m11, m12, m13 = (x2 * y3 - y2 * x3) / det, (y2 - y3) / det, (x3 - x2) / det
+        m21, m22, m23 = (x3 * y1 - y3 * x1) / det, (y3 - y1) / det, (x1 - x3) / det
+        b1 = m11 + m12 * x + y * m13
+        b2 = m21 + m22 * x + y * m23
+        ok = (b1 >= 0.0) & (b2 >= 0.0) & (b1 + b2 <= 1.0)
+        # triangle 1 is tried first in the reference: assign 2, then let 1 override
+        out_k = np.where(ok, k, out_k)
+        out1 = np.where(ok, b1, out1)
+        out2 = np.where(ok, b2, out2)
+    return out_k, out1, out2
+
+
+def _layer_phase(p, i1, yy, default):
+    """Phase of a marker at depth -yy in column i1 (1-based) from the zones' layers
+    (src/init_marker.f90:86-117)."""
+    kph = np.array(default, copy=True)
+    for zone in p["zones"]:
+        if zone["iph_col_trans"] == 1:
+            raise NotImplementedError("interpolated layer depths (iph_col_trans)")
+        sel = (i1 >= zone["ixtb1"]) & (i1 <= zone["ixtb2"])
+        yyy = yy * (-1e-3)
+        nl = zone["nph_layer"]
+        kz = np.full(np.shape(yy), zone["iph_col"][nl - 1])
+        for m in range(nl - 2, -1, -1):   # the first layer whose base lies below the marker
+            kz = np.where(yyy < zone["hc"][m], zone["iph_col"][m], kz)
+        kph = np.where(sel, kz, kph)
+    return kph
+
+
+def _seed_chain(seed, inc, a18, b18):
+    """Start seeds of consecutive elements that draw 18 numbers each: the element adds
+    inc (its i + j) to the seed left by the one before.  The map from one element's
+    end seed to the next is x -> (a18 x + a18 inc + b18) mod m, so the end seeds are a
+    linear recurrence, evaluated by a doubling scan."""
+    m = _LCG_M
+    n = len(inc)
+    b = (a18 * (inc % m) + b18) % m
+    b[0] = (a18 * ((seed + inc[0]) % m) + b18) % m
+    t, apow = b.copy(), a18
+    shift = 1
+    while shift < n:
+        t[shift:] = (t[shift:] + apow * t[:-shift]) % m
+        apow = (apow * apow) % m
+        shift *= 2
+    ends = t
+    starts = np.empty(n, dtype=np.int64)
+    starts[0] = seed + inc[0]
+    starts[1:] = ends[:-1] + inc[1:]
+    return starts, int(ends[-1])
+
+
+def init_markers(p, st):
+    """Markers of a new start, as src/init_marker.f90:1-169 places them in the serial
+    build: 9 per element at the regular points, each moved by a random offset of up
+    to a sixth of the element from the serial myrandom sequence (seed 0, advanced by
+    i + j per element, i outer), the phase from the layer depths, the rectangular
+    and diagonal inhomogeneities (newphase2marker, src/marker_data.f90:110-136), then
+    the phase ratios and iphase of every element (count_phase_ratio,
+    src/marker_data.f90:140-174).  A point that falls outside its element (possible
+    only on its boundary) is drawn again, which shifts the rest of the sequence: the
+    elements are evaluated in chunks, and a chunk is evaluated again from the element
+    after such a redraw."""
+    nx, nz = p["nx"], p["nz"]
+    ez, ex = nz - 1, nx - 1
+    ne = ez * ex
+    cord = st.cord
+    e_all = np.arange(ne)
+    J, I = e_all % ez, e_all // ez          # 0-based element indices, Fortran order
+    mark_a1 = np.zeros((ne, 9))
+    mark_a2 = np.zeros((ne, 9))
+    ntri = np.zeros((ne, 9), dtype=np.int64)
+    phase = np.zeros((ne, 9), dtype=np.int64)
+    # affine map of 18 draws: L^18(s) = (A18 * (s mod m) + B18) mod m
+    a18, b18 = 1, 0
+    for _ in range(18):
+        a18, b18 = (_LCG_A * a18) % _LCG_M, (_LCG_A * b18 + _LCG_C) % _LCG_M
+    inc_ij = (I + 1 + J + 1).astype(np.int64)
+    CH = 1 << 16
+    seed, e0 = 0, 0
+    while e0 < ne:
+        e1 = min(ne, e0 + CH)
+        idx = np.arange(e0, e1)
+        starts, end_seed = _seed_chain(seed, inc_ij[e0:e1], a18, b18)
+        jj, ii = J[idx], I[idx]
+        pts, dx, dy = _marker_points(cord, jj, ii)
+        sd = starts.copy()
+        rej = np.zeros(len(idx), dtype=bool)
+        for l in range(9):
+            sd = _lcg(sd); rx = sd * 1.0 / _LCG_M
+            sd = _lcg(sd); ry = sd * 1.0 / _LCG_M
+            rx = 0.5 - rx
+            ry = 0.5 - ry
+            xx = pts[l][0] + dx * rx / 3
+            yy = pts[l][1] + dy * ry / 3
+            k, b1, b2 = _check_inside(cord, jj, ii, xx, yy)
+            rej |= k == 0
+            mark_a1[idx, l], mark_a2[idx, l] = b1, b2
+            ntri[idx, l] = 2 * (ez * ii + jj) + k
+            phase[idx, l] = _layer_phase(p, ii + 1, yy, st.iphase[jj, ii])
+        if not rej.any():
+            seed, e0 = end_seed, e1
+            continue
+        # the first element with a redraw: serially, then go on after it
+        q = int(np.argmax(rej))
+        e = e0 + q
+        j0, i0 = int(J[e]), int(I[e])
+        pts1, dx1, dy1 = _marker_points(cord, np.array([j0]), np.array([i0]))
+        s = int(starts[q])
+        l = 0
+        while l < 9:
+            s = _lcg(s); rx = 0.5 - s * 1.0 / _LCG_M
+            s = _lcg(s); ry = 0.5 - s * 1.0 / _LCG_M
+            xx = pts1[l][0] + dx1 * rx / 3
+            yy = pts1[l][1] + dy1 * ry / 3
+            k, b1, b2 = _check_inside(cord, np.array([j0]), np.array([i0]), xx, yy)
+            if k[0] == 0:
+                continue
+            mark_a1[e, l], mark_a2[e, l] = b1[0], b2[0]
+            ntri[e, l] = 2 * (ez * i0 + j0) + k[0]
+            phase[e, l] = _layer_phase(p, np.array([i0 + 1]), yy, np.array([st.iphase[j0, i0]]))[0]
+            l += 1
+        seed, e0 = s, e + 1
+    mark_id = np.zeros((32, ez, ex), dtype=np.int32, order="F")
+    ids = (np.arange(ne * 9) + 1).reshape(ne, 9)          # markers numbered element by element
+    mark_id[:9] = ids.T.reshape(9, ez, ex, order="F")
+    nmark = np.full((ez, ex), 9, dtype=np.int32, order="F")
+    mphase = phase.ravel()
+    # initial inhomogeneities (src/init_marker.f90:128-158)
+    for ih in p.get("inhoms", []):
+        if ih["inphase"] < 0:
+            continue
+        cells = []
+        if ih["igeom"] == 0:
+            cells = [(j, i) for i in range(ih["ix1"], ih["ix2"] + 1) for j in range(ih["iy1"], ih["iy2"] + 1)]
+        elif ih["igeom"] in (3, 4):
+            f32 = np.float32
+            slope = f32(ih["iy2"] - ih["iy1"]) / f32(ih["ix2"] - ih["ix1"])
+            for i in range(ih["ix1"], ih["ix2"] + 1):
+                v = slope * f32(i - ih["ix1"])
+                if ih["igeom"] == 3:
+                    k = int(np.rint(v)) + ih["iy1"]   # nint: halves away from zero
+                    if abs(v - np.floor(v) - 0.5) == 0.0:
+                        k = int(np.floor(v) + 1) + ih["iy1"]
+                    cells += [(k, i)]
+                else:
+                    cells += [(j, i) for j in range(int(np.floor(v)) + ih["iy1"], int(np.ceil(v)) + ih["iy1"] + 1)]
+        for j, i in cells:
+            e = (j - 1) + ez * (i - 1)
+            mphase[e * 9:(e + 1) * 9] = ih["inphase"]
+    st.set_markers(mark_a1.ravel(), mark_a2.ravel(), ntri.ravel(), mphase.astype(np.int8),
+                   np.ones(ne * 9, dtype=np.int8), mark_id, nmark)
+    # count_phase_ratio of every element (marker2elem, src/marker2elem.f90:60)
+    nph = p["nphase"]
+    npc = max(nph, int(mphase.max())) + 1
+    counts = np.bincount(np.repeat(e_all, 9) * npc + mphase, minlength=ne * npc).reshape(ne, npc)
+    ratio = counts[:, 1:nph + 1] / (9 * 1.0)
+    st.phase_ratio[...] = ratio.T.reshape(nph, ez, ex, order="F")
+    first_max = np.argmax(counts[:, 1:nph + 1], axis=1) + 1
+    st.iphase[...] = first_max.reshape(ez, ex, order="F").astype(np.int32)
+
+
 def init_temp(p, st):
     """src/init_temp.f90:26-60 with the ictherm 21 profile (:205-213) and sources (:48-52)."""
     nx, nz = p["nx"], p["nz"]
@@ -185,15 +389,40 @@ def init_stress(p, st):
     return pisos
 
 
-def build_box(p, interface_noise=0.0, seed=0):
-    """State and parameter block of a new start (before the device-side set-up)."""
+def perturb_layers(p, interface_noise, seed):
+    """A copy of p whose layer interfaces are moved per zone by a seeded uniform
+    offset of +-interface_noise km (one offset per interface; a build-side extension
+    for heterogeneous benchmarks)."""
+    import copy
+    q = copy.deepcopy(p)
+    rng = np.random.default_rng(seed)
+    for z in q["zones"]:
+        z["hc"] = [h + float(rng.uniform(-interface_noise, interface_noise)) for h in z["hc"]]
+    return q
+
+
+def build_box(p, interface_noise=0.0, seed=0, markers=True, coh_noise=0.0):
+    """State and parameter block of a new start (before the device-side set-up).
+
+    markers: place the reference's markers (init_markers) and take the phases from
+    them, as setflac does; otherwise phases come from the 9 regular points of each
+    element with seeded per-column interface offsets of +-interface_noise km
+    (layer_phases).  coh_noise: a seeded per-element cohesion factor 1 + coh_noise*U,
+    U uniform in [-1, 1] (a build-side extension; 0: none)."""
     nx, nz = p["nx"], p["nz"]
     st = State(nx, nz, p["nphase"])
     xm = mesh1(p["x0"], p["x0"] + p["rxbo"], nx, p["nelz_x"], p["sizez_x"])
     st.cord[:, :, 0] = xm[None, :]
     zm = mesh1(p["z0"], p["z0"] + p["rzbo"], nz, p["nelz_y"], p["sizez_y"])
     st.cord[:, :, 1] = zm[:, None]
-    layer_phases(p, st, interface_noise, seed)
+    if markers:
+        init_markers(p, st)
+    else:
+        layer_phases(p, st, interface_noise, seed)
+    if coh_noise:
+        rng = np.random.default_rng(seed + 1)
+        st.coh_scale = np.asfortranarray(
+            (1.0 + coh_noise * rng.uniform(-1.0, 1.0, size=(nz - 1, nx - 1))).astype(np.float32))
     ivis = int(any(p["phases"][k - 1]["irheol"] in (3,) or p["phases"][k - 1]["irheol"] >= 11
                    for k in np.unique(st.iphase)))
     init_temp(p, st)

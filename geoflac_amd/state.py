@@ -15,6 +15,7 @@ from . import abi
 NODE2 = ("cord", "vel", "force", "bc")
 NODE1 = ("temp", "temp0", "amass", "rmass")
 ELEM1 = ("aps", "visn", "e2sr", "sshrheat", "shrheat", "source")
+MARKERS = ("mark_a1", "mark_a2", "mark_ntriag", "mark_phase", "mark_dead", "mark_id_elem", "nmark_elem")
 
 
 class State:
@@ -41,6 +42,40 @@ class State:
         self.iphase = np.ones((ez, ex), dtype=np.int32, order="F")
         self.phase_ratio = np.zeros((self.nphase, ez, ex), **f8)
         self.clock = abi.GfClock()
+        # per-element cohesion factor (seeded heterogeneity; no reference counterpart)
+        self.coh_scale = None
+        # markers (module marker_data, src/marker_data.f90:1-20), None until set
+        self.nmarkers = 0
+        for n in MARKERS:
+            setattr(self, n, None)
+
+    def set_markers(self, a1, a2, ntriag, phase, dead, id_elem, nmark_elem):
+        """Marker arrays in the reference's layout: per marker a1, a2 (barycentric),
+        ntriag (triangle index), phase, dead (1 alive); per element id_elem[32, ez, ex]
+        (1-based marker numbers) and nmark_elem[ez, ex]."""
+        ez, ex = self.nz - 1, self.nx - 1
+        self.nmarkers = int(len(a1))
+        self.mark_a1 = np.ascontiguousarray(a1, dtype=np.float64)
+        self.mark_a2 = np.ascontiguousarray(a2, dtype=np.float64)
+        self.mark_ntriag = np.ascontiguousarray(ntriag, dtype=np.int32)
+        self.mark_phase = np.ascontiguousarray(phase, dtype=np.int8)
+        self.mark_dead = np.ascontiguousarray(dead, dtype=np.int8)
+        self.mark_id_elem = np.asfortranarray(np.asarray(id_elem, dtype=np.int32).reshape(
+            (abi.MAX_MARKERS_PER_ELEM, ez, ex), order="F"))
+        self.nmark_elem = np.asfortranarray(np.asarray(nmark_elem, dtype=np.int32).reshape((ez, ex), order="F"))
+
+    @property
+    def has_markers(self):
+        return self.mark_a1 is not None
+
+    def marker_arrays(self):
+        m = abi.GfMarkerArrays()
+        m.nmarkers = self.nmarkers
+        for name, ptype in abi.MARKER_FIELDS:
+            a = getattr(self, name)
+            assert a.flags.f_contiguous
+            setattr(m, name, a.ctypes.data_as(ptype))
+        return m
 
     @property
     def nelem(self):
@@ -52,6 +87,9 @@ class State:
             a = getattr(self, name)
             assert a.flags.f_contiguous
             setattr(h, name, a.ctypes.data_as(ptype))
+        if self.coh_scale is not None:
+            assert self.coh_scale.flags.f_contiguous and self.coh_scale.dtype == np.float32
+            h.coh_scale = self.coh_scale.ctypes.data_as(abi._FP)
         return h
 
     def copy(self):
@@ -59,6 +97,10 @@ class State:
         for n in self.FIELDS:
             getattr(s, n)[...] = getattr(self, n)
         C.memmove(C.addressof(s.clock), C.addressof(self.clock), C.sizeof(abi.GfClock))
+        if self.coh_scale is not None:
+            s.coh_scale = self.coh_scale.copy(order="F")
+        if self.has_markers:
+            s.set_markers(*(getattr(self, n).copy(order="F") for n in MARKERS))
         return s
 
     def clock_dict(self):
@@ -94,6 +136,13 @@ def load_dump(path, nphase=None):
         arr = np.fromfile(os.path.join(path, name + ext), dtype=dt)
         tgt = getattr(st, name)
         tgt[...] = arr.reshape(tgt.shape, order="F")
+    if os.path.exists(os.path.join(path, "mark_a1.f64")):
+        rd = lambda n, dt: np.fromfile(os.path.join(path, n + (".f64" if dt is np.float64 else ".i32")), dtype=dt)
+        st.set_markers(rd("mark_a1", np.float64), rd("mark_a2", np.float64), rd("mark_ntriag", np.int32),
+                       rd("mark_phase", np.int32).astype(np.int8), rd("mark_dead", np.int32).astype(np.int8),
+                       rd("mark_id_elem", np.int32), rd("nmark_elem", np.int32))
+    elif os.path.exists(os.path.join(path, "mark_phase.i32")):
+        st.mark_phase_only = np.fromfile(os.path.join(path, "mark_phase.i32"), dtype=np.int32).astype(np.int8)
     c = st.clock
     c.nloop, c.nsrate = scal["nloop"], scal["nsrate"]
     c.time, c.dt, c.dtavg = scal["time"], scal["dt"], scal["dtavg"]
@@ -104,7 +153,13 @@ def load_dump(path, nphase=None):
 
 def save_npz(state, path, fields=None):
     fields = state.FIELDS if fields is None else fields
-    d = {n: getattr(state, n) for n in fields}
+    d = {n: getattr(state, n) for n in fields if n != "mark_phase"}
+    if state.has_markers and fields is state.FIELDS:
+        d.update({n: getattr(state, n) for n in MARKERS})
+    elif "mark_phase" in fields:
+        mp = state.mark_phase if state.has_markers else getattr(state, "mark_phase_only", None)
+        if mp is not None:
+            d["mark_phase"] = mp
     d["_dims"] = np.array([state.nx, state.nz, state.nphase], dtype=np.int64)
     d["_clock_i"] = np.array([state.clock.nloop, state.clock.nsrate], dtype=np.int64)
     d["_clock_f"] = np.array([state.clock.time, state.clock.dt, state.clock.dtavg,
@@ -119,6 +174,10 @@ def load_npz(path):
     for n in State.FIELDS:
         if n in z.files:
             getattr(st, n)[...] = z[n]
+    if "mark_a1" in z.files:
+        st.set_markers(*(z[n] for n in MARKERS))
+    elif "mark_phase" in z.files:
+        st.mark_phase_only = z["mark_phase"]
     st.clock.nloop, st.clock.nsrate = (int(v) for v in z["_clock_i"])
     (st.clock.time, st.clock.dt, st.clock.dtavg,
      st.clock.dt_elastic, st.clock.dt_maxwell) = (float(v) for v in z["_clock_f"])

@@ -46,6 +46,7 @@ extern "C" {
 #define GF_ERR_UNSUPPORTED 1002   /* parameter selects a path this library does not provide */
 #define GF_ERR_STATE 1003         /* call out of order (e.g. step before upload) */
 #define GF_ERR_THERM_SWEEPS 1004  /* thermal fixed-point sweeps did not converge bitwise */
+#define GF_ERR_PHASES 1005        /* change_phase produced more distinct phases than the device keeps planes for */
 
 /* Mirror of the scalars and per-phase tables of module params (src/params.f90:17-135)
  * that the explicit step reads.  Integers first, then reals; the Fortran
@@ -61,6 +62,8 @@ typedef struct gf_params {
     int32_t irheol[GF_MAXPH];
     int32_t ix1[GF_MAXINH], ix2[GF_MAXINH], iy1[GF_MAXINH], iy2[GF_MAXINH], igeom[GF_MAXINH];
     int32_t flexure_base;                   /* 1 when side 2 carries nbc == 200 (src/fl_flexure.f90:15-24) */
+    int32_t nelem_serp;                     /* elements below searched by the serpentinite rule (src/change_phase.f90:139) */
+    int32_t pad_i;
     double t_top, t_bot, bot_bc, g, demf, ten_off, tau_heal, v_min, v_max;
     double frac, fracm, strain_inert, dt_scale, xReyn, vbc, rxbo, rzbo;
     double pisos, drosub, fmagma_max, weaken_ratio_plastic, weaken_ratio_viscous, rho_magma;
@@ -81,6 +84,10 @@ typedef struct gf_host_arrays {
     double *aps, *visn, *e2sr, *sshrheat, *shrheat, *source;
     int32_t *iphase;
     double *phase_ratio;
+    /* No reference counterpart: a per-element factor on the mixed cohesion of pre_plast
+     * (src/rh_plastic.f90:231-291), for seeded heterogeneity; NULL or never uploaded
+     * means 1 everywhere. */
+    float *coh_scale;
 } gf_host_arrays;
 
 /* field bits for gf_upload / gf_download */
@@ -91,8 +98,26 @@ enum {
     GF_STRAIN = 1u << 12, GF_SE2SR = 1u << 13, GF_APS = 1u << 14, GF_VISN = 1u << 15,
     GF_E2SR = 1u << 16, GF_SSHRHEAT = 1u << 17, GF_SHRHEAT = 1u << 18, GF_SOURCE = 1u << 19,
     GF_IPHASE = 1u << 20, GF_PHASE_RATIO = 1u << 21,
-    GF_ALL = (1u << 22) - 1u
+    GF_ALL = (1u << 22) - 1u,   /* every array of module arrays above */
+    GF_MARKERS = 1u << 22,      /* upload: every marker array; download: mark_phase */
+    GF_COH_SCALE = 1u << 23     /* upload only */
 };
+
+/* Markers of module marker_data (src/marker_data.f90:1-20), read and written by
+ * change_phase (src/change_phase.f90:1-260), which the step calls every
+ * ifreq_rmasses steps (src/flac.f90:29).  Each marker sits in triangle mark_ntriag
+ * = 2*((nz-1)*(i-1)+j-1)+k (k 1 or 2, src/euler2bar.f90:153) at barycentric
+ * coordinates (mark_a1, mark_a2); mark_id_elem(1:nmark_elem(j,i), j, i) lists an
+ * element's markers (1-based).  Only the first nmarkers entries are read. */
+#define GF_MAX_MARKERS_PER_ELEM 32   /* src/marker_data.f90:5 */
+typedef struct gf_marker_arrays {
+    int32_t nmarkers, pad;
+    double *mark_a1, *mark_a2;
+    int32_t *mark_ntriag;
+    int8_t *mark_phase, *mark_dead;
+    int32_t *mark_id_elem;   /* (GF_MAX_MARKERS_PER_ELEM, nz-1, nx-1) */
+    int32_t *nmark_elem;     /* (nz-1, nx-1) */
+} gf_marker_arrays;
 
 /* Time-stepping scalars of module params that change during a run
  * (nloop, nsrate, time, dt, dtavg, dt_elastic, dt_maxwell; src/params.f90:38,96-106). */
@@ -122,6 +147,11 @@ const char *gf_last_error(const gf_ctx *ctx);
 int gf_error_site(const gf_ctx *ctx, int32_t *j, int32_t *i);
 
 int gf_bind_host(gf_ctx *ctx, const gf_host_arrays *h);
+/* Raw pointers to the marker arrays (module marker_data, allocated by allocate_markers,
+ * src/marker_data.f90:22-38).  The step runs change_phase on the device once they are
+ * uploaded (GF_MARKERS); without them a step refuses (GF_ERR_STATE) when a phase with a
+ * change_phase rule is present, and change_phase is a no-op otherwise. */
+int gf_bind_markers(gf_ctx *ctx, const gf_marker_arrays *m);
 /* Host -> device and device -> host copies of the fields in `mask`, with the
  * layout conversion to the device's own planes.  Upload of everything is what
  * the reference does at setflac/rsflac time (src/setflac.f90:4-100); download is
@@ -146,6 +176,7 @@ int gf_fl_node(gf_ctx *ctx, gf_clock *clock);    /* src/fl_node.f90:3-393  */
 int gf_fl_move(gf_ctx *ctx, gf_clock *clock);    /* src/fl_move.f90:4-132  */
 int gf_rmasses(gf_ctx *ctx, gf_clock *clock);    /* src/rmasses.f90:4-63   */
 int gf_dt_mass(gf_ctx *ctx, gf_clock *clock);    /* src/dt_mass.f90:16-160 */
+int gf_change_phase(gf_ctx *ctx, gf_clock *clock); /* src/change_phase.f90:1-260 (itype_melting=0 part) */
 int gf_init_areas(gf_ctx *ctx);                  /* src/init_areas.f90:7-55 */
 int gf_init_visc(gf_ctx *ctx);                   /* src/init_visc.f90:1-18  */
 
@@ -156,6 +187,14 @@ int gf_init_visc(gf_ctx *ctx);                   /* src/init_visc.f90:1-18  */
  * proofs needed; stats[2]: tile border values that differed from their owner's;
  * stats[3]: the largest such difference in units in the last place. */
 int gf_therm_stats(gf_ctx *ctx, int64_t stats[4]);
+
+/* Most fixed-point passes of the thermal step per call (1..4; default 4).  The FP64
+ * thermal step reproduces the reference's in-place serial sweep (src/fl_therm.f90:
+ * 393-571) by a tiled fixed-point iteration whose result a border check proves equal
+ * to the serial one; when no pass within the limit is proven, the step goes on with
+ * the last pass and the call returns GF_ERR_THERM_SWEEPS.  No reference counterpart
+ * (a limit for tests and callers that prefer an error to a fallback pass). */
+int gf_set_therm_passes(gf_ctx *ctx, int32_t npass);
 
 /* Synchronise the context's stream (all queued steps finished). */
 int gf_sync(gf_ctx *ctx);

@@ -289,6 +289,13 @@ static void pre_plast(const gf_params *p, const gf_host_arrays *s, int i, int j,
     *phi = 1 / *phi;
     *coh = 1 / *coh;
     /* melt weakening needs itype_melting == 1, rejected by the ABI; fmagma is zero */
+    /* seeded per-element cohesion factor (no reference counterpart; absent: 1) */
+    if (s->coh_scale) *coh = *coh * (double)s->coh_scale[IE(j, i)];
+}
+
+void gfo_pre_plast(const gf_params *p, const gf_host_arrays *s, int j, int i, double out[4])
+{
+    pre_plast(p, s, i, j, &out[0], &out[1], &out[2], &out[3]);
 }
 
 /* src/matprops.f90:136-215 (power-law creep viscosity, harmonic phase mix) */
@@ -1132,13 +1139,154 @@ int gfo_init_visc(const gf_params *p, gf_host_arrays *s)
     return 0;
 }
 
+/* ---------------------------------------------------------------------------
+ * change_phase  src/change_phase.f90:1-260 (the itype_melting = 0 part), with
+ * count_phase_ratio  src/marker_data.f90:140-174
+ * ------------------------------------------------------------------------- */
+static const gf_marker_arrays *g_mk = NULL;
+
+void gfo_bind_markers(const gf_marker_arrays *m) { g_mk = m; }
+
+/* phase ids of src/phases.f90:4-22 */
+enum { KSED1 = 1, KCONT1 = 2, KOCEAN1 = 3, KMANT1 = 4, KSCHIST = 5, KCONT2 = 6, KOCEAN2 = 7, KMANT2 = 8,
+       KSERP = 9, KOCEANDRY = 10, KSED2 = 11, KECLG = 13, KHYDMANT = 16, KMETASED = 17, KDRYMANT = 18,
+       KAMPHI = 19 };
+
+/* phase_ratio of phase ph, 0 when the deck has fewer phases */
+static double pr_or0(const gf_params *p, const gf_host_arrays *s, int ph, int j, int i)
+{
+    return ph <= p->nphase ? PR(ph, j, i) : 0.0;
+}
+
+int gfo_change_phase(const gf_params *p, gf_host_arrays *s, gf_clock *c)
+{
+    const gf_marker_arrays *m = g_mk;
+    (void)c;
+    if (!m) return 0;
+    const double max_basalt_depth = 150.e3, min_eclogite_temp = 400.0, min_eclogite_depth = 20e3,
+                 mantle_density = 3000.0, max_melting_depth = 200.e3, serpentine_temp = 550.0;
+    char *itmp = (char *)calloc((size_t)NE, 1);
+    if (!itmp) return 1000;
+    /* the MORB search of src/change_phase.f90:39-52 feeds only a commented-out loop */
+    for (long kk = 0; kk < m->nmarkers; kk++) {
+        if (m->mark_dead[kk] == 0) continue;
+        int n = m->mark_ntriag[kk];
+        int k = (n - 1) % 2 + 1;
+        int j = ((n - k) / 2) % (NZ - 1) + 1;
+        int i = (n - k) / 2 / (NZ - 1) + 1;
+        double a1 = m->mark_a1[kk], a2 = m->mark_a2[kk], yy, tmpr, depth, trpres, trpres2, press, solidus;
+        if (k == 1) {
+            yy = CORD(j, i, 2) * a1 + CORD(j + 1, i, 2) * a2 + CORD(j, i + 1, 2) * (1 - a1 - a2);
+            tmpr = TEMP(j, i) * a1 + TEMP(j + 1, i) * a2 + TEMP(j, i + 1) * (1 - a1 - a2);
+        } else {
+            yy = CORD(j, i + 1, 2) * a1 + CORD(j + 1, i, 2) * a2 + CORD(j + 1, i + 1, 2) * (1 - a1 - a2);
+            tmpr = TEMP(j, i + 1) * a1 + TEMP(j + 1, i) * a2 + TEMP(j + 1, i + 1) * (1 - a1 - a2);
+        }
+        depth = 0.5 * (CORD(1, i, 2) + CORD(1, i + 1, 2)) - yy;
+        if (depth > max_melting_depth) continue;
+        int iph = m->mark_phase[kk], newph = 0;
+        switch (iph) {
+        case KMANT1:
+        case KMANT2:
+            if (depth > max_basalt_depth) break;
+            trpres = 2.1e9 + (7.5e9 - 2.1e9) * (tmpr - 730.0) / (500.0 - 730.0);
+            trpres2 = 2.1e9 + (0.6e9 - 2.1e9) * (tmpr - 730.0) / (670.0 - 730.0);
+            press = mantle_density * p->g * depth;
+            solidus = -5.02 * (fabs(press * 1e-9) * fabs(press * 1e-9)) + 1.32e2 * fabs(press * 1e-9) + 1.09e3 + 35.0;
+            if (press < trpres && press > trpres2) {
+                int jhi = j + p->nelem_serp < NZ - 1 ? j + p->nelem_serp : NZ - 1;
+                for (int jb = (j + 1 < NZ - 1 ? j + 1 : NZ - 1); jb <= jhi; jb++) {
+                    if (pr_or0(p, s, KOCEAN1, jb, i) > 0.8 || pr_or0(p, s, KOCEAN2, jb, i) > 0.8 ||
+                        pr_or0(p, s, KSED1, jb, i) > 0.8 || pr_or0(p, s, KSED2, jb, i) > 0.8) {
+                        newph = KSERP;
+                        break;
+                    }
+                }
+            } else if (tmpr >= solidus + 35.0) {
+                newph = KDRYMANT;
+            }
+            break;
+        case KOCEANDRY:
+        case KOCEAN1:
+        case KOCEAN2:
+            trpres = -0.3e9 + 2.2e6 * tmpr;
+            trpres2 = 20.1e9 - 0.0375e9 * tmpr;
+            press = mantle_density * p->g * depth;
+            if (tmpr < min_eclogite_temp || depth < min_eclogite_depth || press < trpres || press < trpres2) break;
+            newph = KECLG;
+            break;
+        case KSERP:
+            trpres = 2.1e9 + (7.5e9 - 2.1e9) * (tmpr - 730.0) / (500.0 - 730.0);
+            trpres2 = 2.1e9 + (0.2e9 - 2.1e9) * (tmpr - 730.0) / (650.0 - 730.0);
+            press = mantle_density * p->g * depth;
+            if (tmpr < serpentine_temp || (press < trpres && press > trpres2)) break;
+            newph = KMANT1;
+            break;
+        case KSED1:
+            if (tmpr < 80.0 && depth < 2e3) break;
+            newph = KSED2;
+            break;
+        case KSED2:
+            if (tmpr < 250.0 && depth < 7e3) break;
+            newph = KMETASED;
+            break;
+        case KMETASED:
+            trpres = 0.88e9 - (0.88e9 - 0.704e9) * (710.8 - tmpr) / (710.8 - 662.3);
+            press = mantle_density * p->g * depth;
+            if (press < trpres) break;
+            newph = KSCHIST;
+            break;
+        case KSCHIST:
+            trpres = 0.0e9 - (0.0e9 - 1.2e9) * (500.0 - tmpr) / (500.0 - 600.0);
+            press = mantle_density * p->g * depth;
+            if (press > trpres) break;
+            newph = KAMPHI;
+            break;
+        case KHYDMANT: {
+            double trtmpr = 880 - 35e-9 * ((depth - 62e3) * (depth - 62e3));
+            if (tmpr < trtmpr) break;
+            newph = KMANT1;
+            break;
+        }
+        default:
+            break;
+        }
+        if (newph) {
+            itmp[IE(j, i)] = 1;
+            m->mark_phase[kk] = (int8_t)newph;
+        }
+    }
+    /* recount the elements with changed markers; aps loses the share of one marker
+     * (dummye holds aps before any change) */
+    for (int i = 1; i <= NX - 1; i++)
+        for (int j = 1; j <= NZ - 1; j++) {
+            if (!itmp[IE(j, i)]) continue;
+            int nm = m->nmark_elem[IE(j, i)], cnt[GF_MAXPH + 1] = {0};
+            if (nm == 0) { free(itmp); return 1001; }   /* 'No markers in element' */
+            for (int q = 0; q < nm; q++) {
+                int kk = m->mark_id_elem[q + (long)GF_MAX_MARKERS_PER_ELEM * IE(j, i)];
+                cnt[m->mark_phase[kk - 1]] += 1;
+            }
+            for (int kp = 1; kp <= p->nphase; kp++) PR(kp, j, i) = cnt[kp] / (nm * 1.0);
+            int iph = 1, nmax = 0;
+            for (int kp = 1; kp <= p->nphase; kp++)
+                if (cnt[kp] > nmax) { nmax = cnt[kp]; iph = kp; }
+            IPHASE(j, i) = iph;
+            double old = APS(j, i);
+            double v = old - old / (double)(float)nm;
+            APS(j, i) = v > 0.0 ? v : 0.0;
+        }
+    free(itmp);
+    return 0;
+}
+
 /* src/flac.f90:18-83 */
 int gfo_flac(const gf_params *p, gf_host_arrays *s, gf_clock *c)
 {
     int st = gfo_fl_therm(p, s, c);
     if (st) return st;
     if (p->itherm != 2) {
-        /* change_phase (every ifreq_rmasses) changes nothing for the supported phase sets */
+        if (c->nloop % p->ifreq_rmasses == 0 && (st = gfo_change_phase(p, s, c))) return st;
         if ((st = gfo_fl_rheol(p, s, c))) return st;
         if ((st = gfo_fl_node(p, s, c))) return st;
         if ((st = gfo_fl_move(p, s, c))) return st;
@@ -1146,6 +1294,16 @@ int gfo_flac(const gf_params *p, gf_host_arrays *s, gf_clock *c)
         if (c->nloop % p->ifreq_imasses == 0 && (st = gfo_dt_mass(p, s, c))) return st;
     }
     return 0;
+}
+
+/* sizes of the ABI structs as the C compiler lays them out (a check of the Python
+ * and Fortran mirrors): gf_params, gf_host_arrays, gf_clock, gf_marker_arrays */
+void gfo_abi_sizes(long out[4])
+{
+    out[0] = (long)sizeof(gf_params);
+    out[1] = (long)sizeof(gf_host_arrays);
+    out[2] = (long)sizeof(gf_clock);
+    out[3] = (long)sizeof(gf_marker_arrays);
 }
 
 /* the transcendental functions of the library (fn: 0 sin, 1 tan, 2 exp, 3 pow) */

@@ -22,17 +22,23 @@ int gfo_fl_node(const gf_params *p, gf_host_arrays *s, gf_clock *c);
 int gfo_fl_move(const gf_params *p, gf_host_arrays *s, gf_clock *c);
 int gfo_rmasses(const gf_params *p, gf_host_arrays *s, gf_clock *c);
 int gfo_dt_mass(const gf_params *p, gf_host_arrays *s, gf_clock *c);
+/* change_phase runs on the markers bound here (NULL: none, it does nothing) */
+void gfo_bind_markers(const gf_marker_arrays *m);
+int gfo_change_phase(const gf_params *p, gf_host_arrays *s, gf_clock *c);
 int gfo_init_areas(const gf_params *p, gf_host_arrays *s);
 int gfo_init_visc(const gf_params *p, gf_host_arrays *s);
 /* one call of subroutine flac (src/flac.f90:5-91) */
 int gfo_flac(const gf_params *p, gf_host_arrays *s, gf_clock *c);
 /* nsteps iterations of src/par.f90:81-91 */
 int gfo_step(const gf_params *p, gf_host_arrays *s, gf_clock *c, int nsteps);
+void gfo_abi_sizes(long out[4]);
 /* the transcendental functions in use (fn: 0 sin, 1 tan, 2 exp, 3 pow) */
 double gfo_math(int fn, double x, double y);
 /* closed-form pieces, exposed for known-answer tests */
 double gfo_eff_visc(const gf_params *p, const gf_host_arrays *s, int j, int i);
 double gfo_eff_dens(const gf_params *p, const gf_host_arrays *s, int j, int i);
+/* pre_plast of element (j,i): out = coh, phi, psi, hardn */
+void gfo_pre_plast(const gf_params *p, const gf_host_arrays *s, int j, int i, double out[4]);
 void gfo_plastic(double bulkm, double rmu, double coh, double phi, double psi, double *depls,
                  int *ipls, double *diss, double hardn, double *s11, double *s22, double *s33,
                  double *s12, double de11, double de22, double de33, double de12, double ten_off,

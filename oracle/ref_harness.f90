@@ -126,6 +126,16 @@ subroutine dump_state(base, step)
     call wr8(dir, 'source', source, ne)
     call wr4(dir, 'iphase', iphase, ne)
     call wr8(dir, 'phase_ratio', phase_ratio, ne*nphase)
+    ! markers (module marker_data, src/marker_data.f90:1-20): the first nmarkers entries
+    call wr4(dir, 'mark_phase', int(mark_phase(1:nmarkers)), nmarkers)
+    call wr4(dir, 'nmark_elem', nmark_elem, ne)
+    if (step == 0) then
+        call wr8(dir, 'mark_a1', mark_a1, nmarkers)
+        call wr8(dir, 'mark_a2', mark_a2, nmarkers)
+        call wr4(dir, 'mark_ntriag', mark_ntriag, nmarkers)
+        call wr4(dir, 'mark_dead', int(mark_dead(1:nmarkers)), nmarkers)
+        call wr4(dir, 'mark_id_elem', mark_id_elem, max_markers_per_elem*ne)
+    endif
     open(newunit=u, file=trim(dir)//'/scalars.txt', status='replace')
     write(u, '(A,I12)') 'nloop ', nloop
     write(u, '(A,I12)') 'nsrate ', nsrate
@@ -135,6 +145,7 @@ subroutine dump_state(base, step)
     write(u, '(A,1X,ES25.17E3)') 'dt_elastic ', dt_elastic
     write(u, '(A,1X,ES25.17E3)') 'dt_maxwell ', dt_maxwell
     write(u, '(A,1X,ES25.17E3)') 'frac ', frac
+    write(u, '(A,I12)') 'nmarkers ', nmarkers
     close(u)
 end subroutine
 
@@ -150,6 +161,7 @@ subroutine dump_params(base)
     write(u, '(A,99(1X,ES25.17E3))') 'reals ', t_top, t_bot, bot_bc, g, demf, ten_off, tau_heal, v_min, &
         v_max, frac, fracm, strain_inert, dt_scale, xReyn, vbc, rxbo, rzbo, pisos, drosub, &
         fmagma_max, weaken_ratio_plastic, weaken_ratio_viscous, rho_magma, extra_pres
+    write(u, '(A,99I12)') 'flags2 ', nelem_serp
     do k = 1, nphase
         write(u, '(A,I4,I4,99(1X,ES25.17E3))') 'phase ', k, irheol(k), den(k), alfa(k), beta(k), pln(k), &
             acoef(k), eactiv(k), vactiv(k), rl(k), rm(k), plstrain1(k), plstrain2(k), fric1(k), &

@@ -25,11 +25,22 @@ from geoflac_amd import state  # noqa: E402
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
 
 MUTABLE = ["cord", "vel", "temp", "temp0", "force", "amass", "rmass", "stress0", "area", "dvol",
-           "strain", "se2sr", "aps", "visn", "e2sr", "sshrheat", "shrheat"]
+           "strain", "se2sr", "aps", "visn", "e2sr", "sshrheat", "shrheat", "iphase", "phase_ratio",
+           "mark_phase"]
 
 CASES = {
     # 60x20 elements, 300 km x 100 km, crust/mantle, extension 0.5 cm/yr per side
     "box60x20": dict(inp="box60x20.inp", steps=[1, 10, 100, 1000, 10000]),
+    # 400x100 elements over 8000 km x 200 km, four layers; the 35 km interface lies
+    # inside an element row (fractional phase ratios)
+    "layers400x100": dict(inp="layers400x100.inp", steps=[1, 10, 100],
+                          fields=["cord", "vel", "temp", "stress0", "area", "dvol", "strain", "aps", "visn",
+                                  "e2sr", "iphase", "phase_ratio", "mark_phase"]),
+    # 80x40, four layers of distinct rheology (irheol 6, 12, 12, 3) plus an elastic
+    # block (irheol 1), every interface inside an element row
+    "hetero80x40": dict(inp="hetero80x40.inp", steps=[1, 10, 100, 1000]),
+    # 60x30 with marker phase changes (sediment, oceanic crust, serpentinite)
+    "phases60x30": dict(inp="phases60x30.inp", steps=[1, 11, 21, 100, 1000]),
 }
 
 
@@ -48,7 +59,7 @@ def run_case(name, spec):
         state.save_npz(s0, os.path.join(HERE, "%s_s0.npz" % name))
         for n in steps:
             sn = state.load_dump(os.path.join(out, "step_%07d" % n))
-            state.save_npz(sn, os.path.join(HERE, "%s_s%d.npz" % (name, n)), fields=MUTABLE)
+            state.save_npz(sn, os.path.join(HERE, "%s_s%d.npz" % (name, n)), fields=spec.get("fields", MUTABLE))
             print(name, n, "written")
 
 

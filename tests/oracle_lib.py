@@ -34,11 +34,29 @@ def golden_state(case, step):
     return state.load_npz(os.path.join(GOLDEN, "%s_s%d.npz" % (case, step)))
 
 
+def _bind_markers(L, st):
+    """change_phase in the oracle runs on the state's markers (none: it does nothing)."""
+    if st.has_markers:
+        m = st.marker_arrays()
+        L.gfo_bind_markers(C.byref(m))
+        return m
+    L.gfo_bind_markers(None)
+    return None
+
+
 def step(p, st, nsteps, cr=False):
     h = st.host_arrays()
-    return lib(cr).gfo_step(C.byref(p), C.byref(h), C.byref(st.clock), int(nsteps))
+    L = lib(cr)
+    keep = _bind_markers(L, st)  # noqa: F841 (kept alive over the call)
+    rc = L.gfo_step(C.byref(p), C.byref(h), C.byref(st.clock), int(nsteps))
+    L.gfo_bind_markers(None)
+    return rc
 
 
 def call(name, p, st, cr=False):
     h = st.host_arrays()
-    return getattr(lib(cr), name)(C.byref(p), C.byref(h), C.byref(st.clock))
+    L = lib(cr)
+    keep = _bind_markers(L, st)  # noqa: F841
+    rc = getattr(L, name)(C.byref(p), C.byref(h), C.byref(st.clock))
+    L.gfo_bind_markers(None)
+    return rc

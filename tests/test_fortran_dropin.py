@@ -50,22 +50,29 @@ def test_fortran_interface_mirrors_c_structs():
     assert c_fields("gf_params") == f_fields("gf_params_t")
     assert c_fields("gf_host_arrays") == f_fields("gf_host_arrays_t")
     assert c_fields("gf_clock") == f_fields("gf_clock_t")
+    assert c_fields("gf_marker_arrays") == f_fields("gf_marker_arrays_t")
 
 
 @pytest.mark.gpu
-def test_fortran_dropin_matches_reference(tmp_path):
+@pytest.mark.parametrize("case,dumps", [("box60x20", (10, 100)), ("phases60x30", (11, 100))])
+def test_fortran_dropin_matches_reference(tmp_path, case, dumps):
+    """box60x20, and phases60x30 whose markers change phase on the device (the marker
+    phases, phase ratios and iphase come back to the Fortran arrays)."""
     if not os.path.exists(EXE):
         pytest.skip("oracle/_ref/dropin_harness is built only where the reference sources are present")
-    inp = os.path.join(ROOT, "tests", "golden", "inputs", CASE + ".inp")
+    inp = os.path.join(ROOT, "tests", "golden", "inputs", case + ".inp")
     out = str(tmp_path / "out")
-    print("case: Fortran drop-in, 100 steps of %s (dumps at 10 and 100)" % CASE, flush=True)
-    r = subprocess.run([EXE, inp, out, "100", "10", "100"], cwd=str(tmp_path), capture_output=True,
-                       text=True, timeout=300)
+    print("case: Fortran drop-in, %d steps of %s (dumps at %s)" % (max(dumps), case, dumps), flush=True)
+    r = subprocess.run([EXE, inp, out, str(max(dumps))] + [str(d) for d in dumps], cwd=str(tmp_path),
+                       capture_output=True, text=True, timeout=300)
     sys.stdout.write(r.stdout[-2000:])
     assert r.returncode == 0, r.stderr[-2000:]
-    for step in (10, 100):
+    for step in dumps:
         st = state.load_dump(os.path.join(out, "step_%07d" % step))
-        ref = ol.golden_state(CASE, step)
+        ref = ol.golden_state(case, step)
+        assert np.array_equal(st.mark_phase_only, ref.mark_phase_only)
+        assert np.array_equal(st.iphase, ref.iphase)
+        assert np.array_equal(st.phase_ratio, ref.phase_ratio)
         worst = {}
         for f in ("cord", "vel", "temp", "stress0", "aps", "strain", "area", "visn", "e2sr", "se2sr",
                   "rmass", "amass", "temp0"):

@@ -35,14 +35,16 @@ def test_library_exports_every_declared_symbol():
     assert declared <= set(gpu.EXPORTS) | {"gf_params", "gf_host_arrays", "gf_clock"}
 
 
-def golden_full(step):
+def golden_full(step, case=CASE):
     """Step-N golden: constant fields from step 0, changing fields from step N."""
-    s0 = ol.golden_state(CASE, 0)
+    s0 = ol.golden_state(case, 0)
     if step == 0:
         return s0
-    sn = ol.golden_state(CASE, step)
+    sn = ol.golden_state(case, step)
     for f in sn.present:
-        if not f.startswith("_"):
+        if f == "mark_phase":
+            s0.mark_phase[...] = sn.mark_phase_only
+        elif not f.startswith("_"):
             getattr(s0, f)[...] = getattr(sn, f)
     s0.clock.nloop, s0.clock.nsrate = sn.clock.nloop, sn.clock.nsrate
     s0.clock.time, s0.clock.dt, s0.clock.dtavg = sn.clock.time, sn.clock.dt, sn.clock.dtavg
@@ -51,10 +53,16 @@ def golden_full(step):
 
 
 def compare(st, ref, fields=None, tol=TOL):
-    fields = fields or [f for f in state.State.FIELDS]
+    fields = fields or [f for f in state.State.FIELDS] + (["mark_phase"] if st.has_markers else [])
     worst, nexact = {}, 0
     for f in fields:
         a, b = getattr(st, f), getattr(ref, f)
+        if f in ("iphase", "mark_phase", "ncod"):  # integers: equal or not
+            if np.array_equal(a, b):
+                nexact += 1
+            else:
+                worst[f] = float("inf")
+            continue
         if np.array_equal(a, b):
             nexact += 1
             continue
@@ -63,16 +71,23 @@ def compare(st, ref, fields=None, tol=TOL):
     return bad, worst, nexact
 
 
+SUB_STATES = [("box60x20", 10), ("box60x20", 1000), ("hetero80x40", 100), ("layers400x100", 10),
+              ("phases60x30", 11)]
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("sub", ["fl_therm", "fl_rheol", "fl_node", "fl_move", "rmasses", "dt_mass"])
-@pytest.mark.parametrize("step", [10, 1000])
-def test_subroutine_bitwise_equal_to_oracle(sub, step):
+@pytest.mark.parametrize("sub", ["fl_therm", "change_phase", "fl_rheol", "fl_node", "fl_move", "rmasses",
+                                 "dt_mass"])
+@pytest.mark.parametrize("case,step", SUB_STATES)
+def test_subroutine_bitwise_equal_to_oracle(sub, case, step):
     """One reference subroutine on the device equals the oracle (with the device's
-    correctly rounded libm) bit for bit, from a reference state."""
+    correctly rounded libm) bit for bit, from a reference state: pure-phase
+    (box60x20), mixed-phase with four rheologies (hetero80x40), four layers at 400x100,
+    and marker phase changes (phases60x30)."""
     from geoflac_amd import gpu
-    _say("case: single %s from the %d-step golden state of %s" % (sub, step, CASE))
-    p = ol.golden_params(CASE)
-    st = golden_full(step)
+    _say("case: single %s from the %d-step golden state of %s" % (sub, step, case))
+    p = ol.golden_params(case)
+    st = golden_full(step, case)
     st.clock.nloop = 10          # a viscosity-update and mass-update step
     ref = st.copy()
     with gpu.Model(p, st) as m:
@@ -221,20 +236,31 @@ def test_interleaved_calls_bitwise_equal_to_oracle():
     assert st.clock.dt == ref.clock.dt
 
 
+REF_RUNS = [("box60x20", n) for n in (1, 10, 100, 1000)] + \
+    [("layers400x100", n) for n in (1, 10, 100)] + \
+    [("hetero80x40", n) for n in (1, 10, 100, 1000)] + \
+    [("phases60x30", n) for n in (1, 11, 21, 100, 1000)]
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("nsteps", [1, 10, 100, 1000])
-def test_steps_match_reference(nsteps):
+@pytest.mark.parametrize("case,nsteps", REF_RUNS)
+def test_steps_match_reference(case, nsteps):
+    """N device steps from the reference's step-0 state against the reference's own
+    state after N steps: every floating-point field within 1e-10 rel-L2, the integer
+    fields (iphase, marker phases) equal."""
     from geoflac_amd import gpu
-    _say("case: %d steps of %s from the reference's step-0 state" % (nsteps, CASE))
-    p = ol.golden_params(CASE)
-    st = golden_full(0)
+    _say("case: %d steps of %s from the reference's step-0 state" % (nsteps, case))
+    p = ol.golden_params(case)
+    st = golden_full(0, case)
     with gpu.Model(p, st) as m:
         m.step(nsteps)
         m.download()
         unconv, extra, mism, mulp = m.therm_stats()
-    ref = golden_full(nsteps)
-    fields = ["cord", "vel", "temp", "stress0", "aps", "strain", "area", "visn", "e2sr",
-              "se2sr", "rmass", "amass", "force", "temp0", "sshrheat", "shrheat"]
+    ref = golden_full(nsteps, case)
+    fields = ["cord", "vel", "temp", "stress0", "aps", "strain", "area", "visn", "e2sr", "iphase", "phase_ratio",
+              "mark_phase"]
+    if case != "layers400x100":  # (its fixture keeps the fields above)
+        fields += ["se2sr", "rmass", "amass", "force", "temp0", "sshrheat", "shrheat"]
     bad, worst, nexact = compare(st, ref, fields)
     # dvol = det*area_old - 1 is a cancellation residual of size ~1e-13: its relative
     # L2 amplifies round-off of O(1) quantities, so it is held to an absolute bound in
@@ -252,6 +278,27 @@ def test_steps_match_reference(nsteps):
     assert st.clock.nloop == ref.clock.nloop
     assert abs(st.clock.time - ref.clock.time) <= 1e-12 * ref.clock.time
     assert abs(st.clock.dt - ref.clock.dt) <= 1e-12 * ref.clock.dt
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case,nsteps", [("hetero80x40", 1000), ("phases60x30", 1000), ("layers400x100", 100)])
+def test_steps_bitwise_equal_to_oracle(case, nsteps):
+    """Mixed phases, four rheologies and marker phase changes: N device steps equal the
+    correctly rounded oracle bit for bit (markers included)."""
+    from geoflac_amd import gpu
+    _say("case: %d steps of %s vs the correctly rounded oracle" % (nsteps, case))
+    p = ol.golden_params(case)
+    st = golden_full(0, case)
+    ref = st.copy()
+    with gpu.Model(p, st) as m:
+        m.step(nsteps)
+        m.download()
+        unconv = m.therm_unconverged()
+    assert ol.step(p, ref, nsteps, cr=True) == 0
+    bad, worst, nexact = compare(st, ref, tol=0.0)
+    _say("  exact fields %d, differing %s, thermal unproven %d" % (nexact, worst, unconv))
+    assert unconv == 0
+    assert not bad, bad
 
 
 @pytest.mark.gpu

@@ -25,27 +25,62 @@ def test_inp_reader_matches_reference_read_params():
 
 
 def test_params_layout_matches_header():
-    # 546 int32 (even count, no hidden padding) followed by 23 + 19*20 + 1 doubles
-    assert abi.SIZEOF_PARAMS == 546 * 4 + (23 + 19 * abi.MAXPH + 1) * 8
+    # 548 int32 (even count, no hidden padding) followed by 23 + 19*20 + 1 doubles
+    assert abi.SIZEOF_PARAMS == 548 * 4 + (23 + 19 * abi.MAXPH + 1) * 8
     assert abi.SIZEOF_CLOCK == 8 + 5 * 8
+    # the C compiler's layout of include/gf_abi.h (through the oracle, built against it)
+    out = (C.c_long * 4)()
+    ol.lib().gfo_abi_sizes(out)
+    assert list(out) == [abi.SIZEOF_PARAMS, abi.SIZEOF_HOST, abi.SIZEOF_CLOCK, C.sizeof(abi.GfMarkerArrays)]
 
 
-@pytest.mark.parametrize("nsteps", [1, 10, 100, 1000, 10000])
-def test_oracle_bitwise_equal_to_reference(nsteps):
-    p = ol.golden_params(CASE)
-    st = ol.golden_state(CASE, 0)
+GOLDEN_RUNS = [("box60x20", n) for n in (1, 10, 100, 1000, 10000)] + \
+    [("layers400x100", n) for n in (1, 10, 100)] + \
+    [("hetero80x40", n) for n in (1, 10, 100, 1000)] + \
+    [("phases60x30", n) for n in (1, 11, 21, 100, 1000)]
+
+
+@pytest.mark.parametrize("case,nsteps", GOLDEN_RUNS)
+def test_oracle_bitwise_equal_to_reference(case, nsteps):
+    """From the reference's step-0 state (markers included), N oracle steps reproduce
+    every array the reference dumped after N steps, bit for bit: the mixed-phase
+    elements of layers400x100 and hetero80x40 (harmonic viscosity and softening means,
+    phase-weighted density), the rheology branches irheol 1/3/6/12 of hetero80x40, and
+    the marker phase changes of phases60x30 (mark_phase, phase_ratio, iphase, aps)."""
+    p = ol.golden_params(case)
+    st = ol.golden_state(case, 0)
+    assert st.has_markers
     assert ol.step(p, st, nsteps) == 0
-    ref = ol.golden_state(CASE, nsteps)
+    ref = ol.golden_state(case, nsteps)
     for f in ref.present:
         if f.startswith("_"):
             continue
-        a, b = getattr(st, f), getattr(ref, f)
+        a = st.mark_phase if f == "mark_phase" else getattr(st, f)
+        b = ref.mark_phase_only if f == "mark_phase" else getattr(ref, f)
         assert np.array_equal(a, b), (f, state.rel_l2(a, b))
     assert st.clock.nloop == ref.clock.nloop == nsteps
     assert st.clock.time == ref.clock.time
     assert st.clock.dt == ref.clock.dt
     assert st.clock.dtavg == ref.clock.dtavg
     assert st.clock.dt_elastic == ref.clock.dt_elastic
+
+
+def test_golden_cases_cover_mixed_phases_and_phase_changes():
+    """The fixtures exercise what a pure-phase box cannot: fractional phase ratios in
+    layers400x100 and hetero80x40, several rheology types, and phase changes."""
+    for case in ("layers400x100", "hetero80x40"):
+        s0 = ol.golden_state(case, 0)
+        pr = s0.phase_ratio
+        assert np.count_nonzero((pr > 0) & (pr < 1)) > 100, case
+    p = ol.golden_params("hetero80x40")
+    s0 = ol.golden_state("hetero80x40", 0)
+    assert sorted({p.irheol[k - 1] for k in np.unique(s0.iphase)}) == [1, 3, 6, 12]
+    a = ol.golden_state("phases60x30", 0).mark_phase
+    b = ol.golden_state("phases60x30", 100).mark_phase_only
+    changed = set(zip(a[a != b].tolist(), b[a != b].tolist()))
+    # sediment -> sedimentary rock -> schist, oceanic crust -> eclogite, mantle <-> serpentinite
+    assert {(1, 11), (3, 13), (4, 9)} <= changed, changed
+    assert 5 in set(b.tolist())
 
 
 # ---------------------------------------------------------------- closed forms
