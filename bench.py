@@ -101,13 +101,107 @@ def traffic_from_profile(kernel):
     return {"traffic": k["bytes_per_launch"], "traffic_source": "profiles/traffic.json (%s)" % t.get("command", "")}
 
 
-def build_model(cfg, precision):
+INTERFACE_NOISE_KM = 2.0   # seeded per-column offsets of the layer interfaces
+COH_NOISE = 0.01           # seeded per-element cohesion factor 1 +- 1%
+
+
+def build_model(cfg, precision, seed):
     from geoflac_amd import inp, model, gpu
     p = inp.read_inp(os.path.join(ROOT, "geoflac_amd", "configs", cfg + ".inp"))
-    gp, st = model.build_box(p)
+    gp, st = model.build_box(p, interface_noise=INTERFACE_NOISE_KM, coh_noise=COH_NOISE, seed=seed)
     m = gpu.Model(gp, st, precision=precision)
     model.device_setup(m)
     return p, gp, st, m
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _ref_deck(cfg_text, ncols, nsteps):
+    """The benchmark deck cut to its first ncols element columns (the same cells,
+    layers and boundary conditions) running exactly nsteps steps (ntest_rem = nsteps,
+    time_max = 0; output intervals beyond the run)."""
+    lines = cfg_text.splitlines()
+    nelx, nelz = (int(v) for v in lines[7].split(","))
+    x0, z0 = (float(v) for v in lines[8].split(","))
+    rx, rz = (float(v) for v in lines[9].split(","))
+    lines[7] = "%d,%d" % (ncols, nelz)
+    lines[9] = "%r,%r" % (rx * ncols / nelx, rz)
+    out = []
+    for ln in lines:
+        t = ln.split()
+        if len(t) >= 4 and t[0] == "2" and t[2] == str(nelx + 1):          # bottom boundary nodes
+            ln = ln.replace(" %d " % (nelx + 1), " %d " % (ncols + 1), 1)
+        elif ln.strip().startswith("21,") and ln.rstrip().endswith(str(nelx + 1)):  # zone columns
+            ln = ln[:ln.rstrip().rfind(str(nelx + 1))] + str(ncols + 1)
+        elif ln.strip() == "0  3  10  11.0":                                # ny_rem, mode_rem, ntest_rem
+            ln = "0  3  %d  11.0" % nsteps
+        out.append(ln)
+    return "\n".join(out) + "\n"
+
+
+def ref_cpu_baseline(cfg, ncols=500, n1=10, n2=60, repeats=3):
+    """The reference's own OpenMP program (oracle/_ref/ref_flac_omp, built from the
+    reference sources with flang -O2 -fopenmp) on the host cores: loop-only time per
+    step t = [t(n2) - t(n1)] / (n2 - n1) from runs of the benchmark deck cut to its
+    first ncols element columns (same cells and physics; init, output and the final
+    restart dump cancel in the difference), each run length timed `repeats` times and
+    the shortest kept.  Runs as child processes before this process touches the GPU."""
+    import resource
+    import shutil
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_flac_omp")
+    if not os.path.exists(exe):
+        return None
+    with open(os.path.join(ROOT, "geoflac_amd", "configs", cfg + ".inp")) as f:
+        text = f.read()
+    nelz = int(text.splitlines()[7].split(",")[1])
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PLACES="cores", OMP_PROC_BIND="close",
+               OMP_STACKSIZE="1G")
+
+    def unlimited_stack():
+        resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+    base = "/dev/shm" if os.path.isdir("/dev/shm") else None
+    times = {n1: [], n2: []}
+    for _ in range(repeats):
+        for n in (n1, n2):
+            d = tempfile.mkdtemp(prefix="gf_refomp_", dir=base)
+            try:
+                with open(os.path.join(d, "deck.inp"), "w") as f:
+                    f.write(_ref_deck(text, ncols, n))
+                t0 = time.perf_counter()
+                r = subprocess.run([exe, "deck.inp"], cwd=d, env=env, preexec_fn=unlimited_stack,
+                                   stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=600)
+                times[n].append(time.perf_counter() - t0)
+                if r.returncode != 0:
+                    return None
+            finally:
+                shutil.rmtree(d, ignore_errors=True)
+            print("bench: reference OpenMP run of %d steps: %.2f s" % (n, times[n][-1]), file=sys.stderr,
+                  flush=True)
+    times = {n: min(v) for n, v in times.items()}
+    t_step = (times[n2] - times[n1]) / (n2 - n1)
+    cpu = _cpu_model()
+    E = ncols * nelz
+    return {"value": E / t_step, "unit": "element-steps/s", "cores": threads, "kind": "reference",
+            "cpu_model": cpu,
+            "sample": "reference OpenMP program (flang -O2 -fopenmp), %d threads, OMP_PLACES=cores "
+                      "OMP_PROC_BIND=close: the %s deck cut to its first %d of %d element columns "
+                      "(%d elements, same cells and layers), loop time [t(%d)-t(%d)]/%d = %.3f s/step "
+                      "(shortest of %d runs each: %.1f s and %.1f s)" % (threads, cfg, ncols, int(text.splitlines()[7].split(",")[0]),
+                                                        E, n2, n1, n2 - n1, t_step, repeats, times[n1],
+                                                        times[n2])}
 
 
 def cpu_baseline(gp, st, seconds_cap=20.0):
@@ -127,6 +221,7 @@ def cpu_baseline(gp, st, seconds_cap=20.0):
             break
     el = time.perf_counter() - t0
     return {"value": E * n / el, "unit": "element-steps/s", "cores": 1, "kind": "port",
+            "cpu_model": _cpu_model(),
             "sample": "%d step(s) of the same %dx%d model by the oracle (serial C restatement "
                       "of the reference step), %.1f s" % (n, gp.nx - 1, gp.nz - 1, el)}
 
@@ -162,7 +257,10 @@ def mixed_run(gp, host_state, fp64_state, args, E, precision, learned=None):
     err = {}
     for f in ("vel", "stress0", "temp", "cord", "aps"):
         a, b = getattr(st, f), getattr(fp64_state, f)
-        err[f] = {"rel_l2": float("%.3e" % state.rel_l2(a, b)), "max_abs": float("%.3e" % np.max(np.abs(a - b)))}
+        r = state.rel_l2(a, b)
+        # rel-L2 is undefined where the FP64 field is zero (no plastic strain yet): null
+        err[f] = {"rel_l2": float("%.3e" % r) if np.isfinite(r) else None,
+                  "max_abs": float("%.3e" % np.max(np.abs(a - b)))}
     dtype = "f32 arithmetic, f32 planes" if precision == abi.PREC_MIXED else "f64 arithmetic, f32 planes"
     out = {"value": E * args.steps / el, "ms_per_step": el / args.steps * 1e3, "dtype": dtype,
            "steps_compared": args.warmup + args.steps, "error_vs_fp64": err,
@@ -238,6 +336,8 @@ def main():
     ap.add_argument("--sweep-steps", type=int, default=2000)
     ap.add_argument("--sweep-check", type=int, default=4,
                     help="models of the sweep also run alone in FP64 for the error report")
+    ap.add_argument("--seed", type=int, default=0,
+                    help="seed of the layer-interface offsets and the per-element cohesion factors")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="short copy probe only (for profiler counter runs, where k_copy over 1 GiB is the "
                          "kernel of known traffic that scales FETCH_SIZE/WRITE_SIZE)")
@@ -247,8 +347,11 @@ def main():
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    # the reference's OpenMP program on the host cores, before this process touches the GPU
+    ref_cpu = None if args.no_cpu_baseline else ref_cpu_baseline(args.config)
+
     from geoflac_amd import abi, gpu
-    p, gp, st, m = build_model(args.config, abi.PREC_FP64)
+    p, gp, st, m = build_model(args.config, abi.PREC_FP64, args.seed)
     E = (gp.nx - 1) * (gp.nz - 1)
     N = gp.nx * gp.nz
     m.download()
@@ -341,7 +444,9 @@ def main():
            "data": "synthetic",
            "config": {"workload": "%s: %dx%d-element four-layer extension box, plain FP64 drop-in path"
                                   % (args.config, gp.nx - 1, gp.nz - 1),
-                      "elements": E, "nodes": N, "precision": "fp64", "parallelism": "single GPU"},
+                      "elements": E, "nodes": N, "precision": "fp64", "parallelism": "single GPU",
+                      "markers": int(st.nmarkers), "change_phase": "every %d steps" % gp.ifreq_rmasses,
+                      "interface_noise_km": INTERFACE_NOISE_KM, "cohesion_noise": COH_NOISE, "seed": args.seed},
            "roofline": roofline}
     if mixed:
         for v in mixed.values():
@@ -350,7 +455,12 @@ def main():
     if not args.no_sweep:
         out["sweep"] = sweep_run(args)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(gp, host_state)
+        port = cpu_baseline(gp, host_state)
+        if ref_cpu is not None:
+            ref_cpu["port_1core"] = port
+            out["cpu_baseline"] = ref_cpu
+        else:
+            out["cpu_baseline"] = port
     print(json.dumps(out), flush=True)
 
 

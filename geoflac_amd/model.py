@@ -192,11 +192,12 @@ def _check_inside(cord, j, i, x, y):
     return out_k, out1, out2
 
 
-def _layer_phase(p, i1, yy, default):
+def _layer_phase(p, i1, yy, default, hshift=None):
     """Phase of a marker at depth -yy in column i1 (1-based) from the zones' layers
-    (src/init_marker.f90:86-117)."""
+    (src/init_marker.f90:86-117); hshift[zone][i1 - 1, m]: km added to interface m
+    in that column (seeded heterogeneity), None for the deck's depths."""
     kph = np.array(default, copy=True)
-    for zone in p["zones"]:
+    for nzn, zone in enumerate(p["zones"]):
         if zone["iph_col_trans"] == 1:
             raise NotImplementedError("interpolated layer depths (iph_col_trans)")
         sel = (i1 >= zone["ixtb1"]) & (i1 <= zone["ixtb2"])
@@ -204,7 +205,8 @@ def _layer_phase(p, i1, yy, default):
         nl = zone["nph_layer"]
         kz = np.full(np.shape(yy), zone["iph_col"][nl - 1])
         for m in range(nl - 2, -1, -1):   # the first layer whose base lies below the marker
-            kz = np.where(yyy < zone["hc"][m], zone["iph_col"][m], kz)
+            h = zone["hc"][m] if hshift is None else zone["hc"][m] + hshift[nzn][i1 - 1, m]
+            kz = np.where(yyy < h, zone["iph_col"][m], kz)
         kph = np.where(sel, kz, kph)
     return kph
 
@@ -231,7 +233,7 @@ def _seed_chain(seed, inc, a18, b18):
     return starts, int(ends[-1])
 
 
-def init_markers(p, st):
+def init_markers(p, st, interface_noise=0.0, seed=0):
     """Markers of a new start, as src/init_marker.f90:1-169 places them in the serial
     build: 9 per element at the regular points, each moved by a random offset of up
     to a sixth of the element from the serial myrandom sequence (seed 0, advanced by
@@ -241,7 +243,9 @@ def init_markers(p, st):
     src/marker_data.f90:140-174).  A point that falls outside its element (possible
     only on its boundary) is drawn again, which shifts the rest of the sequence: the
     elements are evaluated in chunks, and a chunk is evaluated again from the element
-    after such a redraw."""
+    after such a redraw.  interface_noise > 0 moves each layer interface by a seeded
+    uniform offset of +-interface_noise km per element column (a build-side
+    extension; 0 places the layers as the reference does)."""
     nx, nz = p["nx"], p["nz"]
     ez, ex = nz - 1, nx - 1
     ne = ez * ex
@@ -257,6 +261,11 @@ def init_markers(p, st):
     for _ in range(18):
         a18, b18 = (_LCG_A * a18) % _LCG_M, (_LCG_A * b18 + _LCG_C) % _LCG_M
     inc_ij = (I + 1 + J + 1).astype(np.int64)
+    hshift = None
+    if interface_noise:
+        rng = np.random.default_rng(seed)
+        hshift = [rng.uniform(-interface_noise, interface_noise, size=(ex, max(1, z["nph_layer"] - 1)))
+                  for z in p["zones"]]
     CH = 1 << 16
     seed, e0 = 0, 0
     while e0 < ne:
@@ -278,7 +287,7 @@ def init_markers(p, st):
             rej |= k == 0
             mark_a1[idx, l], mark_a2[idx, l] = b1, b2
             ntri[idx, l] = 2 * (ez * ii + jj) + k
-            phase[idx, l] = _layer_phase(p, ii + 1, yy, st.iphase[jj, ii])
+            phase[idx, l] = _layer_phase(p, ii + 1, yy, st.iphase[jj, ii], hshift)
         if not rej.any():
             seed, e0 = end_seed, e1
             continue
@@ -299,7 +308,7 @@ def init_markers(p, st):
                 continue
             mark_a1[e, l], mark_a2[e, l] = b1[0], b2[0]
             ntri[e, l] = 2 * (ez * i0 + j0) + k[0]
-            phase[e, l] = _layer_phase(p, np.array([i0 + 1]), yy, np.array([st.iphase[j0, i0]]))[0]
+            phase[e, l] = _layer_phase(p, np.array([i0 + 1]), yy, np.array([st.iphase[j0, i0]]), hshift)[0]
             l += 1
         seed, e0 = s, e + 1
     mark_id = np.zeros((32, ez, ex), dtype=np.int32, order="F")
@@ -406,8 +415,8 @@ def build_box(p, interface_noise=0.0, seed=0, markers=True, coh_noise=0.0):
 
     markers: place the reference's markers (init_markers) and take the phases from
     them, as setflac does; otherwise phases come from the 9 regular points of each
-    element with seeded per-column interface offsets of +-interface_noise km
-    (layer_phases).  coh_noise: a seeded per-element cohesion factor 1 + coh_noise*U,
+    element (layer_phases).  interface_noise: seeded per-column offsets of the layer
+    interfaces, +-interface_noise km.  coh_noise: a seeded per-element cohesion factor 1 + coh_noise*U,
     U uniform in [-1, 1] (a build-side extension; 0: none)."""
     nx, nz = p["nx"], p["nz"]
     st = State(nx, nz, p["nphase"])
@@ -416,7 +425,7 @@ def build_box(p, interface_noise=0.0, seed=0, markers=True, coh_noise=0.0):
     zm = mesh1(p["z0"], p["z0"] + p["rzbo"], nz, p["nelz_y"], p["sizez_y"])
     st.cord[:, :, 1] = zm[:, None]
     if markers:
-        init_markers(p, st)
+        init_markers(p, st, interface_noise, seed)
     else:
         layer_phases(p, st, interface_noise, seed)
     if coh_noise:

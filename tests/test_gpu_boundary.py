@@ -39,6 +39,7 @@ def test_unproven_thermal_step_returns_its_status():
                 code = e.code
                 break
         assert code == abi.GF_ERR_THERM_SWEEPS, code
+        m.therm_stats()   # (clears the counters, which include the unproven step)
         m.set_therm_passes(4)
         m.step(10)
         unconv, extra, _, _ = m.therm_stats()

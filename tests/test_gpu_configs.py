@@ -1,0 +1,176 @@
+"""The benchmark configurations at their full size, on the device.
+
+  cfg2  the 4000x1000-element four-layer box with seeded +-2 km layer interfaces and a
+        seeded +-1% cohesion factor per element (markers, change_phase every 10
+        steps): bitwise equal to the correctly rounded oracle over 3 steps.
+  cfg3  the same box in both FP32 modes: per-field error against the FP64 run after
+        100 steps, within stated bounds.
+  cfg4  a seeded sweep of 8 models with meshes of 100-600 x 40-150 elements: the
+        batched launches (with graph replay) end bit for bit where each model ends
+        alone, in FP64 and in FP32 arithmetic.
+  cfg5  the learned viscosity on the 4000x1000 state: trained on the device from the
+        solver's own samples, gated and audited inside the step.
+
+The host set-up of the 4000x1000 box (36 M markers) takes about a minute and is
+shared by the tests of this module.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from geoflac_amd import abi, inp, model, state, sweep
+from tests import oracle_lib as ol
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CFG = os.path.join(ROOT, "geoflac_amd", "configs", "layers4000x1000.inp")
+SEED = 0
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def big():
+    """The cfg2 new-start state after the device part of setflac (areas, masses,
+    viscosity, dt)."""
+    from geoflac_amd import gpu
+    _say("case: host set-up of layers4000x1000 with seeded interfaces (2 km) and cohesion (1%%), seed %d" % SEED)
+    p = inp.read_inp(CFG)
+    gp, st = model.build_box(p, interface_noise=2.0, coh_noise=0.01, seed=SEED)
+    with gpu.Model(gp, st) as m:
+        model.device_setup(m)
+        m.download()
+    _say("  %d elements, %d markers, phases %s" % (st.nelem, st.nmarkers, sorted(set(np.unique(st.iphase)))))
+    return gp, st
+
+
+def _run(gp, st0, nsteps, precision=abi.PREC_FP64):
+    from geoflac_amd import gpu
+    st = st0.copy()
+    with gpu.Model(gp, st, precision=precision) as m:
+        m.step(nsteps)
+        m.download()
+        stats = m.therm_stats()
+    return st, stats
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(600)
+def test_cfg2_seeded_box_bitwise_equal_to_oracle(big):
+    gp, st0 = big
+    _say("case: cfg2, 3 FP64 steps of the seeded 4000x1000 box vs the correctly rounded oracle")
+    st, (unconv, extra, mism, mulp) = _run(gp, st0, 3)
+    ref = st0.copy()
+    assert ol.step(gp, ref, 3, cr=True) == 0
+    diff = {}
+    for f in state.State.FIELDS + ["mark_phase"]:
+        if not np.array_equal(getattr(st, f), getattr(ref, f)):
+            diff[f] = state.rel_l2(getattr(st, f), getattr(ref, f))
+    _say("  differing fields %s; thermal: unproven %d, extra passes %d, border mismatches %d"
+         % (diff, unconv, extra, mism))
+    assert unconv == 0
+    assert not diff, diff
+    assert st.clock.dt == ref.clock.dt and st.clock.time == ref.clock.time
+
+
+CFG3_BOUNDS = {
+    abi.PREC_MIXED_STORAGE: {"vel": 1e-3, "stress0": 1e-6, "temp": 1e-10, "cord": 1e-10},
+    abi.PREC_MIXED: {"vel": 3e-2, "stress0": 3e-5, "temp": 1e-9, "cord": 1e-9},
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("precision", [abi.PREC_MIXED_STORAGE, abi.PREC_MIXED])
+def test_cfg3_fp32_modes_within_bounds_after_100_steps(big, precision):
+    gp, st0 = big
+    name = {abi.PREC_MIXED: "FP32 arithmetic", abi.PREC_MIXED_STORAGE: "FP32 storage"}[precision]
+    _say("case: cfg3, 100 steps of the seeded 4000x1000 box, %s vs FP64" % name)
+    ref, _ = _run(gp, st0, 100)
+    mix, _ = _run(gp, st0, 100, precision)
+    err = {f: state.rel_l2(getattr(mix, f), getattr(ref, f)) for f in ("vel", "stress0", "temp", "cord", "aps")}
+    mx = {f: float(np.max(np.abs(getattr(mix, f) - getattr(ref, f)))) for f in err}
+    _say("  rel-L2 %s" % {k: "%.2e" % v for k, v in err.items()})
+    _say("  max abs %s" % {k: "%.2e" % v for k, v in mx.items()})
+    for f, bound in CFG3_BOUNDS[precision].items():
+        assert err[f] <= bound, (f, err[f], bound)
+    assert np.array_equal(mix.mark_phase, ref.mark_phase)
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("precision", [abi.PREC_FP64, abi.PREC_MIXED])
+def test_cfg4_sweep_batch_equals_single_runs(precision):
+    from geoflac_amd import gpu  # noqa: F401
+    base = inp.read_inp(os.path.join(ROOT, "geoflac_amd", "configs", "box60x20.inp"))
+    ps = sweep.sweep_params(base, n=8, seed=0)
+    _say("case: cfg4, 8 models of sweep_params(seed=0) %s, precision %d, 20 steps batched (graph) vs alone"
+         % ([(q["nelx"], q["nelz"]) for q in ps], precision))
+    models = [sweep.make_model(q, precision)[2] for q in ps]
+    singles = []
+    for q in ps:
+        gp, st, m = sweep.make_model(q, precision)
+        m.step(20)
+        m.download()
+        singles.append(st)
+        m.close()
+    with sweep.Batch(models) as b:
+        b.set_graph(True)
+        b.step(20)
+    bad = []
+    for k, (m, s) in enumerate(zip(models, singles)):
+        m.download()
+        for f in state.State.FIELDS + ["mark_phase"]:
+            if not np.array_equal(getattr(m.state, f), getattr(s, f)):
+                bad.append((k, f, state.rel_l2(getattr(m.state, f), getattr(s, f))))
+        assert m.state.clock.nloop == s.clock.nloop == 20
+        assert m.state.clock.time == s.clock.time
+        m.close()
+    _say("  differing %s" % bad)
+    assert not bad, bad
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(900)
+def test_cfg5_learned_viscosity_on_the_large_box(big):
+    """Samples of the solver's own viscosities at steps 0 and 10 of the FP32-arithmetic
+    run, 1500 Adam steps on the device, then 30 steps with the surrogate in every
+    viscosity update.  The gate sends the elements outside the training data to the
+    exact formula, the audits compare whole waves with it; the run must keep the
+    surrogate on, audit a share of the elements, and stay within a bounded distance
+    of the same run with the exact formula."""
+    from geoflac_amd import gpu
+    gp, st0 = big
+    _say("case: cfg5, surrogate trained on the 4000x1000 box, 30 learned steps vs exact")
+    st = st0.copy()
+    E = st.nelem
+    with gpu.Model(gp, st, precision=abi.PREC_MIXED) as m:
+        m.surrogate_sample(2 * E)
+        m.step(10)
+        n = m.surrogate_sample(2 * E)
+        l0, l1 = m.surrogate_train(1500, batch=8192, lr=2e-3, seed=1)
+        _say("  %d samples, loss %.3e -> %.3e" % (n, l0, l1))
+        w, cfg = m.surrogate_weights()
+    assert l1 < 0.05 * l0
+    exact, _ = _run(gp, st0, 30, abi.PREC_MIXED)
+    learned = st0.copy()
+    with gpu.Model(gp, learned, precision=abi.PREC_MIXED) as m:
+        m.surrogate_weights(w, cfg)
+        m.surrogate_enable(True, tol=0.5, audit_stride=64)
+        m.step(30)
+        m.download()
+        s = m.surrogate_stats()
+    tot = s["evals"] + s["fallbacks"] + s["audits"]
+    err = {f: state.rel_l2(getattr(learned, f), getattr(exact, f)) for f in ("vel", "stress0", "visn", "temp")}
+    _say("  stats %s, fallback rate %.3f; rel-L2 vs exact %s" % (s, s["fallbacks"] / max(tot, 1),
+                                                                  {k: "%.2e" % v for k, v in err.items()}))
+    assert s["off"] == 0
+    assert s["evals"] > 0.3 * tot
+    assert s["audits"] > 0
+    assert s["max_audit_err"] < 2.0
+    assert err["temp"] < 1e-8
+    assert err["vel"] < 0.2 and err["stress0"] < 1e-3

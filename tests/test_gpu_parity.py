@@ -242,16 +242,30 @@ REF_RUNS = [("box60x20", n) for n in (1, 10, 100, 1000)] + \
     [("phases60x30", n) for n in (1, 11, 21, 100, 1000)]
 
 
+# Runs long enough for the reference's own libm to matter: glibc's sin/exp/pow are not
+# correctly rounded for about 0.1% of arguments, and once plastic yielding starts the
+# step amplifies a last-bit difference (the plastic/viscous branch choice is a
+# discontinuous function of the state).  Restated on the CPU with glibc and with
+# correctly rounded functions, 1000 steps differ by 2.7e-10 (hetero80x40) and 2.9e-8
+# (phases60x30) in velocity rel-L2 (tests/test_oracle.py::test_libm_floor_of_long_runs).
+# There the device (correctly rounded, like the oracle it is checked against) must
+# equal the correctly rounded oracle bit for bit and lie exactly as far from the
+# reference as that oracle does.
+LIBM_BOUND_RUNS = {("hetero80x40", 1000), ("phases60x30", 1000)}
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("case,nsteps", REF_RUNS)
 def test_steps_match_reference(case, nsteps):
     """N device steps from the reference's step-0 state against the reference's own
     state after N steps: every floating-point field within 1e-10 rel-L2, the integer
-    fields (iphase, marker phases) equal."""
+    fields (iphase, marker phases) equal; for the long runs of LIBM_BOUND_RUNS, bitwise
+    the correctly rounded oracle instead (see above)."""
     from geoflac_amd import gpu
     _say("case: %d steps of %s from the reference's step-0 state" % (nsteps, case))
     p = ol.golden_params(case)
     st = golden_full(0, case)
+    cr = st.copy() if (case, nsteps) in LIBM_BOUND_RUNS else None
     with gpu.Model(p, st) as m:
         m.step(nsteps)
         m.download()
@@ -261,7 +275,17 @@ def test_steps_match_reference(case, nsteps):
               "mark_phase"]
     if case != "layers400x100":  # (its fixture keeps the fields above)
         fields += ["se2sr", "rmass", "amass", "force", "temp0", "sshrheat", "shrheat"]
+    if cr is not None:
+        assert ol.step(p, cr, nsteps, cr=True) == 0
+        bad, worst, nexact = compare(st, cr, fields + ["dvol"], tol=0.0)
+        _say("  vs the correctly rounded oracle: exact %d, differing %s; vs the reference: %s"
+             % (nexact, worst, {f: "%.1e" % state.rel_l2(getattr(st, f), getattr(ref, f))
+                                for f in ("vel", "stress0", "temp", "aps")}))
+        assert unconv == 0
+        assert not bad, bad
+        return
     bad, worst, nexact = compare(st, ref, fields)
+    _say("  exact fields %d/%d, differing %s" % (nexact, len(fields), {k: "%.2e" % v for k, v in worst.items()}))
     # dvol = det*area_old - 1 is a cancellation residual of size ~1e-13: its relative
     # L2 amplifies round-off of O(1) quantities, so it is held to an absolute bound in
     # units of those O(1) factors (1e-13 ~ 450 ulp of 1.0).  It differs from the
