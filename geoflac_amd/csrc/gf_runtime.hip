@@ -849,8 +849,12 @@ static int enq_therm(gf_ctx *ctx, bool clock_done = false)
         }
         const bool dis = ctx->p.ishearh == 1 && ctx->p.itherm != 2;
         // pass 0 starts from the extrapolation 2 temp - temp0 once temp0 holds the
-        // previous step's starting temperature
-        const bool ex = ctx->temp0_prev && !ctx->therm_noguess;
+        // previous step's starting temperature.  A tile reads temp0 of its halo, which
+        // the neighbouring tiles overwrite with this step's value as they go: the
+        // start then depends on the order the tiles run in, which only the proven
+        // passes are free of (their fixed point is exact), so the unproven single pass
+        // of FP32-arithmetic mode starts from the old temperature.
+        const bool ex = ctx->temp0_prev && !ctx->therm_noguess && ctx->therm_proof;
         auto kern = pass > 0 ? (dis ? k_therm_pipe<1, true> : k_therm_pipe<1, false>)
                     : ex     ? (dis ? k_therm_pipe<2, true> : k_therm_pipe<2, false>)
                              : (dis ? k_therm_pipe<0, true> : k_therm_pipe<0, false>);
@@ -1584,6 +1588,9 @@ struct gf_batch {
     int nm = 0;
     long tot_e = 0, tot_n = 0, tot_t = 0, tot_b = 0, tot_m = 0;
     bool markers = false;   // some model has markers (change_phase runs)
+    // every model's temp0 holds the temperature at the start of its previous thermal
+    // step: pass 0 starts from the extrapolation 2 temp - temp0, as a model alone does
+    bool temp0_prev = false, g_t0prev = false, noguess = false;
     bool movegrid = true;   // the models move their grids (all or none)
     hipStream_t stream = nullptr;
     int32_t nloop = 0, nsrate = 0;
@@ -1687,6 +1694,9 @@ extern "C" int gf_batch_create(gf_ctx *const *models, int32_t n, gf_batch **out)
     b->nm = n;
     b->tot_e = pe; b->tot_n = pn; b->tot_t = pt; b->tot_b = pb; b->tot_m = pm;
     b->movegrid = c0->p.movegrid != 0;
+    b->temp0_prev = true;
+    for (gf_ctx *c : b->m) b->temp0_prev = b->temp0_prev && c->temp0_prev;
+    b->noguess = c0->therm_noguess;
     b->therm_grid = c0->therm_grid;
     const size_t nmap = me.size() + mn.size() + mt.size() + mb.size() + mm.size() + 1;
     if (hipMalloc((void **)&b->dmap, sizeof(int) * nmap) != hipSuccess) return fail(GF_ERR_HIP, "allocation");
@@ -1761,7 +1771,11 @@ static int enq_batch_flac(gf_batch *b)
     for (int pass = 0; pass < npass; pass++) {
         // pass 0 through the pipeline; a fallback pass, which only some models need,
         // with one workgroup per tile that returns at once for a proven model
-        if (pass == 0)
+        const bool ex = b->temp0_prev && !b->noguess && proof;  // (as a model alone, enq_therm)
+        if (pass == 0 && ex)
+            BLAUNCH((dis ? kb_therm_pipe<2, true> : kb_therm_pipe<2, false>), b->therm_grid, TH_J * TH_I,
+                    b->dtab, map, b->tot_t, pass, proof ? 0.0 : GF_THERM_STOL, b->area_pending ? 1 : 0);
+        else if (pass == 0)
             BLAUNCH((dis ? kb_therm_pipe<0, true> : kb_therm_pipe<0, false>), b->therm_grid, TH_J * TH_I,
                     b->dtab, map, b->tot_t, pass, proof ? 0.0 : GF_THERM_STOL, b->area_pending ? 1 : 0);
         else
@@ -1769,6 +1783,7 @@ static int enq_batch_flac(gf_batch *b)
         if (proof) BLAUNCH(kb_therm_check, bgrid(b->tot_b, 256), 256, b->dtab, map, pass);
     }
     BLAUNCH(kb_therm_bc, bgrid(b->tot_n, 256), 256, b->dtab, map, npass, proof ? 1 : 0);
+    b->temp0_prev = true;
     if (!b->no_mech) {
         const gf_params &p = b->p0;
         const int do_visc = (b->nloop % p.ifreq_visc) == 0;
@@ -1869,13 +1884,13 @@ extern "C" int gf_batch_step(gf_batch *b, int32_t nsteps)
     for (int32_t n = 0; n < nsteps;) {
         if (b->use_graph && !b->timing && nsteps - n >= 10 && graph_cycle_ok(b)) {
             if (!b->gexec || b->g_nsrate0 != b->nsrate || b->g_rot0 != b->rot_pending ||
-                b->g_area0 != b->area_pending) {
+                b->g_area0 != b->area_pending || b->g_t0prev != b->temp0_prev) {
                 if (b->gexec) {
                     BCHK(hipGraphExecDestroy(b->gexec));
                     b->gexec = nullptr;
                 }
                 const int32_t ns0 = b->nsrate, nl0 = b->nloop;
-                const bool rot0 = b->rot_pending, area0 = b->area_pending;
+                const bool rot0 = b->rot_pending, area0 = b->area_pending, t0p = b->temp0_prev;
                 hipGraph_t g = nullptr;
                 BCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
                 for (int k = 0; k < 10; k++)
@@ -1894,16 +1909,19 @@ extern "C" int gf_batch_step(gf_batch *b, int32_t nsteps)
                 b->g_rot_end = b->rot_pending;
                 b->g_rot0 = rot0;
                 b->g_area0 = area0;
+                b->g_t0prev = t0p;
                 b->nloop = nl0;  // the capture only recorded the launches
                 b->nsrate = ns0;
                 b->rot_pending = rot0;
                 b->area_pending = area0;
+                b->temp0_prev = t0p;
             }
             BCHK(hipGraphLaunch(b->gexec, b->stream));
             b->nloop += 10;
             b->nsrate = b->g_nsrate1;
             b->rot_pending = b->g_rot_end;  // the cycle ends with a move when the models move
             b->area_pending = b->g_area_end;
+            b->temp0_prev = true;
             n += 10;
             continue;
         }
@@ -1919,6 +1937,7 @@ extern "C" int gf_batch_step(gf_batch *b, int32_t nsteps)
         c->nsrate = b->nsrate;
         c->rot_pending = b->rot_pending;  // completed by the model's own k_move_rot where needed
         c->area_pending = b->area_pending;
+        c->temp0_prev = b->temp0_prev;
         int r = fetch_clock(c, nullptr);
         if (r && first == GF_OK) first = r;
     }

@@ -77,9 +77,12 @@ def test_cfg2_seeded_box_bitwise_equal_to_oracle(big):
     assert st.clock.dt == ref.clock.dt and st.clock.time == ref.clock.time
 
 
+# measured on the device after 100 steps (rel-L2 of velocity, stress, temperature,
+# coordinates): FP32 storage 4.6e-4, 1.5e-7, 1.4e-12, 2.4e-12; FP32 arithmetic 1.3e-2,
+# 6.7e-6, 2.2e-10, 1.8e-10.  The bounds leave about 4x room.
 CFG3_BOUNDS = {
-    abi.PREC_MIXED_STORAGE: {"vel": 1e-3, "stress0": 1e-6, "temp": 1e-10, "cord": 1e-10},
-    abi.PREC_MIXED: {"vel": 3e-2, "stress0": 3e-5, "temp": 1e-9, "cord": 1e-9},
+    abi.PREC_MIXED_STORAGE: {"vel": 2e-3, "stress0": 1e-6, "temp": 1e-11, "cord": 1e-11},
+    abi.PREC_MIXED: {"vel": 5e-2, "stress0": 3e-5, "temp": 1e-9, "cord": 1e-9},
 }
 
 
