@@ -14,6 +14,7 @@
 #include <string.h>
 
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1550,6 +1551,72 @@ extern "C" int gf_mlp_grad(const float *x, const float *t, int64_t n, const floa
     for (void *b : {(void *)ds, (void *)dw, (void *)dp, (void *)dg, (void *)dc})
         if (b) (void)hipFree(b);
     return rc;
+}
+
+// ---- gf_mlp_grad_noalloc: the gradient kernels on device-resident data with their
+// scratch memory in a fixed device array (no allocation per call).  Layout of scratch:
+// the identity normalisation (MlpCfg) in the first 512 bytes, then the partial
+// gradients of at most MLP_GRAD_BLOCKS workgroups (MLP_NP + 1 floats each).  Every call
+// rewrites both before they are read; calls that overlap on one GPU race on them.
+#define GF_NOALLOC_CFG_BYTES 512
+#define GF_NOALLOC_SCRATCH_BYTES (GF_NOALLOC_CFG_BYTES + sizeof(float) * (MLP_NP + 1) * MLP_GRAD_BLOCKS)
+static_assert(sizeof(MlpCfg) <= GF_NOALLOC_CFG_BYTES, "the normalisation block fits its slot");
+static __device__ __align__(256) unsigned char scratch[GF_NOALLOC_SCRATCH_BYTES];
+
+#define GF_NOALLOC_MAXDEV 64
+static void *g_scratch_ptr[GF_NOALLOC_MAXDEV];  // device address of scratch, per device
+static std::mutex g_scratch_mu;
+
+// the device address of scratch on the current device (its host name is only a
+// stand-in), looked up once per device
+static int noalloc_scratch(void **out)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= GF_NOALLOC_MAXDEV) return GF_ERR_HIP;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return GF_ERR_HIP;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return GF_OK;
+}
+
+__global__ void k_mlp_cfg_identity(MlpCfg *c)
+{
+    const int f = threadIdx.x;
+    if (f < 8) {
+        c->xmean[f] = 0.0f;
+        c->xstd[f] = 1.0f;
+        c->xlo[f] = 0.0f;
+        c->xhi[f] = 0.0f;
+    }
+    if (f == 0) {
+        c->ymean = 0.0f;
+        c->ystd = 1.0f;
+    }
+}
+
+extern "C" int gf_mlp_grad_noalloc(const float *samples, int64_t n, const float *w, float *grad, void *stream)
+{
+    if (!samples || !w || !grad || n < 1) return GF_ERR_ARG;
+    if (n > GF_MLP_NOALLOC_MAXN) return GF_ERR_ARG;  // (per-workgroup sample counts are 32-bit)
+    void *base = nullptr;
+    int rc = noalloc_scratch(&base);
+    if (rc) return rc;
+    MlpCfg *cfg = (MlpCfg *)base;
+    float *partial = (float *)((unsigned char *)base + GF_NOALLOC_CFG_BYTES);
+    int nb, per_block;
+    grad_grid((long)n, nb, per_block);
+    if (nb > MLP_GRAD_BLOCKS) return GF_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_mlp_cfg_identity, dim3(1), dim3(64), 0, s, cfg);
+    hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(256), 0, s, w, samples, (long)n, 0L, (long)n, 1L, 0L, cfg,
+                       per_block, partial);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_NP + MLP_RED_PARAMS) / MLP_RED_PARAMS), dim3(256), 0, s, partial, nb,
+                       grad);
+    return hipGetLastError() == hipSuccess ? GF_OK : GF_ERR_HIP;
 }
 
 extern "C" int gf_mlp_infer(const float *x, int64_t n, const float *w, float *y)

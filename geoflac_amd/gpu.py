@@ -20,7 +20,7 @@ EXPORTS = [
     "gf_init_visc", "gf_therm_stats", "gf_set_therm_passes", "gf_sync",
     "gf_set_timing", "gf_kernel_time", "gf_math_check",
     "gf_surrogate_sample", "gf_surrogate_train", "gf_surrogate_enable", "gf_surrogate_stats",
-    "gf_surrogate_weights", "gf_mlp_grad", "gf_mlp_infer",
+    "gf_surrogate_weights", "gf_mlp_grad", "gf_mlp_infer", "gf_mlp_grad_noalloc",
     "gf_batch_create", "gf_batch_destroy", "gf_batch_last_error", "gf_batch_step", "gf_batch_set_graph",
     "gf_batch_autotune", "gf_stream_probe",
 ]
@@ -75,6 +75,7 @@ def lib():
         L.gf_surrogate_weights.argtypes = [vp, fp, fp, C.c_int32]
         L.gf_mlp_grad.argtypes = [fp, fp, C.c_int64, fp, fp, fp]
         L.gf_mlp_infer.argtypes = [fp, C.c_int64, fp, fp]
+        L.gf_mlp_grad_noalloc.argtypes = [vp, C.c_int64, vp, vp, vp]
         L.gf_batch_create.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(vp)]
         L.gf_batch_destroy.argtypes = [vp]
         L.gf_batch_destroy.restype = None

@@ -251,6 +251,18 @@ int gf_surrogate_weights(gf_ctx *ctx, float *w, float *cfg, int32_t get);
  * the BF16 matrix-core forward pass. */
 int gf_mlp_grad(const float *x, const float *t, int64_t n, const float *w, float *grad, float *loss);
 int gf_mlp_infer(const float *x, int64_t n, const float *w, float *y);
+/* No counterpart in the reference: gf_mlp_grad for callers that cannot allocate device
+ * memory (code under HIP graph capture, training loops that forbid hipMalloc).  Every
+ * pointer is device memory the caller owns: samples[n][8] (seven features, then the
+ * target; a NaN target gives the sample weight 0), w[GF_MLP_NPARAM], and grad[GF_MLP_NPARAM + 1]
+ * which receives the gradient of the mean squared error and, last, the summed squared
+ * error.  The kernels' partial gradients live in a fixed device array of the library
+ * (4.9 MB on each GPU it runs on, held until the process ends); nothing is allocated
+ * or freed.  The launches are queued on `stream` (a hipStream_t; NULL: the default
+ * stream) and the call returns without waiting.  1 <= n <= GF_MLP_NOALLOC_MAXN.  Calls
+ * that overlap on one GPU share that array and race. */
+#define GF_MLP_NOALLOC_MAXN 2147483647LL
+int gf_mlp_grad_noalloc(const float *samples, int64_t n, const float *w, float *grad, void *stream);
 
 /* ---- Batches of independent models (parameter sweeps); no reference counterpart.
  * The models are contexts created, bound and uploaded as usual, with the same
