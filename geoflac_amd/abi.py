@@ -51,7 +51,7 @@ class GfParams(C.Structure):
     _fields_ = ([(n, _I) for n in _INT_FIELDS]
                 + [("irheol", _I * MAXPH)]
                 + [(n, _I * MAXINH) for n in ("ix1", "ix2", "iy1", "iy2", "igeom")]
-                + [("flexure_base", _I), ("nelem_serp", _I), ("pad_i", _I)]
+                + [("flexure_base", _I), ("nelem_serp", _I), ("nopbmax", _I)]
                 + [(n, _D) for n in _REAL_FIELDS]
                 + [(n, _D * MAXPH) for n in PHASE_TABLES]
                 + [("topo_kappa", _D)])
@@ -72,7 +72,7 @@ HOST_FIELDS = [
 
 
 class GfHostArrays(C.Structure):
-    _fields_ = HOST_FIELDS + [("coh_scale", _FP)]
+    _fields_ = HOST_FIELDS + [("nopbou", _IP), ("bcstress", _DP), ("coh_scale", _FP)]
 
 
 MAX_MARKERS_PER_ELEM = 32  # src/marker_data.f90:5
@@ -94,6 +94,7 @@ FIELD_BIT = {name: 1 << k for k, (name, _) in enumerate(HOST_FIELDS)}
 GF_ALL = (1 << len(HOST_FIELDS)) - 1
 GF_MARKERS = 1 << 22
 GF_COH_SCALE = 1 << 23
+GF_STRESSBC = 1 << 24
 
 SIZEOF_PARAMS = C.sizeof(GfParams)
 SIZEOF_HOST = C.sizeof(GfHostArrays)

@@ -40,7 +40,7 @@ struct GfDevParams {
     int itherm, istress_therm, ishearh, itemp_bc;
     int nystressbc, nyhydro, i_prestress, idt_scale;
     int i_rey, ivis_present, iphsub, inhom;
-    int nelem_serp, pad_ns;
+    int nelem_serp, nydrsides;
     int active[GF_MAXPH];  // phase ids (1-based) present anywhere with ratio >= 0.01
     int ix1[GF_MAXINH], ix2[GF_MAXINH], iy1[GF_MAXINH], iy2[GF_MAXINH], igeom[GF_MAXINH];
     double t_top, t_bot, bot_bc, g, demf, ten_off, tau_heal, v_min, v_max;

@@ -34,6 +34,14 @@ struct GfDevArrays {
     int *nmark_elem;
     int *phstamp;         // per element: the change_phase call that last changed one of its markers
     long nmarkers;
+    // stress boundary conditions (bc_update): per segment its two nodes (linear
+    // indices) and normal and shear stress; per boundary node (edge index, bc_edge)
+    // its segments in increasing order, as 2 * segment + (0: first node, 1: second);
+    // the hydrostatic pressure of the elements of the two side columns
+    const int *bc_nodes;
+    const double *bc_str;
+    const int *bc_estart, *bc_elist;
+    double *bc_press;
 };
 
 // the isotropic reference added back to an SV stress (FP64 path: none)
@@ -973,10 +981,13 @@ GF_DEV void therm_check_body(const GfDevParams *__restrict__ P, GfDevClock *ck, 
     }
 }
 
+// (grid-stride: a launch for a fallback pass may use fewer workgroups than entries)
 __global__ void __launch_bounds__(256) k_therm_check(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
-                                                     int pass, int arp)
+                                                     int pass, int arp, long total)
 {
-    therm_check_body(P, ck, A, pass, (long)blockIdx.x * blockDim.x + threadIdx.x, arp);
+    if (pass > 0 && ck->therm_pass_changed[pass - 1] == 0u) return;  // that pass did not run
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x)
+        therm_check_body(P, ck, A, pass, t, arp);
 }
 
 // Boundary conditions src/fl_therm.f90:575-598 applied to the proven iterate.
@@ -1504,6 +1515,7 @@ GF_DEV void move_rotate(T &s11, T &s22, T &s12, T w)
 // round trip for its inputs instead of a chain of them.
 #define RH_ROT 1
 #define RH_AREA 2
+#define RH_NODVOL 4  // (with RH_AREA) dvol is formed and used but not stored: nothing reads it back
 template <bool SHEARH, typename SV, typename AT>
 GF_DEV void rheol_single(const GfDevParams *__restrict__ P, const GfPhase *__restrict__ phs, GfDevClock *ck,
                          const GfDevArrays &A, long e, int avgsr, int rot)
@@ -1571,7 +1583,7 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, const GfPhase *__res
                 dvl[k] = det * ar[k] - 1.0;
                 ar[k] = 1.0 / det;
                 bad |= (ar[k] <= 0);
-                bst(bD, eo, k * pe, dvl[k]);
+                if (!(rot & RH_NODVOL)) bst(bD, eo, k * pe, dvl[k]);
                 bst(bA, eo, k * pe, ar[k]);
             }
             if (bad) raise_error(ck, GF_STOP_NEGATIVE_AREA, e);
@@ -2633,7 +2645,7 @@ __global__ void __launch_bounds__(256) k_init_visc(const GfDevParams *__restrict
 // rot: the last element move's rotation is pending; the loaded stress is rotated as
 // k_move_rot would store it, for the pressure only (the move stays pending)
 template <int M>
-GF_DEV void dens_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e, int rot)
+GF_DEV double eff_dens_value(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e, int rot)
 {
     typedef typename Prec<M>::SV SV;
     typedef typename Prec<M>::AT AT;
@@ -2703,7 +2715,12 @@ GF_DEV void dens_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const G
         ed = ed + ratio * dens;
     }
     if (PHASE(iph).is_mantle) ed = ed * den_amp;
-    A.dens[e] = ed;
+    return ed;
+}
+template <int M>
+GF_DEV void dens_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e, int rot)
+{
+    A.dens[e] = eff_dens_value<M>(P, ck, A, e, rot);
 }
 template <int M>
 __global__ void __launch_bounds__(256) k_dens(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A, int rot)

@@ -48,6 +48,14 @@ GF_DEV double phase_ratio_of(const GfDevParams *__restrict__ P, const GfDevArray
     return 0.0;
 }
 
+// phase ph has a ratio plane (some element holds it)
+GF_DEV bool phase_has_plane(const GfDevParams *__restrict__ P, int ph)
+{
+    for (int a = 0; a < P->nactive; a++)
+        if (P->active[a] == ph) return true;
+    return false;
+}
+
 // src/change_phase.f90:58-230 for marker kk
 GF_DEV void phase_rules_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long kk)
 {
@@ -60,16 +68,43 @@ GF_DEV void phase_rules_body(const GfDevParams *__restrict__ P, GfDevClock *ck, 
     const int k = (n - 1) % 2 + 1;
     const int j = ((n - k) / 2) % (NZ - 1) + 1;
     const int i = (n - k) / 2 / (NZ - 1) + 1;
-    const double a1 = A.mark_a1[kk], a2 = A.mark_a2[kk];
-    double yy, tmpr;
-    if (k == 1) {
-        yy = CORD(j, i, 2) * a1 + CORD(j + 1, i, 2) * a2 + CORD(j, i + 1, 2) * (1 - a1 - a2);
-        tmpr = TEMP(j, i) * a1 + TEMP(j + 1, i) * a2 + TEMP(j, i + 1) * (1 - a1 - a2);
-    } else {
-        yy = CORD(j, i + 1, 2) * a1 + CORD(j + 1, i, 2) * a2 + CORD(j + 1, i + 1, 2) * (1 - a1 - a2);
-        tmpr = TEMP(j, i + 1) * a1 + TEMP(j + 1, i) * a2 + TEMP(j + 1, i + 1) * (1 - a1 - a2);
+    // the marker's triangle: corners (c1, c2, c3) as the interpolation below weighs them
+    const int j1 = k == 1 ? j : j, i1 = k == 1 ? i : i + 1;
+    const int j3 = k == 1 ? j : j + 1, i3 = i + 1;
+    const double y1 = CORD(j1, i1, 2), y2 = CORD(j + 1, i, 2), y3 = CORD(j3, i3, 2);
+    const double t1 = TEMP(j1, i1), t2 = TEMP(j + 1, i), t3 = TEMP(j3, i3);
+    const double ysurf = 0.5 * (CORD(1, i, 2) + CORD(1, i + 1, 2));
+    // Markers whose rules cannot fire anywhere in their triangle are skipped before
+    // their barycentric coordinates are read.  The interpolated y and temperature lie
+    // between the corner values (a convex combination, up to a few ulp of rounding,
+    // which the margins cover), so bounds from the corners decide it exactly.
+    {
+        const double tmax = fmax(fmax(t1, t2), t3);
+        const double ylo = fmin(fmin(y1, y2), y3), yhi = fmax(fmax(y1, y2), y3);
+        const double dm = 1e-6 * (fabs(ysurf) + fabs(ylo) + fabs(yhi) + 1.0), tm = 1e-9 * (fabs(tmax) + 1.0);
+        const double dlo = ysurf - yhi - dm, dhi = ysurf - ylo + dm;  // depth range
+        if (dlo > max_melting_depth) return;
+        if (iph == PH_MANT1 || iph == PH_MANT2) {
+            if (dlo > max_basalt_depth) return;
+            // without a phase that triggers serpentinisation only the dry-mantle rule
+            // remains: tmpr >= solidus + 35, and solidus + 35 = -5.02 x^2 + 132 x + 1160
+            // (x the pressure in GPa) is concave, so on [0, xmax] at least its value at
+            // an end of the interval
+            if (!(phase_has_plane(P, PH_OCEAN1) || phase_has_plane(P, PH_OCEAN2) || phase_has_plane(P, PH_SED1) ||
+                  phase_has_plane(P, PH_SED2))) {
+                const double dabs = fmax(fabs(dlo), fabs(fmin(dhi, max_basalt_depth)));
+                const double xmax = mantle_density * fabs(P->g) * dabs * 1e-9 * (1.0 + 1e-12);
+                const double smin = fmin(1160.0, -5.02 * (xmax * xmax) + 1.32e2 * xmax + 1.16e3) - 1e-6;
+                if (tmax + tm < smin) return;
+            }
+        } else if (iph == PH_OCEANDRY || iph == PH_OCEAN1 || iph == PH_OCEAN2) {
+            if (tmax + tm < min_eclogite_temp || dhi < min_eclogite_depth) return;
+        }
     }
-    const double depth = 0.5 * (CORD(1, i, 2) + CORD(1, i + 1, 2)) - yy;
+    const double a1 = A.mark_a1[kk], a2 = A.mark_a2[kk];
+    const double yy = y1 * a1 + y2 * a2 + y3 * (1 - a1 - a2);
+    const double tmpr = t1 * a1 + t2 * a2 + t3 * (1 - a1 - a2);
+    const double depth = ysurf - yy;
     if (depth > max_melting_depth) return;
     int newph = 0;
     double trpres, trpres2, press;

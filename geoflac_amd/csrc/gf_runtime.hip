@@ -20,6 +20,7 @@
 
 #include "gf_kernels.h"
 #include "gf_phase.h"
+#include "gf_bc.h"
 #include "gf_mlp.h"
 #include "gf_batch.h"
 
@@ -39,6 +40,10 @@ struct gf_ctx {
     bool markers_bound = false, markers_uploaded = false;
     long mark_cap = 0;
     float *coh_scale_dev = nullptr;
+    // stress boundary conditions on the device (bc_update)
+    int *bc_nodes = nullptr, *bc_estart = nullptr, *bc_elist = nullptr;
+    double *bc_str = nullptr, *bc_press = nullptr;
+    bool stressbc_uploaded = false;
     uint32_t uploaded = 0;
     int32_t nloop = 0, nsrate = 0;
     long ne = 0, nn = 0;
@@ -83,6 +88,10 @@ struct gf_ctx {
     // passes when needed); 1: FP32-arithmetic mode takes the single tile pass as it is
     int therm_passes = TH_MAXPASS;
     bool therm_proof = true;  // false: FP32-arithmetic mode, the single pass is not proven
+    // Fallback thermal passes launch with few workgroups (they exit at once when the
+    // pass before was proven) unless the last call needed one: then with the full grid
+    bool therm_fb_full = true;
+    unsigned long long therm_extra_seen = 0;
     // per-kernel event timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -132,6 +141,7 @@ static int build_dev_params(gf_ctx *ctx)
     d.nystressbc = p.nystressbc; d.nyhydro = p.nyhydro; d.i_prestress = p.i_prestress; d.idt_scale = p.idt_scale;
     d.i_rey = p.i_rey; d.ivis_present = p.ivis_present; d.iphsub = p.iphsub; d.inhom = p.inhom;
     d.nelem_serp = p.nelem_serp;
+    d.nydrsides = p.nydrsides;
     for (int k = 0; k < GF_MAXINH; k++) {
         d.ix1[k] = p.ix1[k]; d.ix2[k] = p.ix2[k]; d.iy1[k] = p.iy1[k]; d.iy2[k] = p.iy2[k]; d.igeom[k] = p.igeom[k];
     }
@@ -207,7 +217,13 @@ extern "C" int gf_create(const gf_params *p, int32_t precision, gf_ctx **out)
     if (precision != GF_PREC_FP64 && precision != GF_PREC_MIXED && precision != GF_PREC_MIXED_STORAGE)
         return GF_ERR_UNSUPPORTED;
     // paths of the reference this library does not provide (listed in gf_abi.h)
-    if (p->itype_melting != 0 || p->nystressbc != 0 || p->ndim != 2) return GF_ERR_UNSUPPORTED;
+    if (p->itype_melting != 0 || p->ndim != 2) return GF_ERR_UNSUPPORTED;
+    // the hydrostatic side walls hold a column of pressures in LDS
+    if (p->nystressbc == 1 && p->nydrsides == 1 && p->nz > BC_MAXNZ) return GF_ERR_UNSUPPORTED;
+    if (p->nystressbc == 1 && p->nopbmax < 0) return GF_ERR_ARG;
+    // idt_scale 0 (dt_elastic from the wave speed): the reference's minimum starts from
+    // the dt_elastic it was given (0 after setflac) and the step divides by it
+    if (p->idt_scale == 0) return GF_ERR_UNSUPPORTED;
     // the flexural base and surface processes of fl_move (src/fl_move.f90:37-40)
     if (p->flexure_base != 0 || p->topo_kappa > 0.0) return GF_ERR_UNSUPPORTED;
     if (p->inhom < 0 || p->inhom > GF_MAXINH) return GF_ERR_ARG;
@@ -319,6 +335,7 @@ extern "C" void gf_destroy(gf_ctx *ctx)
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     void *bufs[] = {ctx->dp, ctx->dck, ctx->a.mark_a1, ctx->a.mark_a2, ctx->a.mark_ntriag, ctx->a.mark_phase,
                     ctx->a.mark_dead, ctx->a.mark_id_elem, ctx->a.nmark_elem, ctx->a.phstamp, ctx->coh_scale_dev,
+                    ctx->bc_nodes, ctx->bc_estart, ctx->bc_elist, ctx->bc_str, ctx->bc_press,
                     ctx->a.cord, ctx->a.vel, ctx->a.temp, ctx->a.temp0,
                     ctx->a.force, ctx->a.amass, ctx->a.rmass, ctx->a.bc, ctx->a.ncod, ctx->a.stress0,
                     ctx->a.area, ctx->a.dvol, ctx->a.strain, ctx->a.se2sr, ctx->a.aps, ctx->a.visn,
@@ -533,6 +550,69 @@ static int upload_markers(gf_ctx *ctx)
     return GF_OK;
 }
 
+// the stress boundary segments (nopbou/bcstress) as per-node gather lists
+static int upload_stressbc(gf_ctx *ctx)
+{
+    const gf_params &p = ctx->p;
+    const int nz = p.nz, nx = p.nx, ns = p.nopbmax;
+    const long rows = 2L * ((nz - 1) + (nx - 1));
+    if (ns > rows) return set_err(ctx, GF_ERR_ARG, "nopbmax %d exceeds the %ld segment rows", ns, rows);
+    if (ns > 0 && (!ctx->host.nopbou || !ctx->host.bcstress))
+        return set_err(ctx, GF_ERR_ARG, "host pointer for nopbou or bcstress is null");
+    const int nedge = 2 * nz + 2 * (nx - 2);
+    std::vector<int> nodes(2 * std::max(ns, 1)), cnt(nedge + 1, 0);
+    std::vector<double> str(2 * std::max(ns, 1));
+    auto edge = [&](int j, int i) {
+        if (i == 1) return j - 1;
+        if (i == nx) return nz + j - 1;
+        if (j == 1) return 2 * nz + (i - 2);
+        if (j == nz) return 2 * nz + (nx - 2) + (i - 2);
+        return -1;
+    };
+    std::vector<std::vector<int>> lists(nedge);
+    for (int q = 0; q < ns; q++) {
+        const int *nb = ctx->host.nopbou;
+        const int ii1 = nb[q], ii2 = nb[q + rows], jj1 = nb[q + 2 * rows], jj2 = nb[q + 3 * rows];
+        if (ii1 < 1 || ii1 > nx || ii2 < 1 || ii2 > nx || jj1 < 1 || jj1 > nz || jj2 < 1 || jj2 > nz)
+            return set_err(ctx, GF_ERR_ARG, "boundary segment %d outside the mesh", q + 1);
+        const int ea = edge(jj1, ii1), eb = edge(jj2, ii2);
+        if (ea < 0 || eb < 0) return set_err(ctx, GF_ERR_ARG, "boundary segment %d is not on the boundary", q + 1);
+        nodes[2 * q] = (jj1 - 1) + nz * (ii1 - 1);
+        nodes[2 * q + 1] = (jj2 - 1) + nz * (ii2 - 1);
+        str[2 * q] = ctx->host.bcstress[q];
+        str[2 * q + 1] = ctx->host.bcstress[q + rows];
+        lists[ea].push_back(2 * q);
+        if (eb != ea) lists[eb].push_back(2 * q + 1);
+    }
+    std::vector<int> start(nedge + 1, 0), flat;
+    for (int e = 0; e < nedge; e++) {
+        start[e] = (int)flat.size();
+        flat.insert(flat.end(), lists[e].begin(), lists[e].end());
+    }
+    start[nedge] = (int)flat.size();
+    if (flat.empty()) flat.push_back(0);
+    for (void *b : {(void *)ctx->bc_nodes, (void *)ctx->bc_estart, (void *)ctx->bc_elist, (void *)ctx->bc_str})
+        if (b) (void)hipFree(b);
+    ctx->bc_nodes = ctx->bc_estart = ctx->bc_elist = nullptr;
+    ctx->bc_str = nullptr;
+    HIPCHK(hipMalloc((void **)&ctx->bc_nodes, sizeof(int) * nodes.size()));
+    HIPCHK(hipMalloc((void **)&ctx->bc_str, sizeof(double) * str.size()));
+    HIPCHK(hipMalloc((void **)&ctx->bc_estart, sizeof(int) * start.size()));
+    HIPCHK(hipMalloc((void **)&ctx->bc_elist, sizeof(int) * flat.size()));
+    if (!ctx->bc_press) HIPCHK(hipMalloc((void **)&ctx->bc_press, sizeof(double) * 2 * (nz - 1)));
+    HIPCHK(hipMemcpy(ctx->bc_nodes, nodes.data(), sizeof(int) * nodes.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->bc_str, str.data(), sizeof(double) * str.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->bc_estart, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->bc_elist, flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice));
+    ctx->a.bc_nodes = ctx->bc_nodes;
+    ctx->a.bc_str = ctx->bc_str;
+    ctx->a.bc_estart = ctx->bc_estart;
+    ctx->a.bc_elist = ctx->bc_elist;
+    ctx->a.bc_press = ctx->bc_press;
+    ctx->stressbc_uploaded = true;
+    return GF_OK;
+}
+
 static int upload_coh_scale(gf_ctx *ctx)
 {
     if (!ctx->host.coh_scale) {  // none: factor 1
@@ -614,6 +694,10 @@ extern "C" int gf_upload(gf_ctx *ctx, uint32_t mask)
     }
     if (mask & GF_COH_SCALE) {
         int rc = upload_coh_scale(ctx);
+        if (rc) return rc;
+    }
+    if (mask & GF_STRESSBC) {
+        int rc = upload_stressbc(ctx);
         if (rc) return rc;
     }
     if (mask & GF_PHASE_RATIO) {
@@ -702,6 +786,10 @@ static int fetch_clock(gf_ctx *ctx, gf_clock *c)
     GfDevClock ck;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipMemcpy(&ck, ctx->dck, sizeof(ck), hipMemcpyDeviceToHost));
+    // the last call's fallback passes decide the grid of the next call's fallback launches
+    if (ck.therm_extra_passes < ctx->therm_extra_seen) ctx->therm_extra_seen = 0;  // (statistics were reset)
+    ctx->therm_fb_full = ck.therm_extra_passes > ctx->therm_extra_seen;
+    ctx->therm_extra_seen = ck.therm_extra_passes;
     if (c) {
         c->nloop = ctx->nloop; c->nsrate = ctx->nsrate;
         c->time = ck.time; c->dt = ck.dt; c->dtavg = ck.dtavg;
@@ -841,7 +929,7 @@ static int enq_therm(gf_ctx *ctx, bool clock_done = false)
     const int ntiles = ((ctx->p.nz + TH_OJ - 1) / TH_OJ) * ((ctx->p.nx + TH_OI - 1) / TH_OI);
     const int npass = ctx->therm_passes;
     for (int pass = 0; pass < npass; pass++) {
-        const dim3 grid(ctx->therm_grid);
+        const dim3 grid(pass == 0 || ctx->therm_fb_full ? ctx->therm_grid : 32u);
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (ctx->timing) {
             e0 = next_event(ctx);
@@ -866,15 +954,20 @@ static int enq_therm(gf_ctx *ctx, bool clock_done = false)
             (void)hipEventRecord(e1, ctx->stream);
             ctx->pending.push_back({pass == 0 ? "k_therm_pipe" : "k_therm_pipe_extra", {e0, e1}});
         }
-        if (ctx->therm_proof)
-            LAUNCH(k_therm_check, (long)ntiles * TH_NB, ctx->dp, ctx->dck, ctx->a, pass, ctx->area_pending ? 1 : 0);
+        if (ctx->therm_proof) {
+            const long nchk = (long)ntiles * TH_NB;
+            LAUNCH(k_therm_check, pass == 0 || ctx->therm_fb_full ? nchk : 64L * 256, ctx->dp, ctx->dck, ctx->a, pass,
+                   ctx->area_pending ? 1 : 0, nchk);
+        }
     }
     LAUNCH(k_therm_bc, ctx->nn, ctx->dp, ctx->dck, ctx->a, npass, ctx->therm_proof ? 1 : 0);
     ctx->temp0_prev = true;  // pass 0 wrote temp0 at every node
     return GF_OK;
 }
 
-static int enq_rheol(gf_ctx *ctx, bool clock_done = false)
+// last: no later step of the same call follows (a step in the middle of a call may
+// leave a pending move's dvol unstored: the next step's fl_rheol forms it again)
+static int enq_rheol(gf_ctx *ctx, bool clock_done = false, bool last = true)
 {
     int rc;
     const int do_visc = (ctx->nloop % ctx->p.ifreq_visc) == 0;
@@ -889,7 +982,8 @@ static int enq_rheol(gf_ctx *ctx, bool clock_done = false)
     // loads itself (the move stays pending for k_rheol1); the surrogate's passes and
     // the quad rheology find the move completed
     if (ctx->rheol_quad && (rc = flush_rot(ctx))) return rc;
-    const int rot = (ctx->rot_pending ? RH_ROT : 0) | (ctx->area_pending ? RH_AREA : 0);
+    const int rot = (ctx->rot_pending ? RH_ROT : 0) | (ctx->area_pending ? RH_AREA : 0) |
+                    (ctx->area_pending && !last && ctx->p.movegrid != 0 ? RH_NODVOL : 0);
     ctx->rot_pending = false;
     ctx->area_pending = false;
     if (do_visc && ctx->sur_on) {
@@ -1037,6 +1131,8 @@ static int check_ready(gf_ctx *ctx)
 {
     if ((ctx->uploaded & kRequired) != kRequired)
         return set_err(ctx, GF_ERR_STATE, "state not uploaded (mask 0x%x missing)", kRequired & ~ctx->uploaded);
+    if (ctx->p.nystressbc == 1 && !ctx->stressbc_uploaded)
+        return set_err(ctx, GF_ERR_STATE, "nystressbc = 1: upload the boundary segments (GF_STRESSBC)");
     // change_phase needs the markers as soon as one present phase has a rule
     if (!ctx->markers_uploaded && ctx->p.itherm != 2)
         for (int a = 0; a < ctx->hp.nactive; a++)
@@ -1044,6 +1140,34 @@ static int check_ready(gf_ctx *ctx)
                 return set_err(ctx, GF_ERR_STATE,
                                "phase %d has change_phase rules (src/change_phase.f90): upload the markers (GF_MARKERS)",
                                ctx->hp.active[a]);
+    return GF_OK;
+}
+
+// bc_update, src/bc_update.f90:6-143 (gf_bc.h): the boundary forces fl_node starts from
+static int enq_bc_update(gf_ctx *ctx)
+{
+    // a call outside the step (nystressbc = 0) with no applied stresses builds the empty
+    // segment lists here; with segments the caller must have uploaded them
+    if (!ctx->stressbc_uploaded) {
+        if (ctx->p.nopbmax > 0)
+            return set_err(ctx, GF_ERR_STATE, "bc_update: upload the boundary segments (GF_STRESSBC)");
+        int urc = upload_stressbc(ctx);
+        if (urc) return urc;
+    }
+    if (!ctx->a.bc_estart || !ctx->a.bc_elist || !ctx->a.bc_nodes || !ctx->a.bc_str || !ctx->a.bc_press)
+        return set_err(ctx, GF_ERR_STATE, "bc_update: boundary segment lists missing");
+    if (ctx->p.nydrsides == 1 && ctx->p.nz > BC_MAXNZ)
+        return set_err(ctx, GF_ERR_UNSUPPORTED, "hydrostatic side walls need nz <= %d", BC_MAXNZ);
+    int frc = flush_rot(ctx);  // (nothing is pending after the step's fl_rheol)
+    if (frc) return frc;
+    if (ctx->p.nydrsides == 1) {
+        const int M = ctx->precision == GF_PREC_MIXED ? 1 : ctx->precision == GF_PREC_MIXED_STORAGE ? 2 : 0;
+        auto kern = M == 1 ? k_bc_side<1> : M == 2 ? k_bc_side<2> : k_bc_side<0>;
+        hipLaunchKernelGGL(kern, dim3(2), dim3(256), 0, ctx->stream, ctx->dp, ctx->dck, ctx->a);
+        int rc = launch_check(ctx, "k_bc_side");
+        if (rc) return rc;
+    }
+    LAUNCH_SV(k_bc_force, ctx->nn, ctx->dp, ctx->dck, ctx->a);
     return GF_OK;
 }
 
@@ -1072,7 +1196,7 @@ static int enq_area_check(gf_ctx *ctx)
 
 // one call of subroutine flac, src/flac.f90:18-83; pending: clock updates left by the
 // previous step (its time advance), launched together with this step's own
-static int enq_flac(gf_ctx *ctx, int pending = 0)
+static int enq_flac(gf_ctx *ctx, int pending = 0, bool last = true)
 {
     int rc;
     const bool mech = ctx->p.itherm != 2;
@@ -1081,7 +1205,8 @@ static int enq_flac(gf_ctx *ctx, int pending = 0)
     if ((rc = enq_therm(ctx, true))) return rc;
     if (mech) {
         if (ctx->nloop % ctx->p.ifreq_rmasses == 0 && (rc = enq_change_phase(ctx))) return rc;
-        if ((rc = enq_rheol(ctx, true))) return rc;
+        if ((rc = enq_rheol(ctx, true, last))) return rc;
+        if (ctx->p.nystressbc == 1 && (rc = enq_bc_update(ctx))) return rc;
         if ((rc = enq_node(ctx, true))) return rc;
         if ((rc = enq_move(ctx))) return rc;
         if (ctx->nloop % ctx->p.ifreq_rmasses == 0 && (rc = enq_rmasses(ctx))) return rc;
@@ -1097,7 +1222,7 @@ extern "C" int gf_step(gf_ctx *ctx, int32_t nsteps, gf_clock *clock)
     if (rc) return rc;
     for (int32_t n = 0; n < nsteps; n++) {
         // the previous step's time advance goes with this step's first clock update
-        if ((rc = enq_flac(ctx, n > 0 ? CK_TIME_ADVANCE : 0))) return rc;
+        if ((rc = enq_flac(ctx, n > 0 ? CK_TIME_ADVANCE : 0, n == nsteps - 1))) return rc;
         ctx->nloop = ctx->nloop + 1;
     }
     if (nsteps > 0 && (rc = enq_clock(ctx, CK_TIME_ADVANCE))) return rc;
@@ -1141,6 +1266,7 @@ SINGLE(gf_fl_move, enq_move(ctx))
 SINGLE(gf_rmasses, enq_rmasses(ctx))
 SINGLE(gf_dt_mass, enq_dt_mass(ctx))
 SINGLE(gf_change_phase, enq_change_phase(ctx))
+SINGLE(gf_bc_update, enq_bc_update(ctx))
 
 extern "C" int gf_init_areas(gf_ctx *ctx)
 {
@@ -1737,7 +1863,8 @@ extern "C" int gf_batch_create(gf_ctx *const *models, int32_t n, gf_batch **out)
         if (c->precision != c0->precision || q.ishearh != q0.ishearh || q.itherm != q0.itherm ||
             q.ifreq_visc != q0.ifreq_visc || q.ifreq_avgsr != q0.ifreq_avgsr || q.ifreq_rmasses != q0.ifreq_rmasses ||
             q.ifreq_imasses != q0.ifreq_imasses || c->nloop != c0->nloop || c->nsrate != c0->nsrate || c->sur_on ||
-            c->rheol_quad || (q.movegrid != 0) != (q0.movegrid != 0) || c->therm_passes != c0->therm_passes)
+            c->rheol_quad || (q.movegrid != 0) != (q0.movegrid != 0) || c->therm_passes != c0->therm_passes ||
+            q.nystressbc != 0)
             return fail(GF_ERR_ARG,
                         "models of a batch share precision, switches, frequencies, movegrid and loop counters");
         if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(GF_ERR_HIP, "sync");

@@ -71,7 +71,7 @@ p%ivis_present = ivis_present; p%iphsub = iphsub; p%ifreq_rmasses = ifreq_rmasse
 p%ifreq_imasses = ifreq_imasses; p%ifreq_visc = ifreq_visc; p%ifreq_avgsr = ifreq_avgsr
 p%inhom = inhom
 p%nelem_serp = nelem_serp
-p%pad_i = 0
+p%nopbmax = nopbmax
 p%irheol = 0; p%ix1 = 0; p%ix2 = 0; p%iy1 = 0; p%iy2 = 0; p%igeom = 0
 ! the flexural base (src/fl_flexure.f90:15-24) and surface diffusion (src/fl_move.f90:40)
 ! are refused by gf_create
@@ -113,11 +113,13 @@ h%aps = gf_addr8(aps); h%visn = gf_addr8(visn); h%e2sr = gf_addr8(e2sr)
 h%sshrheat = gf_addr8(sshrheat); h%shrheat = gf_addr8(shrheat); h%source = gf_addr8(source)
 h%iphase = gf_addr4(iphase)
 h%phase_ratio = gf_addr8(phase_ratio)
+h%nopbou = gf_addr4(nopbou)
+h%bcstress = gf_addr8(bcstress)
 h%coh_scale = c_null_ptr
 rc = gf_bind_host(gf_ctx_handle, h)
 if (rc /= GF_OK) call flac_hip_fail(rc)
 call flac_hip_bind_markers
-rc = gf_upload(gf_ctx_handle, ior(GF_ALL, GF_MARKERS))
+rc = gf_upload(gf_ctx_handle, ior(ior(GF_ALL, GF_MARKERS), GF_STRESSBC))
 if (rc /= GF_OK) call flac_hip_fail(rc)
 ck%nloop = nloop; ck%nsrate = nsrate; ck%time = time; ck%dt = dt; ck%dtavg = dtavg
 ck%dt_elastic = dt_elastic; ck%dt_maxwell = dt_maxwell

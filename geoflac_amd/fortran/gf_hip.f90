@@ -14,6 +14,7 @@ integer(c_int32_t), parameter :: GF_ALL = int(z'3FFFFF', c_int32_t)
 integer(c_int32_t), parameter :: GF_MARKERS = int(z'400000', c_int32_t)
 integer(c_int32_t), parameter :: GF_PHASE_RATIO = int(z'200000', c_int32_t), GF_IPHASE = int(z'100000', c_int32_t)
 integer(c_int32_t), parameter :: GF_APS = int(z'4000', c_int32_t)
+integer(c_int32_t), parameter :: GF_STRESSBC = int(z'1000000', c_int32_t)
 integer, parameter :: GF_OK = 0, GF_STOP_MAXWELL = 22, GF_STOP_NEGATIVE_AREA = 40
 
 type, bind(C) :: gf_params_t
@@ -27,7 +28,7 @@ type, bind(C) :: gf_params_t
     integer(c_int32_t) :: irheol(GF_MAXPH)
     integer(c_int32_t) :: ix1(GF_MAXINH), ix2(GF_MAXINH), iy1(GF_MAXINH), iy2(GF_MAXINH), igeom(GF_MAXINH)
     integer(c_int32_t) :: flexure_base
-    integer(c_int32_t) :: nelem_serp, pad_i
+    integer(c_int32_t) :: nelem_serp, nopbmax
     real(c_double) :: t_top, t_bot, bot_bc, g, demf, ten_off, tau_heal, v_min, v_max
     real(c_double) :: frac, fracm, strain_inert, dt_scale, xReyn, vbc, rxbo, rzbo
     real(c_double) :: pisos, drosub, fmagma_max, weaken_ratio_plastic, weaken_ratio_viscous, rho_magma
@@ -46,6 +47,7 @@ type, bind(C) :: gf_host_arrays_t
     type(c_ptr) :: aps, visn, e2sr, sshrheat, shrheat, source
     type(c_ptr) :: iphase
     type(c_ptr) :: phase_ratio
+    type(c_ptr) :: nopbou, bcstress
     type(c_ptr) :: coh_scale
 end type
 

@@ -16,7 +16,8 @@ _lib = None
 EXPORTS = [
     "gf_create", "gf_destroy", "gf_last_error", "gf_error_site", "gf_bind_host", "gf_bind_markers", "gf_upload",
     "gf_download", "gf_set_clock", "gf_get_clock", "gf_step", "gf_flac", "gf_fl_therm",
-    "gf_fl_rheol", "gf_fl_node", "gf_fl_move", "gf_rmasses", "gf_dt_mass", "gf_change_phase", "gf_init_areas",
+    "gf_fl_rheol", "gf_fl_node", "gf_fl_move", "gf_rmasses", "gf_dt_mass", "gf_change_phase", "gf_bc_update",
+    "gf_init_areas",
     "gf_init_visc", "gf_therm_stats", "gf_set_therm_passes", "gf_sync",
     "gf_set_timing", "gf_kernel_time", "gf_math_check",
     "gf_surrogate_sample", "gf_surrogate_train", "gf_surrogate_enable", "gf_surrogate_stats",
@@ -56,7 +57,7 @@ def lib():
         L.gf_step.argtypes = [vp, C.c_int32, C.POINTER(abi.GfClock)]
         L.gf_flac.argtypes = [vp, C.POINTER(abi.GfClock)]
         for n in ("gf_fl_therm", "gf_fl_rheol", "gf_fl_node", "gf_fl_move", "gf_rmasses", "gf_dt_mass",
-                  "gf_change_phase"):
+                  "gf_change_phase", "gf_bc_update"):
             getattr(L, n).argtypes = [vp, C.POINTER(abi.GfClock)]
         L.gf_init_areas.argtypes = [vp]
         L.gf_init_visc.argtypes = [vp]
@@ -122,6 +123,8 @@ class Model:
                 mask |= abi.GF_MARKERS
             if self.state.coh_scale is not None:
                 mask |= abi.GF_COH_SCALE
+            if self.params.nystressbc:
+                mask |= abi.GF_STRESSBC
         return mask
 
     def upload(self, mask=abi.GF_ALL):
@@ -132,7 +135,7 @@ class Model:
         self._check(lib().gf_set_clock(self._ctx, C.byref(self.state.clock)))
 
     def download(self, mask=abi.GF_ALL):
-        self._check(lib().gf_download(self._ctx, self._mask(mask) & ~abi.GF_COH_SCALE))
+        self._check(lib().gf_download(self._ctx, self._mask(mask) & ~(abi.GF_COH_SCALE | abi.GF_STRESSBC)))
         self._check(lib().gf_get_clock(self._ctx, C.byref(self.state.clock)))
 
     def step(self, nsteps):

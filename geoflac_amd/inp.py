@@ -274,6 +274,8 @@ def read_params_txt(path):
                     setattr(g, k, int(v))
             elif t[0] == "flags2":
                 g.nelem_serp = int(t[1])
+                if len(t) > 2:
+                    g.nopbmax = int(t[2])
             elif t[0] == "reals":
                 for k, v in zip(reals, t[1:]):
                     if hasattr(g, k):

@@ -49,14 +49,53 @@ def _bc_fun(b, x):
 
 
 def init_bc(p, st):
-    """Velocity boundary conditions (src/init_bc.f90:57-200 velbc part) and vbc (vbcal)."""
+    """Velocity and stress boundary conditions (src/init_bc.f90:57-200 with velbc and
+    stressbc, :226-330) and vbc (vbcal).  Returns (vbc, nopbmax)."""
     nx, nz = p["nx"], p["nz"]
     cord = st.cord
+    nnop = 0
     for b in p["bcs"]:
         side, n1, n2, nbc = b["nofside"], b["nbc1"], b["nbc2"], b["nbc"]
-        if nbc in (2, 20, 40):
-            raise NotImplementedError("stress boundary conditions (nystressbc) are not provided")
         ndbc = n2 - n1 + 1
+        if nbc in (2, 20, 40):
+            ndbc = ndbc - 1
+            if n2 < nz:     # (the reference compares with nz on every side)
+                ndbc = ndbc + 1
+            if side in (1, 3):
+                col = 0 if side == 1 else nx - 1
+                x1, x2 = cord[n1 - 1, col, 1], cord[n2 - 1, col, 1]
+            else:
+                row = nz - 1 if side == 2 else 0
+                x1, x2 = cord[row, n1 - 1, 0], cord[row, n2 - 1, 0]
+            for n in range(1, ndbc + 1):
+                numbp = n + n1 - 1
+                if side == 1:
+                    numbp1 = numbp + 1
+                    x, xn = (cord[numbp - 1, 0, 1] - x1) / (x2 - x1), (cord[numbp1 - 1, 0, 1] - x1) / (x2 - x1)
+                    nodes = (1, 1, numbp, numbp1)
+                elif side == 3:
+                    numbp1 = numbp + 1
+                    x = (cord[numbp - 1, nx - 1, 1] - x1) / (x2 - x1)
+                    xn = (cord[numbp1 - 1, nx - 1, 1] - x1) / (x2 - x1)
+                    nodes = (nx, nx, numbp, numbp1)
+                elif side == 2:
+                    numbp1 = n + 1  # (the reference's second node of a bottom segment)
+                    x = (cord[nz - 1, numbp - 1, 0] - x1) / (x2 - x1)
+                    xn = (cord[nz - 1, numbp1 - 1, 0] - x1) / (x2 - x1)
+                    nodes = (numbp, numbp1, nz, nz)
+                else:
+                    numbp1 = numbp + 1
+                    x, xn = (cord[0, numbp - 1, 0] - x1) / (x2 - x1), (cord[0, numbp1 - 1, 0] - x1) / (x2 - x1)
+                    nodes = (numbp, numbp1, 1, 1)
+                nnop += 1
+                xa = 0.5 * (x + xn)
+                fun = _bc_fun(b, xa)
+                st.nopbou[nnop - 1, :] = nodes
+                if nbc == 20:
+                    st.bcstress[nnop - 1, 0] = fun
+                if nbc == 2:
+                    st.bcstress[nnop - 1, 1] = fun
+            continue
         if side == 1:
             x1, x2 = cord[n1 - 1, 0, 1], cord[n2 - 1, 0, 1]
             nodes = [(n + n1 - 1, 1) for n in range(1, ndbc + 1)]
@@ -93,7 +132,7 @@ def init_bc(p, st):
         vbc = float(np.max(np.abs(st.bc[m])))
     if vbc == 0.0:
         vbc = 1.0e-10
-    return vbc
+    return vbc, nnop
 
 
 def layer_phases(p, st, interface_noise=0.0, seed=0):
@@ -436,7 +475,7 @@ def build_box(p, interface_noise=0.0, seed=0, markers=True, coh_noise=0.0):
                    for k in np.unique(st.iphase)))
     init_temp(p, st)
     pisos = init_stress(p, st)
-    vbc = init_bc(p, st)
+    vbc, nopbmax = init_bc(p, st)
     st.temp0[...] = st.temp
     st.e2sr[...] = 1e-16
     st.se2sr[...] = 1e-16
@@ -444,6 +483,7 @@ def build_box(p, interface_noise=0.0, seed=0, markers=True, coh_noise=0.0):
     c.nloop, c.nsrate, c.time, c.dt, c.dtavg = 0, -1, 0.0, 0.0, 0.0
     c.dt_elastic, c.dt_maxwell = 0.0, 0.0
     gp = inp.to_gf_params(p, vbc=vbc, ivis_present=ivis, pisos=pisos)
+    gp.nopbmax = nopbmax
     return gp, st
 
 

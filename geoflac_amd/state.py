@@ -42,6 +42,10 @@ class State:
         self.iphase = np.ones((ez, ex), dtype=np.int32, order="F")
         self.phase_ratio = np.zeros((self.nphase, ez, ex), **f8)
         self.clock = abi.GfClock()
+        # stress boundary segments (src/init_bc.f90:226-280): node columns/rows, stresses
+        rows = 2 * (ez + ex)
+        self.nopbou = np.zeros((rows, 4), dtype=np.int32, order="F")
+        self.bcstress = np.zeros((rows, 3), **f8)
         # per-element cohesion factor (seeded heterogeneity; no reference counterpart)
         self.coh_scale = None
         # markers (module marker_data, src/marker_data.f90:1-20), None until set
@@ -87,6 +91,8 @@ class State:
             a = getattr(self, name)
             assert a.flags.f_contiguous
             setattr(h, name, a.ctypes.data_as(ptype))
+        h.nopbou = self.nopbou.ctypes.data_as(abi._IP)
+        h.bcstress = self.bcstress.ctypes.data_as(abi._DP)
         if self.coh_scale is not None:
             assert self.coh_scale.flags.f_contiguous and self.coh_scale.dtype == np.float32
             h.coh_scale = self.coh_scale.ctypes.data_as(abi._FP)
@@ -97,6 +103,8 @@ class State:
         for n in self.FIELDS:
             getattr(s, n)[...] = getattr(self, n)
         C.memmove(C.addressof(s.clock), C.addressof(self.clock), C.sizeof(abi.GfClock))
+        s.nopbou[...] = self.nopbou
+        s.bcstress[...] = self.bcstress
         if self.coh_scale is not None:
             s.coh_scale = self.coh_scale.copy(order="F")
         if self.has_markers:
@@ -136,6 +144,9 @@ def load_dump(path, nphase=None):
         arr = np.fromfile(os.path.join(path, name + ext), dtype=dt)
         tgt = getattr(st, name)
         tgt[...] = arr.reshape(tgt.shape, order="F")
+    if os.path.exists(os.path.join(path, "nopbou.i32")):
+        st.nopbou[...] = np.fromfile(os.path.join(path, "nopbou.i32"), dtype=np.int32).reshape(st.nopbou.shape, order="F")
+        st.bcstress[...] = np.fromfile(os.path.join(path, "bcstress.f64")).reshape(st.bcstress.shape, order="F")
     if os.path.exists(os.path.join(path, "mark_a1.f64")):
         rd = lambda n, dt: np.fromfile(os.path.join(path, n + (".f64" if dt is np.float64 else ".i32")), dtype=dt)
         st.set_markers(rd("mark_a1", np.float64), rd("mark_a2", np.float64), rd("mark_ntriag", np.int32),
@@ -156,6 +167,8 @@ def save_npz(state, path, fields=None):
     d = {n: getattr(state, n) for n in fields if n != "mark_phase"}
     if state.has_markers and fields is state.FIELDS:
         d.update({n: getattr(state, n) for n in MARKERS})
+    if fields is state.FIELDS:
+        d["nopbou"], d["bcstress"] = state.nopbou, state.bcstress
     elif "mark_phase" in fields:
         mp = state.mark_phase if state.has_markers else getattr(state, "mark_phase_only", None)
         if mp is not None:
@@ -174,6 +187,9 @@ def load_npz(path):
     for n in State.FIELDS:
         if n in z.files:
             getattr(st, n)[...] = z[n]
+    if "nopbou" in z.files:
+        st.nopbou[...] = z["nopbou"]
+        st.bcstress[...] = z["bcstress"]
     if "mark_a1" in z.files:
         st.set_markers(*(z[n] for n in MARKERS))
     elif "mark_phase" in z.files:

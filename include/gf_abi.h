@@ -63,7 +63,7 @@ typedef struct gf_params {
     int32_t ix1[GF_MAXINH], ix2[GF_MAXINH], iy1[GF_MAXINH], iy2[GF_MAXINH], igeom[GF_MAXINH];
     int32_t flexure_base;                   /* 1 when side 2 carries nbc == 200 (src/fl_flexure.f90:15-24) */
     int32_t nelem_serp;                     /* elements below searched by the serpentinite rule (src/change_phase.f90:139) */
-    int32_t pad_i;
+    int32_t nopbmax;                        /* boundary segments carrying an applied stress (src/init_bc.f90:207) */
     double t_top, t_bot, bot_bc, g, demf, ten_off, tau_heal, v_min, v_max;
     double frac, fracm, strain_inert, dt_scale, xReyn, vbc, rxbo, rzbo;
     double pisos, drosub, fmagma_max, weaken_ratio_plastic, weaken_ratio_viscous, rho_magma;
@@ -84,6 +84,12 @@ typedef struct gf_host_arrays {
     double *aps, *visn, *e2sr, *sshrheat, *shrheat, *source;
     int32_t *iphase;
     double *phase_ratio;
+    /* stress boundary conditions (init_bc/stressbc, src/init_bc.f90:226-280):
+     * nopbou(rows,4) the segment's node columns ii1, ii2 and rows jj1, jj2, and
+     * bcstress(rows,3) its normal and shear stress, rows = 2((nz-1)+(nx-1));
+     * segments 1..nopbmax are read by bc_update (src/bc_update.f90:96-140) */
+    int32_t *nopbou;
+    double *bcstress;
     /* No reference counterpart: a per-element factor on the mixed cohesion of pre_plast
      * (src/rh_plastic.f90:231-291), for seeded heterogeneity; NULL or never uploaded
      * means 1 everywhere. */
@@ -100,7 +106,8 @@ enum {
     GF_IPHASE = 1u << 20, GF_PHASE_RATIO = 1u << 21,
     GF_ALL = (1u << 22) - 1u,   /* every array of module arrays above */
     GF_MARKERS = 1u << 22,      /* upload: every marker array; download: mark_phase */
-    GF_COH_SCALE = 1u << 23     /* upload only */
+    GF_COH_SCALE = 1u << 23,    /* upload only */
+    GF_STRESSBC = 1u << 24      /* upload only: nopbou and bcstress */
 };
 
 /* Markers of module marker_data (src/marker_data.f90:1-20), read and written by
@@ -177,6 +184,7 @@ int gf_fl_move(gf_ctx *ctx, gf_clock *clock);    /* src/fl_move.f90:4-132  */
 int gf_rmasses(gf_ctx *ctx, gf_clock *clock);    /* src/rmasses.f90:4-63   */
 int gf_dt_mass(gf_ctx *ctx, gf_clock *clock);    /* src/dt_mass.f90:16-160 */
 int gf_change_phase(gf_ctx *ctx, gf_clock *clock); /* src/change_phase.f90:1-260 (itype_melting=0 part) */
+int gf_bc_update(gf_ctx *ctx, gf_clock *clock);    /* src/bc_update.f90:6-143 (nystressbc = 1) */
 int gf_init_areas(gf_ctx *ctx);                  /* src/init_areas.f90:7-55 */
 int gf_init_visc(gf_ctx *ctx);                   /* src/init_visc.f90:1-18  */
 

@@ -1280,6 +1280,65 @@ int gfo_change_phase(const gf_params *p, gf_host_arrays *s, gf_clock *c)
     return 0;
 }
 
+/* ---------------------------------------------------------------------------
+ * bc_update  src/bc_update.f90:6-143: hydrostatic side pressure (nydrsides = 1)
+ * and applied boundary stresses, accumulated into force before fl_node
+ * ------------------------------------------------------------------------- */
+int gfo_bc_update(const gf_params *p, gf_host_arrays *s, gf_clock *c)
+{
+    (void)c;
+    for (long q = 0; q < 2 * NN; q++) s->force[q] = 0.0;
+    if (p->nydrsides == 1) {
+        for (int side = 0; side < 2; side++) {
+            const int ic = side == 0 ? 1 : NX - 1, in = side == 0 ? 1 : NX;  /* element column, node column */
+            double rogh = 0.0;
+            for (int j = 1; j <= NZ - 1; j++) {
+                int iph = IPHASE(j, ic);
+                double fr = p->frac;
+                double densT = eff_dens_frac(p, s, j, ic, &fr);
+                double dh1 = CORD(j, ic, 2) - CORD(j + 1, ic, 2);
+                double dh2 = CORD(j, ic + 1, 2) - CORD(j + 1, ic + 1, 2);
+                double dh = 0.5 * (dh1 + dh2);
+                double dPT = densT * p->g * dh;
+                double dP = dPT * (1 - PH(beta, iph) * rogh) / (1 + PH(beta, iph) / 2 * dPT);
+                double press = rogh + 0.5 * dP;
+                double dlx = CORD(j + 1, in, 1) - CORD(j, in, 1);
+                double dly = CORD(j + 1, in, 2) - CORD(j, in, 2);
+                double press_norm = -press;
+                if (side == 0) {
+                    FORCE(j, in, 1) = FORCE(j, in, 1) + 0.5 * press_norm * dly;
+                    FORCE(j + 1, in, 1) = FORCE(j + 1, in, 1) + 0.5 * press_norm * dly;
+                    FORCE(j, in, 2) = FORCE(j, in, 2) - 0.5 * press_norm * dlx;
+                    FORCE(j + 1, in, 2) = FORCE(j + 1, in, 2) - 0.5 * press_norm * dlx;
+                } else {
+                    FORCE(j, in, 1) = FORCE(j, in, 1) - 0.5 * press_norm * dly;
+                    FORCE(j + 1, in, 1) = FORCE(j + 1, in, 1) - 0.5 * press_norm * dly;
+                    FORCE(j, in, 2) = FORCE(j, in, 2) + 0.5 * press_norm * dlx;
+                    FORCE(j + 1, in, 2) = FORCE(j + 1, in, 2) + 0.5 * press_norm * dlx;
+                }
+                rogh = rogh + dP;
+            }
+        }
+    }
+    const long rows = 2L * ((NZ - 1) + (NX - 1));
+    for (int q = 0; q < p->nopbmax; q++) {
+        const int ii1 = s->nopbou[q], ii2 = s->nopbou[q + rows], jj1 = s->nopbou[q + 2 * rows],
+                  jj2 = s->nopbou[q + 3 * rows];
+        double dlx = CORD(jj2, ii2, 1) - CORD(jj1, ii1, 1);
+        double dly = CORD(jj2, ii2, 2) - CORD(jj1, ii1, 2);
+        double sn = s->bcstress[q], ss = s->bcstress[q + rows];
+        FORCE(jj1, ii1, 1) = FORCE(jj1, ii1, 1) - 0.5 * sn * dly;
+        FORCE(jj2, ii2, 1) = FORCE(jj2, ii2, 1) - 0.5 * sn * dly;
+        FORCE(jj1, ii1, 2) = FORCE(jj1, ii1, 2) + 0.5 * sn * dlx;
+        FORCE(jj2, ii2, 2) = FORCE(jj2, ii2, 2) + 0.5 * sn * dlx;
+        FORCE(jj1, ii1, 1) = FORCE(jj1, ii1, 1) + 0.5 * ss * dlx;
+        FORCE(jj2, ii2, 1) = FORCE(jj2, ii2, 1) + 0.5 * ss * dlx;
+        FORCE(jj1, ii1, 2) = FORCE(jj1, ii1, 2) + 0.5 * ss * dly;
+        FORCE(jj2, ii2, 2) = FORCE(jj2, ii2, 2) + 0.5 * ss * dly;
+    }
+    return 0;
+}
+
 /* src/flac.f90:18-83 */
 int gfo_flac(const gf_params *p, gf_host_arrays *s, gf_clock *c)
 {
@@ -1288,6 +1347,7 @@ int gfo_flac(const gf_params *p, gf_host_arrays *s, gf_clock *c)
     if (p->itherm != 2) {
         if (c->nloop % p->ifreq_rmasses == 0 && (st = gfo_change_phase(p, s, c))) return st;
         if ((st = gfo_fl_rheol(p, s, c))) return st;
+        if (p->nystressbc == 1 && (st = gfo_bc_update(p, s, c))) return st;
         if ((st = gfo_fl_node(p, s, c))) return st;
         if ((st = gfo_fl_move(p, s, c))) return st;
         if (c->nloop % p->ifreq_rmasses == 0 && (st = gfo_rmasses(p, s, c))) return st;

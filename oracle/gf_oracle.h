@@ -25,6 +25,7 @@ int gfo_dt_mass(const gf_params *p, gf_host_arrays *s, gf_clock *c);
 /* change_phase runs on the markers bound here (NULL: none, it does nothing) */
 void gfo_bind_markers(const gf_marker_arrays *m);
 int gfo_change_phase(const gf_params *p, gf_host_arrays *s, gf_clock *c);
+int gfo_bc_update(const gf_params *p, gf_host_arrays *s, gf_clock *c);
 int gfo_init_areas(const gf_params *p, gf_host_arrays *s);
 int gfo_init_visc(const gf_params *p, gf_host_arrays *s);
 /* one call of subroutine flac (src/flac.f90:5-91) */

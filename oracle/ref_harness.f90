@@ -130,6 +130,9 @@ subroutine dump_state(base, step)
     call wr4(dir, 'mark_phase', int(mark_phase(1:nmarkers)), nmarkers)
     call wr4(dir, 'nmark_elem', nmark_elem, ne)
     if (step == 0) then
+        ! stress boundary segments (init_bc, src/init_bc.f90:226-280)
+        call wr4(dir, 'nopbou', nopbou, size(nopbou))
+        call wr8(dir, 'bcstress', bcstress, size(bcstress))
         call wr8(dir, 'mark_a1', mark_a1, nmarkers)
         call wr8(dir, 'mark_a2', mark_a2, nmarkers)
         call wr4(dir, 'mark_ntriag', mark_ntriag, nmarkers)
@@ -161,7 +164,7 @@ subroutine dump_params(base)
     write(u, '(A,99(1X,ES25.17E3))') 'reals ', t_top, t_bot, bot_bc, g, demf, ten_off, tau_heal, v_min, &
         v_max, frac, fracm, strain_inert, dt_scale, xReyn, vbc, rxbo, rzbo, pisos, drosub, &
         fmagma_max, weaken_ratio_plastic, weaken_ratio_viscous, rho_magma, extra_pres
-    write(u, '(A,99I12)') 'flags2 ', nelem_serp
+    write(u, '(A,99I12)') 'flags2 ', nelem_serp, nopbmax
     do k = 1, nphase
         write(u, '(A,I4,I4,99(1X,ES25.17E3))') 'phase ', k, irheol(k), den(k), alfa(k), beta(k), pln(k), &
             acoef(k), eactiv(k), vactiv(k), rl(k), rm(k), plstrain1(k), plstrain2(k), fric1(k), &

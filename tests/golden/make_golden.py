@@ -41,6 +41,11 @@ CASES = {
     "hetero80x40": dict(inp="hetero80x40.inp", steps=[1, 10, 100, 1000]),
     # 60x30 with marker phase changes (sediment, oceanic crust, serpentinite)
     "phases60x30": dict(inp="phases60x30.inp", steps=[1, 11, 21, 100, 1000]),
+    # stress boundary conditions (bc_update): hydrostatic side walls, a normal stress on
+    # part of the surface, a Winkler base and the sea-water load
+    "bcs60x20": dict(inp="bcs60x20.inp", steps=[1, 10, 100, 1000]),
+    # dt_elastic fixed by dt_scale (idt_scale 1) and the prestress freeze (i_prestress 1)
+    "idt1_60x20": dict(inp="idt1_60x20.inp", steps=[1, 10, 100]),
 }
 
 

@@ -54,7 +54,7 @@ def test_fortran_interface_mirrors_c_structs():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case,dumps", [("box60x20", (10, 100)), ("phases60x30", (11, 100))])
+@pytest.mark.parametrize("case,dumps", [("box60x20", (10, 100)), ("phases60x30", (11, 100)), ("bcs60x20", (10, 100))])
 def test_fortran_dropin_matches_reference(tmp_path, case, dumps):
     """box60x20, and phases60x30 whose markers change phase on the device (the marker
     phases, phase ratios and iphase come back to the Fortran arrays)."""

@@ -72,12 +72,12 @@ def compare(st, ref, fields=None, tol=TOL):
 
 
 SUB_STATES = [("box60x20", 10), ("box60x20", 1000), ("hetero80x40", 100), ("layers400x100", 10),
-              ("phases60x30", 11)]
+              ("phases60x30", 11), ("bcs60x20", 100), ("idt1_60x20", 10)]
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("sub", ["fl_therm", "change_phase", "fl_rheol", "fl_node", "fl_move", "rmasses",
-                                 "dt_mass"])
+@pytest.mark.parametrize("sub", ["fl_therm", "change_phase", "fl_rheol", "bc_update", "fl_node", "fl_move",
+                                 "rmasses", "dt_mass"])
 @pytest.mark.parametrize("case,step", SUB_STATES)
 def test_subroutine_bitwise_equal_to_oracle(sub, case, step):
     """One reference subroutine on the device equals the oracle (with the device's
@@ -239,19 +239,21 @@ def test_interleaved_calls_bitwise_equal_to_oracle():
 REF_RUNS = [("box60x20", n) for n in (1, 10, 100, 1000)] + \
     [("layers400x100", n) for n in (1, 10, 100)] + \
     [("hetero80x40", n) for n in (1, 10, 100, 1000)] + \
-    [("phases60x30", n) for n in (1, 11, 21, 100, 1000)]
+    [("phases60x30", n) for n in (1, 11, 21, 100, 1000)] + \
+    [("bcs60x20", n) for n in (1, 10, 100, 1000)] + \
+    [("idt1_60x20", n) for n in (1, 10, 100)]
 
 
 # Runs long enough for the reference's own libm to matter: glibc's sin/exp/pow are not
 # correctly rounded for about 0.1% of arguments, and once plastic yielding starts the
 # step amplifies a last-bit difference (the plastic/viscous branch choice is a
 # discontinuous function of the state).  Restated on the CPU with glibc and with
-# correctly rounded functions, 1000 steps differ by 2.7e-10 (hetero80x40) and 2.9e-8
-# (phases60x30) in velocity rel-L2 (tests/test_oracle.py::test_libm_floor_of_long_runs).
+# correctly rounded functions, 1000 steps differ by 2.7e-10 (hetero80x40), 2.9e-8
+# (phases60x30) and 1.2e-10 (bcs60x20) in velocity rel-L2 (tests/test_oracle.py::test_libm_floor_of_long_runs).
 # There the device (correctly rounded, like the oracle it is checked against) must
 # equal the correctly rounded oracle bit for bit and lie exactly as far from the
 # reference as that oracle does.
-LIBM_BOUND_RUNS = {("hetero80x40", 1000), ("phases60x30", 1000)}
+LIBM_BOUND_RUNS = {("hetero80x40", 1000), ("phases60x30", 1000), ("bcs60x20", 1000)}
 
 
 @pytest.mark.gpu

@@ -37,7 +37,9 @@ def test_params_layout_matches_header():
 GOLDEN_RUNS = [("box60x20", n) for n in (1, 10, 100, 1000, 10000)] + \
     [("layers400x100", n) for n in (1, 10, 100)] + \
     [("hetero80x40", n) for n in (1, 10, 100, 1000)] + \
-    [("phases60x30", n) for n in (1, 11, 21, 100, 1000)]
+    [("phases60x30", n) for n in (1, 11, 21, 100, 1000)] + \
+    [("bcs60x20", n) for n in (1, 10, 100, 1000)] + \
+    [("idt1_60x20", n) for n in (1, 10, 100)]
 
 
 @pytest.mark.parametrize("case,nsteps", GOLDEN_RUNS)
@@ -66,13 +68,13 @@ def test_oracle_bitwise_equal_to_reference(case, nsteps):
 
 
 @pytest.mark.parametrize("case,nsteps,floor", [("hetero80x40", 1000, 1e-10), ("phases60x30", 1000, 1e-8),
-                                                ("hetero80x40", 100, 1e-20)])
+                                                ("bcs60x20", 1000, 1e-10), ("hetero80x40", 100, 1e-20)])
 def test_libm_floor_of_long_runs(case, nsteps, floor):
     """The same restatement with glibc's functions (bitwise the reference) and with
     correctly rounded ones (the device's) drifts apart once plastic yielding starts:
     a last-bit difference in one sin/exp/pow grows with the steps.  Over 100 steps the
-    two agree to round-off; over 1000 steps of the mixed-rheology and phase-change
-    boxes the velocity differs by more than the 1e-10 drop-in bar, which therefore
+    two agree to round-off; over 1000 steps of the mixed-rheology, phase-change and
+    stress-boundary boxes the velocity differs by more than the 1e-10 drop-in bar, which therefore
     applies to short runs (the device is checked bitwise against the correctly rounded
     oracle over long ones)."""
     p = ol.golden_params(case)
