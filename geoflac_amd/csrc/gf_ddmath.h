@@ -1,24 +1,37 @@
 // gf_ddmath.h -- correctly rounded sin, cos, tan, exp, log and pow for the FP64
 // drop-in path, evaluated in double-double arithmetic on the device.
 //
-// The reference's host libm returns the correctly rounded value for every argument
-// the step produces (checked on the CPU against extended-precision evaluations), and
-// the model is chaotic in the last bit (plastic yield and branch choices), so the
-// device must round the same way.  Each function computes the value with a relative
-// error near 2^-100 and rounds it once to double; a misrounding needs the exact value
-// within that distance of a rounding boundary.  Methods (textbook, no table of any
-// library): error-free transformations (Knuth two-sum, FMA two-product), Taylor series
-// with coefficients 1/n! held as exact double-double pairs (generated from rationals),
-// exp by ln2 reduction, 2^-10 scaling and repeated squaring of e^s - 1, log by one
-// Newton step on exp, pow as exp(y log x).
+// The model is chaotic in the last bit (plastic yield and branch choices), so the
+// device and the oracle it is checked against must round these functions the same
+// way.  Neither glibc's nor the device library's functions are correctly rounded for
+// every argument (glibc's sin/exp/pow differ from the correctly rounded value for
+// about 0.1% of the arguments a long run produces), so both sides use the correctly
+// rounded value, computed here; over short runs that value and the reference's glibc
+// agree to round-off.
+//
+// Each function first takes a fast path: a table-driven evaluation with a relative
+// error below 2^-72, accepted when the rounding test (cr_decided) shows that no value
+// within the error bound rounds differently -- the result is then the correctly
+// rounded one.  Otherwise (about one call in 2^15) it takes the slow path: the value
+// with a relative error near 2^-100, rounded once.  Methods (textbook, no table of any
+// library): error-free transformations (Knuth two-sum, FMA two-product); slow path:
+// Taylor series with coefficients 1/n! held as exact double-double pairs, exp by ln2
+// reduction, 2^-10 scaling and repeated squaring of e^s - 1; fast path: sin/cos by
+// the addition theorem from sin(k/32), cos(k/32), exp by 2^(j/32) times a short
+// series (Tang's reduction), both tables computed from the definitions by
+// gen_ddtables.py; log by one Newton step on exp from the library's log; pow as
+// exp(y log x).
 #pragma once
 #ifdef __HIPCC__
 #include <hip/hip_runtime.h>
 #define DD_DEV static __host__ __device__ __forceinline__
+#define DD_SLOW static __host__ __device__ __noinline__  // one out-of-line copy per kernel
 #else  // host-only build of the same code (the CPU oracle variant and its unit test)
 #include <math.h>
 #define DD_DEV static inline
+#define DD_SLOW static
 #endif
+#include "gf_ddtables.h"
 
 struct ddv {
     double hi, lo;
@@ -61,8 +74,12 @@ struct ddv {
 
 #ifdef __HIPCC__
 __device__ static const double kInvFactD[32][2] = {GF_KINVFACT_DATA};
+__device__ static const double kSinCosD[51][4] = {GF_SINCOS_TAB_DATA};
+__device__ static const double kExp2D[32][2] = {GF_EXP2_TAB_DATA};
 #endif
 static const double kInvFactH[32][2] = {GF_KINVFACT_DATA};
+static const double kSinCosH[51][4] = {GF_SINCOS_TAB_DATA};
+static const double kExp2H[32][2] = {GF_EXP2_TAB_DATA};
 
 DD_DEV ddv dd_two_sum(double a, double b)
 {
@@ -183,7 +200,7 @@ DD_DEV void dd_sincos(double x, ddv &s, ddv &c)
 
 DD_DEV bool dd_trig_range(double x) { return x >= 0.0 && x <= 1.5707963267948966; }
 
-DD_DEV double cr_sin(double x)
+DD_SLOW double cr_sin_slow(double x)
 {
     if (!dd_trig_range(x)) return sin(x);
     ddv s, c;
@@ -191,7 +208,7 @@ DD_DEV double cr_sin(double x)
     return s.hi + s.lo;
 }
 
-DD_DEV double cr_tan(double x)
+DD_SLOW double cr_tan_slow(double x)
 {
     if (!dd_trig_range(x) || x == 1.5707963267948966) return tan(x);
     ddv s, c;
@@ -218,7 +235,7 @@ DD_DEV ddv dd_exp(ddv x)
     return {ldexp(e.hi, ki), ldexp(e.lo, ki)};
 }
 
-DD_DEV double cr_exp(double x)
+DD_SLOW double cr_exp_slow(double x)
 {
     if (!(fabs(x) < 700.0)) return exp(x);
     ddv e = dd_exp({x, 0.0});
@@ -235,7 +252,7 @@ DD_DEV ddv dd_log(double x)
     return dd_add_d(t, y0);
 }
 
-DD_DEV double cr_pow(double x, double y)
+DD_SLOW double cr_pow_slow(double x, double y)
 {
     if (!(x > 0.0) || isinf(x) || x < 0x1p-1000 || y == 0.0) return pow(x, y);
     ddv l = dd_log(x);
@@ -243,4 +260,183 @@ DD_DEV double cr_pow(double x, double y)
     if (!(fabs(a.hi) < 700.0)) return pow(x, y);
     ddv e = dd_exp(a);
     return e.hi + e.lo;
+}
+
+// ------------------------------------------------------------------ fast paths
+DD_DEV long long dd_bits(double x)
+{
+    long long b;
+    __builtin_memcpy(&b, &x, sizeof b);
+    return b;
+}
+DD_DEV double dd_from_bits(long long b)
+{
+    double x;
+    __builtin_memcpy(&x, &b, sizeof x);
+    return x;
+}
+
+// Rounding test.  v = hi + lo with hi = RN(hi + lo) approximates a value u with
+// |u - v| <= rel |hi|.  True when every such u rounds to hi: the remainder lo, widened
+// by the error bound, stays inside the half-ulp interval around hi on its side.
+// Values outside [2^-1000, 2^1000] are left to the slow path.
+DD_DEV bool cr_decided(ddv v, double rel)
+{
+    const double a = fabs(v.hi);
+    if (!(a >= 0x1p-1000 && a <= 0x1p1000)) return false;
+    const long long b = dd_bits(a);
+    const double up = dd_from_bits(b + 1) - a;  // exact: neighbouring doubles
+    const double dn = a - dd_from_bits(b - 1);
+    const double lo = v.hi > 0.0 ? v.lo : -v.lo;  // the remainder in the direction of |hi|
+    const double err = rel * a;
+    return lo >= 0.0 ? lo + err < 0.5 * up : err - lo < 0.5 * dn;
+}
+
+// sin x and cos x for 0 <= x <= 1.2, relative errors below 2^-78: x = k/32 + r,
+// |r| <= 1/64 (exact: Sterbenz), sin x = S cos r + C sin r, cos x = C cos r - S sin r
+// with S, C = sin, cos(k/32) from the table; sin r = r + r q and cos r = 1 + m with the
+// r^2 terms in double-double and the rest (below 2.5e-9) in double.
+DD_DEV void dd_sincos_fast(double x, ddv &s, ddv &c)
+{
+    const int k = (int)rint(x * 32.0);
+    const double r = x - (double)k * 0.03125;
+    const ddv r2 = dd_two_prod(r, r);
+    const double z = r2.hi;
+    const ddv m6 = {-0x1.5555555555555p-3, -0x1.5555555555555p-57};  // -1/6
+    ddv q = dd_mul(r2, m6);
+    q = dd_add_d(q, z * z * (1.0 / 120.0 + z * (-1.0 / 5040.0 + z * (1.0 / 362880.0))));
+    const ddv sr = dd_add_d(dd_mul_d(q, r), r);
+    const ddv cm = dd_add_d({-0.5 * r2.hi, -0.5 * r2.lo},
+                            z * z * (1.0 / 24.0 + z * (-1.0 / 720.0 + z * (1.0 / 40320.0))));
+#if defined(__HIP_DEVICE_COMPILE__)
+    const ddv S = {kSinCosD[k][0], kSinCosD[k][1]}, C = {kSinCosD[k][2], kSinCosD[k][3]};
+#else
+    const ddv S = {kSinCosH[k][0], kSinCosH[k][1]}, C = {kSinCosH[k][2], kSinCosH[k][3]};
+#endif
+    const ddv ssr = dd_mul(S, sr);
+    s = dd_add(S, dd_add(dd_mul(S, cm), dd_mul(C, sr)));
+    c = dd_add(C, dd_add(dd_mul(C, cm), {-ssr.hi, -ssr.lo}));
+}
+
+// e^x for a double-double x with |x.hi| < 700, relative error below 2^-72 (Tang):
+// x = k ln2/32 + r, |r| <= ln2/64 (ln2/32 as L1 + L2 + L3, k L1 exact, x.hi - k L1
+// exact), e^x = 2^m 2^(j/32) e^r with k = 32 m + j; e^r - 1 = h + h^2/2 + tail + l e^h
+// for r = h + l, the h^2 term in double-double, the tail h^3/6 .. h^9/9! in double.
+DD_DEV ddv dd_exp_fast(ddv x)
+{
+    const double kd = rint(x.hi * GF_INV_LN2_32);
+    const int k = (int)kd;
+    const double r0 = x.hi - kd * GF_LN2_32_1;
+    const ddv p2 = dd_two_prod(kd, GF_LN2_32_2);
+    ddv r = dd_add({r0, 0.0}, {-p2.hi, -p2.lo});
+    r = dd_add_d(r, x.lo - kd * GF_LN2_32_3);
+    const double h = r.hi, l = r.lo;
+    const ddv h2 = dd_two_prod(h, h);
+    const double tail =
+        h * h2.hi *
+        (1.0 / 6.0 +
+         h * (1.0 / 24.0 + h * (1.0 / 120.0 + h * (1.0 / 720.0 + h * (1.0 / 5040.0 + h * (1.0 / 40320.0 + h * (1.0 / 362880.0)))))));
+    ddv p = dd_add_d({0.5 * h2.hi, 0.5 * h2.lo}, tail + (l + l * (h + 0.5 * h * h)));
+    p = dd_add_d(p, h);
+    const int j = k & 31, m = (k - j) / 32;
+#if defined(__HIP_DEVICE_COMPILE__)
+    const ddv T = {kExp2D[j][0], kExp2D[j][1]};
+#else
+    const ddv T = {kExp2H[j][0], kExp2H[j][1]};
+#endif
+    const ddv e = dd_add(T, dd_mul(T, p));
+    return {ldexp(e.hi, m), ldexp(e.lo, m)};
+}
+
+// Fast paths: true (and the correctly rounded value in v) when the rounding test
+// accepts the fast value, whose error bounds it takes with a margin of 2^4 or more.
+DD_DEV bool cr_sin_fast(double x, double &v)
+{
+    if (!(x > 0.0 && x <= 1.2)) return false;
+    ddv s, c;
+    dd_sincos_fast(x, s, c);
+    v = s.hi;
+    return cr_decided(s, 0x1p-72);
+}
+
+DD_DEV bool cr_tan_fast(double x, double &v)
+{
+    if (!(x > 0.0 && x <= 1.2)) return false;
+    ddv s, c;
+    dd_sincos_fast(x, s, c);
+    const ddv t = dd_div(s, c);
+    v = t.hi;
+    return cr_decided(t, 0x1p-72);
+}
+
+DD_DEV bool cr_exp_fast(double x, double &v)
+{
+    if (!(fabs(x) < 700.0)) return false;
+    const ddv e = dd_exp_fast({x, 0.0});
+    v = e.hi;
+    return cr_decided(e, 0x1p-69);
+}
+
+// pow(x, y) = e^(y log x): log x = y0 + (x e^-y0 - 1) to 2^-72 absolute (one Newton
+// step from the library's log y0), then the fast exp; relative error below
+// (1 + |y|) 2^-72
+DD_DEV bool cr_pow_fast(double x, double y, double &v)
+{
+    if (!(x > 0.0) || isinf(x) || x < 0x1p-1000 || y == 0.0) return false;
+    const double y0 = log(x);
+    if (!(fabs(y0) < 700.0)) return false;
+    const ddv e1 = dd_exp_fast({-y0, 0.0});
+    const ddv l = dd_add_d(dd_add_d(dd_mul_d(e1, x), -1.0), y0);
+    const ddv a = dd_mul_d(l, y);
+    if (!(fabs(a.hi) < 699.0)) return false;
+    const ddv e = dd_exp_fast(a);
+    v = e.hi;
+    return cr_decided(e, 0x1p-68 * (1.0 + fabs(y)));
+}
+
+// The functions of the FP64 path (the slow path handles everything the fast one
+// refuses, including the arguments outside its ranges)
+DD_DEV double cr_sin(double x)
+{
+    double v;
+    if (x == 0.0) return 0.0;  // (+0 for -0 too, as the slow path)
+    return cr_sin_fast(x, v) ? v : cr_sin_slow(x);
+}
+
+DD_DEV double cr_tan(double x)
+{
+    double v;
+    if (x == 0.0) return 0.0;
+    return cr_tan_fast(x, v) ? v : cr_tan_slow(x);
+}
+
+// sin x and (want_tan) tan x of one argument from one table evaluation
+DD_DEV void cr_sin_tan(double x, double &sv, double &tv, bool want_tan)
+{
+    if (x > 0.0 && x <= 1.2) {
+        ddv s, c;
+        dd_sincos_fast(x, s, c);
+        sv = cr_decided(s, 0x1p-72) ? s.hi : cr_sin_slow(x);
+        if (want_tan) {
+            const ddv t = dd_div(s, c);
+            tv = cr_decided(t, 0x1p-72) ? t.hi : cr_tan_slow(x);
+        }
+        return;
+    }
+    sv = cr_sin(x);
+    if (want_tan) tv = cr_tan(x);
+}
+
+DD_DEV double cr_exp(double x)
+{
+    double v;
+    if (!(fabs(x) < 700.0)) return exp(x);
+    return cr_exp_fast(x, v) ? v : cr_exp_slow(x);
+}
+
+DD_DEV double cr_pow(double x, double y)
+{
+    double v;
+    if (!(x > 0.0) || isinf(x) || x < 0x1p-1000 || y == 0.0) return pow(x, y);
+    return cr_pow_fast(x, y, v) ? v : cr_pow_slow(x, y);
 }

@@ -1256,20 +1256,7 @@ GF_DEV void rheol_quad(const GfDevParams *__restrict__ P, GfDevClock *ck, const 
             const double degrad = 3.14159265358979323846 / 180.0;
             const double xarg = (k == 1 ? psi : phi) * degrad;
             double sv, tv = 0.0;
-            if (dd_trig_range(xarg)) {
-                ddv s_, c_;
-                dd_sincos(xarg, s_, c_);
-                sv = s_.hi + s_.lo;
-                if (k == 0 && phi != 0.0 && xarg != 1.5707963267948966) {
-                    ddv t_ = dd_div(s_, c_);
-                    tv = t_.hi + t_.lo;
-                } else if (k == 0 && phi != 0.0) {
-                    tv = tan(xarg);
-                }
-            } else {
-                sv = sin(xarg);
-                if (k == 0 && phi != 0.0) tv = tan(xarg);
-            }
+            cr_sin_tan(xarg, sv, tv, k == 0 && phi != 0.0);
             sphi = QL(sv, 0);
             spsi = QL(sv, 1);
             tphi = QL(tv, 0);
@@ -1405,9 +1392,9 @@ static __device__ __noinline__ SinSinTan cr_sin_sin_tan(double phi, double psi)
 {
     const double degrad = 3.14159265358979323846 / 180.0;
     SinSinTan r;
-    r.sphi = cr_sin(phi * degrad);
-    r.spsi = cr_sin(psi * degrad);
-    r.tphi = phi == 0.0 ? 0.0 : cr_tan(phi * degrad);
+    r.tphi = 0.0;
+    cr_sin_tan(phi * degrad, r.sphi, r.tphi, phi != 0.0);
+    r.spsi = psi == phi ? r.sphi : cr_sin(psi * degrad);
     return r;
 }
 

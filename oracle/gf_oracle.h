@@ -35,6 +35,7 @@ int gfo_step(const gf_params *p, gf_host_arrays *s, gf_clock *c, int nsteps);
 void gfo_abi_sizes(long out[4]);
 /* the transcendental functions in use (fn: 0 sin, 1 tan, 2 exp, 3 pow) */
 double gfo_math(int fn, double x, double y);
+long gfo_math_fast(int fn, const double *x, const double *y, double *out, long n, long *nslow);
 /* closed-form pieces, exposed for known-answer tests */
 double gfo_eff_visc(const gf_params *p, const gf_host_arrays *s, int j, int i);
 double gfo_eff_dens(const gf_params *p, const gf_host_arrays *s, int j, int i);

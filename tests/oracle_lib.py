@@ -22,6 +22,8 @@ def lib(cr=False):
         L.gfo_eff_dens.restype = C.c_double
         L.gfo_math.restype = C.c_double
         L.gfo_math.argtypes = [C.c_int, C.c_double, C.c_double]
+        L.gfo_math_fast.restype = C.c_long
+        L.gfo_math_fast.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.POINTER(C.c_long)]
         _LIBS[cr] = L
     return _LIBS[cr]
 
