@@ -6,6 +6,7 @@
 // against the cited source line; every scatter of the reference is a gather here
 // (no atomics on field data), in the reference's summation order.
 #pragma once
+#include <type_traits>
 #include "gf_device.h"
 
 struct GfDevArrays {
@@ -321,6 +322,18 @@ GF_DEV double therm_eval(const ThermCoef &c, double xu, double xl, double xd, do
     if (c.mask & 8) rhs = rhs + c.P[3] - c.Q[3] * ((xl * c.k41 + xd * c.k42) + c.c43);
     if (c.mask & 16) rhs = rhs + c.P[4] - c.Q[4] * ((c.c51 + xd * c.k52) + c.c53);
     if (c.mask & 32) rhs = rhs + c.P[5] - c.qs6;
+    return c.told + div_fma(rhs * dt, c.area_n, c.rarea);
+}
+// the same for a node with all six triangles (mask 63), without the selects
+GF_DEV double therm_eval_full(const ThermCoef &c, double xu, double xl, double xd, double dt)
+{
+    double rhs = 0.0;
+    rhs = rhs + c.P[0] - c.Q[0] * ((xu * c.k11 + xl * c.k12) + c.c13);
+    rhs = rhs + c.P[1] - c.Q[1] * ((xu * c.k21 + c.c22) + c.c23);
+    rhs = rhs + c.P[2] - c.qs3;
+    rhs = rhs + c.P[3] - c.Q[3] * ((xl * c.k41 + xd * c.k42) + c.c43);
+    rhs = rhs + c.P[4] - c.Q[4] * ((c.c51 + xd * c.k52) + c.c53);
+    rhs = rhs + c.P[5] - c.qs6;
     return c.told + div_fma(rhs * dt, c.area_n, c.rarea);
 }
 
@@ -759,7 +772,8 @@ GF_DEV void therm_tile_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, c
     auto elem_store = [&](int e, int round) {
         if (round == 0) {
             const int ph = R.iph[e];
-            const double dm = (R.src[e] + R.sh[e] / sPh[0][ph - 1]) / sPh[1][ph - 1];
+            // (without shear heating the dissipation term is +0: src + 0 = src, -0 included)
+            const double dm = (DIS ? R.src[e] + R.sh[e] / sPh[0][ph - 1] : R.src[e] + 0.0) / sPh[1][ph - 1];
             const double df = sPh[2][ph - 1];
             for (int tri = 0; tri < 2; tri++) {
                 const double inv_area = 1.0 / (tri == 0 ? R.a0[e] : R.a1[e]);
@@ -870,33 +884,47 @@ GF_DEV void therm_tile_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, c
     const int me = nb, up = nb - 1, lf = nb - TH_LJ, dl = nb - TH_LJ + 1;
     int cur = 0, nsweep = 0;
     bool still = false;
-    for (; nsweep < TH_SWEEP0; nsweep++) {
-        if (valid) sX[cur ^ 1][me] = therm_eval(c, sX[cur][up], sX[cur][lf], sX[cur][dl], dt);
-        lds_barrier();
-        cur ^= 1;
-    }
-    // stillness: flag nsweep % 3 is raised before the barrier and read after it; the
-    // flag used two sweeps later is cleared meanwhile
-    while (nsweep < TH_MAXSWEEP) {
-        if (valid) {
-            const double tn = therm_eval(c, sX[cur][up], sX[cur][lf], sX[cur][dl], dt);
-            // still: no bit changes (stol 0), or no change above stol kelvin (the
-            // single unproven pass of FP32-arithmetic mode)
-            const double old = sX[cur][me];
-            if (stol > 0.0 ? fabs(tn - old) > stol : __double_as_longlong(tn) != __double_as_longlong(old))
-                sFlag[nsweep % 3] = 1;
-            sX[cur ^ 1][me] = tn;
+    // A wave whose nodes all lie inside the mesh with all six triangles takes the
+    // evaluation without selects (the same operations); every wave runs the same
+    // sweeps and barriers.
+    auto sweeps = [&](auto full_tag) {
+        constexpr bool FULL = decltype(full_tag)::value;
+        auto eval = [&]() {
+            return FULL ? therm_eval_full(c, sX[cur][up], sX[cur][lf], sX[cur][dl], dt)
+                        : therm_eval(c, sX[cur][up], sX[cur][lf], sX[cur][dl], dt);
+        };
+        for (; nsweep < TH_SWEEP0; nsweep++) {
+            if (FULL || valid) sX[cur ^ 1][me] = eval();
+            lds_barrier();
+            cur ^= 1;
         }
-        lds_barrier();
-        const int any = sFlag[nsweep % 3];
-        if (threadIdx.x == 0) sFlag[(nsweep + 2) % 3] = 0;
-        cur ^= 1;
-        nsweep++;
-        if (!any) {
-            still = true;
-            break;
+        // stillness: flag nsweep % 3 is raised before the barrier and read after it; the
+        // flag used two sweeps later is cleared meanwhile
+        while (nsweep < TH_MAXSWEEP) {
+            if (FULL || valid) {
+                const double tn = eval();
+                // still: no bit changes (stol 0), or no change above stol kelvin (the
+                // single unproven pass of FP32-arithmetic mode)
+                const double old = sX[cur][me];
+                if (stol > 0.0 ? fabs(tn - old) > stol : __double_as_longlong(tn) != __double_as_longlong(old))
+                    sFlag[nsweep % 3] = 1;
+                sX[cur ^ 1][me] = tn;
+            }
+            lds_barrier();
+            const int any = sFlag[nsweep % 3];
+            if (threadIdx.x == 0) sFlag[(nsweep + 2) % 3] = 0;
+            cur ^= 1;
+            nsweep++;
+            if (!any) {
+                still = true;
+                break;
+            }
         }
-    }
+    };
+    if (__all(valid && c.mask == 63))
+        sweeps(std::true_type{});
+    else
+        sweeps(std::false_type{});
     // the next tile's inputs go to R before this tile's outputs are stored, so that
     // waiting for the loads does not wait for the stores
     if (next >= 0) therm_store_raw<XIN, DIS>(pf, R);
