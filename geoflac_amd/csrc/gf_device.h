@@ -63,6 +63,7 @@ struct GfDevClock {
     double dt_move;             // dt of the last element move (its stress rotation may still be pending)
     // change_phase (src/change_phase.f90:1-260) on the device
     unsigned int ph_newmask;    // phases (bit id-1) that markers turned into in the current call
+    unsigned int ph_present;    // phases (bit id-1) markers may hold: those uploaded and every new one since
     int ph_call;                // change_phase calls so far: the stamp of the current call's changed elements
     int ph_replan, ph_nold;     // the phase-ratio planes were re-ordered in the current call; planes before
     int ph_map[GF_MAXACT];      // new plane of each old plane (-1: dropped)

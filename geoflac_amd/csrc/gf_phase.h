@@ -6,9 +6,12 @@
 // recounts the phase ratios, the dominant phase and the plastic strain of the
 // flagged elements.  Here that is four launches every ifreq_rmasses steps:
 //
-//   k_phase_rules    one thread per marker: the rules, the new phase, the element's
+//   k_phase_rules    one thread per element: unless bounds from the element's corners
+//                    show that no rule of a phase its markers may hold can fire, the
+//                    rules for each of its markers: the new phase, the element's
 //                    stamp (the call number, so no flag array is cleared) and the
-//                    new phase's bit in a mask;
+//                    new phase's bit in a mask (k_phase_rules_mk: one thread per
+//                    marker, where the element lists do not hold every marker once);
 //   k_phase_plan     one thread: the phase-ratio planes the device keeps are the
 //                    phases present anywhere, in increasing id (the order of the
 //                    reference's loops over phases, so every phase mix adds in the
@@ -56,6 +59,37 @@ GF_DEV bool phase_has_plane(const GfDevParams *__restrict__ P, int ph)
     return false;
 }
 
+// False when no marker of phase iph whose triangle lies within the given bounds can
+// change: depth between dlo and dhi, temperature at most tmax (with the margins dm,
+// tm of the callers included).  The interpolated depth and temperature of a marker
+// lie between its corner values (a convex combination, up to a few ulp of rounding,
+// which the margins cover), so the decision is exact.
+GF_DEV bool phase_rule_may_fire(const GfDevParams *__restrict__ P, int iph, double dlo, double dhi, double tmax,
+                                double tm)
+{
+    const double max_basalt_depth = 150.e3, min_eclogite_temp = 400.0, min_eclogite_depth = 20e3,
+                 mantle_density = 3000.0, max_melting_depth = 200.e3;
+    if (dlo > max_melting_depth) return false;
+    if (iph == PH_MANT1 || iph == PH_MANT2) {
+        if (dlo > max_basalt_depth) return false;
+        // without a phase that triggers serpentinisation only the dry-mantle rule
+        // remains: tmpr >= solidus + 35, and solidus + 35 = -5.02 x^2 + 132 x + 1160
+        // (x the pressure in GPa) is concave, so on [0, xmax] at least its value at
+        // an end of the interval
+        if (!(phase_has_plane(P, PH_OCEAN1) || phase_has_plane(P, PH_OCEAN2) || phase_has_plane(P, PH_SED1) ||
+              phase_has_plane(P, PH_SED2))) {
+            const double dabs = fmax(fabs(dlo), fabs(fmin(dhi, max_basalt_depth)));
+            const double xmax = mantle_density * fabs(P->g) * dabs * 1e-9 * (1.0 + 1e-12);
+            const double smin = fmin(1160.0, -5.02 * (xmax * xmax) + 1.32e2 * xmax + 1.16e3) - 1e-6;
+            if (tmax + tm < smin) return false;
+        }
+        return true;
+    }
+    if (iph == PH_OCEANDRY || iph == PH_OCEAN1 || iph == PH_OCEAN2)
+        return !(tmax + tm < min_eclogite_temp || dhi < min_eclogite_depth);
+    return phase_has_rule(iph);
+}
+
 // src/change_phase.f90:58-230 for marker kk
 GF_DEV void phase_rules_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long kk)
 {
@@ -75,31 +109,13 @@ GF_DEV void phase_rules_body(const GfDevParams *__restrict__ P, GfDevClock *ck, 
     const double t1 = TEMP(j1, i1), t2 = TEMP(j + 1, i), t3 = TEMP(j3, i3);
     const double ysurf = 0.5 * (CORD(1, i, 2) + CORD(1, i + 1, 2));
     // Markers whose rules cannot fire anywhere in their triangle are skipped before
-    // their barycentric coordinates are read.  The interpolated y and temperature lie
-    // between the corner values (a convex combination, up to a few ulp of rounding,
-    // which the margins cover), so bounds from the corners decide it exactly.
+    // their barycentric coordinates are read.
     {
         const double tmax = fmax(fmax(t1, t2), t3);
         const double ylo = fmin(fmin(y1, y2), y3), yhi = fmax(fmax(y1, y2), y3);
         const double dm = 1e-6 * (fabs(ysurf) + fabs(ylo) + fabs(yhi) + 1.0), tm = 1e-9 * (fabs(tmax) + 1.0);
         const double dlo = ysurf - yhi - dm, dhi = ysurf - ylo + dm;  // depth range
-        if (dlo > max_melting_depth) return;
-        if (iph == PH_MANT1 || iph == PH_MANT2) {
-            if (dlo > max_basalt_depth) return;
-            // without a phase that triggers serpentinisation only the dry-mantle rule
-            // remains: tmpr >= solidus + 35, and solidus + 35 = -5.02 x^2 + 132 x + 1160
-            // (x the pressure in GPa) is concave, so on [0, xmax] at least its value at
-            // an end of the interval
-            if (!(phase_has_plane(P, PH_OCEAN1) || phase_has_plane(P, PH_OCEAN2) || phase_has_plane(P, PH_SED1) ||
-                  phase_has_plane(P, PH_SED2))) {
-                const double dabs = fmax(fabs(dlo), fabs(fmin(dhi, max_basalt_depth)));
-                const double xmax = mantle_density * fabs(P->g) * dabs * 1e-9 * (1.0 + 1e-12);
-                const double smin = fmin(1160.0, -5.02 * (xmax * xmax) + 1.32e2 * xmax + 1.16e3) - 1e-6;
-                if (tmax + tm < smin) return;
-            }
-        } else if (iph == PH_OCEANDRY || iph == PH_OCEAN1 || iph == PH_OCEAN2) {
-            if (tmax + tm < min_eclogite_temp || dhi < min_eclogite_depth) return;
-        }
+        if (!phase_rule_may_fire(P, iph, dlo, dhi, tmax, tm)) return;
     }
     const double a1 = A.mark_a1[kk], a2 = A.mark_a2[kk];
     const double yy = y1 * a1 + y2 * a2 + y3 * (1 - a1 - a2);
@@ -184,12 +200,43 @@ GF_DEV void phase_rules_body(const GfDevParams *__restrict__ P, GfDevClock *ck, 
     }
 }
 
-__global__ void __launch_bounds__(256) k_phase_rules(const GfDevParams *__restrict__ P, GfDevClock *ck,
-                                                     GfDevArrays A)
+__global__ void __launch_bounds__(256) k_phase_rules_mk(const GfDevParams *__restrict__ P, GfDevClock *ck,
+                                                        GfDevArrays A)
 {
     const long kk = xcd_lin();
     if (kk >= A.nmarkers) return;
     phase_rules_body(P, ck, A, kk);
+}
+
+// Element e: bounds over its four corners contain every marker triangle of the element
+// (and the same ysurf), so when no phase that markers may hold (ck->ph_present, a
+// superset) can change there, none of its markers is read.  Requires that the element
+// lists hold every live marker exactly once (checked when the markers are uploaded).
+GF_DEV void phase_rules_elem_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e)
+{
+    const unsigned present = ck->ph_present;
+    const int j = (int)((unsigned)e % (unsigned)(NZ - 1)) + 1, i = (int)((unsigned)e / (unsigned)(NZ - 1)) + 1;
+    const double y1 = CORD(j, i, 2), y2 = CORD(j + 1, i, 2), y3 = CORD(j, i + 1, 2), y4 = CORD(j + 1, i + 1, 2);
+    const double t1 = TEMP(j, i), t2 = TEMP(j + 1, i), t3 = TEMP(j, i + 1), t4 = TEMP(j + 1, i + 1);
+    const double ysurf = 0.5 * (CORD(1, i, 2) + CORD(1, i + 1, 2));
+    const double tmax = fmax(fmax(t1, t2), fmax(t3, t4));
+    const double ylo = fmin(fmin(y1, y2), fmin(y3, y4)), yhi = fmax(fmax(y1, y2), fmax(y3, y4));
+    const double dm = 1e-6 * (fabs(ysurf) + fabs(ylo) + fabs(yhi) + 1.0), tm = 1e-9 * (fabs(tmax) + 1.0);
+    const double dlo = ysurf - yhi - dm, dhi = ysurf - ylo + dm;
+    bool may = false;
+    for (int ph = 1; ph <= GF_MAXPH && !may; ph++)
+        if (present & (1u << (ph - 1))) may = phase_rule_may_fire(P, ph, dlo, dhi, tmax, tm);
+    if (!may) return;
+    const int nm = A.nmark_elem[e];
+    for (int q = 0; q < nm; q++) phase_rules_body(P, ck, A, (long)A.mark_id_elem[q + (long)GF_MAX_MARKERS_PER_ELEM * e] - 1);
+}
+
+__global__ void __launch_bounds__(256) k_phase_rules(const GfDevParams *__restrict__ P, GfDevClock *ck,
+                                                     GfDevArrays A)
+{
+    const long e = xcd_lin();
+    if (e >= NE) return;
+    phase_rules_elem_body(P, ck, A, e);
 }
 
 // The planes after this call's rules: the phases of the old planes that some element
@@ -199,6 +246,7 @@ GF_DEV void phase_plan_body(GfDevParams *P, GfDevClock *ck)
 {
     const unsigned newmask = ck->ph_newmask;
     ck->ph_newmask = 0u;
+    ck->ph_present |= newmask;
     ck->ph_call = ck->ph_call + 1;
     ck->ph_replan = 0;
     const int nold = P->nactive;

@@ -9,6 +9,7 @@
 
 #include <math.h>
 #include <stdarg.h>
+#include <stddef.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -38,6 +39,10 @@ struct gf_ctx {
     // markers (module marker_data): bound host arrays, device capacity
     gf_marker_arrays hmark{};
     bool markers_bound = false, markers_uploaded = false;
+    // the element lists hold every live marker once, in its own triangle's element:
+    // change_phase's rules run per element (k_phase_rules), else per marker
+    bool mark_lists_ok = false;
+    bool rules_per_marker = false;  // GF_PHASE_RULES=marker: always the per-marker kernel
     long mark_cap = 0;
     float *coh_scale_dev = nullptr;
     // stress boundary conditions on the device (bc_update)
@@ -248,6 +253,8 @@ extern "C" int gf_create(const gf_params *p, int32_t precision, gf_ctx **out)
         ctx->node_direct = nv && strcmp(nv, "direct") == 0;
         const char *gv = getenv("GF_THERM_GUESS");
         ctx->therm_noguess = gv && strcmp(gv, "0") == 0;
+        const char *pv = getenv("GF_PHASE_RULES");
+        ctx->rules_per_marker = pv && strcmp(pv, "marker") == 0;
     }
     const long ne = ctx->ne, nn = ctx->nn;
     int rc = GF_OK;
@@ -547,6 +554,33 @@ static int upload_markers(gf_ctx *ctx)
     HIPCHK(hipMemcpy(ctx->a.nmark_elem, m.nmark_elem, 4 * (size_t)ne, hipMemcpyHostToDevice));
     ctx->a.nmarkers = nmk;
     ctx->markers_uploaded = true;
+    {
+        bool ok = true;
+        std::vector<unsigned char> seen((size_t)std::max<long>(nmk, 1), 0);
+        for (long e = 0; e < ne && ok; e++)
+            for (int q = 0; q < m.nmark_elem[e]; q++) {
+                const long id = m.mark_id_elem[q + (long)GF_MAX_MARKERS_PER_ELEM * e] - 1;
+                if (m.mark_dead[id] == 0) continue;  // (the rules skip dead markers)
+                const int n = m.mark_ntriag[id], k = (n - 1) % 2 + 1;
+                if ((n - k) / 2 != e || seen[id]) {
+                    ok = false;
+                    break;
+                }
+                seen[id] = 1;
+            }
+        for (long q = 0; q < nmk && ok; q++)
+            if (m.mark_dead[q] != 0 && !seen[q]) ok = false;
+        ctx->mark_lists_ok = ok;
+        unsigned present = 0u;  // phases the live markers hold
+        for (long q = 0; q < nmk; q++)
+            if (m.mark_dead[q] != 0) {
+                const int ph = m.mark_phase[q];
+                present |= (ph >= 1 && ph <= GF_MAXPH) ? 1u << (ph - 1) : 0xffffffffu;
+            }
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        HIPCHK(hipMemcpy((char *)ctx->dck + offsetof(GfDevClock, ph_present), &present, sizeof present,
+                         hipMemcpyHostToDevice));
+    }
     return GF_OK;
 }
 
@@ -1176,7 +1210,10 @@ static int enq_bc_update(gf_ctx *ctx)
 static int enq_change_phase(gf_ctx *ctx)
 {
     if (!ctx->markers_uploaded) return GF_OK;
-    if (ctx->a.nmarkers > 0) LAUNCH(k_phase_rules, ctx->a.nmarkers, ctx->dp, ctx->dck, ctx->a);
+    if (ctx->a.nmarkers > 0 && ctx->mark_lists_ok && !ctx->rules_per_marker)
+        LAUNCH(k_phase_rules, ctx->ne, ctx->dp, ctx->dck, ctx->a);
+    else if (ctx->a.nmarkers > 0)
+        LAUNCH(k_phase_rules_mk, ctx->a.nmarkers, ctx->dp, ctx->dck, ctx->a);
     hipLaunchKernelGGL(k_phase_plan, dim3(1), dim3(64), 0, ctx->stream, ctx->dp, ctx->dck);
     int rc = launch_check(ctx, "k_phase_plan");
     if (rc) return rc;

@@ -102,17 +102,19 @@ def test_subroutine_bitwise_equal_to_oracle(sub, case, step):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("env", [("GF_NODE", "direct"), ("GF_RHEOL", "quad")])
-def test_kernel_variants_bitwise_equal_to_oracle(env, monkeypatch):
+@pytest.mark.parametrize("env,case,start", [(("GF_NODE", "direct"), CASE, 10), (("GF_RHEOL", "quad"), CASE, 10),
+                                            (("GF_PHASE_RULES", "marker"), "phases60x30", 11)])
+def test_kernel_variants_bitwise_equal_to_oracle(env, case, start, monkeypatch):
     """The alternative kernels, chosen when a context is created: the node update
-    gathering straight from global memory (instead of the LDS tiles) and the rheology
-    with four lanes per element.  20 steps from the 10-step golden state stay bitwise
+    gathering straight from global memory (instead of the LDS tiles), the rheology
+    with four lanes per element, change_phase's rules with one thread per marker
+    (instead of per element).  20 steps from the 10-step golden state stay bitwise
     equal to the oracle."""
     from geoflac_amd import gpu
     monkeypatch.setenv(*env)
-    _say("case: %s=%s, 20 steps from the 10-step golden state of %s" % (env[0], env[1], CASE))
-    p = ol.golden_params(CASE)
-    st = golden_full(10)
+    _say("case: %s=%s, 20 steps from the %d-step golden state of %s" % (env[0], env[1], start, case))
+    p = ol.golden_params(case)
+    st = golden_full(start, case)
     ref = st.copy()
     with gpu.Model(p, st) as m:
         m.step(20)
@@ -121,6 +123,8 @@ def test_kernel_variants_bitwise_equal_to_oracle(env, monkeypatch):
     bad, worst, nexact = compare(st, ref, tol=0.0)
     _say("  exact fields %d, differing %s" % (nexact, worst))
     assert not bad, bad
+    if case == "phases60x30":
+        assert np.array_equal(st.mark_phase, ref.mark_phase)
 
 
 @pytest.mark.gpu
