@@ -86,3 +86,30 @@ def test_too_many_phases_for_the_ratio_planes():
         with pytest.raises(gpu.GfError) as exc:
             m.step(1)
         assert exc.value.code == abi.GF_ERR_PHASES
+
+
+@pytest.mark.gpu
+def test_maxwell_stop_22_on_the_device():
+    """src/rh_maxwell.f90:24-30: an element whose Maxwell number theta = rm dt / (2 visn)
+    exceeds 0.5 stops the run (STOP 22).  Two elements of box60x20's 10-step state get
+    a viscosity with theta = 0.6, and the loop counter is set so that the step updates
+    no viscosity (nloop 11, not a multiple of ifreq_visc): the device
+    returns GF_STOP_MAXWELL naming the first of them in the reference's loop order
+    (column by column), and so does the oracle."""
+    from geoflac_amd import gpu
+    from tests.test_gpu_parity import golden_full
+    _say("case: box60x20 10-step state, theta 0.6 at elements (5,7) and (12,3), one step")
+    p = ol.golden_params("box60x20")
+    st = golden_full(10, "box60x20")
+    st.clock.nloop = 11
+    for j, i in ((5, 7), (12, 3)):
+        rm = p.rm[int(st.iphase[j - 1, i - 1]) - 1]
+        st.visn[j - 1, i - 1] = rm * st.clock.dt / (2 * 0.6)
+    ref = st.copy()
+    assert ol.step(p, ref, 1, cr=True) == abi.GF_STOP_MAXWELL
+    with gpu.Model(p, st) as m:
+        with pytest.raises(gpu.GfError) as exc:
+            m.step(1)
+    _say("  %s" % exc.value)
+    assert exc.value.code == abi.GF_STOP_MAXWELL
+    assert "(12,3)" in str(exc.value)
