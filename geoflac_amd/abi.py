@@ -89,6 +89,18 @@ class GfClock(C.Structure):
                 ("dt_elastic", _D), ("dt_maxwell", _D)]
 
 
+IO_FLAGS = ["io_vel", "io_srII", "io_eII", "io_aps", "io_sII", "io_sxx", "io_szz", "io_sxz", "io_pres", "io_temp",
+            "io_phase", "io_visc", "io_unused", "io_density", "io_src", "io_diss", "io_forc", "io_hfl", "io_topo"]
+# host-owned arrays of the output files (none: zeros)
+OUTPUT_ARRAYS = ["xoriginal", "zoriginal", "dhacc", "extr_acc", "dhacc_correction", "q_init", "vel_flex_old",
+                 "fmagma", "fmagma2", "dike_backlog", "dike_marker_vol", "mor_marker_vol", "mark_age"]
+
+
+class GfOutputOpts(C.Structure):
+    _fields_ = ([(n, _I) for n in IO_FLAGS] + [("lastout", _I), ("hs", _D)]
+                + [(n, _DP) for n in OUTPUT_ARRAYS])
+
+
 # field bits, in the order of HOST_FIELDS
 FIELD_BIT = {name: 1 << k for k, (name, _) in enumerate(HOST_FIELDS)}
 GF_ALL = (1 << len(HOST_FIELDS)) - 1

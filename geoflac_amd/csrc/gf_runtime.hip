@@ -22,6 +22,7 @@
 #include "gf_kernels.h"
 #include "gf_phase.h"
 #include "gf_bc.h"
+#include "gf_output.h"
 #include "gf_mlp.h"
 #include "gf_batch.h"
 
@@ -48,6 +49,7 @@ struct gf_ctx {
     // stress boundary conditions on the device (bc_update)
     int *bc_nodes = nullptr, *bc_estart = nullptr, *bc_elist = nullptr;
     double *bc_str = nullptr, *bc_press = nullptr;
+    float *out_buf = nullptr;  // outflac's FP32 staging buffer (allocated at the first call)
     bool stressbc_uploaded = false;
     uint32_t uploaded = 0;
     int32_t nloop = 0, nsrate = 0;
@@ -350,7 +352,7 @@ extern "C" void gf_destroy(gf_ctx *ctx)
                     ctx->a.xa, ctx->a.xb, ctx->a.dens, ctx->therm_border, ctx->a.stress0f, ctx->a.velf,
                     ctx->a.forcef, ctx->a.pref, ctx->sur_samples, ctx->sur_w, ctx->sur_m, ctx->sur_v,
                     ctx->sur_grad, ctx->sur_partial, ctx->sur_cfg, ctx->sur_stats, ctx->sur_list,
-                    ctx->sur_cnt, ctx->sur_lval, ctx->sur_wgs, ctx->cord_alt};
+                    ctx->sur_cnt, ctx->sur_lval, ctx->sur_wgs, ctx->cord_alt, ctx->out_buf};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1304,6 +1306,248 @@ SINGLE(gf_rmasses, enq_rmasses(ctx))
 SINGLE(gf_dt_mass, enq_dt_mass(ctx))
 SINGLE(gf_change_phase, enq_change_phase(ctx))
 SINGLE(gf_bc_update, enq_bc_update(ctx))
+
+// ------------------------------------------------------------------ output and restart files
+// one record of a direct-access file (src/outflac.f90, src/saveflac.f90): record nrec
+// at byte (nrec-1)*recl, the rest of a short record zero (as the reference's runtime
+// writes it); the file is created if missing and otherwise kept
+static int write_record(gf_ctx *ctx, const std::string &path, long nrec, size_t recl, const void *data,
+                        size_t nbytes)
+{
+    FILE *f = fopen(path.c_str(), "r+b");
+    if (!f) f = fopen(path.c_str(), "w+b");
+    if (!f) return set_err(ctx, GF_ERR_ARG, "cannot open %s for writing", path.c_str());
+    bool ok = fseek(f, (long)((nrec - 1) * (long)recl), SEEK_SET) == 0;
+    if (ok && nbytes) ok = fwrite(data, 1, nbytes, f) == nbytes;
+    if (ok && nbytes < recl) {
+        std::vector<char> z(recl - nbytes, 0);
+        ok = fwrite(z.data(), 1, z.size(), f) == z.size();
+    }
+    if (fclose(f) != 0) ok = false;
+    return ok ? GF_OK : set_err(ctx, GF_ERR_ARG, "writing %s failed", path.c_str());
+}
+
+static int output_ready(gf_ctx *ctx, const gf_clock *clock, const char *dir, const gf_output_opts *o)
+{
+    if (!ctx) return GF_ERR_ARG;
+    if (!clock || !dir || !o) return set_err(ctx, GF_ERR_ARG, "null clock, directory or options");
+    if (ctx->precision != GF_PREC_FP64)
+        return set_err(ctx, GF_ERR_UNSUPPORTED, "output files are written from FP64 contexts only");
+    int rc = flush_rot(ctx);  // the move's rotation and areas complete the state
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return GF_OK;
+}
+
+// the host array p (n values) or n zeros, as FP32 frame values
+static std::vector<float> host_frame(const double *p, size_t n)
+{
+    std::vector<float> v(n, 0.0f);
+    if (p)
+        for (size_t q = 0; q < n; q++) v[q] = (float)p[q];
+    return v;
+}
+
+// src/outflac.f90:5-431
+extern "C" int gf_outflac(gf_ctx *ctx, const gf_clock *clock, const char *dir, const gf_output_opts *o)
+{
+    int rc = output_ready(ctx, clock, dir, o);
+    if (rc) return rc;
+    const long nz = ctx->p.nz, nx = ctx->p.nx, nn = ctx->nn, ne = ctx->ne;
+    const std::string d = std::string(dir) + "/";
+    // record number: 1 with lastout, else one past the last line of _contents.0 (:16-30)
+    long nrec = 1;
+    const double myr = clock->time / GF_SEC_YEAR / 1.e6;
+    {
+        const std::string cp = d + "_contents.0";
+        if (o->lastout != 1) {
+            FILE *f = fopen(cp.c_str(), "r");
+            long last = 0;
+            if (f) {
+                char line[512];
+                while (fgets(line, sizeof line, f)) {
+                    long v;
+                    if (sscanf(line, "%ld", &v) == 1) last = v;
+                }
+                fclose(f);
+            }
+            nrec = last + 1;
+        }
+        FILE *f = fopen(cp.c_str(), o->lastout == 1 ? "w" : "a");
+        if (!f) return set_err(ctx, GF_ERR_ARG, "cannot write %s", cp.c_str());
+        fprintf(f, "%6ld %8d %10.6f\n", nrec, clock->nloop, myr);
+        if (fclose(f) != 0) return set_err(ctx, GF_ERR_ARG, "writing %s failed", cp.c_str());
+    }
+    const float rtime = (float)clock->time;
+    if ((rc = write_record(ctx, d + "time.0", nrec, 4, &rtime, 4))) return rc;
+    // the frames on the device
+    const size_t nf = (size_t)ON_COUNT * nn + (size_t)OE_COUNT * ne + (size_t)OC_COUNT * nx;
+    if (!ctx->out_buf) HIPCHK(hipMalloc((void **)&ctx->out_buf, nf * sizeof(float)));
+    float *dn = ctx->out_buf, *de = dn + (size_t)ON_COUNT * nn, *dc = de + (size_t)OE_COUNT * ne;
+    const OutScalars sc{clock->dt, clock->dtavg, o->hs};
+    LAUNCH(k_out_nodes, nn, ctx->dp, ctx->a, sc, dn);
+    LAUNCH(k_out_elems, ne, ctx->dp, ctx->dck, ctx->a, sc, de);
+    LAUNCH(k_out_cols, nx, ctx->dp, ctx->a, dc);
+    std::vector<float> h(nf);
+    HIPCHK(hipMemcpyAsync(h.data(), ctx->out_buf, nf * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const float *hn = h.data(), *he = hn + (size_t)ON_COUNT * nn, *hc = he + (size_t)OE_COUNT * ne;
+    auto node = [&](const char *name, int plane, int nplanes) {
+        const size_t b = (size_t)nplanes * nn * 4;
+        return write_record(ctx, d + name, nrec, b, hn + (size_t)plane * nn, b);
+    };
+    auto elem = [&](const char *name, int plane) {
+        return write_record(ctx, d + name, nrec, (size_t)ne * 4, he + (size_t)plane * ne, (size_t)ne * 4);
+    };
+    auto elem_host = [&](const char *name, const double *p) {
+        const std::vector<float> v = host_frame(p, (size_t)ne);
+        return write_record(ctx, d + name, nrec, (size_t)ne * 4, v.data(), (size_t)ne * 4);
+    };
+    auto col = [&](const char *name, const float *v, long n) {
+        return write_record(ctx, d + name, nrec, (size_t)n * 4, v, (size_t)n * 4);
+    };
+#define OUT(x) \
+    if ((rc = (x))) return rc
+    OUT(node("mesh.0", ON_MESH, 2));
+    if (o->io_vel == 1) OUT(node("vel.0", ON_VEL, 2));
+    if (o->io_temp == 1) OUT(node("temperature.0", ON_TEMP, 1));
+    OUT(node("coolingrate.0", ON_COOL, 1));
+    {
+        const std::vector<float> xo = host_frame(o->xoriginal, (size_t)nn), zo = host_frame(o->zoriginal, (size_t)nn);
+        OUT(write_record(ctx, d + "xoriginal.0", nrec, (size_t)nn * 4, xo.data(), (size_t)nn * 4));
+        OUT(write_record(ctx, d + "zoriginal.0", nrec, (size_t)nn * 4, zo.data(), (size_t)nn * 4));
+    }
+    if (o->io_srII == 1) {
+        OUT(elem("srII.0", OE_SRII));
+        OUT(elem("srxx.0", OE_SRXX));
+        OUT(elem("srzz.0", OE_SRZZ));
+        OUT(elem("srxz.0", OE_SRXZ));
+    }
+    if (o->io_eII == 1) {
+        OUT(elem("eII.0", OE_EII));
+        OUT(elem("eI.0", OE_EI));
+        OUT(elem("exx.0", OE_EXX));
+        OUT(elem("ezz.0", OE_EZZ));
+        OUT(elem("exz.0", OE_EXZ));
+    }
+    if (o->io_density == 1) {
+        OUT(elem("density.0", OE_DENSITY));
+        OUT(elem("area.0", OE_AREA));
+    }
+    if (o->io_aps == 1) OUT(elem("aps.0", OE_APS));
+    if (o->io_sII == 1) OUT(elem("sII.0", OE_SII));
+    if (o->io_sxx == 1) OUT(elem("sxx.0", OE_SXX));
+    if (o->io_szz == 1) OUT(elem("szz.0", OE_SZZ));
+    if (o->io_sxz == 1) {
+        OUT(elem("sxz.0", OE_SXZ));
+        OUT(elem("syy.0", OE_SYY));
+    }
+    if (o->io_pres == 1) OUT(elem("pres.0", OE_PRES));
+    if (o->io_temp == 1) OUT(elem("temp.0", OE_TEMP));
+    if (o->io_phase == 1) OUT(elem("phase.0", OE_PHASE));
+    if (o->io_visc == 1) OUT(elem("visc.0", OE_VISC));
+    if (o->io_src == 1) {
+        OUT(elem("src.0", OE_SRC));
+        // melt fractions: no melting model runs (itype_melting = 0), they stay 0
+        OUT(elem_host("fmelt.0", nullptr));
+        OUT(elem_host("fmelt2.0", nullptr));
+        OUT(elem_host("fmagma.0", o->fmagma));
+        OUT(elem_host("fmagma2.0", o->fmagma2));
+    }
+    if (o->io_diss == 1) OUT(elem("diss.0", OE_DISS));
+    if (o->io_hfl == 1) OUT(col("hfl.0", hc + (size_t)OC_HFL * nx, nx));
+    if (o->io_topo == 1) {
+        OUT(col("topo.0", hc + (size_t)OC_TOPO * nx, nx));
+        // no surface process runs (topo_kappa = 0): dtopo and extrusion stay 0
+        const std::vector<float> z((size_t)nx, 0.0f);
+        OUT(col("dtopo.0", z.data(), nx));
+        OUT(col("extrusion.0", z.data(), nx - 1));
+    }
+#undef OUT
+    (void)nz;
+    return GF_OK;
+}
+
+// src/saveflac.f90:5-172
+extern "C" int gf_saveflac(gf_ctx *ctx, const gf_clock *clock, const char *dir, const gf_output_opts *o)
+{
+    int rc = output_ready(ctx, clock, dir, o);
+    if (rc) return rc;
+    const long nx = ctx->p.nx, nn = ctx->nn, ne = ctx->ne, nmk = ctx->markers_uploaded ? ctx->a.nmarkers : 0;
+    const std::string d = std::string(dir) + "/";
+    {
+        const std::string cp = d + "_contents.save";
+        FILE *f = fopen(cp.c_str(), "w");
+        if (!f) return set_err(ctx, GF_ERR_ARG, "cannot write %s", cp.c_str());
+        fprintf(f, "%4d %8d %6.2f %9ld %9d\n", 1, clock->nloop, clock->time / GF_SEC_YEAR / 1.e6, nmk, 0);
+        if (fclose(f) != 0) return set_err(ctx, GF_ERR_ARG, "writing %s failed", cp.c_str());
+    }
+#define OUT(x) \
+    if ((rc = (x))) return rc
+    const double tdt[2] = {clock->time, clock->dt};
+    OUT(write_record(ctx, d + "time.rs", 1, 16, tdt, 16));
+    std::vector<double> buf;
+    auto dev = [&](const char *name, const void *p, size_t n) {  // n doubles of device array p
+        buf.resize(n);
+        HIPCHK(hipMemcpy(buf.data(), p, n * 8, hipMemcpyDeviceToHost));
+        return write_record(ctx, d + name, 1, n * 8, buf.data(), n * 8);
+    };
+    auto host = [&](const char *name, const double *p, size_t n) {
+        if (p) return write_record(ctx, d + name, 1, n * 8, p, n * 8);
+        const std::vector<double> z(n, 0.0);
+        return write_record(ctx, d + name, 1, n * 8, z.data(), n * 8);
+    };
+    OUT(dev("cord.rs", ctx->a.cord, (size_t)nn * 2));
+    OUT(host("dhacc.rs", o->dhacc, (size_t)nx - 1));
+    OUT(host("dhacc_corr.rs", o->dhacc_correction, (size_t)nx));
+    OUT(host("extr_acc.rs", o->extr_acc, (size_t)nx - 1));
+    OUT(host("q_init.rs", o->q_init, (size_t)nx));
+    OUT(host("vel_flex_old.rs", o->vel_flex_old, (size_t)nx));
+    OUT(dev("vel.rs", ctx->a.vel, (size_t)nn * 2));
+    OUT(dev("strain.rs", ctx->a.strain, (size_t)ne * 3));
+    OUT(dev("stress.rs", ctx->a.stress0, (size_t)ne * 16));
+    OUT(dev("temp.rs", ctx->a.temp, (size_t)nn));
+    OUT(host("xoriginal.rs", o->xoriginal, (size_t)nn));
+    OUT(host("zoriginal.rs", o->zoriginal, (size_t)nn));
+    {  // integer phases in a record of 8-byte words (the rest zero)
+        std::vector<int32_t> ip((size_t)ne);
+        HIPCHK(hipMemcpy(ip.data(), ctx->a.iphase, (size_t)ne * 4, hipMemcpyDeviceToHost));
+        OUT(write_record(ctx, d + "phase.rs", 1, (size_t)ne * 8, ip.data(), (size_t)ne * 4));
+    }
+    OUT(dev("aps.rs", ctx->a.aps, (size_t)ne));
+    OUT(host("fmagma.rs", o->fmagma, (size_t)ne));
+    OUT(host("fmagma2.rs", o->fmagma2, (size_t)ne));
+    OUT(dev("source.rs", ctx->a.source, (size_t)ne));
+    OUT(host("dike_backlog.rs", o->dike_backlog, (size_t)ne));
+    OUT(host("dike_marker_vol.rs", o->dike_marker_vol, (size_t)ne));
+    OUT(host("mor_marker_vol.rs", o->mor_marker_vol, (size_t)ne));
+    if (nmk > 0) {
+        const size_t n = (size_t)nmk;
+        std::vector<double> a(n);
+        HIPCHK(hipMemcpy(a.data(), ctx->a.mark_a1, n * 8, hipMemcpyDeviceToHost));
+        OUT(write_record(ctx, d + "marker1.rs", 1, n * 8, a.data(), n * 8));
+        HIPCHK(hipMemcpy(a.data(), ctx->a.mark_a2, n * 8, hipMemcpyDeviceToHost));
+        OUT(write_record(ctx, d + "marker1.rs", 2, n * 8, a.data(), n * 8));
+        if (o->mark_age) {
+            OUT(write_record(ctx, d + "marker1.rs", 3, n * 8, o->mark_age, n * 8));
+        } else {
+            std::fill(a.begin(), a.end(), 0.0);
+            OUT(write_record(ctx, d + "marker1.rs", 3, n * 8, a.data(), n * 8));
+        }
+        std::vector<signed char> b(n);
+        std::vector<int32_t> w(n);
+        HIPCHK(hipMemcpy(b.data(), ctx->a.mark_dead, n, hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < n; q++) w[q] = b[q];
+        OUT(write_record(ctx, d + "marker2.rs", 1, n * 4, w.data(), n * 4));
+        HIPCHK(hipMemcpy(w.data(), ctx->a.mark_ntriag, n * 4, hipMemcpyDeviceToHost));
+        OUT(write_record(ctx, d + "marker2.rs", 2, n * 4, w.data(), n * 4));
+        HIPCHK(hipMemcpy(b.data(), ctx->a.mark_phase, n, hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < n; q++) w[q] = b[q];
+        OUT(write_record(ctx, d + "marker2.rs", 3, n * 4, w.data(), n * 4));
+    }
+#undef OUT
+    return GF_OK;
+}
 
 extern "C" int gf_init_areas(gf_ctx *ctx)
 {

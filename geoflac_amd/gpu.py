@@ -17,7 +17,7 @@ EXPORTS = [
     "gf_create", "gf_destroy", "gf_last_error", "gf_error_site", "gf_bind_host", "gf_bind_markers", "gf_upload",
     "gf_download", "gf_set_clock", "gf_get_clock", "gf_step", "gf_flac", "gf_fl_therm",
     "gf_fl_rheol", "gf_fl_node", "gf_fl_move", "gf_rmasses", "gf_dt_mass", "gf_change_phase", "gf_bc_update",
-    "gf_init_areas",
+    "gf_init_areas", "gf_outflac", "gf_saveflac",
     "gf_init_visc", "gf_therm_stats", "gf_set_therm_passes", "gf_sync",
     "gf_set_timing", "gf_kernel_time", "gf_math_check",
     "gf_surrogate_sample", "gf_surrogate_train", "gf_surrogate_enable", "gf_surrogate_stats",
@@ -60,6 +60,8 @@ def lib():
                   "gf_change_phase", "gf_bc_update"):
             getattr(L, n).argtypes = [vp, C.POINTER(abi.GfClock)]
         L.gf_init_areas.argtypes = [vp]
+        for n in ("gf_outflac", "gf_saveflac"):
+            getattr(L, n).argtypes = [vp, C.POINTER(abi.GfClock), C.c_char_p, C.POINTER(abi.GfOutputOpts)]
         L.gf_init_visc.argtypes = [vp]
         L.gf_therm_stats.argtypes = [vp, C.POINTER(C.c_int64)]
         L.gf_sync.argtypes = [vp]
@@ -150,6 +152,38 @@ class Model:
 
     def init_areas(self):
         return self._check(lib().gf_init_areas(self._ctx))
+
+    @staticmethod
+    def output_opts(io_flags=None, lastout=0, hs=0.0, **arrays):
+        """gf_output_opts: the 19 frame switches (all on by default), lastout, hs and the
+        host-owned arrays by name (abi.OUTPUT_ARRAYS; missing ones are written as zeros).
+        The arrays must stay alive while the options are used."""
+        import numpy as np
+        o = abi.GfOutputOpts()
+        for n, v in zip(abi.IO_FLAGS, io_flags if io_flags is not None else [1] * len(abi.IO_FLAGS)):
+            setattr(o, n, int(v))
+        o.lastout, o.hs = int(lastout), float(hs)
+        keep = []
+        for n, a in arrays.items():
+            if n not in abi.OUTPUT_ARRAYS:
+                raise KeyError(n)
+            if a is None:
+                continue
+            a = np.asfortranarray(a, dtype=np.float64)
+            keep.append(a)
+            setattr(o, n, a.ctypes.data_as(C.POINTER(C.c_double)))
+        o._keep = keep
+        return o
+
+    def outflac(self, directory, opts):
+        """outflac's frame files of the current device state (src/outflac.f90), into directory."""
+        return self._check(lib().gf_outflac(self._ctx, C.byref(self.state.clock), str(directory).encode(),
+                                            C.byref(opts)))
+
+    def saveflac(self, directory, opts):
+        """saveflac's restart files of the current device state (src/saveflac.f90), into directory."""
+        return self._check(lib().gf_saveflac(self._ctx, C.byref(self.state.clock), str(directory).encode(),
+                                             C.byref(opts)))
 
     def init_visc(self):
         return self._check(lib().gf_init_visc(self._ctx))

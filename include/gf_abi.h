@@ -186,6 +186,29 @@ int gf_dt_mass(gf_ctx *ctx, gf_clock *clock);    /* src/dt_mass.f90:16-160 */
 int gf_change_phase(gf_ctx *ctx, gf_clock *clock); /* src/change_phase.f90:1-260 (itype_melting=0 part) */
 int gf_bc_update(gf_ctx *ctx, gf_clock *clock);    /* src/bc_update.f90:6-143 (nystressbc = 1) */
 int gf_init_areas(gf_ctx *ctx);                  /* src/init_areas.f90:7-55 */
+
+/* ---- output and restart files (src/outflac.f90:5-431, src/saveflac.f90:5-172) ----
+ * The reference's frame and restart files of the current device state, written into
+ * directory `dir` in the reference's formats (direct-access records, the _contents
+ * lines).  Frames are evaluated on the device.  Arrays the explicit step never changes
+ * are the caller's (host state of the reference's arrays module); a null pointer
+ * writes zeros.  The clock gives nloop, time, dt and dtavg.  FP64 contexts only.
+ * Amin.rs (a list-directed scalar of setflac) stays with the caller. */
+typedef struct {
+    /* frame switches and lastout, src/read_params.f90:329-333 */
+    int32_t io_vel, io_srII, io_eII, io_aps, io_sII, io_sxx, io_szz, io_sxz, io_pres, io_temp, io_phase,
+        io_visc, io_unused, io_density, io_src, io_diss, io_forc, io_hfl, io_topo;
+    int32_t lastout;
+    double hs;                                /* heat source scale of the diss frame */
+    const double *xoriginal, *zoriginal;      /* (nz, nx) */
+    const double *dhacc, *extr_acc;           /* (nx-1) */
+    const double *dhacc_correction, *q_init, *vel_flex_old; /* (nx) */
+    const double *fmagma, *fmagma2;           /* (nz-1, nx-1) */
+    const double *dike_backlog, *dike_marker_vol, *mor_marker_vol; /* (nz-1, nx-1) */
+    const double *mark_age;                   /* (nmarkers) */
+} gf_output_opts;
+int gf_outflac(gf_ctx *ctx, const gf_clock *clock, const char *dir, const gf_output_opts *o);  /* src/outflac.f90:5-431 */
+int gf_saveflac(gf_ctx *ctx, const gf_clock *clock, const char *dir, const gf_output_opts *o); /* src/saveflac.f90:5-172 */
 int gf_init_visc(gf_ctx *ctx);                   /* src/init_visc.f90:1-18  */
 
 /* Thermal update statistics since the last query (instrumentation; no reference

@@ -1361,12 +1361,13 @@ int gfo_flac(const gf_params *p, gf_host_arrays *s, gf_clock *c)
 
 /* sizes of the ABI structs as the C compiler lays them out (a check of the Python
  * and Fortran mirrors): gf_params, gf_host_arrays, gf_clock, gf_marker_arrays */
-void gfo_abi_sizes(long out[4])
+void gfo_abi_sizes(long out[5])
 {
     out[0] = (long)sizeof(gf_params);
     out[1] = (long)sizeof(gf_host_arrays);
     out[2] = (long)sizeof(gf_clock);
     out[3] = (long)sizeof(gf_marker_arrays);
+    out[4] = (long)sizeof(gf_output_opts);
 }
 
 /* the transcendental functions of the library (fn: 0 sin, 1 tan, 2 exp, 3 pow) */

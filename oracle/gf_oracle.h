@@ -32,7 +32,7 @@ int gfo_init_visc(const gf_params *p, gf_host_arrays *s);
 int gfo_flac(const gf_params *p, gf_host_arrays *s, gf_clock *c);
 /* nsteps iterations of src/par.f90:81-91 */
 int gfo_step(const gf_params *p, gf_host_arrays *s, gf_clock *c, int nsteps);
-void gfo_abi_sizes(long out[4]);
+void gfo_abi_sizes(long out[5]);
 /* the transcendental functions in use (fn: 0 sin, 1 tan, 2 exp, 3 pow) */
 double gfo_math(int fn, double x, double y);
 long gfo_math_fast(int fn, const double *x, const double *y, double *out, long n, long *nslow);

@@ -69,6 +69,16 @@ do istep = 1, nsteps
         call dump_state(outdir, istep)
     endif
 enddo
+! GF_REF_WRITE=1: the reference's own writers on the final state, into the working
+! directory (outflac's frames, saveflac's restart files)
+call get_environment_variable('GF_REF_WRITE', arg)
+if (trim(arg) == '1') then
+#ifdef GF_DROPIN
+    call flac_hip_sync_host
+#endif
+    call outflac
+    call saveflac
+endif
 #ifdef GF_DROPIN
 call flac_hip_finalize
 #endif

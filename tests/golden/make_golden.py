@@ -6,6 +6,8 @@ project) on the inputs in tests/golden/inputs, serially, and stores:
 
   <case>_s0.npz        the complete state after setflac (step 0)
   <case>_s<N>.npz      the fields the step changes, after N steps
+  output_<case>_s<N>.npz  the files the reference's own outflac and saveflac write
+                       after N steps (GF_REF_WRITE=1), each as its raw bytes
 
 Only run where the reference sources and oracle/_ref are present; the committed
 fixtures are what the tests read elsewhere.
@@ -49,6 +51,29 @@ CASES = {
 }
 
 
+# files outflac (src/outflac.f90) and saveflac (src/saveflac.f90) write; Amin.rs (a
+# list-directed scalar) and the files setflac writes are not among them
+OUTPUT_CASES = {"box60x20": 10, "hetero80x40": 10}
+SETUP_FILES = {"nxnz.0", "pisos.rs", "vbc.s", "output.asc", "Amin.rs"}
+
+
+def run_output(name, nsteps):
+    import numpy as np
+    inp = os.path.join(HERE, "inputs", CASES[name]["inp"])
+    with tempfile.TemporaryDirectory() as tmp:
+        out = os.path.join(tmp, "dump")
+        env = dict(os.environ, GF_REF_WRITE="1")
+        subprocess.run([HARNESS, inp, out, str(nsteps)], cwd=tmp, check=True, stdout=subprocess.DEVNULL, env=env)
+        files = {}
+        for f in sorted(os.listdir(tmp)):
+            path = os.path.join(tmp, f)
+            if os.path.isfile(path) and f not in SETUP_FILES:
+                with open(path, "rb") as fh:
+                    files[f] = np.frombuffer(fh.read(), dtype=np.uint8)
+        np.savez_compressed(os.path.join(HERE, "output_%s_s%d.npz" % (name, nsteps)), **files)
+        print(name, "output files", len(files))
+
+
 def run_case(name, spec):
     inp = os.path.join(HERE, "inputs", spec["inp"])
     steps = spec["steps"]
@@ -71,4 +96,9 @@ def run_case(name, spec):
 if __name__ == "__main__":
     names = sys.argv[1:] or list(CASES)
     for n in names:
-        run_case(n, CASES[n])
+        if n.startswith("output_"):
+            run_output(n[len("output_"):], OUTPUT_CASES[n[len("output_"):]])
+        else:
+            run_case(n, CASES[n])
+            if n in OUTPUT_CASES:
+                run_output(n, OUTPUT_CASES[n])

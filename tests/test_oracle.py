@@ -29,9 +29,10 @@ def test_params_layout_matches_header():
     assert abi.SIZEOF_PARAMS == 548 * 4 + (23 + 19 * abi.MAXPH + 1) * 8
     assert abi.SIZEOF_CLOCK == 8 + 5 * 8
     # the C compiler's layout of include/gf_abi.h (through the oracle, built against it)
-    out = (C.c_long * 4)()
+    out = (C.c_long * 5)()
     ol.lib().gfo_abi_sizes(out)
-    assert list(out) == [abi.SIZEOF_PARAMS, abi.SIZEOF_HOST, abi.SIZEOF_CLOCK, C.sizeof(abi.GfMarkerArrays)]
+    assert list(out) == [abi.SIZEOF_PARAMS, abi.SIZEOF_HOST, abi.SIZEOF_CLOCK, C.sizeof(abi.GfMarkerArrays),
+                         C.sizeof(abi.GfOutputOpts)]
 
 
 GOLDEN_RUNS = [("box60x20", n) for n in (1, 10, 100, 1000, 10000)] + \
