@@ -276,6 +276,58 @@ def mixed_run(gp, host_state, fp64_state, args, E, precision, learned=None):
     return out
 
 
+def mixed_report(args):
+    """Per-field error of both FP32 modes against the FP64 run after 100, 1000 and
+    10000 steps of the benchmark model, beside the FP64 run's own sensitivity: the same
+    FP64 run from an initial stress perturbed by one ulp (every stress value moved to
+    the next double up), compared after 100 and 1000 steps (the chaos floor)."""
+    import numpy as np
+    from geoflac_amd import abi, gpu, state
+    checkpoints = [c for c in (100, 1000, 10000) if c <= args.report_steps]
+    fields = ("vel", "stress0", "temp", "cord", "aps")
+    p, gp, st, m = build_model(args.config, abi.PREC_FP64, args.seed)
+    m.download()
+    host0 = st.copy()
+
+    def run(precision, start, points):
+        s = start.copy()
+        mm = gpu.Model(gp, s, precision=precision)
+        snaps, done, t = {}, 0, 0.0
+        for c in points:
+            mm.sync()
+            t0 = time.perf_counter()
+            mm.step(c - done)
+            mm.sync()
+            t += time.perf_counter() - t0
+            done = c
+            mm.download()
+            snaps[c] = {f: getattr(s, f).copy() for f in fields}
+            print("bench: mixed report, precision %d, %d steps" % (precision, c), file=sys.stderr, flush=True)
+        mm.close()
+        return snaps, t / max(done, 1) * 1e3
+
+    def errs(a, b):
+        out = {}
+        for f in fields:
+            r = state.rel_l2(a[f], b[f])
+            out[f] = {"rel_l2": float("%.3e" % r) if np.isfinite(r) else None,
+                      "max_abs": float("%.3e" % np.max(np.abs(a[f] - b[f])))}
+        return out
+
+    m.close()
+    ref, ms64 = run(abi.PREC_FP64, host0, checkpoints)
+    pert = host0.copy()
+    pert.stress0[...] = np.nextafter(pert.stress0, np.inf)
+    fl, _ = run(abi.PREC_FP64, pert, [c for c in checkpoints if c <= 1000])
+    out = {"config": args.config, "seed": args.seed, "fp64_ms_per_step": round(ms64, 4),
+           "fp64_one_ulp_floor": {str(c): errs(fl[c], ref[c]) for c in fl}}
+    for name, prec in (("fp32_arithmetic", abi.PREC_MIXED), ("fp32_storage", abi.PREC_MIXED_STORAGE)):
+        mx, ms = run(prec, host0, checkpoints)
+        out[name] = {"ms_per_step": round(ms, 4), "speedup_vs_fp64": round(ms64 / ms, 3),
+                     "error_vs_fp64": {str(c): errs(mx[c], ref[c]) for c in checkpoints}}
+    print(json.dumps(out), flush=True)
+
+
 def sweep_run(args):
     """Batched parameter sweep: seeded variants of the 60x20 box (mesh nx 100-600 and
     nz 40-150 elements over the same domain, friction 10-35 deg, cohesion 10-60 MPa,
@@ -330,7 +382,6 @@ def main():
                     help="skip the mixed-precision runs (FP32 stress/strain rate/force/velocity)")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-viscosity run (surrogate trained on this model's samples)")
-    ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched parameter sweep")
     ap.add_argument("--sweep-models", type=int, default=256)
     ap.add_argument("--sweep-steps", type=int, default=2000)
@@ -338,6 +389,10 @@ def main():
                     help="models of the sweep also run alone in FP64 for the error report")
     ap.add_argument("--seed", type=int, default=0,
                     help="seed of the layer-interface offsets and the per-element cohesion factors")
+    ap.add_argument("--mixed-report", action="store_true",
+                    help="only the FP32 modes' per-field error after 100/1000/10000 steps and the FP64 one-ulp "
+                         "floor (one JSON line)")
+    ap.add_argument("--report-steps", type=int, default=10000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="short copy probe only (for profiler counter runs, where k_copy over 1 GiB is the "
                          "kernel of known traffic that scales FETCH_SIZE/WRITE_SIZE)")
@@ -347,6 +402,9 @@ def main():
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    if args.mixed_report:
+        mixed_report(args)
+        return
     # the reference's OpenMP program on the host cores, before this process touches the GPU
     ref_cpu = None if args.no_cpu_baseline else ref_cpu_baseline(args.config)
 
@@ -357,16 +415,7 @@ def main():
     m.download()
     host_state = st.copy()  # complete new-start state, for the CPU baseline
 
-    # warm-up; the viscosity samples of the surrogate are logged along the way
-    n_samples = 0
-    if args.no_learned:
-        m.step(args.warmup)
-    else:
-        snaps = 3  # three snapshots in the warm-up; a fourth after the timed steps
-        for k in range(snaps):
-            m.step(max(1, args.warmup // snaps))
-            n_samples = m.surrogate_sample((snaps + 1) * E)
-        m.step(max(0, args.warmup - snaps * max(1, args.warmup // snaps)))
+    m.step(args.warmup)
     m.sync()
     t0 = time.perf_counter()
     m.step(args.steps)       # returns after the device finished (clock read-back)
@@ -376,15 +425,17 @@ def main():
     value = E * args.steps / el
     m.download()
     fp64_state = st.copy()   # after warmup + steps, for the mixed-precision comparison
-    learned_w = None
+    # the viscosity network: trained beforehand on a separate run (a different seed),
+    # stored by geoflac_amd/train_surrogate.py
+    learned_w, learned_note = None, None
     if not args.no_learned:
-        n_samples = m.surrogate_sample(4 * E)
-        t_tr = time.perf_counter()
-        loss0, loss1 = m.surrogate_train(args.train_steps, batch=8192, lr=2e-3, seed=1)
-        m.sync()
-        learned_w = m.surrogate_weights() + ({"samples": n_samples, "train_steps": args.train_steps, "batch": 8192,
-                                              "loss_first": loss0, "loss_last": loss1,
-                                              "train_s": round(time.perf_counter() - t_tr, 3)},)
+        from geoflac_amd import train_surrogate
+        wpath = os.path.join(ROOT, "geoflac_amd", "weights", "visc_mlp_%s.npz" % args.config)
+        if os.path.exists(wpath):
+            w, wcfg, meta = train_surrogate.load(wpath)
+            learned_w = (w, wcfg, dict(meta, weights=os.path.relpath(wpath, ROOT)))
+        else:
+            learned_note = "no stored network for %s (python -m geoflac_amd.train_surrogate)" % args.config
 
     # profiled run: HIP events around every launch on the library stream
     args.profile_steps = max(args.profile_steps, 1)
@@ -430,8 +481,16 @@ def main():
                  "fp32_storage": mixed_run(gp, host_state, fp64_state, args, E, abi.PREC_MIXED_STORAGE)}
     if learned_w is not None:
         mixed = mixed or {}
-        mixed["fp32_arithmetic_learned_visc"] = mixed_run(gp, host_state, fp64_state, args, E, abi.PREC_MIXED,
-                                                          learned=learned_w)
+        lr = mixed_run(gp, host_state, fp64_state, args, E, abi.PREC_MIXED, learned=learned_w)
+        ex = mixed.get("fp32_arithmetic")
+        if ex:  # the network's viscosity update against the exact one of the same mode
+            lr["visc_update_gain_vs_exact"] = round(ex["viscosity_update_ms_per_call"]["total"] /
+                                                    lr["viscosity_update_ms_per_call"]["total"], 3)
+            lr["step_gain_vs_exact"] = round(lr["value"] / ex["value"], 3)
+        mixed["fp32_arithmetic_learned_visc"] = lr
+    elif learned_note:
+        mixed = mixed or {}
+        mixed["fp32_arithmetic_learned_visc"] = {"skipped": learned_note}
     # measured copy bandwidth (k_copy over 1 GiB: 2 GiB of traffic per launch)
     copy_ms = gpu.stream_probe(1 << 30, 10 if not args.copy_calibration else 3)
     roofline["copy_bw_GBs"] = round(2.0 * (1 << 30) / (copy_ms * 1e-3) / 1e9, 1)
@@ -450,7 +509,8 @@ def main():
            "roofline": roofline}
     if mixed:
         for v in mixed.values():
-            v["speedup_vs_fp64"] = round(v["value"] / value, 3)
+            if "value" in v:
+                v["speedup_vs_fp64"] = round(v["value"] / value, 3)
         out["mixed"] = mixed
     if not args.no_sweep:
         out["sweep"] = sweep_run(args)

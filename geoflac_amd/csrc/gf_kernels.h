@@ -1084,6 +1084,9 @@ GF_DEV ViscIn visc_inputs(const GfDevParams *P, const GfDevArrays &A, int j, int
     v.srat = A.e2sr[IE(j, i)];
     return v;
 }
+// CR: the correctly rounded exp and pow of the FP64 path; false (the FP32 modes, whose
+// step is not bitwise the reference's anyway): the device library's
+template <bool CR = true>
 GF_DEV double dev_eff_visc_in(const GfDevParams *P, int j, int i, double s11, double s22, double s33,
                               const double *prv, const ViscIn &v)
 {
@@ -1101,10 +1104,10 @@ GF_DEV double dev_eff_visc_in(const GfDevParams *P, int j, int i, double s11, do
     double deptmpr410 = -101.58 * (tmpr + 273.0) - 216828.0;
     double deptmpr660 = 66.55 * (tmpr + 273.0) - 788494.0;
     if (zcord < deptmpr660) {
-        ev = 1e22 * cr_exp(-0.0115 * (tmpr - P->t_bot));
+        ev = 1e22 * (CR ? cr_exp(-0.0115 * (tmpr - P->t_bot)) : exp(-0.0115 * (tmpr - P->t_bot)));
         return fmin(P->v_max, fmax(P->v_min, ev));
     } else if (zcord < deptmpr410) {
-        ev = 1e21 * cr_exp(-0.0115 * (tmpr - P->t_bot));
+        ev = 1e21 * (CR ? cr_exp(-0.0115 * (tmpr - P->t_bot)) : exp(-0.0115 * (tmpr - P->t_bot)));
         return fmin(P->v_max, fmax(P->v_min, ev));
     }
     double srat = v.srat;
@@ -1113,8 +1116,9 @@ GF_DEV double dev_eff_visc_in(const GfDevParams *P, int j, int i, double s11, do
         double r = prv[a];
         if (r < 0.01) continue;
         const GfPhase &ph = PHASE(P->active[a]);
-        double vis = 0.25 * cr_pow(srat, ph.pw) * ph.acoef_pow *
-                     cr_exp((ph.eactiv + ph.vactiv * pres) / (ph.pln_r * (tmpr + 273.0))) * 1.0e+6;
+        const double arg = (ph.eactiv + ph.vactiv * pres) / (ph.pln_r * (tmpr + 273.0));
+        double vis = 0.25 * (CR ? cr_pow(srat, ph.pw) : pow(srat, ph.pw)) * ph.acoef_pow *
+                     (CR ? cr_exp(arg) : exp(arg)) * 1.0e+6;
         if (vis < P->v_min) vis = P->v_min;
         if (vis > P->v_max) vis = P->v_max;
         ev = ev + r / vis;
@@ -1978,7 +1982,7 @@ GF_DEV void visc_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const G
     if (rot) move_dw_elem<M>(P, ck, A, j, i, dw);
     visc_load<SV, AT>(P, A, e, j, i, iph, prv, a11, a22, a33, vin, rot != 0, dw);
     if (!visc_rheology(PHASE(iph).irheol)) return;
-    A.visn[e] = dev_eff_visc_in(P, j, i, a11, a22, a33, prv, vin);
+    A.visn[e] = dev_eff_visc_in<M == 0>(P, j, i, a11, a22, a33, prv, vin);
 }
 template <int M>
 __global__ void __launch_bounds__(256) k_visc(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A, int rot)

@@ -613,7 +613,7 @@ __global__ void __launch_bounds__(256) k_visc_list(const GfDevParams *__restrict
         double dw[4] = {0.0, 0.0, 0.0, 0.0};
         if (rot) move_dw_elem<M>(P, ck, A, j, i, dw);
         visc_load<SV, AT>(P, A, e, j, i, iph, prv, a11, a22, a33, vin, rot != 0, dw);
-        const double ex = dev_eff_visc_in(P, j, i, a11, a22, a33, prv, vin);
+        const double ex = dev_eff_visc_in<M == 0>(P, j, i, a11, a22, a33, prv, vin);
         A.visn[e] = ex;
         if (item & MLP_LIST_AUDIT) {
             const float err = (float)fabs(log10(mlp_visc(P, lval[at])) - log10(ex));
