@@ -178,6 +178,68 @@ rc = gf_set_clock(gf_ctx_handle, ck)
 if (rc /= GF_OK) call flac_hip_fail(rc)
 end subroutine flac_hip_sync_device
 
+! The reference's output and restart writers (src/outflac.f90, src/saveflac.f90) from
+! the device state, into the working directory, without copying the state back:
+! outflac's frames are evaluated on the device.  The arrays the explicit step does
+! not change come from the Fortran modules.  Amin.rs, a list-directed scalar of
+! setflac, is still written by the caller's own code.
+subroutine flac_hip_output_opts(o)
+use iso_c_binding
+use gf_hip
+use arrays
+use params
+use marker_data
+implicit none
+type(gf_output_opts_t), intent(out) :: o
+o%io_vel = io_vel; o%io_srII = io_srII; o%io_eII = io_eII; o%io_aps = io_aps; o%io_sII = io_sII
+o%io_sxx = io_sxx; o%io_szz = io_szz; o%io_sxz = io_sxz; o%io_pres = io_pres; o%io_temp = io_temp
+o%io_phase = io_phase; o%io_visc = io_visc; o%io_unused = io_unused; o%io_density = io_density
+o%io_src = io_src; o%io_diss = io_diss; o%io_forc = io_forc; o%io_hfl = io_hfl; o%io_topo = io_topo
+o%lastout = lastout
+o%hs = hs
+o%xoriginal = gf_addr8(xoriginal); o%zoriginal = gf_addr8(zoriginal)
+o%dhacc = gf_addr8(dhacc); o%extr_acc = gf_addr8(extr_acc); o%dhacc_correction = gf_addr8(dhacc_correction)
+o%q_init = gf_addr8(q_init); o%vel_flex_old = gf_addr8(vel_flex_old)
+o%fmagma = gf_addr8(fmagma); o%fmagma2 = gf_addr8(fmagma2)
+o%dike_backlog = gf_addr8(dike_backlog); o%dike_marker_vol = gf_addr8(dike_marker_vol)
+o%mor_marker_vol = gf_addr8(mor_marker_vol)
+o%mark_age = gf_addr8(mark_age)
+end subroutine flac_hip_output_opts
+
+subroutine flac_hip_outflac
+use iso_c_binding
+use gf_hip
+use flac_hip_state
+use params
+implicit none
+type(gf_output_opts_t) :: o
+type(gf_clock_t) :: ck
+integer(c_int) :: rc
+if (.not. c_associated(gf_ctx_handle)) call flac_hip_init
+call flac_hip_output_opts(o)
+ck%nloop = nloop; ck%nsrate = nsrate; ck%time = time; ck%dt = dt; ck%dtavg = dtavg
+ck%dt_elastic = dt_elastic; ck%dt_maxwell = dt_maxwell
+rc = gf_outflac(gf_ctx_handle, ck, '.'//c_null_char, o)
+if (rc /= GF_OK) call flac_hip_fail(rc)
+end subroutine flac_hip_outflac
+
+subroutine flac_hip_saveflac
+use iso_c_binding
+use gf_hip
+use flac_hip_state
+use params
+implicit none
+type(gf_output_opts_t) :: o
+type(gf_clock_t) :: ck
+integer(c_int) :: rc
+if (.not. c_associated(gf_ctx_handle)) call flac_hip_init
+call flac_hip_output_opts(o)
+ck%nloop = nloop; ck%nsrate = nsrate; ck%time = time; ck%dt = dt; ck%dtavg = dtavg
+ck%dt_elastic = dt_elastic; ck%dt_maxwell = dt_maxwell
+rc = gf_saveflac(gf_ctx_handle, ck, '.'//c_null_char, o)
+if (rc /= GF_OK) call flac_hip_fail(rc)
+end subroutine flac_hip_saveflac
+
 subroutine flac_hip_finalize
 use iso_c_binding
 use gf_hip

@@ -61,6 +61,15 @@ type, bind(C) :: gf_clock_t
     real(c_double) :: time, dt, dtavg, dt_elastic, dt_maxwell
 end type
 
+type, bind(C) :: gf_output_opts_t
+    integer(c_int32_t) :: io_vel, io_srII, io_eII, io_aps, io_sII, io_sxx, io_szz, io_sxz, io_pres, io_temp, io_phase
+    integer(c_int32_t) :: io_visc, io_unused, io_density, io_src, io_diss, io_forc, io_hfl, io_topo
+    integer(c_int32_t) :: lastout
+    real(c_double) :: hs
+    type(c_ptr) :: xoriginal, zoriginal, dhacc, extr_acc, dhacc_correction, q_init, vel_flex_old
+    type(c_ptr) :: fmagma, fmagma2, dike_backlog, dike_marker_vol, mor_marker_vol, mark_age
+end type
+
 interface
     integer(c_int) function gf_create(p, precision, ctx) bind(C, name='gf_create')
         import :: c_int, c_int32_t, c_ptr, gf_params_t
@@ -121,6 +130,20 @@ interface
         import :: c_int, c_ptr, gf_clock_t
         type(c_ptr), value :: ctx
         type(gf_clock_t), intent(inout) :: c
+    end function
+    integer(c_int) function gf_outflac(ctx, c, dir, o) bind(C, name='gf_outflac')
+        import :: c_int, c_ptr, c_char, gf_clock_t, gf_output_opts_t
+        type(c_ptr), value :: ctx
+        type(gf_clock_t), intent(in) :: c
+        character(kind=c_char), dimension(*), intent(in) :: dir
+        type(gf_output_opts_t), intent(in) :: o
+    end function
+    integer(c_int) function gf_saveflac(ctx, c, dir, o) bind(C, name='gf_saveflac')
+        import :: c_int, c_ptr, c_char, gf_clock_t, gf_output_opts_t
+        type(c_ptr), value :: ctx
+        type(gf_clock_t), intent(in) :: c
+        character(kind=c_char), dimension(*), intent(in) :: dir
+        type(gf_output_opts_t), intent(in) :: o
     end function
     integer(c_int) function gf_sync(ctx) bind(C, name='gf_sync')
         import :: c_int, c_ptr

@@ -194,7 +194,7 @@ int gf_init_areas(gf_ctx *ctx);                  /* src/init_areas.f90:7-55 */
  * are the caller's (host state of the reference's arrays module); a null pointer
  * writes zeros.  The clock gives nloop, time, dt and dtavg.  FP64 contexts only.
  * Amin.rs (a list-directed scalar of setflac) stays with the caller. */
-typedef struct {
+typedef struct gf_output_opts {
     /* frame switches and lastout, src/read_params.f90:329-333 */
     int32_t io_vel, io_srII, io_eII, io_aps, io_sII, io_sxx, io_szz, io_sxz, io_pres, io_temp, io_phase,
         io_visc, io_unused, io_density, io_src, io_diss, io_forc, io_hfl, io_topo;

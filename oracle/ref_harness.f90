@@ -80,6 +80,12 @@ if (trim(arg) == '1') then
     call saveflac
 endif
 #ifdef GF_DROPIN
+! GF_HIP_WRITE=1: the drop-in's writers from the device state instead
+call get_environment_variable('GF_HIP_WRITE', arg)
+if (trim(arg) == '1') then
+    call flac_hip_outflac
+    call flac_hip_saveflac
+endif
 call flac_hip_finalize
 #endif
 close(333)

@@ -31,7 +31,7 @@ def test_fortran_interface_mirrors_c_structs():
         body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
         out = []
         for decl in body.split(";"):
-            decl = decl.strip()
+            decl = re.sub(r"^const\s+", "", decl.strip())
             if not decl:
                 continue
             names = decl.split(None, 1)[1]
@@ -51,6 +51,7 @@ def test_fortran_interface_mirrors_c_structs():
     assert c_fields("gf_host_arrays") == f_fields("gf_host_arrays_t")
     assert c_fields("gf_clock") == f_fields("gf_clock_t")
     assert c_fields("gf_marker_arrays") == f_fields("gf_marker_arrays_t")
+    assert c_fields("gf_output_opts") == f_fields("gf_output_opts_t")
 
 
 @pytest.mark.gpu
@@ -82,3 +83,31 @@ def test_fortran_dropin_matches_reference(tmp_path, case, dumps):
         assert max(worst.values()) <= TOL, worst
         assert st.clock.nloop == ref.clock.nloop
         assert abs(st.clock.dt - ref.clock.dt) <= 1e-12 * ref.clock.dt
+
+
+@pytest.mark.gpu
+def test_fortran_dropin_writes_the_reference_files(tmp_path):
+    """The drop-in's writers (flac_hip_outflac / flac_hip_saveflac: outflac's frames
+    evaluated on the device, saveflac's dumps, without copying the state to the
+    Fortran arrays) after 10 steps of box60x20 write the reference writers' files byte
+    for byte (tests/golden/output_box60x20_s10.npz)."""
+    if not os.path.exists(EXE):
+        pytest.skip("oracle/_ref/dropin_harness is built only where the reference sources are present")
+    inp = os.path.join(ROOT, "tests", "golden", "inputs", "box60x20.inp")
+    print("case: Fortran drop-in writers after 10 steps of box60x20", flush=True)
+    env = dict(os.environ, GF_HIP_WRITE="1")
+    r = subprocess.run([EXE, inp, str(tmp_path / "out"), "10"], cwd=str(tmp_path), capture_output=True, text=True,
+                       timeout=300, env=env)
+    assert r.returncode == 0, r.stderr[-2000:]
+    ref = dict(np.load(os.path.join(ROOT, "tests", "golden", "output_box60x20_s10.npz")))
+    bad = {}
+    for name, want in sorted(ref.items()):
+        path = tmp_path / name
+        if not path.exists():
+            bad[name] = "missing"
+            continue
+        mine = np.frombuffer(path.read_bytes(), dtype=np.uint8)
+        if mine.size != want.size or not np.array_equal(mine, want):
+            bad[name] = "differs"
+    print("  %d files, differing %s" % (len(ref), bad), flush=True)
+    assert not bad, bad
