@@ -8,9 +8,13 @@ loop (src/par.f90:81-91: flac; nloop += 1; time += dt).  Prints one JSON line.
 
 roofline: per-kernel durations are measured with HIP events recorded on the
 library's own stream (gf_set_timing / gf_kernel_time) over a separate profiled
-run; the dominant kernel's algorithmic bytes per launch (table ALG_BYTES) over its
-average duration is `achieved`.  cpu_baseline: the C oracle (serial restatement
-of the reference step) on a bounded sample of the same model.
+run; the dominant kernel's algorithmic bytes per launch (alg_bytes) over its
+average duration is `achieved`, and `traffic` its counter-measured bytes
+(profiles/traffic.json).  cpu_baseline: the reference's own OpenMP program on the
+host cores, on the benchmark deck cut to a bounded number of columns, timed before
+this process touches the GPU; the C oracle on one core is a second field.
+--mixed-report: the FP32 modes' per-field errors after 100, 1000 and 10000 steps
+beside the FP64 run's one-ulp floor.
 """
 import argparse
 import ctypes as C
