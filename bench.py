@@ -82,10 +82,10 @@ def alg_bytes(kernel, E, N, nact, p, area_frac=0.0):
     if kernel == "k_area_check":
         return N * 16                                                  # cord (the move's negative-area test)
     if kernel == "k_therm_check":
-        # the border check: 76 border values of each 64x16-node tile (52x12 outputs)
+        # the border check: 78 border values of each 64x16-node tile (54x12 outputs)
         # against the assembled iterate; fallback passes are added where they ran
-        ntiles = -(-p.nz // 52) * -(-p.nx // 12)
-        return ntiles * 76 * 16
+        ntiles = -(-p.nz // 54) * -(-p.nx // 12)
+        return ntiles * 78 * 16
     return 0
 
 
