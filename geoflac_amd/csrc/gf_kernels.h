@@ -6,6 +6,7 @@
 // against the cited source line; every scatter of the reference is a gather here
 // (no atomics on field data), in the reference's summation order.
 #pragma once
+#include <stddef.h>
 #include <type_traits>
 #include "gf_device.h"
 
@@ -2306,6 +2307,30 @@ struct NodeLds {
     double pr[M != 0 ? ND_NEL : 1];
     double cx[ND_NND], cy[ND_NND];
 };
+// (plane p, row sj, column si) of the flat index v = threadIdx.x + 256 r over planes of
+// NJ rows x NI columns, advanced round by round without a division (256 = DI NJ + DJ;
+// a round wraps at most one row and one plane)
+template <int NJ, int NI>
+struct TileWalk {
+    static_assert(256 < NJ * NI && 256 / NJ + 1 <= NI, "one wrap per round");
+    int p, sj, si;
+    __device__ __forceinline__ TileWalk() : p(0), sj((int)threadIdx.x % NJ), si((int)threadIdx.x / NJ) {}
+    __device__ __forceinline__ void next()
+    {
+        sj += 256 % NJ;
+        si += 256 / NJ;
+        if (sj >= NJ) {
+            sj -= NJ;
+            si += 1;
+        }
+        if (si >= NI) {
+            si -= NI;
+            p += 1;
+        }
+    }
+};
+static_assert(offsetof(NodeLds<0>, cy) == offsetof(NodeLds<0>, cx) + sizeof(double) * ND_NND, "cx, cy adjacent");
+static_assert(offsetof(NodeLds<1>, cy) == offsetof(NodeLds<1>, cx) + sizeof(double) * ND_NND, "cx, cy adjacent");
 GF_DEV int node_ntj(const GfDevParams *P) { return (P->nz + ND_TJ - 1) / ND_TJ; }
 GF_DEV int node_nti(const GfDevParams *P) { return (P->nx + ND_TI - 1) / ND_TI; }
 // one thread's share of a tile's inputs: element values v = tid + 256 r (plane p = v /
@@ -2343,10 +2368,11 @@ GF_DEV NodePf<M> node_prefetch(const GfDevParams *__restrict__ P, const GfDevArr
     const Buf bS = mkbuf(sv_ptr<SV>(A.stress0, A.stress0f), 16ull * sizeof(SV) * neu);
     constexpr unsigned OOB = 0x80000000u;  // an offset past every array (reads 0)
     F f;
+    TileWalk<ND_EJ, ND_TI + 1> we;
 #pragma unroll
-    for (int r = 0; r < F::NR; r++) {
-        const int v = threadIdx.x + 256 * r, p = v / ND_NEL, sl = v % ND_NEL;
-        const int je = j0 - 1 + sl % ND_EJ, ie = i0 - 1 + sl / ND_EJ;
+    for (int r = 0; r < F::NR; r++, we.next()) {
+        const int v = threadIdx.x + 256 * r, p = we.p;
+        const int je = j0 - 1 + we.sj, ie = i0 - 1 + we.si;
         const bool ok = v < F::NV && je >= 1 && je <= nz - 1 && ie >= 1 && ie <= nx - 1;
         const unsigned e = (unsigned)((je - 1) + (nz - 1) * (ie - 1));
         f.dv[r] = 0.0;
@@ -2356,10 +2382,11 @@ GF_DEV NodePf<M> node_prefetch(const GfDevParams *__restrict__ P, const GfDevArr
         else if (M != 0 && ok)
             f.dv[r] = ref_press<SV>(A, (long)e);
     }
+    TileWalk<ND_NJ, ND_TI + 2> wn;
 #pragma unroll
-    for (int r = 0; r < F::NC; r++) {
-        const int v = threadIdx.x + 256 * r, p = v / ND_NND, sl = v % ND_NND;
-        const int jj = j0 - 1 + sl % ND_NJ, ii = i0 - 1 + sl / ND_NJ;
+    for (int r = 0; r < F::NC; r++, wn.next()) {
+        const int v = threadIdx.x + 256 * r, p = wn.p;
+        const int jj = j0 - 1 + wn.sj, ii = i0 - 1 + wn.si;
         const bool ok = v < 2 * ND_NND && jj >= 1 && jj <= nz && ii >= 1 && ii <= nx;
         f.cv[r] = bld(bC, ok ? (unsigned)((jj - 1) + nz * (ii - 1)) * 8u : OOB, (unsigned)(p & 1) * nnu * 8u);
     }
@@ -2385,22 +2412,23 @@ GF_DEV void node_stage(const NodePf<M> &f, NodeLds<M> &L)
 {
     typedef typename Prec<M>::SV SV;
     typedef NodePf<M> F;
+    // (the 12 stress planes are one array: slot v of plane p is element v of it; the
+    // isotropic reference follows them; cx and cy likewise)
+    SV *ls = &L.s[0][0];
+    double *lc = &L.cx[0];
 #pragma unroll
     for (int r = 0; r < F::NR; r++) {
-        const int v = threadIdx.x + 256 * r, p = v / ND_NEL, sl = v % ND_NEL;
+        const int v = threadIdx.x + 256 * r;
         if (v >= F::NV) continue;
-        if (p < 12)
-            L.s[p][sl] = SV(f.dv[r]);
+        if (v < 12 * ND_NEL)
+            ls[v] = SV(f.dv[r]);
         else if (M != 0)
-            L.pr[sl] = f.dv[r];
+            L.pr[v - 12 * ND_NEL] = f.dv[r];
     }
 #pragma unroll
     for (int r = 0; r < F::NC; r++) {
-        const int v = threadIdx.x + 256 * r, sl = v % ND_NND;
-        if (v < ND_NND)
-            L.cx[sl] = f.cv[r];
-        else if (v < 2 * ND_NND)
-            L.cy[sl] = f.cv[r];
+        const int v = threadIdx.x + 256 * r;
+        if (v < 2 * ND_NND) lc[v] = f.cv[r];
     }
 }
 // the update of the thread's node of `tile` from the staged values
