@@ -77,7 +77,8 @@ def alg_bytes(kernel, E, N, nact, p, area_frac=0.0):
         return E * (32 + 8 + 4) + N * 32                               # area visn iphase | cord vel
     if kernel == "k_move_cord":
         return N * 48
-    if kernel == "k_therm_bc":
+    if kernel in ("k_therm_bc", "k_therm_tail"):
+        # the boundary conditions (a fallback pass inside k_therm_tail is added where it ran)
         return N * 16
     if kernel == "k_area_check":
         return N * 16                                                  # cord (the move's negative-area test)
@@ -450,7 +451,7 @@ def main():
     afrac = area_pending_fraction(gp, n0, args.profile_steps)
     kernels = ["k_rheol1<false>", "k_rheol1<true>", "k_rheol", "k_visc<double>", "k_therm_pipe",
                "k_therm_pipe_extra", "k_therm_check", "k_node<0>", "k_node_t<0>", "k_move_elem<0>",
-               "k_move_area", "k_move_rot<0>", "k_move_cord<double>", "k_therm_bc", "k_dens<double>", "k_rmass", "k_dt_reduce<double>", "k_amass",
+               "k_move_area", "k_move_rot<0>", "k_move_cord<double>", "k_therm_bc", "k_therm_tail", "k_dens<double>", "k_rmass", "k_dt_reduce<double>", "k_amass",
                "k_area_check", "k_clock"]
     times = {k: m.kernel_time(k) for k in kernels}
     m.set_timing(False)

@@ -69,6 +69,10 @@ struct GfDevClock {
     int ph_map[GF_MAXACT];      // new plane of each old plane (-1: dropped)
     int ph_nz[GF_MAXACT];       // elements with a nonzero ratio, per plane
     int ph_count[GF_MAXPH];     // elements whose iphase is id-1 (the dt_mass minima run over ids present)
+    // grid barrier of the thermal step's fallback passes: arrivals, generation, a
+    // bounded wait that ran out
+    unsigned int bar_cnt, bar_gen;
+    int bar_timeout, bar_pad;
 };
 
 // ---------------------------------------------------------------- buffer access

@@ -646,7 +646,7 @@ GF_DEV void therm_tile_origin(const GfDevParams *P, int tile, int &j0, int &i0)
 // point is the same).  DIS: the shear heating is loaded
 template <int XIN, bool DIS>
 GF_DEV ThermPf therm_prefetch(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, int pass,
-                              int tile, int arp = 0)
+                              int tile, int arp = 0, int gate = 1)
 {
     int j0, i0;
     therm_tile_origin(P, tile, j0, i0);
@@ -666,7 +666,7 @@ GF_DEV ThermPf therm_prefetch(const GfDevParams *__restrict__ P, GfDevClock *ck,
     f.act = 1;
     if (threadIdx.x == 0) {
         f.dtv = ck->dt;
-        f.act = (pass > 0 && ck->therm_pass_changed[pass - 1] == 0u) ? 0 : 1;
+        f.act = (gate && pass > 0 && ck->therm_pass_changed[pass - 1] == 0u) ? 0 : 1;
     }
     asm volatile("" ::"v"(f.phv), "v"(f.dtv), "v"(f.act));
 #pragma unroll
@@ -734,7 +734,7 @@ GF_DEV void therm_store_raw(const ThermPf &f, ThermRaw &R)
 template <int XIN, bool DIS>
 GF_DEV void therm_tile_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, int pass,
                             int tile, const GfDevParams *__restrict__ Pn, GfDevClock *ckn, const GfDevArrays &An,
-                            int next, ThermRaw &R, double *sU, int *sFlag, double stol, int arp = 0)
+                            int next, ThermRaw &R, double *sU, int *sFlag, double stol, int arp = 0, int gate = 1)
 {
     double *__restrict__ xout = (pass % 2 == 0) ? A.xa : A.xb;
     double *__restrict__ bord = A.tbord;
@@ -749,7 +749,7 @@ GF_DEV void therm_tile_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, c
     }
     if (!R.active) {  // a proven model (passes > 0): nothing to do but the prefetch
         lds_barrier();
-        if (next >= 0) therm_store_raw<XIN, DIS>(therm_prefetch<XIN, DIS>(Pn, ckn, An, pass, next, arp), R);
+        if (next >= 0) therm_store_raw<XIN, DIS>(therm_prefetch<XIN, DIS>(Pn, ckn, An, pass, next, arp, gate), R);
         return;
     }
     const double(*sPh)[GF_MAXPH] = R.ph;
@@ -885,7 +885,7 @@ GF_DEV void therm_tile_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, c
     const double dt = R.dt;
     asm volatile("" ::"v"(dt));
     ThermPf pf;
-    if (next >= 0) pf = therm_prefetch<XIN, DIS>(Pn, ckn, An, pass, next, arp);  // R is free now
+    if (next >= 0) pf = therm_prefetch<XIN, DIS>(Pn, ckn, An, pass, next, arp, gate);  // R is free now
 
     const int me = nb, up = nb - 1, lf = nb - TH_LJ, dl = nb - TH_LJ + 1;
     int cur = 0, nsweep = 0;
@@ -978,9 +978,9 @@ __global__ void __launch_bounds__(1024) k_therm_pipe(const GfDevParams *__restri
 // rows' rounding still show up as a few ulp there.  Any remaining difference (or a
 // tile that did not settle) sends the next pass.
 GF_DEV void therm_check_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, int pass,
-                             long t, int arp = 0)
+                             long t, int arp = 0, int gate = 1)
 {
-    if (pass > 0 && ck->therm_pass_changed[pass - 1] == 0u) return;
+    if (gate && pass > 0 && ck->therm_pass_changed[pass - 1] == 0u) return;
     const double *__restrict__ x = (pass % 2 == 0) ? A.xa : A.xb;
     const double *__restrict__ bord = A.tbord;
     const int ntj = therm_ntj(P);
@@ -1029,15 +1029,12 @@ __global__ void __launch_bounds__(256) k_therm_check(const GfDevParams *__restri
 // proof (FP64), and a call that no pass proved raises GF_ERR_THERM_SWEEPS (its last
 // pass is used, but the result is not known to be the serial one); without proof the
 // single tile pass is taken as it is (FP32-arithmetic mode)
-GF_DEV void therm_bc_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long t,
-                          int passes, int proof)
+// proven: the first pass whose border check found no difference (-1: none)
+GF_DEV void therm_bc_apply(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long t,
+                           int passes, int proof, int proven)
 {
     const double *__restrict__ xa = A.xa, *__restrict__ xb = A.xb;
-    // pass p writes xa when p is even, xb when odd; the first pass whose border check
-    // found no difference holds the fixed point
-    int proven = -1;
-    for (int k = 0; k < passes; k++)
-        if (ck->therm_pass_changed[k] == 0u) { proven = k; break; }
+    // pass p writes xa when p is even, xb when odd
     if (!proof) proven = 0;
     const double *x = (proven < 0 ? (passes - 1) : proven) % 2 == 0 ? xa : xb;
     if (t == 0 && (proven < 0 || (!proof && ck->therm_pass_changed[0] != 0u)))
@@ -1069,10 +1066,107 @@ GF_DEV void therm_bc_body(const GfDevParams *__restrict__ P, GfDevClock *ck, con
     A.temp[t] = v;
 }
 
+GF_DEV void therm_bc_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long t,
+                          int passes, int proof)
+{
+    int proven = -1;
+    for (int k = 0; k < passes; k++)
+        if (ck->therm_pass_changed[k] == 0u) {
+            proven = k;
+            break;
+        }
+    therm_bc_apply(P, ck, A, t, passes, proof, proven);
+}
+
 __global__ void __launch_bounds__(256) k_therm_bc(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
                                                   int passes, int proof)
 {
     therm_bc_body(P, ck, A, xcd_lin(), passes, proof);
+}
+
+// Grid-wide barrier of a launch whose workgroups are all resident.  Every wave
+// waits for its own memory operations; thread 0 then publishes the workgroup's writes
+// (agent-scope release), counts in, waits for the last workgroup to bump the
+// generation, and acquires.  The wait is bounded: a workgroup that polls GF_BAR_SPIN
+// times raises bar_timeout and returns false, so the launch always ends (the host
+// then reports the call as failed).
+#define GF_BAR_SPIN (1 << 22)
+GF_DEV bool grid_sync(GfDevClock *ck)
+{
+    __shared__ int s_ok;
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        const unsigned g0 = __hip_atomic_load(&ck->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const unsigned arrived =
+            __hip_atomic_fetch_add(&ck->bar_cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+        if (arrived == gridDim.x) {
+            __hip_atomic_store(&ck->bar_cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&ck->bar_gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            int spin = 0;
+            while (__hip_atomic_load(&ck->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g0) {
+                if (++spin > GF_BAR_SPIN) {
+                    ok = 0;
+                    __hip_atomic_fetch_or(&ck->bar_timeout, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+// The rest of fl_therm after pass 0 and its border check, in one launch (one
+// workgroup per CU, at most the CUs).  When pass 0 is proven (nearly every step) this is the
+// boundary conditions alone.  Otherwise the fallback passes run here, each over all
+// tiles with the launch's workgroups and followed by its border check, with grid
+// barriers between the phases, until one is proven or TH passes ran; then the
+// boundary conditions.  The same arithmetic as the launch-per-phase sequence
+// (k_therm_pipe, k_therm_check, k_therm_bc).  The flags that decide another pass are
+// read with coherent loads after a barrier, so every workgroup takes the same
+// decision; flag 0 was written by earlier launches.
+template <bool DIS>
+__global__ void __launch_bounds__(1024) k_therm_tail(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
+                                                     double stol, int arp, int npass, int proof, long nchk)
+{
+    const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x, nthr = (long)gridDim.x * blockDim.x;
+    int proven = -1;
+    if (!proof || ck->therm_pass_changed[0] == 0u) {
+        proven = 0;
+    } else {
+        __shared__ ThermRaw raw;
+        __shared__ double sU[TH_NF * TH_NE];
+        __shared__ int sFlag[3];
+        const int ntiles = therm_ntj(P) * therm_nti(P);
+        const int q = (ntiles + 7) / 8, x = blockIdx.x % 8, r = blockIdx.x / 8, RW = gridDim.x / 8;
+        const int first = x * q + r, end = min((x + 1) * q, ntiles);
+        for (int pass = 1; pass < npass; pass++) {
+            if (first < end) {
+                therm_store_raw<1, DIS>(therm_prefetch<1, DIS>(P, ck, A, pass, first, arp, 0), raw);
+                for (int t = first; t < end; t += RW) {
+                    lds_barrier();
+                    therm_tile_pipe<1, DIS>(P, ck, A, pass, t, P, ck, A, t + RW < end ? t + RW : -1, raw, sU, sFlag,
+                                            stol, arp, 0);
+                }
+            }
+            if (!grid_sync(ck)) return;  // every tile's iterate is written
+            for (long t = tid; t < nchk; t += nthr) therm_check_body(P, ck, A, pass, t, arp, 0);
+            if (!grid_sync(ck)) return;  // every border is checked
+            if (__hip_atomic_load(&ck->therm_pass_changed[pass], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+                proven = pass;
+                break;
+            }
+        }
+    }
+    const long nn = (long)P->nz * P->nx;
+    for (long t = tid; t < nn; t += nthr) therm_bc_apply(P, ck, A, t, npass, proof, proven);
 }
 
 // ------------------------------------------------------------------ material properties

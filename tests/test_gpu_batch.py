@@ -125,18 +125,23 @@ def test_batch_then_single_steps_equal_single_runs(precision):
 
 
 @pytest.mark.gpu
-def test_thermal_fallback_passes_bitwise_equal_to_oracle():
+@pytest.mark.parametrize("therm", ["tail", "multi"])
+def test_thermal_fallback_passes_bitwise_equal_to_oracle(therm, monkeypatch):
     """Cells 1.7 times taller than wide (200x40 elements over the 300x100 km box)
     couple neighbouring columns strongly, so that the thermal tiles' halos do not
     settle every output node and fallback passes run (the pipelined tile kernel from
-    the assembled iterate).  The run stays bitwise equal to the correctly rounded
-    oracle started from the same state."""
+    the assembled iterate), either inside the cooperative tail launch with grid
+    barriers between the phases (the default) or as a launch per phase
+    (GF_THERM=multi).  The run stays bitwise equal to the correctly rounded oracle
+    started from the same state."""
     import os
     from tests import oracle_lib as ol
+    if therm == "multi":
+        monkeypatch.setenv("GF_THERM", "multi")
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     q = sweep.variant(inp.read_inp(os.path.join(root, BASE)), 200, 40)
     n = 30
-    _say("case: FP64 200x40 variant, %d steps, device vs oracle" % n)
+    _say("case: FP64 200x40 variant, %d steps, thermal %s, device vs oracle" % (n, therm))
     gp, st, m = sweep.make_model(q, abi.PREC_FP64)
     m.download()
     ref = st.copy()
