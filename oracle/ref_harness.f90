@@ -8,9 +8,13 @@
 ! every array the explicit step reads or writes.
 !
 ! The step loop follows src/par.f90:81-91 (flac; nloop += 1; time += dt)
-! without remeshing or periodic output.  Compiled with -DGF_DROPIN the same
-! harness drives the HIP drop-in `flac` (geoflac_amd/fortran/flac_hip.f90)
-! and syncs device state to the Fortran arrays before each dump.
+! without periodic output.  With GF_REM=1 it also runs the driver's remeshing
+! block after every ntest_rem steps (src/par.f90:94-110: itest_mesh, then
+! bar2euler, remesh, lpeuler2bar, marker2elem when needed).  Compiled with
+! -DGF_DROPIN the same harness drives the HIP drop-in `flac`
+! (geoflac_amd/fortran/flac_hip.f90), syncs device state to the Fortran arrays
+! before each dump and before the remeshing test, and back to the device after a
+! remeshing, as a driver using the drop-in does.
 !
 ! usage: ref_harness <inputfile> <outdir> <nsteps> [dump_step ...]
 program gf_ref_harness
@@ -20,7 +24,8 @@ use marker_data
 implicit none
 character(len=200) :: inputfile
 character(len=512) :: outdir, arg
-integer :: nsteps, nd, k, istep, narg
+integer :: nsteps, nd, k, istep, narg, nrem
+logical :: do_rem
 integer, allocatable :: dumps(:)
 
 narg = command_argument_count()
@@ -58,10 +63,14 @@ do k = 1, nd
 enddo
 if (any(dumps(1:nd) == 0)) call dump_state(outdir, 0)
 
+call get_environment_variable('GF_REM', arg)
+do_rem = trim(arg) == '1'
+nrem = 0
 do istep = 1, nsteps
     call flac
     nloop = nloop + 1
     time = time + dt
+    if (do_rem .and. mod(istep, ntest_rem) == 0) call remesh_block
     if (any(dumps(1:nd) == istep)) then
 #ifdef GF_DROPIN
         call flac_hip_sync_host
@@ -79,6 +88,7 @@ if (trim(arg) == '1') then
     call outflac
     call saveflac
 endif
+if (do_rem) write(*, '(A,I6)') 'remeshings ', nrem
 #ifdef GF_DROPIN
 ! GF_HIP_WRITE=1: the drop-in's writers from the device state instead
 call get_environment_variable('GF_HIP_WRITE', arg)
@@ -91,6 +101,26 @@ call flac_hip_finalize
 close(333)
 
 contains
+
+! src/par.f90:94-110
+subroutine remesh_block
+    integer :: need_remeshing
+    if (ny_rem /= 1 .or. itherm == 2) return
+#ifdef GF_DROPIN
+    call flac_hip_sync_host
+#endif
+    call itest_mesh(need_remeshing)
+    if (need_remeshing == 0) return
+    nrem = nrem + 1
+    call bar2euler
+    call remesh
+    call lpeuler2bar
+    call marker2elem
+    dhacc_correction(:) = 0.d0
+#ifdef GF_DROPIN
+    call flac_hip_sync_device
+#endif
+end subroutine
 
 subroutine wr8(dir, name, a, n)
     character(len=*), intent(in) :: dir, name

@@ -48,6 +48,9 @@ CASES = {
     "bcs60x20": dict(inp="bcs60x20.inp", steps=[1, 10, 100, 1000]),
     # dt_elastic fixed by dt_scale (idt_scale 1) and the prestress freeze (i_prestress 1)
     "idt1_60x20": dict(inp="idt1_60x20.inp", steps=[1, 10, 100]),
+    # box60x20 with the driver's remeshing (GF_REM=1 of the harness) every few tens of
+    # steps: five remeshings in 300 steps
+    "remesh60x20": dict(inp="remesh60x20.inp", steps=[100, 300], env={"GF_REM": "1"}),
 }
 
 
@@ -80,7 +83,7 @@ def run_case(name, spec):
     with tempfile.TemporaryDirectory() as tmp:
         out = os.path.join(tmp, "dump")
         cmd = [HARNESS, inp, out, str(max(steps)), "0"] + [str(s) for s in steps]
-        subprocess.run(cmd, cwd=tmp, check=True, stdout=subprocess.DEVNULL)
+        subprocess.run(cmd, cwd=tmp, check=True, stdout=subprocess.DEVNULL, env=dict(os.environ, **spec.get("env", {})))
         with open(os.path.join(out, "params.txt")) as f:
             ptxt = f.read()
         with open(os.path.join(HERE, name + "_params.txt"), "w") as f:

@@ -54,18 +54,16 @@ def test_fortran_interface_mirrors_c_structs():
     assert c_fields("gf_output_opts") == f_fields("gf_output_opts_t")
 
 
-@pytest.mark.gpu
-@pytest.mark.parametrize("case,dumps", [("box60x20", (10, 100)), ("phases60x30", (11, 100)), ("bcs60x20", (10, 100))])
-def test_fortran_dropin_matches_reference(tmp_path, case, dumps):
-    """box60x20, and phases60x30 whose markers change phase on the device (the marker
-    phases, phase ratios and iphase come back to the Fortran arrays)."""
+def _run_dropin(tmp_path, case, dumps, env=None):
+    """Runs the drop-in harness on tests/golden/inputs/<case>.inp and checks its dumps
+    against the reference's goldens: marker phases, iphase and phase ratios equal, the
+    floating-point fields within TOL rel-L2, nloop equal.  Returns the harness's stdout."""
     if not os.path.exists(EXE):
         pytest.skip("oracle/_ref/dropin_harness is built only where the reference sources are present")
     inp = os.path.join(ROOT, "tests", "golden", "inputs", case + ".inp")
     out = str(tmp_path / "out")
-    print("case: Fortran drop-in, %d steps of %s (dumps at %s)" % (max(dumps), case, dumps), flush=True)
     r = subprocess.run([EXE, inp, out, str(max(dumps))] + [str(d) for d in dumps], cwd=str(tmp_path),
-                       capture_output=True, text=True, timeout=300)
+                       capture_output=True, text=True, timeout=300, env=dict(os.environ, **(env or {})))
     sys.stdout.write(r.stdout[-2000:])
     assert r.returncode == 0, r.stderr[-2000:]
     for step in dumps:
@@ -83,6 +81,32 @@ def test_fortran_dropin_matches_reference(tmp_path, case, dumps):
         assert max(worst.values()) <= TOL, worst
         assert st.clock.nloop == ref.clock.nloop
         assert abs(st.clock.dt - ref.clock.dt) <= 1e-12 * ref.clock.dt
+    return r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case,dumps", [("box60x20", (10, 100)), ("phases60x30", (11, 100)), ("bcs60x20", (10, 100))])
+def test_fortran_dropin_matches_reference(tmp_path, case, dumps):
+    """box60x20, and phases60x30 whose markers change phase on the device (the marker
+    phases, phase ratios and iphase come back to the Fortran arrays)."""
+    print("case: Fortran drop-in, %d steps of %s (dumps at %s)" % (max(dumps), case, dumps), flush=True)
+    _run_dropin(tmp_path, case, dumps)
+
+
+@pytest.mark.gpu
+def test_fortran_dropin_with_host_remeshing_matches_reference(tmp_path):
+    """Remeshing stays with the driver, on the host: every ntest_rem steps the harness
+    (GF_REM=1) brings the device state to the Fortran arrays (flac_hip_sync_host), runs
+    the reference's itest_mesh and, when it asks for it, bar2euler, remesh, lpeuler2bar
+    and marker2elem (src/par.f90:94-110), then hands the remeshed arrays, the markers
+    and the new dt back to the device (flac_hip_sync_device).  remesh60x20 remeshes
+    five times in 300 steps; the states after 100 and 300 steps match the reference
+    run of the same driver loop (tests/golden/remesh60x20_s*.npz)."""
+    print("case: Fortran drop-in with the driver's remeshing, 300 steps of remesh60x20", flush=True)
+    out = _run_dropin(tmp_path, "remesh60x20", (100, 300), env={"GF_REM": "1"})
+    import re
+    m = re.search(r"remeshings\s+(\d+)", out)
+    assert m and int(m.group(1)) == 5, out[-500:]
 
 
 @pytest.mark.gpu
