@@ -54,13 +54,16 @@ class GfParams(C.Structure):
                 + [("flexure_base", _I), ("nelem_serp", _I), ("nopbmax", _I)]
                 + [(n, _D) for n in _REAL_FIELDS]
                 + [(n, _D * MAXPH) for n in PHASE_TABLES]
-                + [("topo_kappa", _D)])
+                + [("topo_kappa", _D), ("erodibility", _D * MAXPH)])
 
 
 _DP = C.POINTER(C.c_double)
 _IP = C.POINTER(C.c_int32)
 _FP = C.POINTER(C.c_float)
 _BP = C.POINTER(C.c_int8)
+
+# surface processes (topo_kappa > 0): dtopo(nx), dhacc(nx-1), dhacc_correction(nx)
+TOPO_FIELDS = [("dtopo", _DP), ("dhacc", _DP), ("dhacc_correction", _DP)]
 
 HOST_FIELDS = [
     ("cord", _DP), ("vel", _DP), ("temp", _DP), ("temp0", _DP), ("force", _DP),
@@ -72,7 +75,7 @@ HOST_FIELDS = [
 
 
 class GfHostArrays(C.Structure):
-    _fields_ = HOST_FIELDS + [("nopbou", _IP), ("bcstress", _DP), ("coh_scale", _FP)]
+    _fields_ = HOST_FIELDS + [("nopbou", _IP), ("bcstress", _DP), ("coh_scale", _FP)] + TOPO_FIELDS
 
 
 MAX_MARKERS_PER_ELEM = 32  # src/marker_data.f90:5
@@ -81,7 +84,7 @@ MARKER_FIELDS = [("mark_a1", _DP), ("mark_a2", _DP), ("mark_ntriag", _IP), ("mar
 
 
 class GfMarkerArrays(C.Structure):
-    _fields_ = [("nmarkers", _I), ("pad", _I)] + MARKER_FIELDS
+    _fields_ = [("nmarkers", _I), ("capacity", _I)] + MARKER_FIELDS + [("mark_age", _DP)]
 
 
 class GfClock(C.Structure):
@@ -107,6 +110,7 @@ GF_ALL = (1 << len(HOST_FIELDS)) - 1
 GF_MARKERS = 1 << 22
 GF_COH_SCALE = 1 << 23
 GF_STRESSBC = 1 << 24
+GF_TOPO = 1 << 25
 
 SIZEOF_PARAMS = C.sizeof(GfParams)
 SIZEOF_HOST = C.sizeof(GfHostArrays)

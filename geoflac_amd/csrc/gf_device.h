@@ -32,6 +32,7 @@ struct GfPhase {
     // friction and dilation angles of a one-phase element before softening starts,
     // and their correctly rounded sin(phi), sin(psi), tan(phi) (src/rh_plastic.f90:33-46)
     double phi0, psi0, sphi0, spsi0, tphi0;
+    double erod;         // erodibility of the phase at the surface (src/fl_move.f90:170-174)
 };
 
 struct GfDevParams {
@@ -46,6 +47,7 @@ struct GfDevParams {
     double t_top, t_bot, bot_bc, g, demf, ten_off, tau_heal, v_min, v_max;
     double frac, fracm, strain_inert, dt_scale, xReyn, vbc, rxbo, rzbo, pisos, drosub;
     double vbc_over_rxbo;
+    double topo_kappa;   // surface diffusion (src/fl_move.f90:40), 0: none
     GfPhase ph[GF_MAXPH];
 };
 
@@ -69,6 +71,7 @@ struct GfDevClock {
     int ph_map[GF_MAXACT];      // new plane of each old plane (-1: dropped)
     int ph_nz[GF_MAXACT];       // elements with a nonzero ratio, per plane
     int ph_count[GF_MAXPH];     // elements whose iphase is id-1 (the dt_mass minima run over ids present)
+    int nmarkers, mk_pad;       // markers in use (resurface appends)
     // grid barrier of the thermal step's fallback passes: arrivals, generation, a
     // bounded wait that ran out
     unsigned int bar_cnt, bar_gen;

@@ -35,7 +35,12 @@ struct GfDevArrays {
     int *mark_id_elem;    // GF_MAX_MARKERS_PER_ELEM per element, element-major, 1-based
     int *nmark_elem;
     int *phstamp;         // per element: the change_phase call that last changed one of its markers
-    long nmarkers;
+    long nmarkers;        // (the count at upload; the device's own count is ck->nmarkers)
+    double *mark_age;     // (null when the caller keeps no ages)
+    long mark_cap;        // entries the per-marker arrays hold
+    // surface processes (topo_kappa > 0): dtopo(nx), dhacc(nx-1), dhacc_correction(nx),
+    // and nx doubles of scratch
+    double *dtopo, *dhacc, *dhacc_corr, *topo_tmp;
     // stress boundary conditions (bc_update): per segment its two nodes (linear
     // indices) and normal and shear stress; per boundary node (edge index, bc_edge)
     // its segments in increasing order, as 2 * segment + (0: first node, 1: second);

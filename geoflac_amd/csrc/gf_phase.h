@@ -204,7 +204,7 @@ __global__ void __launch_bounds__(256) k_phase_rules_mk(const GfDevParams *__res
                                                         GfDevArrays A)
 {
     const long kk = xcd_lin();
-    if (kk >= A.nmarkers) return;
+    if (kk >= ck->nmarkers) return;  // (the device's count: resurface appends markers)
     phase_rules_body(P, ck, A, kk);
 }
 
@@ -312,8 +312,10 @@ __global__ void __launch_bounds__(256) k_phase_replane(const GfDevParams *__rest
     phase_replane_body(P, ck, A, e);
 }
 
-// src/change_phase.f90:247-260 with count_phase_ratio (src/marker_data.f90:140-174)
-GF_DEV void phase_recount_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e)
+// src/change_phase.f90:247-260 with count_phase_ratio (src/marker_data.f90:140-174);
+// aps_share 0: count_phase_ratio alone (resurface, src/fl_move.f90:413-416)
+GF_DEV void phase_recount_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long e,
+                               int aps_share = 1)
 {
     if (A.phstamp[e] != ck->ph_call) return;
     const int nm = A.nmark_elem[e];
@@ -350,15 +352,16 @@ GF_DEV void phase_recount_body(const GfDevParams *__restrict__ P, GfDevClock *ck
         A.iphase[e] = iph;
     }
     // the recrystallised markers lose their share of the plastic strain
+    if (!aps_share) return;
     const double aps = A.aps[e];
     const double v = aps - aps / (double)(float)nm;
     A.aps[e] = v > 0.0 ? v : 0.0;
 }
 
 __global__ void __launch_bounds__(256) k_phase_recount(const GfDevParams *__restrict__ P, GfDevClock *ck,
-                                                       GfDevArrays A)
+                                                       GfDevArrays A, int aps_share)
 {
     const long e = xcd_lin();
     if (e >= NE) return;
-    phase_recount_body(P, ck, A, e);
+    phase_recount_body(P, ck, A, e, aps_share);
 }

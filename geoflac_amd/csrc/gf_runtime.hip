@@ -23,6 +23,7 @@
 #include "gf_phase.h"
 #include "gf_bc.h"
 #include "gf_output.h"
+#include "gf_topo.h"
 #include "gf_mlp.h"
 #include "gf_batch.h"
 
@@ -161,6 +162,7 @@ static int build_dev_params(gf_ctx *ctx)
     d.frac = p.frac; d.fracm = p.fracm; d.strain_inert = p.strain_inert; d.dt_scale = p.dt_scale;
     d.xReyn = p.xReyn; d.vbc = p.vbc; d.rxbo = p.rxbo; d.rzbo = p.rzbo; d.pisos = p.pisos; d.drosub = p.drosub;
     d.vbc_over_rxbo = p.vbc / p.rxbo;
+    d.topo_kappa = p.topo_kappa;
     for (int k = 0; k < GF_MAXPH; k++) {
         GfPhase &ph = d.ph[k];
         ph.irheol = p.irheol[k];
@@ -172,6 +174,7 @@ static int build_dev_params(gf_ctx *ctx)
         ph.cohesion1 = p.cohesion1[k]; ph.cohesion2 = p.cohesion2[k];
         ph.dilat1 = p.dilat1[k]; ph.dilat2 = p.dilat2[k];
         ph.conduct = p.conduct[k]; ph.cp = p.cp[k];
+        ph.erod = p.erodibility[k];
         // the expressions below are the reference's, evaluated once per phase
         ph.bulkm = p.rl[k] + 2.0 * p.rm[k] / 3.0;               // src/fl_rheol.f90:191
         ph.pwave = p.rl[k] + 0.6666 * p.rm[k];                  // src/dt_mass.f90:86
@@ -235,8 +238,11 @@ extern "C" int gf_create(const gf_params *p, int32_t precision, gf_ctx **out)
     // idt_scale 0 (dt_elastic from the wave speed): the reference's minimum starts from
     // the dt_elastic it was given (0 after setflac) and the step divides by it
     if (p->idt_scale == 0) return GF_ERR_UNSUPPORTED;
-    // the flexural base and surface processes of fl_move (src/fl_move.f90:37-40)
-    if (p->flexure_base != 0 || p->topo_kappa > 0.0) return GF_ERR_UNSUPPORTED;
+    // the flexural base of fl_move (src/fl_move.f90:37, src/fl_flexure.f90): on a new
+    // start the reference compares q_init, which nothing initialises (allocated at
+    // src/arrays.f90:114, read only by rsflac), with zero, so its first load is
+    // undefined; surface diffusion (topo_kappa > 0) runs on the device
+    if (p->flexure_base != 0) return GF_ERR_UNSUPPORTED;
     if (p->inhom < 0 || p->inhom > GF_MAXINH) return GF_ERR_ARG;
     // the kernels address each array with 32-bit byte offsets: the 16 FP64 stress
     // planes (128 bytes per element) must stay under 4 GiB
@@ -330,6 +336,17 @@ extern "C" int gf_create(const gf_params *p, int32_t precision, gf_ctx **out)
     ALLOC(ctx->therm_border, 8 * (size_t)TH_NB * ((p->nz + TH_OJ - 1) / TH_OJ) * ((p->nx + TH_OI - 1) / TH_OI));
     ctx->a.tbord = ctx->therm_border;
     ALLOC(ctx->a.dens, 8 * ne);
+    if (p->topo_kappa > 0.0) {
+        ALLOC(ctx->a.dtopo, 8 * (size_t)p->nx);
+        ALLOC(ctx->a.dhacc, 8 * (size_t)p->nx);
+        ALLOC(ctx->a.dhacc_corr, 8 * (size_t)p->nx);
+        ALLOC(ctx->a.topo_tmp, 8 * (size_t)p->nx);
+        for (double *q : {ctx->a.dtopo, ctx->a.dhacc, ctx->a.dhacc_corr, ctx->a.topo_tmp})
+            if (hipMemset(q, 0, 8 * (size_t)p->nx) != hipSuccess) {
+                rc = set_err(ctx, GF_ERR_HIP, "hipMemset failed");
+                goto fail;
+            }
+    }
 #undef ALLOC
     {
         GfDevClock ck{};
@@ -362,7 +379,8 @@ extern "C" void gf_destroy(gf_ctx *ctx)
                     ctx->a.xa, ctx->a.xb, ctx->a.dens, ctx->therm_border, ctx->a.stress0f, ctx->a.velf,
                     ctx->a.forcef, ctx->a.pref, ctx->sur_samples, ctx->sur_w, ctx->sur_m, ctx->sur_v,
                     ctx->sur_grad, ctx->sur_partial, ctx->sur_cfg, ctx->sur_stats, ctx->sur_list,
-                    ctx->sur_cnt, ctx->sur_lval, ctx->sur_wgs, ctx->cord_alt, ctx->out_buf};
+                    ctx->sur_cnt, ctx->sur_lval, ctx->sur_wgs, ctx->cord_alt, ctx->out_buf, ctx->a.mark_age,
+                    ctx->a.dtopo, ctx->a.dhacc, ctx->a.dhacc_corr, ctx->a.topo_tmp};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -391,6 +409,8 @@ extern "C" int gf_bind_markers(gf_ctx *ctx, const gf_marker_arrays *m)
 {
     if (!ctx || !m) return GF_ERR_ARG;
     if (m->nmarkers < 0) return set_err(ctx, GF_ERR_ARG, "negative marker count");
+    if (m->capacity != 0 && m->capacity < m->nmarkers)
+        return set_err(ctx, GF_ERR_ARG, "marker capacity %d below the count %d", m->capacity, m->nmarkers);
     ctx->hmark = *m;
     ctx->markers_bound = true;
     return GF_OK;
@@ -532,22 +552,29 @@ static int upload_markers(gf_ctx *ctx)
             if (id < 1 || id > nmk) return set_err(ctx, GF_ERR_ARG, "element %ld lists marker %d", e, id);
         }
     }
-    if (nmk > ctx->mark_cap) {
+    // with surface processes resurface appends markers on the device, up to the
+    // capacity of the caller's arrays
+    const bool topo = ctx->p.topo_kappa > 0.0;
+    const long want = topo ? std::max<long>(nmk, m.capacity) : nmk;
+    const bool ages = topo && m.mark_age;
+    if (want > ctx->mark_cap || (ages && !ctx->a.mark_age)) {
         for (void *q : {(void *)ctx->a.mark_a1, (void *)ctx->a.mark_a2, (void *)ctx->a.mark_ntriag,
-                        (void *)ctx->a.mark_phase, (void *)ctx->a.mark_dead})
+                        (void *)ctx->a.mark_phase, (void *)ctx->a.mark_dead, (void *)ctx->a.mark_age})
             if (q) (void)hipFree(q);
-        ctx->a.mark_a1 = ctx->a.mark_a2 = nullptr;
+        ctx->a.mark_a1 = ctx->a.mark_a2 = ctx->a.mark_age = nullptr;
         ctx->a.mark_ntriag = nullptr;
         ctx->a.mark_phase = ctx->a.mark_dead = nullptr;
         ctx->mark_cap = 0;
-        const size_t n = (size_t)std::max<long>(nmk, 1);
+        const size_t n = (size_t)std::max<long>(want, 1);
         HIPCHK(hipMalloc((void **)&ctx->a.mark_a1, 8 * n));
         HIPCHK(hipMalloc((void **)&ctx->a.mark_a2, 8 * n));
         HIPCHK(hipMalloc((void **)&ctx->a.mark_ntriag, 4 * n));
         HIPCHK(hipMalloc((void **)&ctx->a.mark_phase, n));
         HIPCHK(hipMalloc((void **)&ctx->a.mark_dead, n));
-        ctx->mark_cap = nmk;
+        if (ages) HIPCHK(hipMalloc((void **)&ctx->a.mark_age, 8 * n));
+        ctx->mark_cap = want;
     }
+    ctx->a.mark_cap = ctx->mark_cap;
     if (!ctx->a.mark_id_elem) {
         HIPCHK(hipMalloc((void **)&ctx->a.mark_id_elem, 4 * (size_t)GF_MAX_MARKERS_PER_ELEM * ne));
         HIPCHK(hipMalloc((void **)&ctx->a.nmark_elem, 4 * (size_t)ne));
@@ -560,11 +587,18 @@ static int upload_markers(gf_ctx *ctx)
         HIPCHK(hipMemcpy(ctx->a.mark_ntriag, m.mark_ntriag, 4 * nmk, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(ctx->a.mark_phase, m.mark_phase, nmk, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(ctx->a.mark_dead, m.mark_dead, nmk, hipMemcpyHostToDevice));
+        if (ctx->a.mark_age && m.mark_age)
+            HIPCHK(hipMemcpy(ctx->a.mark_age, m.mark_age, 8 * nmk, hipMemcpyHostToDevice));
     }
     HIPCHK(hipMemcpy(ctx->a.mark_id_elem, m.mark_id_elem, 4 * (size_t)GF_MAX_MARKERS_PER_ELEM * ne,
                      hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->a.nmark_elem, m.nmark_elem, 4 * (size_t)ne, hipMemcpyHostToDevice));
     ctx->a.nmarkers = nmk;
+    {
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        const int n32 = (int)nmk;
+        HIPCHK(hipMemcpy((char *)ctx->dck + offsetof(GfDevClock, nmarkers), &n32, sizeof n32, hipMemcpyHostToDevice));
+    }
     ctx->markers_uploaded = true;
     {
         bool ok = true;
@@ -593,6 +627,24 @@ static int upload_markers(gf_ctx *ctx)
         HIPCHK(hipMemcpy((char *)ctx->dck + offsetof(GfDevClock, ph_present), &present, sizeof present,
                          hipMemcpyHostToDevice));
     }
+    return GF_OK;
+}
+
+// dtopo(nx), dhacc(nx-1), dhacc_correction(nx) between the host and the device
+static int copy_topo(gf_ctx *ctx, hipMemcpyKind dir)
+{
+    if (ctx->p.topo_kappa <= 0.0) return GF_OK;  // (no surface process: nothing on the device)
+    const gf_host_arrays &h = ctx->host;
+    if (!h.dtopo || !h.dhacc || !h.dhacc_correction)
+        return set_err(ctx, GF_ERR_ARG, "host pointer for dtopo, dhacc or dhacc_correction is null");
+    const size_t nx = (size_t)ctx->p.nx;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const bool up = dir == hipMemcpyHostToDevice;
+    HIPCHK(hipMemcpy(up ? (void *)ctx->a.dtopo : (void *)h.dtopo, up ? (const void *)h.dtopo : ctx->a.dtopo, 8 * nx, dir));
+    HIPCHK(hipMemcpy(up ? (void *)ctx->a.dhacc : (void *)h.dhacc, up ? (const void *)h.dhacc : ctx->a.dhacc,
+                     8 * (nx - 1), dir));
+    HIPCHK(hipMemcpy(up ? (void *)ctx->a.dhacc_corr : (void *)h.dhacc_correction,
+                     up ? (const void *)h.dhacc_correction : ctx->a.dhacc_corr, 8 * nx, dir));
     return GF_OK;
 }
 
@@ -746,6 +798,10 @@ extern "C" int gf_upload(gf_ctx *ctx, uint32_t mask)
         int rc = upload_stressbc(ctx);
         if (rc) return rc;
     }
+    if (mask & GF_TOPO) {
+        int rc = copy_topo(ctx, hipMemcpyHostToDevice);
+        if (rc) return rc;
+    }
     if (mask & GF_PHASE_RATIO) {
         if (!ctx->host.phase_ratio) return set_err(ctx, GF_ERR_ARG, "host pointer for phase_ratio is null");
         int rc = upload_phase_ratio(ctx);
@@ -807,8 +863,47 @@ extern "C" int gf_download(gf_ctx *ctx, uint32_t mask)
             memcpy(ctx->host.phase_ratio, ctx->phase_ratio_host.data(), sizeof(double) * ctx->phase_ratio_host.size());
         }
     }
-    if ((mask & GF_MARKERS) && ctx->markers_uploaded && ctx->hmark.mark_phase && ctx->a.nmarkers > 0)
+    if (mask & GF_TOPO) {
+        int rc = copy_topo(ctx, hipMemcpyDeviceToHost);
+        if (rc) return rc;
+    }
+    if ((mask & GF_MARKERS) && ctx->markers_uploaded && ctx->p.topo_kappa > 0.0) {
+        // resurface removes and appends markers: every marker array comes back
+        const gf_marker_arrays &m = ctx->hmark;
+        int n32 = 0;
+        HIPCHK(hipMemcpy(&n32, (char *)ctx->dck + offsetof(GfDevClock, nmarkers), sizeof n32, hipMemcpyDeviceToHost));
+        const long nmk = n32, ne = ctx->ne;
+        const long cap = m.capacity > 0 ? m.capacity : m.nmarkers;
+        if (nmk > cap) return set_err(ctx, GF_ERR_ARG, "%ld markers do not fit the host arrays (%ld)", nmk, cap);
+        if (nmk > 0) {
+            HIPCHK(hipMemcpy(m.mark_a1, ctx->a.mark_a1, 8 * nmk, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(m.mark_a2, ctx->a.mark_a2, 8 * nmk, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(m.mark_ntriag, ctx->a.mark_ntriag, 4 * nmk, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(m.mark_phase, ctx->a.mark_phase, nmk, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(m.mark_dead, ctx->a.mark_dead, nmk, hipMemcpyDeviceToHost));
+            if (m.mark_age && ctx->a.mark_age)
+                HIPCHK(hipMemcpy(m.mark_age, ctx->a.mark_age, 8 * nmk, hipMemcpyDeviceToHost));
+        }
+        HIPCHK(hipMemcpy(m.mark_id_elem, ctx->a.mark_id_elem, 4 * (size_t)GF_MAX_MARKERS_PER_ELEM * ne,
+                         hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(m.nmark_elem, ctx->a.nmark_elem, 4 * (size_t)ne, hipMemcpyDeviceToHost));
+        ctx->a.nmarkers = nmk;
+        ctx->hmark.nmarkers = (int32_t)nmk;
+    } else if ((mask & GF_MARKERS) && ctx->markers_uploaded && ctx->hmark.mark_phase && ctx->a.nmarkers > 0) {
         HIPCHK(hipMemcpy(ctx->hmark.mark_phase, ctx->a.mark_phase, ctx->a.nmarkers, hipMemcpyDeviceToHost));
+    }
+    return GF_OK;
+}
+
+extern "C" int gf_marker_count(gf_ctx *ctx, int32_t *nmarkers)
+{
+    if (!ctx || !nmarkers) return GF_ERR_ARG;
+    *nmarkers = 0;
+    if (!ctx->markers_uploaded) return GF_OK;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    int n32 = 0;
+    HIPCHK(hipMemcpy(&n32, (char *)ctx->dck + offsetof(GfDevClock, nmarkers), sizeof n32, hipMemcpyDeviceToHost));
+    *nmarkers = n32;
     return GF_OK;
 }
 
@@ -836,6 +931,7 @@ static int fetch_clock(gf_ctx *ctx, gf_clock *c)
     if (ck.therm_extra_passes < ctx->therm_extra_seen) ctx->therm_extra_seen = 0;  // (statistics were reset)
     ctx->therm_fb_full = ck.therm_extra_passes > ctx->therm_extra_seen;
     ctx->therm_extra_seen = ck.therm_extra_passes;
+    if (ctx->markers_uploaded && ctx->p.topo_kappa > 0.0) ctx->a.nmarkers = ck.nmarkers;  // (resurface appends)
     if (ck.bar_timeout != 0) {  // (not expected: every workgroup of the launch is resident)
         GfDevClock z = ck;
         z.bar_cnt = 0;
@@ -1142,6 +1238,7 @@ static int enq_node(gf_ctx *ctx, bool move)
     return GF_OK;
 }
 
+static int enq_diff_topo(gf_ctx *ctx);
 static int enq_move(gf_ctx *ctx)
 {
     if (ctx->p.movegrid == 0) return GF_OK;
@@ -1150,6 +1247,8 @@ static int enq_move(gf_ctx *ctx)
     const bool fused = ctx->cord_moved;   // the step's fl_node moved the coordinates
     if (!fused) LAUNCH_SV(k_move_cord, ctx->nn, ctx->dp, ctx->dck, ctx->a);
     ctx->cord_moved = false;
+    // (the flexural base, src/fl_move.f90:37, is refused at gf_create)
+    if (ctx->p.topo_kappa > 0.0 && (rc = enq_diff_topo(ctx))) return rc;
     if (ctx->rheol_quad) {
         LAUNCH_M(k_move_elem, ctx->ne, ctx->dp, ctx->dck, ctx->a);
     } else if (fused) {
@@ -1258,13 +1357,30 @@ static int enq_change_phase(gf_ctx *ctx)
     if (!ctx->markers_uploaded) return GF_OK;
     if (ctx->a.nmarkers > 0 && ctx->mark_lists_ok && !ctx->rules_per_marker)
         LAUNCH(k_phase_rules, ctx->ne, ctx->dp, ctx->dck, ctx->a);
-    else if (ctx->a.nmarkers > 0)
-        LAUNCH(k_phase_rules_mk, ctx->a.nmarkers, ctx->dp, ctx->dck, ctx->a);
+    else if (ctx->a.nmarkers > 0)  // (surface processes: every entry the arrays hold may be in use)
+        LAUNCH(k_phase_rules_mk, ctx->p.topo_kappa > 0.0 ? ctx->mark_cap : ctx->a.nmarkers, ctx->dp, ctx->dck, ctx->a);
     hipLaunchKernelGGL(k_phase_plan, dim3(1), dim3(64), 0, ctx->stream, ctx->dp, ctx->dck);
     int rc = launch_check(ctx, "k_phase_plan");
     if (rc) return rc;
     LAUNCH(k_phase_replane, ctx->ne, ctx->dp, ctx->dck, ctx->a);
-    LAUNCH(k_phase_recount, ctx->ne, ctx->dp, ctx->dck, ctx->a);
+    LAUNCH(k_phase_recount, ctx->ne, ctx->dp, ctx->dck, ctx->a, 1);
+    return GF_OK;
+}
+
+// diff_topo (src/fl_move.f90:138-315) after the coordinate move, with resurface every
+// ifreq_avgsr steps and count_phase_ratio of the elements it changed
+static int enq_diff_topo(gf_ctx *ctx)
+{
+    hipLaunchKernelGGL(k_diff_topo, dim3(1), dim3(TOPO_WG), 0, ctx->stream, ctx->dp, ctx->dck, ctx->a);
+    int rc = launch_check(ctx, "k_diff_topo");
+    if (rc) return rc;
+    if (ctx->nloop % ctx->p.ifreq_avgsr != 0 || !ctx->markers_uploaded) return GF_OK;
+    hipLaunchKernelGGL(k_resurface, dim3(1), dim3(TOPO_WG), 0, ctx->stream, ctx->dp, ctx->dck, ctx->a, ctx->nloop);
+    if ((rc = launch_check(ctx, "k_resurface"))) return rc;
+    hipLaunchKernelGGL(k_phase_plan, dim3(1), dim3(64), 0, ctx->stream, ctx->dp, ctx->dck);
+    if ((rc = launch_check(ctx, "k_phase_plan"))) return rc;
+    LAUNCH(k_phase_replane, ctx->ne, ctx->dp, ctx->dck, ctx->a);
+    LAUNCH(k_phase_recount, ctx->ne, ctx->dp, ctx->dck, ctx->a, 0);
     return GF_OK;
 }
 
@@ -1502,9 +1618,16 @@ extern "C" int gf_outflac(gf_ctx *ctx, const gf_clock *clock, const char *dir, c
     if (o->io_hfl == 1) OUT(col("hfl.0", hc + (size_t)OC_HFL * nx, nx));
     if (o->io_topo == 1) {
         OUT(col("topo.0", hc + (size_t)OC_TOPO * nx, nx));
-        // no surface process runs (topo_kappa = 0): dtopo and extrusion stay 0
-        const std::vector<float> z((size_t)nx, 0.0f);
+        // the rate of the last topography change in mm/yr (0 without surface processes);
+        // extrusion stays 0 without melting
+        std::vector<float> z((size_t)nx, 0.0f);
+        if (ctx->p.topo_kappa > 0.0) {
+            std::vector<double> dt8((size_t)nx);
+            HIPCHK(hipMemcpy(dt8.data(), ctx->a.dtopo, 8 * (size_t)nx, hipMemcpyDeviceToHost));
+            for (long i = 0; i < nx; i++) z[i] = (float)(dt8[i] / clock->dt * 1000 * GF_SEC_YEAR);
+        }
         OUT(col("dtopo.0", z.data(), nx));
+        std::fill(z.begin(), z.end(), 0.0f);
         OUT(col("extrusion.0", z.data(), nx - 1));
     }
 #undef OUT
@@ -1517,6 +1640,12 @@ extern "C" int gf_saveflac(gf_ctx *ctx, const gf_clock *clock, const char *dir, 
 {
     int rc = output_ready(ctx, clock, dir, o);
     if (rc) return rc;
+    const bool topo = ctx->p.topo_kappa > 0.0;
+    if (topo && ctx->markers_uploaded) {  // (resurface appends markers on the device)
+        int32_t n32 = 0;
+        if ((rc = gf_marker_count(ctx, &n32))) return rc;
+        ctx->a.nmarkers = n32;
+    }
     const long nx = ctx->p.nx, nn = ctx->nn, ne = ctx->ne, nmk = ctx->markers_uploaded ? ctx->a.nmarkers : 0;
     const std::string d = std::string(dir) + "/";
     {
@@ -1542,8 +1671,13 @@ extern "C" int gf_saveflac(gf_ctx *ctx, const gf_clock *clock, const char *dir, 
         return write_record(ctx, d + name, 1, n * 8, z.data(), n * 8);
     };
     OUT(dev("cord.rs", ctx->a.cord, (size_t)nn * 2));
-    OUT(host("dhacc.rs", o->dhacc, (size_t)nx - 1));
-    OUT(host("dhacc_corr.rs", o->dhacc_correction, (size_t)nx));
+    if (topo) {  // the surface processes keep these on the device
+        OUT(dev("dhacc.rs", ctx->a.dhacc, (size_t)nx - 1));
+        OUT(dev("dhacc_corr.rs", ctx->a.dhacc_corr, (size_t)nx));
+    } else {
+        OUT(host("dhacc.rs", o->dhacc, (size_t)nx - 1));
+        OUT(host("dhacc_corr.rs", o->dhacc_correction, (size_t)nx));
+    }
     OUT(host("extr_acc.rs", o->extr_acc, (size_t)nx - 1));
     OUT(host("q_init.rs", o->q_init, (size_t)nx));
     OUT(host("vel_flex_old.rs", o->vel_flex_old, (size_t)nx));
@@ -1572,7 +1706,10 @@ extern "C" int gf_saveflac(gf_ctx *ctx, const gf_clock *clock, const char *dir, 
         OUT(write_record(ctx, d + "marker1.rs", 1, n * 8, a.data(), n * 8));
         HIPCHK(hipMemcpy(a.data(), ctx->a.mark_a2, n * 8, hipMemcpyDeviceToHost));
         OUT(write_record(ctx, d + "marker1.rs", 2, n * 8, a.data(), n * 8));
-        if (o->mark_age) {
+        if (ctx->a.mark_age) {  // (resurface records the ages of the markers it appends)
+            HIPCHK(hipMemcpy(a.data(), ctx->a.mark_age, n * 8, hipMemcpyDeviceToHost));
+            OUT(write_record(ctx, d + "marker1.rs", 3, n * 8, a.data(), n * 8));
+        } else if (o->mark_age) {
             OUT(write_record(ctx, d + "marker1.rs", 3, n * 8, o->mark_age, n * 8));
         } else {
             std::fill(a.begin(), a.end(), 0.0);
@@ -2189,9 +2326,10 @@ extern "C" int gf_batch_create(gf_ctx *const *models, int32_t n, gf_batch **out)
             q.ifreq_visc != q0.ifreq_visc || q.ifreq_avgsr != q0.ifreq_avgsr || q.ifreq_rmasses != q0.ifreq_rmasses ||
             q.ifreq_imasses != q0.ifreq_imasses || c->nloop != c0->nloop || c->nsrate != c0->nsrate || c->sur_on ||
             c->rheol_quad || (q.movegrid != 0) != (q0.movegrid != 0) || c->therm_passes != c0->therm_passes ||
-            q.nystressbc != 0)
+            q.nystressbc != 0 || q.topo_kappa > 0.0)
             return fail(GF_ERR_ARG,
-                        "models of a batch share precision, switches, frequencies, movegrid and loop counters");
+                        "models of a batch share precision, switches, frequencies, movegrid and loop counters "
+                        "(and run without stress boundaries and surface processes)");
         if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(GF_ERR_HIP, "sync");
         const long ntiles = (long)((q.nz + TH_OJ - 1) / TH_OJ) * ((q.nx + TH_OI - 1) / TH_OI);
         const long nmk = c->markers_uploaded ? c->a.nmarkers : 0;

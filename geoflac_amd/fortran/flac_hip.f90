@@ -73,8 +73,7 @@ p%inhom = inhom
 p%nelem_serp = nelem_serp
 p%nopbmax = nopbmax
 p%irheol = 0; p%ix1 = 0; p%ix2 = 0; p%iy1 = 0; p%iy2 = 0; p%igeom = 0
-! the flexural base (src/fl_flexure.f90:15-24) and surface diffusion (src/fl_move.f90:40)
-! are refused by gf_create
+! the flexural base (src/fl_flexure.f90:15-24) is refused by gf_create
 p%flexure_base = 0
 do k = 1, nofbc
     if (nbc(k) == 200 .and. nofside(k) == 2) p%flexure_base = 1
@@ -98,7 +97,7 @@ do k = 1, GF_MAXPH
     p%rl(k) = rl(k); p%rm(k) = rm(k); p%plstrain1(k) = plstrain1(k); p%plstrain2(k) = plstrain2(k)
     p%fric1(k) = fric1(k); p%fric2(k) = fric2(k); p%cohesion1(k) = cohesion1(k)
     p%cohesion2(k) = cohesion2(k); p%dilat1(k) = dilat1(k); p%dilat2(k) = dilat2(k)
-    p%conduct(k) = conduct(k); p%cp(k) = cp(k)
+    p%conduct(k) = conduct(k); p%cp(k) = cp(k); p%erodibility(k) = erodibility(k)
 enddo
 
 rc = gf_create(p, GF_PREC_FP64, gf_ctx_handle)
@@ -116,10 +115,11 @@ h%phase_ratio = gf_addr8(phase_ratio)
 h%nopbou = gf_addr4(nopbou)
 h%bcstress = gf_addr8(bcstress)
 h%coh_scale = c_null_ptr
+h%dtopo = gf_addr8(dtopo); h%dhacc = gf_addr8(dhacc); h%dhacc_correction = gf_addr8(dhacc_correction)
 rc = gf_bind_host(gf_ctx_handle, h)
 if (rc /= GF_OK) call flac_hip_fail(rc)
 call flac_hip_bind_markers
-rc = gf_upload(gf_ctx_handle, ior(ior(GF_ALL, GF_MARKERS), GF_STRESSBC))
+rc = gf_upload(gf_ctx_handle, ior(ior(ior(GF_ALL, GF_MARKERS), GF_STRESSBC), merge(GF_TOPO, 0_c_int32_t, topo_kappa > 0.d0)))
 if (rc /= GF_OK) call flac_hip_fail(rc)
 ck%nloop = nloop; ck%nsrate = nsrate; ck%time = time; ck%dt = dt; ck%dtavg = dtavg
 ck%dt_elastic = dt_elastic; ck%dt_maxwell = dt_maxwell
@@ -138,25 +138,33 @@ implicit none
 type(gf_marker_arrays_t) :: m
 integer(c_int) :: rc
 m%nmarkers = nmarkers
-m%pad = 0
+m%capacity = max_markers   ! (src/marker_data.f90:27) resurface appends up to it
 m%mark_a1 = gf_addr8(mark_a1); m%mark_a2 = gf_addr8(mark_a2)
 m%mark_ntriag = gf_addr4(mark_ntriag)
 m%mark_phase = gf_addr1(mark_phase); m%mark_dead = gf_addr1(mark_dead)
 m%mark_id_elem = gf_addr4(mark_id_elem); m%nmark_elem = gf_addr4(nmark_elem)
+m%mark_age = gf_addr8(mark_age)
 rc = gf_bind_markers(gf_ctx_handle, m)
 if (rc /= GF_OK) call flac_hip_fail(rc)
 end subroutine flac_hip_bind_markers
 
-! Device -> Fortran arrays (before outflac / saveflac / remeshing read them).
+! Device -> Fortran arrays (before outflac / saveflac / remeshing read them).  With
+! surface processes the markers resurface added come back too, with their count.
 subroutine flac_hip_sync_host
 use iso_c_binding
 use gf_hip
 use flac_hip_state
+use params
+use marker_data
 implicit none
 integer(c_int) :: rc
+integer(c_int32_t) :: n
 if (.not. c_associated(gf_ctx_handle)) return
-rc = gf_download(gf_ctx_handle, ior(GF_ALL, GF_MARKERS))
+rc = gf_download(gf_ctx_handle, ior(ior(GF_ALL, GF_MARKERS), merge(GF_TOPO, 0_c_int32_t, topo_kappa > 0.d0)))
 if (rc /= GF_OK) call flac_hip_fail(rc)
+rc = gf_marker_count(gf_ctx_handle, n)
+if (rc /= GF_OK) call flac_hip_fail(rc)
+nmarkers = n
 end subroutine flac_hip_sync_host
 
 ! Fortran arrays -> device (after the host changed them, e.g. remeshing).
@@ -170,7 +178,7 @@ integer(c_int) :: rc
 type(gf_clock_t) :: ck
 if (.not. c_associated(gf_ctx_handle)) return
 call flac_hip_bind_markers
-rc = gf_upload(gf_ctx_handle, ior(GF_ALL, GF_MARKERS))
+rc = gf_upload(gf_ctx_handle, ior(ior(GF_ALL, GF_MARKERS), merge(GF_TOPO, 0_c_int32_t, topo_kappa > 0.d0)))
 if (rc /= GF_OK) call flac_hip_fail(rc)
 ck%nloop = nloop; ck%nsrate = nsrate; ck%time = time; ck%dt = dt; ck%dtavg = dtavg
 ck%dt_elastic = dt_elastic; ck%dt_maxwell = dt_maxwell

@@ -15,6 +15,7 @@ integer(c_int32_t), parameter :: GF_MARKERS = int(z'400000', c_int32_t)
 integer(c_int32_t), parameter :: GF_PHASE_RATIO = int(z'200000', c_int32_t), GF_IPHASE = int(z'100000', c_int32_t)
 integer(c_int32_t), parameter :: GF_APS = int(z'4000', c_int32_t)
 integer(c_int32_t), parameter :: GF_STRESSBC = int(z'1000000', c_int32_t)
+integer(c_int32_t), parameter :: GF_TOPO = int(z'2000000', c_int32_t)
 integer, parameter :: GF_OK = 0, GF_STOP_MAXWELL = 22, GF_STOP_NEGATIVE_AREA = 40
 
 type, bind(C) :: gf_params_t
@@ -38,6 +39,7 @@ type, bind(C) :: gf_params_t
     real(c_double) :: cohesion1(GF_MAXPH), cohesion2(GF_MAXPH), dilat1(GF_MAXPH), dilat2(GF_MAXPH)
     real(c_double) :: conduct(GF_MAXPH), cp(GF_MAXPH)
     real(c_double) :: topo_kappa
+    real(c_double) :: erodibility(GF_MAXPH)
 end type
 
 type, bind(C) :: gf_host_arrays_t
@@ -49,11 +51,13 @@ type, bind(C) :: gf_host_arrays_t
     type(c_ptr) :: phase_ratio
     type(c_ptr) :: nopbou, bcstress
     type(c_ptr) :: coh_scale
+    type(c_ptr) :: dtopo, dhacc, dhacc_correction
 end type
 
 type, bind(C) :: gf_marker_arrays_t
-    integer(c_int32_t) :: nmarkers, pad
+    integer(c_int32_t) :: nmarkers, capacity
     type(c_ptr) :: mark_a1, mark_a2, mark_ntriag, mark_phase, mark_dead, mark_id_elem, nmark_elem
+    type(c_ptr) :: mark_age
 end type
 
 type, bind(C) :: gf_clock_t
@@ -99,6 +103,11 @@ interface
         import :: c_int, c_ptr, gf_marker_arrays_t
         type(c_ptr), value :: ctx
         type(gf_marker_arrays_t), intent(in) :: m
+    end function
+    integer(c_int) function gf_marker_count(ctx, n) bind(C, name='gf_marker_count')
+        import :: c_int, c_int32_t, c_ptr
+        type(c_ptr), value :: ctx
+        integer(c_int32_t), intent(out) :: n
     end function
     integer(c_int) function gf_upload(ctx, mask) bind(C, name='gf_upload')
         import :: c_int, c_int32_t, c_ptr

@@ -23,7 +23,7 @@ EXPORTS = [
     "gf_surrogate_sample", "gf_surrogate_train", "gf_surrogate_enable", "gf_surrogate_stats",
     "gf_surrogate_weights", "gf_mlp_grad", "gf_mlp_infer", "gf_mlp_grad_noalloc",
     "gf_batch_create", "gf_batch_destroy", "gf_batch_last_error", "gf_batch_step", "gf_batch_set_graph",
-    "gf_batch_autotune", "gf_stream_probe",
+    "gf_batch_autotune", "gf_stream_probe", "gf_marker_count",
 ]
 MLP_NPARAM = 4737
 
@@ -52,6 +52,7 @@ def lib():
         L.gf_bind_markers.argtypes = [vp, C.POINTER(abi.GfMarkerArrays)]
         L.gf_upload.argtypes = [vp, C.c_uint32]
         L.gf_download.argtypes = [vp, C.c_uint32]
+        L.gf_marker_count.argtypes = [vp, C.POINTER(C.c_int32)]
         L.gf_set_clock.argtypes = [vp, C.POINTER(abi.GfClock)]
         L.gf_get_clock.argtypes = [vp, C.POINTER(abi.GfClock)]
         L.gf_step.argtypes = [vp, C.c_int32, C.POINTER(abi.GfClock)]
@@ -127,6 +128,8 @@ class Model:
                 mask |= abi.GF_COH_SCALE
             if self.params.nystressbc:
                 mask |= abi.GF_STRESSBC
+            if self.params.topo_kappa > 0.0:
+                mask |= abi.GF_TOPO
         return mask
 
     def upload(self, mask=abi.GF_ALL):
@@ -137,8 +140,16 @@ class Model:
         self._check(lib().gf_set_clock(self._ctx, C.byref(self.state.clock)))
 
     def download(self, mask=abi.GF_ALL):
-        self._check(lib().gf_download(self._ctx, self._mask(mask) & ~(abi.GF_COH_SCALE | abi.GF_STRESSBC)))
+        mask = self._mask(mask)
+        self._check(lib().gf_download(self._ctx, mask & ~(abi.GF_COH_SCALE | abi.GF_STRESSBC)))
         self._check(lib().gf_get_clock(self._ctx, C.byref(self.state.clock)))
+        if mask & abi.GF_MARKERS:  # (resurface appends markers with surface processes)
+            self.state.nmarkers = self.marker_count()
+
+    def marker_count(self):
+        n = C.c_int32()
+        self._check(lib().gf_marker_count(self._ctx, C.byref(n)))
+        return n.value
 
     def step(self, nsteps):
         """nsteps iterations of the reference's inner loop (flac; nloop+=1; time+=dt)."""

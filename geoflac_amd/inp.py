@@ -245,6 +245,7 @@ def to_gf_params(p, vbc, ivis_present, pisos=None):
         g.irheol[k] = int(ph["irheol"])
         for name in abi.PHASE_TABLES:
             getattr(g, name)[k] = float(ph[name])
+        g.erodibility[k] = float(ph["erodibility"])
     for k, ih in enumerate(p["inhoms"]):
         for name in ("ix1", "ix2", "iy1", "iy2", "igeom"):
             getattr(g, name)[k] = int(ih[name])
@@ -285,6 +286,10 @@ def read_params_txt(path):
                 g.irheol[k] = int(t[2])
                 for name, v in zip(abi.PHASE_TABLES, t[3:]):
                     getattr(g, name)[k] = float(v)
+            elif t[0] == "topo":
+                g.topo_kappa = float(t[1])
+                for k, v in enumerate(t[2:]):
+                    g.erodibility[k] = float(v)
             elif t[0] == "inhom":
                 for name, v in zip(("ix1", "ix2", "iy1", "iy2"), t[1:5]):
                     getattr(g, name)[ninh] = int(v)

@@ -377,8 +377,11 @@ def init_markers(p, st, interface_noise=0.0, seed=0):
         for j, i in cells:
             e = (j - 1) + ez * (i - 1)
             mphase[e * 9:(e + 1) * 9] = ih["inphase"]
+    # with surface processes resurface appends markers: room for max_markers of them
+    # (src/marker_data.f90:27), ages 0 as init_marker leaves them
+    cap = int(p["nz"]) * int(p["nx"]) * abi.MAX_MARKERS_PER_ELEM if float(p.get("topo_kappa", 0.0)) > 0.0 else None
     st.set_markers(mark_a1.ravel(), mark_a2.ravel(), ntri.ravel(), mphase.astype(np.int8),
-                   np.ones(ne * 9, dtype=np.int8), mark_id, nmark)
+                   np.ones(ne * 9, dtype=np.int8), mark_id, nmark, capacity=cap)
     # count_phase_ratio of every element (marker2elem, src/marker2elem.f90:60)
     nph = p["nphase"]
     npc = max(nph, int(mphase.max())) + 1
@@ -449,6 +452,22 @@ def perturb_layers(p, interface_noise, seed):
     return q
 
 
+def inhom_aps(p, st):
+    """The initial plastic strain of the inhomogeneities (init_phase,
+    src/init_phase.f90:21-47): xinitaps in rectangles (igeom 0) and in the 45-degree
+    bands of igeom 4 (single-precision slope, floor and ceiling, as the reference)."""
+    for ih in p.get("inhoms", []):
+        ix1, ix2, iy1, iy2 = ih["ix1"], ih["ix2"], ih["iy1"], ih["iy2"]
+        if ih["igeom"] == 0:
+            st.aps[iy1 - 1:iy2, ix1 - 1:ix2] = ih["xinitaps"]
+        elif ih["igeom"] == 4:
+            slope = np.float32(np.float32(iy2 - iy1) / np.float32(ix2 - ix1))
+            for j in range(ix1, ix2 + 1):
+                v = np.float32(slope * np.float32(j - ix1))
+                k1, k2 = int(np.floor(v)) + iy1, int(np.ceil(v)) + iy1
+                st.aps[k1 - 1:k2, j - 1] = ih["xinitaps"]
+
+
 def build_box(p, interface_noise=0.0, seed=0, markers=True, coh_noise=0.0):
     """State and parameter block of a new start (before the device-side set-up).
 
@@ -463,6 +482,7 @@ def build_box(p, interface_noise=0.0, seed=0, markers=True, coh_noise=0.0):
     st.cord[:, :, 0] = xm[None, :]
     zm = mesh1(p["z0"], p["z0"] + p["rzbo"], nz, p["nelz_y"], p["sizez_y"])
     st.cord[:, :, 1] = zm[:, None]
+    inhom_aps(p, st)
     if markers:
         init_markers(p, st, interface_noise, seed)
     else:

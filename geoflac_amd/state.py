@@ -16,6 +16,8 @@ NODE2 = ("cord", "vel", "force", "bc")
 NODE1 = ("temp", "temp0", "amass", "rmass")
 ELEM1 = ("aps", "visn", "e2sr", "sshrheat", "shrheat", "source")
 MARKERS = ("mark_a1", "mark_a2", "mark_ntriag", "mark_phase", "mark_dead", "mark_id_elem", "nmark_elem")
+# surface processes (topo_kappa > 0, src/arrays.f90:18): per surface node / element
+TOPO = ("dtopo", "dhacc", "dhacc_correction")
 
 
 class State:
@@ -48,22 +50,38 @@ class State:
         self.bcstress = np.zeros((rows, 3), **f8)
         # per-element cohesion factor (seeded heterogeneity; no reference counterpart)
         self.coh_scale = None
-        # markers (module marker_data, src/marker_data.f90:1-20), None until set
+        # topography change and its accumulations (diff_topo, src/fl_move.f90:138-315)
+        self.dtopo = np.zeros(self.nx, **f8)
+        self.dhacc = np.zeros(self.nx - 1, **f8)
+        self.dhacc_correction = np.zeros(self.nx, **f8)
+        # markers (module marker_data, src/marker_data.f90:1-20), None until set; the
+        # per-marker arrays may hold more entries than nmarkers (room for resurface)
         self.nmarkers = 0
         for n in MARKERS:
             setattr(self, n, None)
+        self.mark_age = None
 
-    def set_markers(self, a1, a2, ntriag, phase, dead, id_elem, nmark_elem):
+    def set_markers(self, a1, a2, ntriag, phase, dead, id_elem, nmark_elem, capacity=None, age=None):
         """Marker arrays in the reference's layout: per marker a1, a2 (barycentric),
         ntriag (triangle index), phase, dead (1 alive); per element id_elem[32, ez, ex]
-        (1-based marker numbers) and nmark_elem[ez, ex]."""
+        (1-based marker numbers) and nmark_elem[ez, ex].  capacity: entries the
+        per-marker arrays hold (at least len(a1); the rest zero), for the markers that
+        resurface appends; age: per-marker ages (kept only when given)."""
         ez, ex = self.nz - 1, self.nx - 1
-        self.nmarkers = int(len(a1))
-        self.mark_a1 = np.ascontiguousarray(a1, dtype=np.float64)
-        self.mark_a2 = np.ascontiguousarray(a2, dtype=np.float64)
-        self.mark_ntriag = np.ascontiguousarray(ntriag, dtype=np.int32)
-        self.mark_phase = np.ascontiguousarray(phase, dtype=np.int8)
-        self.mark_dead = np.ascontiguousarray(dead, dtype=np.int8)
+        n = int(len(a1))
+        cap = max(n, int(capacity or 0))
+        self.nmarkers = n
+
+        def pad(a, dt):
+            out = np.zeros(cap, dtype=dt)
+            out[:n] = np.asarray(a, dtype=dt)[:n]
+            return out
+        self.mark_a1 = pad(a1, np.float64)
+        self.mark_a2 = pad(a2, np.float64)
+        self.mark_ntriag = pad(ntriag, np.int32)
+        self.mark_phase = pad(phase, np.int8)
+        self.mark_dead = pad(dead, np.int8)
+        self.mark_age = None if age is None and capacity is None else pad(age if age is not None else np.zeros(n), np.float64)
         self.mark_id_elem = np.asfortranarray(np.asarray(id_elem, dtype=np.int32).reshape(
             (abi.MAX_MARKERS_PER_ELEM, ez, ex), order="F"))
         self.nmark_elem = np.asfortranarray(np.asarray(nmark_elem, dtype=np.int32).reshape((ez, ex), order="F"))
@@ -75,11 +93,18 @@ class State:
     def marker_arrays(self):
         m = abi.GfMarkerArrays()
         m.nmarkers = self.nmarkers
+        m.capacity = len(self.mark_a1)
         for name, ptype in abi.MARKER_FIELDS:
             a = getattr(self, name)
             assert a.flags.f_contiguous
             setattr(m, name, a.ctypes.data_as(ptype))
+        if self.mark_age is not None:
+            m.mark_age = self.mark_age.ctypes.data_as(abi._DP)
         return m
+
+    def live_markers(self, name):
+        """The first nmarkers entries of a per-marker array."""
+        return getattr(self, name)[:self.nmarkers]
 
     @property
     def nelem(self):
@@ -96,6 +121,8 @@ class State:
         if self.coh_scale is not None:
             assert self.coh_scale.flags.f_contiguous and self.coh_scale.dtype == np.float32
             h.coh_scale = self.coh_scale.ctypes.data_as(abi._FP)
+        for name in TOPO:
+            setattr(h, name, getattr(self, name).ctypes.data_as(abi._DP))
         return h
 
     def copy(self):
@@ -107,8 +134,13 @@ class State:
         s.bcstress[...] = self.bcstress
         if self.coh_scale is not None:
             s.coh_scale = self.coh_scale.copy(order="F")
+        for n in TOPO:
+            getattr(s, n)[...] = getattr(self, n)
         if self.has_markers:
-            s.set_markers(*(getattr(self, n).copy(order="F") for n in MARKERS))
+            arrs = [getattr(self, n).copy(order="F") for n in MARKERS]
+            s.set_markers(*arrs, capacity=len(self.mark_a1),
+                          age=None if self.mark_age is None else self.mark_age.copy())
+            s.nmarkers = self.nmarkers
         return s
 
     def clock_dict(self):
@@ -147,11 +179,16 @@ def load_dump(path, nphase=None):
     if os.path.exists(os.path.join(path, "nopbou.i32")):
         st.nopbou[...] = np.fromfile(os.path.join(path, "nopbou.i32"), dtype=np.int32).reshape(st.nopbou.shape, order="F")
         st.bcstress[...] = np.fromfile(os.path.join(path, "bcstress.f64")).reshape(st.bcstress.shape, order="F")
+    for name in TOPO:
+        fp = os.path.join(path, name + ".f64")
+        if os.path.exists(fp):
+            getattr(st, name)[...] = np.fromfile(fp)
     if os.path.exists(os.path.join(path, "mark_a1.f64")):
         rd = lambda n, dt: np.fromfile(os.path.join(path, n + (".f64" if dt is np.float64 else ".i32")), dtype=dt)
+        age = rd("mark_age", np.float64) if os.path.exists(os.path.join(path, "mark_age.f64")) else None
         st.set_markers(rd("mark_a1", np.float64), rd("mark_a2", np.float64), rd("mark_ntriag", np.int32),
                        rd("mark_phase", np.int32).astype(np.int8), rd("mark_dead", np.int32).astype(np.int8),
-                       rd("mark_id_elem", np.int32), rd("nmark_elem", np.int32))
+                       rd("mark_id_elem", np.int32), rd("nmark_elem", np.int32), age=age)
     elif os.path.exists(os.path.join(path, "mark_phase.i32")):
         st.mark_phase_only = np.fromfile(os.path.join(path, "mark_phase.i32"), dtype=np.int32).astype(np.int8)
     c = st.clock
@@ -164,13 +201,19 @@ def load_dump(path, nphase=None):
 
 def save_npz(state, path, fields=None):
     fields = state.FIELDS if fields is None else fields
-    d = {n: getattr(state, n) for n in fields if n != "mark_phase"}
-    if state.has_markers and fields is state.FIELDS:
-        d.update({n: getattr(state, n) for n in MARKERS})
+    d = {n: getattr(state, n) for n in fields if n != "mark_phase" and n not in MARKERS and n not in TOPO}
+    if state.has_markers and (fields is state.FIELDS or "mark_a1" in fields):
+        d.update({n: (state.live_markers(n) if n not in ("mark_id_elem", "nmark_elem") else getattr(state, n))
+                  for n in MARKERS})
+        if state.mark_age is not None:
+            d["mark_age"] = state.live_markers("mark_age")
+    for n in TOPO:
+        if n in fields:
+            d[n] = getattr(state, n)
     if fields is state.FIELDS:
         d["nopbou"], d["bcstress"] = state.nopbou, state.bcstress
-    elif "mark_phase" in fields:
-        mp = state.mark_phase if state.has_markers else getattr(state, "mark_phase_only", None)
+    elif "mark_phase" in fields and "mark_a1" not in fields:
+        mp = state.live_markers("mark_phase") if state.has_markers else getattr(state, "mark_phase_only", None)
         if mp is not None:
             d["mark_phase"] = mp
     d["_dims"] = np.array([state.nx, state.nz, state.nphase], dtype=np.int64)
@@ -190,8 +233,11 @@ def load_npz(path):
     if "nopbou" in z.files:
         st.nopbou[...] = z["nopbou"]
         st.bcstress[...] = z["bcstress"]
+    for n in TOPO:
+        if n in z.files:
+            getattr(st, n)[...] = z[n]
     if "mark_a1" in z.files:
-        st.set_markers(*(z[n] for n in MARKERS))
+        st.set_markers(*(z[n] for n in MARKERS), age=z["mark_age"] if "mark_age" in z.files else None)
     elif "mark_phase" in z.files:
         st.mark_phase_only = z["mark_phase"]
     st.clock.nloop, st.clock.nsrate = (int(v) for v in z["_clock_i"])

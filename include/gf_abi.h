@@ -73,6 +73,7 @@ typedef struct gf_params {
     double cohesion1[GF_MAXPH], cohesion2[GF_MAXPH], dilat1[GF_MAXPH], dilat2[GF_MAXPH];
     double conduct[GF_MAXPH], cp[GF_MAXPH];
     double topo_kappa;                      /* surface diffusion, src/fl_move.f90:40 */
+    double erodibility[GF_MAXPH];           /* erosion factor of the surface phase, src/fl_move.f90:170-174 */
 } gf_params;
 
 /* Raw pointers to the host arrays (module arrays, src/arrays.f90:9-53).  Any
@@ -94,6 +95,11 @@ typedef struct gf_host_arrays {
      * (src/rh_plastic.f90:231-291), for seeded heterogeneity; NULL or never uploaded
      * means 1 everywhere. */
     float *coh_scale;
+    /* surface processes (diff_topo, src/fl_move.f90:138-315; topo_kappa > 0 only):
+     * dtopo(nx) the step's topography change, dhacc(nx-1) its accumulation per surface
+     * element since the last resurfacing of it, dhacc_correction(nx) its accumulation
+     * per surface node since the last remeshing (module arrays, src/arrays.f90:18) */
+    double *dtopo, *dhacc, *dhacc_correction;
 } gf_host_arrays;
 
 /* field bits for gf_upload / gf_download */
@@ -105,9 +111,10 @@ enum {
     GF_E2SR = 1u << 16, GF_SSHRHEAT = 1u << 17, GF_SHRHEAT = 1u << 18, GF_SOURCE = 1u << 19,
     GF_IPHASE = 1u << 20, GF_PHASE_RATIO = 1u << 21,
     GF_ALL = (1u << 22) - 1u,   /* every array of module arrays above */
-    GF_MARKERS = 1u << 22,      /* upload: every marker array; download: mark_phase */
+    GF_MARKERS = 1u << 22,      /* upload: every marker array; download: mark_phase (every marker array with topo_kappa > 0) */
     GF_COH_SCALE = 1u << 23,    /* upload only */
-    GF_STRESSBC = 1u << 24      /* upload only: nopbou and bcstress */
+    GF_STRESSBC = 1u << 24,     /* upload only: nopbou and bcstress */
+    GF_TOPO = 1u << 25          /* dtopo, dhacc, dhacc_correction (topo_kappa > 0) */
 };
 
 /* Markers of module marker_data (src/marker_data.f90:1-20), read and written by
@@ -118,12 +125,16 @@ enum {
  * element's markers (1-based).  Only the first nmarkers entries are read. */
 #define GF_MAX_MARKERS_PER_ELEM 32   /* src/marker_data.f90:5 */
 typedef struct gf_marker_arrays {
-    int32_t nmarkers, pad;
+    /* capacity: entries the per-marker arrays hold (0: nmarkers).  With topo_kappa > 0
+     * resurface (src/fl_move.f90:318-419) appends markers up to it, as the reference
+     * up to max_markers (src/marker_data.f90:27, 77); gf_marker_count gives the count. */
+    int32_t nmarkers, capacity;
     double *mark_a1, *mark_a2;
     int32_t *mark_ntriag;
     int8_t *mark_phase, *mark_dead;
     int32_t *mark_id_elem;   /* (GF_MAX_MARKERS_PER_ELEM, nz-1, nx-1) */
     int32_t *nmark_elem;     /* (nz-1, nx-1) */
+    double *mark_age;        /* may be NULL: the ages of appended markers are then not kept */
 } gf_marker_arrays;
 
 /* Time-stepping scalars of module params that change during a run
@@ -159,6 +170,9 @@ int gf_bind_host(gf_ctx *ctx, const gf_host_arrays *h);
  * uploaded (GF_MARKERS); without them a step refuses (GF_ERR_STATE) when a phase with a
  * change_phase rule is present, and change_phase is a no-op otherwise. */
 int gf_bind_markers(gf_ctx *ctx, const gf_marker_arrays *m);
+/* The device's marker count (nmarkers of module marker_data), which resurface grows;
+ * gf_download(GF_MARKERS) copies that many markers back. */
+int gf_marker_count(gf_ctx *ctx, int32_t *nmarkers);
 /* Host -> device and device -> host copies of the fields in `mask`, with the
  * layout conversion to the device's own planes.  Upload of everything is what
  * the reference does at setflac/rsflac time (src/setflac.f90:4-100); download is

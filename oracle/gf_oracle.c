@@ -808,6 +808,8 @@ int gfo_fl_node(const gf_params *p, gf_host_arrays *s, gf_clock *c)
 /* ---------------------------------------------------------------------------
  * fl_move  src/fl_move.f90:4-132 (flexure and topography diffusion excluded)
  * ------------------------------------------------------------------------- */
+static int diff_topo(const gf_params *p, gf_host_arrays *s, gf_clock *c);
+
 int gfo_fl_move(const gf_params *p, gf_host_arrays *s, gf_clock *c)
 {
     static const int nd1[4] = {1, 3, 1, 1}, nd2[4] = {2, 2, 2, 4}, nd3[4] = {3, 4, 4, 3};
@@ -818,6 +820,11 @@ int gfo_fl_move(const gf_params *p, gf_host_arrays *s, gf_clock *c)
             CORD(j, i, 1) = CORD(j, i, 1) + VEL(j, i, 1) * dt;
             CORD(j, i, 2) = CORD(j, i, 2) + VEL(j, i, 2) * dt;
         }
+    /* fl_flexure (src/fl_move.f90:37) is refused with a flexural base; diff_topo :40 */
+    if (p->topo_kappa > 0.0) {
+        int st = diff_topo(p, s, c);
+        if (st) return st;
+    }
     for (int i = 1; i <= NX - 1; i++) {
         for (int j = 1; j <= NZ - 1; j++) {
             double xc[5], yc[5], xv[5], yv[5];
@@ -1146,9 +1153,9 @@ int gfo_init_visc(const gf_params *p, gf_host_arrays *s)
  * change_phase  src/change_phase.f90:1-260 (the itype_melting = 0 part), with
  * count_phase_ratio  src/marker_data.f90:140-174
  * ------------------------------------------------------------------------- */
-static const gf_marker_arrays *g_mk = NULL;
+static gf_marker_arrays *g_mk = NULL;
 
-void gfo_bind_markers(const gf_marker_arrays *m) { g_mk = m; }
+void gfo_bind_markers(gf_marker_arrays *m) { g_mk = m; }
 
 /* phase ids of src/phases.f90:4-22 */
 enum { KSED1 = 1, KCONT1 = 2, KOCEAN1 = 3, KMANT1 = 4, KSCHIST = 5, KCONT2 = 6, KOCEAN2 = 7, KMANT2 = 8,
@@ -1163,7 +1170,7 @@ static double pr_or0(const gf_params *p, const gf_host_arrays *s, int ph, int j,
 
 int gfo_change_phase(const gf_params *p, gf_host_arrays *s, gf_clock *c)
 {
-    const gf_marker_arrays *m = g_mk;
+    gf_marker_arrays *m = g_mk;
     (void)c;
     if (!m) return 0;
     const double max_basalt_depth = 150.e3, min_eclogite_temp = 400.0, min_eclogite_depth = 20e3,
@@ -1280,6 +1287,168 @@ int gfo_change_phase(const gf_params *p, gf_host_arrays *s, gf_clock *c)
             APS(j, i) = v > 0.0 ? v : 0.0;
         }
     free(itmp);
+    return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * diff_topo  src/fl_move.f90:138-315 (topo_kappa > 0, no melting: the arc extrusion
+ * and the mid-ocean-ridge part add zero, fmelt and fmagma2 being zero), with
+ * resurface :318-419, add_marker_at_top :432-472, add_marker
+ * (src/marker_data.f90:41-104), check_inside (src/euler2bar.f90:102-161), myrandom
+ * (src/myrandom_mod.f90:9-16) and count_phase_ratio (src/marker_data.f90:140-174)
+ * ------------------------------------------------------------------------- */
+static double topo_myrandom(int *iseed)
+{
+    *iseed = *iseed % 134456;
+    *iseed = (8121 * *iseed + 28411) % 134456;
+    return (*iseed * 1.0) / 134456;
+}
+
+static int topo_check_inside(const gf_params *p, const gf_host_arrays *s, double x, double y, double *bar1,
+                             double *bar2, int *ntr, int i, int j)
+{
+    for (int k = 1; k <= 2; k++) {
+        double x1, x2, x3, y1, y2, y3;
+        if (k == 1) {
+            x1 = CORD(j, i, 1); x2 = CORD(j + 1, i, 1); x3 = CORD(j, i + 1, 1);
+            y1 = CORD(j, i, 2); y2 = CORD(j + 1, i, 2); y3 = CORD(j, i + 1, 2);
+        } else {
+            x1 = CORD(j, i + 1, 1); x2 = CORD(j + 1, i, 1); x3 = CORD(j + 1, i + 1, 1);
+            y1 = CORD(j, i + 1, 2); y2 = CORD(j + 1, i, 2); y3 = CORD(j + 1, i + 1, 2);
+        }
+        double det = ((x2 * y3 - y2 * x3) - (x1 * y3 - y1 * x3) + (x1 * y2 - y1 * x2));
+        double x11 = (x2 * y3 - y2 * x3) / det, x12 = (y2 - y3) / det, x13 = (x3 - x2) / det;
+        double x21 = (x3 * y1 - y3 * x1) / det, x22 = (y3 - y1) / det, x23 = (x1 - x3) / det;
+        *bar1 = x11 + x12 * x + y * x13;
+        *bar2 = x21 + x22 * x + y * x23;
+        if (*bar1 >= 0.0 && *bar2 >= 0.0 && *bar1 + *bar2 <= 1.0) {
+            *ntr = 2 * ((NZ - 1) * (i - 1) + j - 1) + k;
+            return 1;
+        }
+    }
+    return 0;
+}
+
+/* add_marker: 1 added, 0 outside the element, -1 skipped (no room) */
+static int topo_add_marker(const gf_params *p, const gf_host_arrays *s, gf_marker_arrays *m, double x, double y,
+                           int iph, double age, int j, int i)
+{
+    double bar1, bar2;
+    int ntr;
+    if (!topo_check_inside(p, s, x, y, &bar1, &bar2, &ntr, i, j)) return 0;
+    const long cap = m->capacity > 0 ? m->capacity : m->nmarkers;
+    const int kk = m->nmarkers + 1;
+    const long e = IE(j, i);
+    const int nm = m->nmark_elem[e] + 1;
+    if (nm > GF_MAX_MARKERS_PER_ELEM || kk > cap) return -1;
+    m->nmarkers = kk;
+    m->nmark_elem[e] = nm;
+    m->mark_id_elem[(nm - 1) + (long)GF_MAX_MARKERS_PER_ELEM * e] = kk;
+    m->mark_dead[kk - 1] = 1;
+    m->mark_a1[kk - 1] = bar1;
+    m->mark_a2[kk - 1] = bar2;
+    if (m->mark_age) m->mark_age[kk - 1] = age;
+    m->mark_ntriag[kk - 1] = ntr;
+    m->mark_phase[kk - 1] = (int8_t)iph;
+    return 1;
+}
+
+static void topo_count_phase_ratio(const gf_params *p, gf_host_arrays *s, const gf_marker_arrays *m, int j, int i)
+{
+    int nm = m->nmark_elem[IE(j, i)], cnt[GF_MAXPH + 1] = {0};
+    for (int q = 0; q < nm; q++) cnt[m->mark_phase[m->mark_id_elem[q + (long)GF_MAX_MARKERS_PER_ELEM * IE(j, i)] - 1]] += 1;
+    for (int kp = 1; kp <= p->nphase; kp++) PR(kp, j, i) = cnt[kp] / (nm * 1.0);
+    int iph = 1, nmax = 0;
+    for (int kp = 1; kp <= p->nphase; kp++)
+        if (cnt[kp] > nmax) { nmax = cnt[kp]; iph = kp; }
+    IPHASE(j, i) = iph;
+}
+
+static int resurface(const gf_params *p, gf_host_arrays *s, gf_clock *c)
+{
+    gf_marker_arrays *m = g_mk;
+    if (!m) return 0;
+    for (int i = 1; i <= NX - 1; i++) {
+        const double elz = 0.5 * (CORD(1, i, 2) - CORD(2, i, 2) + CORD(1, i + 1, 2) - CORD(2, i + 1, 2));
+        const double chgtopo = s->dhacc[i - 1];
+        const long e = IE(1, i);
+        const int kinc = m->nmark_elem[e];
+        int *ids = m->mark_id_elem + (long)GF_MAX_MARKERS_PER_ELEM * e;
+        int ichanged = 0;
+        if (-chgtopo * kinc >= elz && kinc > 1) {
+            double ymax = -1e30;
+            int kmax = 0;
+            for (int k = 1; k <= kinc; k++) {
+                int n = ids[k - 1], mm = m->mark_ntriag[n - 1];
+                int kk = (mm - 1) % 2 + 1, jj = ((mm - kk) / 2) % (NZ - 1) + 1, ii = (mm - kk) / 2 / (NZ - 1) + 1;
+                double ba1 = m->mark_a1[n - 1], ba2 = m->mark_a2[n - 1], ba3 = 1.0 - ba1 - ba2, y;
+                int i1, i2, i3, j1, j2, j3;
+                if (kk == 1) { i1 = ii; i2 = ii; i3 = ii + 1; j1 = jj; j2 = jj + 1; j3 = jj; }
+                else { i1 = ii + 1; i2 = ii; i3 = ii + 1; j1 = jj; j2 = jj + 1; j3 = jj + 1; }
+                y = CORD(j1, i1, 2) * ba1 + CORD(j2, i2, 2) * ba2 + CORD(j3, i3, 2) * ba3;
+                if (ymax < y) { ymax = y; kmax = k; }
+            }
+            if (kmax != 0) {
+                int nmax = ids[kmax - 1];
+                ids[kmax - 1] = ids[kinc - 1];
+                ids[kinc - 1] = 0;
+                m->mark_dead[nmax - 1] = 0;
+                m->nmark_elem[e] = m->nmark_elem[e] - 1;
+            }
+            s->dhacc[i - 1] = 0.0;
+            ichanged = 1;
+        }
+        if (chgtopo * kinc >= elz && kinc != GF_MAX_MARKERS_PER_ELEM) {
+            /* add_marker_at_top(i, 0.1d0, time, nloop, ksed1) */
+            int iseed = c->nloop + i, icount = 0;
+            for (;;) {
+                double r1 = topo_myrandom(&iseed), r2 = topo_myrandom(&iseed);
+                double x1 = CORD(1, i, 1) * (1 - r1) + CORD(1, i + 1, 1) * r1;
+                double y1 = CORD(1, i, 2) * (1 - r1) + CORD(1, i + 1, 2) * r1;
+                double x2 = CORD(2, i, 1) * (1 - r1) + CORD(2, i + 1, 1) * r1;
+                double y2 = CORD(2, i, 2) * (1 - r1) + CORD(2, i + 1, 2) * r1;
+                r2 = r2 * 0.1;
+                double xx = x1 * (1 - r2) + x2 * r2, yy = y1 * (1 - r2) + y2 * r2;
+                int inc = topo_add_marker(p, s, m, xx, yy, KSED1, c->time, 1, i);
+                if (inc == 1 || inc == -1) break;
+                icount = icount + 1;
+                if (icount > 100) return 134;
+            }
+            s->dhacc[i - 1] = 0.0;
+            ichanged = 1;
+        }
+        /* extrusion (:400-411): extr_acc stays 0 without melting, so only an element of
+         * non-positive thickness takes this branch, adding no marker */
+        if (0.0 * kinc >= elz && kinc != GF_MAX_MARKERS_PER_ELEM) ichanged = 1;
+        if (ichanged) topo_count_phase_ratio(p, s, m, 1, i);
+    }
+    return 0;
+}
+
+static int diff_topo(const gf_params *p, gf_host_arrays *s, gf_clock *c)
+{
+    const double dt = c->dt;
+    double *stmpn = (double *)malloc(sizeof(double) * NX);
+    if (!stmpn) return 1000;
+    for (int i = 1; i <= NX; i++) stmpn[i - 1] = p->topo_kappa;
+    for (int i = 2; i <= NX - 1; i++) {
+        double snder = (stmpn[i] * (CORD(1, i + 1, 2) - CORD(1, i, 2)) / (CORD(1, i + 1, 1) - CORD(1, i, 1)) -
+                        stmpn[i - 2] * (CORD(1, i, 2) - CORD(1, i - 1, 2)) / (CORD(1, i, 1) - CORD(1, i - 1, 1))) /
+                       (CORD(1, i + 1, 1) - CORD(1, i - 1, 1));
+        if (snder < 0) snder = snder * PH(erodibility, IPHASE(1, i));
+        s->dtopo[i - 1] = dt * snder;
+    }
+    free(stmpn);
+    s->dtopo[0] = s->dtopo[1];
+    s->dtopo[NX - 1] = s->dtopo[NX - 2];
+    for (int i = 1; i <= NX; i++) {
+        double h = 0.5 * (CORD(1, i, 2) - CORD(2, i, 2));
+        s->dtopo[i - 1] = s->dtopo[i - 1] < h ? s->dtopo[i - 1] : h;
+    }
+    for (int i = 1; i <= NX; i++) s->dhacc_correction[i - 1] = s->dhacc_correction[i - 1] + s->dtopo[i - 1];
+    for (int i = 1; i <= NX - 1; i++) s->dhacc[i - 1] = s->dhacc[i - 1] + 0.5 * (s->dtopo[i - 1] + s->dtopo[i]);
+    for (int i = 1; i <= NX; i++) CORD(1, i, 2) = CORD(1, i, 2) + s->dtopo[i - 1];
+    if (c->nloop % p->ifreq_avgsr == 0) return resurface(p, s, c);
     return 0;
 }
 

@@ -23,7 +23,7 @@ int gfo_fl_move(const gf_params *p, gf_host_arrays *s, gf_clock *c);
 int gfo_rmasses(const gf_params *p, gf_host_arrays *s, gf_clock *c);
 int gfo_dt_mass(const gf_params *p, gf_host_arrays *s, gf_clock *c);
 /* change_phase runs on the markers bound here (NULL: none, it does nothing) */
-void gfo_bind_markers(const gf_marker_arrays *m);
+void gfo_bind_markers(gf_marker_arrays *m);
 int gfo_change_phase(const gf_params *p, gf_host_arrays *s, gf_clock *c);
 int gfo_bc_update(const gf_params *p, gf_host_arrays *s, gf_clock *c);
 int gfo_init_areas(const gf_params *p, gf_host_arrays *s);

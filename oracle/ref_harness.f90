@@ -175,6 +175,22 @@ subroutine dump_state(base, step)
     ! markers (module marker_data, src/marker_data.f90:1-20): the first nmarkers entries
     call wr4(dir, 'mark_phase', int(mark_phase(1:nmarkers)), nmarkers)
     call wr4(dir, 'nmark_elem', nmark_elem, ne)
+    ! surface processes (diff_topo, src/fl_move.f90:138-315): the topography change of the
+    ! step and its accumulations, and the markers resurface added or removed
+    if (topo_kappa > 0.d0) then
+        call wr8(dir, 'dtopo', dtopo, nx)
+        call wr8(dir, 'dhacc', dhacc, nx-1)
+        call wr8(dir, 'dhacc_correction', dhacc_correction, nx)
+        call wr8(dir, 'extr_acc', extr_acc, nx-1)
+        call wr8(dir, 'mark_age', mark_age, nmarkers)
+        if (step > 0) then
+            call wr8(dir, 'mark_a1', mark_a1, nmarkers)
+            call wr8(dir, 'mark_a2', mark_a2, nmarkers)
+            call wr4(dir, 'mark_ntriag', mark_ntriag, nmarkers)
+            call wr4(dir, 'mark_dead', int(mark_dead(1:nmarkers)), nmarkers)
+            call wr4(dir, 'mark_id_elem', mark_id_elem, max_markers_per_elem*ne)
+        endif
+    endif
     if (step == 0) then
         ! stress boundary segments (init_bc, src/init_bc.f90:226-280)
         call wr4(dir, 'nopbou', nopbou, size(nopbou))
@@ -216,6 +232,7 @@ subroutine dump_params(base)
             acoef(k), eactiv(k), vactiv(k), rl(k), rm(k), plstrain1(k), plstrain2(k), fric1(k), &
             fric2(k), cohesion1(k), cohesion2(k), dilat1(k), dilat2(k), conduct(k), cp(k)
     enddo
+    write(u, '(A,99(1X,ES25.17E3))') 'topo ', topo_kappa, (erodibility(k), k = 1, nphase)
     do k = 1, inhom
         write(u, '(A,6I8,1X,ES25.17E3)') 'inhom ', ix1(k), ix2(k), iy1(k), iy2(k), inphase(k), igeom(k), xinitaps(k)
     enddo

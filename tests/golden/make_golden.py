@@ -51,12 +51,18 @@ CASES = {
     # box60x20 with the driver's remeshing (GF_REM=1 of the harness) every few tens of
     # steps: five remeshings in 300 steps
     "remesh60x20": dict(inp="remesh60x20.inp", steps=[100, 300], env={"GF_REM": "1"}),
+    # surface processes (diff_topo with resurface): topography diffusion over a weakened
+    # block; resurface appends sediment markers from step ~5000 and removes eroded ones
+    # by 12000
+    "topo60x20": dict(inp="topo60x20.inp", steps=[10, 100, 1000, 5000, 8000, 12000],
+                      fields=MUTABLE + ["dtopo", "dhacc", "dhacc_correction", "mark_a1", "mark_a2", "mark_ntriag",
+                                        "mark_dead", "mark_id_elem", "nmark_elem"]),
 }
 
 
 # files outflac (src/outflac.f90) and saveflac (src/saveflac.f90) write; Amin.rs (a
 # list-directed scalar) and the files setflac writes are not among them
-OUTPUT_CASES = {"box60x20": 10, "hetero80x40": 10}
+OUTPUT_CASES = {"box60x20": 10, "hetero80x40": 10, "topo60x20": 10}
 SETUP_FILES = {"nxnz.0", "pisos.rs", "vbc.s", "output.asc", "Amin.rs"}
 
 

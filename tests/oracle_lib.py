@@ -49,16 +49,46 @@ def _bind_markers(L, st):
 def step(p, st, nsteps, cr=False):
     h = st.host_arrays()
     L = lib(cr)
-    keep = _bind_markers(L, st)  # noqa: F841 (kept alive over the call)
+    keep = _bind_markers(L, st)  # (kept alive over the call)
     rc = L.gfo_step(C.byref(p), C.byref(h), C.byref(st.clock), int(nsteps))
     L.gfo_bind_markers(None)
+    if keep is not None:
+        st.nmarkers = keep.nmarkers  # (resurface appends markers)
     return rc
 
 
 def call(name, p, st, cr=False):
     h = st.host_arrays()
     L = lib(cr)
-    keep = _bind_markers(L, st)  # noqa: F841
+    keep = _bind_markers(L, st)
     rc = getattr(L, name)(C.byref(p), C.byref(h), C.byref(st.clock))
     L.gfo_bind_markers(None)
+    if keep is not None:
+        st.nmarkers = keep.nmarkers
     return rc
+
+
+def golden_full(case, step, marker_room=False):
+    """The complete state after `step` reference steps: the fields the step never
+    changes from the step-0 golden, the others (markers included, when the golden
+    holds them all) from the step-N golden.  marker_room: room for the markers
+    resurface appends, max_markers entries (nz nx 32, src/marker_data.f90:27)."""
+    s0 = golden_state(case, 0)
+    sn = s0 if step == 0 else golden_state(case, step)
+    if step:
+        for f in sn.present:
+            if f.startswith("_") or f in state.MARKERS or f == "mark_age":
+                continue
+            if f == "mark_phase":
+                s0.mark_phase[...] = sn.mark_phase_only
+            else:
+                getattr(s0, f)[...] = getattr(sn, f)
+        C.memmove(C.addressof(s0.clock), C.addressof(sn.clock), C.sizeof(abi.GfClock))
+    src = sn if (step and sn.has_markers) else s0
+    if src.has_markers:
+        n = src.nmarkers
+        arrs = [getattr(src, k)[:n] if k not in ("mark_id_elem", "nmark_elem") else getattr(src, k)
+                for k in state.MARKERS]
+        s0.set_markers(*arrs, capacity=s0.nz * s0.nx * abi.MAX_MARKERS_PER_ELEM if marker_room else None,
+                       age=None if src.mark_age is None else src.mark_age[:n])
+    return s0

@@ -69,12 +69,18 @@ def _run_dropin(tmp_path, case, dumps, env=None):
     for step in dumps:
         st = state.load_dump(os.path.join(out, "step_%07d" % step))
         ref = ol.golden_state(case, step)
-        assert np.array_equal(st.mark_phase_only, ref.mark_phase_only)
+        if ref.has_markers:  # (surface processes: resurface appends and removes markers)
+            assert st.nmarkers == ref.nmarkers
+            for f in ("mark_phase", "mark_dead", "mark_ntriag", "mark_id_elem", "nmark_elem"):
+                assert np.array_equal(getattr(st, f), getattr(ref, f)), f
+        else:
+            assert np.array_equal(st.mark_phase_only, ref.mark_phase_only)
         assert np.array_equal(st.iphase, ref.iphase)
         assert np.array_equal(st.phase_ratio, ref.phase_ratio)
         worst = {}
+        topo = ("dtopo", "dhacc", "dhacc_correction", "mark_a1", "mark_a2") if ref.has_markers else ()
         for f in ("cord", "vel", "temp", "stress0", "aps", "strain", "area", "visn", "e2sr", "se2sr",
-                  "rmass", "amass", "temp0"):
+                  "rmass", "amass", "temp0") + topo:
             a, b = getattr(st, f), getattr(ref, f)
             worst[f] = 0.0 if np.array_equal(a, b) else state.rel_l2(a, b)
         print("  step %d: %s" % (step, {k: "%.1e" % v for k, v in worst.items() if v}), flush=True)
@@ -85,12 +91,45 @@ def _run_dropin(tmp_path, case, dumps, env=None):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case,dumps", [("box60x20", (10, 100)), ("phases60x30", (11, 100)), ("bcs60x20", (10, 100))])
+@pytest.mark.parametrize("case,dumps", [("box60x20", (10, 100)), ("phases60x30", (11, 100)), ("bcs60x20", (10, 100)),
+                                        ("topo60x20", (10, 1000))])
 def test_fortran_dropin_matches_reference(tmp_path, case, dumps):
-    """box60x20, and phases60x30 whose markers change phase on the device (the marker
-    phases, phase ratios and iphase come back to the Fortran arrays)."""
+    """box60x20; phases60x30, whose markers change phase on the device (the marker
+    phases, phase ratios and iphase come back to the Fortran arrays); bcs60x20 with
+    stress boundaries; topo60x20 with surface processes (the topography arrays come
+    back to the Fortran module)."""
     print("case: Fortran drop-in, %d steps of %s (dumps at %s)" % (max(dumps), case, dumps), flush=True)
     _run_dropin(tmp_path, case, dumps)
+
+
+@pytest.mark.gpu
+def test_fortran_dropin_resurface_markers_match_the_oracle(tmp_path):
+    """topo60x20 through the drop-in for 8000 steps: resurface appends sediment markers
+    on the device from step ~5000 on, and flac_hip_sync_host brings them back to the
+    Fortran arrays with their count.  Over 8000 steps the device's correctly rounded
+    libm and the reference's host libm take different paths (the run is chaotic), so
+    the dump is compared with the correctly rounded oracle from the reference's step-0
+    state, bit for bit: every field, the topography arrays and the markers."""
+    import re
+    if not os.path.exists(EXE):
+        pytest.skip("oracle/_ref/dropin_harness is built only where the reference sources are present")
+    case, n = "topo60x20", 8000
+    inp = os.path.join(ROOT, "tests", "golden", "inputs", case + ".inp")
+    print("case: Fortran drop-in, %d steps of %s against the correctly rounded oracle" % (n, case), flush=True)
+    r = subprocess.run([EXE, inp, str(tmp_path / "out"), str(n), str(n)], cwd=str(tmp_path),
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    st = state.load_dump(os.path.join(str(tmp_path / "out"), "step_%07d" % n))
+    p = ol.golden_params(case)
+    ref = ol.golden_full(case, 0, marker_room=True)
+    assert ol.step(p, ref, n, cr=True) == 0
+    assert st.nmarkers == ref.nmarkers > ol.golden_state(case, 0).nmarkers, (st.nmarkers, ref.nmarkers)
+    bad = [f for f in state.State.FIELDS + list(state.TOPO) + ["mark_id_elem", "nmark_elem"]
+           if not np.array_equal(getattr(st, f), getattr(ref, f))]
+    bad += [f for f in ("mark_a1", "mark_a2", "mark_ntriag", "mark_phase", "mark_dead", "mark_age")
+            if not np.array_equal(st.live_markers(f), ref.live_markers(f))]
+    print("  markers %d, differing %s" % (st.nmarkers, bad), flush=True)
+    assert not bad, bad
 
 
 @pytest.mark.gpu

@@ -17,7 +17,7 @@ from tests import oracle_lib as ol
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLDEN = os.path.join(ROOT, "tests", "golden")
-CASES = [("box60x20", 10), ("hetero80x40", 10)]
+CASES = [("box60x20", 10), ("hetero80x40", 10), ("topo60x20", 10)]
 
 
 def _say(*a):
@@ -38,12 +38,11 @@ def _host_arrays(ref, nmarkers):
 @pytest.mark.parametrize("case,step", CASES)
 def test_output_files_byte_equal_to_reference(case, step, tmp_path):
     from geoflac_amd import gpu
-    from tests.test_gpu_parity import golden_full
     _say("case: outflac and saveflac of the %d-step state of %s" % (step, case))
     ref = dict(np.load(os.path.join(GOLDEN, "output_%s_s%d.npz" % (case, step))))
     deck = inp.read_inp(os.path.join(GOLDEN, "inputs", case + ".inp"))
     p = ol.golden_params(case)
-    st = golden_full(step, case)
+    st = ol.golden_full(case, step)
     arrays = _host_arrays(ref, st.nmarkers)
     with gpu.Model(p, st) as m:
         opts = m.output_opts(io_flags=deck["io_flags"], lastout=deck["lastout"], hs=deck["hs"], **arrays)

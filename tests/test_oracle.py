@@ -26,7 +26,7 @@ def test_inp_reader_matches_reference_read_params():
 
 def test_params_layout_matches_header():
     # 548 int32 (even count, no hidden padding) followed by 23 + 19*20 + 1 doubles
-    assert abi.SIZEOF_PARAMS == 548 * 4 + (23 + 19 * abi.MAXPH + 1) * 8
+    assert abi.SIZEOF_PARAMS == 548 * 4 + (23 + 20 * abi.MAXPH + 1) * 8
     assert abi.SIZEOF_CLOCK == 8 + 5 * 8
     # the C compiler's layout of include/gf_abi.h (through the oracle, built against it)
     out = (C.c_long * 5)()
@@ -66,6 +66,35 @@ def test_oracle_bitwise_equal_to_reference(case, nsteps):
     assert st.clock.dt == ref.clock.dt
     assert st.clock.dtavg == ref.clock.dtavg
     assert st.clock.dt_elastic == ref.clock.dt_elastic
+
+
+@pytest.mark.parametrize("start,end", [(0, 1000), (5000, 12000)])
+def test_surface_processes_bitwise_equal_to_reference(start, end):
+    """topo60x20 (topography diffusion with resurface, src/fl_move.f90:138-472): from
+    the reference's state after `start` steps, the oracle reproduces the reference's
+    state after `end` steps bit for bit: the topography change and its accumulations,
+    the surface, and from step ~5000 on the markers resurface removes (erosion: dead,
+    out of their element's list) and appends (sediment markers at add_marker_at_top's
+    random points: triangles, barycentric coordinates, ages), with the recounted phase
+    ratios and dominant phases."""
+    p = ol.golden_params("topo60x20")
+    assert p.topo_kappa > 0.0 and p.erodibility[1] == 1.0
+    st = ol.golden_full("topo60x20", start, marker_room=True)
+    assert ol.step(p, st, end - start) == 0
+    ref = ol.golden_state("topo60x20", end)
+    assert st.nmarkers == ref.nmarkers
+    for f in ref.present:
+        if f.startswith("_"):
+            continue
+        a = st.live_markers(f) if f in ("mark_a1", "mark_a2", "mark_ntriag", "mark_phase", "mark_dead",
+                                        "mark_age") else getattr(st, f)
+        b = getattr(ref, f)
+        assert np.array_equal(a, b), (f, state.rel_l2(a, b))
+    assert st.clock.nloop == ref.clock.nloop == end
+    assert st.clock.dt == ref.clock.dt
+    if end == 12000:  # resurface both appended and removed markers on the way
+        s5 = ol.golden_state("topo60x20", 5000)
+        assert ref.nmarkers > s5.nmarkers and (ref.mark_dead == 0).sum() > (s5.mark_dead == 0).sum()
 
 
 @pytest.mark.parametrize("case,nsteps,floor", [("hetero80x40", 1000, 1e-10), ("phases60x30", 1000, 1e-8),
