@@ -41,6 +41,9 @@ struct GfDevArrays {
     // surface processes (topo_kappa > 0): dtopo(nx), dhacc(nx-1), dhacc_correction(nx),
     // and nx doubles of scratch
     double *dtopo, *dhacc, *dhacc_corr, *topo_tmp;
+    // iphase with IPHX_PURE set where the element's phase ratios are exactly 1 for
+    // iphase and 0 for every other phase (kept with iphase and the ratio planes)
+    int *iphx;
     // stress boundary conditions (bc_update): per segment its two nodes (linear
     // indices) and normal and shear stress; per boundary node (edge index, bc_edge)
     // its segments in increasing order, as 2 * segment + (0: first node, 1: second);
@@ -50,6 +53,9 @@ struct GfDevArrays {
     const int *bc_estart, *bc_elist;
     double *bc_press;
 };
+
+#define IPHX_PHASE 0xff
+#define IPHX_PURE 0x100
 
 // the isotropic reference added back to an SV stress (FP64 path: none)
 template <typename SV> GF_DEV double ref_press(const GfDevArrays &A, long e);
@@ -1661,14 +1667,20 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, const GfPhase *__res
     const Buf bSE = mkbuf(A.se2sr, 3ull * pe), bST = mkbuf(A.strain, 3ull * pe);
     const Buf bAPS = mkbuf(A.aps, pe), bVIS = mkbuf(A.visn, pe), bSH = mkbuf(A.sshrheat, pe);
     const Buf bV = mkbuf(sv_ptr<SV>(A.vel, A.velf), 2ull * pnv), bC = mkbuf(A.cord, 2ull * pn);
-    const Buf bIP = mkbuf(A.iphase, 4ull * ne);
+    const Buf bIP = mkbuf(A.iphx, 4ull * ne);
     // phase ratios: GF_MAXACT planes are read; those past nactive lie outside the
     // buffer and read 0 without touching memory
     const Buf bPR = mkbuf(A.pr, (unsigned long long)P->nactive * pe);
-    const int iph = bldi(bIP, (unsigned)e * 4u, 0);
+    const int iphx = bldi(bIP, (unsigned)e * 4u, 0), iph = iphx & IPHX_PHASE;
+    // an element of one phase (ratio exactly 1, the others 0) reads no ratio plane
     double prv[GF_MAXACT];
+    if (iphx & IPHX_PURE) {
 #pragma unroll
-    for (int a = 0; a < GF_MAXACT; a++) prv[a] = bld(bPR, eo, a * pe);
+        for (int a = 0; a < GF_MAXACT; a++) prv[a] = P->active[a] == iph ? 1.0 : 0.0;
+    } else {
+#pragma unroll
+        for (int a = 0; a < GF_MAXACT; a++) prv[a] = bld(bPR, eo, a * pe);
+    }
     double aps = bld(bAPS, eo, 0);
     const float csc = A.coh_scale ? A.coh_scale[e] : 1.0f;
     const double vis = bld(bVIS, eo, 0);  // refreshed by k_visc on viscosity-update steps

@@ -351,11 +351,37 @@ GF_DEV void phase_recount_body(const GfDevParams *__restrict__ P, GfDevClock *ck
         atomicAdd(&ck->ph_count[iph - 1], 1);
         A.iphase[e] = iph;
     }
+    // one phase: every marker holds it (ratio nm/nm = 1, the others 0)
+    A.iphx[e] = iph | (nmax == nm ? IPHX_PURE : 0);
     // the recrystallised markers lose their share of the plastic strain
     if (!aps_share) return;
     const double aps = A.aps[e];
     const double v = aps - aps / (double)(float)nm;
     A.aps[e] = v > 0.0 ? v : 0.0;
+}
+
+// the one-phase flag of every element from iphase and the ratio planes (after an upload)
+GF_DEV void iphx_body(const GfDevParams *__restrict__ P, const GfDevArrays &A, long e)
+{
+    const int iph = A.iphase[e];
+    bool found = false, pure = true;
+    for (int a = 0; a < P->nactive; a++) {
+        const double r = A.pr[(long)a * NE + e];
+        if (P->active[a] == iph) {
+            found = true;
+            pure = pure && r == 1.0;
+        } else {
+            pure = pure && r == 0.0;
+        }
+    }
+    A.iphx[e] = iph | (found && pure ? IPHX_PURE : 0);
+}
+
+__global__ void __launch_bounds__(256) k_iphx(const GfDevParams *__restrict__ P, GfDevArrays A)
+{
+    const long e = xcd_lin();
+    if (e >= NE) return;
+    iphx_body(P, A, e);
 }
 
 __global__ void __launch_bounds__(256) k_phase_recount(const GfDevParams *__restrict__ P, GfDevClock *ck,

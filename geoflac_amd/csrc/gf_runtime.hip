@@ -336,6 +336,11 @@ extern "C" int gf_create(const gf_params *p, int32_t precision, gf_ctx **out)
     ALLOC(ctx->therm_border, 8 * (size_t)TH_NB * ((p->nz + TH_OJ - 1) / TH_OJ) * ((p->nx + TH_OI - 1) / TH_OI));
     ctx->a.tbord = ctx->therm_border;
     ALLOC(ctx->a.dens, 8 * ne);
+    ALLOC(ctx->a.iphx, 4 * ne);
+    if (hipMemset(ctx->a.iphx, 0, 4 * ne) != hipSuccess) {
+        rc = set_err(ctx, GF_ERR_HIP, "hipMemset failed");
+        goto fail;
+    }
     if (p->topo_kappa > 0.0) {
         ALLOC(ctx->a.dtopo, 8 * (size_t)p->nx);
         ALLOC(ctx->a.dhacc, 8 * (size_t)p->nx);
@@ -380,7 +385,7 @@ extern "C" void gf_destroy(gf_ctx *ctx)
                     ctx->a.forcef, ctx->a.pref, ctx->sur_samples, ctx->sur_w, ctx->sur_m, ctx->sur_v,
                     ctx->sur_grad, ctx->sur_partial, ctx->sur_cfg, ctx->sur_stats, ctx->sur_list,
                     ctx->sur_cnt, ctx->sur_lval, ctx->sur_wgs, ctx->cord_alt, ctx->out_buf, ctx->a.mark_age,
-                    ctx->a.dtopo, ctx->a.dhacc, ctx->a.dhacc_corr, ctx->a.topo_tmp};
+                    ctx->a.dtopo, ctx->a.dhacc, ctx->a.dhacc_corr, ctx->a.topo_tmp, ctx->a.iphx};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -806,6 +811,11 @@ extern "C" int gf_upload(gf_ctx *ctx, uint32_t mask)
         if (!ctx->host.phase_ratio) return set_err(ctx, GF_ERR_ARG, "host pointer for phase_ratio is null");
         int rc = upload_phase_ratio(ctx);
         if (rc) return rc;
+    }
+    if (mask & (GF_IPHASE | GF_PHASE_RATIO)) {  // the one-phase flags
+        hipLaunchKernelGGL(k_iphx, dim3(nblk(ctx->ne)), dim3(256), 0, ctx->stream, ctx->dp, ctx->a);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return set_err(ctx, GF_ERR_HIP, "launch of k_iphx failed: %s", hipGetErrorString(e));
     }
     ctx->uploaded |= mask;
     return GF_OK;
