@@ -1643,6 +1643,15 @@ GF_DEV void move_rotate(T &s11, T &s22, T &s12, T w)
 // phs: the phase table (a workgroup's LDS copy of P->ph, or P->ph itself).  Every
 // per-element load is issued before the first use, so that a wave pays one memory
 // round trip for its inputs instead of a chain of them.
+// the LDS rows of rheol_single's two branch results (see there): one column per
+// thread of a workgroup of at most GF_RHEOL_WG threads
+#define GF_RHEOL_WG 256
+template <typename AT>
+GF_DEV AT (*rheol_branch_lds())[GF_RHEOL_WG]
+{
+    __shared__ AT rows[32][GF_RHEOL_WG];
+    return rows;
+}
 #define RH_ROT 1
 #define RH_AREA 2
 #define RH_NODVOL 4  // (with RH_AREA) dvol is formed and used but not stored: nothing reads it back
@@ -1849,6 +1858,11 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, const GfPhase *__res
             for (int c = 0; c < 4; c++) s_start[k][c] = st[k][c];
     AT depl[4] = {AT(0), AT(0), AT(0), AT(0)};
     bool ok = true, plastic_branch = true;
+    // irheol >= 11: both branches' stresses wait in LDS (a column per thread: plastic
+    // in rows 0..15, viscous in rows 16..31, component c of triangle k in row c + 4k)
+    // while the choice is made, so that neither is evaluated twice
+    AT(*const brs)[GF_RHEOL_WG] = rheol_branch_lds<AT>();
+    const int lt = threadIdx.x;
     if (irh >= 11) {
         AT s11p = 0, s12p = 0, s11v = 0, s12v = 0, s22p[4], s22v[4];
         for (int k = 0; k < 4; k++) {
@@ -1858,11 +1872,14 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, const GfPhase *__res
             s11p = k == 0 ? o[0] : s11p + o[0];
             s12p = k == 0 ? o[2] : s12p + o[2];
             s22p[k] = o[1];
+            for (int c = 0; c < 4; c++) brs[c + 4 * k][lt] = o[c];
             AT dm;
             ok &= rheol_tri<AT>(irh, false, pc, pd, bulkm, rmu, hardn, rp.rm, vis, dt, dtS, stherm, st[k], sr[k], dv, o, dm);
             s11v = k == 0 ? o[0] : s11v + o[0];
             s12v = k == 0 ? o[2] : s12v + o[2];
             s22v[k] = o[1];
+            for (int c = 0; c < 4; c++) brs[16 + c + 4 * k][lt] = o[c];
+            asm volatile("" ::: "memory");  // (the values are read back from LDS, not kept)
         }
         AT a = s11p - s22p[0] - s22p[1] - s22p[2] - s22p[3];
         AT sII_plas = a * a + AT(4.0) * (s12p * s12p);
@@ -1870,7 +1887,11 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, const GfPhase *__res
         AT sII_visc = av * av + AT(4.0) * (s12v * s12v);
         plastic_branch = sII_plas < sII_visc;
     }
-    if (irh == 1 || irh == 3 || irh == 6 || irh >= 11) {
+    if (irh >= 11) {
+        const int row = plastic_branch ? 0 : 16;
+        for (int k = 0; k < 4; k++)
+            for (int c = 0; c < 4; c++) st[k][c] = brs[row + c + 4 * k][lt];
+    } else if (irh == 1 || irh == 3 || irh == 6) {
         for (int k = 0; k < 4; k++) {
             AT o[4], dm = AT(0);
             const AT dv = AT(dvl[k] - 0.0);
