@@ -1562,7 +1562,7 @@ struct PlasticDbl {
 
 template <typename T>
 GF_DEV bool rheol_tri(int irh, bool plastic_branch, const PlasticConstT<T> &pc, const PlasticDbl &pd, T bulkm,
-                      T rmu, T hardn, double rmu0, double vis, double dt_d, T dt, T stherm, const T st_k[4],
+                      T rmu, T hardn, const MaxwellConst &mc, T dt, T stherm, const T st_k[4],
                       const T sr_k[3], T dv, T out[4], T &depl)
 {
     T de11 = sr_k[0] * dt, de22 = sr_k[1] * dt, de12 = sr_k[2] * dt;
@@ -1574,7 +1574,6 @@ GF_DEV bool rheol_tri(int irh, bool plastic_branch, const PlasticConstT<T> &pc, 
     if (irh == 1) {
         dev_elastic(bulkm, rmu, out[0], out[1], out[3], out[2], de11, de22, de12);
     } else if (irh == 3 || (irh >= 11 && !plastic_branch)) {
-        const MaxwellConst mc = maxwell_const(rmu0, vis, dt_d);
         ok = mc.ok;
         dev_maxwell(bulkm, T(mc.rmu), T(mc.vic1), T(mc.vic2), out[0], out[1], out[3], out[2], de11, de22, de12, dv);
     } else if (irh == 6 || irh >= 11) {
@@ -1862,19 +1861,21 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, const GfPhase *__res
     // in rows 0..15, viscous in rows 16..31, component c of triangle k in row c + 4k)
     // while the choice is made, so that neither is evaluated twice
     AT(*const brs)[GF_RHEOL_WG] = rheol_branch_lds<AT>();
+    // src/rh_maxwell.f90:16-30, the same for the element's four triangles
+    const MaxwellConst mc = maxwell_const(rp.rm, vis, dt);
     const int lt = threadIdx.x;
     if (irh >= 11) {
         AT s11p = 0, s12p = 0, s11v = 0, s12v = 0, s22p[4], s22v[4];
         for (int k = 0; k < 4; k++) {
             AT o[4];
             const AT dv = AT(dvl[k] - 0.0);
-            rheol_tri<AT>(irh, true, pc, pd, bulkm, rmu, hardn, rp.rm, vis, dt, dtS, stherm, st[k], sr[k], dv, o, depl[k]);
+            rheol_tri<AT>(irh, true, pc, pd, bulkm, rmu, hardn, mc, dtS, stherm, st[k], sr[k], dv, o, depl[k]);
             s11p = k == 0 ? o[0] : s11p + o[0];
             s12p = k == 0 ? o[2] : s12p + o[2];
             s22p[k] = o[1];
             for (int c = 0; c < 4; c++) brs[c + 4 * k][lt] = o[c];
             AT dm;
-            ok &= rheol_tri<AT>(irh, false, pc, pd, bulkm, rmu, hardn, rp.rm, vis, dt, dtS, stherm, st[k], sr[k], dv, o, dm);
+            ok &= rheol_tri<AT>(irh, false, pc, pd, bulkm, rmu, hardn, mc, dtS, stherm, st[k], sr[k], dv, o, dm);
             s11v = k == 0 ? o[0] : s11v + o[0];
             s12v = k == 0 ? o[2] : s12v + o[2];
             s22v[k] = o[1];
@@ -1895,8 +1896,8 @@ GF_DEV void rheol_single(const GfDevParams *__restrict__ P, const GfPhase *__res
         for (int k = 0; k < 4; k++) {
             AT o[4], dm = AT(0);
             const AT dv = AT(dvl[k] - 0.0);
-            bool okk = rheol_tri<AT>(irh, plastic_branch, pc, pd, bulkm, rmu, hardn, rp.rm, vis, dt, dtS, stherm, st[k],
-                                     sr[k], dv, o, dm);
+            bool okk = rheol_tri<AT>(irh, plastic_branch, pc, pd, bulkm, rmu, hardn, mc, dtS, stherm, st[k], sr[k], dv,
+                                     o, dm);
             if (irh == 6) depl[k] = dm;
             if (irh == 3) ok &= okk;
             for (int c = 0; c < 4; c++) st[k][c] = o[c];
