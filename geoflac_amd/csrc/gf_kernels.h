@@ -2059,11 +2059,18 @@ GF_DEV void visc_load(const GfDevParams *__restrict__ P, const GfDevArrays &A, l
                       bool rotate = false, const double (&dw)[4] = {0.0, 0.0, 0.0, 0.0})
 {
     const unsigned ne = (unsigned)NE, eo = (unsigned)e;
-    iph = A.iphase[e];
-    // phase ratios past nactive lie outside the buffer and read 0
+    const int iphx = A.iphx[e];
+    iph = iphx & IPHX_PHASE;
+    // phase ratios past nactive lie outside the buffer and read 0; an element of one
+    // phase (ratio exactly 1, the others 0) reads no ratio plane
     const Buf bPR = mkbuf(A.pr, (unsigned long long)P->nactive * ne * 8ull);
+    if (iphx & IPHX_PURE) {
 #pragma unroll
-    for (int a = 0; a < GF_MAXACT; a++) prv[a] = bld(bPR, eo * 8u, (unsigned)a * ne * 8u);
+        for (int a = 0; a < GF_MAXACT; a++) prv[a] = P->active[a] == iph ? 1.0 : 0.0;
+    } else {
+#pragma unroll
+        for (int a = 0; a < GF_MAXACT; a++) prv[a] = bld(bPR, eo * 8u, (unsigned)a * ne * 8u);
+    }
     const Buf bS = mkbuf(sv_ptr<SV>(A.stress0, A.stress0f), 16ull * sizeof(SV) * ne);
     SV sv[3][4];  // components 1, 2, 4 of triangles 1..4
     constexpr int comp[3] = {0, 1, 3};
@@ -2836,13 +2843,18 @@ GF_DEV double eff_dens_value(const GfDevParams *__restrict__ P, GfDevClock *ck, 
     typedef typename Prec<M>::AT AT;
     const int j = (int)((unsigned)e % (unsigned)(NZ - 1)) + 1, i = (int)((unsigned)e / (unsigned)(NZ - 1)) + 1;
     const double adi_grad = 3.0e-1, den_ramp_dist = 30.0e3, amp410 = 0.05, amp660 = 0.085;
-    const int iph = A.iphase[e];
+    const int iphx = A.iphx[e], iph = iphx & IPHX_PHASE;
     // every load issued before the first use
     const unsigned ne = (unsigned)NE, eo = (unsigned)e;
     const Buf bPR = mkbuf(A.pr, (unsigned long long)P->nactive * ne * 8ull);
     double prv[GF_MAXACT];
+    if (iphx & IPHX_PURE) {  // one phase: no ratio plane is read
 #pragma unroll
-    for (int a = 0; a < GF_MAXACT; a++) prv[a] = bld(bPR, eo * 8u, (unsigned)a * ne * 8u);
+        for (int a = 0; a < GF_MAXACT; a++) prv[a] = P->active[a] == iph ? 1.0 : 0.0;
+    } else {
+#pragma unroll
+        for (int a = 0; a < GF_MAXACT; a++) prv[a] = bld(bPR, eo * 8u, (unsigned)a * ne * 8u);
+    }
     const Buf bS = mkbuf(sv_ptr<SV>(A.stress0, A.stress0f), 16ull * sizeof(SV) * ne);
     SV sv[4][3];  // components 1, 2, 4 of triangles 1..4
 #pragma unroll
