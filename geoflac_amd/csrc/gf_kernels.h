@@ -354,8 +354,9 @@ GF_DEV double therm_eval_full(const ThermCoef &c, double xu, double xl, double x
     return c.told + div_fma(rhs * dt, c.area_n, c.rarea);
 }
 
-// pass == 0: old temperature from A.temp (also copied to temp0 = src/fl_therm.f90:52);
-// pass > 0: old temperature from A.temp0 and the starting iterate from xin.
+// The old temperature comes from A.temp in every pass (nothing writes temp before the
+// boundary conditions at the end of the call); pass 0 also copies it to temp0
+// (src/fl_therm.f90:52); passes > 0 start from the previous pass's iterate xin.
 // A tile's global reads (its node footprint: temperature and coordinates; its 65 x 17
 // elements: areas, phase, source) are issued together into LDS, so that each tile pays
 // the memory latency once; everything after that reads LDS.
@@ -383,7 +384,7 @@ GF_DEV void therm_tile_one(const GfDevParams *__restrict__ P, GfDevClock *ck, co
     const int tj = threadIdx.x % TH_J, ti = threadIdx.x / TH_J;
     const int j0 = (tile % ntj) * TH_OJ + 1 - TH_HJ;  // 1-based row of tile row 0
     const int i0 = (tile / ntj) * TH_OI + 1 - TH_HI;  // 1-based column of tile column 0
-    const double *told_g = pass == 0 ? A.temp : A.temp0;
+    const double *told_g = A.temp;
 #define LIDX(lj, li) (((li) + 1) * TH_LJ + ((lj) + 1))
     // Each thread has two node slots (q and q + 1024 of the 66 x 18 footprint) and two
     // element slots (e and e + 1024 of the 65 x 17 table).  All loads are issued before
@@ -652,9 +653,10 @@ GF_DEV void therm_tile_origin(const GfDevParams *P, int tile, int &j0, int &i0)
 // XIN: the starting iterate -- 0: the old temperature (pass 0), 1: loaded from the
 // previous pass's iterate (passes > 0), 2: extrapolated in time from the old
 // temperature and temp0, the temperature at the start of the previous step (pass 0;
-// temp0 of this tile's neighbours may already hold this step's value, and then the
-// extrapolation is the old temperature: only the number of sweeps changes, the fixed
-// point is the same).  DIS: the shear heating is loaded
+// where pass 0 copies the old temperature to temp0, temp0 of this tile's neighbours
+// may already hold this step's value, and then the extrapolation is the old
+// temperature: only the number of sweeps changes, the fixed point is the same).
+// DIS: the shear heating is loaded
 template <int XIN, bool DIS>
 GF_DEV ThermPf therm_prefetch(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, int pass,
                               int tile, int arp = 0, int gate = 1)
@@ -662,7 +664,7 @@ GF_DEV ThermPf therm_prefetch(const GfDevParams *__restrict__ P, GfDevClock *ck,
     int j0, i0;
     therm_tile_origin(P, tile, j0, i0);
     const long nz = P->nz, nx = P->nx, nn = nz * nx, ne = (nz - 1) * (nx - 1);
-    const double *told_g = pass == 0 ? A.temp : A.temp0;
+    const double *told_g = A.temp;
     const double *xin = (pass % 2 == 1) ? A.xa : A.xb;
     ThermPf f;
     // the model's scalars first, used at once: waiting for them later would also wait
@@ -741,11 +743,14 @@ GF_DEV void therm_store_raw(const ThermPf &f, ThermRaw &R)
 // and leaves it in R.  sU: element table by rounds, then the two iterates.  Nothing
 // after the prefetch is issued reads global memory (a wait for it would also wait
 // for the prefetch): the mesh size is held in registers, the phase table, dt and the
-// activity flag came with the tile's inputs.
+// activity flag came with the tile's inputs.  wt0: pass 0 copies the old temperature
+// to temp0 (0 where the caller turns temp0 over by itself: the rotating buffers of
+// the single-model step).
 template <int XIN, bool DIS>
 GF_DEV void therm_tile_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, int pass,
                             int tile, const GfDevParams *__restrict__ Pn, GfDevClock *ckn, const GfDevArrays &An,
-                            int next, ThermRaw &R, double *sU, int *sFlag, double stol, int arp = 0, int gate = 1)
+                            int next, ThermRaw &R, double *sU, int *sFlag, double stol, int arp = 0, int gate = 1,
+                            int wt0 = 1)
 {
     double *__restrict__ xout = (pass % 2 == 0) ? A.xa : A.xb;
     double *__restrict__ bord = A.tbord;
@@ -947,7 +952,7 @@ GF_DEV void therm_tile_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, c
     if (next >= 0) therm_store_raw<XIN, DIS>(pf, R);
     if (is_out) {
         xout[oix] = sX[cur][me];
-        if (pass == 0) A.temp0[oix] = c.told;
+        if (pass == 0 && wt0) A.temp0[oix] = c.told;
     }
     if (threadIdx.x < TH_NB) {
         int lj, li;
@@ -964,7 +969,7 @@ GF_DEV void therm_tile_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, c
 template <int XIN, bool DIS>
 // arp: the element areas of the last move are pending (formed from the coordinates)
 __global__ void __launch_bounds__(1024) k_therm_pipe(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
-                                                     int pass, double stol, int arp)
+                                                     int pass, double stol, int arp, int wt0)
 {
     if (pass > 0 && ck->therm_pass_changed[pass - 1] == 0u) return;  // already proven
     __shared__ ThermRaw raw;
@@ -977,7 +982,8 @@ __global__ void __launch_bounds__(1024) k_therm_pipe(const GfDevParams *__restri
     therm_store_raw<XIN, DIS>(therm_prefetch<XIN, DIS>(P, ck, A, pass, first, arp), raw);
     for (int t = first; t < end; t += RW) {
         lds_barrier();  // this tile's inputs are in; the previous tile is done with sU
-        therm_tile_pipe<XIN, DIS>(P, ck, A, pass, t, P, ck, A, t + RW < end ? t + RW : -1, raw, sU, sFlag, stol, arp);
+        therm_tile_pipe<XIN, DIS>(P, ck, A, pass, t, P, ck, A, t + RW < end ? t + RW : -1, raw, sU, sFlag, stol, arp,
+                                  1, wt0);
     }
 }
 
@@ -1009,7 +1015,7 @@ GF_DEV void therm_check_body(const GfDevParams *__restrict__ P, GfDevClock *ck, 
     // output nodes of this tile that read (j,i): as up (j+1,i), left (j,i+1), left-down (j-1,i+1)
     const int dj[3] = {1, 0, -1}, di[3] = {0, 1, 1};
     const double dt = ck->dt;
-    auto T = [&](int jj, int ii) { return A.temp0[IN(jj, ii)]; };
+    auto T = [&](int jj, int ii) { return A.temp[IN(jj, ii)]; };  // the old temperature
     for (int q = 0; q < 3; q++) {
         const int mj = j + dj[q], mi = i + di[q];
         const int mlj = mj - j0, mli = mi - i0;
@@ -1041,20 +1047,11 @@ __global__ void __launch_bounds__(256) k_therm_check(const GfDevParams *__restri
 // pass is used, but the result is not known to be the serial one); without proof the
 // single tile pass is taken as it is (FP32-arithmetic mode)
 // proven: the first pass whose border check found no difference (-1: none)
-GF_DEV void therm_bc_apply(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long t,
-                           int passes, int proof, int proven)
+// the value of node (j, i) after the conditions (:575-598), from the iterate x: the
+// top and bottom conditions (:575-588), then the side columns copy their neighbours
+// (only boundary nodes change: an interior node keeps x)
+GF_DEV double therm_bc_value(const GfDevParams *__restrict__ P, const GfDevArrays &A, const double *x, int j, int i)
 {
-    const double *__restrict__ xa = A.xa, *__restrict__ xb = A.xb;
-    // pass p writes xa when p is even, xb when odd
-    if (!proof) proven = 0;
-    const double *x = (proven < 0 ? (passes - 1) : proven) % 2 == 0 ? xa : xb;
-    if (t == 0 && (proven < 0 || (!proof && ck->therm_pass_changed[0] != 0u)))
-        atomicAdd(&ck->therm_unconverged, 1ull);
-    if (t == 0 && proof && proven < 0) raise_error(ck, GF_ERR_THERM_SWEEPS, 0);
-    if (t == 0 && proven > 0) atomicAdd(&ck->therm_extra_passes, (unsigned long long)proven);
-    if (t >= NN) return;
-    const int j = (int)((unsigned)t % (unsigned)NZ) + 1, i = (int)((unsigned)t / (unsigned)NZ) + 1;
-    // value of node (jj, ii) after the top/bottom conditions (:575-588)
     auto tb = [&](int jj, int ii) -> double {
         if (jj == 1) return P->t_top;
         if (jj == NZ) {
@@ -1067,14 +1064,62 @@ GF_DEV void therm_bc_apply(const GfDevParams *__restrict__ P, GfDevClock *ck, co
         }
         return x[IN(jj, ii)];
     };
-    double v;
-    if (i == 1)
-        v = tb(j, 2);
-    else if (i == NX)
-        v = tb(j, NX - 1);
-    else
-        v = tb(j, i);
-    A.temp[t] = v;
+    if (i == 1) return tb(j, 2);
+    if (i == NX) return tb(j, NX - 1);
+    return tb(j, i);
+}
+// the counters of the call's outcome; the iterate that is the result (pass p wrote xa
+// when p is even, xb when odd)
+GF_DEV const double *therm_bc_outcome(GfDevClock *ck, const GfDevArrays &A, bool first, int passes, int proof,
+                                      int &proven)
+{
+    if (!proof) proven = 0;
+    if (first && (proven < 0 || (!proof && ck->therm_pass_changed[0] != 0u)))
+        atomicAdd(&ck->therm_unconverged, 1ull);
+    if (first && proof && proven < 0) raise_error(ck, GF_ERR_THERM_SWEEPS, 0);
+    if (first && proven > 0) atomicAdd(&ck->therm_extra_passes, (unsigned long long)proven);
+    return (proven < 0 ? (passes - 1) : proven) % 2 == 0 ? A.xa : A.xb;
+}
+GF_DEV void therm_bc_apply(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long t,
+                           int passes, int proof, int proven)
+{
+    const double *__restrict__ x = therm_bc_outcome(ck, A, t == 0, passes, proof, proven);
+    if (t >= NN) return;
+    const int j = (int)((unsigned)t % (unsigned)NZ) + 1, i = (int)((unsigned)t / (unsigned)NZ) + 1;
+    A.temp[t] = therm_bc_value(P, A, x, j, i);
+}
+// The same conditions where the result becomes the new temperature in place (xa, which
+// the caller then takes as temp): after an even pass the result is in xa already and
+// only its boundary nodes (rows 1 and nz, columns 1 and nx) change, each computed from
+// interior nodes, or from boundary nodes that keep their value; after an odd pass every
+// node is formed from xb into xa.  Thread tid of nthr.  (nz, nx >= 3.)
+GF_DEV void therm_bc_rot(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long tid,
+                         long nthr, int passes, int proof, int proven)
+{
+    const double *x = therm_bc_outcome(ck, A, tid == 0, passes, proof, proven);
+    double *out = A.xa;
+    const int nz = NZ, nx = NX;
+    if (x == out) {
+        const long nb = 2L * nx + 2L * (nz - 2);
+        for (long t = tid; t < nb; t += nthr) {
+            int j, i;
+            if (t < nx) {
+                j = 1; i = (int)t + 1;
+            } else if (t < 2L * nx) {
+                j = nz; i = (int)(t - nx) + 1;
+            } else if (t < 2L * nx + nz - 2) {
+                j = (int)(t - 2L * nx) + 2; i = 1;
+            } else {
+                j = (int)(t - 2L * nx - (nz - 2)) + 2; i = nx;
+            }
+            out[IN(j, i)] = therm_bc_value(P, A, x, j, i);
+        }
+    } else {
+        for (long t = tid; t < NN; t += nthr) {
+            const int j = (int)((unsigned)t % (unsigned)nz) + 1, i = (int)((unsigned)t / (unsigned)nz) + 1;
+            out[t] = therm_bc_value(P, A, x, j, i);
+        }
+    }
 }
 
 GF_DEV void therm_bc_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long t,
@@ -1142,10 +1187,11 @@ GF_DEV bool grid_sync(GfDevClock *ck)
 // boundary conditions.  The same arithmetic as the launch-per-phase sequence
 // (k_therm_pipe, k_therm_check, k_therm_bc).  The flags that decide another pass are
 // read with coherent loads after a barrier, so every workgroup takes the same
-// decision; flag 0 was written by earlier launches.
+// decision; flag 0 was written by earlier launches.  rot: the result becomes the new
+// temperature in xa (therm_bc_rot) instead of being copied to temp.
 template <bool DIS>
 __global__ void __launch_bounds__(1024) k_therm_tail(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
-                                                     double stol, int arp, int npass, int proof, long nchk)
+                                                     double stol, int arp, int npass, int proof, long nchk, int rot)
 {
     const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x, nthr = (long)gridDim.x * blockDim.x;
     int proven = -1;
@@ -1175,6 +1221,10 @@ __global__ void __launch_bounds__(1024) k_therm_tail(const GfDevParams *__restri
                 break;
             }
         }
+    }
+    if (rot) {  // the result becomes temp in place (therm_bc_rot)
+        therm_bc_rot(P, ck, A, tid, nthr, npass, proof, proven);
+        return;
     }
     const long nn = (long)P->nz * P->nx;
     for (long t = tid; t < nn; t += nthr) therm_bc_apply(P, ck, A, t, npass, proof, proven);

@@ -1087,6 +1087,13 @@ static int enq_therm(gf_ctx *ctx, bool clock_done = false)
     if (!clock_done && (rc = enq_clock(ctx, CK_THERM_BEGIN))) return rc;
     const int ntiles = ((ctx->p.nz + TH_OJ - 1) / TH_OJ) * ((ctx->p.nx + TH_OI - 1) / TH_OI);
     const int npass = ctx->therm_passes;
+    // The single-model call turns three buffers over instead of copying: the old
+    // temperature (temp) is read, temp0 holds the previous call's starting temperature
+    // (the extrapolated start), and the result is formed in xa; then xa becomes temp,
+    // temp becomes temp0 (src/fl_therm.f90:52) and temp0 the next call's xa.  Not while
+    // a batch holds the context's array pointers (in_batch), and not in the
+    // launch-per-phase sequence.
+    const bool rot = ctx->therm_tail && ctx->in_batch == 0 && ctx->p.nz >= 3 && ctx->p.nx >= 3;
     for (int pass = 0; pass < (ctx->therm_tail ? 1 : npass); pass++) {
         const dim3 grid(pass == 0 || ctx->therm_fb_full ? ctx->therm_grid : 32u);
         hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1107,7 +1114,7 @@ static int enq_therm(gf_ctx *ctx, bool clock_done = false)
                     : ex     ? (dis ? k_therm_pipe<2, true> : k_therm_pipe<2, false>)
                              : (dis ? k_therm_pipe<0, true> : k_therm_pipe<0, false>);
         hipLaunchKernelGGL(kern, grid, dim3(TH_J * TH_I), 0, ctx->stream, ctx->dp, ctx->dck, ctx->a, pass,
-                           ctx->therm_proof ? 0.0 : GF_THERM_STOL, ctx->area_pending ? 1 : 0);
+                           ctx->therm_proof ? 0.0 : GF_THERM_STOL, ctx->area_pending ? 1 : 0, rot ? 0 : 1);
         if ((rc = launch_check(ctx, "k_therm_pipe"))) return rc;
         if (ctx->timing && e0 && e1) {
             (void)hipEventRecord(e1, ctx->stream);
@@ -1122,9 +1129,9 @@ static int enq_therm(gf_ctx *ctx, bool clock_done = false)
     if (ctx->therm_tail) {
         const bool dis = ctx->p.ishearh == 1 && ctx->p.itherm != 2;
         double stol = ctx->therm_proof ? 0.0 : GF_THERM_STOL;
-        int arp = ctx->area_pending ? 1 : 0, np = npass, proof = ctx->therm_proof ? 1 : 0;
+        int arp = ctx->area_pending ? 1 : 0, np = npass, proof = ctx->therm_proof ? 1 : 0, irot = rot ? 1 : 0;
         long nchk = (long)ntiles * TH_NB;
-        void *args[] = {(void *)&ctx->dp, (void *)&ctx->dck, (void *)&ctx->a, &stol, &arp, &np, &proof, &nchk};
+        void *args[] = {(void *)&ctx->dp, (void *)&ctx->dck, (void *)&ctx->a, &stol, &arp, &np, &proof, &nchk, &irot};
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (ctx->timing) {
             e0 = next_event(ctx);
@@ -1147,7 +1154,13 @@ static int enq_therm(gf_ctx *ctx, bool clock_done = false)
     } else {
         LAUNCH(k_therm_bc, ctx->nn, ctx->dp, ctx->dck, ctx->a, npass, ctx->therm_proof ? 1 : 0);
     }
-    ctx->temp0_prev = true;  // pass 0 wrote temp0 at every node
+    if (rot) {
+        double *t = ctx->a.temp;
+        ctx->a.temp = ctx->a.xa;
+        ctx->a.xa = ctx->a.temp0;
+        ctx->a.temp0 = t;
+    }
+    ctx->temp0_prev = true;  // temp0 holds this call's starting temperature at every node
     return GF_OK;
 }
 

@@ -130,10 +130,11 @@ def test_thermal_fallback_passes_bitwise_equal_to_oracle(therm, monkeypatch):
     """Cells 1.7 times taller than wide (200x40 elements over the 300x100 km box)
     couple neighbouring columns strongly, so that the thermal tiles' halos do not
     settle every output node and fallback passes run (the pipelined tile kernel from
-    the assembled iterate), either inside the cooperative tail launch with grid
-    barriers between the phases (the default) or as a launch per phase
-    (GF_THERM=multi).  The run stays bitwise equal to the correctly rounded oracle
-    started from the same state."""
+    the assembled iterate), either inside the tail launch with grid barriers between
+    the phases (the default, whose result becomes the new temperature in place: after
+    an odd pass it is formed from the second iterate buffer) or as a launch per phase
+    (GF_THERM=multi, copied to temp).  The run stays bitwise equal to the correctly
+    rounded oracle started from the same state."""
     import os
     from tests import oracle_lib as ol
     if therm == "multi":
