@@ -69,12 +69,13 @@ def alg_bytes(kernel, E, N, nact, p, area_frac=0.0):
         return E * (96 + 8 + 8 * nact + 4 + 8) + N * 8                 # stress0(1,2,4) e2sr pr iphase, visn out | temp
     if kernel == "k_dens":
         return E * (96 + 8 * nact + 4 + 8) + N * 8                     # stress0(1,2,4) pr iphase, dens out | temp
+    # (the step's masses and dt minima form the move's pending areas from the coordinates)
     if kernel == "k_rmass":
-        return E * (8 + 32) + N * 8                                    # dens area | rmass out
+        return E * 8 + N * (8 + 16)                                    # dens | rmass out, cord
     if kernel == "k_amass":
-        return E * (32 + 4) + N * 8                                    # area iphase | amass out
+        return E * 4 + N * (8 + 16)                                    # iphase | amass out, cord
     if kernel == "k_dt_reduce":
-        return E * (32 + 8 + 4) + N * 32                               # area visn iphase | cord vel
+        return E * (8 + 4) + N * 32                                    # visn iphase | cord vel
     if kernel == "k_move_cord":
         return N * 48
     if kernel in ("k_therm_bc", "k_therm_tail"):

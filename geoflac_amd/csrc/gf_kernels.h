@@ -3012,21 +3012,72 @@ GF_DEV CornerSlots corner_slots(const GfDevParams *P, int j, int i)
     }
     return cs;
 }
+// The areas node (j, i) reads (triangle kp[sl][q] of the element in slot sl; 0 outside
+// the mesh).  arp: the last move's areas are pending: they are formed from the moved
+// coordinates of the 3 x 3 nodes around (j, i), with move_area_body's arithmetic,
+// instead of being read.
+GF_DEV void corner_areas(const GfDevParams *__restrict__ P, const GfDevArrays &A, int j, int i,
+                         const CornerSlots &cs, int arp, double (&ar)[4][3])
+{
+    const unsigned ne = (unsigned)NE, nn = (unsigned)NN, nz = (unsigned)NZ;
+    if (!arp) {
+        const Buf bA = mkbuf(A.area, 32ull * ne);
+#pragma unroll
+        for (int sl = 0; sl < 4; sl++)
+#pragma unroll
+            for (int q = 0; q < 3; q++) ar[sl][q] = bld(bA, cs.e[sl] * 8u + cs.kp[sl][q] * ne * 8u, 0);
+        return;
+    }
+    constexpr unsigned OOB = 0x80000000u;  // an offset past the array (reads 0)
+    const Buf bC = mkbuf(A.cord, 16ull * nn);
+    double cx[3][3], cy[3][3];  // node (j + dj, i + di) at [dj + 1][di + 1]
+#pragma unroll
+    for (int dj = -1; dj <= 1; dj++)
+#pragma unroll
+        for (int di = -1; di <= 1; di++) {
+            const int jj = j + dj, ii = i + di;
+            const bool in = jj >= 1 && jj <= NZ && ii >= 1 && ii <= NX;
+            const unsigned o = in ? ((unsigned)(jj - 1) + nz * (unsigned)(ii - 1)) * 8u : OOB;
+            cx[dj + 1][di + 1] = bld(bC, o, 0);
+            cy[dj + 1][di + 1] = bld(bC, o, nn * 8u);
+        }
+    // the areas of the three triangles through the node of the element at corner
+    // position c (je = j - (c & 1), ie = i - (c >> 1)), with constant indices, then
+    // each slot's by value (slot sl holds position sl ^ c0)
+    constexpr int tri[4][3] = {{0, 2, 3}, {0, 1, 2}, {0, 1, 3}, {1, 2, 3}};  // corner_slots' TRI
+    double v[4][3];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int a = 1 - (c & 1), b = 1 - (c >> 1);  // the element's corner 1 in the block
+        const double xc[5] = {0.0, cx[a][b], cx[a + 1][b], cx[a][b + 1], cx[a + 1][b + 1]};
+        const double yc[5] = {0.0, cy[a][b], cy[a + 1][b], cy[a][b + 1], cy[a + 1][b + 1]};
+#pragma unroll
+        for (int q = 0; q < 3; q++) v[c][q] = 1.0 / move_det(xc, yc, tri[c][q]);
+    }
+    const int c0 = 2 * ((i - 1) & 1) + ((j - 1) & 1);
+#pragma unroll
+    for (int sl = 0; sl < 4; sl++) {
+        const int c = sl ^ c0;
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const double w = c == 0 ? v[0][q] : c == 1 ? v[1][q] : c == 2 ? v[2][q] : v[3][q];
+            ar[sl][q] = cs.ok[sl] ? w : 0.0;
+        }
+    }
+}
 
-GF_DEV void rmass_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long t)
+// arp: the last move's areas are pending (corner_areas)
+GF_DEV void rmass_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long t, int arp = 0)
 {
     const int j = (int)((unsigned)t % (unsigned)NZ) + 1, i = (int)((unsigned)t / (unsigned)NZ) + 1;
     const double c1d12 = 1.0 / 12.0;
     const CornerSlots cs = corner_slots(P, j, i);
     const unsigned ne = (unsigned)NE;
-    const Buf bA = mkbuf(A.area, 32ull * ne), bD = mkbuf(A.dens, 8ull * ne);
+    const Buf bD = mkbuf(A.dens, 8ull * ne);
     double ar[4][3], dn[4];
 #pragma unroll
-    for (int sl = 0; sl < 4; sl++) {
-        dn[sl] = bld(bD, cs.e[sl] * 8u, 0);
-#pragma unroll
-        for (int q = 0; q < 3; q++) ar[sl][q] = bld(bA, cs.e[sl] * 8u + cs.kp[sl][q] * ne * 8u, 0);
-    }
+    for (int sl = 0; sl < 4; sl++) dn[sl] = bld(bD, cs.e[sl] * 8u, 0);
+    corner_areas(P, A, j, i, cs, arp, ar);
     double sum = 0.0;
 #pragma unroll
     for (int sl = 0; sl < 4; sl++)
@@ -3035,11 +3086,12 @@ GF_DEV void rmass_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const 
             for (int q = 0; q < 3; q++) sum = sum + c1d12 / ar[sl][q] * dn[sl];
     A.rmass[t] = sum;
 }
-__global__ void __launch_bounds__(256) k_rmass(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+__global__ void __launch_bounds__(256) k_rmass(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
+                                               int arp)
 {
     long t = xcd_lin();
     if (t >= NN) return;
-    rmass_body(P, ck, A, t);
+    rmass_body(P, ck, A, t, arp);
 }
 
 // ------------------------------------------------------------------ dt_mass
@@ -3052,28 +3104,40 @@ GF_DEV double side_len(const GfDevParams *P, const GfDevArrays &A, int ja, int i
 // dlmin_prop (src/dt_mass.f90:168-232) and vel_max (:55-62): grid-stride loops, a
 // wave then block reduction, one atomic per block; min/max of positive doubles are
 // exact in any order, and their bit patterns order like the values.
+// ARP: the last move's areas are pending: formed from the moved coordinates
+// (move_area_body's arithmetic) instead of being read
+template <bool ARP = false>
 GF_DEV double elem_dlmin(const GfDevParams *P, const GfDevArrays &A, long t)
 {
     const int j = (int)((unsigned)t % (unsigned)(NZ - 1)) + 1, i = (int)((unsigned)t / (unsigned)(NZ - 1)) + 1;
+    double ar[4];
+    if (ARP) {
+        double xc[5], yc[5];
+        move_load_corners<double>(P, A, j, i, xc, yc, nullptr, nullptr);
+#pragma unroll
+        for (int k = 0; k < 4; k++) ar[k] = 1.0 / move_det(xc, yc, k);
+    }
+#define DLAREA(j, i, k) (ARP ? ar[(k) - 1] : A.area[IE(j, i) + NE * ((k) - 1)])
     double dlmin = 1.0e28, dl;
     dl = side_len(P, A, j, i, j + 1, i);
-    dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 1) * dl));
-    dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 3) * dl));
+    dlmin = fmin(dlmin, 1.0 / (DLAREA(j, i, 1) * dl));
+    dlmin = fmin(dlmin, 1.0 / (DLAREA(j, i, 3) * dl));
     dl = side_len(P, A, j + 1, i, j + 1, i + 1);
-    dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 2) * dl));
-    dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 3) * dl));
+    dlmin = fmin(dlmin, 1.0 / (DLAREA(j, i, 2) * dl));
+    dlmin = fmin(dlmin, 1.0 / (DLAREA(j, i, 3) * dl));
     dl = side_len(P, A, j, i + 1, j + 1, i + 1);
-    dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 2) * dl));
-    dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 4) * dl));
+    dlmin = fmin(dlmin, 1.0 / (DLAREA(j, i, 2) * dl));
+    dlmin = fmin(dlmin, 1.0 / (DLAREA(j, i, 4) * dl));
     dl = side_len(P, A, j, i, j, i + 1);
-    dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 1) * dl));
-    dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 4) * dl));
+    dlmin = fmin(dlmin, 1.0 / (DLAREA(j, i, 1) * dl));
+    dlmin = fmin(dlmin, 1.0 / (DLAREA(j, i, 4) * dl));
     dl = side_len(P, A, j, i, j + 1, i + 1);
-    dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 3) * dl));
-    dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 4) * dl));
+    dlmin = fmin(dlmin, 1.0 / (DLAREA(j, i, 3) * dl));
+    dlmin = fmin(dlmin, 1.0 / (DLAREA(j, i, 4) * dl));
     dl = side_len(P, A, j, i + 1, j + 1, i);
-    dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 1) * dl));
-    dlmin = fmin(dlmin, 1.0 / (AREA(j, i, 2) * dl));
+    dlmin = fmin(dlmin, 1.0 / (DLAREA(j, i, 1) * dl));
+    dlmin = fmin(dlmin, 1.0 / (DLAREA(j, i, 2) * dl));
+#undef DLAREA
     return dlmin;
 }
 
@@ -3081,7 +3145,7 @@ GF_DEV double elem_dlmin(const GfDevParams *P, const GfDevArrays &A, long t)
 // block `blk` of `nblk` workgroups of 256 threads working on one model
 template <typename SV>
 GF_DEV void dt_reduce_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, int blk,
-                           int nblk)
+                           int nblk, int arp = 0)
 {
     const SV *vel = sv_ptr<SV>(A.vel, A.velf);
     __shared__ double s_min[4], s_max[4];
@@ -3093,7 +3157,10 @@ GF_DEV void dt_reduce_body(const GfDevParams *__restrict__ P, GfDevClock *ck, co
     for (long t = (long)blk * blockDim.x + threadIdx.x; t < ne; t += 4 * stride) {
         double d[4];
 #pragma unroll
-        for (int u = 0; u < 4; u++) d[u] = t + u * stride < ne ? elem_dlmin(P, A, t + u * stride) : 1.0e28;
+        for (int u = 0; u < 4; u++)
+            d[u] = t + u * stride >= ne ? 1.0e28
+                   : arp            ? elem_dlmin<true>(P, A, t + u * stride)
+                                    : elem_dlmin<false>(P, A, t + u * stride);
 #pragma unroll
         for (int u = 0; u < 4; u++) dlmin = fmin(dlmin, d[u]);
     }
@@ -3125,9 +3192,10 @@ GF_DEV void dt_reduce_body(const GfDevParams *__restrict__ P, GfDevClock *ck, co
 }
 
 template <typename SV>
-__global__ void __launch_bounds__(256) k_dt_reduce(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+__global__ void __launch_bounds__(256) k_dt_reduce(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
+                                                   int arp)
 {
-    dt_reduce_body<SV>(P, ck, A, blockIdx.x, gridDim.x);
+    dt_reduce_body<SV>(P, ck, A, blockIdx.x, gridDim.x, arp);
 }
 
 // Scalars of src/dt_mass.f90:31-51,122-158.  dtmax_therm and dt_maxwell depend on
@@ -3166,7 +3234,7 @@ __global__ void k_dt_scalars(const GfDevParams *__restrict__ P, GfDevClock *ck)
 }
 
 // inertial masses, src/dt_mass.f90:64-120 (gather form of the coloured scatter)
-GF_DEV void amass_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long t)
+GF_DEV void amass_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long t, int arp = 0)
 {
     if (P->idt_scale == 0) {
         A.amass[t] = A.rmass[t];
@@ -3179,15 +3247,12 @@ GF_DEV void amass_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const 
     const double dt_elastic = ck->dt_elastic;
     const CornerSlots cs = corner_slots(P, j, i);
     const unsigned ne = (unsigned)NE;
-    const Buf bA = mkbuf(A.area, 32ull * ne), bI = mkbuf(A.iphase, 4ull * ne);
+    const Buf bI = mkbuf(A.iphase, 4ull * ne);
     double ar[4][3], pw[4];
     int ip[4];
 #pragma unroll
-    for (int sl = 0; sl < 4; sl++) {
-        ip[sl] = bldi(bI, cs.e[sl] * 4u, 0);
-#pragma unroll
-        for (int q = 0; q < 3; q++) ar[sl][q] = bld(bA, cs.e[sl] * 8u + cs.kp[sl][q] * ne * 8u, 0);
-    }
+    for (int sl = 0; sl < 4; sl++) ip[sl] = bldi(bI, cs.e[sl] * 4u, 0);
+    corner_areas(P, A, j, i, cs, arp, ar);
 #pragma unroll
     for (int sl = 0; sl < 4; sl++) pw[sl] = cs.ok[sl] ? PHASE(ip[sl]).pwave : 0.0;
     double sum = 0.0;
@@ -3205,9 +3270,10 @@ GF_DEV void amass_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const 
     }
     A.amass[t] = sum;
 }
-__global__ void __launch_bounds__(256) k_amass(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+__global__ void __launch_bounds__(256) k_amass(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
+                                               int arp)
 {
     long t = xcd_lin();
     if (t >= NN) return;
-    amass_body(P, ck, A, t);
+    amass_body(P, ck, A, t, arp);
 }

@@ -1276,11 +1276,9 @@ static int enq_move(gf_ctx *ctx)
         LAUNCH_M(k_move_elem, ctx->ne, ctx->dp, ctx->dck, ctx->a);
     } else if (fused) {
         // the areas and dvol wait for the next fl_rheol (or the first kernel that
-        // needs them) unless this step's rmasses or dt_mass read them next; the fused
-        // fl_node recorded the move's dt
-        const bool masses = ctx->nloop % ctx->p.ifreq_rmasses == 0 || ctx->nloop % ctx->p.ifreq_imasses == 0;
-        if (masses) LAUNCH(k_move_area, ctx->ne, ctx->dp, ctx->dck, ctx->a, 0);
-        ctx->area_pending = !masses;
+        // needs them stored; rmasses and dt_mass form the areas they read themselves);
+        // the fused fl_node recorded the move's dt
+        ctx->area_pending = true;
         ctx->rot_pending = true;
     } else {
         LAUNCH(k_move_area, ctx->ne, ctx->dp, ctx->dck, ctx->a, 1);
@@ -1291,8 +1289,8 @@ static int enq_move(gf_ctx *ctx)
 
 static int enq_rmasses(gf_ctx *ctx)
 {
-    int rc;
-    if ((rc = flush_area(ctx))) return rc;  // the masses read the areas
+    // the masses form pending areas from the moved coordinates (corner_areas)
+    const int arp = ctx->area_pending ? 1 : 0;
     // Eff_dens rotates the stress it reads by a pending move itself (the move stays
     // pending for the next fl_rheol)
     const int rot = ctx->rot_pending ? 1 : 0;
@@ -1302,19 +1300,20 @@ static int enq_rmasses(gf_ctx *ctx)
         LAUNCH_NAMED("k_dens<float>", k_dens<2>, ctx->ne, ctx->dp, ctx->dck, ctx->a, rot);
     else
         LAUNCH_NAMED("k_dens<double>", k_dens<0>, ctx->ne, ctx->dp, ctx->dck, ctx->a, rot);
-    LAUNCH(k_rmass, ctx->nn, ctx->dp, ctx->dck, ctx->a);
+    LAUNCH(k_rmass, ctx->nn, ctx->dp, ctx->dck, ctx->a, arp);
     return GF_OK;
 }
 
 static int enq_dt_mass(gf_ctx *ctx)
 {
     int rc;
-    if ((rc = flush_area(ctx))) return rc;  // the dt minima and inertial masses read the areas
+    // the dt minima and inertial masses form pending areas from the moved coordinates
+    const int arp = ctx->area_pending ? 1 : 0;
     if ((rc = enq_clock(ctx, CK_DT_BEGIN))) return rc;
-    LAUNCH_SV(k_dt_reduce, (long)DT_RED_BLOCKS * 256, ctx->dp, ctx->dck, ctx->a);
+    LAUNCH_SV(k_dt_reduce, (long)DT_RED_BLOCKS * 256, ctx->dp, ctx->dck, ctx->a, arp);
     hipLaunchKernelGGL(k_dt_scalars, dim3(1), dim3(64), 0, ctx->stream, ctx->dp, ctx->dck);
     if ((rc = launch_check(ctx, "k_dt_scalars"))) return rc;
-    LAUNCH(k_amass, ctx->nn, ctx->dp, ctx->dck, ctx->a);
+    LAUNCH(k_amass, ctx->nn, ctx->dp, ctx->dck, ctx->a, arp);
     return GF_OK;
 }
 
