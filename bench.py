@@ -41,12 +41,14 @@ def area_pending_fraction(p, n0, nsteps):
     return cnt / max(nsteps, 1)
 
 
-def alg_bytes(kernel, E, N, nact, p, area_frac=0.0):
+def alg_bytes(kernel, E, N, nact, p, area_frac=0.0, mixfrac=1.0):
     """Algorithmic (compulsory) HBM bytes of one launch: each array the kernel must
-    read once and write once, FP64 = 8 B, int32 = 4 B."""
+    read once and write once, FP64 = 8 B, int32 = 4 B.  The nact phase-ratio planes
+    are read only for the fraction mixfrac of elements holding more than one phase
+    (an element of one phase is flagged beside its phase and reads none)."""
     kernel = kernel.replace("<double>", "").replace("<0>", "")
     if kernel.startswith("k_rheol"):
-        per_e = (32 + 32 + 128 + 8 + 8 + 8 * nact + 4 + 24 + 8 + 24)  # area dvol stress0 aps visn pr iphase se2sr sshrheat strain
+        per_e = (32 + 32 + 128 + 8 + 8 + 8 * nact * mixfrac + 4 + 24 + 8 + 24)  # area dvol stress0 aps visn pr iphase se2sr sshrheat strain
         per_e += 128 + 8 + 24 + 24 + 8                                 # writes stress0 aps strain se2sr sshrheat
         per_n = 16 + 16                                                # vel, cord
         # with the move's areas pending: no dvol read, area and dvol written
@@ -66,9 +68,9 @@ def alg_bytes(kernel, E, N, nact, p, area_frac=0.0):
     if kernel == "k_move_rot":
         return E * (96 + 24 + 96 + 24) + N * 32                        # stress0(1:3) strain in and out | cord vel
     if kernel == "k_visc":
-        return E * (96 + 8 + 8 * nact + 4 + 8) + N * 8                 # stress0(1,2,4) e2sr pr iphase, visn out | temp
+        return E * (96 + 8 + 8 * nact * mixfrac + 4 + 8) + N * 8       # stress0(1,2,4) e2sr pr iphase, visn out | temp
     if kernel == "k_dens":
-        return E * (96 + 8 * nact + 4 + 8) + N * 8                     # stress0(1,2,4) pr iphase, dens out | temp
+        return E * (96 + 8 * nact * mixfrac + 4 + 8) + N * 8           # stress0(1,2,4) pr iphase, dens out | temp
     # (the step's masses and dt minima form the move's pending areas from the coordinates)
     if kernel == "k_rmass":
         return E * 8 + N * (8 + 16)                                    # dens | rmass out, cord
@@ -460,10 +462,11 @@ def main():
     dom = max(times, key=lambda k: times[k][0])
     ms_tot, nl = times[dom]
     nact = int((host_state.phase_ratio >= 0.01).any(axis=(1, 2)).sum())
-    ab = alg_bytes(dom, E, N, nact, gp, afrac)
+    mixfrac = float((host_state.phase_ratio.max(axis=0) < 1.0).mean())  # elements of more than one phase
+    ab = alg_bytes(dom, E, N, nact, gp, afrac, mixfrac)
     # a fallback pass and its check count only where the pass ran (the launches after a
     # proven pass exit at once)
-    step_bytes = sum(alg_bytes(k, E, N, nact, gp, afrac) * v[1] for k, v in times.items()
+    step_bytes = sum(alg_bytes(k, E, N, nact, gp, afrac, mixfrac) * v[1] for k, v in times.items()
                      if k not in ("k_therm_pipe_extra", "k_therm_check"))
     step_bytes += alg_bytes("k_therm_pipe_extra", E, N, nact, gp) * th[1]
     step_bytes += alg_bytes("k_therm_check", E, N, nact, gp) * (args.profile_steps + th[1])
@@ -473,7 +476,8 @@ def main():
     achieved = ab / avg_s / 1e9 if ab else 0.0
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "alg_bytes_per_launch": ab, "avg_launch_us": round(avg_s * 1e6, 2),
+                "alg_bytes_per_launch": ab, "mixed_element_fraction": round(mixfrac, 5),
+                "avg_launch_us": round(avg_s * 1e6, 2),
                 "kernel_ms_per_step": {k: round(v[0] / args.profile_steps, 4) for k, v in times.items() if v[1]},
                 "therm_extra_passes_per_step": th[1] / args.profile_steps,
                 # the whole step: algorithmic bytes of every launch over the kernels' time
