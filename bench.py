@@ -76,6 +76,8 @@ def alg_bytes(kernel, E, N, nact, p, area_frac=0.0, mixfrac=1.0):
         return E * 8 + N * (8 + 16)                                    # dens | rmass out, cord
     if kernel == "k_amass":
         return E * 4 + N * (8 + 16)                                    # iphase | amass out, cord
+    if kernel == "k_masses":
+        return E * (8 + 4) + N * (16 + 16)                             # dens iphase | rmass amass out, cord
     if kernel == "k_dt_reduce":
         return E * (8 + 4) + N * 32                                    # visn iphase | cord vel
     if kernel == "k_move_cord":
@@ -454,7 +456,7 @@ def main():
     afrac = area_pending_fraction(gp, n0, args.profile_steps)
     kernels = ["k_rheol1<false>", "k_rheol1<true>", "k_rheol", "k_visc<double>", "k_therm_pipe",
                "k_therm_pipe_extra", "k_therm_check", "k_node<0>", "k_node_t<0>", "k_move_elem<0>",
-               "k_move_area", "k_move_rot<0>", "k_move_cord<double>", "k_therm_bc", "k_therm_tail", "k_dens<double>", "k_rmass", "k_dt_reduce<double>", "k_amass",
+               "k_move_area", "k_move_rot<0>", "k_move_cord<double>", "k_therm_bc", "k_therm_tail", "k_dens<double>", "k_rmass", "k_dt_reduce<double>", "k_amass", "k_masses",
                "k_area_check", "k_clock"]
     times = {k: m.kernel_time(k) for k in kernels}
     m.set_timing(False)

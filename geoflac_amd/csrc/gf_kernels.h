@@ -2969,11 +2969,15 @@ GF_DEV void dens_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const G
 {
     A.dens[e] = eff_dens_value<M>(P, ck, A, e, rot);
 }
+// dtb: also starts dt_mass's minima for a k_dt_reduce that follows (CK_DT_BEGIN; no
+// kernel of this launch reads them)
 template <int M>
-__global__ void __launch_bounds__(256) k_dens(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A, int rot)
+__global__ void __launch_bounds__(256) k_dens(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A, int rot,
+                                              int dtb)
 {
     long e = xcd_lin();
     if (e >= NE) return;
+    if (dtb && e == 0) clock_op(ck, CK_DT_BEGIN);
     dens_body<M>(P, ck, A, e, rot);
 }
 
@@ -3234,6 +3238,8 @@ __global__ void k_dt_scalars(const GfDevParams *__restrict__ P, GfDevClock *ck)
 }
 
 // inertial masses, src/dt_mass.f90:64-120 (gather form of the coloured scatter)
+GF_DEV double amass_sum(const GfDevParams *__restrict__ P, GfDevClock *ck, const CornerSlots &cs,
+                        const double (&ar)[4][3], const int (&ip)[4]);
 GF_DEV void amass_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long t, int arp = 0)
 {
     if (P->idt_scale == 0) {
@@ -3241,18 +3247,25 @@ GF_DEV void amass_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const 
         return;
     }
     const int j = (int)((unsigned)t % (unsigned)NZ) + 1, i = (int)((unsigned)t / (unsigned)NZ) + 1;
-    const double c1d12 = 1.0 / 12.0;
-    const double dlmin = __longlong_as_double((long long)ck->dlmin_bits);
-    const double vel_max = __longlong_as_double((long long)ck->velmax_bits);
-    const double dt_elastic = ck->dt_elastic;
     const CornerSlots cs = corner_slots(P, j, i);
     const unsigned ne = (unsigned)NE;
     const Buf bI = mkbuf(A.iphase, 4ull * ne);
-    double ar[4][3], pw[4];
+    double ar[4][3];
     int ip[4];
 #pragma unroll
     for (int sl = 0; sl < 4; sl++) ip[sl] = bldi(bI, cs.e[sl] * 4u, 0);
     corner_areas(P, A, j, i, cs, arp, ar);
+    A.amass[t] = amass_sum(P, ck, cs, ar, ip);
+}
+// the inertial mass of a node from its slots' areas and phases (src/dt_mass.f90:74-154)
+GF_DEV double amass_sum(const GfDevParams *__restrict__ P, GfDevClock *ck, const CornerSlots &cs,
+                        const double (&ar)[4][3], const int (&ip)[4])
+{
+    const double c1d12 = 1.0 / 12.0;
+    const double dlmin = __longlong_as_double((long long)ck->dlmin_bits);
+    const double vel_max = __longlong_as_double((long long)ck->velmax_bits);
+    const double dt_elastic = ck->dt_elastic;
+    double pw[4];
 #pragma unroll
     for (int sl = 0; sl < 4; sl++) pw[sl] = cs.ok[sl] ? PHASE(ip[sl]).pwave : 0.0;
     double sum = 0.0;
@@ -3268,7 +3281,7 @@ GF_DEV void amass_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const 
 #pragma unroll
         for (int q = 0; q < 3; q++) sum = sum + c1d12 * rho_inert / ar[sl][q];
     }
-    A.amass[t] = sum;
+    return sum;
 }
 __global__ void __launch_bounds__(256) k_amass(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
                                                int arp)
@@ -3276,4 +3289,38 @@ __global__ void __launch_bounds__(256) k_amass(const GfDevParams *__restrict__ P
     long t = xcd_lin();
     if (t >= NN) return;
     amass_body(P, ck, A, t, arp);
+}
+// rmasses and dt_mass's inertial masses in one pass, in a step that runs both (after
+// k_dens, k_dt_reduce and k_dt_scalars, none of which reads rmass): the node's areas
+// are loaded or formed once for both sums, each summed in its own kernel's order
+GF_DEV void masses_body(const GfDevParams *__restrict__ P, GfDevClock *ck, const GfDevArrays &A, long t, int arp)
+{
+    const int j = (int)((unsigned)t % (unsigned)NZ) + 1, i = (int)((unsigned)t / (unsigned)NZ) + 1;
+    const double c1d12 = 1.0 / 12.0;
+    const CornerSlots cs = corner_slots(P, j, i);
+    const unsigned ne = (unsigned)NE;
+    const Buf bD = mkbuf(A.dens, 8ull * ne), bI = mkbuf(A.iphase, 4ull * ne);
+    double ar[4][3], dn[4];
+    int ip[4];
+#pragma unroll
+    for (int sl = 0; sl < 4; sl++) {
+        dn[sl] = bld(bD, cs.e[sl] * 8u, 0);
+        ip[sl] = bldi(bI, cs.e[sl] * 4u, 0);
+    }
+    corner_areas(P, A, j, i, cs, arp, ar);
+    double sum = 0.0;
+#pragma unroll
+    for (int sl = 0; sl < 4; sl++)
+        if (cs.ok[sl])
+#pragma unroll
+            for (int q = 0; q < 3; q++) sum = sum + c1d12 / ar[sl][q] * dn[sl];
+    A.rmass[t] = sum;
+    A.amass[t] = P->idt_scale == 0 ? sum : amass_sum(P, ck, cs, ar, ip);  // (dt_mass.f90:66: amass = rmass)
+}
+__global__ void __launch_bounds__(256) k_masses(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A,
+                                                int arp)
+{
+    long t = xcd_lin();
+    if (t >= NN) return;
+    masses_body(P, ck, A, t, arp);
 }

@@ -1287,7 +1287,8 @@ static int enq_move(gf_ctx *ctx)
     return GF_OK;
 }
 
-static int enq_rmasses(gf_ctx *ctx)
+// dtb: k_dens also starts the dt minima of a dt_mass that follows (enq_masses)
+static int enq_rmasses(gf_ctx *ctx, int dtb = 0)
 {
     // the masses form pending areas from the moved coordinates (corner_areas)
     const int arp = ctx->area_pending ? 1 : 0;
@@ -1295,12 +1296,12 @@ static int enq_rmasses(gf_ctx *ctx)
     // pending for the next fl_rheol)
     const int rot = ctx->rot_pending ? 1 : 0;
     if (ctx->precision == GF_PREC_MIXED)
-        LAUNCH_NAMED("k_dens<float>", k_dens<1>, ctx->ne, ctx->dp, ctx->dck, ctx->a, rot);
+        LAUNCH_NAMED("k_dens<float>", k_dens<1>, ctx->ne, ctx->dp, ctx->dck, ctx->a, rot, dtb);
     else if (ctx->precision == GF_PREC_MIXED_STORAGE)
-        LAUNCH_NAMED("k_dens<float>", k_dens<2>, ctx->ne, ctx->dp, ctx->dck, ctx->a, rot);
+        LAUNCH_NAMED("k_dens<float>", k_dens<2>, ctx->ne, ctx->dp, ctx->dck, ctx->a, rot, dtb);
     else
-        LAUNCH_NAMED("k_dens<double>", k_dens<0>, ctx->ne, ctx->dp, ctx->dck, ctx->a, rot);
-    LAUNCH(k_rmass, ctx->nn, ctx->dp, ctx->dck, ctx->a, arp);
+        LAUNCH_NAMED("k_dens<double>", k_dens<0>, ctx->ne, ctx->dp, ctx->dck, ctx->a, rot, dtb);
+    if (!dtb) LAUNCH(k_rmass, ctx->nn, ctx->dp, ctx->dck, ctx->a, arp);
     return GF_OK;
 }
 
@@ -1314,6 +1315,22 @@ static int enq_dt_mass(gf_ctx *ctx)
     hipLaunchKernelGGL(k_dt_scalars, dim3(1), dim3(64), 0, ctx->stream, ctx->dp, ctx->dck);
     if ((rc = launch_check(ctx, "k_dt_scalars"))) return rc;
     LAUNCH(k_amass, ctx->nn, ctx->dp, ctx->dck, ctx->a, arp);
+    return GF_OK;
+}
+
+// rmasses and dt_mass of one step (src/flac.f90:68-72) with one pass over the nodes
+// for both masses: k_dens (which also starts the dt minima), k_dt_reduce,
+// k_dt_scalars, then k_masses (rmass and amass; dt_mass does not read rmass except as
+// amass = rmass, src/dt_mass.f90:66)
+static int enq_masses(gf_ctx *ctx)
+{
+    int rc;
+    const int arp = ctx->area_pending ? 1 : 0;
+    if ((rc = enq_rmasses(ctx, 1))) return rc;
+    LAUNCH_SV(k_dt_reduce, (long)DT_RED_BLOCKS * 256, ctx->dp, ctx->dck, ctx->a, arp);
+    hipLaunchKernelGGL(k_dt_scalars, dim3(1), dim3(64), 0, ctx->stream, ctx->dp, ctx->dck);
+    if ((rc = launch_check(ctx, "k_dt_scalars"))) return rc;
+    LAUNCH(k_masses, ctx->nn, ctx->dp, ctx->dck, ctx->a, arp);
     return GF_OK;
 }
 
@@ -1430,8 +1447,13 @@ static int enq_flac(gf_ctx *ctx, int pending = 0, bool last = true)
         if (ctx->p.nystressbc == 1 && (rc = enq_bc_update(ctx))) return rc;
         if ((rc = enq_node(ctx, true))) return rc;
         if ((rc = enq_move(ctx))) return rc;
-        if (ctx->nloop % ctx->p.ifreq_rmasses == 0 && (rc = enq_rmasses(ctx))) return rc;
-        if (ctx->nloop % ctx->p.ifreq_imasses == 0 && (rc = enq_dt_mass(ctx))) return rc;
+        const bool rm = ctx->nloop % ctx->p.ifreq_rmasses == 0, im = ctx->nloop % ctx->p.ifreq_imasses == 0;
+        if (rm && im) {
+            if ((rc = enq_masses(ctx))) return rc;
+        } else {
+            if (rm && (rc = enq_rmasses(ctx))) return rc;
+            if (im && (rc = enq_dt_mass(ctx))) return rc;
+        }
     }
     return GF_OK;
 }
