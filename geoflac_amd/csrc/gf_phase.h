@@ -64,8 +64,15 @@ GF_DEV bool phase_has_plane(const GfDevParams *__restrict__ P, int ph)
 // tm of the callers included).  The interpolated depth and temperature of a marker
 // lie between its corner values (a convex combination, up to a few ulp of rounding,
 // which the margins cover), so the decision is exact.
+// sed_ocean: some element holds a sediment or oceanic phase (phase_sed_ocean; -1:
+// look it up here)
+GF_DEV bool phase_sed_ocean(const GfDevParams *__restrict__ P)
+{
+    return phase_has_plane(P, PH_OCEAN1) || phase_has_plane(P, PH_OCEAN2) || phase_has_plane(P, PH_SED1) ||
+           phase_has_plane(P, PH_SED2);
+}
 GF_DEV bool phase_rule_may_fire(const GfDevParams *__restrict__ P, int iph, double dlo, double dhi, double tmax,
-                                double tm)
+                                double tm, int sed_ocean = -1)
 {
     const double max_basalt_depth = 150.e3, min_eclogite_temp = 400.0, min_eclogite_depth = 20e3,
                  mantle_density = 3000.0, max_melting_depth = 200.e3;
@@ -76,8 +83,7 @@ GF_DEV bool phase_rule_may_fire(const GfDevParams *__restrict__ P, int iph, doub
         // remains: tmpr >= solidus + 35, and solidus + 35 = -5.02 x^2 + 132 x + 1160
         // (x the pressure in GPa) is concave, so on [0, xmax] at least its value at
         // an end of the interval
-        if (!(phase_has_plane(P, PH_OCEAN1) || phase_has_plane(P, PH_OCEAN2) || phase_has_plane(P, PH_SED1) ||
-              phase_has_plane(P, PH_SED2))) {
+        if (!(sed_ocean < 0 ? phase_sed_ocean(P) : sed_ocean != 0)) {
             const double dabs = fmax(fabs(dlo), fabs(fmin(dhi, max_basalt_depth)));
             const double xmax = mantle_density * fabs(P->g) * dabs * 1e-9 * (1.0 + 1e-12);
             const double smin = fmin(1160.0, -5.02 * (xmax * xmax) + 1.32e2 * xmax + 1.16e3) - 1e-6;
@@ -223,9 +229,11 @@ GF_DEV void phase_rules_elem_body(const GfDevParams *__restrict__ P, GfDevClock 
     const double ylo = fmin(fmin(y1, y2), fmin(y3, y4)), yhi = fmax(fmax(y1, y2), fmax(y3, y4));
     const double dm = 1e-6 * (fabs(ysurf) + fabs(ylo) + fabs(yhi) + 1.0), tm = 1e-9 * (fabs(tmax) + 1.0);
     const double dlo = ysurf - yhi - dm, dhi = ysurf - ylo + dm;
+    // the phases markers may hold, lowest id first (the plane lookup once per element)
+    const int sed_ocean = phase_sed_ocean(P) ? 1 : 0;
     bool may = false;
-    for (int ph = 1; ph <= GF_MAXPH && !may; ph++)
-        if (present & (1u << (ph - 1))) may = phase_rule_may_fire(P, ph, dlo, dhi, tmax, tm);
+    for (unsigned m = present; m != 0u && !may; m &= m - 1u)
+        may = phase_rule_may_fire(P, __ffs(m), dlo, dhi, tmax, tm, sed_ocean);
     if (!may) return;
     const int nm = A.nmark_elem[e];
     for (int q = 0; q < nm; q++) phase_rules_body(P, ck, A, (long)A.mark_id_elem[q + (long)GF_MAX_MARKERS_PER_ELEM * e] - 1);
