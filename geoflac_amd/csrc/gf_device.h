@@ -76,6 +76,9 @@ struct GfDevClock {
     // bounded wait that ran out
     unsigned int bar_cnt, bar_gen;
     int bar_timeout, bar_pad;
+    // dt_mass's minimum side measure as the largest product area x side length over
+    // the elements (bits of a non-negative double); dlmin is its reciprocal
+    unsigned long long dly_bits;
 };
 
 // ---------------------------------------------------------------- buffer access

@@ -122,6 +122,7 @@ GF_DEV void clock_op(GfDevClock *ck, int op)
     if (op & CK_DTAVG_RESET) ck->dtavg = 0.0;
     if (op & CK_DT_BEGIN) {
         ck->dlmin_bits = (unsigned long long)__double_as_longlong(1.0e28);
+        ck->dly_bits = 0ull;
         ck->velmax_bits = 0ull;
     }
 }
@@ -3108,10 +3109,17 @@ GF_DEV double side_len(const GfDevParams *P, const GfDevArrays &A, int ja, int i
 // dlmin_prop (src/dt_mass.f90:168-232) and vel_max (:55-62): grid-stride loops, a
 // wave then block reduction, one atomic per block; min/max of positive doubles are
 // exact in any order, and their bit patterns order like the values.
+// The reference takes dlmin = min over the element's six sides and the two triangles
+// at each of 1/(area * side).  For positive products y, RN(1/y) does not increase
+// with y, so that minimum is RN(1/max y): the element returns max y (0 where none is
+// positive), the reduction takes the maximum over the elements and k_dt_scalars the
+// one reciprocal (dlmin_from_dly); the same bits without twelve divisions per element.
+// (A product that is not positive only arises from an inverted element, whose
+// negative-area STOP ends the run.)
 // ARP: the last move's areas are pending: formed from the moved coordinates
 // (move_area_body's arithmetic) instead of being read
 template <bool ARP = false>
-GF_DEV double elem_dlmin(const GfDevParams *P, const GfDevArrays &A, long t)
+GF_DEV double elem_dly(const GfDevParams *P, const GfDevArrays &A, long t)
 {
     const int j = (int)((unsigned)t % (unsigned)(NZ - 1)) + 1, i = (int)((unsigned)t / (unsigned)(NZ - 1)) + 1;
     double ar[4];
@@ -3122,27 +3130,27 @@ GF_DEV double elem_dlmin(const GfDevParams *P, const GfDevArrays &A, long t)
         for (int k = 0; k < 4; k++) ar[k] = 1.0 / move_det(xc, yc, k);
     }
 #define DLAREA(j, i, k) (ARP ? ar[(k) - 1] : A.area[IE(j, i) + NE * ((k) - 1)])
-    double dlmin = 1.0e28, dl;
+    double dly = 0.0, dl;
     dl = side_len(P, A, j, i, j + 1, i);
-    dlmin = fmin(dlmin, 1.0 / (DLAREA(j, i, 1) * dl));
-    dlmin = fmin(dlmin, 1.0 / (DLAREA(j, i, 3) * dl));
+    dly = fmax(dly, DLAREA(j, i, 1) * dl);
+    dly = fmax(dly, DLAREA(j, i, 3) * dl);
     dl = side_len(P, A, j + 1, i, j + 1, i + 1);
-    dlmin = fmin(dlmin, 1.0 / (DLAREA(j, i, 2) * dl));
-    dlmin = fmin(dlmin, 1.0 / (DLAREA(j, i, 3) * dl));
+    dly = fmax(dly, DLAREA(j, i, 2) * dl);
+    dly = fmax(dly, DLAREA(j, i, 3) * dl);
     dl = side_len(P, A, j, i + 1, j + 1, i + 1);
-    dlmin = fmin(dlmin, 1.0 / (DLAREA(j, i, 2) * dl));
-    dlmin = fmin(dlmin, 1.0 / (DLAREA(j, i, 4) * dl));
+    dly = fmax(dly, DLAREA(j, i, 2) * dl);
+    dly = fmax(dly, DLAREA(j, i, 4) * dl);
     dl = side_len(P, A, j, i, j, i + 1);
-    dlmin = fmin(dlmin, 1.0 / (DLAREA(j, i, 1) * dl));
-    dlmin = fmin(dlmin, 1.0 / (DLAREA(j, i, 4) * dl));
+    dly = fmax(dly, DLAREA(j, i, 1) * dl);
+    dly = fmax(dly, DLAREA(j, i, 4) * dl);
     dl = side_len(P, A, j, i, j + 1, i + 1);
-    dlmin = fmin(dlmin, 1.0 / (DLAREA(j, i, 3) * dl));
-    dlmin = fmin(dlmin, 1.0 / (DLAREA(j, i, 4) * dl));
+    dly = fmax(dly, DLAREA(j, i, 3) * dl);
+    dly = fmax(dly, DLAREA(j, i, 4) * dl);
     dl = side_len(P, A, j, i + 1, j + 1, i);
-    dlmin = fmin(dlmin, 1.0 / (DLAREA(j, i, 1) * dl));
-    dlmin = fmin(dlmin, 1.0 / (DLAREA(j, i, 2) * dl));
+    dly = fmax(dly, DLAREA(j, i, 1) * dl);
+    dly = fmax(dly, DLAREA(j, i, 2) * dl);
 #undef DLAREA
-    return dlmin;
+    return dly;
 }
 
 #define DT_RED_BLOCKS 1024
@@ -3152,21 +3160,21 @@ GF_DEV void dt_reduce_body(const GfDevParams *__restrict__ P, GfDevClock *ck, co
                            int nblk, int arp = 0)
 {
     const SV *vel = sv_ptr<SV>(A.vel, A.velf);
-    __shared__ double s_min[4], s_max[4];
+    __shared__ double s_dly[4], s_max[4];
     const long stride = (long)nblk * blockDim.x;
-    double dlmin = 1.0e28, vmax = 0.0;
+    double dly = 0.0, vmax = 0.0;
     // four elements (eight velocities) per iteration, so that their loads are in
-    // flight together; min and max are exact in any order
+    // flight together; maxima are exact in any order
     const long ne = NE, n2 = 2 * NN;
     for (long t = (long)blk * blockDim.x + threadIdx.x; t < ne; t += 4 * stride) {
         double d[4];
 #pragma unroll
         for (int u = 0; u < 4; u++)
-            d[u] = t + u * stride >= ne ? 1.0e28
-                   : arp            ? elem_dlmin<true>(P, A, t + u * stride)
-                                    : elem_dlmin<false>(P, A, t + u * stride);
+            d[u] = t + u * stride >= ne ? 0.0
+                   : arp            ? elem_dly<true>(P, A, t + u * stride)
+                                    : elem_dly<false>(P, A, t + u * stride);
 #pragma unroll
-        for (int u = 0; u < 4; u++) dlmin = fmin(dlmin, d[u]);
+        for (int u = 0; u < 4; u++) dly = fmax(dly, d[u]);
     }
     for (long n = (long)blk * blockDim.x + threadIdx.x; n < n2; n += 8 * stride) {
         double v[8];
@@ -3176,21 +3184,21 @@ GF_DEV void dt_reduce_body(const GfDevParams *__restrict__ P, GfDevClock *ck, co
         for (int u = 0; u < 8; u++) vmax = fmax(vmax, v[u]);
     }
     for (int off = 32; off > 0; off >>= 1) {
-        dlmin = fmin(dlmin, __shfl_xor(dlmin, off));
+        dly = fmax(dly, __shfl_xor(dly, off));
         vmax = fmax(vmax, __shfl_xor(vmax, off));
     }
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        s_min[w] = dlmin;
+        s_dly[w] = dly;
         s_max[w] = vmax;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int q = 1; q < (int)(blockDim.x >> 6); q++) {
-            dlmin = fmin(dlmin, s_min[q]);
+            dly = fmax(dly, s_dly[q]);
             vmax = fmax(vmax, s_max[q]);
         }
-        atomicMin(&ck->dlmin_bits, (unsigned long long)__double_as_longlong(dlmin));
+        atomicMax(&ck->dly_bits, (unsigned long long)__double_as_longlong(dly));
         atomicMax(&ck->velmax_bits, (unsigned long long)__double_as_longlong(vmax));
     }
 }
@@ -3207,6 +3215,10 @@ __global__ void __launch_bounds__(256) k_dt_reduce(const GfDevParams *__restrict
 GF_DEV void dt_scalars_body(const GfDevParams *__restrict__ P, GfDevClock *ck)
 {
     const double sec_year = 3.1558e+7;
+    {  // dlmin from the reduction's largest product (elem_dly): min(1e28, 1/max)
+        const double dly = __longlong_as_double((long long)ck->dly_bits);
+        if (dly > 0.0) ck->dlmin_bits = (unsigned long long)__double_as_longlong(fmin(1.0e28, 1.0 / dly));
+    }
     double dlmin = __longlong_as_double((long long)ck->dlmin_bits);
     if (P->idt_scale == 1) {
         ck->dt_elastic = P->dt_scale;
