@@ -82,9 +82,12 @@ def alg_bytes(kernel, E, N, nact, p, area_frac=0.0, mixfrac=1.0):
         return E * (8 + 4) + N * 32                                    # visn iphase | cord vel
     if kernel == "k_move_cord":
         return N * 48
-    if kernel in ("k_therm_bc", "k_therm_tail"):
-        # the boundary conditions (a fallback pass inside k_therm_tail is added where it ran)
+    if kernel == "k_therm_bc":  # the boundary conditions, copied to temp
         return N * 16
+    if kernel == "k_therm_tail":
+        # the boundary conditions in place on the boundary nodes of the result, which
+        # becomes temp (a fallback pass inside k_therm_tail is added where it ran)
+        return (2 * p.nx + 2 * p.nz) * 16
     if kernel == "k_area_check":
         return N * 16                                                  # cord (the move's negative-area test)
     if kernel == "k_therm_check":
