@@ -79,6 +79,8 @@ struct GfDevClock {
     // dt_mass's minimum side measure as the largest product area x side length over
     // the elements (bits of a non-negative double); dlmin is its reciprocal
     unsigned long long dly_bits;
+    // itest_mesh's smallest triangle angle, as an order-preserving key (rem_angle_key)
+    unsigned long long rem_key;
 };
 
 // ---------------------------------------------------------------- buffer access

@@ -3099,6 +3099,68 @@ __global__ void __launch_bounds__(256) k_rmass(const GfDevParams *__restrict__ P
     rmass_body(P, ck, A, t, arp);
 }
 
+// ------------------------------------------------------------------ itest_mesh
+// src/rem_test.f90:54-96: the smallest angle of the four triangles of every element
+// (two corners by acos, the third as 180 minus both), reduced over the mesh by an
+// atomic minimum on an order-preserving key of the double (any sign)
+GF_DEV unsigned long long rem_angle_key(double x)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+GF_DEV double rem_elem_angle(const GfDevParams *__restrict__ P, const GfDevArrays &A, int j, int i)
+{
+    const double pi = 3.14159265358979323846, raddeg = 180.0 / pi;
+    double amin = 180.0;
+#pragma unroll
+    for (int ii = 1; ii <= 4; ii++) {
+        int iv[4], jv[4];
+        if (ii == 1) {
+            iv[1] = i; jv[1] = j; iv[2] = i; jv[2] = j + 1; iv[3] = i + 1; jv[3] = j;
+        } else if (ii == 2) {
+            iv[1] = i; jv[1] = j + 1; iv[2] = i + 1; jv[2] = j + 1; iv[3] = i + 1; jv[3] = j;
+        } else if (ii == 3) {
+            iv[1] = i; jv[1] = j; iv[2] = i; jv[2] = j + 1; iv[3] = i + 1; jv[3] = j + 1;
+        } else {
+            iv[1] = i; jv[1] = j; iv[2] = i + 1; jv[2] = j + 1; iv[3] = i + 1; jv[3] = j;
+        }
+        iv[0] = iv[1]; jv[0] = jv[1];  // (the reference's iv(4), jv(4): wraps to corner 1)
+        double angle[4];
+#pragma unroll
+        for (int k = 2; k <= 3; k++) {
+            const int kn = k + 1 == 4 ? 0 : k + 1;
+            const double xa = CORD(jv[kn], iv[kn], 1) - CORD(jv[k], iv[k], 1);
+            const double ya = CORD(jv[kn], iv[kn], 2) - CORD(jv[k], iv[k], 2);
+            const double xxal = sqrt(xa * xa + ya * ya);
+            const double xb = CORD(jv[k - 1], iv[k - 1], 1) - CORD(jv[k], iv[k], 1);
+            const double yb = CORD(jv[k - 1], iv[k - 1], 2) - CORD(jv[k], iv[k], 2);
+            const double xxbl = sqrt(xb * xb + yb * yb);
+            angle[k] = raddeg * acos((xa * xb + ya * yb) / (xxal * xxbl));
+        }
+        angle[1] = 180.0 - angle[2] - angle[3];
+        const double a1 = fmin(fmin(angle[1], angle[2]), angle[3]);
+        amin = fmin(amin, a1);
+    }
+    return amin;
+}
+__global__ void __launch_bounds__(256) k_rem_angle(const GfDevParams *__restrict__ P, GfDevClock *ck, GfDevArrays A)
+{
+    __shared__ double s_min[4];
+    const long e = xcd_lin();
+    double amin = 180.0;
+    if (e < NE) {
+        const int j = (int)((unsigned)e % (unsigned)(NZ - 1)) + 1, i = (int)((unsigned)e / (unsigned)(NZ - 1)) + 1;
+        amin = rem_elem_angle(P, A, j, i);
+    }
+    for (int off = 32; off > 0; off >>= 1) amin = fmin(amin, __shfl_xor(amin, off));
+    if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = amin;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < (int)(blockDim.x >> 6); q++) amin = fmin(amin, s_min[q]);
+        atomicMin(&ck->rem_key, rem_angle_key(amin));
+    }
+}
+
 // ------------------------------------------------------------------ dt_mass
 GF_DEV double side_len(const GfDevParams *P, const GfDevArrays &A, int ja, int ia, int jb, int ib)
 {

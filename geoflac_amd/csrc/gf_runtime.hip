@@ -917,6 +917,77 @@ extern "C" int gf_marker_count(gf_ctx *ctx, int32_t *nmarkers)
     return GF_OK;
 }
 
+static int check_ready(gf_ctx *ctx);
+// src/rem_test.f90:9-139 (check_nan is empty in the reference)
+extern "C" int gf_itest_mesh(gf_ctx *ctx, int32_t mode_rem, double dx_rem, double angle_rem,
+                             double ratio_mantle_mzone, int32_t *need_remeshing, double *anglemin, double *value)
+{
+    if (!ctx || !need_remeshing || !anglemin || !value) return GF_ERR_ARG;
+    int rc = check_ready(ctx);
+    if (rc) return rc;
+    *need_remeshing = 0;
+    *anglemin = 180.0;
+    *value = 0.0;
+    const long nz = ctx->p.nz, nx = ctx->p.nx, nn = nz * nx;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    auto cord = [&](long j, long i, int d, double &v) {  // one coordinate from the device
+        return hipMemcpy(&v, ctx->a.cord + (j - 1) + nz * (i - 1) + nn * (d - 1), sizeof(double),
+                         hipMemcpyDeviceToHost);
+    };
+    if (mode_rem == 11 || mode_rem == 3) {  // :15-29
+        double x1 = 0.0, xn = 0.0;
+        HIPCHK(cord(1, 1, 1, x1));
+        HIPCHK(cord(1, nx, 1, xn));
+        const double testcr = dx_rem * ctx->p.rxbo / (nx - 1);
+        const double shortening = fabs(xn - x1 - ctx->p.rxbo);
+        if (shortening > testcr) {
+            *need_remeshing = 2;
+            *value = shortening;
+            return GF_OK;
+        }
+    }
+    const double arc_extrusion_rate = 1.0 - ratio_mantle_mzone;  // :31-49
+    if (ctx->p.topo_kappa > 0 || arc_extrusion_rate > 0) {
+        double y11 = 0.0, y21 = 0.0;
+        HIPCHK(cord(1, 1, 2, y11));
+        HIPCHK(cord(2, 1, 2, y21));
+        const double testcr = (y11 - y21) * 0.3333;
+        // dhacc (diff_topo's accumulated change; zero where diff_topo does not run);
+        // extr_acc is zero without melting (refused at gf_create)
+        std::vector<double> dh(nx - 1, 0.0);
+        if (ctx->a.dhacc) HIPCHK(hipMemcpy(dh.data(), ctx->a.dhacc, sizeof(double) * (nx - 1), hipMemcpyDeviceToHost));
+        double topochanges = 0.0;
+        for (long i = 1; i <= nx - 1; i++) {
+            topochanges = fabs(dh[i - 1]) + 0.0;
+            if (topochanges > testcr) *need_remeshing = 3;
+        }
+        if (*need_remeshing == 3) {
+            *value = topochanges;  // (the last column's, as the reference prints it)
+            return GF_OK;
+        }
+    }
+    // :51-129, the smallest angle
+    const GfDevClock init_key = [] {
+        GfDevClock z{};
+        const double a = 180.0;
+        unsigned long long u;
+        memcpy(&u, &a, sizeof u);
+        z.rem_key = (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+        return z;
+    }();
+    HIPCHK(hipMemcpy((char *)ctx->dck + offsetof(GfDevClock, rem_key), &init_key.rem_key, sizeof init_key.rem_key,
+                     hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_rem_angle, dim3(nblk(ctx->ne)), dim3(256), 0, ctx->stream, ctx->dp, ctx->dck, ctx->a);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    unsigned long long key = 0;
+    HIPCHK(hipMemcpy(&key, (char *)ctx->dck + offsetof(GfDevClock, rem_key), sizeof key, hipMemcpyDeviceToHost));
+    const unsigned long long u = (key >> 63) ? (key & 0x7fffffffffffffffull) : ~key;
+    memcpy(anglemin, &u, sizeof u);
+    if (*anglemin <= angle_rem) *need_remeshing = 1;
+    return GF_OK;
+}
+
 extern "C" int gf_set_clock(gf_ctx *ctx, const gf_clock *c)
 {
     if (!ctx || !c) return GF_ERR_ARG;

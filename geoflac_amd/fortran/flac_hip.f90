@@ -167,6 +167,58 @@ if (rc /= GF_OK) call flac_hip_fail(rc)
 nmarkers = n
 end subroutine flac_hip_sync_host
 
+! The driver's remeshing test, itest_mesh (src/rem_test.f90:9-139), on the device's
+! coordinates, with the reference's messages: a driver copies the state back
+! (flac_hip_sync_host) only when it remeshes.
+subroutine flac_hip_itest_mesh(need_remeshing)
+use iso_c_binding
+use gf_hip
+use flac_hip_state
+use params
+implicit none
+integer, intent(out) :: need_remeshing
+integer(c_int) :: rc
+integer(c_int32_t) :: need
+real(c_double) :: anglemint, val
+need_remeshing = 0
+if (.not. c_associated(gf_ctx_handle)) return
+rc = gf_itest_mesh(gf_ctx_handle, int(mode_rem, c_int32_t), dx_rem, angle_rem, ratio_mantle_mzone, need, &
+                   anglemint, val)
+if (rc /= GF_OK) call flac_hip_fail(rc)
+need_remeshing = need
+if (need_remeshing == 2) then
+    if( dtout_screen .ne. 0 ) then
+        print *, 'Remeshing due to shortening required: ', val
+        write(333,*) 'Remeshing due to shortening required: ', val
+    else
+        call SysMsg('TEST_MESH: Remeshing due to shortening required')
+    endif
+    return
+endif
+if (need_remeshing == 3) then
+    if( dtout_screen .ne. 0 ) then
+        print *, 'Remeshing due to surface topo changes required: ', val
+        write(333,*) 'Remeshing due to surface topo changes required: ', val
+    else
+        call SysMsg('TEST_MESH: Remeshing due to surface topo changes required')
+    endif
+    return
+endif
+if( dtout_screen .ne. 0 ) then
+    write (6,'(A,F6.2,A,F10.6)') '        min.angle=',anglemint, '     dt(yr)=',dt/sec_year
+    write (333,'(A,F6.2,A,F10.6)') '        min.angle=',anglemint, '     dt(yr)=',dt/sec_year
+    flush (333)
+endif
+if (need_remeshing == 1) then
+    if( dtout_screen .ne. 0 ) then
+        print *, 'Remeshing due to angle required.'
+        write(333,*) 'Remeshing due to angle required.'
+    else
+        call SysMsg('TEST_MESH: Remeshing due to angle required.')
+    endif
+endif
+end subroutine flac_hip_itest_mesh
+
 ! Fortran arrays -> device (after the host changed them, e.g. remeshing).
 subroutine flac_hip_sync_device
 use iso_c_binding

@@ -109,6 +109,15 @@ interface
         type(c_ptr), value :: ctx
         integer(c_int32_t), intent(out) :: n
     end function
+    integer(c_int) function gf_itest_mesh(ctx, mode_rem, dx_rem, angle_rem, ratio_mantle_mzone, need, anglemin, &
+                                          val) bind(C, name='gf_itest_mesh')
+        import :: c_int, c_int32_t, c_double, c_ptr
+        type(c_ptr), value :: ctx
+        integer(c_int32_t), value :: mode_rem
+        real(c_double), value :: dx_rem, angle_rem, ratio_mantle_mzone
+        integer(c_int32_t), intent(out) :: need
+        real(c_double), intent(out) :: anglemin, val
+    end function
     integer(c_int) function gf_upload(ctx, mask) bind(C, name='gf_upload')
         import :: c_int, c_int32_t, c_ptr
         type(c_ptr), value :: ctx

@@ -23,7 +23,7 @@ EXPORTS = [
     "gf_surrogate_sample", "gf_surrogate_train", "gf_surrogate_enable", "gf_surrogate_stats",
     "gf_surrogate_weights", "gf_mlp_grad", "gf_mlp_infer", "gf_mlp_grad_noalloc",
     "gf_batch_create", "gf_batch_destroy", "gf_batch_last_error", "gf_batch_step", "gf_batch_set_graph",
-    "gf_batch_autotune", "gf_stream_probe", "gf_marker_count",
+    "gf_batch_autotune", "gf_stream_probe", "gf_marker_count", "gf_itest_mesh",
 ]
 MLP_NPARAM = 4737
 
@@ -53,6 +53,8 @@ def lib():
         L.gf_upload.argtypes = [vp, C.c_uint32]
         L.gf_download.argtypes = [vp, C.c_uint32]
         L.gf_marker_count.argtypes = [vp, C.POINTER(C.c_int32)]
+        L.gf_itest_mesh.argtypes = [vp, C.c_int32, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int32),
+                                    C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.gf_set_clock.argtypes = [vp, C.POINTER(abi.GfClock)]
         L.gf_get_clock.argtypes = [vp, C.POINTER(abi.GfClock)]
         L.gf_step.argtypes = [vp, C.c_int32, C.POINTER(abi.GfClock)]
@@ -150,6 +152,14 @@ class Model:
         n = C.c_int32()
         self._check(lib().gf_marker_count(self._ctx, C.byref(n)))
         return n.value
+
+    def itest_mesh(self, mode_rem, dx_rem, angle_rem, ratio_mantle_mzone=1.0):
+        """The remeshing test itest_mesh (src/rem_test.f90:9-139) on the device's
+        coordinates: (need_remeshing, smallest angle in degrees, printed value)."""
+        need, amin, val = C.c_int32(), C.c_double(), C.c_double()
+        self._check(lib().gf_itest_mesh(self._ctx, int(mode_rem), float(dx_rem), float(angle_rem),
+                                        float(ratio_mantle_mzone), C.byref(need), C.byref(amin), C.byref(val)))
+        return need.value, amin.value, val.value
 
     def step(self, nsteps):
         """nsteps iterations of the reference's inner loop (flac; nloop+=1; time+=dt)."""

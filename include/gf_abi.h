@@ -173,6 +173,19 @@ int gf_bind_markers(gf_ctx *ctx, const gf_marker_arrays *m);
 /* The device's marker count (nmarkers of module marker_data), which resurface grows;
  * gf_download(GF_MARKERS) copies that many markers back. */
 int gf_marker_count(gf_ctx *ctx, int32_t *nmarkers);
+/* itest_mesh (src/rem_test.f90:9-139; the driver's remeshing test, src/par.f90:94-95)
+ * on the device's current coordinates, with the remeshing parameters of module params
+ * that gf_params does not carry: need_remeshing as the reference sets it (0 none;
+ * 1 smallest triangle angle <= angle_rem; 2 shortening beyond dx_rem, mode_rem 11 or
+ * 3; 3 surface topography changes beyond a third of the top element height, where
+ * topo_kappa > 0 or 1 - ratio_mantle_mzone > 0), the smallest angle in degrees
+ * (180 when the angle test did not run), and the value the reference prints with a
+ * shortening or topography reason (0 otherwise).  A driver then copies the state
+ * back (gf_download) only when a remeshing is needed.  The angles use the device's
+ * acos: the smallest angle agrees with the reference's to ~1e-13 relative, and the
+ * decision unless that angle lies within this of angle_rem. */
+int gf_itest_mesh(gf_ctx *ctx, int32_t mode_rem, double dx_rem, double angle_rem, double ratio_mantle_mzone,
+                  int32_t *need_remeshing, double *anglemin, double *value);
 /* Host -> device and device -> host copies of the fields in `mask`, with the
  * layout conversion to the device's own planes.  Upload of everything is what
  * the reference does at setflac/rsflac time (src/setflac.f90:4-100); download is

@@ -1058,6 +1058,73 @@ static double dlmin_prop(const gf_params *p, const gf_host_arrays *s)
     return dlmin;
 }
 
+/* src/rem_test.f90:9-139 (check_nan is empty in the reference) */
+int gfo_itest_mesh(const gf_params *p, const gf_host_arrays *s, int mode_rem, double dx_rem, double angle_rem,
+                   double ratio_mantle_mzone, int *need, double *anglemin, double *value)
+{
+    *need = 0;
+    *anglemin = 180.0;
+    *value = 0.0;
+    if (mode_rem == 11 || mode_rem == 3) { /* :15-29 */
+        const double testcr = dx_rem * p->rxbo / (NX - 1);
+        const double shortening = fabs(CORD(1, NX, 1) - CORD(1, 1, 1) - p->rxbo);
+        if (shortening > testcr) {
+            *need = 2;
+            *value = shortening;
+            return 0;
+        }
+    }
+    const double arc_extrusion_rate = 1.0 - ratio_mantle_mzone; /* :31-49 */
+    if (p->topo_kappa > 0 || arc_extrusion_rate > 0) {
+        double topochanges = 0.0;
+        for (int i = 1; i <= NX - 1; i++) {
+            const double testcr = (CORD(1, 1, 2) - CORD(2, 1, 2)) * 0.3333;
+            topochanges = fabs(s->dhacc ? s->dhacc[i - 1] : 0.0) + 0.0; /* extr_acc: 0 without melting */
+            if (topochanges > testcr) *need = 3;
+        }
+        if (*need == 3) {
+            *value = topochanges;
+            return 0;
+        }
+    }
+    const double pi = 3.14159265358979323846, raddeg = 180.0 / pi; /* :51-129 */
+    double anglemint = 180.0;
+    for (int i = 1; i <= NX - 1; i++)
+        for (int j = 1; j <= NZ - 1; j++)
+            for (int ii = 1; ii <= 4; ii++) {
+                int iv[5], jv[5];
+                if (ii == 1) {
+                    iv[1] = i; jv[1] = j; iv[2] = i; jv[2] = j + 1; iv[3] = i + 1; jv[3] = j;
+                } else if (ii == 2) {
+                    iv[1] = i; jv[1] = j + 1; iv[2] = i + 1; jv[2] = j + 1; iv[3] = i + 1; jv[3] = j;
+                } else if (ii == 3) {
+                    iv[1] = i; jv[1] = j; iv[2] = i; jv[2] = j + 1; iv[3] = i + 1; jv[3] = j + 1;
+                } else {
+                    iv[1] = i; jv[1] = j; iv[2] = i + 1; jv[2] = j + 1; iv[3] = i + 1; jv[3] = j;
+                }
+                iv[4] = iv[1];
+                jv[4] = jv[1];
+                double angle[4];
+                for (int k = 2; k <= 3; k++) {
+                    const double xa = CORD(jv[k + 1], iv[k + 1], 1) - CORD(jv[k], iv[k], 1);
+                    const double ya = CORD(jv[k + 1], iv[k + 1], 2) - CORD(jv[k], iv[k], 2);
+                    const double xxal = sqrt(xa * xa + ya * ya);
+                    const double xb = CORD(jv[k - 1], iv[k - 1], 1) - CORD(jv[k], iv[k], 1);
+                    const double yb = CORD(jv[k - 1], iv[k - 1], 2) - CORD(jv[k], iv[k], 2);
+                    const double xxbl = sqrt(xb * xb + yb * yb);
+                    angle[k] = raddeg * acos((xa * xb + ya * yb) / (xxal * xxbl));
+                }
+                angle[1] = 180.0 - angle[2] - angle[3];
+                double a1 = angle[1];
+                if (angle[2] < a1) a1 = angle[2];
+                if (angle[3] < a1) a1 = angle[3];
+                if (a1 < anglemint) anglemint = a1;
+            }
+    *anglemin = anglemint;
+    if (anglemint <= angle_rem) *need = 1;
+    return 0;
+}
+
 int gfo_dt_mass(const gf_params *p, gf_host_arrays *s, gf_clock *c)
 {
     const double c1d12 = 1.0 / 12.0;

@@ -22,6 +22,10 @@ int gfo_fl_node(const gf_params *p, gf_host_arrays *s, gf_clock *c);
 int gfo_fl_move(const gf_params *p, gf_host_arrays *s, gf_clock *c);
 int gfo_rmasses(const gf_params *p, gf_host_arrays *s, gf_clock *c);
 int gfo_dt_mass(const gf_params *p, gf_host_arrays *s, gf_clock *c);
+/* itest_mesh (src/rem_test.f90:9-139): need_remeshing, the smallest angle and the
+ * printed value, as gf_itest_mesh returns them */
+int gfo_itest_mesh(const gf_params *p, const gf_host_arrays *s, int mode_rem, double dx_rem, double angle_rem,
+                   double ratio_mantle_mzone, int *need, double *anglemin, double *value);
 /* change_phase runs on the markers bound here (NULL: none, it does nothing) */
 void gfo_bind_markers(gf_marker_arrays *m);
 int gfo_change_phase(const gf_params *p, gf_host_arrays *s, gf_clock *c);

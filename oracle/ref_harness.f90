@@ -107,10 +107,14 @@ subroutine remesh_block
     integer :: need_remeshing
     if (ny_rem /= 1 .or. itherm == 2) return
 #ifdef GF_DROPIN
+    ! the test on the device; the state comes back only for a remeshing
+    call flac_hip_itest_mesh(need_remeshing)
+    if (need_remeshing == 0) return
     call flac_hip_sync_host
-#endif
+#else
     call itest_mesh(need_remeshing)
     if (need_remeshing == 0) return
+#endif
     nrem = nrem + 1
     call bar2euler
     call remesh

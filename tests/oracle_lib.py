@@ -92,3 +92,17 @@ def golden_full(case, step, marker_room=False):
         s0.set_markers(*arrs, capacity=s0.nz * s0.nx * abi.MAX_MARKERS_PER_ELEM if marker_room else None,
                        age=None if src.mark_age is None else src.mark_age[:n])
     return s0
+
+
+def itest_mesh(p, st, mode_rem, dx_rem, angle_rem, ratio_mantle_mzone=1.0):
+    """The oracle's itest_mesh (src/rem_test.f90:9-139) on a state: (need, angle, value)."""
+    L = lib()
+    f = L.gfo_itest_mesh
+    f.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int),
+                  C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    h = st.host_arrays()
+    need, amin, val = C.c_int(), C.c_double(), C.c_double()
+    rc = f(C.byref(p), C.byref(h), int(mode_rem), float(dx_rem), float(angle_rem), float(ratio_mantle_mzone),
+           C.byref(need), C.byref(amin), C.byref(val))
+    assert rc == 0
+    return need.value, amin.value, val.value

@@ -355,3 +355,37 @@ def test_thermal_sine_mode_decays_at_the_analytic_rate():
         assert ol.call("gfo_fl_therm", p, st) == 0
     rate = -math.log(st.temp[20, 2]) / (dt * nstep)
     assert rate / (kappa * (math.pi / H) ** 2) == pytest.approx(1.0, abs=0.02)
+
+
+def test_itest_mesh_closed_forms():
+    """The oracle's remeshing test (src/rem_test.f90:9-139) on the undeformed box60x20
+    mesh: rectangles dx x dz cut into right triangles, so the smallest angle is
+    atan(min(dx, dz) / max(dx, dz)) in degrees; angle_rem just above it asks for a
+    remeshing (1), just below not; a shortening beyond dx_rem (mode_rem 11) gives 2
+    with the shortening; a surface change beyond a third of the top element height
+    (topo_kappa > 0) gives 3 with the last column's change."""
+    p = ol.golden_params(CASE)
+    st = ol.golden_state(CASE, 0)
+    nz, nx = p.nz, p.nx
+    cx, cy = st.cord[:, :, 0], st.cord[:, :, 1]  # node (j, i) at [j - 1, i - 1]
+    dx, dz = abs(cx[0, 1] - cx[0, 0]), abs(cy[0, 0] - cy[1, 0])
+    want = math.degrees(math.atan(min(dx, dz) / max(dx, dz)))
+    need, amin, val = ol.itest_mesh(p, st, 1, 0.5, want * (1 - 1e-9))
+    assert need == 0 and abs(amin - want) < 1e-10 * want and val == 0.0
+    assert ol.itest_mesh(p, st, 1, 0.5, want * (1 + 1e-9))[0] == 1
+    # shortening: the right wall moved in by 3 element widths
+    sh = st.copy()
+    sh.cord[:, nx - 1, 0] -= 3 * dx
+    need, amin, val = ol.itest_mesh(p, sh, 11, 2.0, 0.0)
+    assert need == 2 and abs(val - 3 * dx) < 1e-9 * dx
+    assert ol.itest_mesh(p, sh, 11, 4.0, 0.0)[0] == 0  # within dx_rem element widths
+    # surface changes: dhacc beyond a third of the top element height
+    q = type(p).from_buffer_copy(p)
+    q.topo_kappa = 1e-6
+    tp = st.copy()
+    tp.dhacc[:] = 0.0
+    tp.dhacc[3] = 0.34 * dz
+    need, amin, val = ol.itest_mesh(q, tp, 1, 0.5, 0.0)
+    assert need == 3 and val == abs(tp.dhacc[nx - 2])
+    tp.dhacc[3] = 0.33 * dz
+    assert ol.itest_mesh(q, tp, 1, 0.5, 0.0)[0] == 0
