@@ -1017,6 +1017,10 @@ GF_DEV void therm_check_body(const GfDevParams *__restrict__ P, GfDevClock *ck, 
     const int dj[3] = {1, 0, -1}, di[3] = {0, 1, 1};
     const double dt = ck->dt;
     auto T = [&](int jj, int ii) { return A.temp[IN(jj, ii)]; };  // the old temperature
+    // the (up to three) evaluations first, the flag after them, so that no atomic
+    // orders the loads of one evaluation after another's
+    bool bad = false;
+#pragma unroll
     for (int q = 0; q < 3; q++) {
         const int mj = j + dj[q], mi = i + di[q];
         const int mlj = mj - j0, mli = mi - i0;
@@ -1028,9 +1032,9 @@ GF_DEV void therm_check_body(const GfDevParams *__restrict__ P, GfDevClock *ck, 
         const double xl = mi > 1 ? x[IN(mj, mi - 1)] : 0.0;
         const double xd = (mi > 1 && mj < NZ) ? x[IN(mj + 1, mi - 1)] : 0.0;
         const double f = therm_eval(c, xu, xl, xd, dt);
-        if (__double_as_longlong(f) != __double_as_longlong(x[IN(mj, mi)]))
-            atomicOr(&ck->therm_pass_changed[pass], 1u);
+        bad |= __double_as_longlong(f) != __double_as_longlong(x[IN(mj, mi)]);
     }
+    if (bad) atomicOr(&ck->therm_pass_changed[pass], 1u);
 }
 
 // (grid-stride: a launch for a fallback pass may use fewer workgroups than entries)
