@@ -1011,8 +1011,6 @@ GF_DEV void therm_check_body(const GfDevParams *__restrict__ P, GfDevClock *ck, 
     if (j < 1 || j > NZ || i < 1 || i > NX) return;
     const long long a = __double_as_longlong(bord[t]), b = __double_as_longlong(x[IN(j, i)]);
     if (a == b) return;
-    atomicAdd(&ck->therm_mismatch, 1ull);
-    atomicMax(&ck->therm_mismatch_ulp, (unsigned long long)(a > b ? a - b : b - a));
     // output nodes of this tile that read (j,i): as up (j+1,i), left (j,i+1), left-down (j-1,i+1)
     const int dj[3] = {1, 0, -1}, di[3] = {0, 1, 1};
     const double dt = ck->dt;
@@ -1034,6 +1032,8 @@ GF_DEV void therm_check_body(const GfDevParams *__restrict__ P, GfDevClock *ck, 
         const double f = therm_eval(c, xu, xl, xd, dt);
         bad |= __double_as_longlong(f) != __double_as_longlong(x[IN(mj, mi)]);
     }
+    atomicAdd(&ck->therm_mismatch, 1ull);  // (statistics)
+    atomicMax(&ck->therm_mismatch_ulp, (unsigned long long)(a > b ? a - b : b - a));
     if (bad) atomicOr(&ck->therm_pass_changed[pass], 1u);
 }
 
